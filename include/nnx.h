@@ -1,0 +1,363 @@
+/*
+ * nnx.h - C ABI of the MI355X (gfx950) spiking-network timestep.
+ *
+ * One library (libnnx.so, built with hipcc for gfx950 only) implements the
+ * per-timestep hot path of the neuronephys simulator:
+ *   ephys     - Hodgkin-Huxley neuron update (reference leapfrog or RK4),
+ *   coupling  - spike detection, ascending spike list, propagation over CSR/CSC,
+ *   paracrine - release deposit and the implicit-diffusion CG solve.
+ *
+ * Conventions
+ *   - Every pointer named d_* or documented as "device" is device memory.
+ *   - All calls are asynchronous on the context's stream unless documented.
+ *   - No call exits the process: errors are returned as nnx_status and the
+ *     message is kept in the context (nnx_last_error).
+ *   - A context is not thread-safe: one per host thread / stream.
+ *   - Spike counts stay in device memory; nothing on the step path reads
+ *     them back unless the caller asks (nnx_read_count).
+ *
+ * The reference interfaces each entry point replaces are cited as
+ * file:line of the neuronephys_clean sources.
+ */
+#ifndef NNX_H
+#define NNX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nnx_ctx nnx_ctx;
+
+typedef enum nnx_status {
+    NNX_OK = 0,
+    NNX_ERR_ARG = 1,         /* invalid argument (shape, pointer, enum)           */
+    NNX_ERR_HIP = 2,         /* HIP runtime error                                  */
+    NNX_ERR_NONFINITE = 3,   /* a NaN/Inf was found where the caller asked to check */
+    NNX_ERR_NOCONV = 4,      /* iterative solver hit maxit before rtol            */
+    NNX_ERR_UNSUPPORTED = 5, /* valid but unimplemented combination               */
+    NNX_ERR_STATE = 6        /* call order / context state error                  */
+} nnx_status;
+
+/* element types */
+enum { NNX_F32 = 0, NNX_F64 = 1 };
+
+/* HH integrators.  LEAPFROG is the reference's semi-implicit update with the
+ * three kinetic synapses (src/ephys.cu:28-96); RK4 is the classical 4-stage
+ * Runge-Kutta with one exponential synapse (no reference counterpart).      */
+enum { NNX_INT_LEAPFROG = 0, NNX_INT_RK4 = 1 };
+
+/* spike rules.  LEVEL: output > out_thresh (src/coupling.cu:103-109).
+ * CROSSING: V_prev < v_thresh <= V_new (upward crossing).                  */
+enum { NNX_SPIKE_LEVEL = 0, NNX_SPIKE_CROSSING = 1 };
+
+/* coupling modes (what one presynaptic spike contributes to each target).
+ *   WEIGHTED_PULL  : g_in[t] = sum of w over spiking sources, summed in
+ *                    ascending source order, dense gather over the CSC.
+ *   WEIGHTED_EVENT : same sum, bit-identical to WEIGHTED_PULL, but reads only
+ *                    the out-edges of spiking sources (event-driven).
+ *   COUNT          : +1 per spiking source (src/coupling.cu:34-48).
+ *   SIGNAL         : +output[source] per spiking source
+ *                    (src_wholebrain/coupling.cu:12-33).                    */
+enum { NNX_COUPLE_WEIGHTED_PULL = 0, NNX_COUPLE_WEIGHTED_EVENT = 1,
+       NNX_COUPLE_COUNT = 2, NNX_COUPLE_SIGNAL = 3 };
+
+/* stencils / time schemes for the paracrine grid */
+enum { NNX_STENCIL_7 = 7, NNX_STENCIL_27 = 27 };
+enum { NNX_PRECOND_NONE = 0, NNX_PRECOND_JACOBI = 1, NNX_PRECOND_LEARNED = 2 };
+
+/* ------------------------------------------------------------------------ */
+/* context                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Creates a context on `device`; `hip_stream` (hipStream_t, may be NULL for
+ * the null stream) is borrowed, not owned.                                  */
+nnx_status nnx_create(nnx_ctx** out, int device, void* hip_stream);
+nnx_status nnx_destroy(nnx_ctx* ctx);
+const char* nnx_last_error(const nnx_ctx* ctx);
+nnx_status nnx_set_stream(nnx_ctx* ctx, void* hip_stream);
+nnx_status nnx_synchronize(nnx_ctx* ctx);
+/* library version string, e.g. "nnx 0.1 gfx950" */
+const char* nnx_version(void);
+
+/* Device-memory helpers so that host code compiled without a HIP toolchain
+ * (the compat classes) can own buffers.  The reference uses cudaMalloc /
+ * thrust::device_vector for the same buffers (src/main.cu:327-372).        */
+nnx_status nnx_malloc(nnx_ctx* ctx, void** d_ptr, size_t bytes);
+nnx_status nnx_free(nnx_ctx* ctx, void* d_ptr);
+nnx_status nnx_memcpy_h2d(nnx_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+nnx_status nnx_memcpy_d2h(nnx_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+nnx_status nnx_memcpy_d2d(nnx_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+nnx_status nnx_memset(nnx_ctx* ctx, void* d_dst, int value, size_t bytes);
+
+/* Stream-ordered timing on the context's stream (hipEvent pairs), for
+ * callers that time kernels without a HIP toolchain of their own.          */
+nnx_status nnx_event_create(nnx_ctx* ctx, void** ev);
+nnx_status nnx_event_destroy(nnx_ctx* ctx, void* ev);
+nnx_status nnx_event_record(nnx_ctx* ctx, void* ev);
+/* milliseconds between two recorded events (synchronises on `end`)        */
+nnx_status nnx_event_elapsed(nnx_ctx* ctx, void* start, void* end, float* ms);
+/* the context's hipStream_t                                               */
+void* nnx_get_stream(const nnx_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* ephys                                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Structure-of-arrays neuron state.  Each pointer is a device array of n
+ * elements of `dtype`.  The names follow the reference's SoA indices
+ * (src/defines.cuh:57-78, src_wholebrain/ephys.cuh:62-78).  Fields a mode
+ * does not use may be NULL.                                                 */
+typedef struct nnx_hh_state {
+    int32_t dtype;            /* NNX_F32 | NNX_F64 */
+    int32_t pad_;
+    int64_t n;
+    void* v;                  /* membrane potential, mV        (SIM_PARAMS_V) */
+    void* m;                  /* Na activation                 (SIM_PARAMS_M) */
+    void* h;                  /* Na inactivation               (SIM_PARAMS_H) */
+    void* ng;                 /* K activation "n"              (SIM_PARAMS_N) */
+    /* RK4 / exponential synapse */
+    void* s;                  /* synaptic conductance, mS/cm^2 */
+    const void* i_ext;        /* static drive, uA/cm^2 (RK4) */
+    /* leapfrog / kinetic synapses */
+    void* y_local;            /* SIM_PARAMS_LOCAL_Y */
+    void* y_i;                /* SIM_PARAMS_IY */
+    void* y_e;                /* SIM_PARAMS_EY */
+    const void* in_local;     /* SIM_PARAMS_LOCAL_IN */
+    const void* count_local;  /* SIM_PARAMS_LOCAL_COUNT (in-degree) */
+    const void* in_i;         /* SIM_PARAMS_IIN */
+    const void* count_i;      /* SIM_PARAMS_ICOUNT */
+    const void* in_e;         /* SIM_PARAMS_EIN */
+    const void* count_e;      /* SIM_PARAMS_ECOUNT */
+    const void* appcur;       /* SIM_PARAMS_APPCUR */
+    void* out_local;          /* SIM_PARAMS_LOCAL_OUTPUT */
+    void* out_i;              /* SIM_PARAMS_IOUTPUT */
+    void* out_e;              /* SIM_PARAMS_EOUTPUT */
+} nnx_hh_state;
+
+/* Model parameters (all in the reference's units: mV, ms, mS/cm^2, uA/cm^2).
+ * The six rate functions are the reference's (src/ephys.cu:57-63), written
+ * for a -60 mV rest, evaluated at V + v_shift; v_shift = 5 gives the
+ * classic -65 mV squid axon.                                               */
+typedef struct nnx_hh_params {
+    double C, gna, gk, gl, ena, ek, el;
+    double v_shift;
+    /* RK4: exponential synapse, I_syn = s * (V - e_syn), s decays with tau_syn */
+    double tau_syn, e_syn;
+    /* leapfrog: kinetic synapses + sigmoid output (src/defines.cuh:94-101) */
+    double tmax, vt, kp, ar, ad;
+    double g_local, e_local, g_i, e_i, g_e, e_e;
+    double noise_mean, noise_std;   /* white noise added to the drive; std 0 = deterministic */
+    uint64_t noise_seed;
+    /* spike rules */
+    double v_thresh;                /* CROSSING threshold, mV */
+    double out_thresh;              /* LEVEL threshold on the output */
+} nnx_hh_params;
+
+/* Fills `p` with the reference HH constants (src/defines.cuh:87-102):
+ * leapfrog, v_shift 0, ENa 55, EK -72, EL -50, noise mean 3 / std 1.5.    */
+void nnx_hh_params_reference(nnx_hh_params* p);
+/* Fills `p` with the squid constants of the RK4 configuration: C 1,
+ * gNa 120, gK 36, gL 0.3, ENa 50, EK -77, EL -54.4, v_shift 5, tau 2 ms,
+ * E_syn 0 mV, threshold 0 mV.                                             */
+void nnx_hh_params_squid(nnx_hh_params* p);
+
+/* One timestep of every neuron.  Replaces HHEphys::SimulateEphys
+ * (src/ephys.cu:98-103, src_wholebrain/ephys.cu:219-224) and its kernel
+ * Leapfrog_HH (src/ephys.cu:28-96).
+ *   g_in       RK4 only: per-neuron conductance increment of this step
+ *              (coupling output), dtype of the state; NULL = none.
+ *   spike_bits out: bit i of word i/64 = neuron i spiked at this step under
+ *              `spike_rule`; ceil(n/64) uint64 words; NULL = not wanted.
+ *   step       global step index (keys the counter-based noise).          */
+nnx_status nnx_hh_step(nnx_ctx* ctx, const nnx_hh_state* st, const nnx_hh_params* p,
+                       int integrator, int spike_rule, double dt, uint64_t step,
+                       const void* g_in, uint64_t* spike_bits);
+
+/* Returns NNX_ERR_NONFINITE if any V is NaN/Inf (synchronous).             */
+nnx_status nnx_check_finite(nnx_ctx* ctx, int dtype, int64_t n, const void* d_x);
+
+/* ------------------------------------------------------------------------ */
+/* spikes and coupling                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Level rule from outputs: bits[i] = out[i] > thresh.  Replaces the thrust
+ * transform with CheckFiringOperator (src/coupling.cu:103-109,118).        */
+nnx_status nnx_spike_threshold(nnx_ctx* ctx, int dtype, int64_t n, const void* d_out,
+                               double thresh, uint64_t* d_bits);
+
+/* Ascending list of set bits (stable compaction) and its length, both left
+ * in device memory.  Replaces thrust::copy_if over a counting iterator
+ * (src/coupling.cu:121-127, src_wholebrain/coupling.cu:52-57).             */
+nnx_status nnx_spike_compact(nnx_ctx* ctx, int64_t n, const uint64_t* d_bits,
+                             int32_t* d_list, int32_t* d_count);
+
+/* Synchronous read of a device int32 (the count), through pinned memory.   */
+nnx_status nnx_read_count(nnx_ctx* ctx, const int32_t* d_count, int32_t* h_count);
+
+/* Connectivity of one network, both directions, all device arrays.
+ * out_*: CSR of out-edges exactly as the reference's AdjacencyList
+ *        (indices / offset_in_indices, src/coupling.cuh:4-8).
+ * in_* : its transpose (CSC), in-edges of each target sorted by source.
+ * out_to_in[e]: position in the CSC of CSR edge e (WEIGHTED_EVENT only).   */
+typedef struct nnx_graph {
+    int64_t n;
+    int64_t nnz;
+    const int32_t* out_offsets;   /* n+1 */
+    const int32_t* out_targets;   /* nnz */
+    const float*   out_weights;   /* nnz or NULL */
+    const int32_t* in_offsets;    /* n+1 */
+    const int32_t* in_sources;    /* nnz */
+    const float*   in_weights;    /* nnz or NULL */
+    const int32_t* out_to_in;     /* nnz or NULL */
+} nnx_graph;
+
+/* Builds the CSC (in_offsets, in_sources, in_weights) and out_to_in of a CSR
+ * on the device.  Deterministic: in-edges end up sorted by source, ties
+ * (duplicate edges) by CSR position.  Replaces the host loop that counts
+ * in-degree in Network::ToAdjacencyList (src/network_factory.cu:5-27) and
+ * cusparseCsr2cscEx2 (src/paracrine.cu:403-432).  in_degree (optional,
+ * dtype) receives the in-degree as the reference's COUNT array.            */
+nnx_status nnx_graph_transpose(nnx_ctx* ctx, int64_t n, int64_t nnz,
+                               const int32_t* d_out_offsets, const int32_t* d_out_targets,
+                               const float* d_out_weights,
+                               int32_t* d_in_offsets, int32_t* d_in_sources, float* d_in_weights,
+                               int32_t* d_out_to_in, int in_degree_dtype, void* d_in_degree);
+
+/* Spike propagation.  Replaces AtomicCoupling::operator()
+ * (src/coupling.cu:111-178) and AtomicCoupling::PerformCoupling
+ * (src_wholebrain/coupling.cu:43-69).  Inputs are the spike bitmap of the
+ * previous step (and, for COUNT/SIGNAL/EVENT, its ascending list + device
+ * count from nnx_spike_compact).  The result overwrites d_g_in (n elements
+ * of `dtype`); every mode is deterministic.                                */
+nnx_status nnx_couple(nnx_ctx* ctx, const nnx_graph* g, int mode, int dtype,
+                      const uint64_t* d_bits, const int32_t* d_list, const int32_t* d_count,
+                      const void* d_outputs, void* d_g_in);
+
+/* ------------------------------------------------------------------------ */
+/* paracrine                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* A box grid, x fastest: index = i + nx*(j + ny*k) (src/paracrine.cu:172-178).
+ * Operators (Dirichlet 0 outside the box):
+ *   STENCIL_7 : A = diag*I - c*sum(6 neighbours), diag = 1 + 6c, i.e.
+ *               (I - c*L7) of a backward-Euler diffusion step, c = dt*D/h^2;
+ *               with kx/ky/kz set, c varies per face: kx[idx] couples voxel
+ *               idx to idx+1 (x), and the diagonal is 1 + sum of its faces.
+ *   STENCIL_27: A = (1 + decay_dt/2) I - (c/2) L27, L27 the reference's
+ *               high-order compact Laplacian {1,3,14,-128}/30 (src/paracrine.cu:40-43,80-81),
+ *               c = dt*D/h^2: the Crank-Nicolson operator.                  */
+typedef struct nnx_grid {
+    int32_t nx, ny, nz;
+    int32_t stencil;
+    double c;
+    double decay_dt;
+    const float* kx;   /* variable face coefficients or NULL (STENCIL_7 only) */
+    const float* ky;
+    const float* kz;
+} nnx_grid;
+
+typedef struct nnx_cg_opts {
+    double rtol;          /* stop when ||r||/||b|| <= rtol (recursive residual) */
+    int32_t maxit;
+    int32_t precond;      /* NNX_PRECOND_* */
+    int32_t check_every;  /* iterations between host convergence checks (>=1) */
+    int32_t use_x0;       /* 1: start from x; 0: start from 0 */
+    int32_t stall_iters;  /* flexible CG: fall back to Jacobi after this many non-improving iterations */
+    int32_t pad_;
+} nnx_cg_opts;
+
+typedef struct nnx_cg_result {
+    int32_t iters;        /* inner iterations performed */
+    int32_t outer;        /* refinement steps (nnx_cg_solve_refine) */
+    int32_t converged;
+    int32_t precond_used; /* precond of the final sweep (fallback reports JACOBI) */
+    double relres;        /* final recursive residual ||r||/||b|| */
+    double true_relres;   /* ||b - A x||/||b|| recomputed in FP64 */
+    double ms;            /* wall time of the solve (host clock, includes syncs) */
+} nnx_cg_result;
+
+/* y = A x for the grid's operator (fp32).                                   */
+nnx_status nnx_stencil_apply(nnx_ctx* ctx, const nnx_grid* g, const float* d_x, float* d_y);
+
+/* Preconditioned CG on A x = b, fp32 vectors, FP64 dots/norms, scalars kept
+ * on the device; the host reads convergence every check_every iterations.
+ * Replaces Density_on_grid_3d::update_density's CG loop
+ * (src/paracrine.cu:106-129).                                              */
+nnx_status nnx_cg_solve(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float* d_x,
+                        const nnx_cg_opts* o, nnx_cg_result* res);
+
+/* Mixed-precision iterative refinement: FP64 b and x, the outer residual
+ * r = b - A x recomputed in FP64, inner fp32 CG (inner_rtol) on A d = r,
+ * x += d, until ||r||/||b|| <= o->rtol.                                     */
+nnx_status nnx_cg_solve_refine(nnx_ctx* ctx, const nnx_grid* g, const double* d_b, double* d_x,
+                               const nnx_cg_opts* o, double inner_rtol, int max_outer,
+                               nnx_cg_result* res);
+
+/* One reference paracrine update (Crank-Nicolson 27-point, decay, explicit-
+ * Euler initial guess, at most max_iter CG iterations, stop on the absolute
+ * squared residual < error_bound), restating Density_on_grid_3d::update_density
+ * (src/paracrine.cu:63-134) including its use of h where dt is meant in b
+ * (src/paracrine.cu:105) and its early return (src/paracrine.cu:114-117).
+ * `data` is updated in place.  res->iters/relres report the loop.          */
+nnx_status nnx_update_density_reference(nnx_ctx* ctx, int nx, int ny, int nz, double h,
+                                        double diffusion, double decay, double timestep,
+                                        double error_bound, int max_iter, float* d_data,
+                                        nnx_cg_result* res);
+
+/* Release deposit: voxel_counts[voxel_of[i]] += 1 for each spiking neuron i
+ * of the list (integer atomics: order-independent).                        */
+nnx_status nnx_deposit_count(nnx_ctx* ctx, const int32_t* d_voxel_of, const int32_t* d_list,
+                             const int32_t* d_count, int32_t* d_voxel_counts);
+
+/* b = c_prev + amount * voxel_counts, then voxel_counts = 0.               */
+nnx_status nnx_rhs_from_counts(nnx_ctx* ctx, int64_t nvox, const float* d_c_prev,
+                               int32_t* d_voxel_counts, double amount, float* d_b);
+
+/* Trilinear neuron<->grid transfer (Scatter_grid_converter_3d,
+ * src/paracrine.cu:136-296), both as CSR SpMV over device CSR matrices the
+ * host builder produced (nnx_host.h).  interp: y[n] = sum_j W[n,j] g[j];
+ * scatter: g[j] += sum_n W^T[j,n] x[n] (rows of the transpose, neurons in
+ * ascending order: deterministic).                                         */
+nnx_status nnx_csr_spmv(nnx_ctx* ctx, int64_t rows, const int32_t* d_offsets, const int32_t* d_cols,
+                        const float* d_vals, const float* d_x, float* d_y, int accumulate);
+
+/* ------------------------------------------------------------------------ */
+/* composite timestep                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Everything one timestep touches, in the reference's order
+ * coupling -> paracrine -> ephys (src/main.cu:166-181,
+ * src_wholebrain/mouse_cortex.cu:104-120).                                 */
+typedef struct nnx_step_plan {
+    nnx_hh_state state;
+    nnx_hh_params params;
+    nnx_graph graph;
+    int32_t integrator, spike_rule, couple_mode, record;  /* record: append rasters */
+    double dt;
+    void* g_in;              /* n, state dtype: coupling output / ephys input */
+    uint64_t* bits[2];       /* ping-pong spike bitmaps, ceil(n/64) words each */
+    int32_t* list;           /* n: ascending spike list of the last step */
+    int32_t* count;          /* 1: its length */
+    /* paracrine release accumulation (optional; voxel_of NULL = off) */
+    const int32_t* voxel_of; /* n */
+    int32_t* voxel_counts;   /* nvox */
+    /* raster recording (optional): (step, neuron) pairs appended while the
+     * total stays below raster_cap; raster_len is a device int64.          */
+    int32_t* raster;         /* 2*raster_cap */
+    int64_t* raster_len;
+    int64_t raster_cap;
+} nnx_step_plan;
+
+/* Runs n_steps timesteps starting at global step `first_step`.  With
+ * use_graph = 1 the step sequence is captured once into a hipGraph per
+ * (plan, n_steps) and replayed; plans must not move between calls.         */
+nnx_status nnx_run_steps(nnx_ctx* ctx, nnx_step_plan* plan, uint64_t first_step, int n_steps,
+                         int use_graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NNX_H */

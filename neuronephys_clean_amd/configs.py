@@ -1,0 +1,117 @@
+"""Synthetic inputs of the benchmark configurations (host side, numpy).
+
+Every draw comes from a Philox4x64-10 stream keyed by (seed, purpose), so
+the same seed gives the same network and initial state on any machine and
+thread count.  Network construction stays on the host, as in the reference
+(Network::Generate, src/network_factory.cuh:23-38).
+
+  c1 : 1,024 squid HH neurons, Erdos-Renyi with exactly 100 distinct
+       out-edges per neuron (self-loops excluded), w ~ U[0, 0.1], positions
+       U[0,32)^3 on a 32^3 grid with h = 1, dt*D/h^2 = 1.
+  c2 : 1,048,576 neurons in a 128^3 box, out-degree LogNormal(ln 200 - 1/8,
+       1/2) (mean 200), targets drawn without replacement with probability
+       proportional to exp(-d/10) up to d = 60, w ~ LogNormal(-3, 1), a 256^3
+       grid (h = 0.5).  `scale` shrinks N and the box at constant density.
+  c4 : ensemble members: N ~ round(2^U(10,16)), out-degree Poisson(50),
+       I_ext ~ N(mu_i, 2) with mu_i ~ U(6, 14).
+Initial state: V ~ U(-70, -60) mV, gates at their steady state for V,
+s = 0; I_ext ~ N(10, 2) uA/cm^2; dt = 0.01 ms.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import host
+
+# Philox purposes (one stream per kind of draw)
+P_V0, P_IEXT, P_W, P_POS, P_MU, P_NSIZE, P_GRIDSRC = 101, 102, 103, 104, 105, 106, 107
+
+SQUID_VSHIFT = 5.0
+
+
+@dataclass
+class Network:
+    n: int
+    off: np.ndarray
+    tgt: np.ndarray
+    w: np.ndarray
+    V0: np.ndarray
+    m0: np.ndarray
+    h0: np.ndarray
+    n0: np.ndarray
+    iext: np.ndarray
+    pos: np.ndarray
+    grid: tuple = (32, 32, 32)
+    h_grid: float = 1.0
+    c: float = 1.0           # dt*D/h^2 of the implicit paracrine step
+    amount: float = 0.01     # release per spike
+    dt: float = 0.01
+    name: str = ""
+    csc: tuple = field(default=None, repr=False)   # in_off, in_src, in_w, out_to_in, in_degree
+    voxel: np.ndarray = field(default=None, repr=False)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.off[-1])
+
+    def ensure_csc(self):
+        if self.csc is None:
+            self.csc = host.csr_to_csc(self.n, self.off, self.tgt, self.w)
+        return self.csc
+
+    def ensure_voxel(self):
+        if self.voxel is None:
+            self.voxel = host.voxel_of(self.pos, self.h_grid, *self.grid)
+        return self.voxel
+
+
+def initial_state(seed, n, v_shift=SQUID_VSHIFT, iext_mu=10.0, iext_sigma=2.0):
+    V0 = host.uniform(seed, P_V0, n, -70.0, -60.0)
+    m0, h0, n0 = host.steady_state(V0, v_shift)
+    iext = host.normal(seed, P_IEXT, n, iext_mu, iext_sigma)
+    return V0, m0, h0, n0, iext
+
+
+def c1(seed: int = 0, n: int = 1024, k: int = 100) -> Network:
+    off, tgt = host.er_fixed(seed, n, k, allow_self=False)
+    w = host.uniform(seed, P_W, len(tgt), 0.0, 0.1).astype(np.float32)
+    V0, m0, h0, n0, iext = initial_state(seed, n)
+    pos = host.uniform(seed, P_POS, 3 * n, 0.0, 32.0).astype(np.float32).reshape(n, 3)
+    return Network(n, off, tgt, w, V0, m0, h0, n0, iext, pos, grid=(32, 32, 32), h_grid=1.0,
+                   c=1.0, name=f"c1_n{n}_k{k}")
+
+
+def c2(seed: int = 0, n: int = 1 << 20, lam: float = 10.0, cutoff: float = 60.0,
+       grid: int | None = None) -> Network:
+    scale = (n / float(1 << 20)) ** (1.0 / 3.0)
+    box = 128.0 * scale
+    pos = host.uniform(seed, P_POS, 3 * n, 0.0, box).astype(np.float32).reshape(n, 3)
+    deg = host.outdeg_lognormal(seed, n, math.log(200.0) - 0.125, 0.5, min(n - 1, 20000))
+    off = host.offsets_of(deg)
+    tgt = host.spatial_targets(seed, pos, box, lam, cutoff, off)
+    w = host.lognormal(seed, P_W, len(tgt), -3.0, 1.0).astype(np.float32)
+    V0, m0, h0, n0, iext = initial_state(seed, n)
+    g = grid if grid is not None else max(8, int(round(256 * scale)))
+    return Network(n, off, tgt, w, V0, m0, h0, n0, iext, pos, grid=(g, g, g), h_grid=box / g,
+                   c=1.0, name=f"c2_n{n}")
+
+
+def c4_sizes(seed: int = 0, count: int = 256):
+    u = host.uniform(seed, P_NSIZE, count, 10.0, 16.0)
+    return np.rint(2.0 ** u).astype(np.int64)
+
+
+def c4_member(seed: int, index: int, n: int, lam_deg: float = 50.0) -> Network:
+    s = (seed << 16) + index + 1
+    deg = np.minimum(host.outdeg_poisson(s, n, lam_deg), n - 1)
+    off = host.offsets_of(deg)
+    tgt = host.uniform_targets(s, off, allow_self=False)
+    w = host.uniform(s, P_W, len(tgt), 0.0, 0.1).astype(np.float32)
+    mu = host.uniform(seed, P_MU, index + 1, 6.0, 14.0)[index]
+    V0, m0, h0, n0, iext = initial_state(s, n, iext_mu=mu)
+    pos = host.uniform(s, P_POS, 3 * n, 0.0, 64.0).astype(np.float32).reshape(n, 3)
+    return Network(n, off, tgt.astype(np.int32), w, V0, m0, h0, n0, iext, pos,
+                   grid=(64, 64, 64), h_grid=1.0, c=1.0, name=f"c4_{index}_n{n}")

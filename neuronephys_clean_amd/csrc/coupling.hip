@@ -1,0 +1,537 @@
+// coupling.hip - spike detection, ascending spike list, deterministic spike
+// propagation.
+//
+// Replaces AtomicCoupling::operator() (src/coupling.cu:111-178) and
+// PerformCoupling (src_wholebrain/coupling.cu:43-69).  The reference flags
+// firing neurons, compacts them with thrust::copy_if, reads the count back to
+// the host and scatters with fp32 atomicAdd (order-nondeterministic).  Here:
+//   - spikes are a bitmap (one ballot word per 64 neurons);
+//   - the ascending list is built by one workgroup with a block scan, the
+//     count stays on the device;
+//   - every target's input is summed over its spiking sources in ascending
+//     source order, so all modes are deterministic and WEIGHTED_PULL and
+//     WEIGHTED_EVENT are bit-identical.
+#include "internal.h"
+
+namespace nnx {
+
+__device__ __forceinline__ bool bit_of(const uint64_t* bits, int32_t s) {
+    return (bits[s >> 6] >> (s & 63)) & 1ull;
+}
+
+// ------------------------------------------------------------------------
+// level rule (CheckFiringOperator, src/coupling.cu:103-109)
+// ------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_threshold(int64_t n, const T* __restrict__ out, T thr,
+                                                   uint64_t* __restrict__ bits, int64_t nwords) {
+    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    const bool f = i < n && out[i] > thr;
+    const uint64_t w = __ballot(f);
+    if ((threadIdx.x & 63) == 0 && (i >> 6) < nwords) bits[i >> 6] = w;
+}
+
+// ------------------------------------------------------------------------
+// compaction: one workgroup of 1024 threads walks the bitmap in tiles of
+// 4096 words; each thread owns 4 consecutive words of a tile, a block scan
+// of the per-thread popcounts gives every thread its output offset.
+// Optionally appends (step, neuron) raster pairs and counts releases per
+// voxel (integer atomics: order-independent).
+// ------------------------------------------------------------------------
+constexpr int kCompactThreads = 1024;
+constexpr int kWordsPerThread = 4;
+constexpr int kTileWords = kCompactThreads * kWordsPerThread;
+
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh_wave, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh_wave[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        int s = lane < kCompactThreads / 64 ? sh_wave[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
+        }
+        if (lane < kCompactThreads / 64) sh_wave[lane] = s;  // inclusive per wave
+    }
+    __syncthreads();
+    const int before = wave ? sh_wave[wave - 1] : 0;
+    *total = sh_wave[kCompactThreads / 64 - 1];
+    __syncthreads();
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kCompactThreads) void k_compact(int64_t nwords,
+                                                             const uint64_t* __restrict__ bits,
+                                                             int32_t* __restrict__ list,
+                                                             int32_t* __restrict__ count,
+                                                             CompactExtras ex) {
+    __shared__ uint64_t tile[kTileWords];
+    __shared__ int sh_wave[kCompactThreads / 64];
+    int64_t base = 0;
+    int64_t rbase = ex.raster ? *ex.raster_len : 0;
+    const int32_t step = ex.step ? (int32_t)*ex.step : 0;
+    for (int64_t t0 = 0; t0 < nwords; t0 += kTileWords) {
+        for (int q = threadIdx.x; q < kTileWords; q += kCompactThreads)
+            tile[q] = (t0 + q < nwords) ? bits[t0 + q] : 0ull;
+        __syncthreads();
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < kWordsPerThread; ++q)
+            cnt += __popcll(tile[threadIdx.x * kWordsPerThread + q]);
+        int tot;
+        const int off = block_exclusive_scan(cnt, sh_wave, &tot);
+        int64_t o = base + off;
+        for (int q = 0; q < kWordsPerThread; ++q) {
+            uint64_t w = tile[threadIdx.x * kWordsPerThread + q];
+            const int64_t word = t0 + threadIdx.x * kWordsPerThread + q;
+            while (w) {
+                const int b = __ffsll((unsigned long long)w) - 1;
+                w &= w - 1;
+                const int32_t idx = (int32_t)(word * 64 + b);
+                list[o] = idx;
+                if (ex.raster) {
+                    const int64_t r = rbase + o;
+                    if (r < ex.raster_cap) {
+                        ex.raster[2 * r] = step;
+                        ex.raster[2 * r + 1] = idx;
+                    }
+                }
+                if (ex.voxel_of) atomicAdd(&ex.voxel_counts[ex.voxel_of[idx]], 1);
+                ++o;
+            }
+        }
+        base += tot;
+        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *count = (int32_t)base;
+        if (ex.raster) *ex.raster_len = rbase + base;
+        if (ex.step) *ex.step = step + 1;
+    }
+}
+
+// ------------------------------------------------------------------------
+// dense pull over the CSC: one wavefront per target, lanes read 64 in-edges
+// at a time (coalesced), and the contributions of the spiking lanes are
+// added in ascending lane (= source) order by every lane in lockstep.
+// mode: 0 weighted, 2 count, 3 signal
+// ------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_pull(int64_t n, int mode, const int32_t* __restrict__ in_off,
+                                              const int32_t* __restrict__ in_src,
+                                              const float* __restrict__ in_w,
+                                              const uint64_t* __restrict__ bits,
+                                              const T* __restrict__ outputs, T* __restrict__ g) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    for (int64_t t = blockIdx.x * 4ll + (threadIdx.x >> 6); t < n; t += nwaves) {
+        const int32_t e0 = in_off[t], e1 = in_off[t + 1];
+        T acc = T(0);
+        for (int32_t b = e0; b < e1; b += 64) {
+            const int32_t e = b + lane;
+            bool act = false;
+            T val = T(0);
+            if (e < e1) {
+                const int32_t s = in_src[e];
+                act = bit_of(bits, s);
+                if (act) val = mode == 0 ? T(in_w[e]) : mode == 2 ? T(1) : outputs[s];
+            }
+            uint64_t m = __ballot(act);
+            while (m) {
+                const int l = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                acc = acc + __shfl(val, l, 64);
+            }
+        }
+        if (lane == 0) g[t] = acc;
+    }
+}
+
+// ------------------------------------------------------------------------
+// event-driven weighted coupling.
+// push: for each spiking source (wavefront per source, lanes over its
+// out-edges) reserve a slot in the target's bucket and store the CSC
+// position of the edge; buckets live at the target's CSC range, so they can
+// never overflow.  gather: each target sums the weights of its bucket in
+// ascending CSC position, i.e. ascending source order - the same sequence of
+// additions as the dense pull - and resets its count.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_push_event(const int32_t* __restrict__ list,
+                                                    const int32_t* __restrict__ count,
+                                                    const int32_t* __restrict__ out_off,
+                                                    const int32_t* __restrict__ out_tgt,
+                                                    const int32_t* __restrict__ out_to_in,
+                                                    const int32_t* __restrict__ in_off,
+                                                    int32_t* __restrict__ cnt,
+                                                    int32_t* __restrict__ slot) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    const int32_t K = *count;
+    for (int64_t k = blockIdx.x * 4ll + (threadIdx.x >> 6); k < K; k += nwaves) {
+        const int32_t j = list[k];
+        const int32_t e1 = out_off[j + 1];
+        for (int32_t e = out_off[j] + lane; e < e1; e += 64) {
+            const int32_t t = out_tgt[e];
+            const int32_t pos = out_to_in[e];
+            const int32_t q = atomicAdd(&cnt[t], 1);
+            const int32_t b = in_off[t];
+            if (q < in_off[t + 1] - b) slot[b + q] = pos;  // always true for a consistent CSR/CSC
+        }
+    }
+}
+
+constexpr int kSortMax = 8;
+
+template <class T>
+__device__ __forceinline__ T gather_one(int32_t t, int32_t c, const int32_t* __restrict__ in_off,
+                                        const int32_t* __restrict__ in_src,
+                                        const float* __restrict__ in_w,
+                                        const uint64_t* __restrict__ bits,
+                                        const int32_t* __restrict__ slot) {
+    T acc = T(0);
+    if (c == 0) return acc;
+    const int32_t b = in_off[t];
+    if (c <= kSortMax && c <= in_off[t + 1] - b) {
+        // ascending order by repeated minimum (positions are distinct)
+        int32_t last = -1;
+        for (int r = 0; r < c; ++r) {
+            int32_t mn = 0x7fffffff;
+            for (int q = 0; q < c; ++q) {
+                const int32_t p = slot[b + q];
+                mn = (p > last && p < mn) ? p : mn;
+            }
+            acc = acc + T(in_w[mn]);
+            last = mn;
+        }
+    } else {
+        const int32_t e1 = in_off[t + 1];
+        for (int32_t e = b; e < e1; ++e)
+            if (bit_of(bits, in_src[e])) acc = acc + T(in_w[e]);
+    }
+    return acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_event(int64_t n, int32_t* __restrict__ cnt,
+                                                      const int32_t* __restrict__ slot,
+                                                      const int32_t* __restrict__ in_off,
+                                                      const int32_t* __restrict__ in_src,
+                                                      const float* __restrict__ in_w,
+                                                      const uint64_t* __restrict__ bits,
+                                                      T* __restrict__ g) {
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= n) return;
+    const int32_t c = cnt[t];
+    g[t] = gather_one<T>((int32_t)t, c, in_off, in_src, in_w, bits, slot);
+    if (c) cnt[t] = 0;
+}
+
+}  // namespace nnx
+
+using namespace nnx;
+
+namespace nnx {
+
+nnx_status compact_launch(nnx_ctx* c, int64_t n, const uint64_t* bits, int32_t* list,
+                          int32_t* count, const CompactExtras& ex) {
+    const int64_t nwords = ceil_div(n, 64);
+    k_compact<<<1, kCompactThreads, 0, c->stream>>>(nwords, bits, list, count, ex);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+nnx_status event_scratch(nnx_ctx* c, int64_t n, int64_t nnz, int32_t** cnt, int32_t** slot) {
+    void *a, *b;
+    NNX_TRY(scratch(c, "event_cnt", (size_t)(n + 1) * 4, true, &a));
+    NNX_TRY(scratch(c, "event_slot", (size_t)(nnz + 1) * 4, false, &b));
+    *cnt = (int32_t*)a;
+    *slot = (int32_t*)b;
+    return NNX_OK;
+}
+
+nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, const uint64_t* bits,
+                         const int32_t* list, const int32_t* count, const void* outputs,
+                         void* g_in) {
+    const int64_t n = g->n;
+    if (n == 0) return NNX_OK;
+    if (mode == NNX_COUPLE_WEIGHTED_EVENT) {
+        int32_t *cnt, *slot;
+        NNX_TRY(event_scratch(c, n, g->nnz, &cnt, &slot));
+        k_push_event<<<2048, 256, 0, c->stream>>>(list, count, g->out_offsets, g->out_targets,
+                                                  g->out_to_in, g->in_offsets, cnt, slot);
+        NNX_LAUNCHED(c);
+        const unsigned blocks = (unsigned)ceil_div(n, 256);
+        if (dtype == NNX_F32)
+            k_gather_event<float><<<blocks, 256, 0, c->stream>>>(
+                n, cnt, slot, g->in_offsets, g->in_sources, g->in_weights, bits, (float*)g_in);
+        else
+            k_gather_event<double><<<blocks, 256, 0, c->stream>>>(
+                n, cnt, slot, g->in_offsets, g->in_sources, g->in_weights, bits, (double*)g_in);
+        NNX_LAUNCHED(c);
+        return NNX_OK;
+    }
+    const int m = mode == NNX_COUPLE_WEIGHTED_PULL ? 0 : mode == NNX_COUPLE_COUNT ? 2 : 3;
+    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 4), 65536);
+    if (dtype == NNX_F32)
+        k_pull<float><<<blocks, 256, 0, c->stream>>>(n, m, g->in_offsets, g->in_sources,
+                                                     g->in_weights, bits, (const float*)outputs,
+                                                     (float*)g_in);
+    else
+        k_pull<double><<<blocks, 256, 0, c->stream>>>(n, m, g->in_offsets, g->in_sources,
+                                                      g->in_weights, bits, (const double*)outputs,
+                                                      (double*)g_in);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+nnx_status check_graph(nnx_ctx* c, const nnx_graph* g, int mode) {
+    NNX_CHECK(c, g, "null graph");
+    NNX_CHECK(c, g->n >= 0 && g->n < (int64_t)1 << 31, "graph n out of range");
+    NNX_CHECK(c, g->nnz >= 0 && g->nnz < (int64_t)1 << 31, "graph nnz out of int32 range");
+    NNX_CHECK(c, g->in_offsets && (g->in_sources || g->nnz == 0), "graph needs the CSC (in_offsets, in_sources)");
+    if (mode == NNX_COUPLE_WEIGHTED_PULL || mode == NNX_COUPLE_WEIGHTED_EVENT)
+        NNX_CHECK(c, g->in_weights || g->nnz == 0, "weighted coupling needs in_weights");
+    if (mode == NNX_COUPLE_WEIGHTED_EVENT)
+        NNX_CHECK(c, g->out_offsets && (g->nnz == 0 || (g->out_targets && g->out_to_in)),
+                  "event coupling needs out_offsets, out_targets and out_to_in");
+    return NNX_OK;
+}
+
+}  // namespace nnx
+
+extern "C" {
+
+nnx_status nnx_spike_threshold(nnx_ctx* c, int dtype, int64_t n, const void* out, double thr,
+                               uint64_t* bits) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, n >= 0 && (n == 0 || (out && bits)), "bad arguments");
+    if (n == 0) return NNX_OK;
+    const int64_t nwords = ceil_div(n, 64);
+    const unsigned blocks = (unsigned)ceil_div(n, 256);
+    if (dtype == NNX_F32)
+        k_threshold<float><<<blocks, 256, 0, c->stream>>>(n, (const float*)out, (float)thr, bits, nwords);
+    else if (dtype == NNX_F64)
+        k_threshold<double><<<blocks, 256, 0, c->stream>>>(n, (const double*)out, thr, bits, nwords);
+    else
+        return fail(c, NNX_ERR_ARG, "bad dtype");
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+nnx_status nnx_spike_compact(nnx_ctx* c, int64_t n, const uint64_t* bits, int32_t* list,
+                             int32_t* count) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, n >= 0 && count && (n == 0 || (bits && list)), "bad arguments");
+    CompactExtras ex{};
+    return compact_launch(c, n, bits, list, count, ex);
+}
+
+nnx_status nnx_couple(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, const uint64_t* bits,
+                      const int32_t* list, const int32_t* count, const void* outputs, void* g_in) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_graph(c, g, mode));
+    NNX_CHECK(c, mode >= NNX_COUPLE_WEIGHTED_PULL && mode <= NNX_COUPLE_SIGNAL, "bad mode %d", mode);
+    NNX_CHECK(c, dtype == NNX_F32 || dtype == NNX_F64, "bad dtype");
+    NNX_CHECK(c, g->n == 0 || (bits && g_in), "null bits or g_in");
+    if (mode == NNX_COUPLE_WEIGHTED_EVENT) NNX_CHECK(c, list && count, "event coupling needs the spike list and count");
+    if (mode == NNX_COUPLE_SIGNAL) NNX_CHECK(c, outputs, "signal coupling needs the outputs");
+    return couple_launch(c, g, mode, dtype, bits, list, count, outputs, g_in);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// CSR -> CSC on the device (nnx_graph_transpose).
+// 1. in-degree by integer atomics, 2. exclusive scan into in_offsets,
+// 3. atomic fill (arbitrary order inside a target), 4. per-target rank sort
+// by (source, CSR position) - keys are unique, so the result is the same
+// deterministic layout the host builder produces.
+// ------------------------------------------------------------------------
+namespace nnx {
+
+// flag[0] |= 1 if offsets are not non-decreasing from 0 to nnz, 2 if a
+// target is outside [0, n)
+__global__ void k_validate_csr(int64_t n, int64_t nnz, const int32_t* __restrict__ off,
+                               const int32_t* __restrict__ tgt, int32_t* flag) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride)
+        if (off[i + 1] < off[i]) atomicOr(flag, 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (off[0] != 0 || off[n] != nnz)) atomicOr(flag, 1);
+    for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < nnz; e += stride) {
+        const int32_t t = tgt[e];
+        if (t < 0 || t >= n) atomicOr(flag, 2);
+    }
+}
+
+__global__ void k_indeg(int64_t nnz, const int32_t* __restrict__ tgt, int32_t* __restrict__ deg) {
+    for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * 256)
+        atomicAdd(&deg[tgt[e]], 1);
+}
+
+constexpr int kScanThreads = 1024, kScanItems = 4, kScanTile = kScanThreads * kScanItems;
+
+// per-tile sums
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int64_t n, const int32_t* __restrict__ in,
+                                                             int32_t* __restrict__ tile_sum) {
+    __shared__ int sh_wave[kScanThreads / 64];
+    const int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanItems;
+    int v = 0;
+    for (int q = 0; q < kScanItems; ++q) v += base + q < n ? in[base + q] : 0;
+    int tot;
+    block_exclusive_scan(v, sh_wave, &tot);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+// exclusive scan of the tile sums in one block (up to kScanTile tiles)
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(int64_t ntiles, int32_t* __restrict__ s) {
+    __shared__ int sh_wave[kScanThreads / 64];
+    int loc[kScanItems];
+    int v = 0;
+    for (int q = 0; q < kScanItems; ++q) {
+        const int64_t i = threadIdx.x * (int64_t)kScanItems + q;
+        loc[q] = i < ntiles ? s[i] : 0;
+        v += loc[q];
+    }
+    int tot;
+    int off = block_exclusive_scan(v, sh_wave, &tot);
+    for (int q = 0; q < kScanItems; ++q) {
+        const int64_t i = threadIdx.x * (int64_t)kScanItems + q;
+        if (i < ntiles) s[i] = off;
+        off += loc[q];
+    }
+}
+// out[i] = exclusive prefix; out[n] = total
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(int64_t n, const int32_t* __restrict__ in,
+                                                             const int32_t* __restrict__ tile_off,
+                                                             int32_t* __restrict__ out) {
+    __shared__ int sh_wave[kScanThreads / 64];
+    const int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanItems;
+    int loc[kScanItems];
+    int v = 0;
+    for (int q = 0; q < kScanItems; ++q) {
+        loc[q] = base + q < n ? in[base + q] : 0;
+        v += loc[q];
+    }
+    int tot;
+    int off = block_exclusive_scan(v, sh_wave, &tot) + tile_off[blockIdx.x];
+    for (int q = 0; q < kScanItems; ++q) {
+        if (base + q < n) out[base + q] = off;
+        off += loc[q];
+        if (base + q == n - 1) out[n] = off;
+    }
+}
+
+__global__ void k_fill_csc(int64_t n, const int32_t* __restrict__ off, const int32_t* __restrict__ tgt,
+                           int32_t* __restrict__ cursor, int32_t* __restrict__ in_src,
+                           int32_t* __restrict__ tmp_e) {
+    for (int64_t s = blockIdx.x * 256ll + threadIdx.x; s < n; s += (int64_t)gridDim.x * 256)
+        for (int32_t e = off[s]; e < off[s + 1]; ++e) {
+            const int32_t p = atomicAdd(&cursor[tgt[e]], 1);
+            in_src[p] = (int32_t)s;
+            tmp_e[p] = e;
+        }
+}
+
+// one wavefront per target: rank of each (src, e) key within the segment
+__global__ __launch_bounds__(256) void k_sort_csc(int64_t n, const int32_t* __restrict__ in_off,
+                                                  const int32_t* __restrict__ src_unsorted,
+                                                  const int32_t* __restrict__ e_unsorted,
+                                                  const float* __restrict__ w,
+                                                  int32_t* __restrict__ in_src, float* __restrict__ in_w,
+                                                  int32_t* __restrict__ out_to_in) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    for (int64_t t = blockIdx.x * 4ll + (threadIdx.x >> 6); t < n; t += nwaves) {
+        const int32_t b = in_off[t], L = in_off[t + 1] - b;
+        for (int32_t i = lane; i < L; i += 64) {
+            const uint64_t ki = ((uint64_t)(uint32_t)src_unsorted[b + i] << 32) | (uint32_t)e_unsorted[b + i];
+            int32_t rank = 0;
+            for (int32_t j = 0; j < L; ++j) {
+                const uint64_t kj = ((uint64_t)(uint32_t)src_unsorted[b + j] << 32) | (uint32_t)e_unsorted[b + j];
+                rank += kj < ki;
+            }
+            const int32_t p = b + rank, e = (int32_t)(ki & 0xffffffffu);
+            in_src[p] = (int32_t)(ki >> 32);
+            if (in_w) in_w[p] = w ? w[e] : 0.0f;
+            if (out_to_in) out_to_in[e] = p;
+        }
+    }
+}
+
+template <class T> __global__ void k_deg_out(int64_t n, const int32_t* __restrict__ in_off, T* __restrict__ d) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        d[i] = T(in_off[i + 1] - in_off[i]);
+}
+
+}  // namespace nnx
+
+extern "C" nnx_status nnx_graph_transpose(nnx_ctx* c, int64_t n, int64_t nnz, const int32_t* off,
+                                          const int32_t* tgt, const float* w, int32_t* in_off,
+                                          int32_t* in_src, float* in_w, int32_t* out_to_in,
+                                          int deg_dtype, void* in_degree) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, n >= 0 && n < ((int64_t)1 << 31) && nnz >= 0 && nnz < ((int64_t)1 << 31), "size out of range");
+    NNX_CHECK(c, off && in_off && (nnz == 0 || (tgt && in_src)), "null argument");
+    const int64_t ntiles = ceil_div(n, kScanTile);
+    NNX_CHECK(c, ntiles <= kScanTile, "n too large for the single-level tile scan");
+    void *deg, *tsum, *cur, *su, *eu;
+    NNX_TRY(scratch(c, "tr_deg", (n + 1) * 4, false, &deg));
+    NNX_TRY(scratch(c, "tr_tsum", (ntiles + 1) * 4, false, &tsum));
+    NNX_TRY(scratch(c, "tr_cur", (n + 1) * 4, false, &cur));
+    NNX_TRY(scratch(c, "tr_src", (nnz + 1) * 4, false, &su));
+    NNX_TRY(scratch(c, "tr_e", (nnz + 1) * 4, false, &eu));
+    {
+        void* flag;
+        NNX_TRY(scratch(c, "tr_flag", 256, true, &flag));
+        NNX_TRY(fill_i32(c, (int32_t*)flag, 1, 0));
+        const unsigned vb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(std::max(n, nnz), 256), 4096));
+        k_validate_csr<<<vb, 256, 0, c->stream>>>(n, nnz, off, tgt, (int32_t*)flag);
+        NNX_LAUNCHED(c);
+        NNX_HIP(c, hipMemcpyAsync(c->pinned, flag, 4, hipMemcpyDeviceToHost, c->stream));
+        NNX_HIP(c, hipStreamSynchronize(c->stream));
+        const int32_t f = *(int32_t*)c->pinned;
+        if (f & 1) return fail(c, NNX_ERR_ARG, "CSR offsets are not non-decreasing from 0 to nnz");
+        if (f & 2) return fail(c, NNX_ERR_ARG, "CSR target index outside [0, n)");
+    }
+    NNX_TRY(fill_i32(c, (int32_t*)deg, n + 1, 0));
+    const unsigned fb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nnz, 256), 8192));
+    if (nnz) {
+        k_indeg<<<fb, 256, 0, c->stream>>>(nnz, tgt, (int32_t*)deg);
+        NNX_LAUNCHED(c);
+    }
+    if (n) {
+        k_scan_tiles<<<(unsigned)ntiles, kScanThreads, 0, c->stream>>>(n, (int32_t*)deg, (int32_t*)tsum);
+        k_scan_sums<<<1, kScanThreads, 0, c->stream>>>(ntiles, (int32_t*)tsum);
+        k_scan_apply<<<(unsigned)ntiles, kScanThreads, 0, c->stream>>>(n, (int32_t*)deg, (int32_t*)tsum, in_off);
+        NNX_LAUNCHED(c);
+    } else {
+        NNX_HIP(c, hipMemsetAsync(in_off, 0, 4, c->stream));
+    }
+    if (nnz) {
+        NNX_HIP(c, hipMemcpyAsync(cur, in_off, n * 4, hipMemcpyDeviceToDevice, c->stream));
+        const unsigned sb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192));
+        k_fill_csc<<<sb, 256, 0, c->stream>>>(n, off, tgt, (int32_t*)cur, (int32_t*)su, (int32_t*)eu);
+        NNX_LAUNCHED(c);
+        const unsigned wb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 4), 65536));
+        k_sort_csc<<<wb, 256, 0, c->stream>>>(n, in_off, (int32_t*)su, (int32_t*)eu, w, in_src, in_w, out_to_in);
+        NNX_LAUNCHED(c);
+    }
+    if (in_degree && n) {
+        const unsigned db = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096));
+        if (deg_dtype == NNX_F64)
+            k_deg_out<double><<<db, 256, 0, c->stream>>>(n, in_off, (double*)in_degree);
+        else
+            k_deg_out<float><<<db, 256, 0, c->stream>>>(n, in_off, (float*)in_degree);
+        NNX_LAUNCHED(c);
+    }
+    return NNX_OK;
+}

@@ -1,0 +1,158 @@
+// hh_math.h - device-side Hodgkin-Huxley arithmetic (gfx950).
+//
+// Every function is a fixed sequence of IEEE-754 operations (the library is
+// compiled with -ffp-contract=off; fused multiply-adds are written as fma),
+// so a run is reproducible bit for bit and can be checked against a CPU
+// restatement performing the same sequence.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nnx {
+
+// ---------------------------------------------------------------------------
+// exp(x): x = k ln2 + r with |r| <= ln2/2 (Cody-Waite: ln2 = hi + lo, hi with
+// 16 (float) / 42 (double) significant bits so k*hi is exact), e^r by its
+// Taylor polynomial (degree 7 / 13, Horner with fma), times 2^k built in the
+// exponent field.  Arguments are clamped so that the result stays normal.
+// Max error 1 ulp against a correctly rounded exp over the clamped range.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dexp(float x) {
+    if (x != x) return x;
+    x = x > 88.0f ? 88.0f : x;
+    x = x < -80.0f ? -80.0f : x;
+    const float k = __builtin_rintf(x * 0x1.715476p+0f);
+    float r = __builtin_fmaf(-k, 0x1.62e4p-1f, x);
+    r = __builtin_fmaf(-k, 0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;
+    p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);
+    p = __builtin_fmaf(p, r, 0x1.111112p-7f);
+    p = __builtin_fmaf(p, r, 0x1.555556p-5f);
+    p = __builtin_fmaf(p, r, 0x1.555556p-3f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const uint32_t bits = (uint32_t)((int)k + 127) << 23;
+    return p * __builtin_bit_cast(float, bits);
+}
+
+__device__ __forceinline__ double dexp(double x) {
+    if (x != x) return x;
+    x = x > 708.0 ? 708.0 : x;
+    x = x < -700.0 ? -700.0 : x;
+    const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(-k, 0x1.62e42fefa38p-1, x);
+    r = __builtin_fma(-k, 0x1.ef35793c7673p-45, r);
+    double p = 0x1.6124613a86d09p-33;
+    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    const uint64_t bits = (uint64_t)((int64_t)k + 1023) << 52;
+    return p * __builtin_bit_cast(double, bits);
+}
+
+template <class T> __device__ __forceinline__ T dfma(T a, T b, T c);
+template <> __device__ __forceinline__ float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <class T> __device__ __forceinline__ T dabs(T a) { return a < T(0) ? -a : a; }
+
+// a / (1 - e^-a); for |a| < 1/2 the series 1 + a/2 + a^2/12 - a^4/720 +
+// a^6/30240 - a^8/1209600 (Bernoulli numbers of x/(e^x - 1) at x = -a),
+// which removes the 0/0 at a = 0.
+template <class T> __device__ __forceinline__ T vtrap(T a) {
+    if (dabs(a) < T(0.5)) {
+        const T a2 = a * a;
+        T q = dfma(a2, T(-1.0 / 1209600.0), T(1.0 / 30240.0));
+        q = dfma(a2, q, T(-1.0 / 720.0));
+        q = dfma(a2, q, T(1.0 / 12.0));
+        return dfma(a2, q, dfma(a, T(0.5), T(1)));
+    }
+    return a / (T(1) - dexp(-a));
+}
+
+template <class T> struct Rates { T am, bm, ah, bh, an, bn; };
+
+// HH rates of the reference (src/ephys.cu:57-63, written for a -60 mV rest),
+// evaluated at u = V + v_shift.
+template <class T> __device__ __forceinline__ Rates<T> rates(T u) {
+    Rates<T> R;
+    const T a = T(3.5) + T(0.1) * u;
+    R.am = vtrap(a);
+    const T b = T(5) + T(0.1) * u;
+    R.an = T(0.1) * vtrap(b);
+    R.ah = T(0.07) * dexp(-u / T(20) - T(3));
+    R.bm = T(4) * dexp(-(u + T(60)) / T(18));
+    R.bn = T(0.125) * dexp(-(u + T(60)) / T(80));
+    R.bh = T(1) / (dexp(T(-3) - T(0.1) * u) + T(1));
+    return R;
+}
+
+// Model constants in T (cast once on the host from the double parameters).
+template <class T> struct HHConst {
+    T C, gna, gk, gl, ena, ek, el, vshift, esyn;
+    T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean;
+    T v_thresh, out_thresh;
+    // step constants: dt, dt/2, dt/6, exp(-dt/(2 tau)), exp(-dt/tau)
+    T dt, hdt, dt6, eh, ef;
+};
+
+template <class T>
+__device__ __forceinline__ void deriv(const HHConst<T>& p, T V, T m, T h, T n, T s, T iext, T& dV,
+                                      T& dm, T& dh, T& dn) {
+    const Rates<T> R = rates(V + p.vshift);
+    dm = R.am * (T(1) - m) - R.bm * m;
+    dh = R.ah * (T(1) - h) - R.bh * h;
+    dn = R.an * (T(1) - n) - R.bn * n;
+    const T m3h = m * m * m * h;
+    const T n2 = n * n;
+    const T n4 = n2 * n2;
+    const T ina = p.gna * m3h * (V - p.ena);
+    const T ik = p.gk * n4 * (V - p.ek);
+    const T il = p.gl * (V - p.el);
+    const T isyn = s * (V - p.esyn);
+    dV = (iext - ina - ik - il - isyn) / p.C;
+}
+
+// Classical RK4 over (V, m, h, n).  The synaptic conductance jumps by the
+// coupling input at the start of the step and decays exactly within it:
+// s0 = s + g_in, s(dt/2) = s0 eh, s(dt) = s0 ef.
+template <class T>
+__device__ __forceinline__ void rk4(const HHConst<T>& p, T& V, T& m, T& h, T& n, T& s, T iext, T gin) {
+    const T s0 = s + gin;
+    const T sh = s0 * p.eh;
+    const T sf = s0 * p.ef;
+    T k1V, k1m, k1h, k1n, k2V, k2m, k2h, k2n, k3V, k3m, k3h, k3n, k4V, k4m, k4h, k4n;
+    deriv(p, V, m, h, n, s0, iext, k1V, k1m, k1h, k1n);
+    deriv(p, V + p.hdt * k1V, m + p.hdt * k1m, h + p.hdt * k1h, n + p.hdt * k1n, sh, iext, k2V,
+          k2m, k2h, k2n);
+    deriv(p, V + p.hdt * k2V, m + p.hdt * k2m, h + p.hdt * k2h, n + p.hdt * k2n, sh, iext, k3V,
+          k3m, k3h, k3n);
+    deriv(p, V + p.dt * k3V, m + p.dt * k3m, h + p.dt * k3h, n + p.dt * k3n, sf, iext, k4V, k4m,
+          k4h, k4n);
+    V = V + p.dt6 * (k1V + T(2) * k2V + T(2) * k3V + k4V);
+    m = m + p.dt6 * (k1m + T(2) * k2m + T(2) * k3m + k4m);
+    h = h + p.dt6 * (k1h + T(2) * k2h + T(2) * k3h + k4h);
+    n = n + p.dt6 * (k1n + T(2) * k2n + T(2) * k3n + k4n);
+    s = sf;
+}
+
+// Kinetic synapse of the reference (src/ephys.cu:69-76): trapezoid update
+// of Y' = Ar T (1 - Y) - Ad Y with T = IN / COUNT (0 without inputs).
+template <class T>
+__device__ __forceinline__ T kinetic(const HHConst<T>& p, T in, T cnt, T Y) {
+    const T tavg = cnt > T(0) ? in / cnt : T(0);
+    return (p.ar * tavg * p.dt + (T(1) - p.hdt * (p.ar * tavg + p.ad)) * Y) /
+           (p.hdt * (p.ar * tavg + p.ad) + T(1));
+}
+
+}  // namespace nnx

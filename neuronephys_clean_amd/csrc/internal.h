@@ -1,0 +1,98 @@
+// internal.h - context and shared helpers of libnnx.so (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <string>
+
+#include "../../include/nnx.h"
+#include "hh_math.h"
+
+struct nnx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int owns_stream = 0;
+    std::string err;
+    // pinned host words for count / flag read-backs
+    int64_t* pinned = nullptr;
+    // named device scratch buffers, grown on demand, owned by the context
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Buf> bufs;
+    // captured step graphs keyed by (plan address, n_steps)
+    struct Graph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::pair<const void*, int>, Graph> graphs;
+    int capturing = 0;
+};
+
+namespace nnx {
+
+inline nnx_status fail(nnx_ctx* c, nnx_status s, const char* fmt, ...) {
+    char msg[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(msg, sizeof msg, fmt, ap);
+    va_end(ap);
+    if (c) c->err = msg;
+    return s;
+}
+
+#define NNX_HIP(ctx, call)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return nnx::fail((ctx), NNX_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                               \
+    } while (0)
+
+#define NNX_LAUNCHED(ctx) NNX_HIP(ctx, hipGetLastError())
+
+#define NNX_CHECK(ctx, cond, ...)                                  \
+    do {                                                           \
+        if (!(cond)) return nnx::fail((ctx), NNX_ERR_ARG, __VA_ARGS__); \
+    } while (0)
+
+#define NNX_TRY(expr)                      \
+    do {                                   \
+        nnx_status s_ = (expr);            \
+        if (s_ != NNX_OK) return s_;       \
+    } while (0)
+
+// Device scratch buffer `name` of at least `bytes`; grows (never shrinks).
+// New memory is zero-filled when `zero` is set.  Not callable while a graph
+// is being captured if the buffer would have to grow.
+nnx_status scratch(nnx_ctx* c, const char* name, size_t bytes, bool zero, void** out);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+template <class T> HHConst<T> make_const(const nnx_hh_params* q, double dt);
+
+// fill kernel (graph-safe replacement for hipMemsetAsync)
+nnx_status fill_i32(nnx_ctx* c, int32_t* p, int64_t n, int32_t v);
+
+// optional work folded into the spike compaction
+struct CompactExtras {
+    int32_t* raster;      // (step, neuron) pairs
+    int64_t* raster_len;  // device count of pairs written so far
+    int64_t raster_cap;
+    int64_t* step;        // device step counter (raster step numbers); advanced by one per call
+    const int32_t* voxel_of;
+    int32_t* voxel_counts;  // +1 per spike at voxel_of[neuron]
+};
+
+nnx_status compact_launch(nnx_ctx* c, int64_t n, const uint64_t* bits, int32_t* list,
+                          int32_t* count, const CompactExtras& ex);
+nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, const uint64_t* bits,
+                         const int32_t* list, const int32_t* count, const void* outputs,
+                         void* g_in);
+nnx_status check_graph(nnx_ctx* c, const nnx_graph* g, int mode);
+nnx_status event_scratch(nnx_ctx* c, int64_t n, int64_t nnz, int32_t** cnt, int32_t** slot);
+
+}  // namespace nnx

@@ -1,0 +1,116 @@
+// step.hip - the composite timestep and its hipGraph.
+//
+// One step in the reference's order (src/main.cu:166-181,
+// src_wholebrain/mouse_cortex.cu:104-120): coupling on the previous step's
+// spikes, release accumulation for the paracrine grid, the ephys update,
+// then the ascending spike list (with raster and release counts folded into
+// the same workgroup).  The reference synchronises the device after every
+// launch and reads the firing count back each step; here a whole block of
+// steps is enqueued (or replayed from a captured graph) without a host sync.
+#include <cstring>
+
+#include "internal.h"
+
+namespace nnx {
+
+__global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
+
+static uint64_t plan_hash(const nnx_step_plan* p) {
+    const unsigned char* b = (const unsigned char*)p;
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < sizeof(*p); ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
+    NNX_CHECK(c, p, "null plan");
+    const nnx_hh_state& s = p->state;
+    NNX_CHECK(c, s.n == p->graph.n, "state n (%lld) != graph n (%lld)", (long long)s.n,
+              (long long)p->graph.n);
+    NNX_TRY(check_graph(c, &p->graph, p->couple_mode));
+    NNX_CHECK(c, p->bits[0] && p->bits[1] && p->list && p->count, "plan needs bits[2], list, count");
+    if (p->integrator == NNX_INT_RK4) {
+        NNX_CHECK(c, p->g_in, "RK4 plan needs g_in");
+        NNX_CHECK(c, p->couple_mode == NNX_COUPLE_WEIGHTED_PULL || p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT,
+                  "RK4 plan needs weighted coupling");
+    } else {
+        NNX_CHECK(c, s.in_local, "leapfrog plan writes the coupling into state.in_local");
+        NNX_CHECK(c, p->couple_mode == NNX_COUPLE_COUNT || p->couple_mode == NNX_COUPLE_SIGNAL,
+                  "leapfrog plan needs COUNT or SIGNAL coupling");
+        if (p->couple_mode == NNX_COUPLE_SIGNAL) NNX_CHECK(c, s.out_local, "SIGNAL needs out_local");
+    }
+    if (p->record) NNX_CHECK(c, p->raster && p->raster_len && p->raster_cap >= 0, "record needs raster buffers");
+    if (p->voxel_of) NNX_CHECK(c, p->voxel_counts, "voxel_of needs voxel_counts");
+    if (p->params.noise_std != 0.0)
+        return fail(c, NNX_ERR_UNSUPPORTED, "white noise is not implemented (noise_std must be 0)");
+    return NNX_OK;
+}
+
+static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, int64_t* step_ctr) {
+    const uint64_t* prev = p->bits[(s + 1) & 1];
+    uint64_t* cur = p->bits[s & 1];
+    const bool rk4 = p->integrator == NNX_INT_RK4;
+    void* target = rk4 ? p->g_in : (void*)p->state.in_local;
+    NNX_TRY(couple_launch(c, &p->graph, p->couple_mode, p->state.dtype, prev, p->list, p->count,
+                          p->state.out_local, target));
+    NNX_TRY(nnx_hh_step(c, &p->state, &p->params, p->integrator, p->spike_rule, p->dt, s,
+                        rk4 ? p->g_in : nullptr, cur));
+    CompactExtras ex{};
+    if (p->record) {
+        ex.raster = p->raster;
+        ex.raster_len = p->raster_len;
+        ex.raster_cap = p->raster_cap;
+    }
+    ex.step = step_ctr;
+    ex.voxel_of = p->voxel_of;
+    ex.voxel_counts = p->voxel_counts;
+    return compact_launch(c, p->state.n, cur, p->list, p->count, ex);
+}
+
+}  // namespace nnx
+
+using namespace nnx;
+
+extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first_step, int n_steps,
+                                    int use_graph) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_plan(c, p));
+    NNX_CHECK(c, n_steps >= 0, "n_steps < 0");
+    if (n_steps == 0 || p->state.n == 0) return NNX_OK;
+    // everything a step touches is allocated before any capture
+    if (p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT) {
+        int32_t *a, *b;
+        NNX_TRY(event_scratch(c, p->graph.n, p->graph.nnz, &a, &b));
+    }
+    void* ctr;
+    NNX_TRY(scratch(c, "step_ctr", 64, true, &ctr));
+    int64_t* step_ctr = (int64_t*)ctr;
+    k_set_i64<<<1, 1, 0, c->stream>>>(step_ctr, (int64_t)first_step);
+    NNX_LAUNCHED(c);
+    if (!use_graph) {
+        for (int i = 0; i < n_steps; ++i) NNX_TRY(enqueue_step(c, p, first_step + i, step_ctr));
+        return NNX_OK;
+    }
+    NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
+    const std::pair<const void*, int> key((const void*)(uintptr_t)plan_hash(p),
+                                          n_steps * 2 + (int)(first_step & 1));
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        nnx_ctx::Graph g;
+        NNX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = 1;
+        nnx_status st = NNX_OK;
+        for (int i = 0; i < n_steps && st == NNX_OK; ++i) st = enqueue_step(c, p, first_step + i, step_ctr);
+        c->capturing = 0;
+        hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
+        if (st != NNX_OK) {
+            if (g.graph) hipGraphDestroy(g.graph);
+            return st;
+        }
+        if (e != hipSuccess) return fail(c, NNX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        NNX_HIP(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        it = c->graphs.emplace(key, g).first;
+    }
+    NNX_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
+    return NNX_OK;
+}

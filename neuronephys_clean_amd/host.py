@@ -1,0 +1,133 @@
+"""numpy wrappers of the host builders (include/nnx_host.h, libnnx_host.so)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import host
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = (C.c_uint64 * 4)(*[int(v) for v in ctr])
+    k = (C.c_uint64 * 2)(*[int(v) for v in key])
+    o = (C.c_uint64 * 4)()
+    host().nnx_philox4x64_10(c, k, o)
+    return np.array(list(o), dtype=np.uint64)
+
+
+def uniform(seed, purpose, n, lo=0.0, hi=1.0) -> np.ndarray:
+    out = np.empty(int(n), np.float64)
+    host().nnx_gen_uniform(seed, purpose, int(n), lo, hi, _p(out))
+    return out
+
+
+def normal(seed, purpose, n, mu=0.0, sigma=1.0) -> np.ndarray:
+    out = np.empty(int(n), np.float64)
+    host().nnx_gen_normal(seed, purpose, int(n), mu, sigma, _p(out))
+    return out
+
+
+def lognormal(seed, purpose, n, mu=0.0, sigma=1.0) -> np.ndarray:
+    out = np.empty(int(n), np.float64)
+    host().nnx_gen_lognormal(seed, purpose, int(n), mu, sigma, _p(out))
+    return out
+
+
+def er_fixed(seed, n, k, allow_self=False):
+    off = np.empty(n + 1, np.int32)
+    tgt = np.empty(n * k, np.int32)
+    r = host().nnx_gen_er_fixed(seed, n, k, int(allow_self), _p(off), _p(tgt))
+    if r != 0:
+        raise ValueError(f"out-degree {k} exceeds the candidates of {n} neurons")
+    return off, tgt
+
+
+def uniform_targets(seed, off, allow_self=False):
+    n = len(off) - 1
+    tgt = np.empty(int(off[-1]), np.int32)
+    if host().nnx_gen_uniform_targets(seed, n, _p(off), int(allow_self), _p(tgt)) != 0:
+        raise ValueError("an out-degree exceeds the number of candidates")
+    return tgt
+
+
+def outdeg_lognormal(seed, n, mu, sigma, cap):
+    d = np.empty(n, np.int32)
+    host().nnx_gen_outdeg_lognormal(seed, n, mu, sigma, cap, _p(d))
+    return d
+
+
+def outdeg_poisson(seed, n, lam):
+    d = np.empty(n, np.int32)
+    host().nnx_gen_outdeg_poisson(seed, n, lam, _p(d))
+    return d
+
+
+def offsets_of(deg) -> np.ndarray:
+    off = np.zeros(len(deg) + 1, np.int64)
+    np.cumsum(deg, out=off[1:])
+    if off[-1] >= 2 ** 31:
+        raise ValueError("more than 2^31 - 1 edges")
+    return off.astype(np.int32)
+
+
+def spatial_targets(seed, pos, box, lam, cutoff, off):
+    pos = np.ascontiguousarray(pos, np.float32)
+    n = pos.shape[0]
+    tgt = np.empty(int(off[-1]), np.int32)
+    r = host().nnx_gen_spatial_targets(seed, n, _p(pos), box, lam, cutoff, _p(off), _p(tgt))
+    if r != 0:
+        raise ValueError(f"{-r} sources had fewer candidates than their out-degree")
+    return tgt
+
+
+def csr_to_csc(n, off, tgt, w=None):
+    nnz = len(tgt)
+    in_off = np.empty(n + 1, np.int32)
+    in_src = np.empty(nnz, np.int32)
+    in_w = np.empty(nnz, np.float32)
+    o2i = np.empty(nnz, np.int32)
+    deg = np.empty(n, np.float32)
+    w = None if w is None else np.ascontiguousarray(w, np.float32)
+    host().nnx_csr_to_csc(n, nnz, _p(off), _p(tgt), _p(w), _p(in_off), _p(in_src), _p(in_w),
+                          _p(o2i), _p(deg))
+    return in_off, in_src, in_w, o2i, deg
+
+
+def steady_state(V, v_shift):
+    V = np.ascontiguousarray(V, np.float64)
+    m, h, ng = (np.empty_like(V) for _ in range(3))
+    host().nnx_hh_steady_state(len(V), _p(V), v_shift, _p(m), _p(h), _p(ng))
+    return m, h, ng
+
+
+def voxel_of(pos, h, nx, ny, nz):
+    pos = np.ascontiguousarray(pos, np.float32)
+    out = np.empty(pos.shape[0], np.int32)
+    host().nnx_voxel_of(pos.shape[0], _p(pos), h, nx, ny, nz, _p(out))
+    return out
+
+
+def trilinear_matrix(x, y, z, nx, ny, nz, h):
+    x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+    n = len(x)
+    off = np.empty(n + 1, np.int32)
+    cols = np.empty(8 * n, np.int32)
+    vals = np.empty(8 * n, np.float32)
+    outside = host().nnx_trilinear_matrix(n, _p(x), _p(y), _p(z), nx, ny, nz, h, _p(off),
+                                          _p(cols), _p(vals))
+    return off, cols, vals, int(outside)
+
+
+def csr_transpose(rows, cols, off, idx, val):
+    nnz = int(off[-1])
+    t_off = np.empty(cols + 1, np.int32)
+    t_idx = np.empty(nnz, np.int32)
+    t_val = np.empty(nnz, np.float32)
+    host().nnx_csr_transpose(rows, cols, _p(off), _p(idx), _p(np.ascontiguousarray(val, np.float32)),
+                             _p(t_off), _p(t_idx), _p(t_val))
+    return t_off, t_idx, t_val

@@ -1,0 +1,175 @@
+"""CPU oracle (test infrastructure only).
+
+ctypes front end of liboracle.so, the from-scratch C++ restatement of the
+reference's per-timestep algorithm (oracle/oracle.hpp).  Only tests/, the
+smoke check and bench.py's CPU-baseline leg import this package; the product
+never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+PARAM_FIELDS = ("C", "gna", "gk", "gl", "ena", "ek", "el", "v_shift", "tau_syn", "e_syn",
+                "tmax", "vt", "kp", "ar", "ad", "g_local", "e_local", "g_i", "e_i", "g_e",
+                "e_e", "noise_mean", "noise_std", "noise_seed", "v_thresh", "out_thresh")
+
+
+class Params(C.Structure):
+    _fields_ = [(f, C.c_uint64 if f == "noise_seed" else C.c_double) for f in PARAM_FIELDS]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("stencil", C.c_int32),
+                ("c", C.c_double), ("decay_dt", C.c_double), ("kx", C.c_void_p),
+                ("ky", C.c_void_p), ("kz", C.c_void_p)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.orc_run_rk4_f32.restype = C.c_int64
+        _lib.orc_run_rk4_f64.restype = C.c_int64
+        _lib.orc_run_leapfrog_f32.restype = C.c_int64
+        _lib.orc_run_leapfrog_f64.restype = C.c_int64
+        for f in ("orc_cg_f32", "orc_cg_f64", "orc_refine", "orc_update_density_reference"):
+            getattr(_lib, f).restype = C.c_int
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def params(d) -> Params:
+    """Params from a dict or any object with the PARAM_FIELDS attributes."""
+    p = Params()
+    for f in PARAM_FIELDS:
+        setattr(p, f, d[f] if isinstance(d, dict) else getattr(d, f))
+    return p
+
+
+def grid(nx, ny, nz, stencil=7, c=1.0, decay_dt=0.0, kx=None, ky=None, kz=None) -> Grid:
+    return Grid(nx, ny, nz, stencil, c, decay_dt, _p(kx), _p(ky), _p(kz))
+
+
+def _sfx(dtype):
+    return "f32" if np.dtype(dtype) == np.float32 else "f64"
+
+
+def exp(x):
+    x = np.ascontiguousarray(x)
+    y = np.empty_like(x)
+    getattr(lib(), "orc_exp_" + _sfx(x.dtype))(_p(x), _p(y), C.c_int64(x.size))
+    return y
+
+
+def rates(u):
+    u = np.ascontiguousarray(u)
+    out = np.empty((u.size, 6), u.dtype)
+    getattr(lib(), "orc_rates_" + _sfx(u.dtype))(_p(u), _p(out), C.c_int64(u.size))
+    return out  # am, bm, ah, bh, an, bn
+
+
+def hh_rk4(p, dt, V, m, h, ng, s, iext, gin=None):
+    """One RK4 step in place; returns the 0/1 crossing flags."""
+    n = V.size
+    spk = np.zeros(n, np.uint8)
+    getattr(lib(), "orc_hh_rk4_" + _sfx(V.dtype))(C.c_int64(n), C.byref(p), C.c_double(dt),
+                                                   _p(V), _p(m), _p(h), _p(ng), _p(s), _p(iext),
+                                                   _p(gin), _p(spk))
+    return spk
+
+
+def hh_leapfrog(p, dt, sv):
+    """One reference leapfrog step over the 17 SoA arrays (list, in place)."""
+    n = sv[0].size
+    spk = np.zeros(n, np.uint8)
+    arr = (C.c_void_p * 17)(*[a.ctypes.data for a in sv])
+    getattr(lib(), "orc_hh_leapfrog_" + _sfx(sv[0].dtype))(C.c_int64(n), C.byref(p),
+                                                            C.c_double(dt), arr, _p(spk))
+    return spk
+
+
+def couple(off, tgt, w, fire, mode, out=None, dtype=np.float32):
+    n = len(off) - 1
+    g = np.empty(n, dtype)
+    out = None if out is None else np.ascontiguousarray(out, dtype)
+    getattr(lib(), "orc_couple_" + _sfx(dtype))(C.c_int64(n), _p(off), _p(tgt), _p(w),
+                                                _p(np.ascontiguousarray(fire, np.uint8)), mode,
+                                                _p(out), _p(g))
+    return g
+
+
+def apply(g: Grid, x):
+    x = np.ascontiguousarray(x)
+    y = np.empty_like(x)
+    getattr(lib(), "orc_apply_" + _sfx(x.dtype))(C.byref(g), _p(x), _p(y))
+    return y
+
+
+def cg(g: Grid, b, rtol, maxit=10000, jacobi=True, x0=None):
+    b = np.ascontiguousarray(b)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, b.dtype)
+    rel = C.c_double()
+    it = getattr(lib(), "orc_cg_" + _sfx(b.dtype))(C.byref(g), _p(b), _p(x), C.c_double(rtol),
+                                                   maxit, int(jacobi), int(x0 is not None),
+                                                   C.byref(rel))
+    return x, it, rel.value
+
+
+def refine(g: Grid, b, rtol, inner_rtol, maxit=10000, max_outer=50):
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b)
+    rel, inner = C.c_double(), C.c_int()
+    outer = lib().orc_refine(C.byref(g), _p(b), _p(x), C.c_double(rtol), C.c_double(inner_rtol),
+                             maxit, max_outer, C.byref(rel), C.byref(inner))
+    return x, outer, inner.value, rel.value
+
+
+def update_density_reference(data, h, D, k, dt, eb=1e-6, max_iter=20):
+    data = np.array(data, np.float32, order="C")
+    nz, ny, nx = data.shape
+    rr = C.c_double()
+    it = lib().orc_update_density_reference(nx, ny, nz, C.c_double(h), C.c_double(D),
+                                            C.c_double(k), C.c_double(dt), C.c_double(eb),
+                                            max_iter, _p(data), C.byref(rr))
+    return data, it, rr.value
+
+
+def run_rk4(p, dt, steps, off, tgt, w, V, m, h, ng, s, iext, cap=1 << 20, voxel_of=None,
+            nvox=0):
+    """Full RK4 run (state updated in place); returns raster pairs, total
+    count, voxel counts and the wall seconds of the step loop."""
+    T = V.dtype
+    raster = np.zeros(2 * cap, np.int32)
+    vc = np.zeros(max(nvox, 1), np.int32)
+    sec = C.c_double()
+    ln = getattr(lib(), "orc_run_rk4_" + _sfx(T))(
+        C.c_int64(V.size), C.byref(p), C.c_double(dt), C.c_int64(steps), _p(off), _p(tgt), _p(w),
+        _p(V), _p(m), _p(h), _p(ng), _p(s), _p(iext), _p(raster), C.c_int64(cap), _p(voxel_of),
+        _p(vc), C.byref(sec))
+    return raster.reshape(-1, 2)[:min(ln, cap)], ln, vc, sec.value
+
+
+def run_leapfrog(p, dt, steps, off, tgt, mode, sv, cap=1 << 20):
+    raster = np.zeros(2 * cap, np.int32)
+    arr = (C.c_void_p * 17)(*[a.ctypes.data for a in sv])
+    ln = getattr(lib(), "orc_run_leapfrog_" + _sfx(sv[0].dtype))(
+        C.c_int64(sv[0].size), C.byref(p), C.c_double(dt), C.c_int64(steps), _p(off), _p(tgt),
+        mode, arr, _p(raster), C.c_int64(cap))
+    return raster.reshape(-1, 2)[:min(ln, cap)], ln

@@ -1,0 +1,451 @@
+// oracle.hpp - CPU restatement of the neuronephys per-timestep algorithm.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product (neuronephys_clean_amd/,
+// include/) includes, links or calls this file; only tests/, the smoke check
+// and bench.py's CPU-baseline leg load it (through liboracle.so).
+//
+// Templated on T in {float, double}.  Built with -ffp-contract=off and
+// explicit std::fma wherever a fused multiply-add is meant, so that every
+// result is a fixed sequence of IEEE-754 operations: the device kernels
+// perform the same sequence and must agree bit for bit.  Accuracy of that
+// sequence is pinned separately (libm, scipy DOP853, closed forms).
+//
+// Reference files restated (cited per function below):
+//   src/ephys.cu:28-96          Leapfrog_HH
+//   src/defines.cuh:87-102      hh_params
+//   src/coupling.cu:34-48,103-178, src_wholebrain/coupling.cu:12-69
+//   src/network_factory.cu:5-27 CSR + in-degree
+//   src/paracrine.cu:19-134     27-point CN-CG
+//   src/main.cu:166-181         step order coupling -> paracrine -> ephys
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace oracle {
+
+// --------------------------------------------------------------------------
+// exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, Taylor polynomial of
+// e^r by Horner with fma, scale by 2^k through the exponent field.
+// ln2 is split so that k*ln2_hi is exact (16 bits for float, 42 for double).
+// The argument is clamped to [xmin, xmax] so the result stays normal.
+// --------------------------------------------------------------------------
+template <class T> struct ExpK;
+template <> struct ExpK<float> {
+    static constexpr float log2e = 0x1.715476p+0f;
+    static constexpr float ln2_hi = 0x1.62e4p-1f;
+    static constexpr float ln2_lo = 0x1.7f7d1cp-20f;
+    static constexpr float xmax = 88.0f;
+    static constexpr float xmin = -80.0f;
+    static constexpr int deg = 7;
+    static constexpr float c[8] = {1.0f, 1.0f, 0.5f, 0x1.555556p-3f, 0x1.555556p-5f,
+                                   0x1.111112p-7f, 0x1.6c16c2p-10f, 0x1.a01a02p-13f};
+};
+template <> struct ExpK<double> {
+    static constexpr double log2e = 0x1.71547652b82fep+0;
+    static constexpr double ln2_hi = 0x1.62e42fefa38p-1;
+    static constexpr double ln2_lo = 0x1.ef35793c7673p-45;
+    static constexpr double xmax = 708.0;
+    static constexpr double xmin = -700.0;
+    static constexpr int deg = 13;
+    static constexpr double c[14] = {1.0, 1.0, 0.5, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+                                     0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10,
+                                     0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-16,
+                                     0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22,
+                                     0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29,
+                                     0x1.6124613a86d09p-33};
+};
+
+inline float pow2i(float, int k) {
+    uint32_t b = (uint32_t)(k + 127) << 23;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+inline double pow2i(double, int k) {
+    uint64_t b = (uint64_t)(k + 1023) << 52;
+    double f;
+    std::memcpy(&f, &b, 8);
+    return f;
+}
+
+template <class T> inline T oexp(T x) {
+    using K = ExpK<T>;
+    if (x != x) return x;
+    x = x > K::xmax ? K::xmax : x;
+    x = x < K::xmin ? K::xmin : x;
+    T k = std::rint(x * K::log2e);
+    T r = std::fma(-k, K::ln2_hi, x);
+    r = std::fma(-k, K::ln2_lo, r);
+    T p = K::c[K::deg];
+    for (int i = K::deg - 1; i >= 0; --i) p = std::fma(p, r, K::c[i]);
+    return p * pow2i(T(0), (int)k);
+}
+
+// a / (1 - e^-a), with the Bernoulli series 1 + a/2 + a^2/12 - a^4/720 +
+// a^6/30240 - a^8/1209600 for |a| < 1/2 (removes the 0/0 of the reference's
+// alpha_m and alpha_n at V = -35 and V = -50, src/ephys.cu:57-58).
+template <class T> inline T vtrap(T a) {
+    if (std::fabs(a) < T(0.5)) {
+        T a2 = a * a;
+        T q = std::fma(a2, T(-1.0 / 1209600.0), T(1.0 / 30240.0));
+        q = std::fma(a2, q, T(-1.0 / 720.0));
+        q = std::fma(a2, q, T(1.0 / 12.0));
+        return std::fma(a2, q, std::fma(a, T(0.5), T(1)));
+    }
+    return a / (T(1) - oexp(-a));
+}
+
+template <class T> struct Rates { T am, bm, ah, bh, an, bn; };
+
+// The reference's six HH rates (src/ephys.cu:57-63), -60 mV-rest form,
+// evaluated at u = V + v_shift.
+template <class T> inline Rates<T> rates(T u) {
+    Rates<T> R;
+    T a = T(3.5) + T(0.1) * u;
+    R.am = vtrap(a);
+    T b = T(5) + T(0.1) * u;
+    R.an = T(0.1) * vtrap(b);
+    R.ah = T(0.07) * oexp(-u / T(20) - T(3));
+    R.bm = T(4) * oexp(-(u + T(60)) / T(18));
+    R.bn = T(0.125) * oexp(-(u + T(60)) / T(80));
+    R.bh = T(1) / (oexp(T(-3) - T(0.1) * u) + T(1));
+    return R;
+}
+
+// Parameters cast to T once (the host passes doubles; both sides cast the
+// same doubles, so both see the same T values).
+template <class T> struct P {
+    T C, gna, gk, gl, ena, ek, el, vshift, esyn;
+    T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean;
+    T v_thresh, out_thresh;
+};
+
+// dy/dt of the RK4 model (V, m, h, n) with synaptic conductance s.
+template <class T>
+inline void hh_deriv(const P<T>& p, T V, T m, T h, T n, T s, T iext, T& dV, T& dm, T& dh, T& dn) {
+    Rates<T> R = rates(V + p.vshift);
+    dm = R.am * (T(1) - m) - R.bm * m;
+    dh = R.ah * (T(1) - h) - R.bh * h;
+    dn = R.an * (T(1) - n) - R.bn * n;
+    T m3h = m * m * m * h;
+    T n2 = n * n;
+    T n4 = n2 * n2;
+    T ina = p.gna * m3h * (V - p.ena);
+    T ik = p.gk * n4 * (V - p.ek);
+    T il = p.gl * (V - p.el);
+    T isyn = s * (V - p.esyn);
+    dV = (iext - ina - ik - il - isyn) / p.C;
+}
+
+// Classical RK4 over (V, m, h, n); s is integrated exactly: s0 = s + g_in at
+// the start of the step, s(t) = s0 e^{-t/tau}; eh = e^{-dt/(2 tau)},
+// ef = e^{-dt/tau}, hdt = dt/2, dt6 = dt/6 (all cast from double by the host).
+template <class T>
+inline void rk4_step(const P<T>& p, T dt, T hdt, T dt6, T eh, T ef, T& V, T& m, T& h, T& n, T& s,
+                     T iext, T gin) {
+    T s0 = s + gin;
+    T sh = s0 * eh;
+    T sf = s0 * ef;
+    T k1V, k1m, k1h, k1n, k2V, k2m, k2h, k2n, k3V, k3m, k3h, k3n, k4V, k4m, k4h, k4n;
+    hh_deriv(p, V, m, h, n, s0, iext, k1V, k1m, k1h, k1n);
+    hh_deriv(p, V + hdt * k1V, m + hdt * k1m, h + hdt * k1h, n + hdt * k1n, sh, iext, k2V, k2m,
+             k2h, k2n);
+    hh_deriv(p, V + hdt * k2V, m + hdt * k2m, h + hdt * k2h, n + hdt * k2n, sh, iext, k3V, k3m,
+             k3h, k3n);
+    hh_deriv(p, V + dt * k3V, m + dt * k3m, h + dt * k3h, n + dt * k3n, sf, iext, k4V, k4m, k4h,
+             k4n);
+    V = V + dt6 * (k1V + T(2) * k2V + T(2) * k3V + k4V);
+    m = m + dt6 * (k1m + T(2) * k2m + T(2) * k3m + k4m);
+    h = h + dt6 * (k1h + T(2) * k2h + T(2) * k3h + k4h);
+    n = n + dt6 * (k1n + T(2) * k2n + T(2) * k3n + k4n);
+    s = sf;
+}
+
+// Reference semi-implicit ("leapfrog") update, src/ephys.cu:57-94: gates by
+// the trapezoid rule with rates frozen at V_n, three kinetic synapses with
+// T = IN/COUNT, then V linear in the new G and E, then the sigmoid output.
+// powf(N, 4) is written (N*N)*(N*N).  All arithmetic is in T.
+template <class T> struct LeapfrogIO {
+    T V, M, N, H, Yl, Yi, Ye;
+    T in_l, cnt_l, in_i, cnt_i, in_e, cnt_e, appcur;
+    T out;
+};
+template <class T> inline T kinetic(const P<T>& p, T dt, T hdt, T in, T cnt, T Y) {
+    T tavg = cnt > T(0) ? in / cnt : T(0);
+    return (p.ar * tavg * dt + (T(1) - hdt * (p.ar * tavg + p.ad)) * Y) /
+           (hdt * (p.ar * tavg + p.ad) + T(1));
+}
+template <class T> inline void leapfrog_step(const P<T>& p, T dt, T hdt, T noise, LeapfrogIO<T>& s) {
+    Rates<T> R = rates(s.V + p.vshift);
+    T nM = (R.am * dt + (T(1) - hdt * (R.am + R.bm)) * s.M) / (hdt * (R.am + R.bm) + T(1));
+    T nN = (R.an * dt + (T(1) - hdt * (R.an + R.bn)) * s.N) / (hdt * (R.an + R.bn) + T(1));
+    T nH = (R.ah * dt + (T(1) - hdt * (R.ah + R.bh)) * s.H) / (hdt * (R.ah + R.bh) + T(1));
+    T nYl = kinetic(p, dt, hdt, s.in_l, s.cnt_l, s.Yl);
+    T nYi = kinetic(p, dt, hdt, s.in_i, s.cnt_i, s.Yi);
+    T nYe = kinetic(p, dt, hdt, s.in_e, s.cnt_e, s.Ye);
+    T m3h = p.gna * nM * nM * nM * nH;
+    T n2 = nN * nN;
+    T n4 = p.gk * (n2 * n2);
+    T G = m3h + n4 + p.gl + p.g_local * nYl + p.g_i * nYi + p.g_e * nYe;
+    T E = m3h * p.ena + n4 * p.ek + p.gl * p.el + p.g_local * nYl * p.e_local +
+          p.g_i * nYi * p.e_i + p.g_e * nYe * p.e_e;
+    T Vn = ((noise + s.appcur) * dt + E * dt + (T(1) - hdt * G) * s.V) / (T(1) + hdt * G);
+    s.V = Vn;
+    s.M = nM;
+    s.N = nN;
+    s.H = nH;
+    s.Yl = nYl;
+    s.Yi = nYi;
+    s.Ye = nYe;
+    s.out = p.tmax / (T(1) + oexp(-(Vn - p.vt) / p.kp));
+}
+
+// --------------------------------------------------------------------------
+// coupling (src/coupling.cu:34-48,111-141; src_wholebrain/coupling.cu:12-69)
+// --------------------------------------------------------------------------
+inline void compact(int64_t n, const uint8_t* fire, std::vector<int32_t>& list) {
+    list.clear();
+    for (int64_t i = 0; i < n; ++i)
+        if (fire[i]) list.push_back((int32_t)i);
+}
+
+// mode 0/1 weighted (w), 2 count (+1), 3 signal (+out[src]); the sum for each
+// target runs over its spiking sources in ascending order (the CSR is walked
+// source by source in ascending order).  g is overwritten.
+template <class T>
+inline void couple_push(int64_t n, const int32_t* off, const int32_t* tgt, const float* w,
+                        const std::vector<int32_t>& list, int mode, const T* out, T* g) {
+    for (int64_t i = 0; i < n; ++i) g[i] = T(0);
+    for (int32_t j : list)
+        for (int32_t e = off[j]; e < off[j + 1]; ++e) {
+            T add = mode == 2 ? T(1) : mode == 3 ? out[j] : T(w[e]);
+            g[tgt[e]] = g[tgt[e]] + add;
+        }
+}
+
+// --------------------------------------------------------------------------
+// grid operators (x fastest: i + nx*(j + ny*k), Dirichlet 0 outside)
+// --------------------------------------------------------------------------
+struct Grid {
+    int nx, ny, nz, stencil;
+    double c, decay_dt;
+    const float *kx, *ky, *kz;  // face coefficients (nx+1)*ny*nz, nx*(ny+1)*nz, nx*ny*(nz+1)
+};
+
+template <class T> inline T at(const T* x, const Grid& g, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return T(0);
+    return x[i + (int64_t)g.nx * (j + (int64_t)g.ny * k)];
+}
+
+// Reference 27-point stencil sum (weights corner 1, edge 3, face 14, centre
+// -128; src/paracrine.cu:40): ((corners + 3*edges) + 14*faces) - 128*x, each
+// group accumulated in (dk, dj, di) lexicographic order from -1 to 1.
+template <class T> inline T s27(const T* x, const Grid& g, int i, int j, int k) {
+    T corners = T(0), edges = T(0), faces = T(0);
+    for (int dk = -1; dk <= 1; ++dk)
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int di = -1; di <= 1; ++di) {
+                int nzc = (di != 0) + (dj != 0) + (dk != 0);
+                if (nzc == 0) continue;
+                T v = at(x, g, i + di, j + dj, k + dk);
+                if (nzc == 3) corners = corners + v;
+                else if (nzc == 2) edges = edges + v;
+                else faces = faces + v;
+            }
+    return ((corners + T(3) * edges) + T(14) * faces) - T(128) * at(x, g, i, j, k);
+}
+
+// y = A x.  STENCIL_7 constant: y = diag*x - c*(((((xm + xp) + ym) + yp) + zm) + zp).
+// STENCIL_7 variable: y = (1 + sum faces)*x - (((((kxm xm + kxp xp) + kym ym) + kyp yp) + kzm zm) + kzp zp).
+// STENCIL_27 (reference CN): y = a0*x - w*(((corners + 3*edges) + 14*faces) - 128*x),
+// a0 = 1 + decay_dt/2, w = c/60 (= (dt D / 2) / (30 h^2), src/paracrine.cu:40-43,80-81).
+template <class T> inline void apply(const Grid& g, const T* x, T* y) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                int64_t id = i + (int64_t)nx * (j + (int64_t)ny * k);
+                T xc = x[id];
+                if (g.stencil == 7 && !g.kx) {
+                    T c = T(g.c), diag = T(1.0 + 6.0 * g.c);
+                    T s = at(x, g, i - 1, j, k) + at(x, g, i + 1, j, k);
+                    s = s + at(x, g, i, j - 1, k);
+                    s = s + at(x, g, i, j + 1, k);
+                    s = s + at(x, g, i, j, k - 1);
+                    s = s + at(x, g, i, j, k + 1);
+                    y[id] = diag * xc - c * s;
+                } else if (g.stencil == 7) {
+                    int64_t fx = i + (int64_t)(nx + 1) * (j + (int64_t)ny * k);
+                    int64_t fy = i + (int64_t)nx * (j + (int64_t)(ny + 1) * k);
+                    int64_t fz = id;
+                    int64_t sz = (int64_t)nx * ny;
+                    T kxm = T(g.kx[fx]), kxp = T(g.kx[fx + 1]);
+                    T kym = T(g.ky[fy]), kyp = T(g.ky[fy + nx]);
+                    T kzm = T(g.kz[fz]), kzp = T(g.kz[fz + sz]);
+                    T d = T(1) + kxm + kxp + kym + kyp + kzm + kzp;
+                    T s = kxm * at(x, g, i - 1, j, k) + kxp * at(x, g, i + 1, j, k);
+                    s = s + kym * at(x, g, i, j - 1, k);
+                    s = s + kyp * at(x, g, i, j + 1, k);
+                    s = s + kzm * at(x, g, i, j, k - 1);
+                    s = s + kzp * at(x, g, i, j, k + 1);
+                    y[id] = d * xc - s;
+                } else {
+                    T a0 = T(1.0 + 0.5 * g.decay_dt), w = T(g.c / 60.0);
+                    y[id] = a0 * xc - w * s27(x, g, i, j, k);
+                }
+            }
+}
+
+template <class T> inline double diag_of(const Grid& g, int64_t id) {
+    if (g.stencil == 27) return 1.0 + 0.5 * g.decay_dt + 128.0 * g.c / 60.0;
+    if (!g.kx) return 1.0 + 6.0 * g.c;
+    int64_t nx = g.nx, ny = g.ny;
+    int64_t i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
+    int64_t fx = i + (nx + 1) * (j + ny * k), fy = i + nx * (j + (ny + 1) * k);
+    return 1.0 + (double)g.kx[fx] + g.kx[fx + 1] + g.ky[fy] + g.ky[fy + nx] + g.kz[id] +
+           g.kz[id + nx * ny];
+}
+
+// FP64 dot in fixed blocks of 4096 summed in order: independent of threads.
+template <class A, class B> inline double dot(const A* a, const B* b, int64_t n) {
+    const int64_t bs = 4096, nb = (n + bs - 1) / bs;
+    std::vector<double> part(nb);
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < nb; ++q) {
+        double s = 0;
+        for (int64_t i = q * bs; i < std::min(n, (q + 1) * bs); ++i) s += (double)a[i] * (double)b[i];
+        part[q] = s;
+    }
+    double s = 0;
+    for (double v : part) s += v;
+    return s;
+}
+
+// Jacobi-preconditioned CG, vectors in T, dots and scalars in FP64.  Stops on
+// the recursive residual ||r|| <= rtol ||b||.  Returns iterations.
+template <class T>
+inline int cg(const Grid& g, const T* b, T* x, double rtol, int maxit, bool jacobi, bool use_x0,
+              double* relres) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    std::vector<T> r(n), z(n), p(n), q(n), dinv(n);
+    for (int64_t i = 0; i < n; ++i) dinv[i] = jacobi ? T(1.0 / diag_of<T>(g, i)) : T(1);
+    if (!use_x0)
+        for (int64_t i = 0; i < n; ++i) x[i] = T(0);
+    apply(g, x, q.data());
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    for (int64_t i = 0; i < n; ++i) z[i] = r[i] * dinv[i];
+    p = z;
+    double bnorm = std::sqrt(dot(b, b, n));
+    if (bnorm == 0) bnorm = 1;
+    double rz = dot(r.data(), z.data(), n);
+    double rr = dot(r.data(), r.data(), n);
+    int it = 0;
+    while (std::sqrt(rr) > rtol * bnorm && it < maxit) {
+        apply(g, p.data(), q.data());
+        double alpha = rz / dot(p.data(), q.data(), n);
+        T a = T(alpha);
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] = x[i] + a * p[i];
+            r[i] = r[i] - a * q[i];
+            z[i] = r[i] * dinv[i];
+        }
+        double rz2 = dot(r.data(), z.data(), n);
+        rr = dot(r.data(), r.data(), n);
+        T beta = T(rz2 / rz);
+        rz = rz2;
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        ++it;
+    }
+    if (relres) *relres = std::sqrt(rr) / bnorm;
+    return it;
+}
+
+// FP64 iterative refinement around the float CG: r = b - A x in double,
+// inner float CG to inner_rtol on A d = r, x += d.
+inline int refine(const Grid& g, const double* b, double* x, double rtol, double inner_rtol,
+                  int maxit, int max_outer, double* relres, int* inner_total) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    std::vector<double> r(n);
+    std::vector<float> rf(n), d(n);
+    double bnorm = std::sqrt(dot(b, b, n));
+    if (bnorm == 0) bnorm = 1;
+    int outer = 0, tot = 0;
+    double rel = 0;
+    for (;;) {
+        apply(g, x, r.data());
+        for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+        rel = std::sqrt(dot(r.data(), r.data(), n)) / bnorm;
+        if (rel <= rtol || outer >= max_outer) break;
+        for (int64_t i = 0; i < n; ++i) rf[i] = (float)r[i];
+        double ir;
+        tot += cg<float>(g, rf.data(), d.data(), inner_rtol, maxit, true, false, &ir);
+        for (int64_t i = 0; i < n; ++i) x[i] += (double)d[i];
+        ++outer;
+    }
+    if (relres) *relres = rel;
+    if (inner_total) *inner_total = tot;
+    return outer;
+}
+
+// Reference update_density (src/paracrine.cu:63-134), restated on the
+// 27-point operator: explicit-Euler guess, b with h where dt is meant
+// (src/paracrine.cu:105), plain CG with the absolute squared-residual stop and
+// at most max_iter iterations, early return without writing the guess
+// (src/paracrine.cu:114-117).  Vectors float, dots double.
+inline int update_density_reference(int nx, int ny, int nz, double h, double D, double k,
+                                    double dt, double eb, int max_iter, float* data,
+                                    double* rr_out) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    Grid L{nx, ny, nz, 27, 0, 0, nullptr, nullptr, nullptr};
+    // L27 u / h^2 via the operator with a0 = 1, w = c/60: y = x - (c/60) S x.
+    // Pick c = -60 / h^2 / 30 ... instead evaluate S x directly:
+    std::vector<float> lap(n), xg(n), bb(n), r(n), p(n), q(n);
+    {
+        const float inv = (float)(1.0 / (h * h * 30.0));
+#pragma omp parallel for schedule(static)
+        for (int kk = 0; kk < nz; ++kk)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    lap[i + (int64_t)nx * (j + (int64_t)ny * kk)] = s27(data, L, i, j, kk) * inv;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        xg[i] = (float)(1.0 - dt * k) * data[i] + (float)(dt * D) * lap[i];
+    for (int64_t i = 0; i < n; ++i)
+        bb[i] = (float)(1.0 - h * k / 2.0) * data[i] + (float)(h * D / 2.0) * lap[i];
+    Grid A{nx, ny, nz, 27, dt * D / (h * h), dt * k, nullptr, nullptr, nullptr};
+    apply(A, xg.data(), q.data());
+    for (int64_t i = 0; i < n; ++i) r[i] = bb[i] - q[i];
+    p = r;
+    double rr = dot(r.data(), r.data(), n);
+    if (rr_out) *rr_out = rr;
+    if (rr < eb) return 0;
+    int it = 0;
+    for (; it < max_iter;) {
+        apply(A, p.data(), q.data());
+        double pq = dot(p.data(), q.data(), n);
+        float alpha = (float)(rr / pq);
+        for (int64_t i = 0; i < n; ++i) {
+            xg[i] = xg[i] + alpha * p[i];
+            r[i] = r[i] - alpha * q[i];
+        }
+        double rr2 = dot(r.data(), r.data(), n);
+        ++it;
+        if (rr2 < eb) {
+            rr = rr2;
+            break;
+        }
+        float beta = (float)(rr2 / rr);
+        for (int64_t i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
+        rr = rr2;
+    }
+    if (rr_out) *rr_out = rr;
+    std::memcpy(data, xg.data(), n * sizeof(float));
+    (void)L;
+    return it;
+}
+
+}  // namespace oracle

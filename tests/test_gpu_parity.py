@@ -1,0 +1,466 @@
+"""Device kernels (through the C ABI) against the CPU oracle.
+
+Bit-exact for the HH step, spike bitmaps and lists, every coupling mode, the
+CSR->CSC layout, the stencils and whole simulations (rasters and state);
+residual- and error-based for the CG solves, whose reduction order differs
+from the oracle's.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.fft
+
+import oracle
+from neuronephys_clean_amd import _lib, configs, host
+from neuronephys_clean_amd.devmem import DeviceArray as DA
+from neuronephys_clean_amd.sim import SOA, ReferenceSimulation, Simulation, upload_graph
+
+pytestmark = pytest.mark.gpu
+
+DT = {np.float32: _lib.F32, np.float64: _lib.F64}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def bits_to_flags(words, n):
+    b = np.unpackbits(words.view(np.uint8), bitorder="little")
+    return b[:n]
+
+
+# ------------------------------------------------------------------ ephys --
+@pytest.mark.parametrize("T", [np.float32, np.float64])
+def test_hh_rk4_step_bitexact(ctx, T):
+    say("hh_rk4_step", T)
+    rng = np.random.default_rng(0)
+    n = 70000 + 13  # ragged
+    V = rng.uniform(-90, 50, n).astype(T)
+    V[:64] = np.linspace(-45, -35, 64)  # around the am singularity (V + 5 = -35)
+    m, h, ng = (rng.uniform(0, 1, n).astype(T) for _ in range(3))
+    s = rng.uniform(0, 2, n).astype(T)
+    iext = rng.normal(10, 20, n).astype(T)
+    gin = rng.uniform(0, 0.5, n).astype(T)
+    p = _lib.params_squid()
+    d = {k: DA.from_numpy(ctx, a) for k, a in dict(v=V, m=m, h=h, ng=ng, s=s, i_ext=iext,
+                                                   gin=gin).items()}
+    bits = DA.zeros(ctx, (n + 63) // 64, np.uint64)
+    st = _lib.HHState(dtype=DT[T], n=n, v=d["v"].ptr, m=d["m"].ptr, h=d["h"].ptr, ng=d["ng"].ptr,
+                      s=d["s"].ptr, i_ext=d["i_ext"].ptr)
+    ctx.call("nnx_hh_step", C.byref(st), C.byref(p), _lib.INT_RK4, _lib.SPIKE_CROSSING, 0.01, 0,
+             d["gin"].ptr, bits.ptr)
+    flags = oracle.hh_rk4(oracle.params(p.as_dict()), 0.01, V, m, h, ng, s, iext, gin)
+    for k, a in dict(v=V, m=m, h=h, ng=ng, s=s).items():
+        assert np.array_equal(d[k].numpy(), a), k
+    assert np.array_equal(bits_to_flags(bits.numpy(), n), flags)
+    assert flags.sum() > 0
+
+
+@pytest.mark.parametrize("T", [np.float32, np.float64])
+@pytest.mark.parametrize("rule", [_lib.SPIKE_LEVEL, _lib.SPIKE_CROSSING])
+def test_hh_leapfrog_step_bitexact(ctx, T, rule):
+    say("hh_leapfrog_step", T, rule)
+    rng = np.random.default_rng(1)
+    n = 5000 + 7
+    sv = [np.zeros(n, T) for _ in range(17)]
+    sv[0][:] = rng.normal(-55, 25, n)
+    for k in (1, 2, 3, 4, 5, 6):
+        sv[k][:] = rng.uniform(0, 1, n)
+    for k in (8, 11, 14):
+        sv[k][:] = rng.integers(0, 20, n)
+    for k in (9, 12, 15):
+        sv[k][:] = rng.integers(0, 30, n)
+    sv[16][:] = rng.normal(0, 3, n)
+    q = _lib.params_reference()
+    q.noise_std = 0.0
+    dev = [DA.from_numpy(ctx, a) for a in sv]
+    st = _lib.HHState(dtype=DT[T], n=n, v=dev[0].ptr, m=dev[1].ptr, ng=dev[2].ptr, h=dev[3].ptr,
+                      y_local=dev[4].ptr, y_i=dev[5].ptr, y_e=dev[6].ptr, out_local=dev[7].ptr,
+                      in_local=dev[8].ptr, count_local=dev[9].ptr, out_e=dev[10].ptr,
+                      in_e=dev[11].ptr, count_e=dev[12].ptr, out_i=dev[13].ptr, in_i=dev[14].ptr,
+                      count_i=dev[15].ptr, appcur=dev[16].ptr)
+    bits = DA.zeros(ctx, (n + 63) // 64, np.uint64)
+    V0 = sv[0].copy()
+    ctx.call("nnx_hh_step", C.byref(st), C.byref(q), _lib.INT_LEAPFROG, rule, 0.4, 0, None,
+             bits.ptr)
+    flags = oracle.hh_leapfrog(oracle.params(q.as_dict()), 0.4, sv)
+    for k in range(17):
+        assert np.array_equal(dev[k].numpy(), sv[k]), SOA[k]
+    if rule == _lib.SPIKE_CROSSING:
+        flags = ((V0 < 0) & (sv[0] >= 0)).astype(np.uint8)
+    assert np.array_equal(bits_to_flags(bits.numpy(), n), flags)
+
+
+def test_hh_step_rejects_noise_and_bad_args(ctx):
+    n = 10
+    a = DA.zeros(ctx, n, np.float32)
+    st = _lib.HHState(dtype=_lib.F32, n=n, v=a.ptr, m=a.ptr, h=a.ptr, ng=a.ptr, s=a.ptr,
+                      i_ext=a.ptr)
+    q = _lib.params_reference()  # noise_std 1.5
+    with pytest.raises(_lib.NNXError) as e:
+        ctx.call("nnx_hh_step", C.byref(st), C.byref(q), _lib.INT_LEAPFROG, 0, 0.4, 0, None, None)
+    assert e.value.status == 5
+    p = _lib.params_squid()
+    with pytest.raises(_lib.NNXError) as e:
+        ctx.call("nnx_hh_step", C.byref(st), C.byref(p), _lib.INT_RK4, _lib.SPIKE_LEVEL, 0.01, 0,
+                 None, None)
+    assert e.value.status == 1
+
+
+def test_check_finite(ctx):
+    x = np.zeros(1000, np.float32)
+    d = DA.from_numpy(ctx, x)
+    ctx.call("nnx_check_finite", _lib.F32, 1000, d.ptr)
+    x[777] = np.nan
+    d.upload(x)
+    with pytest.raises(_lib.NNXError) as e:
+        ctx.call("nnx_check_finite", _lib.F32, 1000, d.ptr)
+    assert e.value.status == 3
+
+
+# ------------------------------------------------------- spikes, coupling --
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4097, 300001])
+@pytest.mark.parametrize("frac", [0.0, 0.01, 1.0])
+def test_threshold_and_compact(ctx, n, frac):
+    rng = np.random.default_rng(n)
+    out = np.where(rng.random(n) < frac, 0.5, 0.05).astype(np.float32)
+    d = DA.from_numpy(ctx, out)
+    nw = (n + 63) // 64
+    bits, lst, cnt = DA.zeros(ctx, nw, np.uint64), DA.zeros(ctx, n, np.int32), DA.zeros(ctx, 1, np.int32)
+    ctx.call("nnx_spike_threshold", _lib.F32, n, d.ptr, 0.1, bits.ptr)
+    ctx.call("nnx_spike_compact", n, bits.ptr, lst.ptr, cnt.ptr)
+    ref = np.nonzero(out > 0.1)[0]
+    assert np.array_equal(bits_to_flags(bits.numpy(), n), (out > 0.1).astype(np.uint8))
+    k = C.c_int32()
+    ctx.call("nnx_read_count", cnt.ptr, C.byref(k))
+    assert k.value == ref.size and np.array_equal(lst.numpy()[:k.value], ref)
+
+
+def _graph(n, k, seed):
+    off, tgt = host.er_fixed(seed, n, k)
+    w = host.lognormal(seed, 9, len(tgt), -3, 1).astype(np.float32)
+    return configs.Network(n, off, tgt, w, *(np.zeros(n),) * 5, np.zeros((n, 3), np.float32))
+
+
+@pytest.mark.parametrize("T", [np.float32, np.float64])
+@pytest.mark.parametrize("frac", [0.003, 0.05, 0.6])
+def test_couple_all_modes_bitexact(ctx, T, frac):
+    say("couple", T, frac)
+    net = _graph(3001, 60, 3)
+    g, keep = upload_graph(ctx, net)
+    n = net.n
+    rng = np.random.default_rng(4)
+    fire = (rng.random(n) < frac).astype(np.uint8)
+    out = rng.random(n).astype(T)
+    words = np.packbits(np.concatenate([fire, np.zeros(-n % 64, np.uint8)]),
+                        bitorder="little").view(np.uint64)
+    bits = DA.from_numpy(ctx, words)
+    lst = DA.from_numpy(ctx, np.nonzero(fire)[0].astype(np.int32))
+    cnt = DA.from_numpy(ctx, np.array([fire.sum()], np.int32))
+    dout = DA.from_numpy(ctx, out)
+    res = {}
+    for name, mode in (("pull", 0), ("event", 1), ("count", 2), ("signal", 3)):
+        gi = DA.zeros(ctx, n, T)
+        ctx.call("nnx_couple", C.byref(g), mode, DT[T], bits.ptr, lst.ptr, cnt.ptr, dout.ptr, gi.ptr)
+        res[name] = gi.numpy()
+        ref = oracle.couple(net.off, net.tgt, net.w, fire, mode, out=out, dtype=T)
+        assert np.array_equal(res[name], ref), name
+    assert np.array_equal(res["pull"], res["event"])
+    # the event path leaves its counters cleared: a second call is identical
+    gi = DA.zeros(ctx, n, T)
+    ctx.call("nnx_couple", C.byref(g), 1, DT[T], bits.ptr, lst.ptr, cnt.ptr, dout.ptr, gi.ptr)
+    assert np.array_equal(gi.numpy(), res["event"])
+
+
+def test_graph_transpose_device_equals_host(ctx):
+    rng = np.random.default_rng(5)
+    n = 20000
+    deg = rng.integers(0, 90, n).astype(np.int32)
+    deg[7] = 3000  # a hub
+    off = host.offsets_of(deg)
+    tgt = rng.integers(0, n, off[-1]).astype(np.int32)
+    tgt[off[7]:off[7] + 50] = 11  # duplicate edges
+    w = rng.random(off[-1]).astype(np.float32)
+    net = configs.Network(n, off, tgt, w, *(np.zeros(n),) * 5, np.zeros((n, 3), np.float32))
+    _, dev = upload_graph(ctx, net, device_csc=True)
+    h = host.csr_to_csc(n, off, tgt, w)
+    for k, a in zip(("in_off", "in_src", "in_w", "o2i", "deg"), h):
+        assert np.array_equal(dev[k].numpy()[:len(a)], a), k
+
+
+# -------------------------------------------------------------- stencils --
+def _grid_dev(ctx, nx, ny, nz, stencil, c, decay=0.0, var=False, seed=0):
+    keep = []
+    kx = ky = kz = None
+    if var:
+        rng = np.random.default_rng(seed)
+        kx = rng.uniform(0.1, 5, (nz, ny, nx + 1)).astype(np.float32)
+        ky = rng.uniform(0.1, 5, (nz, ny + 1, nx)).astype(np.float32)
+        kz = rng.uniform(0.1, 5, (nz + 1, ny, nx)).astype(np.float32)
+        keep = [DA.from_numpy(ctx, a) for a in (kx, ky, kz)]
+    g = _lib.Grid(nx, ny, nz, stencil, c, decay, *([k.ptr for k in keep] if var else [None] * 3))
+    og = oracle.grid(nx, ny, nz, stencil, c, decay, kx, ky, kz)
+    return g, og, keep
+
+
+@pytest.mark.parametrize("kind", ["7", "7var", "27"])
+def test_stencil_apply_bitexact(ctx, kind):
+    nx, ny, nz = 67, 23, 11
+    g, og, keep = _grid_dev(ctx, nx, ny, nz, 27 if kind == "27" else 7, 1.3, 0.2, kind == "7var")
+    x = np.random.default_rng(6).standard_normal(nx * ny * nz).astype(np.float32)
+    dx, dy = DA.from_numpy(ctx, x), DA.zeros(ctx, x.size, np.float32)
+    ctx.call("nnx_stencil_apply", C.byref(g), dx.ptr, dy.ptr)
+    assert np.array_equal(dy.numpy(), oracle.apply(og, x))
+
+
+def _lam7(nx, ny, nz, c):
+    th = [2 * np.cos(np.pi * np.arange(1, m + 1) / (m + 1)) for m in (nz, ny, nx)]
+    tz, ty, tx = np.meshgrid(*th, indexing="ij")
+    return 1 + c * ((2 - tx) + (2 - ty) + (2 - tz)), (tx, ty, tz)
+
+
+def _dst_solve(b, lam):
+    return scipy.fft.dstn(scipy.fft.dstn(b, type=1, norm="ortho") / lam, type=1, norm="ortho")
+
+
+def _cg(ctx, g, b, rtol, precond=_lib.PRECOND_JACOBI, maxit=5000, x0=None, check=8):
+    db = DA.from_numpy(ctx, b)
+    dx = DA.from_numpy(ctx, x0) if x0 is not None else DA.zeros(ctx, b.size, np.float32)
+    o = _lib.CgOpts(rtol=rtol, maxit=maxit, precond=precond, check_every=check,
+                    use_x0=int(x0 is not None))
+    r = _lib.CgResult()
+    st = ctx._l.nnx_cg_solve(ctx.h, C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(r))
+    return dx.numpy(), r, st
+
+
+@pytest.mark.parametrize("precond", [_lib.PRECOND_NONE, _lib.PRECOND_JACOBI])
+def test_cg_7pt_exact_solution(ctx, precond):
+    nx, ny, nz, c = 64, 48, 40, 1.0
+    g, og, _ = _grid_dev(ctx, nx, ny, nz, 7, c)
+    b = np.random.default_rng(7).random(nx * ny * nz).astype(np.float32)
+    x, r, st = _cg(ctx, g, b, 1e-6, precond)
+    say("cg 7pt", r.as_dict())
+    assert st == 0 and r.converged and r.relres <= 1e-6 and r.true_relres < 2e-6
+    lam, _ = _lam7(nx, ny, nz, c)
+    xs = _dst_solve(b.astype(np.float64).reshape(nz, ny, nx), lam).ravel()
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 2e-6
+    # iteration count close to the oracle's (different reduction order)
+    _, it, _ = oracle.cg(og, b, 1e-6, jacobi=precond == _lib.PRECOND_JACOBI)
+    assert abs(r.iters - it) <= 2
+
+
+def test_cg_variable_coefficient_and_27(ctx):
+    import scipy.sparse.linalg as sla
+    for kind in ("7var", "27"):
+        nx, ny, nz = 20, 18, 16
+        g, og, keep = _grid_dev(ctx, nx, ny, nz, 27 if kind == "27" else 7, 4.0, 0.1,
+                                kind == "7var", seed=3)
+        b = np.random.default_rng(8).random(nx * ny * nz).astype(np.float32)
+        x, r, st = _cg(ctx, g, b, 1e-6)
+        assert st == 0 and r.converged, (kind, r.as_dict())
+        # FP64 residual with the oracle's operator
+        res = b - oracle.apply(og, x.astype(np.float64))
+        assert np.linalg.norm(res) / np.linalg.norm(b) < 3e-6, kind
+        # error against a direct solve of the assembled operator
+        N = nx * ny * nz
+        A = np.column_stack([oracle.apply(og, e) for e in np.eye(N)])
+        xs = np.linalg.solve(A, b.astype(np.float64))
+        assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-5, kind
+        del keep
+
+
+def test_cg_nonconvergence_reports(ctx):
+    nx = 32
+    g, _, _ = _grid_dev(ctx, nx, nx, nx, 7, 50.0)
+    b = np.random.default_rng(9).random(nx ** 3).astype(np.float32)
+    x, r, st = _cg(ctx, g, b, 1e-12, maxit=3)
+    assert st == 4 and r.iters == 3 and not r.converged
+
+
+def test_cg_warm_start_and_zero_rhs(ctx):
+    nx, c = 40, 1.0
+    g, _, _ = _grid_dev(ctx, nx, nx, nx, 7, c)
+    b = np.random.default_rng(10).random(nx ** 3).astype(np.float32)
+    x, r, _ = _cg(ctx, g, b, 1e-6)
+    x2, r2, _ = _cg(ctx, g, b, 1e-6, x0=x)
+    assert r2.iters <= 1
+    z, rz, stz = _cg(ctx, g, np.zeros(nx ** 3, np.float32), 1e-8)
+    assert stz == 0 and rz.iters == 0 and not z.any()
+
+
+def test_refinement_to_1e10(ctx):
+    nx, c = 48, 20.0
+    rng = np.random.default_rng(11)
+    b = np.zeros(nx ** 3)
+    b[rng.choice(nx ** 3, 200, replace=False)] = 1.0
+    g, _, _ = _grid_dev(ctx, nx, nx, nx, 7, c)
+    db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, nx ** 3, np.float64)
+    o = _lib.CgOpts(rtol=1e-10, maxit=5000, precond=_lib.PRECOND_JACOBI, check_every=8)
+    r = _lib.CgResult()
+    ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-5, 30, C.byref(r))
+    say("refine", r.as_dict())
+    assert r.converged and r.relres <= 1e-10 and r.outer >= 2
+    lam, _ = _lam7(nx, nx, nx, c)
+    xs = _dst_solve(b.reshape(nx, nx, nx), lam).ravel()
+    assert np.linalg.norm(dx.numpy() - xs) / np.linalg.norm(xs) < 1e-9
+
+
+def test_update_density_reference_matches_oracle(ctx):
+    nx, h, D, k, dt = 24, 1.0, 0.5, 0.1, 0.4
+    data = np.random.default_rng(12).random((nx, nx, nx)).astype(np.float32)
+    d = DA.from_numpy(ctx, data)
+    r = _lib.CgResult()
+    ctx.call("nnx_update_density_reference", nx, nx, nx, h, D, k, dt, 1e-6, 20, d.ptr, C.byref(r))
+    ref, it, rr = oracle.update_density_reference(data, h, D, k, dt, 1e-6, 20)
+    got = d.numpy().reshape(nx, nx, nx)
+    assert abs(r.iters - it) <= 1
+    np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-5)
+    # early return: A = I, b = u -> data untouched
+    d2 = DA.from_numpy(ctx, data)
+    ctx.call("nnx_update_density_reference", nx, nx, nx, 1.0, 0.0, 0.0, dt, 1e-6, 20, d2.ptr, C.byref(r))
+    assert r.iters == 0 and np.array_equal(d2.numpy(), data)
+
+
+def test_deposit_rhs_and_spmv(ctx):
+    net = configs.c1()
+    vox = net.ensure_voxel()
+    rng = np.random.default_rng(13)
+    lst = np.sort(rng.choice(net.n, 300, replace=False)).astype(np.int32)
+    dv, dl = DA.from_numpy(ctx, vox), DA.from_numpy(ctx, lst)
+    dc = DA.from_numpy(ctx, np.array([lst.size], np.int32))
+    counts = DA.zeros(ctx, 32 ** 3, np.int32)
+    ctx.call("nnx_deposit_count", dv.ptr, dl.ptr, dc.ptr, counts.ptr)
+    ref = np.bincount(vox[lst], minlength=32 ** 3).astype(np.int32)
+    assert np.array_equal(counts.numpy(), ref)
+    cprev = rng.random(32 ** 3).astype(np.float32)
+    dcp, db = DA.from_numpy(ctx, cprev), DA.zeros(ctx, 32 ** 3, np.float32)
+    ctx.call("nnx_rhs_from_counts", 32 ** 3, dcp.ptr, counts.ptr, 0.01, db.ptr)
+    assert np.array_equal(db.numpy(), cprev + np.float32(0.01) * ref.astype(np.float32))
+    assert not counts.numpy().any()
+    # trilinear interpolation / scatter as CSR SpMV
+    nx = 17
+    xyz = (rng.random((3, 500)) - 0.5) * (nx - 1) * 0.9
+    off, cols, vals, _ = host.trilinear_matrix(*xyz, nx, nx, nx, 1.0)
+    grid = rng.random(nx ** 3).astype(np.float32)
+    y = DA.zeros(ctx, 500, np.float32)
+    dd = [DA.from_numpy(ctx, a) for a in (off, cols, vals, grid)]
+    ctx.call("nnx_csr_spmv", 500, dd[0].ptr, dd[1].ptr, dd[2].ptr, dd[3].ptr, y.ptr, 0)
+    ref = np.zeros(500, np.float32)
+    for i in range(500):
+        acc = np.float32(0)
+        for e in range(off[i], off[i + 1]):
+            acc = np.float32(acc + np.float32(vals[e] * grid[cols[e]]))
+        ref[i] = acc
+    assert np.array_equal(y.numpy(), ref)
+
+
+# ------------------------------------------------------------ simulations --
+def _oracle_rk4(net, T, steps, cap, deposit=True):
+    V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
+    s = np.zeros(net.n, T)
+    p = oracle.params(_lib.params_squid().as_dict())
+    ras, ln, vc, sec = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
+                                      np.asarray(net.iext, T), cap=cap,
+                                      voxel_of=net.ensure_voxel() if deposit else None,
+                                      nvox=int(np.prod(net.grid)))
+    return ras, ln, vc, dict(v=V, m=m, h=h, ng=ng, s=s)
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_sim_c1_100ms_rasters_bitexact(ctx, T):
+    """C1 (1,024 neurons, 100 out-edges) for 100 ms = 10,000 RK4 steps:
+    rasters, final state and release counts identical to the oracle."""
+    net = configs.c1()
+    steps = 10000
+    say("sim c1", T, steps)
+    sim = Simulation(net, T, couple="event", record_cap=2_000_000, deposit=True)
+    sim.run(steps, use_graph=True, chunk=500)
+    sim.synchronize()
+    ras, ln = sim.raster()
+    ref, rln, vc, st = _oracle_rk4(net, T, steps, 2_000_000)
+    say("spikes", ln, rln)
+    assert ln == rln and ln > 1000
+    assert np.array_equal(ras, ref)
+    got = sim.state()
+    for k in st:
+        assert np.array_equal(got[k], st[k]), k
+    assert np.array_equal(sim.a["voxel_counts"].numpy(), vc)
+
+
+def test_sim_fp32_vs_fp64_report(ctx):
+    """fp32 against the FP64 run of the same inputs: reported, bounded."""
+    net = configs.c1()
+    out = {}
+    for T in (np.float32, np.float64):
+        sim = Simulation(net, T, couple="event", record_cap=500_000)
+        vs = []
+        for k in (1, 99, 900):
+            sim.run(k)
+            vs.append(sim.state()["v"].astype(np.float64))
+        out[T] = (vs, sim.raster()[0])
+    dv = [np.abs(a - b).max() for a, b in zip(out[np.float32][0], out[np.float64][0])]
+    say("max|dV| fp32 vs fp64 after 1, 100, 1000 steps:", dv)
+    assert dv[0] < 1e-4
+    r32, r64 = (set(map(tuple, out[T][1])) for T in (np.float32, np.float64))
+    agree = len(r32 & r64) / max(len(r64), 1)
+    say("raster agreement (1000 steps):", agree)
+    assert agree > 0.5
+
+
+@pytest.mark.parametrize("couple", ["pull", "event"])
+def test_graph_replay_equals_eager(ctx, couple):
+    net = configs.c1(seed=3, n=777, k=40)
+    a = Simulation(net, np.float32, couple=couple, record_cap=100000)
+    b = Simulation(net, np.float32, couple=couple, record_cap=100000)
+    a.run(600)
+    b.run(600, use_graph=True, chunk=100)
+    a.synchronize()
+    b.synchronize()
+    assert np.array_equal(a.raster()[0], b.raster()[0])
+    for k, v in a.state().items():
+        assert np.array_equal(v, b.state()[k])
+
+
+@pytest.mark.parametrize("mode", ["count", "signal"])
+@pytest.mark.parametrize("T", [np.float32, np.float64])
+def test_reference_leapfrog_sim_bitexact(ctx, mode, T):
+    """The reference's own step (src/main.cu:166-181 with Leapfrog_HH and
+    AtomicCoupling) on a 1,024-neuron UniformRandomNetwork-shaped graph."""
+    net = configs.c1(seed=5)
+    in_deg = net.ensure_csc()[4]
+    rng = np.random.default_rng(14)
+    sv0 = {k: np.zeros(net.n) for k in SOA}
+    sv0["V"] = rng.normal(-55, 25, net.n)
+    sv0["LOCAL_COUNT"] = in_deg.astype(np.float64)
+    sv0["APPCUR"] = rng.normal(0, 2, net.n)
+    steps = 400
+    sim = ReferenceSimulation(net, sv0, T, couple=mode, record_cap=1_000_000)
+    sim.run(steps, use_graph=True)
+    sim.ctx.synchronize()
+    ras, ln = sim.raster()
+    sv = [np.asarray(sv0[k], T).copy() for k in SOA]
+    q = _lib.params_reference()
+    q.noise_std = 0.0
+    ref, rln = oracle.run_leapfrog(oracle.params(q.as_dict()), 0.4, steps, net.off, net.tgt,
+                                   2 if mode == "count" else 3, sv, cap=1_000_000)
+    say("reference sim", mode, T, ln, rln)
+    assert ln == rln and ln > 0 and np.array_equal(ras, ref)
+    got = sim.state()
+    for k, a in zip(SOA, sv):
+        assert np.array_equal(got[k], a), k
+
+
+def test_sim_c2_scaled_bitexact(ctx):
+    """C2's generator at 1/16 scale (65,536 neurons, ~13M edges,
+    distance-dependent targets): 300 steps identical to the oracle."""
+    net = configs.c2(n=1 << 16)
+    say("c2 scaled", net.n, net.nnz)
+    sim = Simulation(net, np.float32, couple="event", record_cap=2_000_000, deposit=True)
+    sim.run(300, use_graph=True, chunk=100)
+    sim.synchronize()
+    ras, ln = sim.raster()
+    ref, rln, vc, st = _oracle_rk4(net, np.float32, 300, 2_000_000)
+    assert ln == rln and np.array_equal(ras, ref)
+    assert np.array_equal(sim.state()["v"], st["v"])
+    assert np.array_equal(sim.a["voxel_counts"].numpy(), vc)

@@ -1,0 +1,405 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each check fixes the oracle against something independent of its own
+formulas: libm, textbook values, closed forms, scipy's DOP853 integrator, a
+dense matrix product, an exact DST-I diagonalisation, or brute force.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.fft
+import scipy.integrate
+import scipy.sparse
+import scipy.sparse.linalg
+
+import oracle
+from neuronephys_clean_amd import _lib
+
+
+def squid():
+    return _lib.params_squid().as_dict()
+
+
+def reference():
+    d = _lib.params_reference().as_dict()
+    d["noise_std"] = 0.0
+    return d
+
+
+# ---------------------------------------------------------------- exp ------
+@pytest.mark.parametrize("dt,lo,hi", [(np.float32, -80, 88), (np.float64, -700, 708)])
+def test_exp_one_ulp(dt, lo, hi):
+    x = np.linspace(lo, hi, 400001).astype(dt)
+    x = np.concatenate([x, np.array([0, -0.0, 1e-30, -1e-30, 0.3465, -0.3466], dt)])
+    y = oracle.exp(x)
+    ref = np.exp(x.astype(np.float64)) if dt == np.float32 else np.exp(x)
+    ref = ref.astype(dt)
+    it = np.int32 if dt == np.float32 else np.int64
+    ulps = np.abs(y.view(it).astype(np.int64) - ref.view(it).astype(np.int64))
+    assert ulps.max() <= 1
+
+
+def test_exp_clamps_and_nan():
+    y = oracle.exp(np.array([1000.0, -1000.0, np.nan], np.float32))
+    assert np.isfinite(y[0]) and y[1] > 0 and np.isnan(y[2])
+
+
+# -------------------------------------------------------------- rates ------
+def test_rates_textbook_at_rest():
+    # Hodgkin & Huxley (1952) rate constants at rest (reference convention,
+    # V = -60 mV): am 0.2236, bm 4, ah 0.07, bh 0.0474, an 0.0582, bn 0.125
+    r = oracle.rates(np.array([-60.0]))[0]
+    am, bm, ah, bh, an, bn = r
+    assert am == pytest.approx(2.5 / (math.e ** 2.5 - 1), rel=1e-14)
+    assert bm == pytest.approx(4.0, rel=1e-14)
+    assert ah == pytest.approx(0.07, rel=1e-14)
+    assert bh == pytest.approx(1 / (math.e ** 3 + 1), rel=1e-14)
+    assert an == pytest.approx(0.1 / (math.e - 1), rel=1e-14)
+    assert bn == pytest.approx(0.125, rel=1e-14)
+
+
+def test_rates_removable_singularities():
+    # the reference divides 0 by 0 at V = -35 (am) and V = -50 (an)
+    r = oracle.rates(np.array([-35.0, -50.0]))
+    assert r[0, 0] == pytest.approx(1.0, rel=1e-15)
+    assert r[1, 4] == pytest.approx(0.1, rel=1e-15)
+    # continuity across the series / direct switch (|a| = 0.5)
+    for v0 in (-35.0, -50.0):
+        u = v0 + np.array([-5.0001, -4.9999, 4.9999, 5.0001])
+        rr = oracle.rates(u)
+        col = 0 if v0 == -35.0 else 4
+        assert abs(rr[0, col] - rr[1, col]) < 1e-4 and abs(rr[2, col] - rr[3, col]) < 1e-4
+
+
+def test_rates_float_close_to_double():
+    u = np.linspace(-100, 60, 20001)
+    r64 = oracle.rates(u)
+    r32 = oracle.rates(u.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(r32 / r64 - 1)) < 2e-6
+
+
+# ----------------------------------------------------------- leapfrog ------
+def _soa(n, V, gates, T=np.float64):
+    sv = [np.zeros(n, T) for _ in range(17)]
+    sv[0][:] = V
+    sv[1][:], sv[2][:], sv[3][:] = gates
+    return sv
+
+
+def test_leapfrog_gate_fixed_point_and_amplification():
+    p = oracle.params(reference())
+    dt = 0.4
+    V = np.linspace(-80, 20, 11)
+    r = oracle.rates(V)
+    am, bm, ah, bh, an, bn = r.T
+    minf, ninf, hinf = am / (am + bm), an / (an + bn), ah / (ah + bh)
+    sv = _soa(V.size, V, (minf, ninf, hinf))
+    oracle.hh_leapfrog(p, dt, sv)
+    np.testing.assert_allclose(sv[1], minf, rtol=1e-13)
+    np.testing.assert_allclose(sv[2], ninf, rtol=1e-13)
+    np.testing.assert_allclose(sv[3], hinf, rtol=1e-13)
+    # perturbation: trapezoid amplification (1 - dt l/2) / (1 + dt l/2)
+    d = 0.01
+    sv = _soa(V.size, V, (minf + d, ninf + d, hinf + d))
+    oracle.hh_leapfrog(p, dt, sv)
+    for k, (a, b, xinf) in zip((1, 2, 3), ((am, bm, minf), (an, bn, ninf), (ah, bh, hinf))):
+        lam = a + b
+        np.testing.assert_allclose(sv[k] - xinf, d * (1 - dt * lam / 2) / (1 + dt * lam / 2),
+                                   rtol=1e-9, atol=1e-15)
+
+
+def test_leapfrog_voltage_fixed_point_and_output():
+    q = reference()
+    p = oracle.params(q)
+    dt = 0.4
+    V0 = np.array([-65.0, -58.0, -40.0])
+    am, bm, ah, bh, an, bn = oracle.rates(V0).T
+    m, ng, h = am / (am + bm), an / (an + bn), ah / (ah + bh)
+    G = q["gna"] * m ** 3 * h + q["gk"] * ng ** 4 + q["gl"]
+    E = q["gna"] * m ** 3 * h * q["ena"] + q["gk"] * ng ** 4 * q["ek"] + q["gl"] * q["el"]
+    # choose the drive that makes V0 the fixed point of dV/dt = I + E - G V
+    sv = _soa(3, V0, (m, ng, h))
+    sv[16][:] = G * V0 - E - q["noise_mean"]
+    oracle.hh_leapfrog(p, dt, sv)
+    np.testing.assert_allclose(sv[0], V0, rtol=1e-12)
+    out = q["tmax"] / (1 + np.exp(-(V0 - q["vt"]) / q["kp"]))
+    np.testing.assert_allclose(sv[7], out, rtol=1e-12)
+    assert np.array_equal(sv[7], sv[10]) and np.array_equal(sv[7], sv[13])
+
+
+def test_leapfrog_kinetic_synapse_fixed_point():
+    q = reference()
+    p = oracle.params(q)
+    V0 = np.full(4, -60.0)
+    am, bm, ah, bh, an, bn = oracle.rates(V0).T
+    sv = _soa(4, V0, (am / (am + bm), an / (an + bn), ah / (ah + bh)))
+    inn, cnt = np.array([0.0, 1.0, 3.0, 5.0]), np.array([0.0, 4.0, 4.0, 5.0])
+    T = np.where(cnt > 0, inn / np.maximum(cnt, 1), 0)
+    yinf = q["ar"] * T / (q["ar"] * T + q["ad"])
+    sv[4][:] = yinf
+    sv[8][:], sv[9][:] = inn, cnt
+    oracle.hh_leapfrog(p, 0.4, sv)
+    np.testing.assert_allclose(sv[4], yinf, rtol=1e-13, atol=1e-300)
+
+
+# ---------------------------------------------------------------- RK4 ------
+def _hh_rhs(q, iext, tau):
+    def f(t, y):
+        V, m, h, n, s = y
+        u = V + q["v_shift"]
+        a = 3.5 + 0.1 * u
+        am = a / (1 - math.exp(-a))
+        b = 5 + 0.1 * u
+        an = 0.1 * b / (1 - math.exp(-b))
+        ah = 0.07 * math.exp(-u / 20 - 3)
+        bm = 4 * math.exp(-(u + 60) / 18)
+        bn = 0.125 * math.exp(-(u + 60) / 80)
+        bh = 1 / (math.exp(-3 - 0.1 * u) + 1)
+        I = (iext - q["gna"] * m ** 3 * h * (V - q["ena"]) - q["gk"] * n ** 4 * (V - q["ek"])
+             - q["gl"] * (V - q["el"]) - s * (V - q["e_syn"]))
+        return [I / q["C"], am * (1 - m) - bm * m, ah * (1 - h) - bh * h,
+                an * (1 - n) - bn * n, -s / tau]
+    return f
+
+
+def _rk4_run(q, dt, steps, V0, iext, s0):
+    p = oracle.params(q)
+    V = np.array([V0])
+    m, h, ng = (np.array([x]) for x in _steady(V0, q["v_shift"]))
+    s = np.array([s0])
+    traj = []
+    for _ in range(steps):
+        oracle.hh_rk4(p, dt, V, m, h, ng, s, np.array([iext]))
+        traj.append(V[0])
+    return np.array(traj), (V[0], m[0], h[0], ng[0], s[0])
+
+
+def _steady(V, shift):
+    am, bm, ah, bh, an, bn = oracle.rates(np.array([V + shift]))[0]
+    return am / (am + bm), ah / (ah + bh), an / (an + bn)
+
+
+@pytest.mark.parametrize("dt,tol", [(0.01, 3e-4), (0.005, 1.5e-5)])
+def test_rk4_matches_dop853(dt, tol):
+    # 20 ms of repetitive firing (I = 10 uA/cm^2) with a decaying synapse:
+    # the whole trajectory against DOP853 at rtol = atol = 1e-12
+    q = squid()
+    V0, iext, s0, T = -65.0, 10.0, 0.3, 20.0
+    steps = int(round(T / dt))
+    traj, _ = _rk4_run(q, dt, steps, V0, iext, s0)
+    y0 = [V0, *_steady(V0, q["v_shift"]), s0]
+    t = dt * np.arange(1, steps + 1)
+    sol = scipy.integrate.solve_ivp(_hh_rhs(q, iext, q["tau_syn"]), (0, T), y0, method="DOP853",
+                                    rtol=1e-12, atol=1e-12, t_eval=t)
+    err = np.max(np.abs(traj - sol.y[0]))
+    assert err < tol, err
+    assert traj.max() > 20  # it spikes
+
+
+def test_rk4_fourth_order():
+    q = squid()
+    V0, iext, s0, T = -65.0, 10.0, 0.2, 5.0
+    y0 = [V0, *_steady(V0, q["v_shift"]), s0]
+    sol = scipy.integrate.solve_ivp(_hh_rhs(q, iext, q["tau_syn"]), (0, T), y0, method="DOP853",
+                                    rtol=1e-13, atol=1e-13)
+    errs = []
+    for dt in (0.04, 0.02):
+        _, fin = _rk4_run(q, dt, int(round(T / dt)), V0, iext, s0)
+        errs.append(abs(fin[0] - sol.y[0][-1]))
+    ratio = errs[0] / errs[1]
+    assert 11 < ratio < 22, (errs, ratio)
+
+
+def test_rk4_rest_without_drive():
+    q = squid()
+    _, fin = _rk4_run(q, 0.01, 5000, -65.0, 0.0, 0.0)
+    assert abs(fin[0] - (-65.0)) < 0.5
+
+
+def test_rk4_crossing_flags():
+    q = squid()
+    p = oracle.params(q)
+    # a huge drive pushes the first two across 0 mV within the step; the
+    # third starts above 0 (no upward crossing), the fourth rests
+    V = np.array([-1.0, -1e-3, 5.0, -70.0])
+    m, h, ng = (np.array(x) for x in zip(*[_steady(v, 5.0) for v in V]))
+    spk = oracle.hh_rk4(p, 0.01, V, m, h, ng, np.zeros(4), np.array([1e5, 1e5, 1e5, 0.0]))
+    assert spk.tolist() == [1, 1, 0, 0] and V[2] > 0
+
+
+# ----------------------------------------------------------- coupling ------
+def _rand_graph(rng, n, k):
+    off = np.arange(n + 1, dtype=np.int32) * k
+    tgt = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for _ in range(n)]).astype(np.int32)
+    w = rng.random(n * k).astype(np.float32)
+    return off, tgt, w
+
+
+def test_couple_weighted_equals_dense_product():
+    rng = np.random.default_rng(1)
+    n, k = 300, 20
+    off, tgt, w = _rand_graph(rng, n, k)
+    fire = (rng.random(n) < 0.2).astype(np.uint8)
+    W = np.zeros((n, n))
+    for j in range(n):
+        W[j, tgt[off[j]:off[j + 1]]] = w[off[j]:off[j + 1]]
+    ref = W.T @ fire.astype(np.float64)
+    g = oracle.couple(off, tgt, w, fire, 0, dtype=np.float64)
+    np.testing.assert_allclose(g, ref, rtol=1e-13, atol=1e-15)
+    cnt = oracle.couple(off, tgt, w, fire, 2, dtype=np.float32)
+    np.testing.assert_array_equal(cnt, (W != 0).T.astype(np.float64) @ fire)
+    out = rng.random(n)
+    sig = oracle.couple(off, tgt, w, fire, 3, out=out, dtype=np.float64)
+    np.testing.assert_allclose(sig, (W != 0).T.astype(np.float64) @ (fire * out), rtol=1e-13)
+
+
+def test_couple_f32_sums_in_ascending_source_order():
+    rng = np.random.default_rng(2)
+    n, k = 200, 30
+    off, tgt, w = _rand_graph(rng, n, k)
+    w = (rng.standard_normal(n * k) * 10).astype(np.float32)  # signs make order visible
+    fire = (rng.random(n) < 0.5).astype(np.uint8)
+    g = oracle.couple(off, tgt, w, fire, 0, dtype=np.float32)
+    ref = np.zeros(n, np.float32)
+    for t in range(n):
+        acc = np.float32(0)
+        for j in range(n):  # ascending source
+            if fire[j]:
+                seg = tgt[off[j]:off[j + 1]]
+                hit = np.nonzero(seg == t)[0]
+                if hit.size:
+                    acc = np.float32(acc + w[off[j] + hit[0]])
+        ref[t] = acc
+    assert np.array_equal(g, ref)
+
+
+# ----------------------------------------------------- grid operators ------
+def _lam7(nx, ny, nz, c):
+    th = [2 * np.cos(np.pi * np.arange(1, m + 1) / (m + 1)) for m in (nz, ny, nx)]
+    tz, ty, tx = np.meshgrid(*th, indexing="ij")
+    return 1 + c * ((2 - tx) + (2 - ty) + (2 - tz)), (tx, ty, tz)
+
+
+def _lam27(nx, ny, nz, c, decay_dt):
+    _, (tx, ty, tz) = _lam7(nx, ny, nz, 0)
+    S = -128 + 14 * (tx + ty + tz) + 3 * (tx * ty + ty * tz + tx * tz) + tx * ty * tz
+    return (1 + 0.5 * decay_dt) - (c / 60.0) * S
+
+
+def _dst_solve(b, lam):
+    f = scipy.fft.dstn(b, type=1, norm="ortho")
+    return scipy.fft.dstn(f / lam, type=1, norm="ortho")
+
+
+def test_operator7_quadratic_and_symmetry():
+    nx, ny, nz, c = 9, 7, 8, 0.7
+    g = oracle.grid(nx, ny, nz, 7, c)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x = (i ** 2 + 2 * j ** 2 + 3 * k ** 2).astype(np.float64)
+    y = oracle.apply(g, x.ravel()).reshape(nz, ny, nx)
+    # interior: (I - c L) x = x - c * (2 + 4 + 6)
+    np.testing.assert_allclose(y[1:-1, 1:-1, 1:-1], (x - 12 * c)[1:-1, 1:-1, 1:-1], rtol=1e-14)
+    rng = np.random.default_rng(0)
+    a, b = rng.random(nx * ny * nz), rng.random(nx * ny * nz)
+    assert abs(a @ oracle.apply(g, b) - b @ oracle.apply(g, a)) < 1e-11
+
+
+def test_operator27_delta_sum_and_dst():
+    # the reference's own disabled check: sum(A * delta) == sum(delta) * sum(A)
+    nx = 7
+    g = oracle.grid(nx, nx, nx, 27, c=0.8, decay_dt=0.3)
+    d = np.zeros(nx ** 3)
+    d[(4 * nx + 5) * nx + 5] = 1.0
+    y = oracle.apply(g, d)
+    stencil_sum = (1 + 0.15) - (0.8 / 60) * 0.0  # S sums to 0
+    assert y.sum() == pytest.approx(stencil_sum, rel=1e-14)
+    assert y[(4 * nx + 5) * nx + 5] == pytest.approx(1.15 + 0.8 / 60 * 128, rel=1e-14)
+    # spectrum: A applied to a DST mode
+    lam = _lam27(nx, nx, nx, 0.8, 0.3)
+    rng = np.random.default_rng(3)
+    b = rng.random((nx, nx, nx))
+    x = _dst_solve(b, lam)
+    np.testing.assert_allclose(oracle.apply(g, x.ravel()), b.ravel(), rtol=1e-11, atol=1e-12)
+
+
+def test_operator7_variable_matches_sparse_matrix():
+    rng = np.random.default_rng(4)
+    nx, ny, nz = 6, 5, 4
+    kx = rng.random((nz, ny, nx + 1)).astype(np.float32)
+    ky = rng.random((nz, ny + 1, nx)).astype(np.float32)
+    kz = rng.random((nz + 1, ny, nx)).astype(np.float32)
+    g = oracle.grid(nx, ny, nz, 7, 0.0, 0.0, kx, ky, kz)
+    N = nx * ny * nz
+    A = np.zeros((N, N))
+    idx = lambda i, j, k: i + nx * (j + ny * k)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                r = idx(i, j, k)
+                faces = [(kx[k, j, i], (i - 1, j, k)), (kx[k, j, i + 1], (i + 1, j, k)),
+                         (ky[k, j, i], (i, j - 1, k)), (ky[k, j + 1, i], (i, j + 1, k)),
+                         (kz[k, j, i], (i, j, k - 1)), (kz[k + 1, j, i], (i, j, k + 1))]
+                A[r, r] = 1 + sum(float(f) for f, _ in faces)
+                for f, (a, b, cc) in faces:
+                    if 0 <= a < nx and 0 <= b < ny and 0 <= cc < nz:
+                        A[r, idx(a, b, cc)] -= float(f)
+    x = rng.random(N)
+    np.testing.assert_allclose(oracle.apply(g, x), A @ x, rtol=1e-13)
+    assert np.allclose(A, A.T)
+
+
+def test_cg_reaches_rtol_and_exact_solution():
+    nx, ny, nz, c = 24, 20, 16, 1.0
+    rng = np.random.default_rng(5)
+    b = rng.random(nx * ny * nz).astype(np.float32)
+    g = oracle.grid(nx, ny, nz, 7, c)
+    x, it, rel = oracle.cg(g, b, 1e-6)
+    assert rel <= 1e-6 and it < 60
+    lam, _ = _lam7(nx, ny, nz, c)
+    xs = _dst_solve(b.astype(np.float64).reshape(nz, ny, nx), lam).ravel()
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-5
+    x64, it64, rel64 = oracle.cg(g, b.astype(np.float64), 1e-10)
+    assert rel64 <= 1e-10
+    assert np.linalg.norm(x64 - xs) / np.linalg.norm(xs) < 1e-9
+
+
+def test_refinement_to_1e10():
+    nx, c = 20, 20.0
+    g = oracle.grid(nx, nx, nx, 7, c)
+    rng = np.random.default_rng(6)
+    b = np.zeros(nx ** 3)
+    b[rng.choice(nx ** 3, 50, replace=False)] = 1.0
+    x, outer, inner, rel = oracle.refine(g, b, 1e-10, 1e-4)
+    assert rel <= 1e-10 and outer >= 2
+    lam, _ = _lam7(nx, nx, nx, c)
+    xs = _dst_solve(b.reshape(nx, nx, nx), lam).ravel()
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+
+
+def test_update_density_reference_semantics():
+    nx, h, D, k, dt = 12, 1.0, 0.5, 0.1, 0.4
+    rng = np.random.default_rng(7)
+    data = rng.random((nx, nx, nx)).astype(np.float32)
+    out, it, rr = oracle.update_density_reference(data, h, D, k, dt, eb=1e-10, max_iter=200)
+    # exact CN solve with the reference's right-hand side (h where dt is meant)
+    lam = _lam27(nx, nx, nx, dt * D / h ** 2, dt * k)
+    _, (tx, ty, tz) = _lam7(nx, nx, nx, 0)
+    S = -128 + 14 * (tx + ty + tz) + 3 * (tx * ty + ty * tz + tx * tz) + tx * ty * tz
+    u = data.astype(np.float64)
+    Su = scipy.fft.dstn(scipy.fft.dstn(u, type=1, norm="ortho") * S, type=1, norm="ortho")
+    b = (1 - h * k / 2) * u + (h * D / 2) * Su / (30 * h * h)
+    xs = _dst_solve(b, lam)
+    assert it > 0 and rr < 1e-10
+    np.testing.assert_allclose(out, xs, rtol=2e-5, atol=2e-5)
+    # at most 20 iterations by default, stop on the absolute squared residual
+    _, it20, _ = oracle.update_density_reference(data, h, D, k, dt)
+    assert 0 < it20 <= 20
+
+
+def test_update_density_reference_early_return_keeps_data():
+    # zero diffusion and decay: A = I, b = u, explicit guess = u -> r = 0 below
+    # the bound; the reference returns without assigning the guess
+    data = np.random.default_rng(8).random((6, 6, 6)).astype(np.float32)
+    out, it, rr = oracle.update_density_reference(data, 1.0, 0.0, 0.0, 0.4)
+    assert it == 0 and rr < 1e-6 and np.array_equal(out, data)
