@@ -1,0 +1,221 @@
+"""Benchmark: neuron-steps/s of the 1M-neuron HH + coupling timestep, and
+paracrine CG milliseconds to a 1e-8 relative residual, on one MI355X.
+
+Workload (the single-GPU configuration of the headline metric): 1,048,576
+squid HH neurons, RK4 at dt = 0.01 ms in fp32, ~210M-edge CSR with
+distance-dependent targets (exp(-d/10) in a 128^3 box), log-normal weights,
+event-driven deterministic coupling, release deposit onto a 256^3 grid.
+A step = coupling on the previous step's spikes + release deposit + HH RK4
+update + ascending spike list + raster append, enqueued as a captured
+hipGraph (chunks of --chunk steps).  Inputs are resident in HBM before the
+timed region starts.
+
+Prints ONE JSON line (rank 0).  Extra objects:
+  roofline     - the dominant kernel's algorithmic bytes per launch over its
+                 average launch time (HIP events on the library's stream),
+                 against 8 TB/s
+  cpu_baseline - the CPU oracle (fp32 restatement of the same step) on the
+                 host cores for a bounded sample of the same workload
+  cg           - plain Jacobi-CG to 1e-8 on the 256^3 paracrine grid (and the
+                 FP64-refined solve), milliseconds and iterations
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+METRIC = "neuron-steps/s at 1M HH; CG ms to 1e-8 resid (learned-PC vs plain)"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_events(ctx, fn, reps):
+    """Average ms of fn() over reps launches, HIP events on ctx's stream."""
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ctx.call("nnx_event_create", C.byref(e0))
+    ctx.call("nnx_event_create", C.byref(e1))
+    fn()
+    ctx.call("nnx_event_record", e0)
+    for _ in range(reps):
+        fn()
+    ctx.call("nnx_event_record", e1)
+    ms = C.c_float()
+    ctx.call("nnx_event_elapsed", e0, e1, C.byref(ms))
+    ctx.call("nnx_event_destroy", e0)
+    ctx.call("nnx_event_destroy", e1)
+    return ms.value / reps
+
+
+def cpu_baseline(net, steps):
+    import oracle
+    from neuronephys_clean_amd import _lib
+    T = np.float32
+    V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
+    s = np.zeros(net.n, T)
+    p = oracle.params(_lib.params_squid().as_dict())
+    _, ln, _, sec = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
+                                   np.asarray(net.iext, T), cap=1,
+                                   voxel_of=net.ensure_voxel(), nvox=int(np.prod(net.grid)))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": net.n * steps / sec, "unit": "neuron-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {steps} steps of the same network and initial state, fp32 oracle "
+                      f"(RK4 + push coupling in ascending source order + release counting), "
+                      f"OpenMP over neurons, {sec:.2f} s"}
+
+
+def cg_bench(ctx, grid_n, b_host, rtol=1e-8):
+    from neuronephys_clean_amd import _lib
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+    nvox = grid_n ** 3
+    g = _lib.Grid(grid_n, grid_n, grid_n, 7, 1.0, 0.0, None, None, None)
+    db = DA.from_numpy(ctx, b_host.astype(np.float32))
+    dx = DA.zeros(ctx, nvox, np.float32)
+    o = _lib.CgOpts(rtol=rtol, maxit=2000, precond=_lib.PRECOND_JACOBI, check_every=8)
+    r = _lib.CgResult()
+    ctx._l.nnx_cg_solve(ctx.h, C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(r))  # warm
+    runs = []
+    for _ in range(3):
+        r = _lib.CgResult()
+        st = ctx._l.nnx_cg_solve(ctx.h, C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(r))
+        runs.append((r.ms, r.iters, r.relres, r.true_relres, st))
+    runs.sort()
+    ms, it, rel, true_rel, st = runs[1]
+    # mixed precision: FP64 outer residual to a true 1e-8
+    db64 = DA.from_numpy(ctx, b_host.astype(np.float64))
+    dx64 = DA.zeros(ctx, nvox, np.float64)
+    r2 = _lib.CgResult()
+    ctx._l.nnx_cg_solve_refine(ctx.h, C.byref(g), db64.ptr, dx64.ptr, C.byref(o), 1e-5, 20,
+                               C.byref(r2))
+    # dominant CG kernels: per-iteration time from the solve
+    return {"grid": f"{grid_n}^3", "c": 1.0, "rtol": rtol, "precond": "jacobi",
+            "plain_ms": ms, "iters": it, "relres": rel, "true_relres_fp64": true_rel,
+            "ms_per_iter": ms / max(it, 1),
+            "bytes_per_iter_alg": 44 * nvox,
+            "refined_ms": r2.ms, "refined_outer": r2.outer, "refined_inner_iters": r2.iters,
+            "refined_relres_fp64": r2.relres, "learned_pc_ms": None,
+            "status": int(st)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--couple", default="event", choices=["event", "pull"])
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+                                   "(multi-GPU sharding is design only)"}))
+        sys.exit(2)
+    if args.chunk < 1 or args.steps % args.chunk or args.warmup % args.chunk:
+        args.chunk = 1 if args.steps % 2 or args.warmup % 2 else 2
+
+    from neuronephys_clean_amd import _lib, configs
+    from neuronephys_clean_amd.sim import Simulation
+
+    t0 = time.time()
+    net = configs.c2(n=args.n)
+    log(f"bench: network n={net.n} nnz={net.nnz} grid={net.grid} built in {time.time() - t0:.1f}s")
+    ctx = _lib.context(0)
+    t0 = time.time()
+    sim = Simulation(net, np.float32, couple=args.couple, record_cap=1 << 24, deposit=True,
+                     ctx=ctx, device_csc=True)
+    sim.synchronize()
+    log(f"bench: uploaded + device CSC in {time.time() - t0:.1f}s")
+
+    sim.run(args.warmup, use_graph=True, chunk=args.chunk)
+    sim.synchronize()
+    len0 = sim.raster()[1]
+    log(f"bench: warmup {args.warmup} steps done, {len0} spikes")
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ctx.call("nnx_event_create", C.byref(e0))
+    ctx.call("nnx_event_create", C.byref(e1))
+    sim.synchronize()
+    wall0 = time.perf_counter()
+    ctx.call("nnx_event_record", e0)
+    sim.run(args.steps, use_graph=True, chunk=args.chunk)
+    ctx.call("nnx_event_record", e1)
+    sim.synchronize()
+    wall = time.perf_counter() - wall0
+    ev_ms = C.c_float()
+    ctx.call("nnx_event_elapsed", e0, e1, C.byref(ev_ms))
+    spikes = sim.raster()[1] - len0
+    ms_step = wall * 1e3 / args.steps
+    value = net.n * args.steps / wall
+    log(f"bench: {args.steps} steps in {wall * 1e3:.1f} ms (events {ev_ms.value:.1f} ms), "
+        f"{spikes} spikes, {value:.3e} neuron-steps/s")
+
+    # dominant kernel: the HH RK4 step (one launch per step), timed alone on
+    # the library stream over the same state
+    st = sim.plan.state
+    bits = sim.a["bits0"].ptr
+    gin = sim.a["g_in"].ptr
+    p = sim.params
+
+    def hh():
+        ctx.call("nnx_hh_step", C.byref(st), C.byref(p), _lib.INT_RK4, _lib.SPIKE_CROSSING,
+                 net.dt, 0, gin, bits)
+    hh_ms = timed_events(ctx, hh, 200)
+    hh_bytes = 48.125 * net.n
+    roof = {"kernel": "k_hh_rk4<float>", "bound": "hbm", "achieved": hh_bytes / (hh_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hh_bytes / (hh_ms * 1e-3) / HBM_PEAK,
+            "traffic": None, "avg_launch_us": hh_ms * 1e3, "alg_bytes_per_launch": hh_bytes}
+
+    # per-step share of the coupling: eager per-kernel timing of one step's
+    # coupling on the current spikes
+    def couple():
+        ctx.call("nnx_couple", C.byref(sim.graph), 1 if args.couple == "event" else 0, _lib.F32,
+                 sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
+    couple_ms = timed_events(ctx, couple, 50)
+
+    cg = None
+    if not args.no_cg:
+        vc = sim.a["voxel_counts"].numpy().astype(np.float32)
+        b = 0.01 * vc
+        if not b.any():
+            b[len(b) // 2] = 1.0
+        cg = cg_bench(ctx, net.grid[0], b)
+        log(f"bench: CG {cg}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(net, args.cpu_steps)
+        log(f"bench: cpu {cpu}")
+
+    out = {"metric": METRIC, "value": value, "unit": "neuron-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (Philox4x64-10, seed 0)",
+           "config": {"workload": "c2: 1,048,576 squid HH neurons, RK4 dt 0.01 ms, ~210M-edge "
+                                  "exp(-d/10) CSR, log-normal weights, event-driven deterministic "
+                                  "coupling, release deposit on 256^3",
+                      "n": net.n, "nnz": net.nnz, "dt": net.dt, "couple": args.couple,
+                      "graph_chunk": args.chunk, "spikes_timed": int(spikes),
+                      "spikes_per_step": spikes / args.steps},
+           "roofline": roof, "cpu_baseline": cpu, "cg": cg,
+           "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
+                                 "step_graph": ms_step * 1e3},
+           "event_ms": ev_ms.value}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -29,6 +29,7 @@ nnx_status scratch(nnx_ctx* c, const char* name, size_t bytes, bool zero, void**
 template <class T> HHConst<T> make_const(const nnx_hh_params* q, double dt) {
     HHConst<T> p;
     p.C = T(q->C);
+    p.invC = T(1.0 / q->C);
     p.gna = T(q->gna);
     p.gk = T(q->gk);
     p.gl = T(q->gl);

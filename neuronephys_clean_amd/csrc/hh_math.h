@@ -13,8 +13,8 @@ namespace nnx {
 // ---------------------------------------------------------------------------
 // exp(x): x = k ln2 + r with |r| <= ln2/2 (Cody-Waite: ln2 = hi + lo, hi with
 // 16 (float) / 42 (double) significant bits so k*hi is exact), e^r by its
-// Taylor polynomial (degree 7 / 13, Horner with fma), times 2^k built in the
-// exponent field.  Arguments are clamped so that the result stays normal.
+// Taylor polynomial (degree 7 / 13, Horner with fma), scaled by 2^k with
+// ldexp (exact).  Arguments are clamped so that the result stays normal.
 // Max error 1 ulp against a correctly rounded exp over the clamped range.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float dexp(float x) {
@@ -32,8 +32,7 @@ __device__ __forceinline__ float dexp(float x) {
     p = __builtin_fmaf(p, r, 0.5f);
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
-    const uint32_t bits = (uint32_t)((int)k + 127) << 23;
-    return p * __builtin_bit_cast(float, bits);
+    return __builtin_amdgcn_ldexpf(p, (int)k);  // exact: the result is normal
 }
 
 __device__ __forceinline__ double dexp(double x) {
@@ -57,8 +56,7 @@ __device__ __forceinline__ double dexp(double x) {
     p = __builtin_fma(p, r, 0.5);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
-    const uint64_t bits = (uint64_t)((int64_t)k + 1023) << 52;
-    return p * __builtin_bit_cast(double, bits);
+    return __builtin_amdgcn_ldexp(p, (int)k);
 }
 
 template <class T> __device__ __forceinline__ T dfma(T a, T b, T c);
@@ -66,40 +64,48 @@ template <> __device__ __forceinline__ float dfma(float a, float b, float c) { r
 template <> __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <class T> __device__ __forceinline__ T dabs(T a) { return a < T(0) ? -a : a; }
 
-// a / (1 - e^-a); for |a| < 1/2 the series 1 + a/2 + a^2/12 - a^4/720 +
-// a^6/30240 - a^8/1209600 (Bernoulli numbers of x/(e^x - 1) at x = -a),
-// which removes the 0/0 at a = 0.
-template <class T> __device__ __forceinline__ T vtrap(T a) {
-    if (dabs(a) < T(0.5)) {
-        const T a2 = a * a;
-        T q = dfma(a2, T(-1.0 / 1209600.0), T(1.0 / 30240.0));
-        q = dfma(a2, q, T(-1.0 / 720.0));
-        q = dfma(a2, q, T(1.0 / 12.0));
-        return dfma(a2, q, dfma(a, T(0.5), T(1)));
-    }
-    return a / (T(1) - dexp(-a));
+// a / (1 - e^-a) given ea = e^-a; for |a| < 1/2 the series 1 + a/2 + a^2/12
+// - a^4/720 + a^6/30240 - a^8/1209600 (Bernoulli numbers of x/(e^x - 1) at
+// x = -a) is selected instead, which removes the 0/0 at a = 0.  Both are
+// evaluated (no divergent branch).
+template <class T> __device__ __forceinline__ T vtrap(T a, T ea) {
+    const T a2 = a * a;
+    T q = dfma(a2, T(-1.0 / 1209600.0), T(1.0 / 30240.0));
+    q = dfma(a2, q, T(-1.0 / 720.0));
+    q = dfma(a2, q, T(1.0 / 12.0));
+    const T ser = dfma(a2, q, dfma(a, T(0.5), T(1)));
+    const T dir = a / (T(1) - ea);
+    return dabs(a) < T(0.5) ? ser : dir;
 }
 
 template <class T> struct Rates { T am, bm, ah, bh, an, bn; };
 
 // HH rates of the reference (src/ephys.cu:57-63, written for a -60 mV rest),
-// evaluated at u = V + v_shift.
+// evaluated at u = V + v_shift with three exponentials:
+//   a = 3.5 + 0.1u, ea = e^-a:  am = a/(1-e^-a); b = a + 1.5, e^-b = ea e^-1.5,
+//   an = 0.1 b/(1-e^-b); bh = 1/(e^(-3-0.1u) + 1) = 1/(ea e^0.5 + 1);
+//   w = u + 60:  bn = 0.125 e^(-w/80), ah = 0.07 e^(-u/20-3) = 0.07 (e^(-w/80))^4,
+//   bm = 4 e^(-w/18).
 template <class T> __device__ __forceinline__ Rates<T> rates(T u) {
     Rates<T> R;
     const T a = T(3.5) + T(0.1) * u;
-    R.am = vtrap(a);
-    const T b = T(5) + T(0.1) * u;
-    R.an = T(0.1) * vtrap(b);
-    R.ah = T(0.07) * dexp(-u / T(20) - T(3));
-    R.bm = T(4) * dexp(-(u + T(60)) / T(18));
-    R.bn = T(0.125) * dexp(-(u + T(60)) / T(80));
-    R.bh = T(1) / (dexp(T(-3) - T(0.1) * u) + T(1));
+    const T ea = dexp(-a);
+    R.am = vtrap(a, ea);
+    const T b = a + T(1.5);
+    R.an = T(0.1) * vtrap(b, ea * T(0x1.c8f87724b5c1dp-3));
+    R.bh = T(1) / (ea * T(0x1.a61298e1e069cp+0) + T(1));
+    const T w = u + T(60);
+    const T e80 = dexp(w * T(-0.0125));
+    const T e40 = e80 * e80;
+    R.bn = T(0.125) * e80;
+    R.ah = T(0.07) * (e40 * e40);
+    R.bm = T(4) * dexp(w * T(-1.0 / 18.0));
     return R;
 }
 
 // Model constants in T (cast once on the host from the double parameters).
 template <class T> struct HHConst {
-    T C, gna, gk, gl, ena, ek, el, vshift, esyn;
+    T C, invC, gna, gk, gl, ena, ek, el, vshift, esyn;
     T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean;
     T v_thresh, out_thresh;
     // step constants: dt, dt/2, dt/6, exp(-dt/(2 tau)), exp(-dt/tau)
@@ -120,7 +126,7 @@ __device__ __forceinline__ void deriv(const HHConst<T>& p, T V, T m, T h, T n, T
     const T ik = p.gk * n4 * (V - p.ek);
     const T il = p.gl * (V - p.el);
     const T isyn = s * (V - p.esyn);
-    dV = (iext - ina - ik - il - isyn) / p.C;
+    dV = (iext - ina - ik - il - isyn) * p.invC;
 }
 
 // Classical RK4 over (V, m, h, n).  The synaptic conductance jumps by the

@@ -119,225 +119,363 @@ __global__ __launch_bounds__(256) void k_apply(GridK g, const U* __restrict__ x,
 }
 
 // ---------------------------------------------------------------------------
-// deterministic block reductions and the last-block pattern
+// deterministic reductions without atomics or fences
+//
+// Every reducing kernel runs on a fixed grid of kRedBlocks blocks and writes
+// one partial per block.  The kernels that need the total (the next CG
+// kernels) re-sum those partials in every block, in a fixed order, so every
+// block obtains the same bits and no block waits for another.  (A last-block
+// pattern would need a device-scope fence per block, i.e. an L2 write-back
+// on each of the 8 XCDs per block.)
 // ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 1024;
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
-// sums NV doubles over a 256-thread block; result valid in thread 0
-template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[4][NV];
+// Sums NV values over the block (256 or 512 threads); every thread receives
+// the total: lane 0's butterfly result of each wave, waves added in order.
+template <int NV> __device__ __forceinline__ void block_sum_all(double (&v)[NV]) {
+    __shared__ double sh[8][NV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+    __syncthreads();
     if (lane == 0)
 #pragma unroll
         for (int q = 0; q < NV; ++q) sh[wave][q] = v[q];
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double t = sh[0][q];
+        for (int w = 1; w < nw; ++w) t += sh[w][q];
+        v[q] = t;
+    }
+}
+
+// Total of nb partials of NV doubles with a fixed tree whatever the block
+// size: threads t < 256 add entries t, t+256, ... in order, the other threads
+// contribute zeros (appending +0 to a sum leaves it unchanged), then the
+// block sum.  Identical in every block of every kernel.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb, double (&out)[NV]) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = 0;
+    if (threadIdx.x < 256)
+        for (int t = threadIdx.x; t < nb; t += 256)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] += part[t * NV + q];
+    block_sum_all<NV>(out);
+}
+
+template <int NV> __device__ __forceinline__ void put_partial(double (&v)[NV], double* part) {
+    block_sum_all<NV>(v);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) v[q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
-    __syncthreads();
-}
-
-__device__ __forceinline__ double load_coherent(const double* p) {
-    unsigned long long b = __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(double, b);
-}
-
-// Every block deposits its NV partial sums; the last block to arrive sums all
-// partials in block order and returns true with the totals in `tot` (all
-// threads of that block).  The counter is reset for the next launch.
-template <int NV>
-__device__ __forceinline__ bool last_block(double (&v)[NV], double* partials, unsigned* counter,
-                                           double (&tot)[NV]) {
-    __shared__ int am_last;
-    block_sum<NV>(v);
-    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) partials[(size_t)b * NV + q] = v[q];
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        am_last = prev == nb - 1;
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    double s[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) s[q] = 0;
-    for (unsigned t = threadIdx.x; t < nb; t += 256)
-#pragma unroll
-        for (int q = 0; q < NV; ++q) s[q] += load_coherent(partials + (size_t)t * NV + q);
-    block_sum<NV>(s);
-    __shared__ double bc[NV];
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) bc[q] = s[q];
-        *counter = 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] = bc[q];
-    return true;
+        for (int q = 0; q < NV; ++q) part[blockIdx.x * NV + q] = v[q];
 }
 
 // ---------------------------------------------------------------------------
 // CG state on the device
 // ---------------------------------------------------------------------------
-enum { SC_RZ = 0, SC_RR, SC_PQ, SC_ALPHA, SC_BETA, SC_BB, SC_THR, SC_TRUE, SC_RR0, SC_N };
-enum { FL_DONE = 0, FL_ITERS, FL_MAXIT, FL_ABS, FL_INITDONE, FL_N };
+enum { SC_BB = 0, SC_THR, SC_RR, SC_RZ, SC_N };
+enum { FL_DONE = 0, FL_ITERS, FL_INITDONE, FL_ABS, FL_N };
 
 struct CgDev {
-    float *r, *p, *q, *dinv;  // dinv NULL: constant (scalar dinv_c)
+    float *r, *q, *dinv;  // dinv NULL: constant diagonal (dinv_c)
     float dinv_c;
+    double *pu0, *pu1;    // (rz, rr) partials of the updates, ping-pong; pu0 = init's
+    double *ppq;          // pq partials of the operator application
+    int npq;              // their count (blocks of the operator kernel)
+    double *pb;           // bb partials of the init
     double* sc;
     int32_t* fl;
-    double* partials;
-    unsigned* counter;  // 4 slots
 };
 
-__device__ __forceinline__ bool cg_done(const CgDev& d) {
-    return __hip_atomic_load(&d.fl[FL_DONE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-}
 __device__ __forceinline__ float dv(const CgDev& d, int64_t i) { return d.dinv ? d.dinv[i] : d.dinv_c; }
 
-// init: r = b - A x (x0) or r = b with x = 0; p = 0; beta = 0; norms.
+// Scalars entering iteration k (rz and rr of the current residual, and rz of
+// the previous one when want_prev).  Returns false once the solve is over:
+// the persistent done flag is set, or this residual already meets the
+// tolerance (then block 0 sets the flag).  Uniform across the grid.
+__device__ __forceinline__ bool cg_scalars(const CgDev& d, int k, bool want_prev, double& rz,
+                                           double& rz_prev, double& rr) {
+    if (d.fl[FL_DONE]) return false;
+    double t[2];
+    sum_partials<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, t);
+    rz = t[0];
+    rr = t[1];
+    const double thr = d.sc[SC_THR];
+    const bool conv = d.fl[FL_ABS] ? (rr < thr) : (rr <= thr);
+    if (conv) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) d.fl[FL_DONE] = 1;
+        return false;
+    }
+    rz_prev = 0;
+    if (want_prev && k > 0) {
+        sum_partials<2>((k & 1) ? d.pu0 : d.pu1, kRedBlocks, t);
+        rz_prev = t[0];
+    }
+    return true;
+}
+
+// r = b - A x (use_x0) or r = b with x = 0; p0 = 0; partials bb, (rz, rr).
 __global__ __launch_bounds__(256) void k_cg_init(GridK g, CgDev d, const float* __restrict__ b,
-                                                 float* __restrict__ x, int use_x0, double tol,
-                                                 int maxit, int abs_mode) {
+                                                 float* __restrict__ x, int use_x0,
+                                                 float* __restrict__ p0) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    double v[3] = {0, 0, 0};  // bb, rr, rz
+    double vb[1] = {0}, vu[2] = {0, 0};
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const float bi = b[i];
         float ri = bi;
         if (use_x0) {
             const int ii = (int)(i % g.nx), jj = (int)((i / g.nx) % g.ny), kk = (int)(i / ((int64_t)g.nx * g.ny));
             ri = bi - apply_at<float, float>(x, g, ii, jj, kk);
-        } else {
-            x[i] = 0.0f;
         }
         d.r[i] = ri;
-        d.p[i] = 0.0f;
-        const float zi = ri * dv(d, i);
-        v[0] += (double)bi * bi;
-        v[1] += (double)ri * ri;
-        v[2] += (double)ri * zi;
+        p0[i] = 0.0f;
+        vb[0] += (double)bi * bi;
+        vu[0] += (double)ri * (ri * dv(d, i));
+        vu[1] += (double)ri * ri;
     }
-    double t[3];
-    if (last_block<3>(v, d.partials, d.counter + 0, t) && threadIdx.x == 0) {
-        d.sc[SC_BB] = t[0];
-        d.sc[SC_RR] = t[1];
-        d.sc[SC_RR0] = t[1];
-        d.sc[SC_RZ] = t[2];
-        d.sc[SC_BETA] = 0.0;
-        const double thr = abs_mode ? tol : tol * tol * t[0];
+    put_partial<1>(vb, d.pb);
+    put_partial<2>(vu, d.pu0);
+    if (!use_x0) {
+        // x is not read by the init: zero it after the reads of other blocks cannot race
+        for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) x[i] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cg_init_fin(CgDev d, double tol, int abs_mode, int maxit) {
+    double bb[1], u[2];
+    sum_partials<1>(d.pb, kRedBlocks, bb);
+    sum_partials<2>(d.pu0, kRedBlocks, u);
+    if (threadIdx.x == 0) {
+        const double thr = abs_mode ? tol : tol * tol * bb[0];
+        const int conv = abs_mode ? (u[1] < thr) : (u[1] <= thr);
+        d.sc[SC_BB] = bb[0];
         d.sc[SC_THR] = thr;
-        const int done = abs_mode ? (t[1] < thr) : (t[1] <= thr);
-        d.fl[FL_DONE] = done || maxit <= 0;
-        d.fl[FL_INITDONE] = done;
-        d.fl[FL_ITERS] = 0;
-        d.fl[FL_MAXIT] = maxit;
+        d.sc[SC_RZ] = u[0];
+        d.sc[SC_RR] = u[1];
         d.fl[FL_ABS] = abs_mode;
+        d.fl[FL_INITDONE] = conv;
+        d.fl[FL_DONE] = conv || maxit <= 0;
+        d.fl[FL_ITERS] = 0;
     }
 }
 
-// p = z + beta p, z = dinv r
-__global__ __launch_bounds__(256) void k_cg_dir(int64_t n, CgDev d) {
-    if (cg_done(d)) return;
-    const float beta = (float)d.sc[SC_BETA];
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        d.p[i] = d.r[i] * dv(d, i) + beta * d.p[i];
+// residual norms after the last executed update (host read-back)
+__global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
+    const int it = d.fl[FL_ITERS];
+    double u[2];
+    sum_partials<2>((it & 1) ? d.pu1 : d.pu0, kRedBlocks, u);
+    if (threadIdx.x == 0) {
+        d.sc[SC_RZ] = u[0];
+        d.sc[SC_RR] = u[1];
+        const double thr = d.sc[SC_THR];
+        if (d.fl[FL_ABS] ? (u[1] < thr) : (u[1] <= thr)) d.fl[FL_DONE] = 1;
+    }
 }
 
-// q = A p (z-marching, 8 planes per thread for the constant 7-point stencil),
-// pq partials, alpha = rz / pq by the last block.
-constexpr int kZc = 8;
-__global__ __launch_bounds__(256) void k_cg_spmv7(GridK g, CgDev d) {
-    if (cg_done(d)) return;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k0 = blockIdx.z * kZc;
-    const bool in = i < g.nx && j < g.ny;
-    const int64_t sxy = (int64_t)g.nx * g.ny;
-    const float* __restrict__ p = d.p;
+// Constant 7-point stencil, 2.5D LDS tiling with four voxels per thread.
+// A block of 512 threads owns a 256 (x) x 8 (y) tile and marches kZc planes
+// in z; thread (lx, ly) holds x = 4lx .. 4lx+3 of row ly as a float4.  Each
+// plane (plus its one-voxel halo in x and y) goes into a double-buffered LDS
+// tile; the column's z neighbours stay in registers; the next plane's loads
+// are issued before the plane's barrier (a barrier drains outstanding loads,
+// so 16-byte loads per thread are what keeps enough bytes in flight).
+// Requires nx % 4 == 0 (16-byte aligned rows).
+//   FUSED = false: y = A x (summation order of the CPU restatement).
+//   FUSED = true : CG iteration k - p_new = dinv r + beta p_old at every
+//                  point, q = A p_new, p_new stored, pq partials.
+constexpr int kTX4 = 256, kTY4 = 8, kZ4 = 16, kT4 = 512, kRow4 = kTX4 + 8;
+template <bool FUSED>
+__global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int nparts,
+                                                    const float* __restrict__ src,
+                                                    const float* __restrict__ pold,
+                                                    float* __restrict__ pnew,
+                                                    float* __restrict__ dst) {
+    __shared__ float t[2][kTY4 + 2][kRow4];
+    float beta = 0.0f, di = 1.0f;
+    if (FUSED) {
+        double rz, rzp, rr;
+        if (!cg_scalars(d, k, true, rz, rzp, rr)) return;
+        beta = k ? (float)(rz / rzp) : 0.0f;
+        di = d.dinv_c;
+    }
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int tx = (nx + kTX4 - 1) / kTX4, ty = (ny + kTY4 - 1) / kTY4, tz = (nz + kZ4 - 1) / kZ4;
+    const int ntiles = tx * ty * tz;
+    const int64_t sxy = (int64_t)nx * ny;
+    const int tt = threadIdx.x, lx = tt & 63, ly = tt >> 6;
+    // halo roles: 0..63 row y=-1, 64..127 row y=kTY4 (float4 each), 128..135
+    // column x=-1, 136..143 column x=kTX4 (one voxel each)
+    const bool hrow = tt < 128, hcol = tt >= 128 && tt < 144;
+    const int hy = tt < 64 ? -1 : tt < 128 ? kTY4 : (tt < 136 ? tt - 128 : tt - 136);
+    const int hl = tt < 64 ? tt : tt - 64;
+    const int hslot = tt < 136 ? 3 : 4 + kTX4;
+    const float cdiag = g.diag, cc = g.c;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     double v[1] = {0};
-    int64_t id = i + (int64_t)g.nx * j + sxy * k0;
-    float pm = (in && k0 > 0) ? p[id - sxy] : 0.0f;
-    float pc = in ? p[id] : 0.0f;
-    for (int kk = 0; kk < kZc; ++kk, id += sxy) {
-        const int k = k0 + kk;
-        if (k >= g.nz) break;
-        const float pp = (in && k + 1 < g.nz) ? p[id + sxy] : 0.0f;
-        if (in) {
-            float s = (i > 0 ? p[id - 1] : 0.0f) + (i + 1 < g.nx ? p[id + 1] : 0.0f);
-            s = s + (j > 0 ? p[id - g.nx] : 0.0f);
-            s = s + (j + 1 < g.ny ? p[id + g.nx] : 0.0f);
-            s = s + pm;
-            s = s + pp;
-            const float qv = g.diag * pc - g.c * s;
-            d.q[id] = qv;
-            v[0] += (double)pc * qv;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
+        const int x0 = bx * kTX4, k0 = bz * kZ4, kend = min(k0 + kZ4, nz);
+        const int j = by * kTY4 + ly;
+        const int xo = x0 + 4 * lx;
+        const bool own = xo < nx && j < ny;
+        const int gj = by * kTY4 + hy;
+        const bool hr_ok = hrow && gj >= 0 && gj < ny && x0 + 4 * hl < nx;
+        const int hx = tt < 136 ? x0 - 1 : x0 + kTX4;
+        const bool hc_ok = hcol && gj < ny && hx >= 0 && hx < nx;
+        const int64_t ocol = xo + (int64_t)nx * j;
+        const int64_t rcol = x0 + 4 * hl + (int64_t)nx * gj;
+        const int64_t ccol = hx + (int64_t)nx * gj;
+        auto val4 = [&](bool ok, int64_t e) -> float4 {
+            if (!ok) return z4;
+            if (!FUSED) return *reinterpret_cast<const float4*>(src + e);
+            const float4 a = *reinterpret_cast<const float4*>(d.r + e);
+            const float4 b = *reinterpret_cast<const float4*>(pold + e);
+            return make_float4(a.x * di + beta * b.x, a.y * di + beta * b.y, a.z * di + beta * b.z,
+                               a.w * di + beta * b.w);
+        };
+        auto val1 = [&](bool ok, int64_t e) -> float {
+            if (!ok) return 0.0f;
+            return FUSED ? d.r[e] * di + beta * pold[e] : src[e];
+        };
+        float4 pm = val4(own && k0 > 0, ocol + sxy * (k0 - 1));
+        float4 pc = val4(own, ocol + sxy * k0);
+        *reinterpret_cast<float4*>(&t[0][ly + 1][4 + 4 * lx]) = pc;
+        if (hrow) *reinterpret_cast<float4*>(&t[0][hy + 1][4 + 4 * hl]) = val4(hr_ok, rcol + sxy * k0);
+        if (hcol) t[0][hy + 1][hslot] = val1(hc_ok, ccol + sxy * k0);
+        for (int kz = k0; kz < kend; ++kz) {
+            const int b = (kz - k0) & 1;
+            const bool nxt = kz + 1 < nz;
+            const float4 pp = val4(own && nxt, ocol + sxy * (kz + 1));
+            const float4 hr = val4(hr_ok && nxt, rcol + sxy * (kz + 1));
+            const float hc = val1(hc_ok && nxt, ccol + sxy * (kz + 1));
+            __syncthreads();
+            if (own) {
+                const float left = t[b][ly + 1][3 + 4 * lx], right = t[b][ly + 1][8 + 4 * lx];
+                const float4 ym = *reinterpret_cast<const float4*>(&t[b][ly][4 + 4 * lx]);
+                const float4 yp = *reinterpret_cast<const float4*>(&t[b][ly + 2][4 + 4 * lx]);
+                float4 o;
+                o.x = cdiag * pc.x - cc * (left + pc.y + ym.x + yp.x + pm.x + pp.x);
+                o.y = cdiag * pc.y - cc * (pc.x + pc.z + ym.y + yp.y + pm.y + pp.y);
+                o.z = cdiag * pc.z - cc * (pc.y + pc.w + ym.z + yp.z + pm.z + pp.z);
+                o.w = cdiag * pc.w - cc * (pc.z + right + ym.w + yp.w + pm.w + pp.w);
+                const int64_t id = ocol + sxy * kz;
+                *reinterpret_cast<float4*>(dst + id) = o;
+                if (FUSED) {
+                    *reinterpret_cast<float4*>(pnew + id) = pc;
+                    v[0] += (double)pc.x * o.x;
+                    v[0] += (double)pc.y * o.y;
+                    v[0] += (double)pc.z * o.z;
+                    v[0] += (double)pc.w * o.w;
+                }
+            }
+            *reinterpret_cast<float4*>(&t[b ^ 1][ly + 1][4 + 4 * lx]) = pp;
+            if (hrow) *reinterpret_cast<float4*>(&t[b ^ 1][hy + 1][4 + 4 * hl]) = hr;
+            if (hcol) t[b ^ 1][hy + 1][hslot] = hc;
+            pm = pc;
+            pc = pp;
         }
-        pm = pc;
-        pc = pp;
+        __syncthreads();  // the next tile reuses both buffers
     }
-    double t[1];
-    if (last_block<1>(v, d.partials, d.counter + 1, t) && threadIdx.x == 0) {
-        d.sc[SC_PQ] = t[0];
-        d.sc[SC_ALPHA] = d.sc[SC_RZ] / t[0];
+    if (FUSED) {
+        put_partial<1>(v, d.ppq);
+        (void)nparts;
     }
 }
 
-// generic operator (variable 7-point, 27-point), one voxel per thread
-__global__ __launch_bounds__(256) void k_cg_spmv_gen(GridK g, CgDev d) {
-    if (cg_done(d)) return;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
+static int stencil4_blocks(const GridK& g) {
+    const int64_t nt = ceil_div(g.nx, kTX4) * ceil_div(g.ny, kTY4) * ceil_div(g.nz, kZ4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nt, 2048));
+}
+
+// generic operators: p_new elementwise, then q = A p_new
+__global__ __launch_bounds__(256) void k_cg_dir(int64_t n, CgDev d, int k,
+                                                const float* __restrict__ pold,
+                                                float* __restrict__ pnew) {
+    double rz, rzp, rr;
+    if (!cg_scalars(d, k, true, rz, rzp, rr)) return;
+    const float beta = k ? (float)(rz / rzp) : 0.0f;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        pnew[i] = d.r[i] * dv(d, i) + beta * pold[i];
+}
+
+__global__ __launch_bounds__(256) void k_cg_spmv_gen(GridK g, CgDev d, int k,
+                                                     const float* __restrict__ p) {
+    double rz, rzp, rr;
+    if (!cg_scalars(d, k, false, rz, rzp, rr)) return;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     double v[1] = {0};
-    if (i < g.nx && j < g.ny) {
-        const int64_t id = i + (int64_t)g.nx * (j + (int64_t)g.ny * k);
-        const float qv = apply_at<float, float>(d.p, g, i, j, k);
+    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
+        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), kk = (int)(id / ((int64_t)g.nx * g.ny));
+        const float qv = apply_at<float, float>(p, g, i, j, kk);
         d.q[id] = qv;
-        v[0] = (double)d.p[id] * qv;
+        v[0] += (double)p[id] * qv;
     }
-    double t[1];
-    if (last_block<1>(v, d.partials, d.counter + 1, t) && threadIdx.x == 0) {
-        d.sc[SC_PQ] = t[0];
-        d.sc[SC_ALPHA] = d.sc[SC_RZ] / t[0];
-    }
+    put_partial<1>(v, d.ppq);
 }
 
-// x += alpha p; r -= alpha q; rz, rr; beta; stop test (last block)
-__global__ __launch_bounds__(256) void k_cg_update(int64_t n, CgDev d, float* __restrict__ x) {
-    if (cg_done(d)) return;
-    const float alpha = (float)d.sc[SC_ALPHA];
+// x += alpha p, r -= alpha q with alpha = rz / pq; partials of the new rz, rr.
+// Four consecutive voxels per thread and pass (float4), all loads of a pass
+// issued before its stores.
+__global__ __launch_bounds__(256) void k_cg_update(int64_t n, CgDev d, int k,
+                                                   const float* __restrict__ p,
+                                                   float* __restrict__ x, int vec) {
+    double rz, rzp, rr;
+    if (!cg_scalars(d, k, false, rz, rzp, rr)) return;
+    double pq[1];
+    sum_partials<1>(d.ppq, d.npq, pq);
+    const float alpha = (float)(rz / pq[0]);
+    float* __restrict__ r = d.r;
+    const float* __restrict__ q = d.q;
+    const float* __restrict__ dinv = d.dinv;
+    const float dc = d.dinv_c;
     double v[2] = {0, 0};
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        x[i] = x[i] + alpha * d.p[i];
-        const float ri = d.r[i] - alpha * d.q[i];
-        d.r[i] = ri;
-        v[0] += (double)ri * (ri * dv(d, i));
+    const int64_t n4 = vec ? n / 4 : 0;  // float4 path needs a 16-byte aligned x
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = blockIdx.x * 256ll + threadIdx.x; c < n4; c += stride) {
+        const float4 xv = reinterpret_cast<const float4*>(x)[c];
+        const float4 pv = reinterpret_cast<const float4*>(p)[c];
+        const float4 rv = reinterpret_cast<const float4*>(r)[c];
+        const float4 qv = reinterpret_cast<const float4*>(q)[c];
+        float4 dd = make_float4(dc, dc, dc, dc);
+        if (dinv) dd = reinterpret_cast<const float4*>(dinv)[c];
+        float4 xo, ro;
+        xo.x = xv.x + alpha * pv.x;
+        xo.y = xv.y + alpha * pv.y;
+        xo.z = xv.z + alpha * pv.z;
+        xo.w = xv.w + alpha * pv.w;
+        ro.x = rv.x - alpha * qv.x;
+        ro.y = rv.y - alpha * qv.y;
+        ro.z = rv.z - alpha * qv.z;
+        ro.w = rv.w - alpha * qv.w;
+        reinterpret_cast<float4*>(x)[c] = xo;
+        reinterpret_cast<float4*>(r)[c] = ro;
+        v[0] += (double)ro.x * (ro.x * dd.x);
+        v[0] += (double)ro.y * (ro.y * dd.y);
+        v[0] += (double)ro.z * (ro.z * dd.z);
+        v[0] += (double)ro.w * (ro.w * dd.w);
+        v[1] += (double)ro.x * ro.x;
+        v[1] += (double)ro.y * ro.y;
+        v[1] += (double)ro.z * ro.z;
+        v[1] += (double)ro.w * ro.w;
+    }
+    for (int64_t i = n4 * 4 + blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
+        x[i] = x[i] + alpha * p[i];
+        const float ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        v[0] += (double)ri * (ri * (dinv ? dinv[i] : dc));
         v[1] += (double)ri * ri;
     }
-    double t[2];
-    if (last_block<2>(v, d.partials, d.counter + 2, t) && threadIdx.x == 0) {
-        d.sc[SC_BETA] = t[0] / d.sc[SC_RZ];
-        d.sc[SC_RZ] = t[0];
-        d.sc[SC_RR] = t[1];
-        const int it = d.fl[FL_ITERS] + 1;
-        d.fl[FL_ITERS] = it;
-        const double thr = d.sc[SC_THR];
-        const int conv = d.fl[FL_ABS] ? (t[1] < thr) : (t[1] <= thr);
-        if (conv || it >= d.fl[FL_MAXIT]) d.fl[FL_DONE] = 1;
-    }
+    put_partial<2>(v, (k & 1) ? d.pu0 : d.pu1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.fl[FL_ITERS] = k + 1;
 }
 
 __global__ void k_dinv(GridK g, float* dinv) {
@@ -346,26 +484,27 @@ __global__ void k_dinv(GridK g, float* dinv) {
         dinv[i] = dinv_at(g, i);
 }
 
-// ||b - A x||^2 and ||b||^2 in FP64 (x float or double), into sc[SC_TRUE], sc[SC_BB]
+// ||b - A x||^2 and ||b||^2 in FP64 (x float or double): partials, then one
+// block adds them (out2[0], out2[1]); r_out optionally receives b - A x.
 template <class U, class B>
 __global__ __launch_bounds__(256) void k_true_res(GridK g, const U* __restrict__ x,
-                                                  const B* __restrict__ b, double* partials,
-                                                  unsigned* counter, double* out2,
+                                                  const B* __restrict__ b, double* part,
                                                   double* __restrict__ r_out) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     double v[2] = {0, 0};
-    if (i < g.nx && j < g.ny) {
-        const int64_t id = i + (int64_t)g.nx * (j + (int64_t)g.ny * k);
-        const double ax = apply_at<double, U>(x, g, i, j, k);
-        const double r = (double)b[id] - ax;
+    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
+        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), k = (int)(id / ((int64_t)g.nx * g.ny));
+        const double r = (double)b[id] - apply_at<double, U>(x, g, i, j, k);
         if (r_out) r_out[id] = r;
-        v[0] = r * r;
-        v[1] = (double)b[id] * (double)b[id];
+        v[0] += r * r;
+        v[1] += (double)b[id] * (double)b[id];
     }
+    put_partial<2>(v, part);
+}
+__global__ __launch_bounds__(256) void k_sum2(const double* part, double* out2) {
     double t[2];
-    if (last_block<2>(v, partials, counter, t) && threadIdx.x == 0) {
+    sum_partials<2>(part, kRedBlocks, t);
+    if (threadIdx.x == 0) {
         out2[0] = t[0];
         out2[1] = t[1];
     }
@@ -391,37 +530,41 @@ static nnx_status check_grid(nnx_ctx* c, const nnx_grid* g) {
 
 struct CgHost {
     CgDev d;
+    float *pa, *pb;      // p ping-pong
+    double* tr_part;     // true-residual partials
     double* out2;
-    unsigned* cnt_true;
 };
 
 static nnx_status cg_buffers(nnx_ctx* c, const GridK& gk, int precond, CgHost* h) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
-    void *r, *p, *q, *sc, *fl, *part, *cnt;
+    void *r, *pa, *pb, *q, *sc, *fl, *part;
     NNX_TRY(scratch(c, "cg_r", n * 4, false, &r));
-    NNX_TRY(scratch(c, "cg_p", n * 4, false, &p));
+    NNX_TRY(scratch(c, "cg_pa", n * 4, false, &pa));
+    NNX_TRY(scratch(c, "cg_pb", n * 4, false, &pb));
     NNX_TRY(scratch(c, "cg_q", n * 4, false, &q));
     NNX_TRY(scratch(c, "cg_sc", 64 * 8, true, &sc));
     NNX_TRY(scratch(c, "cg_fl", 64 * 4, true, &fl));
-    const int64_t nb = std::max<int64_t>((int64_t)ceil_div(gk.nx, 64) * ceil_div(gk.ny, 4) * gk.nz, 4096);
-    NNX_TRY(scratch(c, "cg_part", (size_t)nb * 3 * 8, false, &part));
-    NNX_TRY(scratch(c, "cg_cnt", 64, true, &cnt));
+    NNX_TRY(scratch(c, "cg_part", (size_t)kRedBlocks * 16 * 8, false, &part));
+    double* P = (double*)part;
     h->d.r = (float*)r;
-    h->d.p = (float*)p;
     h->d.q = (float*)q;
+    h->pa = (float*)pa;
+    h->pb = (float*)pb;
     h->d.sc = (double*)sc;
     h->d.fl = (int32_t*)fl;
-    h->d.partials = (double*)part;
-    h->d.counter = (unsigned*)cnt;
+    h->d.pu0 = P;                     // 2 x kRedBlocks
+    h->d.pu1 = P + 2 * kRedBlocks;    // 2 x kRedBlocks
+    h->d.ppq = P + 4 * kRedBlocks;    // up to 2048 operator blocks
+    h->d.pb = P + 6 * kRedBlocks;     // kRedBlocks
+    h->tr_part = P + 7 * kRedBlocks;  // 2 x kRedBlocks
+    h->out2 = (double*)sc + 32;
     h->d.dinv = nullptr;
     h->d.dinv_c = 1.0f;
-    h->out2 = (double*)sc + 32;
-    h->cnt_true = (unsigned*)cnt + 8;
     if (precond == NNX_PRECOND_JACOBI) {
         if (gk.kind == 1) {
-            void* dv;
-            NNX_TRY(scratch(c, "cg_dinv", n * 4, false, &dv));
-            h->d.dinv = (float*)dv;
+            void* dvp;
+            NNX_TRY(scratch(c, "cg_dinv", n * 4, false, &dvp));
+            h->d.dinv = (float*)dvp;
             k_dinv<<<flat_blocks(n), 256, 0, c->stream>>>(gk, h->d.dinv);
             NNX_LAUNCHED(c);
         } else {
@@ -431,42 +574,53 @@ static nnx_status cg_buffers(nnx_ctx* c, const GridK& gk, int precond, CgHost* h
     return NNX_OK;
 }
 
-// Runs CG on the prepared buffers; x holds the guess when use_x0.
+static nnx_status cg_readback(nnx_ctx* c, CgHost& h, int32_t* fl, double* sc) {
+    NNX_HIP(c, hipMemcpyAsync(c->pinned, h.d.fl, 4 * FL_N, hipMemcpyDeviceToHost, c->stream));
+    NNX_HIP(c, hipMemcpyAsync((char*)c->pinned + 64, h.d.sc, 8 * SC_N, hipMemcpyDeviceToHost, c->stream));
+    NNX_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(fl, c->pinned, 4 * FL_N);
+    std::memcpy(sc, (char*)c->pinned + 64, 8 * SC_N);
+    return NNX_OK;
+}
+
+// Runs CG on the prepared buffers; x holds the guess when use_x0.  The host
+// enqueues check_every iterations, then one status kernel and a read-back.
 static nnx_status cg_run(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b, float* x,
                          int use_x0, double tol, int maxit, int abs_mode, int check_every,
-                         nnx_cg_result* res) {
+                         nnx_cg_result* res, int* initdone = nullptr) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
-    const unsigned fb = flat_blocks(n);
-    k_cg_init<<<fb, 256, 0, c->stream>>>(gk, h.d, b, x, use_x0, tol, maxit, abs_mode);
+    k_cg_init<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, b, x, use_x0, h.pa);
+    k_cg_init_fin<<<1, 256, 0, c->stream>>>(h.d, tol, abs_mode, maxit);
     NNX_LAUNCHED(c);
-    const dim3 g7((unsigned)ceil_div(gk.nx, 64), (unsigned)ceil_div(gk.ny, 4), (unsigned)ceil_div(gk.nz, kZc));
     if (check_every < 1) check_every = 1;
-    int iters = 0, done = 0;
-    while (!done) {
-        for (int q = 0; q < check_every; ++q) {
-            k_cg_dir<<<fb, 256, 0, c->stream>>>(n, h.d);
-            if (gk.kind == 0)
-                k_cg_spmv7<<<g7, 256, 0, c->stream>>>(gk, h.d);
-            else
-                k_cg_spmv_gen<<<grid3(gk), 256, 0, c->stream>>>(gk, h.d);
-            k_cg_update<<<fb, 256, 0, c->stream>>>(n, h.d, x);
-        }
-        NNX_LAUNCHED(c);
-        NNX_HIP(c, hipMemcpyAsync(c->pinned, h.d.fl, 4 * FL_N, hipMemcpyDeviceToHost, c->stream));
-        NNX_HIP(c, hipStreamSynchronize(c->stream));
-        const int32_t* f = (const int32_t*)c->pinned;
-        done = f[FL_DONE];
-        iters = f[FL_ITERS];
-        if (iters >= maxit) done = 1;
-    }
+    const int vec = ((uintptr_t)x % 16) == 0;
+    // constant 7-point stencil with 16-byte rows: fused LDS float4 kernel
+    const bool fast = gk.kind == 0 && gk.nx % 4 == 0;
+    const int nb4 = stencil4_blocks(gk);
+    h.d.npq = fast ? nb4 : kRedBlocks;
+    int32_t fl[FL_N];
     double sc[SC_N];
-    NNX_HIP(c, hipMemcpyAsync(c->pinned, h.d.sc, 8 * SC_N, hipMemcpyDeviceToHost, c->stream));
-    NNX_HIP(c, hipStreamSynchronize(c->stream));
-    std::memcpy(sc, c->pinned, 8 * SC_N);
-    const int32_t initdone = 0;
-    (void)initdone;
+    NNX_TRY(cg_readback(c, h, fl, sc));
+    if (initdone) *initdone = fl[FL_INITDONE];
+    int enq = 0;
+    while (!fl[FL_DONE] && enq < maxit) {
+        for (int q = 0; q < check_every && enq < maxit; ++q, ++enq) {
+            float* pold = (enq & 1) ? h.pb : h.pa;
+            float* pnew = (enq & 1) ? h.pa : h.pb;
+            if (fast) {
+                k_stencil7x4<true><<<nb4, kT4, 0, c->stream>>>(gk, h.d, enq, nb4, nullptr, pold, pnew, h.d.q);
+            } else {
+                k_cg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, enq, pold, pnew);
+                k_cg_spmv_gen<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, enq, pnew);
+            }
+            k_cg_update<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, enq, pnew, x, vec);
+        }
+        k_cg_status<<<1, 256, 0, c->stream>>>(h.d);
+        NNX_LAUNCHED(c);
+        NNX_TRY(cg_readback(c, h, fl, sc));
+    }
     if (res) {
-        res->iters = iters;
+        res->iters = fl[FL_ITERS];
         const double bb = sc[SC_BB] > 0 ? sc[SC_BB] : 1.0;
         res->relres = abs_mode ? sc[SC_RR] : std::sqrt(sc[SC_RR] / bb);
         res->converged = abs_mode ? (sc[SC_RR] < sc[SC_THR]) : (sc[SC_RR] <= sc[SC_THR]);
@@ -474,15 +628,20 @@ static nnx_status cg_run(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b,
     return NNX_OK;
 }
 
-static nnx_status true_residual(nnx_ctx* c, const GridK& gk, CgHost& h, const float* x,
-                                const float* b, double* rel) {
-    k_true_res<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, b, h.d.partials, h.cnt_true,
-                                                              h.out2, nullptr);
+template <class U, class B>
+static nnx_status true_residual(nnx_ctx* c, const GridK& gk, CgHost& h, const U* x, const B* b,
+                                double* rel, double* rr_out, double* r_out) {
+    k_true_res<U, B><<<kRedBlocks, 256, 0, c->stream>>>(gk, x, b, h.tr_part, r_out);
+    k_sum2<<<1, 256, 0, c->stream>>>(h.tr_part, h.out2);
     NNX_LAUNCHED(c);
     NNX_HIP(c, hipMemcpyAsync(c->pinned, h.out2, 16, hipMemcpyDeviceToHost, c->stream));
     NNX_HIP(c, hipStreamSynchronize(c->stream));
     const double* o = (const double*)c->pinned;
-    *rel = std::sqrt(o[0] / (o[1] > 0 ? o[1] : 1.0));
+    if (rel) *rel = std::sqrt(o[0] / (o[1] > 0 ? o[1] : 1.0));
+    if (rr_out) {
+        rr_out[0] = o[0];
+        rr_out[1] = o[1];
+    }
     return NNX_OK;
 }
 
@@ -560,7 +719,13 @@ nnx_status nnx_stencil_apply(nnx_ctx* c, const nnx_grid* g, const float* x, floa
     NNX_TRY(check_grid(c, g));
     NNX_CHECK(c, x && y && x != y, "x and y must be distinct device arrays");
     const GridK gk = make_grid(g);
-    k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
+    if (gk.kind == 0 && gk.nx % 4 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0) {
+        CgDev none{};
+        k_stencil7x4<false><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, 0, x, nullptr,
+                                                                       nullptr, y);
+    } else {
+        k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
+    }
     NNX_LAUNCHED(c);
     return NNX_OK;
 }
@@ -581,7 +746,7 @@ nnx_status nnx_cg_solve(nnx_ctx* c, const nnx_grid* g, const float* b, float* x,
     nnx_cg_result r{};
     NNX_TRY(cg_run(c, gk, h, b, x, o->use_x0, o->rtol, o->maxit, 0, o->check_every, &r));
     auto t1 = std::chrono::steady_clock::now();
-    NNX_TRY(true_residual(c, gk, h, x, b, &r.true_relres));
+    NNX_TRY((true_residual<float, float>(c, gk, h, x, b, &r.true_relres, nullptr, nullptr)));
     r.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     r.precond_used = o->precond;
     if (res) *res = r;
@@ -608,12 +773,9 @@ nnx_status nnx_cg_solve_refine(nnx_ctx* c, const nnx_grid* g, const double* b, d
     int outer = 0, inner_total = 0;
     double rel = 0;
     for (;;) {
-        k_true_res<double, double><<<grid3(gk), 256, 0, c->stream>>>(gk, x, b, h.d.partials, h.cnt_true,
-                                                                    h.out2, (double*)r64);
-        NNX_LAUNCHED(c);
-        NNX_HIP(c, hipMemcpyAsync(c->pinned, h.out2, 16, hipMemcpyDeviceToHost, c->stream));
-        NNX_HIP(c, hipStreamSynchronize(c->stream));
-        const double rr = ((const double*)c->pinned)[0], bb = ((const double*)c->pinned)[1];
+        double rb[2];
+        NNX_TRY((true_residual<double, double>(c, gk, h, x, b, nullptr, rb, (double*)r64)));
+        const double rr = rb[0], bb = rb[1];
         rel = std::sqrt(rr / (bb > 0 ? bb : 1.0));
         if (rel <= o->rtol || outer >= max_outer || rr == 0) break;
         // inner solve on the residual scaled to unit norm
@@ -661,10 +823,8 @@ nnx_status nnx_update_density_reference(nnx_ctx* c, int nx, int ny, int nz, doub
                                                  (float*)xg, (float*)bb);
     NNX_LAUNCHED(c);
     nnx_cg_result r{};
-    NNX_TRY(cg_run(c, gk, hh, (const float*)bb, (float*)xg, 1, eb, max_iter, 1, 4, &r));
-    NNX_HIP(c, hipMemcpyAsync(c->pinned, hh.d.fl, 4 * FL_N, hipMemcpyDeviceToHost, c->stream));
-    NNX_HIP(c, hipStreamSynchronize(c->stream));
-    const int initdone = ((const int32_t*)c->pinned)[FL_INITDONE];
+    int initdone = 0;
+    NNX_TRY(cg_run(c, gk, hh, (const float*)bb, (float*)xg, 1, eb, max_iter, 1, 4, &r, &initdone));
     // src/paracrine.cu:114-117: an initial residual below the bound returns
     // without writing the explicit guess; otherwise data = the iterate.
     if (!initdone) NNX_HIP(c, hipMemcpyAsync(data, xg, n * 4, hipMemcpyDeviceToDevice, c->stream));

@@ -64,7 +64,10 @@ def params(d) -> Params:
 
 
 def grid(nx, ny, nz, stencil=7, c=1.0, decay_dt=0.0, kx=None, ky=None, kz=None) -> Grid:
-    return Grid(nx, ny, nz, stencil, c, decay_dt, _p(kx), _p(ky), _p(kz))
+    ks = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (kx, ky, kz)]
+    g = Grid(nx, ny, nz, stencil, c, decay_dt, *(_p(a) for a in ks))
+    g._keep = ks  # the structure holds raw pointers: keep the arrays alive
+    return g
 
 
 def _sfx(dtype):

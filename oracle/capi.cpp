@@ -32,6 +32,7 @@ struct orc_grid {
 template <class T> static P<T> cast(const orc_params* q) {
     P<T> p;
     p.C = T(q->C);
+    p.invC = T(1.0 / q->C);
     p.gna = T(q->gna);
     p.gk = T(q->gk);
     p.gl = T(q->gl);

@@ -28,7 +28,7 @@ namespace oracle {
 
 // --------------------------------------------------------------------------
 // exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, Taylor polynomial of
-// e^r by Horner with fma, scale by 2^k through the exponent field.
+// e^r by Horner with fma, scaled by 2^k with ldexp (exact).
 // ln2 is split so that k*ln2_hi is exact (16 bits for float, 42 for double).
 // The argument is clamped to [xmin, xmax] so the result stays normal.
 // --------------------------------------------------------------------------
@@ -58,19 +58,6 @@ template <> struct ExpK<double> {
                                      0x1.6124613a86d09p-33};
 };
 
-inline float pow2i(float, int k) {
-    uint32_t b = (uint32_t)(k + 127) << 23;
-    float f;
-    std::memcpy(&f, &b, 4);
-    return f;
-}
-inline double pow2i(double, int k) {
-    uint64_t b = (uint64_t)(k + 1023) << 52;
-    double f;
-    std::memcpy(&f, &b, 8);
-    return f;
-}
-
 template <class T> inline T oexp(T x) {
     using K = ExpK<T>;
     if (x != x) return x;
@@ -81,44 +68,52 @@ template <class T> inline T oexp(T x) {
     r = std::fma(-k, K::ln2_lo, r);
     T p = K::c[K::deg];
     for (int i = K::deg - 1; i >= 0; --i) p = std::fma(p, r, K::c[i]);
-    return p * pow2i(T(0), (int)k);
+    return std::ldexp(p, (int)k);  // exact: the result is normal
 }
 
-// a / (1 - e^-a), with the Bernoulli series 1 + a/2 + a^2/12 - a^4/720 +
-// a^6/30240 - a^8/1209600 for |a| < 1/2 (removes the 0/0 of the reference's
-// alpha_m and alpha_n at V = -35 and V = -50, src/ephys.cu:57-58).
-template <class T> inline T vtrap(T a) {
-    if (std::fabs(a) < T(0.5)) {
-        T a2 = a * a;
-        T q = std::fma(a2, T(-1.0 / 1209600.0), T(1.0 / 30240.0));
-        q = std::fma(a2, q, T(-1.0 / 720.0));
-        q = std::fma(a2, q, T(1.0 / 12.0));
-        return std::fma(a2, q, std::fma(a, T(0.5), T(1)));
-    }
-    return a / (T(1) - oexp(-a));
+// a / (1 - e^-a) given ea = e^-a.  For |a| < 1/2 the Bernoulli series
+// 1 + a/2 + a^2/12 - a^4/720 + a^6/30240 - a^8/1209600 is selected instead
+// (removes the 0/0 of the reference's alpha_m and alpha_n at V = -35 and
+// V = -50, src/ephys.cu:57-58); both branches are evaluated.
+template <class T> inline T vtrap(T a, T ea) {
+    const T a2 = a * a;
+    T q = std::fma(a2, T(-1.0 / 1209600.0), T(1.0 / 30240.0));
+    q = std::fma(a2, q, T(-1.0 / 720.0));
+    q = std::fma(a2, q, T(1.0 / 12.0));
+    const T ser = std::fma(a2, q, std::fma(a, T(0.5), T(1)));
+    const T dir = a / (T(1) - ea);
+    return std::fabs(a) < T(0.5) ? ser : dir;
 }
 
 template <class T> struct Rates { T am, bm, ah, bh, an, bn; };
 
 // The reference's six HH rates (src/ephys.cu:57-63), -60 mV-rest form,
-// evaluated at u = V + v_shift.
+// evaluated at u = V + v_shift with three exponentials:
+//   a = 3.5 + 0.1u, ea = e^-a:  am = a/(1-e^-a), an = 0.1 b/(1-e^-b) with
+//   b = a + 1.5 and e^-b = ea e^-1.5, bh = 1/(e^(-3-0.1u) + 1) = 1/(ea e^0.5 + 1);
+//   w = u + 60:  bn = 0.125 e^(-w/80), ah = 0.07 e^(-u/20-3) = 0.07 (e^(-w/80))^4,
+//   bm = 4 e^(-w/18).
 template <class T> inline Rates<T> rates(T u) {
     Rates<T> R;
-    T a = T(3.5) + T(0.1) * u;
-    R.am = vtrap(a);
-    T b = T(5) + T(0.1) * u;
-    R.an = T(0.1) * vtrap(b);
-    R.ah = T(0.07) * oexp(-u / T(20) - T(3));
-    R.bm = T(4) * oexp(-(u + T(60)) / T(18));
-    R.bn = T(0.125) * oexp(-(u + T(60)) / T(80));
-    R.bh = T(1) / (oexp(T(-3) - T(0.1) * u) + T(1));
+    const T a = T(3.5) + T(0.1) * u;
+    const T ea = oexp(-a);
+    R.am = vtrap(a, ea);
+    const T b = a + T(1.5);
+    R.an = T(0.1) * vtrap(b, ea * T(0x1.c8f87724b5c1dp-3));
+    R.bh = T(1) / (ea * T(0x1.a61298e1e069cp+0) + T(1));
+    const T w = u + T(60);
+    const T e80 = oexp(w * T(-0.0125));
+    const T e40 = e80 * e80;
+    R.bn = T(0.125) * e80;
+    R.ah = T(0.07) * (e40 * e40);
+    R.bm = T(4) * oexp(w * T(-1.0 / 18.0));
     return R;
 }
 
 // Parameters cast to T once (the host passes doubles; both sides cast the
 // same doubles, so both see the same T values).
 template <class T> struct P {
-    T C, gna, gk, gl, ena, ek, el, vshift, esyn;
+    T C, invC, gna, gk, gl, ena, ek, el, vshift, esyn;
     T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean;
     T v_thresh, out_thresh;
 };
@@ -137,7 +132,7 @@ inline void hh_deriv(const P<T>& p, T V, T m, T h, T n, T s, T iext, T& dV, T& d
     T ik = p.gk * n4 * (V - p.ek);
     T il = p.gl * (V - p.el);
     T isyn = s * (V - p.esyn);
-    dV = (iext - ina - ik - il - isyn) / p.C;
+    dV = (iext - ina - ik - il - isyn) * p.invC;
 }
 
 // Classical RK4 over (V, m, h, n); s is integrated exactly: s0 = s + g_in at

@@ -204,9 +204,13 @@ def _grid_dev(ctx, nx, ny, nz, stencil, c, decay=0.0, var=False, seed=0):
     return g, og, keep
 
 
-@pytest.mark.parametrize("kind", ["7", "7var", "27"])
-def test_stencil_apply_bitexact(ctx, kind):
-    nx, ny, nz = 67, 23, 11
+@pytest.mark.parametrize("kind,dims", [("7", (67, 23, 11)), ("7", (300, 13, 21)),
+                                       ("7", (256, 16, 32)), ("7var", (67, 23, 11)),
+                                       ("27", (67, 23, 11))])
+def test_stencil_apply_bitexact(ctx, kind, dims):
+    # (300, 13, 21) and (256, 16, 32) take the float4 LDS kernel (nx % 4 == 0),
+    # with partial tiles in x, y and z for the first
+    nx, ny, nz = dims
     g, og, keep = _grid_dev(ctx, nx, ny, nz, 27 if kind == "27" else 7, 1.3, 0.2, kind == "7var")
     x = np.random.default_rng(6).standard_normal(nx * ny * nz).astype(np.float32)
     dx, dy = DA.from_numpy(ctx, x), DA.zeros(ctx, x.size, np.float32)
