@@ -109,6 +109,47 @@ def cg_bench(ctx, grid_n, b_host, rtol=1e-8):
             "status": int(st)}
 
 
+def stencil_roofline(ctx, n=256, reps=50):
+    """y = A x on an n^3 grid (constant 7-point): 8 algorithmic bytes per
+    voxel (read x, write y), HIP events on the library stream."""
+    from neuronephys_clean_amd import _lib
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+    g = _lib.Grid(n, n, n, 7, 1.0, 0.0, None, None, None)
+    x = DA.from_numpy(ctx, np.random.default_rng(0).random(n ** 3).astype(np.float32))
+    y = DA.zeros(ctx, n ** 3, np.float32)
+
+    def f():
+        ctx.call("nnx_stencil_apply", C.byref(g), x.ptr, y.ptr)
+    ms = timed_events(ctx, f, reps)
+    b = 8.0 * n ** 3
+    return {"kernel": "k_stencil7x4<false> (y = A x, 7-point, fp32)", "grid": f"{n}^3",
+            "bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": b / (ms * 1e-3) / HBM_PEAK, "avg_launch_us": ms * 1e3,
+            "alg_bytes_per_launch": b}
+
+
+def profile_pass(ctx, sim, net):
+    from neuronephys_clean_amd import _lib
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+    sim.run(20)  # eager: one dispatch per kernel per step
+    sim.synchronize()
+    stencil_roofline(ctx, 256, 20)
+    vc = sim.a["voxel_counts"].numpy().astype(np.float32)
+    b = 0.01 * vc
+    if not b.any():
+        b[len(b) // 2] = 1.0
+    g = _lib.Grid(256, 256, 256, 7, 1.0, 0.0, None, None, None)
+    db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, b.size, np.float32)
+    o = _lib.CgOpts(rtol=1e-8, maxit=2000, precond=_lib.PRECOND_JACOBI, check_every=8)
+    r = _lib.CgResult()
+    ctx._l.nnx_cg_solve(ctx.h, C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(r))
+    src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
+    ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
+    ctx.synchronize()
+    log("profile pass done: 20 steps, 20 stencil applies, 1 CG solve "
+        f"({r.iters} iterations), 256 MiB copy")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,7 +158,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--couple", default="event", choices=["event", "pull"])
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=1500)
+    ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
+                    help="profile: a short eager sequence for rocprofv3 --pmc passes (20 steps, "
+                         "20 stencil applies and one CG solve on the 256^3 grid, one 256 MiB "
+                         "device copy to calibrate FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     args = ap.parse_args()
@@ -140,6 +185,10 @@ def main():
                      ctx=ctx, device_csc=True)
     sim.synchronize()
     log(f"bench: uploaded + device CSC in {time.time() - t0:.1f}s")
+
+    if args.mode == "profile":
+        profile_pass(ctx, sim, net)
+        return
 
     sim.run(args.warmup, use_graph=True, chunk=args.chunk)
     sim.synchronize()
@@ -186,6 +235,9 @@ def main():
                  sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
     couple_ms = timed_events(ctx, couple, 50)
 
+    roof_stencil = stencil_roofline(ctx)
+    log(f"bench: stencil {roof_stencil}")
+
     cg = None
     if not args.no_cg:
         vc = sim.a["voxel_counts"].numpy().astype(np.float32)
@@ -210,7 +262,7 @@ def main():
                       "n": net.n, "nnz": net.nnz, "dt": net.dt, "couple": args.couple,
                       "graph_chunk": args.chunk, "spikes_timed": int(spikes),
                       "spikes_per_step": spikes / args.steps},
-           "roofline": roof, "cpu_baseline": cpu, "cg": cg,
+           "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},
            "event_ms": ev_ms.value}

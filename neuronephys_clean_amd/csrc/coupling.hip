@@ -11,13 +11,11 @@
 //   - every target's input is summed over its spiking sources in ascending
 //     source order, so all modes are deterministic and WEIGHTED_PULL and
 //     WEIGHTED_EVENT are bit-identical.
+#include "coupling_dev.h"
 #include "internal.h"
 
 namespace nnx {
 
-__device__ __forceinline__ bool bit_of(const uint64_t* bits, int32_t s) {
-    return (bits[s >> 6] >> (s & 63)) & 1ull;
-}
 
 // ------------------------------------------------------------------------
 // level rule (CheckFiringOperator, src/coupling.cu:103-109)
@@ -38,12 +36,19 @@ __global__ __launch_bounds__(256) void k_threshold(int64_t n, const T* __restric
 // Optionally appends (step, neuron) raster pairs and counts releases per
 // voxel (integer atomics: order-independent).
 // ------------------------------------------------------------------------
-constexpr int kCompactThreads = 1024;
-constexpr int kWordsPerThread = 4;
-constexpr int kTileWords = kCompactThreads * kWordsPerThread;
+// Multi-block compaction: block b owns words [1024b, 1024b + 1024), four
+// consecutive words per thread (65,536 neurons per block).  Its output offset
+// is the popcount of every preceding word, summed by the block itself
+// (integers: exact, no inter-block communication; 16-byte loads, four
+// independent accumulators); a block scan orders its own words.  The last
+// block finalises the count, the raster length and the step counter.  Raster
+// length and step live in parity pairs: the launch of parity s reads slot s
+// and its last block writes slot s^1, so no block reads a value that another
+// block of the same launch updates.
+constexpr int kCB = 256, kCW = 4, kCBW = kCB * kCW;
 
-
-__device__ __forceinline__ int block_exclusive_scan(int v, int* sh_wave, int* total) {
+__device__ __forceinline__ int block_scan256(int v, int* total) {
+    __shared__ int sh[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x = v;
 #pragma unroll
@@ -51,52 +56,63 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* sh_wave, int* to
         const int y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
-    if (lane == 63) sh_wave[wave] = x;
     __syncthreads();
-    if (wave == 0) {
-        int s = lane < kCompactThreads / 64 ? sh_wave[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(s, o, 64);
-            if (lane >= o) s += y;
-        }
-        if (lane < kCompactThreads / 64) sh_wave[lane] = s;  // inclusive per wave
-    }
+    if (lane == 63) sh[wave] = x;
     __syncthreads();
-    const int before = wave ? sh_wave[wave - 1] : 0;
-    *total = sh_wave[kCompactThreads / 64 - 1];
-    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += sh[w];
+    *total = sh[0] + sh[1] + sh[2] + sh[3];
     return before + x - v;
 }
 
-__global__ __launch_bounds__(kCompactThreads) void k_compact(int64_t nwords,
-                                                             const uint64_t* __restrict__ bits,
-                                                             int32_t* __restrict__ list,
-                                                             int32_t* __restrict__ count,
-                                                             CompactExtras ex) {
-    __shared__ uint64_t tile[kTileWords];
-    __shared__ int sh_wave[kCompactThreads / 64];
-    int64_t base = 0;
-    int64_t rbase = ex.raster ? *ex.raster_len : 0;
-    const int32_t step = ex.step ? (int32_t)*ex.step : 0;
-    for (int64_t t0 = 0; t0 < nwords; t0 += kTileWords) {
-        for (int q = threadIdx.x; q < kTileWords; q += kCompactThreads)
-            tile[q] = (t0 + q < nwords) ? bits[t0 + q] : 0ull;
-        __syncthreads();
-        int cnt = 0;
+__global__ __launch_bounds__(kCB) void k_compact(int64_t nwords, const uint64_t* __restrict__ bits,
+                                                 int32_t* __restrict__ list,
+                                                 int32_t* __restrict__ count, CompactExtras ex) {
+    const int64_t b0 = blockIdx.x * (int64_t)kCBW;  // first word of this block (< nwords)
+    const bool a16 = ((uintptr_t)bits & 15) == 0;
+    int p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    if (a16) {
+        // b0 is a multiple of 1024 words: pairs of words are 16-byte aligned
+        const ulonglong2* B2 = reinterpret_cast<const ulonglong2*>(bits);
+        const int64_t n2 = b0 / 2;
+        int64_t q = threadIdx.x;
+        for (; q + 3 * kCB < n2; q += 4 * kCB) {
+            const ulonglong2 u0 = B2[q], u1 = B2[q + kCB], u2 = B2[q + 2 * kCB], u3 = B2[q + 3 * kCB];
+            p0 += __popcll(u0.x) + __popcll(u0.y);
+            p1 += __popcll(u1.x) + __popcll(u1.y);
+            p2 += __popcll(u2.x) + __popcll(u2.y);
+            p3 += __popcll(u3.x) + __popcll(u3.y);
+        }
+        for (; q < n2; q += kCB) {
+            const ulonglong2 u = B2[q];
+            p0 += __popcll(u.x) + __popcll(u.y);
+        }
+    } else {
+        for (int64_t q = threadIdx.x; q < b0; q += kCB) p0 += __popcll(bits[q]);
+    }
+    int pre_tot;
+    block_scan256((p0 + p1) + (p2 + p3), &pre_tot);
+    const int64_t w0 = b0 + (int64_t)threadIdx.x * kCW;
+    uint64_t w[kCW];
 #pragma unroll
-        for (int q = 0; q < kWordsPerThread; ++q)
-            cnt += __popcll(tile[threadIdx.x * kWordsPerThread + q]);
-        int tot;
-        const int off = block_exclusive_scan(cnt, sh_wave, &tot);
-        int64_t o = base + off;
-        for (int q = 0; q < kWordsPerThread; ++q) {
-            uint64_t w = tile[threadIdx.x * kWordsPerThread + q];
-            const int64_t word = t0 + threadIdx.x * kWordsPerThread + q;
-            while (w) {
-                const int b = __ffsll((unsigned long long)w) - 1;
-                w &= w - 1;
-                const int32_t idx = (int32_t)(word * 64 + b);
+    for (int q = 0; q < kCW; ++q) w[q] = w0 + q < nwords ? bits[w0 + q] : 0ull;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < kCW; ++q) cnt += __popcll(w[q]);
+    int tot;
+    const int off = block_scan256(cnt, &tot);
+    const int par = ex.parity;
+    const int64_t rbase = ex.raster ? ex.len2[par] : 0;
+    const int32_t step = ex.step2 ? (int32_t)ex.step2[par] : 0;
+    int64_t o = (int64_t)pre_tot + off;
+    if (cnt) {
+#pragma unroll
+        for (int q = 0; q < kCW; ++q) {
+            uint64_t x = w[q];
+            while (x) {
+                const int b = __ffsll((unsigned long long)x) - 1;
+                x &= x - 1;
+                const int32_t idx = (int32_t)((w0 + q) * 64 + b);
                 list[o] = idx;
                 if (ex.raster) {
                     const int64_t r = rbase + o;
@@ -109,14 +125,12 @@ __global__ __launch_bounds__(kCompactThreads) void k_compact(int64_t nwords,
                 ++o;
             }
         }
-        base += tot;
-        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *count = (int32_t)base;
-        if (ex.raster) *ex.raster_len = rbase + base;
-        if (ex.step) *ex.step = step + 1;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const int64_t total = (int64_t)pre_tot + tot;
+        *count = (int32_t)total;
+        if (ex.raster) ex.len2[par ^ 1] = rbase + total;
+        if (ex.step2) ex.step2[par ^ 1] = step + 1;
     }
 }
 
@@ -173,10 +187,15 @@ __global__ __launch_bounds__(256) void k_push_event(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ out_to_in,
                                                     const int32_t* __restrict__ in_off,
                                                     int32_t* __restrict__ cnt,
-                                                    int32_t* __restrict__ slot) {
+                                                    int32_t* __restrict__ slot,
+                                                    int32_t burst, int32_t* __restrict__ dense) {
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = (int64_t)gridDim.x * 4;
     const int32_t K = *count;
+    // a burst step skips the push: the ephys kernel pulls every target
+    const bool is_burst = burst >= 0 && K > burst;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dense = is_burst;
+    if (is_burst) return;
     for (int64_t k = blockIdx.x * 4ll + (threadIdx.x >> 6); k < K; k += nwaves) {
         const int32_t j = list[k];
         const int32_t e1 = out_off[j + 1];
@@ -188,37 +207,6 @@ __global__ __launch_bounds__(256) void k_push_event(const int32_t* __restrict__ 
             if (q < in_off[t + 1] - b) slot[b + q] = pos;  // always true for a consistent CSR/CSC
         }
     }
-}
-
-constexpr int kSortMax = 8;
-
-template <class T>
-__device__ __forceinline__ T gather_one(int32_t t, int32_t c, const int32_t* __restrict__ in_off,
-                                        const int32_t* __restrict__ in_src,
-                                        const float* __restrict__ in_w,
-                                        const uint64_t* __restrict__ bits,
-                                        const int32_t* __restrict__ slot) {
-    T acc = T(0);
-    if (c == 0) return acc;
-    const int32_t b = in_off[t];
-    if (c <= kSortMax && c <= in_off[t + 1] - b) {
-        // ascending order by repeated minimum (positions are distinct)
-        int32_t last = -1;
-        for (int r = 0; r < c; ++r) {
-            int32_t mn = 0x7fffffff;
-            for (int q = 0; q < c; ++q) {
-                const int32_t p = slot[b + q];
-                mn = (p > last && p < mn) ? p : mn;
-            }
-            acc = acc + T(in_w[mn]);
-            last = mn;
-        }
-    } else {
-        const int32_t e1 = in_off[t + 1];
-        for (int32_t e = b; e < e1; ++e)
-            if (bit_of(bits, in_src[e])) acc = acc + T(in_w[e]);
-    }
-    return acc;
 }
 
 template <class T>
@@ -245,7 +233,8 @@ namespace nnx {
 nnx_status compact_launch(nnx_ctx* c, int64_t n, const uint64_t* bits, int32_t* list,
                           int32_t* count, const CompactExtras& ex) {
     const int64_t nwords = ceil_div(n, 64);
-    k_compact<<<1, kCompactThreads, 0, c->stream>>>(nwords, bits, list, count, ex);
+    k_compact<<<(unsigned)std::max<int64_t>(1, ceil_div(nwords, kCBW)), kCB, 0, c->stream>>>(
+        nwords, bits, list, count, ex);
     NNX_LAUNCHED(c);
     return NNX_OK;
 }
@@ -254,8 +243,33 @@ nnx_status event_scratch(nnx_ctx* c, int64_t n, int64_t nnz, int32_t** cnt, int3
     void *a, *b;
     NNX_TRY(scratch(c, "event_cnt", (size_t)(n + 1) * 4, true, &a));
     NNX_TRY(scratch(c, "event_slot", (size_t)(nnz + 1) * 4, false, &b));
+    void* dn;
+    NNX_TRY(scratch(c, "event_dense", 256, true, &dn));  // burst flag
     *cnt = (int32_t*)a;
     *slot = (int32_t*)b;
+    return NNX_OK;
+}
+
+nnx_status push_event_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list,
+                             const int32_t* count, EventIn* ev, const uint64_t* prev_bits) {
+    int32_t *cnt, *slot;
+    NNX_TRY(event_scratch(c, g->n, g->nnz, &cnt, &slot));
+    void* dn;
+    NNX_TRY(scratch(c, "event_dense", 256, true, &dn));
+    // above n/16 spikes a dense pull (~8 B per edge) beats pushing each
+    // active edge through an atomic slot reservation
+    const int32_t burst = (int32_t)std::max<int64_t>(1024, g->n / 16);
+    k_push_event<<<2048, 256, 0, c->stream>>>(list, count, g->out_offsets, g->out_targets,
+                                              g->out_to_in, g->in_offsets, cnt, slot, burst,
+                                              (int32_t*)dn);
+    NNX_LAUNCHED(c);
+    ev->dense = (const int32_t*)dn;
+    ev->cnt = cnt;
+    ev->slot = slot;
+    ev->in_off = g->in_offsets;
+    ev->in_src = g->in_sources;
+    ev->in_w = g->in_weights;
+    ev->prev_bits = prev_bits;
     return NNX_OK;
 }
 
@@ -267,8 +281,11 @@ nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, co
     if (mode == NNX_COUPLE_WEIGHTED_EVENT) {
         int32_t *cnt, *slot;
         NNX_TRY(event_scratch(c, n, g->nnz, &cnt, &slot));
+        void* dn;
+        NNX_TRY(scratch(c, "event_dense", 256, true, &dn));
         k_push_event<<<2048, 256, 0, c->stream>>>(list, count, g->out_offsets, g->out_targets,
-                                                  g->out_to_in, g->in_offsets, cnt, slot);
+                                                  g->out_to_in, g->in_offsets, cnt, slot, -1,
+                                                  (int32_t*)dn);
         NNX_LAUNCHED(c);
         const unsigned blocks = (unsigned)ceil_div(n, 256);
         if (dtype == NNX_F32)
@@ -379,6 +396,33 @@ __global__ void k_indeg(int64_t nnz, const int32_t* __restrict__ tgt, int32_t* _
 }
 
 constexpr int kScanThreads = 1024, kScanItems = 4, kScanTile = kScanThreads * kScanItems;
+
+// exclusive scan over a 1024-thread block (16 waves), total in *total
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh_wave, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh_wave[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        int s = lane < kScanThreads / 64 ? sh_wave[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
+        }
+        if (lane < kScanThreads / 64) sh_wave[lane] = s;  // inclusive per wave
+    }
+    __syncthreads();
+    const int before = wave ? sh_wave[wave - 1] : 0;
+    *total = sh_wave[kScanThreads / 64 - 1];
+    __syncthreads();
+    return before + x - v;
+}
 
 // per-tile sums
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(int64_t n, const int32_t* __restrict__ in,

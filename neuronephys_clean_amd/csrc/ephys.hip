@@ -6,24 +6,52 @@
 // written from a ballot (no atomics).  The reference launches 32 blocks of 64
 // threads with a grid-stride loop and synchronises the device afterwards; here
 // the grid covers N (4096 blocks of 256 at N = 1M) and nothing synchronises.
+#include "coupling_dev.h"
 #include "internal.h"
 
 namespace nnx {
 
 constexpr int kBlock = 256;
 
-template <class T>
+// EVENT: the coupling input is the target's pending event-driven sum.  A
+// target with at most kSortMax bucket entries sums them itself (gather_one);
+// heavier targets - and every target of a burst step, where the push was
+// skipped - are pulled by the whole wavefront cooperatively over their
+// in-edges (wave_pull_one).  Both give the ascending-source-order sum.
+// Otherwise the input is read from gin (NULL = none).
+template <class T, bool EVENT>
 __global__ __launch_bounds__(kBlock) void k_hh_rk4(int64_t n, HHConst<T> p, T* __restrict__ V,
                                                    T* __restrict__ M, T* __restrict__ H,
                                                    T* __restrict__ NG, T* __restrict__ S,
                                                    const T* __restrict__ iext,
-                                                   const T* __restrict__ gin,
+                                                   const T* __restrict__ gin, EventIn ev,
                                                    uint64_t* __restrict__ bits, int64_t nwords) {
     const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    T g = T(0);
+    if (EVENT) {
+        const bool dense = *ev.dense != 0;
+        const int32_t c = (i < n && !dense) ? ev.cnt[i] : 0;
+        bool heavy = i < n && (dense || c > kSortMax);
+        if (!heavy && i < n && c > 0) {
+            const int32_t b = ev.in_off[i];
+            if (c <= ev.in_off[i + 1] - b) g = gather_one<T>((int32_t)i, c, ev.in_off, ev.in_src, ev.in_w, ev.prev_bits, ev.slot);
+            else heavy = true;
+        }
+        uint64_t hm = __ballot(heavy);
+        const int64_t wave0 = i - (threadIdx.x & 63);
+        while (hm) {
+            const int l = __ffsll((unsigned long long)hm) - 1;
+            hm &= hm - 1;
+            const T acc = wave_pull_one<T>(wave0 + l, ev.in_off, ev.in_src, ev.in_w, ev.prev_bits);
+            if ((threadIdx.x & 63) == l) g = acc;
+        }
+        if (c) ev.cnt[i] = 0;
+    } else if (gin && i < n) {
+        g = gin[i];
+    }
     bool spike = false;
     if (i < n) {
         T v = V[i], m = M[i], h = H[i], ng = NG[i], s = S[i];
-        const T g = gin ? gin[i] : T(0);
         const T v0 = v;
         rk4(p, v, m, h, ng, s, iext[i], g);
         V[i] = v;
@@ -105,9 +133,10 @@ static nnx_status hh_step_t(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_par
         NNX_CHECK(c, st->v && st->m && st->h && st->ng && st->s && st->i_ext,
                   "RK4 needs v, m, h, ng, s, i_ext");
         NNX_CHECK(c, q->tau_syn > 0, "tau_syn must be > 0");
-        k_hh_rk4<T><<<blocks, kBlock, 0, c->stream>>>(n, p, (T*)st->v, (T*)st->m, (T*)st->h,
-                                                      (T*)st->ng, (T*)st->s, (const T*)st->i_ext,
-                                                      (const T*)g_in, bits, nwords);
+        EventIn none{};
+        k_hh_rk4<T, false><<<blocks, kBlock, 0, c->stream>>>(n, p, (T*)st->v, (T*)st->m, (T*)st->h,
+                                                             (T*)st->ng, (T*)st->s, (const T*)st->i_ext,
+                                                             (const T*)g_in, none, bits, nwords);
     } else {
         NNX_CHECK(c, st->v && st->m && st->h && st->ng, "leapfrog needs v, m, h, ng");
         LeapPtr<T> a;
@@ -141,6 +170,26 @@ template <class T> __global__ void k_finite(int64_t n, const T* x, int32_t* flag
         const T v = x[i];
         if (!(v == v) || v - v != T(0)) flag[0] = 1;
     }
+}
+
+nnx_status hh_rk4_event_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q,
+                               double dt, const EventIn& ev, uint64_t* bits) {
+    const int64_t n = st->n;
+    const int64_t nwords = ceil_div(n, 64);
+    const unsigned blocks = (unsigned)ceil_div(n, kBlock);
+    if (st->dtype == NNX_F32) {
+        HHConst<float> p = make_const<float>(q, dt);
+        k_hh_rk4<float, true><<<blocks, kBlock, 0, c->stream>>>(
+            n, p, (float*)st->v, (float*)st->m, (float*)st->h, (float*)st->ng, (float*)st->s,
+            (const float*)st->i_ext, nullptr, ev, bits, nwords);
+    } else {
+        HHConst<double> p = make_const<double>(q, dt);
+        k_hh_rk4<double, true><<<blocks, kBlock, 0, c->stream>>>(
+            n, p, (double*)st->v, (double*)st->m, (double*)st->h, (double*)st->ng, (double*)st->s,
+            (const double*)st->i_ext, nullptr, ev, bits, nwords);
+    }
+    NNX_LAUNCHED(c);
+    return NNX_OK;
 }
 
 }  // namespace nnx

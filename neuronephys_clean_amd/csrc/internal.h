@@ -80,9 +80,10 @@ nnx_status fill_i32(nnx_ctx* c, int32_t* p, int64_t n, int32_t v);
 // optional work folded into the spike compaction
 struct CompactExtras {
     int32_t* raster;      // (step, neuron) pairs
-    int64_t* raster_len;  // device count of pairs written so far
+    int64_t* len2;        // raster length, parity pair (read [parity], write [parity^1])
     int64_t raster_cap;
-    int64_t* step;        // device step counter (raster step numbers); advanced by one per call
+    int64_t* step2;       // step number, parity pair (raster step numbers)
+    int32_t parity;
     const int32_t* voxel_of;
     int32_t* voxel_counts;  // +1 per spike at voxel_of[neuron]
 };
@@ -93,6 +94,21 @@ nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, co
                          const int32_t* list, const int32_t* count, const void* outputs,
                          void* g_in);
 nnx_status check_graph(nnx_ctx* c, const nnx_graph* g, int mode);
+
+// event-driven coupling folded into the ephys step
+struct EventIn {
+    int32_t* cnt;               // pending contributions per target (cleared by the step)
+    const int32_t* slot;        // bucket entries: CSC positions
+    const int32_t* in_off;
+    const int32_t* in_src;
+    const float* in_w;
+    const uint64_t* prev_bits;  // spikes of the previous step (dense fallback)
+    const int32_t* dense;       // set by the push when the step is a burst: pull every target
+};
+nnx_status push_event_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list,
+                             const int32_t* count, EventIn* ev, const uint64_t* prev_bits);
+nnx_status hh_rk4_event_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q,
+                               double dt, const EventIn& ev, uint64_t* bits);
 nnx_status event_scratch(nnx_ctx* c, int64_t n, int64_t nnz, int32_t** cnt, int32_t** slot);
 
 }  // namespace nnx

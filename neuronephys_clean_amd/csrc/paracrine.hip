@@ -321,7 +321,12 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
     const float cdiag = g.diag, cc = g.c;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     double v[1] = {0};
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // workgroups go to the 8 XCDs round-robin (block b on XCD b % 8): give
+    // each XCD a contiguous run of tiles so neighbouring tiles' halo rows and
+    // planes are read through the same L2
+    const int nb = gridDim.x;
+    const int lb = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+    for (int tile = lb; tile < ntiles; tile += nb) {
         const int bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
         const int x0 = bx * kTX4, k0 = bz * kZ4, kend = min(k0 + kZ4, nz);
         const int j = by * kTY4 + ly;

@@ -13,7 +13,15 @@
 
 namespace nnx {
 
-__global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
+// run bookkeeping, outside the captured steps: the parity pairs of the
+// raster length and step counter are seeded from the caller's raster length
+// and the first step, and the caller's raster length is updated at the end
+__global__ void k_run_begin(int64_t* len2, int64_t* step2, const int64_t* raster_len, int par,
+                            int64_t first) {
+    len2[par] = raster_len ? *raster_len : 0;
+    step2[par] = first;
+}
+__global__ void k_run_end(const int64_t* len2, int64_t* raster_len, int par) { *raster_len = len2[par]; }
 
 static uint64_t plan_hash(const nnx_step_plan* p) {
     const unsigned char* b = (const unsigned char*)p;
@@ -46,22 +54,31 @@ static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     return NNX_OK;
 }
 
-static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, int64_t* step_ctr) {
+static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, int64_t* ctr) {
     const uint64_t* prev = p->bits[(s + 1) & 1];
     uint64_t* cur = p->bits[s & 1];
     const bool rk4 = p->integrator == NNX_INT_RK4;
-    void* target = rk4 ? p->g_in : (void*)p->state.in_local;
-    NNX_TRY(couple_launch(c, &p->graph, p->couple_mode, p->state.dtype, prev, p->list, p->count,
-                          p->state.out_local, target));
-    NNX_TRY(nnx_hh_step(c, &p->state, &p->params, p->integrator, p->spike_rule, p->dt, s,
-                        rk4 ? p->g_in : nullptr, cur));
+    if (rk4 && p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT) {
+        // push the previous step's spikes into the targets' buckets; the
+        // ephys kernel sums each bucket in source order before its update
+        EventIn ev;
+        NNX_TRY(push_event_launch(c, &p->graph, p->list, p->count, &ev, prev));
+        NNX_TRY(hh_rk4_event_launch(c, &p->state, &p->params, p->dt, ev, cur));
+    } else {
+        void* target = rk4 ? p->g_in : (void*)p->state.in_local;
+        NNX_TRY(couple_launch(c, &p->graph, p->couple_mode, p->state.dtype, prev, p->list,
+                              p->count, p->state.out_local, target));
+        NNX_TRY(nnx_hh_step(c, &p->state, &p->params, p->integrator, p->spike_rule, p->dt, s,
+                            rk4 ? p->g_in : nullptr, cur));
+    }
     CompactExtras ex{};
     if (p->record) {
         ex.raster = p->raster;
-        ex.raster_len = p->raster_len;
         ex.raster_cap = p->raster_cap;
     }
-    ex.step = step_ctr;
+    ex.len2 = ctr;
+    ex.step2 = ctr + 2;
+    ex.parity = (int)(s & 1);
     ex.voxel_of = p->voxel_of;
     ex.voxel_counts = p->voxel_counts;
     return compact_launch(c, p->state.n, cur, p->list, p->count, ex);
@@ -70,6 +87,8 @@ static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, i
 }  // namespace nnx
 
 using namespace nnx;
+
+static nnx_status replay(nnx_ctx* c, nnx_step_plan* p, uint64_t first_step, int n_steps, int64_t* ctr);
 
 extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first_step, int n_steps,
                                     int use_graph) {
@@ -82,15 +101,26 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
         int32_t *a, *b;
         NNX_TRY(event_scratch(c, p->graph.n, p->graph.nnz, &a, &b));
     }
-    void* ctr;
-    NNX_TRY(scratch(c, "step_ctr", 64, true, &ctr));
-    int64_t* step_ctr = (int64_t*)ctr;
-    k_set_i64<<<1, 1, 0, c->stream>>>(step_ctr, (int64_t)first_step);
+    void* ctrp;
+    NNX_TRY(scratch(c, "step_ctr", 64, true, &ctrp));
+    int64_t* ctr = (int64_t*)ctrp;  // [0..1] raster length, [2..3] step, by parity
+    const int par0 = (int)(first_step & 1), par1 = (int)((first_step + n_steps) & 1);
+    k_run_begin<<<1, 1, 0, c->stream>>>(ctr, ctr + 2, p->record ? p->raster_len : nullptr, par0,
+                                        (int64_t)first_step);
     NNX_LAUNCHED(c);
     if (!use_graph) {
-        for (int i = 0; i < n_steps; ++i) NNX_TRY(enqueue_step(c, p, first_step + i, step_ctr));
-        return NNX_OK;
+        for (int i = 0; i < n_steps; ++i) NNX_TRY(enqueue_step(c, p, first_step + i, ctr));
+    } else {
+        NNX_TRY(replay(c, p, first_step, n_steps, ctr));
     }
+    if (p->record) {
+        k_run_end<<<1, 1, 0, c->stream>>>(ctr, p->raster_len, par1);
+        NNX_LAUNCHED(c);
+    }
+    return NNX_OK;
+}
+
+static nnx_status replay(nnx_ctx* c, nnx_step_plan* p, uint64_t first_step, int n_steps, int64_t* ctr) {
     NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
     const std::pair<const void*, int> key((const void*)(uintptr_t)plan_hash(p),
                                           n_steps * 2 + (int)(first_step & 1));
@@ -100,7 +130,7 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
         NNX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         c->capturing = 1;
         nnx_status st = NNX_OK;
-        for (int i = 0; i < n_steps && st == NNX_OK; ++i) st = enqueue_step(c, p, first_step + i, step_ctr);
+        for (int i = 0; i < n_steps && st == NNX_OK; ++i) st = enqueue_step(c, p, first_step + i, ctr);
         c->capturing = 0;
         hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
         if (st != NNX_OK) {
