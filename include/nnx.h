@@ -312,6 +312,13 @@ nnx_status nnx_update_density_reference(nnx_ctx* ctx, int nx, int ny, int nz, do
 nnx_status nnx_deposit_count(nnx_ctx* ctx, const int32_t* d_voxel_of, const int32_t* d_list,
                              const int32_t* d_count, int32_t* d_voxel_counts);
 
+/* release[list[i]] += amount for the count neurons of the ascending spike
+ * list (each neuron at most once: no conflicts).  The per-step release
+ * accumulation of ParacrineModule::update_on_fine_timescale
+ * (src/paracrine.cuh:246-253), applied to the firing neurons.               */
+nnx_status nnx_release_add(nnx_ctx* ctx, const int32_t* d_list, const int32_t* d_count,
+                           double amount, float* d_release);
+
 /* b = c_prev + amount * voxel_counts, then voxel_counts = 0.               */
 nnx_status nnx_rhs_from_counts(nnx_ctx* ctx, int64_t nvox, const float* d_c_prev,
                                int32_t* d_voxel_counts, double amount, float* d_b);

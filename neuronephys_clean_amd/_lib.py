@@ -125,6 +125,7 @@ _SIGS = {
                                       C.c_double, C.c_double, C.c_double, C.c_int, vp,
                                       _P(CgResult)], C.c_int),
     "nnx_deposit_count": ([vp, vp, vp, vp, vp], C.c_int),
+    "nnx_release_add": ([vp, vp, vp, C.c_double, vp], C.c_int),
     "nnx_rhs_from_counts": ([vp, C.c_int64, vp, vp, C.c_double, vp], C.c_int),
     "nnx_csr_spmv": ([vp, C.c_int64, vp, vp, vp, vp, vp, C.c_int], C.c_int),
     "nnx_run_steps": ([vp, _P(StepPlan), C.c_uint64, C.c_int, C.c_int], C.c_int),

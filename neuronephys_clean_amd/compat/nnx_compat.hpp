@@ -1,0 +1,386 @@
+// nnx_compat.hpp - the reference's C++ entry points of the per-timestep hot
+// path (ephys, coupling, paracrine), with their names and signatures, over
+// the C ABI of include/nnx.h.  Header-only, plain C++17: a caller compiles it
+// with any host compiler and links libnnx.so (+ libnnx_host.so for the
+// trilinear transfer matrices); no HIP toolchain is needed.
+//
+// Replaced interfaces (neuronephys_clean sources):
+//   Ephys / HHEphys::SimulateEphys(Float**)     src/ephys.cuh:6-31, src/ephys.cu:98-103
+//   AtomicCoupling(int), operator(), GetFiringNeurons()   src/coupling.cuh:17-38
+//   AtomicCoupling(int, AdjacencyList&, Float*, Float*), PerformCoupling()
+//                                               src_wholebrain/coupling.cuh:26-43
+//   AdjacencyList                               src/coupling.cuh:4-8, src_wholebrain/coupling.cuh:11-16
+//   Density_on_grid_3d                          src/paracrine.cuh:121-173
+//   Scatter_grid_converter_3d                   src/paracrine.cuh:184-209
+//   ParacrineModule                             src/paracrine.cuh:212-267
+//
+// Deliberate differences (also listed in the design notes):
+//   - thrust::device_vector members become nnx_compat::DeviceVector (same
+//     data()/size() use); ParacrineModule takes coordinate arrays as device
+//     pointers; density() returns the device pointer of the field.
+//   - errors throw nnx_compat::Error (the reference prints and exits).
+//   - the white noise of Leapfrog_HH (cuRAND, src/ephys.cu:82) is replaced
+//     by its mean: the step is deterministic.
+//   - update_on_fine_timescale adds the release to the firing neurons (the
+//     reference's transform_if uses the compacted index list as its stencil,
+//     src/paracrine.cuh:250-252, which selects list positions, not neurons).
+#pragma once
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nnx.h"
+#include "../../include/nnx_host.h"
+
+namespace nnx_compat {
+
+using Float = float;  // USE_DOUBLE_PRECISION 0 (src/defines.cuh:50-55)
+
+// SoA indices of the HH state (src/defines.cuh:57-78)
+enum SimParams {
+    SIM_V = 0, SIM_M = 1, SIM_N = 2, SIM_H = 3, SIM_LOCAL_Y = 4, SIM_IY = 5, SIM_EY = 6,
+    SIM_LOCAL_OUTPUT = 7, SIM_LOCAL_IN = 8, SIM_LOCAL_COUNT = 9, SIM_EOUTPUT = 10, SIM_EIN = 11,
+    SIM_ECOUNT = 12, SIM_IOUTPUT = 13, SIM_IIN = 14, SIM_ICOUNT = 15, SIM_APPCUR = 16,
+    NUM_SIM = 17
+};
+
+struct Error : std::runtime_error {
+    nnx_status status;
+    Error(nnx_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+// One context per process (the reference is single-GPU, single-thread).
+inline nnx_ctx* ctx() {
+    static nnx_ctx* c = [] {
+        nnx_ctx* p = nullptr;
+        const nnx_status s = nnx_create(&p, 0, nullptr);
+        if (s != NNX_OK) throw Error(s, "nnx_create failed (no GPU?)");
+        return p;
+    }();
+    return c;
+}
+
+inline void check(nnx_status s) {
+    if (s != NNX_OK) throw Error(s, nnx_last_error(ctx()));
+}
+
+// Device array owning its memory (the role thrust::device_vector plays in
+// the reference).
+template <class T> class DeviceVector {
+    T* p_ = nullptr;
+    size_t n_ = 0;
+
+   public:
+    DeviceVector() = default;
+    explicit DeviceVector(size_t n, bool zero = true) : n_(n) {
+        void* v = nullptr;
+        check(nnx_malloc(ctx(), &v, n * sizeof(T) + 16));
+        p_ = static_cast<T*>(v);
+        if (zero) check(nnx_memset(ctx(), p_, 0, n * sizeof(T) + 16));
+    }
+    explicit DeviceVector(const std::vector<T>& h) : DeviceVector(h.size(), false) { upload(h); }
+    DeviceVector(const DeviceVector&) = delete;
+    DeviceVector& operator=(const DeviceVector&) = delete;
+    DeviceVector(DeviceVector&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DeviceVector& operator=(DeviceVector&& o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+        return *this;
+    }
+    ~DeviceVector() {
+        if (p_) nnx_free(ctx(), p_);
+    }
+    T* data() { return p_; }
+    const T* data() const { return p_; }
+    size_t size() const { return n_; }
+    void upload(const std::vector<T>& h) {
+        if (!h.empty()) check(nnx_memcpy_h2d(ctx(), p_, h.data(), h.size() * sizeof(T)));
+    }
+    std::vector<T> host() const {
+        std::vector<T> h(n_);
+        if (n_) check(nnx_memcpy_d2h(ctx(), h.data(), p_, n_ * sizeof(T)));
+        return h;
+    }
+};
+
+// CSR of out-edges (indices / offset_in_indices) as the reference's
+// AdjacencyList; the transposed layout the deterministic coupling reads is
+// built on the device on first use.
+struct AdjacencyList {
+    DeviceVector<int> indices;
+    DeviceVector<int> offset_in_indices;
+    DeviceVector<Float> weights;  // optional (empty: unweighted)
+
+    struct Transposed {
+        DeviceVector<int32_t> in_off, in_src, out_to_in;
+        DeviceVector<float> in_w, in_deg;
+        bool built = false;
+    };
+    mutable std::shared_ptr<Transposed> t = std::make_shared<Transposed>();
+
+    AdjacencyList() = default;
+    AdjacencyList(const std::vector<int>& offsets, const std::vector<int>& targets)
+        : indices(targets), offset_in_indices(offsets) {}
+
+    int64_t n() const { return offset_in_indices.size() ? (int64_t)offset_in_indices.size() - 1 : 0; }
+    int64_t nnz() const { return (int64_t)indices.size(); }
+
+    nnx_graph graph() const {
+        Transposed& T = *t;
+        if (!T.built) {
+            const int64_t n = this->n(), e = nnz();
+            T.in_off = DeviceVector<int32_t>(n + 1);
+            T.in_src = DeviceVector<int32_t>(e ? e : 1);
+            T.out_to_in = DeviceVector<int32_t>(e ? e : 1);
+            T.in_w = DeviceVector<float>(e ? e : 1);
+            T.in_deg = DeviceVector<float>(n ? n : 1);
+            check(nnx_graph_transpose(ctx(), n, e, offset_in_indices.data(), indices.data(),
+                                      weights.size() ? weights.data() : nullptr, T.in_off.data(),
+                                      T.in_src.data(), T.in_w.data(), T.out_to_in.data(), NNX_F32,
+                                      T.in_deg.data()));
+            T.built = true;
+        }
+        nnx_graph g{};
+        g.n = n();
+        g.nnz = nnz();
+        g.out_offsets = offset_in_indices.data();
+        g.out_targets = indices.data();
+        g.out_weights = weights.size() ? weights.data() : nullptr;
+        g.in_offsets = T.in_off.data();
+        g.in_sources = T.in_src.data();
+        g.in_weights = T.in_w.data();
+        g.out_to_in = T.out_to_in.data();
+        return g;
+    }
+    // in-degree, as Network::ToAdjacencyList writes it into COUNT
+    // (src/network_factory.cu:20-23)
+    const float* in_degree() const {
+        graph();
+        return t->in_deg.data();
+    }
+};
+
+// --------------------------------------------------------------- ephys ----
+class Ephys {
+   protected:
+    int const N;
+
+   public:
+    explicit Ephys(int const N_) : N{N_} { ctx(); }
+    virtual void SimulateEphys(Float** sim_vars) = 0;
+    virtual ~Ephys() = default;
+};
+
+// Leapfrog_HH (src/ephys.cu:28-96) at hh_params::leapfrog_dt = 0.4 ms.
+// sim_vars is a device array of NUM_SIM device pointers, as the reference's
+// thrust::raw_pointer_cast(sim_vars.data()).
+class HHEphys : public Ephys {
+    std::map<Float**, std::vector<Float*>> tables_;
+    nnx_hh_params p_{};
+
+   public:
+    static constexpr Float leapfrog_dt = 0.4f;  // src/defines.cuh:89
+    explicit HHEphys(int const N_) : Ephys(N_) {
+        nnx_hh_params_reference(&p_);
+        p_.noise_std = 0.0;
+    }
+    nnx_hh_params& params() { return p_; }
+    void SimulateEphys(Float** sim_vars) override {
+        auto it = tables_.find(sim_vars);
+        if (it == tables_.end()) {
+            std::vector<Float*> h(NUM_SIM);
+            check(nnx_memcpy_d2h(ctx(), h.data(), sim_vars, NUM_SIM * sizeof(Float*)));
+            it = tables_.emplace(sim_vars, h).first;
+        }
+        const std::vector<Float*>& a = it->second;
+        nnx_hh_state st{};
+        st.dtype = NNX_F32;
+        st.n = N;
+        st.v = a[SIM_V];
+        st.m = a[SIM_M];
+        st.h = a[SIM_H];
+        st.ng = a[SIM_N];
+        st.y_local = a[SIM_LOCAL_Y];
+        st.y_i = a[SIM_IY];
+        st.y_e = a[SIM_EY];
+        st.in_local = a[SIM_LOCAL_IN];
+        st.count_local = a[SIM_LOCAL_COUNT];
+        st.in_i = a[SIM_IIN];
+        st.count_i = a[SIM_ICOUNT];
+        st.in_e = a[SIM_EIN];
+        st.count_e = a[SIM_ECOUNT];
+        st.appcur = a[SIM_APPCUR];
+        st.out_local = a[SIM_LOCAL_OUTPUT];
+        st.out_i = a[SIM_IOUTPUT];
+        st.out_e = a[SIM_EOUTPUT];
+        check(nnx_hh_step(ctx(), &st, &p_, NNX_INT_LEAPFROG, NNX_SPIKE_LEVEL, leapfrog_dt, 0, nullptr,
+                          nullptr));
+        check(nnx_synchronize(ctx()));  // the reference synchronises after the launch
+    }
+};
+
+// ------------------------------------------------------------ coupling ----
+// Threshold (output > 0.1) -> ascending firing list -> deterministic
+// propagation.  The src/ form adds 1 per firing source; the
+// src_wholebrain/ form (constructed with the list and the two arrays) adds
+// the source's output.
+class AtomicCoupling {
+    int N;
+    DeviceVector<uint64_t> bits_;
+    DeviceVector<int32_t> count_;
+    Float* downstream_inputs = nullptr;
+    Float* neuron_outputs = nullptr;
+
+    int couple(const AdjacencyList& l, Float* spikes, const Float* outputs, int mode) {
+        check(nnx_spike_threshold(ctx(), NNX_F32, N, outputs, 0.1, bits_.data()));
+        check(nnx_spike_compact(ctx(), N, bits_.data(), firing_neurons.data(), count_.data()));
+        const nnx_graph g = l.graph();
+        check(nnx_couple(ctx(), &g, mode, NNX_F32, bits_.data(), firing_neurons.data(),
+                         count_.data(), outputs, spikes));
+        int32_t k = 0;
+        check(nnx_read_count(ctx(), count_.data(), &k));  // the reference returns the count
+        return k;
+    }
+
+   public:
+    AdjacencyList list;                    // src_wholebrain form
+    DeviceVector<int32_t> firing_neurons;  // ascending; first count entries valid
+
+    explicit AtomicCoupling(int const N_)
+        : N(N_), bits_((N_ + 63) / 64 + 1), count_(1), firing_neurons(N_ + 1) {}
+    AtomicCoupling(int const N_, AdjacencyList& list_, Float* downstream_inputs_, Float* neuron_outputs_)
+        : AtomicCoupling(N_) {
+        std::swap(list, list_);  // the reference moves the list in (src_wholebrain/coupling.cu:6-9)
+        downstream_inputs = downstream_inputs_;
+        neuron_outputs = neuron_outputs_;
+    }
+    int operator()(const AdjacencyList& connectivity, Float* spikes, const Float* outputs) {
+        return couple(connectivity, spikes, outputs, NNX_COUPLE_COUNT);
+    }
+    int PerformCoupling() { return couple(list, downstream_inputs, neuron_outputs, NNX_COUPLE_SIGNAL); }
+    const DeviceVector<int32_t>& GetFiringNeurons() const { return firing_neurons; }
+    const int32_t* device_count() const { return count_.data(); }
+};
+
+// ----------------------------------------------------------- paracrine ----
+// The grid (x fastest) with the reference's Crank-Nicolson 27-point update.
+struct Density_on_grid_3d {
+    Float spatial_gridsize;
+    int x_count, y_count, z_count;
+    Float diffusion_coefficient, decay_coefficient;
+    DeviceVector<Float> data;
+    Float* raw_data_d_ptr;
+    nnx_cg_result last{};
+
+    Density_on_grid_3d(int x_count_, int y_count_, int z_count_, Float h, Float D, Float decay)
+        : spatial_gridsize(h), x_count(x_count_), y_count(y_count_), z_count(z_count_),
+          diffusion_coefficient(D), decay_coefficient(decay),
+          data((size_t)x_count_ * y_count_ * z_count_), raw_data_d_ptr(data.data()) {}
+    void update_density(const Float timestep, const Float error_bound = 1e-6f) {
+        check(nnx_update_density_reference(ctx(), x_count, y_count, z_count, spatial_gridsize,
+                                           diffusion_coefficient, decay_coefficient, timestep,
+                                           error_bound, 20, data.data(), &last));
+    }
+    void set_delta() {
+        std::vector<Float> one(1, 1.0f);
+        const size_t i = x_count / 2 + (size_t)x_count * (y_count / 2 + (size_t)y_count * (z_count / 2));
+        check(nnx_memcpy_h2d(ctx(), data.data() + i, one.data(), sizeof(Float)));
+    }
+    const Float* density() const { return data.data(); }
+};
+
+// Trilinear neuron <-> grid transfer (src/paracrine.cu:136-296): interpolation
+// CSR (N x G, 8 per row) and its transpose for the scatter.
+class Scatter_grid_converter_3d {
+    DeviceVector<int32_t> w_off_, w_col_, s_off_, s_col_;
+    DeviceVector<float> w_val_, s_val_;
+    int64_t G_;
+
+   public:
+    const int neuron_count;
+    int64_t outside = 0;
+
+    Scatter_grid_converter_3d(const Density_on_grid_3d& g, int N, const Float* d_x, const Float* d_y,
+                              const Float* d_z)
+        : G_((int64_t)g.x_count * g.y_count * g.z_count), neuron_count(N) {
+        std::vector<float> x(N), y(N), z(N);
+        check(nnx_memcpy_d2h(ctx(), x.data(), d_x, N * sizeof(float)));
+        check(nnx_memcpy_d2h(ctx(), y.data(), d_y, N * sizeof(float)));
+        check(nnx_memcpy_d2h(ctx(), z.data(), d_z, N * sizeof(float)));
+        std::vector<int32_t> off(N + 1), col(8 * (size_t)N), toff(G_ + 1), tcol(8 * (size_t)N);
+        std::vector<float> val(8 * (size_t)N), tval(8 * (size_t)N);
+        outside = nnx_trilinear_matrix(N, x.data(), y.data(), z.data(), g.x_count, g.y_count,
+                                       g.z_count, g.spatial_gridsize, off.data(), col.data(),
+                                       val.data());
+        nnx_csr_transpose(N, G_, off.data(), col.data(), val.data(), toff.data(), tcol.data(),
+                          tval.data());
+        w_off_ = DeviceVector<int32_t>(off);
+        w_col_ = DeviceVector<int32_t>(col);
+        w_val_ = DeviceVector<float>(val);
+        s_off_ = DeviceVector<int32_t>(toff);
+        s_col_ = DeviceVector<int32_t>(tcol);
+        s_val_ = DeviceVector<float>(tval);
+    }
+    // grid += W^T scatter (rows of the transpose list neurons in ascending order)
+    void scatter_to_grid(const Float* scatter, Density_on_grid_3d& grid) {
+        check(nnx_csr_spmv(ctx(), G_, s_off_.data(), s_col_.data(), s_val_.data(), scatter,
+                           grid.data.data(), 1));
+    }
+    // out = W grid
+    void interpolate_at_location(Float* out, const Density_on_grid_3d& grid) {
+        check(nnx_csr_spmv(ctx(), neuron_count, w_off_.data(), w_col_.data(), w_val_.data(),
+                           grid.density(), out, 0));
+    }
+};
+
+class ParacrineModule {
+    Density_on_grid_3d grid;
+    Scatter_grid_converter_3d grid_converter;
+
+   public:
+    int x_gridcount, y_gridcount, z_gridcount, timescale;
+
+   private:
+    Float* d_g_paracrine;
+    Float par_dt, coup_strength;
+    DeviceVector<Float> density_at_neurons, density_generated_at_neurons;
+
+   public:
+    ParacrineModule(Float*& d_g_paracrine_, int const neuron_count, int const nx, int const ny,
+                    int const nz, Float const par_dt_, int const timescale_, Float const coup_strength_,
+                    Float const spatial_gridsize, Float const diffusion_coef, Float const decay_coef,
+                    const Float* d_x, const Float* d_y, const Float* d_z)
+        : grid(nx, ny, nz, spatial_gridsize, diffusion_coef, decay_coef),
+          grid_converter(grid, neuron_count, d_x, d_y, d_z),
+          x_gridcount(nx), y_gridcount(ny), z_gridcount(nz), timescale(timescale_),
+          d_g_paracrine(d_g_paracrine_), par_dt(par_dt_), coup_strength(coup_strength_),
+          density_at_neurons(neuron_count), density_generated_at_neurons(neuron_count) {
+        grid_converter.interpolate_at_location(density_at_neurons.data(), grid);
+        check(nnx_memcpy_d2d(ctx(), d_g_paracrine, density_at_neurons.data(),
+                             neuron_count * sizeof(Float)));
+    }
+    // release of this step: +coup_strength at every firing neuron
+    void update_on_fine_timescale(const AtomicCoupling& coupling) {
+        check(nnx_release_add(ctx(), coupling.GetFiringNeurons().data(), coupling.device_count(),
+                              coup_strength, density_generated_at_neurons.data()));
+    }
+    // scatter the release, one Crank-Nicolson update, interpolate back
+    // (src/paracrine.cuh:255-265)
+    void update_on_own_timescale() {
+        grid_converter.scatter_to_grid(density_generated_at_neurons.data(), grid);
+        grid.update_density(par_dt);
+        grid_converter.interpolate_at_location(density_at_neurons.data(), grid);
+        check(nnx_memset(ctx(), density_generated_at_neurons.data(), 0,
+                         density_generated_at_neurons.size() * sizeof(Float)));
+        check(nnx_memcpy_d2d(ctx(), d_g_paracrine, density_at_neurons.data(),
+                             density_at_neurons.size() * sizeof(Float)));
+    }
+    const Density_on_grid_3d& get_grid() const { return grid; }
+};
+
+}  // namespace nnx_compat

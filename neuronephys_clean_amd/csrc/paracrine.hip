@@ -695,6 +695,12 @@ __global__ void k_deposit(const int32_t* __restrict__ voxel_of, const int32_t* _
     for (int64_t q = blockIdx.x * 256ll + threadIdx.x; q < K; q += (int64_t)gridDim.x * 256)
         atomicAdd(&vc[voxel_of[list[q]]], 1);
 }
+__global__ void k_release_add(const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+                              float amount, float* __restrict__ rel) {
+    const int32_t K = *count;
+    for (int64_t q = blockIdx.x * 256ll + threadIdx.x; q < K; q += (int64_t)gridDim.x * 256)
+        rel[list[q]] = rel[list[q]] + amount;
+}
 __global__ void k_rhs_counts(int64_t n, const float* __restrict__ cprev, int32_t* __restrict__ vc,
                              float amount, float* __restrict__ b) {
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
@@ -844,6 +850,15 @@ nnx_status nnx_deposit_count(nnx_ctx* c, const int32_t* voxel_of, const int32_t*
     if (!c) return NNX_ERR_ARG;
     NNX_CHECK(c, voxel_of && list && count && vc, "null argument");
     k_deposit<<<256, 256, 0, c->stream>>>(voxel_of, list, count, vc);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+nnx_status nnx_release_add(nnx_ctx* c, const int32_t* list, const int32_t* count, double amount,
+                           float* rel) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, list && count && rel, "null argument");
+    k_release_add<<<256, 256, 0, c->stream>>>(list, count, (float)amount, rel);
     NNX_LAUNCHED(c);
     return NNX_OK;
 }

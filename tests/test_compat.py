@@ -1,0 +1,112 @@
+"""The reference-interface layer (neuronephys_clean_amd/compat/nnx_compat.hpp).
+
+compat_demo is the reference's loop (src/main.cu:166-181: AtomicCoupling,
+ParacrineModule, HHEphys::SimulateEphys per increment) compiled by the host
+compiler against that header.  The GPU test runs it as a child process and
+checks the raster and final state bit for bit against the oracle's leapfrog
+run, and the paracrine field against the oracle's chain of trilinear scatter,
+Crank-Nicolson update and interpolation.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from neuronephys_clean_amd import _lib, configs, host
+from neuronephys_clean_amd.sim import SOA
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEMO = os.path.join(HERE, "..", "neuronephys_clean_amd", "compat", "compat_demo")
+
+
+def test_compat_demo_links_and_reports_usage():
+    """The header compiles with the host compiler and the program resolves
+    both libraries (no GPU call is made on a usage error)."""
+    assert os.path.exists(DEMO), "compat_demo not built (run __graft_entry__.build())"
+    r = subprocess.run([DEMO], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+def _spmv(off, cols, vals, x):
+    """y = A x with each row summed left to right in float32."""
+    rows = len(off) - 1
+    deg = np.diff(off)
+    y = np.zeros(rows, np.float32)
+    for k in range(int(deg.max()) if rows else 0):
+        m = deg > k
+        e = off[:-1][m] + k
+        y[m] = (y[m] + (vals[e] * x[cols[e]]).astype(np.float32)).astype(np.float32)
+    return y
+
+
+@pytest.mark.gpu
+def test_compat_loop_matches_oracle(tmp_path):
+    net = configs.c1(seed=7, n=1500, k=40)
+    in_deg = net.ensure_csc()[4]
+    rng = np.random.default_rng(21)
+    sv0 = {k: np.zeros(net.n) for k in SOA}
+    sv0["V"] = rng.normal(-55, 25, net.n)
+    sv0["LOCAL_COUNT"] = in_deg.astype(np.float64)
+    sv0["APPCUR"] = rng.normal(0, 2, net.n)
+    nx = ny = nz = 12
+    h, D, decay, ts, coup, nincs = 1.0, 0.5, 0.1, 10, 0.5, 200
+    par_dt = ts * 0.4
+    pos = ((rng.random((3, net.n)) - 0.5) * (nx - 1) * 0.9).astype(np.float32)
+    sv = [np.asarray(sv0[k], np.float32).copy() for k in SOA]
+    net.off.astype(np.int32).tofile(tmp_path / "off.i32")
+    net.tgt.astype(np.int32).tofile(tmp_path / "tgt.i32")
+    np.concatenate(sv).tofile(tmp_path / "sv.f32")
+    pos.tofile(tmp_path / "pos.f32")
+    args = [DEMO, str(tmp_path), str(nincs), str(nx), str(ny), str(nz), repr(h), repr(D),
+            repr(decay), repr(par_dt), str(ts), repr(coup)]
+    print("compat_demo", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+
+    # ephys + coupling: identical to the oracle's leapfrog run
+    q = _lib.params_reference()
+    q.noise_std = 0.0
+    ref, rln = oracle.run_leapfrog(oracle.params(q.as_dict()), 0.4, nincs, net.off, net.tgt, 2, sv,
+                                   cap=1_000_000)
+    ras = np.fromfile(tmp_path / "raster.i32", np.int32).reshape(-1, 2)
+    assert len(ras) == rln > 0 and np.array_equal(ras, ref)
+    out = np.fromfile(tmp_path / "sv_out.f32", np.float32).reshape(len(SOA), net.n)
+    for k, a, b in zip(SOA, out, sv):
+        assert np.array_equal(a, b), k
+    # the coupling's return value is the firing count of the previous increment
+    first = int((np.asarray(sv0["LOCAL_OUTPUT"], np.float32) > 0.1).sum())
+    counts = np.bincount(ref[:, 0], minlength=nincs)
+    nfiring = np.fromfile(tmp_path / "nfiring.i32", np.int32)
+    assert nfiring[0] == first and np.array_equal(nfiring[1:], counts[:-1])
+
+    # paracrine chain
+    fired = [np.zeros(0, np.int64)] + [ref[ref[:, 0] == s, 1] for s in range(nincs - 1)]
+    fired[0] = np.nonzero(np.asarray(sv0["LOCAL_OUTPUT"], np.float32) > 0.1)[0]
+    G = nx * ny * nz
+    off, cols, vals, _ = host.trilinear_matrix(*pos, nx, ny, nz, h)
+    toff, tcol, tval = host.csr_transpose(net.n, G, off, cols, vals)
+    dens = np.zeros(G, np.float32)
+    rel = np.zeros(net.n, np.float32)
+    g = _spmv(off, cols, vals, dens)
+    updates = 0
+    for i in range(nincs):
+        if i % ts == 0:
+            dens = (dens + _spmv(toff, tcol, tval, rel)).astype(np.float32)
+            dens = oracle.update_density_reference(dens.reshape(nz, ny, nx), h, D, decay, par_dt,
+                                                   1e-6, 20)[0].ravel()
+            g = _spmv(off, cols, vals, dens)
+            rel[:] = 0
+            updates += 1
+        else:
+            rel[fired[i]] = (rel[fired[i]] + np.float32(coup)).astype(np.float32)
+    got_d = np.fromfile(tmp_path / "density.f32", np.float32)
+    got_g = np.fromfile(tmp_path / "g_par.f32", np.float32)
+    print("paracrine updates", updates, "max density", dens.max(),
+          "max |diff|", np.abs(got_d - dens).max(), flush=True)
+    assert dens.max() > 0
+    tol = 1e-4 * float(dens.max())
+    np.testing.assert_allclose(got_d, dens, rtol=1e-3, atol=tol)
+    np.testing.assert_allclose(got_g, g, rtol=1e-3, atol=tol)
