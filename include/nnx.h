@@ -72,8 +72,10 @@ enum { NNX_PRECOND_NONE = 0, NNX_PRECOND_JACOBI = 1, NNX_PRECOND_LEARNED = 2 };
 /* context                                                                   */
 /* ------------------------------------------------------------------------ */
 
-/* Creates a context on `device`; `hip_stream` (hipStream_t, may be NULL for
- * the null stream) is borrowed, not owned.                                  */
+/* Creates a context on `device`.  A non-NULL `hip_stream` (hipStream_t) is
+ * borrowed, not owned; with NULL the context creates a blocking stream of
+ * its own (default priority) and destroys it in nnx_destroy.  Every call
+ * enqueues on that stream.                                                  */
 nnx_status nnx_create(nnx_ctx** out, int device, void* hip_stream);
 nnx_status nnx_destroy(nnx_ctx* ctx);
 const char* nnx_last_error(const nnx_ctx* ctx);
@@ -244,8 +246,13 @@ nnx_status nnx_couple(nnx_ctx* ctx, const nnx_graph* g, int mode, int dtype,
  * Operators (Dirichlet 0 outside the box):
  *   STENCIL_7 : A = diag*I - c*sum(6 neighbours), diag = 1 + 6c, i.e.
  *               (I - c*L7) of a backward-Euler diffusion step, c = dt*D/h^2;
- *               with kx/ky/kz set, c varies per face: kx[idx] couples voxel
- *               idx to idx+1 (x), and the diagonal is 1 + sum of its faces.
+ *               with kx/ky/kz set, c varies per face (c is then unused):
+ *               kx has (nx+1)*ny*nz entries, kx[i + (nx+1)*(j + ny*k)] the
+ *               face between voxels i-1 and i of row (j,k) (i = 0 and i = nx
+ *               are the boundary faces); ky has nx*(ny+1)*nz entries indexed
+ *               i + nx*(j + (ny+1)*k); kz has nx*ny*(nz+1) indexed
+ *               i + nx*(j + ny*k).  Each entry is dt*D_face/h^2 and the
+ *               diagonal is 1 + the sum of the voxel's six faces.
  *   STENCIL_27: A = (1 + decay_dt/2) I - (c/2) L27, L27 the reference's
  *               high-order compact Laplacian {1,3,14,-128}/30 (src/paracrine.cu:40-43,80-81),
  *               c = dt*D/h^2: the Crank-Nicolson operator.                  */
