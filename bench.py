@@ -97,7 +97,7 @@ def cg_bench(ctx, grid_n, b_host, rtol=1e-8):
     db64 = DA.from_numpy(ctx, b_host.astype(np.float64))
     dx64 = DA.zeros(ctx, nvox, np.float64)
     r2 = _lib.CgResult()
-    ctx._l.nnx_cg_solve_refine(ctx.h, C.byref(g), db64.ptr, dx64.ptr, C.byref(o), 1e-5, 20,
+    ctx._l.nnx_cg_solve_refine(ctx.h, C.byref(g), db64.ptr, dx64.ptr, C.byref(o), 1e-6, 20,
                                C.byref(r2))
     # dominant CG kernels: per-iteration time from the solve
     return {"grid": f"{grid_n}^3", "c": 1.0, "rtol": rtol, "precond": "jacobi",
@@ -107,6 +107,64 @@ def cg_bench(ctx, grid_n, b_host, rtol=1e-8):
             "refined_ms": r2.ms, "refined_outer": r2.outer, "refined_inner_iters": r2.iters,
             "refined_relres_fp64": r2.relres, "learned_pc_ms": None,
             "status": int(st)}
+
+
+def c3_bench(ctx):
+    """Precision split, paracrine half: 512^3 grid, dt*D/h^2 = 20, 50,000
+    unit point releases; fp32 inner CG with FP64 dots, FP64 outer
+    refinement to a 1e-10 relative residual (inputs resident in HBM)."""
+    from neuronephys_clean_amd import _lib, configs
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+    n, c = configs.C3_GRID, configs.C3_C
+    b = configs.c3_sources(n=n)
+    g = _lib.Grid(n, n, n, 7, c, 0.0, None, None, None)
+    db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, n ** 3, np.float64)
+    del b
+    o = _lib.CgOpts(rtol=1e-10, maxit=5000, precond=_lib.PRECOND_JACOBI, check_every=8)
+    runs = []
+    for _ in range(2):
+        r = _lib.CgResult()
+        ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-6, 30,
+                 C.byref(r))
+        runs.append(r)
+    r = min(runs, key=lambda q: q.ms)
+    return {"grid": f"{n}^3", "c": c, "sources": 50000, "rtol": 1e-10, "inner_rtol": 1e-6,
+            "refined_ms": r.ms, "outer": r.outer, "inner_iters": r.iters,
+            "relres_fp64": r.relres, "converged": bool(r.converged),
+            "bytes_per_inner_iter_alg": 44 * n ** 3}
+
+
+def precision_split(ctx, net, steps=(1, 100, 10000)):
+    """max |V_fp32 - V_fp64| of the same network after 1, 100 and 10,000
+    steps.  The fp64 device run is checked bit for bit against the FP64
+    oracle over the first 100 steps; later steps use it as the FP64
+    trajectory."""
+    import oracle
+    from neuronephys_clean_amd import _lib
+    from neuronephys_clean_amd.sim import Simulation
+    s32 = Simulation(net, np.float32, couple="event", ctx=ctx, device_csc=True)
+    s64 = Simulation(net, np.float64, couple="event", ctx=ctx, device_csc=True)
+    out, done = {}, 0
+    v64_100 = None
+    for k in steps:
+        for sim in (s32, s64):
+            sim.run(k - done, use_graph=True, chunk=100 if (k - done) % 100 == 0 else k - done)
+        done = k
+        v32, v64 = s32.state()["v"], s64.state()["v"]
+        out[str(k)] = float(np.max(np.abs(v32.astype(np.float64) - v64)))
+        if k == 100:
+            v64_100 = v64
+    res = {"max_abs_dV_mV": out}
+    if v64_100 is not None:
+        T = np.float64
+        V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
+        s = np.zeros(net.n, T)
+        p = oracle.params(_lib.params_squid().as_dict())
+        oracle.run_rk4(p, net.dt, 100, net.off, net.tgt, net.w, V, m, h, ng, s,
+                       np.asarray(net.iext, T), cap=1)
+        res["fp64_device_equals_oracle_at_100"] = bool(np.array_equal(V, v64_100))
+    del s32, s64
+    return res
 
 
 def stencil_roofline(ctx, n=256, reps=50):
@@ -165,6 +223,7 @@ def main():
                          "device copy to calibrate FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
@@ -247,6 +306,13 @@ def main():
         cg = cg_bench(ctx, net.grid[0], b)
         log(f"bench: CG {cg}")
 
+    c3 = None
+    if not args.no_c3:
+        c3 = c3_bench(ctx)
+        log(f"bench: C3 {c3}")
+        c3["precision_split"] = precision_split(ctx, net)
+        log(f"bench: precision split {c3['precision_split']}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(net, args.cpu_steps)
@@ -263,6 +329,7 @@ def main():
                       "graph_chunk": args.chunk, "spikes_timed": int(spikes),
                       "spikes_per_step": spikes / args.steps},
            "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
+           "c3": c3,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},
            "event_ms": ev_ms.value}

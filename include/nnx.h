@@ -297,8 +297,10 @@ nnx_status nnx_cg_solve(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float
                         const nnx_cg_opts* o, nnx_cg_result* res);
 
 /* Mixed-precision iterative refinement: FP64 b and x, the outer residual
- * r = b - A x recomputed in FP64, inner fp32 CG (inner_rtol) on A d = r,
- * x += d, until ||r||/||b|| <= o->rtol.                                     */
+ * r = b - A x recomputed in FP64, inner fp32 CG on A d = r, x += d, until
+ * ||r||/||b|| <= o->rtol.  Each inner solve stops at the relative residual
+ * max(inner_rtol, 0.5 * o->rtol / current relres): inner_rtol is the
+ * tightest inner tolerance (about 1e-6 is the useful limit of fp32).       */
 nnx_status nnx_cg_solve_refine(nnx_ctx* ctx, const nnx_grid* g, const double* d_b, double* d_x,
                                const nnx_cg_opts* o, double inner_rtol, int max_outer,
                                nnx_cg_result* res);

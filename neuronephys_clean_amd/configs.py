@@ -28,6 +28,7 @@ from . import host
 
 # Philox purposes (one stream per kind of draw)
 P_V0, P_IEXT, P_W, P_POS, P_MU, P_NSIZE, P_GRIDSRC = 101, 102, 103, 104, 105, 106, 107
+P_C3SRC = 108
 
 SQUID_VSHIFT = 5.0
 
@@ -97,6 +98,18 @@ def c2(seed: int = 0, n: int = 1 << 20, lam: float = 10.0, cutoff: float = 60.0,
     g = grid if grid is not None else max(8, int(round(256 * scale)))
     return Network(n, off, tgt, w, V0, m0, h0, n0, iext, pos, grid=(g, g, g), h_grid=box / g,
                    c=1.0, name=f"c2_n{n}")
+
+
+def c3_sources(seed: int = 0, n: int = 512, count: int = 50_000) -> np.ndarray:
+    """The precision-split solve's right-hand side: `count` unit point
+    releases at uniformly random voxels of an n^3 grid (coincident releases
+    add), FP64, x fastest."""
+    u = host.uniform(seed, P_C3SRC, count, 0.0, 1.0)
+    idx = np.minimum((u * float(n) ** 3).astype(np.int64), n ** 3 - 1)
+    return np.bincount(idx, minlength=n ** 3).astype(np.float64)
+
+
+C3_GRID, C3_C = 512, 20.0
 
 
 def c4_sizes(seed: int = 0, count: int = 256):

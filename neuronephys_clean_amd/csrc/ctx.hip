@@ -105,7 +105,7 @@ nnx_status nnx_create(nnx_ctx** out, int device, void* hip_stream) {
     // ordered with ours)
     if (!c->stream) {
         if (hipStreamCreate(&c->stream) != hipSuccess) {
-            hipHostFree(c->pinned);
+            (void)hipHostFree(c->pinned);
             delete c;
             return NNX_ERR_HIP;
         }
@@ -117,16 +117,16 @@ nnx_status nnx_create(nnx_ctx** out, int device, void* hip_stream) {
 
 nnx_status nnx_destroy(nnx_ctx* c) {
     if (!c) return NNX_OK;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->graphs) {
-        if (g.second.exec) hipGraphExecDestroy(g.second.exec);
-        if (g.second.graph) hipGraphDestroy(g.second.graph);
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+        if (g.second.graph) (void)hipGraphDestroy(g.second.graph);
     }
     for (auto& b : c->bufs)
-        if (b.second.p) hipFree(b.second.p);
-    if (c->pinned) hipHostFree(c->pinned);
-    if (c->owns_stream) hipStreamDestroy(c->stream);
+        if (b.second.p) (void)hipFree(b.second.p);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return NNX_OK;
 }
@@ -137,7 +137,7 @@ nnx_status nnx_set_stream(nnx_ctx* c, void* s) {
     if (!c || !s) return NNX_ERR_ARG;
     if (c->owns_stream) {
         NNX_HIP(c, hipStreamSynchronize(c->stream));
-        hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->stream);
         c->owns_stream = 0;
     }
     c->stream = (hipStream_t)s;

@@ -11,6 +11,7 @@
 // are reproducible), and the host only reads a flag every check_every
 // iterations.
 #include <chrono>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -793,8 +794,10 @@ nnx_status nnx_cg_solve_refine(nnx_ctx* c, const nnx_grid* g, const double* b, d
         const double s = 1.0 / std::sqrt(rr);
         k_f64_to_f32_scaled<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const double*)r64, s, (float*)rhs);
         NNX_LAUNCHED(c);
+        // solve the correction no tighter than the outer target needs
+        const double tol = std::max(inner_rtol, 0.5 * o->rtol / rel);
         nnx_cg_result ir{};
-        NNX_TRY(cg_run(c, gk, h, (const float*)rhs, (float*)d, 0, inner_rtol, o->maxit, 0,
+        NNX_TRY(cg_run(c, gk, h, (const float*)rhs, (float*)d, 0, tol, o->maxit, 0,
                        o->check_every, &ir));
         inner_total += ir.iters;
         k_axpy_f64<<<flat_blocks(n), 256, 0, c->stream>>>(n, 1.0 / s, (const float*)d, x);

@@ -134,7 +134,7 @@ static nnx_status replay(nnx_ctx* c, nnx_step_plan* p, uint64_t first_step, int 
         c->capturing = 0;
         hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
         if (st != NNX_OK) {
-            if (g.graph) hipGraphDestroy(g.graph);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
             return st;
         }
         if (e != hipSuccess) return fail(c, NNX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));

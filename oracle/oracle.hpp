@@ -360,7 +360,7 @@ inline int cg(const Grid& g, const T* b, T* x, double rtol, int maxit, bool jaco
 }
 
 // FP64 iterative refinement around the float CG: r = b - A x in double,
-// inner float CG to inner_rtol on A d = r, x += d.
+// inner float CG on A d = r to max(inner_rtol, 0.5 rtol / relres), x += d.
 inline int refine(const Grid& g, const double* b, double* x, double rtol, double inner_rtol,
                   int maxit, int max_outer, double* relres, int* inner_total) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
@@ -377,7 +377,8 @@ inline int refine(const Grid& g, const double* b, double* x, double rtol, double
         if (rel <= rtol || outer >= max_outer) break;
         for (int64_t i = 0; i < n; ++i) rf[i] = (float)r[i];
         double ir;
-        tot += cg<float>(g, rf.data(), d.data(), inner_rtol, maxit, true, false, &ir);
+        const double tol = std::max(inner_rtol, 0.5 * rtol / rel);
+        tot += cg<float>(g, rf.data(), d.data(), tol, maxit, true, false, &ir);
         for (int64_t i = 0; i < n; ++i) x[i] += (double)d[i];
         ++outer;
     }

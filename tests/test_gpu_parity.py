@@ -302,7 +302,7 @@ def test_refinement_to_1e10(ctx):
     db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, nx ** 3, np.float64)
     o = _lib.CgOpts(rtol=1e-10, maxit=5000, precond=_lib.PRECOND_JACOBI, check_every=8)
     r = _lib.CgResult()
-    ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-5, 30, C.byref(r))
+    ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-6, 30, C.byref(r))
     say("refine", r.as_dict())
     assert r.converged and r.relres <= 1e-10 and r.outer >= 2
     lam, _ = _lam7(nx, nx, nx, c)
