@@ -341,6 +341,66 @@ nnx_status nnx_csr_spmv(nnx_ctx* ctx, int64_t rows, const int32_t* d_offsets, co
                         const float* d_vals, const float* d_x, float* d_y, int accumulate);
 
 /* ------------------------------------------------------------------------ */
+/* learned preconditioner (no counterpart in the reference: its paracrine    */
+/* solve is unpreconditioned CG, src/paracrine.cu:106-129)                   */
+/* ------------------------------------------------------------------------ */
+
+/* A 3-level 3-D convolutional U-Net M(r) ~ A^-1 r, BF16 weights and
+ * activations with FP32 accumulation on matrix cores:
+ *   e0 = relu(conv3(x; 2->16)), e1 = relu(conv3(pool(e0); 16->32)),
+ *   e2 = relu(conv3(pool(e1); 32->32)), d1 = relu(conv3(e1 + up(e2); 32->16)),
+ *   d0 = relu(conv3(e0 + up(d1); 16->16)), z = rms(r) (w_h . d0 + b_h),
+ * x = (r / rms(r), feature) per voxel, conv3 = 3x3x3 with zero padding,
+ * pool = 2x2x2 mean, up = nearest x2.  Parameters (fp32, in this order):
+ * per layer W[cout][27][cin] (tap = dx + 3 dy + 9 dz fastest-x) then b[cout];
+ * then the head's 16 weights and bias: nnx_unet_param_count() floats.
+ * Grids: nx % 64 == 0, ny % 32 == 0, nz % 32 == 0.                          */
+typedef struct nnx_unet nnx_unet;
+int64_t nnx_unet_param_count(void);
+/* h_params (host, param_count floats) or NULL for a seeded He-uniform init */
+nnx_status nnx_unet_create(nnx_ctx* ctx, const float* h_params, uint64_t seed, nnx_unet** out);
+nnx_status nnx_unet_destroy(nnx_ctx* ctx, nnx_unet* net);
+nnx_status nnx_unet_get_params(nnx_ctx* ctx, const nnx_unet* net, float* h_params);
+/* gradient of the last nnx_unet_train_step (same layout as the parameters) */
+nnx_status nnx_unet_get_grad(nnx_ctx* ctx, const nnx_unet* net, float* h_grad);
+/* z = M(r) for `batch` grids stored one after another (fp32; d_feat may be
+ * NULL for a zero feature channel).                                         */
+nnx_status nnx_unet_apply(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int nz, int batch,
+                          const float* d_r, const float* d_feat, float* d_z);
+/* One Adam step (lr; lr = 0 only computes the gradient) on the loss
+ * (1/B) sum_b |N(r_b / s_b) - e_b / s_b|^2 / |e_b / s_b|^2, s_b = rms(r_b),
+ * N the network without the output scale.  *h_loss (if not NULL) receives
+ * the loss before the step (the call then synchronises).                    */
+nnx_status nnx_unet_train_step(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int nz, int batch,
+                               const float* d_r, const float* d_feat, const float* d_e, double lr,
+                               double* h_loss);
+
+/* Single convolution layers (BF16 channels-last tensors as uint16_t, weights
+ * fp32 [cout][27 cin]); (cin, cout) in {(2,16), (16,16), (16,32), (32,16),
+ * (32,32)}.  out = act(conv3(in + coarse_scale * up(coarse)) + bias), pool =
+ * its 2x2x2 mean; wgrad: dw[co][tap cin + ci] = sum_v g[v][co] in'[v + tap][ci]
+ * and db[co] = sum_v g[v][co], in' = in + coarse_scale * up(coarse).        */
+nnx_status nnx_conv3_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny, int nz, int batch,
+                          const uint16_t* d_in, const uint16_t* d_coarse, double coarse_scale,
+                          const float* d_w, const float* d_bias, int relu, uint16_t* d_out,
+                          uint16_t* d_pool);
+nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny, int nz, int batch,
+                                const uint16_t* d_in, const uint16_t* d_coarse, double coarse_scale,
+                                const uint16_t* d_g, float* d_dw, float* d_db);
+
+/* Flexible CG (Notay, one stored direction): fp32 vectors, FP64 reductions,
+ * M = the network (precond LEARNED; d_feat is its second input channel) or
+ * Jacobi; after o->stall_iters (default 10) iterations without a new best
+ * residual the learned preconditioner is replaced by Jacobi (res reports
+ * the preconditioner of the final iterations).                              */
+nnx_status nnx_fcg_solve(nnx_ctx* ctx, const nnx_grid* g, nnx_unet* net, const float* d_feat,
+                         const float* d_b, float* d_x, const nnx_cg_opts* o, nnx_cg_result* res);
+/* FP64 iterative refinement around nnx_fcg_solve (as nnx_cg_solve_refine).  */
+nnx_status nnx_fcg_solve_refine(nnx_ctx* ctx, const nnx_grid* g, nnx_unet* net, const float* d_feat,
+                                const double* d_b, double* d_x, const nnx_cg_opts* o,
+                                double inner_rtol, int max_outer, nnx_cg_result* res);
+
+/* ------------------------------------------------------------------------ */
 /* composite timestep                                                        */
 /* ------------------------------------------------------------------------ */
 

@@ -94,6 +94,17 @@ int64_t nnx_trilinear_matrix(int64_t n, const float* x, const float* y, const fl
 void nnx_csr_transpose(int64_t rows, int64_t cols, const int32_t* off, const int32_t* idx,
                        const float* val, int32_t* t_off, int32_t* t_idx, float* t_val);
 
+/* Gaussian random field, unit variance, covariance exp(-r^2 / (2 corr^2))
+ * (lengths in voxels): white noise from the Philox stream (seed, purpose)
+ * filtered by a separable Gaussian kernel.                                 */
+void nnx_gen_grf(uint64_t seed, uint64_t purpose, int32_t nx, int32_t ny, int32_t nz, double corr,
+                 float* out);
+
+/* Face coefficients (nnx_grid kx/ky/kz layout) from per-voxel coefficients:
+ * harmonic mean across interior faces, the voxel's value on boundary faces. */
+void nnx_face_coefficients(int32_t nx, int32_t ny, int32_t nz, const float* c, float* kx, float* ky,
+                           float* kz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,6 +129,21 @@ _SIGS = {
     "nnx_rhs_from_counts": ([vp, C.c_int64, vp, vp, C.c_double, vp], C.c_int),
     "nnx_csr_spmv": ([vp, C.c_int64, vp, vp, vp, vp, vp, C.c_int], C.c_int),
     "nnx_run_steps": ([vp, _P(StepPlan), C.c_uint64, C.c_int, C.c_int], C.c_int),
+    "nnx_unet_param_count": ([], C.c_int64),
+    "nnx_unet_create": ([vp, vp, C.c_uint64, _P(vp)], C.c_int),
+    "nnx_unet_destroy": ([vp, vp], C.c_int),
+    "nnx_unet_get_params": ([vp, vp, vp], C.c_int),
+    "nnx_unet_get_grad": ([vp, vp, vp], C.c_int),
+    "nnx_unet_apply": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
+    "nnx_unet_train_step": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_double,
+                             _P(C.c_double)], C.c_int),
+    "nnx_conv3_bf16": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
+                        C.c_double, vp, vp, C.c_int, vp, vp], C.c_int),
+    "nnx_conv3_wgrad_bf16": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
+                              C.c_double, vp, vp, vp], C.c_int),
+    "nnx_fcg_solve": ([vp, _P(Grid), vp, vp, vp, vp, _P(CgOpts), _P(CgResult)], C.c_int),
+    "nnx_fcg_solve_refine": ([vp, _P(Grid), vp, vp, vp, vp, _P(CgOpts), C.c_double, C.c_int,
+                              _P(CgResult)], C.c_int),
 }
 
 _HOST_SIGS = {
@@ -149,6 +164,9 @@ _HOST_SIGS = {
     "nnx_trilinear_matrix": ([C.c_int64, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                               vp, vp, vp], C.c_int64),
     "nnx_csr_transpose": ([C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp], None),
+    "nnx_gen_grf": ([C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_double, vp],
+                    None),
+    "nnx_face_coefficients": ([C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp], None),
 }
 
 _lib = None

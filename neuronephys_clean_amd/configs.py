@@ -29,6 +29,7 @@ from . import host
 # Philox purposes (one stream per kind of draw)
 P_V0, P_IEXT, P_W, P_POS, P_MU, P_NSIZE, P_GRIDSRC = 101, 102, 103, 104, 105, 106, 107
 P_C3SRC = 108
+P_DFIELD, P_PCSRC = 109, 110
 
 SQUID_VSHIFT = 5.0
 
@@ -110,6 +111,43 @@ def c3_sources(seed: int = 0, n: int = 512, count: int = 50_000) -> np.ndarray:
 
 
 C3_GRID, C3_C = 512, 20.0
+
+
+@dataclass
+class DiffusionField:
+    """A variable-diffusivity backward-Euler operator (I - dt div D grad) on
+    an n^3 grid: per-voxel c = dt D / h^2, face coefficients and the
+    network's feature channel (normalised log c)."""
+    n: int
+    c: np.ndarray
+    kx: np.ndarray
+    ky: np.ndarray
+    kz: np.ndarray
+    feat: np.ndarray
+
+
+def c5_field(seed: int = 0, n: int = 256, corr: float = 8.0, sigma: float = 1.0,
+             cbar: float = 50.0, purpose: int = P_DFIELD) -> DiffusionField:
+    """log D = sigma * (unit Gaussian random field, correlation length corr
+    voxels); scaled so that the mean of dt D / h^2 is cbar; faces by
+    harmonic mean."""
+    g = host.grf(seed, purpose, n, n, n, corr)
+    d = np.exp(sigma * g.astype(np.float64))
+    c = (cbar * d / d.mean()).astype(np.float32)
+    kx, ky, kz = host.face_coefficients(c)
+    lc = np.log(c.astype(np.float64))
+    feat = ((lc - lc.mean()) / max(lc.std(), 1e-12)).astype(np.float32)
+    return DiffusionField(n, c, kx, ky, kz, feat)
+
+
+def point_sources(seed: int, n: int, count: int, purpose: int = P_PCSRC) -> np.ndarray:
+    """`count` releases of random amplitude U(0.5, 1.5) at uniformly random
+    voxels of an n^3 grid (float32, x fastest)."""
+    u = host.uniform(seed, purpose, 2 * count, 0.0, 1.0)
+    idx = np.minimum((u[:count] * float(n) ** 3).astype(np.int64), n ** 3 - 1)
+    b = np.zeros(n ** 3, np.float64)
+    np.add.at(b, idx, 0.5 + u[count:])
+    return b.astype(np.float32)
 
 
 def c4_sizes(seed: int = 0, count: int = 256):

@@ -112,3 +112,7 @@ nnx_status hh_rk4_event_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_
 nnx_status event_scratch(nnx_ctx* c, int64_t n, int64_t nnz, int32_t** cnt, int32_t** slot);
 
 }  // namespace nnx
+
+// z = M(r) of the U-Net preconditioner (unet.hip); dims already checked
+nnx_status nnx_unet_forward_internal(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch,
+                                     const float* r, const float* feat, float* z);

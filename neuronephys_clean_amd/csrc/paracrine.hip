@@ -689,6 +689,102 @@ __global__ __launch_bounds__(256) void k_ref_init(GridK g, const float* __restri
     bb[id] = c_b * u + d_b * lap;
 }
 
+// ---- flexible CG (Notay's FCG with one stored direction) ----
+// With a preconditioner M that changes from one application to the next:
+//   z = M(r); p = z - (z.q_old / p_old.q_old) p_old; q = A p;
+//   alpha = p.r / p.q; x += alpha p; r -= alpha q.
+// FCG partials: zq (kRedBlocks), (pq, pr) x 2 (ping-pong), rr, bb.
+struct FcgPart {
+    double *zq, *pq0, *pq1, *rr, *bb;
+};
+
+__global__ __launch_bounds__(256) void k_fcg_init(GridK g, const float* __restrict__ b, float* __restrict__ x,
+                                                  int use_x0, float* __restrict__ r, FcgPart P) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    double vb[1] = {0}, vr[1] = {0};
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float bi = b[i];
+        float ri = bi;
+        if (use_x0) {
+            const int ii = (int)(i % g.nx), jj = (int)((i / g.nx) % g.ny), kk = (int)(i / ((int64_t)g.nx * g.ny));
+            ri = bi - apply_at<float, float>(x, g, ii, jj, kk);
+        }
+        r[i] = ri;
+        vb[0] += (double)bi * bi;
+        vr[0] += (double)ri * ri;
+    }
+    put_partial<1>(vb, P.bb);
+    put_partial<1>(vr, P.rr);
+    if (!use_x0)
+        for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) x[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_fcg_jacobi(int64_t n, const float* __restrict__ dinv,
+                                                    const float* __restrict__ r, float* __restrict__ z) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) z[i] = dinv[i] * r[i];
+}
+
+__global__ __launch_bounds__(256) void k_fcg_dot(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
+                                                 double* part) {
+    double v[1] = {0};
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) v[0] += (double)a[i] * b[i];
+    put_partial<1>(v, part);
+}
+
+// p_new = z + beta p_old, beta = -zq / pq_old (0 on a restart)
+__global__ __launch_bounds__(256) void k_fcg_dir(int64_t n, const float* __restrict__ z,
+                                                 const float* __restrict__ pold, float* __restrict__ pnew,
+                                                 const double* __restrict__ pzq, const double* __restrict__ ppq_old,
+                                                 int restart) {
+    float beta = 0.0f;
+    if (!restart) {
+        double a[1], b[2];
+        sum_partials<1>(pzq, kRedBlocks, a);
+        sum_partials<2>(ppq_old, kRedBlocks, b);
+        beta = (float)(-a[0] / b[0]);
+    }
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        pnew[i] = restart ? z[i] : z[i] + beta * pold[i];
+}
+
+// q = A p; partials (p.q, p.r)
+__global__ __launch_bounds__(256) void k_fcg_spmv(GridK g, const float* __restrict__ p, const float* __restrict__ r,
+                                                  float* __restrict__ q, double* ppq) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    double v[2] = {0, 0};
+    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
+        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), kk = (int)(id / ((int64_t)g.nx * g.ny));
+        const float qv = apply_at<float, float>(p, g, i, j, kk);
+        q[id] = qv;
+        v[0] += (double)p[id] * qv;
+        v[1] += (double)p[id] * r[id];
+    }
+    put_partial<2>(v, ppq);
+}
+
+// alpha = pr / pq; x += alpha p; r -= alpha q; partial rr
+__global__ __launch_bounds__(256) void k_fcg_update(int64_t n, const double* __restrict__ ppq,
+                                                    const float* __restrict__ p, const float* __restrict__ q,
+                                                    float* __restrict__ x, float* __restrict__ r, double* prr) {
+    double t[2];
+    sum_partials<2>(ppq, kRedBlocks, t);
+    const float alpha = (float)(t[1] / t[0]);
+    double v[1] = {0};
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        x[i] = x[i] + alpha * p[i];
+        const float ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        v[0] += (double)ri * ri;
+    }
+    put_partial<1>(v, prr);
+}
+
+__global__ __launch_bounds__(256) void k_fcg_sum(const double* part, double* out) {
+    double t[1];
+    sum_partials<1>(part, kRedBlocks, t);
+    if (threadIdx.x == 0) *out = t[0];
+}
+
 // ---- deposit / rhs / spmv ----
 __global__ void k_deposit(const int32_t* __restrict__ voxel_of, const int32_t* __restrict__ list,
                           const int32_t* __restrict__ count, int32_t* __restrict__ vc) {
@@ -814,6 +910,177 @@ nnx_status nnx_cg_solve_refine(nnx_ctx* c, const nnx_grid* g, const double* b, d
     out.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (res) *res = out;
     return out.converged ? NNX_OK : fail(c, NNX_ERR_NOCONV, "refinement: %d outer steps, relres %.3e", outer, rel);
+}
+
+} // extern "C"
+
+namespace {
+
+double read_scalar(nnx_ctx* c, const double* d, hipError_t* err) {
+    *err = hipMemcpyAsync(c->pinned, d, 8, hipMemcpyDeviceToHost, c->stream);
+    if (*err == hipSuccess) *err = hipStreamSynchronize(c->stream);
+    double v = 0;
+    std::memcpy(&v, c->pinned, 8);
+    return v;
+}
+
+// FCG on A x = b (fp32 vectors, FP64 reductions), M = the U-Net when net is
+// set, Jacobi otherwise or after stall_iters iterations without a new best
+// residual.  The host reads the residual norm after every iteration.
+nnx_status fcg_run(nnx_ctx* c, const GridK& gk, nnx_unet* net, const float* feat, const float* b, float* x,
+                   int use_x0, double rtol, int maxit, int stall_iters, nnx_cg_result* res) {
+    const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
+    void *r, *z, *pa, *pb, *qa, *qb, *dinv, *part, *sc;
+    NNX_TRY(scratch(c, "fcg_r", n * 4, false, &r));
+    NNX_TRY(scratch(c, "fcg_z", n * 4, false, &z));
+    NNX_TRY(scratch(c, "fcg_pa", n * 4, false, &pa));
+    NNX_TRY(scratch(c, "fcg_pb", n * 4, false, &pb));
+    NNX_TRY(scratch(c, "fcg_qa", n * 4, false, &qa));
+    NNX_TRY(scratch(c, "fcg_qb", n * 4, false, &qb));
+    NNX_TRY(scratch(c, "fcg_dinv", n * 4, false, &dinv));
+    NNX_TRY(scratch(c, "fcg_part", (size_t)kRedBlocks * 8 * 8, false, &part));
+    NNX_TRY(scratch(c, "fcg_sc", 64, false, &sc));
+    double* Pd = (double*)part;
+    FcgPart P{Pd, Pd + kRedBlocks, Pd + 3 * kRedBlocks, Pd + 5 * kRedBlocks, Pd + 6 * kRedBlocks};
+    float *R = (float*)r, *Z = (float*)z;
+    float* pbuf[2] = {(float*)pa, (float*)pb};
+    float* qbuf[2] = {(float*)qa, (float*)qb};
+    double* pq[2] = {P.pq0, P.pq1};
+    double* S = (double*)sc;
+    k_dinv<<<flat_blocks(n), 256, 0, c->stream>>>(gk, (float*)dinv);
+    k_fcg_init<<<kRedBlocks, 256, 0, c->stream>>>(gk, b, x, use_x0, R, P);
+    k_fcg_sum<<<1, 256, 0, c->stream>>>(P.bb, S);
+    k_fcg_sum<<<1, 256, 0, c->stream>>>(P.rr, S + 1);
+    NNX_LAUNCHED(c);
+    hipError_t e;
+    NNX_HIP(c, hipMemcpyAsync(c->pinned, S, 16, hipMemcpyDeviceToHost, c->stream));
+    NNX_HIP(c, hipStreamSynchronize(c->stream));
+    double bb, rr;
+    std::memcpy(&bb, c->pinned, 8);
+    std::memcpy(&rr, (char*)c->pinned + 8, 8);
+    const double bnorm = std::sqrt(bb > 0 ? bb : 1.0);
+    double rel = std::sqrt(rr) / bnorm, best = rel;
+    int since_best = 0, it = 0, learned = net != nullptr, restart = 1;
+    if (stall_iters <= 0) stall_iters = 10;
+    while (rel > rtol && it < maxit && rr > 0) {
+        const int cur = it & 1, old = cur ^ 1;
+        if (learned) {
+            NNX_TRY(nnx_unet_forward_internal(c, net, gk.nx, gk.ny, gk.nz, 1, R, feat, Z));
+        } else {
+            k_fcg_jacobi<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const float*)dinv, R, Z);
+        }
+        if (!restart) k_fcg_dot<<<kRedBlocks, 256, 0, c->stream>>>(n, Z, qbuf[old], P.zq);
+        k_fcg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, Z, pbuf[old], pbuf[cur], P.zq, pq[old], restart);
+        k_fcg_spmv<<<kRedBlocks, 256, 0, c->stream>>>(gk, pbuf[cur], R, qbuf[cur], pq[cur]);
+        k_fcg_update<<<kRedBlocks, 256, 0, c->stream>>>(n, pq[cur], pbuf[cur], qbuf[cur], x, R, P.rr);
+        k_fcg_sum<<<1, 256, 0, c->stream>>>(P.rr, S + 1);
+        NNX_LAUNCHED(c);
+        rr = read_scalar(c, S + 1, &e);
+        if (e != hipSuccess) return fail(c, NNX_ERR_HIP, "FCG read-back: %s", hipGetErrorString(e));
+        ++it;
+        restart = 0;
+        rel = std::sqrt(rr) / bnorm;
+        if (!(rel == rel)) return fail(c, NNX_ERR_NONFINITE, "FCG: non-finite residual at iteration %d", it);
+        if (rel < best) {
+            best = rel;
+            since_best = 0;
+        } else if (++since_best >= stall_iters && learned) {
+            learned = 0;  // fall back to Jacobi and restart the direction
+            restart = 1;
+            since_best = 0;
+        }
+    }
+    if (res) {
+        res->iters = it;
+        res->relres = rel;
+        res->converged = rel <= rtol;
+        res->precond_used = learned ? NNX_PRECOND_LEARNED : NNX_PRECOND_JACOBI;
+    }
+    return NNX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+nnx_status nnx_fcg_solve(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, const float* d_feat, const float* b,
+                         float* x, const nnx_cg_opts* o, nnx_cg_result* res) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_grid(c, g));
+    NNX_CHECK(c, b && x && o, "null b, x or opts");
+    NNX_CHECK(c, o->rtol >= 0 && o->maxit >= 0, "bad rtol/maxit");
+    NNX_CHECK(c, o->precond == NNX_PRECOND_LEARNED || o->precond == NNX_PRECOND_JACOBI, "FCG precond: LEARNED or JACOBI");
+    NNX_CHECK(c, o->precond != NNX_PRECOND_LEARNED || net, "learned preconditioner needs a network");
+    if (o->precond == NNX_PRECOND_LEARNED)
+        NNX_CHECK(c, g->nx % 64 == 0 && g->ny % 32 == 0 && g->nz % 32 == 0,
+                  "learned preconditioner needs nx %% 64 == 0 and ny, nz %% 32 == 0");
+    const GridK gk = make_grid(g);
+    auto t0 = std::chrono::steady_clock::now();
+    nnx_cg_result r{};
+    NNX_TRY(fcg_run(c, gk, o->precond == NNX_PRECOND_LEARNED ? net : nullptr, d_feat, b, x, o->use_x0, o->rtol,
+                    o->maxit, o->stall_iters, &r));
+    auto t1 = std::chrono::steady_clock::now();
+    CgHost h;
+    NNX_TRY(cg_buffers(c, gk, NNX_PRECOND_NONE, &h));
+    NNX_TRY((true_residual<float, float>(c, gk, h, x, b, &r.true_relres, nullptr, nullptr)));
+    r.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (res) *res = r;
+    return r.converged ? NNX_OK : fail(c, NNX_ERR_NOCONV, "FCG: %d iterations, relres %.3e", r.iters, r.relres);
+}
+
+nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, const float* d_feat,
+                                const double* b, double* x, const nnx_cg_opts* o, double inner_rtol,
+                                int max_outer, nnx_cg_result* res) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_grid(c, g));
+    NNX_CHECK(c, b && x && o, "null b, x or opts");
+    NNX_CHECK(c, o->precond == NNX_PRECOND_LEARNED || o->precond == NNX_PRECOND_JACOBI, "FCG precond: LEARNED or JACOBI");
+    NNX_CHECK(c, o->precond != NNX_PRECOND_LEARNED || net, "learned preconditioner needs a network");
+    if (o->precond == NNX_PRECOND_LEARNED)
+        NNX_CHECK(c, g->nx % 64 == 0 && g->ny % 32 == 0 && g->nz % 32 == 0,
+                  "learned preconditioner needs nx %% 64 == 0 and ny, nz %% 32 == 0");
+    const GridK gk = make_grid(g);
+    const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
+    auto t0 = std::chrono::steady_clock::now();
+    CgHost h;
+    NNX_TRY(cg_buffers(c, gk, NNX_PRECOND_NONE, &h));
+    void *r64, *rhs, *d;
+    NNX_TRY(scratch(c, "ir_r64", n * 8, false, &r64));
+    NNX_TRY(scratch(c, "ir_rhs", n * 4, false, &rhs));
+    NNX_TRY(scratch(c, "ir_d", n * 4, false, &d));
+    if (!o->use_x0) NNX_HIP(c, hipMemsetAsync(x, 0, n * 8, c->stream));
+    nnx_cg_result out{};
+    int outer = 0, inner_total = 0, used = o->precond;
+    double rel = 0;
+    for (;;) {
+        double rb[2];
+        NNX_TRY((true_residual<double, double>(c, gk, h, x, b, nullptr, rb, (double*)r64)));
+        const double rr = rb[0], bb = rb[1];
+        rel = std::sqrt(rr / (bb > 0 ? bb : 1.0));
+        if (rel <= o->rtol || outer >= max_outer || rr == 0) break;
+        const double s = 1.0 / std::sqrt(rr);
+        k_f64_to_f32_scaled<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const double*)r64, s, (float*)rhs);
+        NNX_LAUNCHED(c);
+        const double tol = std::max(inner_rtol, 0.5 * o->rtol / rel);
+        nnx_cg_result ir{};
+        NNX_TRY(fcg_run(c, gk, o->precond == NNX_PRECOND_LEARNED ? net : nullptr, d_feat, (const float*)rhs,
+                        (float*)d, 0, tol, o->maxit, o->stall_iters, &ir));
+        inner_total += ir.iters;
+        if (ir.precond_used == NNX_PRECOND_JACOBI) used = NNX_PRECOND_JACOBI;
+        k_axpy_f64<<<flat_blocks(n), 256, 0, c->stream>>>(n, 1.0 / s, (const float*)d, x);
+        NNX_LAUNCHED(c);
+        ++outer;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    out.iters = inner_total;
+    out.outer = outer;
+    out.relres = rel;
+    out.true_relres = rel;
+    out.converged = rel <= o->rtol;
+    out.precond_used = used;
+    out.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (res) *res = out;
+    return out.converged ? NNX_OK : fail(c, NNX_ERR_NOCONV, "FCG refinement: %d outer steps, relres %.3e", outer, rel);
 }
 
 nnx_status nnx_update_density_reference(nnx_ctx* c, int nx, int ny, int nz, double h, double D,

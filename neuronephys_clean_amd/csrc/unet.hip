@@ -1,0 +1,984 @@
+// unet.hip - the learned preconditioner: a 3-level 3-D convolutional U-Net
+// evaluated and trained with BF16 MFMA implicit-GEMM kernels (FP32
+// accumulation), used as the variable preconditioner of flexible CG.
+// No counterpart in the reference (its paracrine solve is plain CG,
+// src/paracrine.cu:106-129).
+//
+// Network (x = [r / rms(r), feature], channels-last BF16 activations):
+//   e0 = relu(conv3(x; 2->16))            level 0    p0 = avgpool2(e0)
+//   e1 = relu(conv3(p0; 16->32))          level 1    p1 = avgpool2(e1)
+//   e2 = relu(conv3(p1; 32->32))          level 2
+//   d1 = relu(conv3(e1 + up(e2); 32->16)) level 1    (up: nearest, x2)
+//   d0 = relu(conv3(e0 + up(d1); 16->16)) level 0
+//   z  = rms(r) * (w_h . d0 + b_h)        1x1 head, fp32
+// conv3 is a 3x3x3 convolution with zero padding.  63,201 parameters.
+//
+// Convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16: D[co][voxel] +=
+// W[co][k] * X[k][voxel], k = tap*Cin + ci (taps padded to a multiple of 32).
+// A workgroup owns a 16 x TY x TZ output tile; the input tile with its
+// one-voxel halo and the whole weight matrix sit in LDS; each wave keeps
+// the accumulators of its output rows in registers and reuses every weight
+// fragment across them.  Pooling, the skip addition with upsampling, ReLU,
+// bias and the 1x1 head are fused into the staging and epilogue.
+//
+// Backward: data gradients are the same kernel with the taps flipped and
+// the channel roles exchanged; weight gradients are a second MFMA GEMM
+// (D[co][k] += G[co][voxel] * X[voxel][k], K running over voxels) with
+// per-workgroup partials reduced in a fixed order.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace nnx {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float bf2f(uint32_t b16) { return __uint_as_float(b16 << 16); }
+// round to nearest even (finite inputs)
+__device__ __forceinline__ uint32_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ f32x4_t mfma16(u32x4_t a, u32x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                   __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
+constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
+__device__ __forceinline__ int tap_dz(int t) { return t / 9; }
+__device__ __forceinline__ int tap_dy(int t) { return (t / 3) % 3; }
+__device__ __forceinline__ int tap_dx(int t) { return t % 3; }
+
+// 8 bf16 of a + s * c, rounded once
+__device__ __forceinline__ u32x4_t add_scaled8(u32x4_t a, u32x4_t c, float s) {
+    u32x4_t o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float lo = bf2f(a[q] & 0xffffu) + s * bf2f(c[q] & 0xffffu);
+        const float hi = bf2f(a[q] >> 16) + s * bf2f(c[q] >> 16);
+        o[q] = f2bf(lo) | (f2bf(hi) << 16);
+    }
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// forward convolution
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+    int nx, ny, nz, batch;
+    const uint16_t* in;      // [b][z][y][x][CIN]
+    const uint16_t* coarse;  // optional: in + cscale * coarse[parent voxel] (half resolution)
+    float cscale;
+    const uint16_t* w;       // [COUT][kpad(CIN)], zero padded
+    const float* bias;       // [COUT] or null
+    int relu;
+    uint16_t* out;           // [b][z][y][x][COUT] or null
+    uint16_t* pool;          // 2x2x2 average of the activated output, or null
+    const float* head;       // [COUT] weights + bias of the 1x1 head, or null
+    const float* hscale;     // [batch] output scale of the head, or null (1)
+    float* hout;             // [b][z][y][x] fp32 head output
+};
+
+template <int CIN> struct ConvTile { static constexpr int T = CIN == 32 ? 4 : 8; };
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
+    constexpr int TY = ConvTile<CIN>::T, TZ = TY;
+    constexpr int KP = kpad(CIN), NKB = KP / 32, KS = KP + 8;
+    constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
+    constexpr int MT = COUT / 16;
+    constexpr int QPW = (TY / 2) * (TZ / 2) / 4;  // 2x2 (y,z) row quads per wave
+    constexpr int RPW = 4 * QPW;
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[HV * CIN];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[COUT * KS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, xl = lane & 15;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int nzt = nz / TZ;
+    const int b = blockIdx.z / nzt, z0 = (blockIdx.z % nzt) * TZ, y0 = blockIdx.y * TY,
+              x0 = blockIdx.x * 16;
+    for (int i = tid; i < COUT * (KP / 8); i += 256) {
+        const int co = i / (KP / 8), k8 = i - co * (KP / 8);
+        *(u32x4_t*)&s_w[co * KS + 8 * k8] = *(const u32x4_t*)&a.w[(size_t)co * KP + 8 * k8];
+    }
+    if (CIN == 2) {
+        for (int hv = tid; hv < HV; hv += 256) {
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            uint32_t v = 0;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+                v = *(const uint32_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * 2];
+            *(uint32_t*)&s_in[hv * 2] = v;
+        }
+    } else {
+        constexpr int CH = CIN / 8;
+        for (int i = tid; i < HV * CH; i += 256) {
+            const int hv = i / CH, c8 = i - hv * CH;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            u32x4_t v = {0u, 0u, 0u, 0u};
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
+                v = *(const u32x4_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * CIN + 8 * c8];
+                if (a.coarse) {
+                    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+                    const u32x4_t cv = *(const u32x4_t*)&a.coarse
+                        [((((size_t)b * cz + (gz >> 1)) * cy + (gy >> 1)) * cx + (gx >> 1)) * CIN + 8 * c8];
+                    v = add_scaled8(v, cv, a.cscale);
+                }
+            }
+            *(u32x4_t*)&s_in[hv * CIN + 8 * c8] = v;
+        }
+    }
+    __syncthreads();
+    f32x4_t acc[MT][RPW];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) acc[mt][r] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < NKB; ++kb) {
+        u32x4_t af[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) af[mt] = *(const u32x4_t*)&s_w[(16 * mt + xl) * KS + 32 * kb + 8 * g];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int qd = wave * QPW + (r >> 2), t = r & 3;
+            const int ry = 2 * (qd % (TY / 2)) + (t & 1), rz = 2 * (qd / (TY / 2)) + (t >> 1);
+            u32x4_t bv;
+            if (CIN == 2) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    int tap = 16 * kb + 4 * g + u;
+                    tap = tap > 26 ? 26 : tap;  // padded k: zero weights
+                    bv[u] = *(const uint32_t*)&s_in[(((rz + tap_dz(tap)) * HY + ry + tap_dy(tap)) * HX + xl +
+                                                     tap_dx(tap)) * 2];
+                }
+            } else {
+                int tap = CIN == 16 ? 2 * kb + (g >> 1) : kb;
+                tap = tap > 26 ? 26 : tap;
+                const int coff = CIN == 16 ? 8 * (g & 1) : 8 * g;
+                bv = *(const u32x4_t*)&s_in[(((rz + tap_dz(tap)) * HY + ry + tap_dy(tap)) * HX + xl +
+                                             tap_dx(tap)) * CIN + coff];
+            }
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[mt][r] = mfma16(af[mt], bv, acc[mt][r]);
+        }
+    }
+    // epilogue: lane holds voxel x0 + xl, channels 16 mt + 4 g + q
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int qd = wave * QPW + (r >> 2), t = r & 3;
+        const int y = y0 + 2 * (qd % (TY / 2)) + (t & 1), z = z0 + 2 * (qd / (TY / 2)) + (t >> 1);
+        const size_t vox = (((size_t)b * nz + z) * ny + y) * nx + x0 + xl;
+        float hsum = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            f32x4_t v = acc[mt][r];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int co = 16 * mt + 4 * g + q;
+                float s = v[q] + (a.bias ? a.bias[co] : 0.f);
+                if (a.relu) s = fmaxf(s, 0.f);
+                v[q] = s;
+                if (a.head) hsum += a.head[co] * s;
+            }
+            acc[mt][r] = v;
+            if (a.out) {
+                u32x2_t o;
+                o[0] = f2bf(v[0]) | (f2bf(v[1]) << 16);
+                o[1] = f2bf(v[2]) | (f2bf(v[3]) << 16);
+                *(u32x2_t*)&a.out[vox * COUT + 16 * mt + 4 * g] = o;
+            }
+        }
+        if (a.head) {
+            hsum += __shfl_xor(hsum, 16, 64);
+            hsum += __shfl_xor(hsum, 32, 64);
+            if (g == 0) a.hout[vox] = (a.hscale ? a.hscale[b] : 1.f) * (hsum + a.head[COUT]);
+        }
+    }
+    if (a.pool) {
+        const int px = nx >> 1, py = ny >> 1, pz = nz >> 1;
+#pragma unroll
+        for (int j = 0; j < QPW; ++j) {
+            const int qd = wave * QPW + j;
+            const int y = (y0 >> 1) + qd % (TY / 2), z = (z0 >> 1) + qd / (TY / 2);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                f32x4_t s = ((acc[mt][4 * j] + acc[mt][4 * j + 1]) + acc[mt][4 * j + 2]) + acc[mt][4 * j + 3];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) s[q] = (s[q] + __shfl_xor(s[q], 1, 64)) * 0.125f;
+                if ((xl & 1) == 0) {
+                    u32x2_t o;
+                    o[0] = f2bf(s[0]) | (f2bf(s[1]) << 16);
+                    o[1] = f2bf(s[2]) | (f2bf(s[3]) << 16);
+                    const size_t pv = (((size_t)b * pz + z) * py + y) * px + ((x0 + xl) >> 1);
+                    *(u32x2_t*)&a.pool[pv * COUT + 16 * mt + 4 * g] = o;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dW[co][tap*CIN + ci] = sum_v G[v][co] X[v + off(tap)][ci]
+// ---------------------------------------------------------------------------
+struct WgArgs {
+    int nx, ny, nz, batch, ntiles;
+    const uint16_t* in;
+    const uint16_t* coarse;
+    float cscale;
+    const uint16_t* g;  // [b][z][y][x][COUT]
+    float* part;        // [gridDim.x][COUT][kpad(CIN)]
+    float* bpart;       // [gridDim.x][COUT]
+};
+
+constexpr int kWgRows = 264;  // LDS row stride of the gradient tile (256 voxels + pad)
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad(WgArgs a) {
+    constexpr int KP = kpad(CIN), NT = KP / 16, MT = COUT / 16, NTW = (NT + 3) / 4;
+    constexpr int PLANE = 6 * 6 * 16 + 8;  // one (dx, ci) copy of the 18x6x6 tile, x shifted by dx
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[3 * CIN * PLANE];
+    __shared__ __attribute__((aligned(16))) uint16_t s_g[COUT * kWgRows];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, xl = lane & 15;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int tx = nx / 16, ty = ny / 4, tz = nz / 4;
+    f32x4_t acc[NTW][MT];
+#pragma unroll
+    for (int j = 0; j < NTW; ++j)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[j][mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        const int bx = tile % tx, by = (tile / tx) % ty, rest = tile / (tx * ty);
+        const int bz = rest % tz, b = rest / tz;
+        const int x0 = 16 * bx, y0 = 4 * by, z0 = 4 * bz;
+        __syncthreads();
+        constexpr int CH = CIN == 2 ? 1 : CIN / 8;
+        for (int i = tid; i < 18 * 36 * CH; i += 256) {
+            const int hv = i / CH, c8 = i - hv * CH;
+            const int hx = hv % 18, hy = (hv / 18) % 6, hz = hv / 108;
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            const bool in = gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+            const size_t gv = (((size_t)b * nz + gz) * ny + gy) * nx + gx;
+            uint16_t val[8];
+            if (CIN == 2) {
+                const uint32_t v = in ? *(const uint32_t*)&a.in[gv * 2] : 0u;
+                val[0] = (uint16_t)(v & 0xffffu);
+                val[1] = (uint16_t)(v >> 16);
+            } else {
+                u32x4_t v = {0u, 0u, 0u, 0u};
+                if (in) {
+                    v = *(const u32x4_t*)&a.in[gv * CIN + 8 * c8];
+                    if (a.coarse) {
+                        const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+                        const u32x4_t cv = *(const u32x4_t*)&a.coarse
+                            [((((size_t)b * cz + (gz >> 1)) * cy + (gy >> 1)) * cx + (gx >> 1)) * CIN + 8 * c8];
+                        v = add_scaled8(v, cv, a.cscale);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    val[2 * q] = (uint16_t)(v[q] & 0xffffu);
+                    val[2 * q + 1] = (uint16_t)(v[q] >> 16);
+                }
+            }
+            constexpr int NC = CIN == 2 ? 2 : 8;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int xs = hx - dx;
+                if (xs < 0 || xs >= 16) continue;
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    s_in[(dx * CIN + (CIN == 2 ? c : 8 * c8 + c)) * PLANE + (hz * 6 + hy) * 16 + xs] = val[c];
+            }
+        }
+        for (int i = tid; i < 256 * (COUT / 8); i += 256) {
+            const int v = i / (COUT / 8), c8 = i - v * (COUT / 8);
+            const int xs = v % 16, rr = v / 16;
+            const size_t gv = (((size_t)b * nz + z0 + (rr >> 2)) * ny + y0 + (rr & 3)) * nx + x0 + xs;
+            const u32x4_t gg = *(const u32x4_t*)&a.g[gv * COUT + 8 * c8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                s_g[(8 * c8 + 2 * q) * kWgRows + v] = (uint16_t)(gg[q] & 0xffffu);
+                s_g[(8 * c8 + 2 * q + 1) * kWgRows + v] = (uint16_t)(gg[q] >> 16);
+            }
+        }
+        __syncthreads();
+        if (tid < COUT)
+            for (int v = 0; v < 256; ++v) bsum += bf2f(s_g[tid * kWgRows + v]);
+        for (int ch = 0; ch < 8; ++ch) {  // 32 voxels: rows 2ch, 2ch+1 of the 16 (y,z) rows
+            const int rr = 2 * ch + (g >> 1), xs0 = 8 * (g & 1);
+            const int ry = rr & 3, rz = rr >> 2;
+            u32x4_t af[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) af[mt] = *(const u32x4_t*)&s_g[(16 * mt + xl) * kWgRows + rr * 16 + xs0];
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                const int nt = wave + 4 * j;
+                if (nt >= NT) break;
+                const int n = 16 * nt + xl;
+                int tap = n / CIN;
+                const int ci = n - tap * CIN;
+                tap = tap > 26 ? 26 : tap;  // padded columns are discarded
+                const u32x4_t bv = *(const u32x4_t*)&s_in[(tap_dx(tap) * CIN + ci) * PLANE +
+                                                          ((rz + tap_dz(tap)) * 6 + ry + tap_dy(tap)) * 16 + xs0];
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) acc[j][mt] = mfma16(af[mt], bv, acc[j][mt]);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NTW; ++j) {
+        const int nt = wave + 4 * j;
+        if (nt >= NT) break;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                a.part[((size_t)blockIdx.x * COUT + 16 * mt + 4 * g + q) * KP + 16 * nt + xl] = acc[j][mt][q];
+    }
+    if (tid < COUT) a.bpart[blockIdx.x * COUT + tid] = bsum;
+}
+
+// dW[co][k] (k < 27 CIN) and db[co]: partials added in workgroup order
+__global__ void k_wgrad_reduce(const float* __restrict__ part, const float* __restrict__ bpart, int nwg,
+                               int cout, int kp, int kreal, float* __restrict__ dw, float* __restrict__ db) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < cout * kreal) {
+        const int co = i / kreal, k = i - co * kreal;
+        float s = 0.f;
+        for (int w = 0; w < nwg; ++w) s += part[((size_t)w * cout + co) * kp + k];
+        dw[i] = s;
+    } else if (i < cout * kreal + cout) {
+        const int co = i - cout * kreal;
+        float s = 0.f;
+        for (int w = 0; w < nwg; ++w) s += bpart[w * cout + co];
+        db[co] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weights, elementwise and reduction kernels
+// ---------------------------------------------------------------------------
+// fp32 [cout][27 cin] -> bf16 [cout][kpad(cin)] (flip = 0), or the flipped
+// transpose for the data gradient: bf16 [cin][kpad(cout)] with
+// out[ci][(26 - tap) cout + co] = w[co][tap cin + ci] (flip = 1)
+__global__ void k_prep_w(const float* __restrict__ w, int cin, int cout, int flip, uint16_t* __restrict__ out) {
+    const int ci_out = flip ? cout : cin, rows = flip ? cin : cout;
+    const int kp = kpad(ci_out);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= rows * kp) return;
+    const int row = i / kp, k = i - row * kp;
+    float v = 0.f;
+    if (k < 27 * ci_out) {
+        const int tap = k / ci_out, c = k - tap * ci_out;
+        v = flip ? w[(size_t)c * 27 * cin + (26 - tap) * cin + row] : w[(size_t)row * 27 * cin + k];
+    }
+    out[i] = (uint16_t)f2bf(v);
+}
+
+// per-batch sum of squares of a fp32 field: partials [batch][256]
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ x, int64_t nper, double* __restrict__ part) {
+    __shared__ double sh[4];
+    const int b = blockIdx.y;
+    const float* xb = x + (size_t)b * nper;
+    double s = 0;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nper; i += (int64_t)gridDim.x * 256) s += (double)xb[i] * xb[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+// s[b] = sqrt(sum / nper); inv[b] = 1/s (0 for a zero field); sum[b] kept
+__global__ void k_rms_fin(const double* __restrict__ part, int nblk, int64_t nper, int batch, float* __restrict__ s,
+                          float* __restrict__ inv, double* __restrict__ sum) {
+    const int b = threadIdx.x;
+    if (b >= batch) return;
+    double t = 0;
+    for (int i = 0; i < nblk; ++i) t += part[b * nblk + i];
+    const double r = std::sqrt(t / (double)nper);
+    s[b] = (float)r;
+    inv[b] = r > 0 ? (float)(1.0 / r) : 0.f;
+    if (sum) sum[b] = t;
+}
+
+// x[v] = (bf16(r[v] inv[b]), bf16(feat[v]))
+__global__ void k_pack(const float* __restrict__ r, const float* __restrict__ feat, const float* __restrict__ inv,
+                       int64_t nper, int batch, uint32_t* __restrict__ x) {
+    const int64_t n = nper * batch;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int b = (int)(i / nper);
+        x[i] = f2bf(r[i] * inv[b]) | (f2bf(feat ? feat[i] : 0.f) << 16);
+    }
+}
+
+// G = (act > 0) ? skip + cscale * coarse[parent] : 0, channels C (multiple of 8)
+__global__ void k_grad_combine(int nx, int ny, int nz, int batch, int C, const uint16_t* __restrict__ act,
+                               const uint16_t* __restrict__ skip, const uint16_t* __restrict__ coarse, float cscale,
+                               uint16_t* __restrict__ out) {
+    const int64_t nv = (int64_t)nx * ny * nz * batch, nch = nv * (C / 8);
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nch; i += (int64_t)gridDim.x * 256) {
+        const int64_t v = i / (C / 8);
+        const int c8 = (int)(i - v * (C / 8));
+        u32x4_t s = skip ? *(const u32x4_t*)&skip[v * C + 8 * c8] : u32x4_t{0u, 0u, 0u, 0u};
+        if (coarse) {
+            const int x = (int)(v % nx), y = (int)((v / nx) % ny);
+            const int64_t zb = v / ((int64_t)nx * ny);
+            const int z = (int)(zb % nz), b = (int)(zb / nz);
+            const size_t pv = (((size_t)b * (nz >> 1) + (z >> 1)) * (ny >> 1) + (y >> 1)) * (nx >> 1) + (x >> 1);
+            s = add_scaled8(s, *(const u32x4_t*)&coarse[pv * C + 8 * c8], cscale);
+        }
+        const u32x4_t m = *(const u32x4_t*)&act[v * C + 8 * c8];
+        u32x4_t o;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t lo = bf2f(m[q] & 0xffffu) > 0.f ? (s[q] & 0xffffu) : 0u;
+            const uint32_t hi = bf2f(m[q] >> 16) > 0.f ? (s[q] >> 16) : 0u;
+            o[q] = lo | (hi << 16);
+        }
+        *(u32x4_t*)&out[v * C + 8 * c8] = o;
+    }
+}
+
+// out[u] = sum of the 8 children of coarse voxel u (adjoint of nearest upsampling)
+__global__ void k_sumpool8(int cx, int cy, int cz, int batch, int C, const uint16_t* __restrict__ in,
+                           uint16_t* __restrict__ out) {
+    const int64_t nv = (int64_t)cx * cy * cz * batch;
+    const int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nv * C; i += (int64_t)gridDim.x * 256) {
+        const int64_t u = i / C;
+        const int c = (int)(i - u * C);
+        const int x = (int)(u % cx), y = (int)((u / cx) % cy);
+        const int64_t zb = u / ((int64_t)cx * cy);
+        const int z = (int)(zb % cz), b = (int)(zb / cz);
+        float s = 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const size_t fv = (((size_t)b * fz + 2 * z + (d >> 2)) * fy + 2 * y + ((d >> 1) & 1)) * fx + 2 * x + (d & 1);
+            s += bf2f(in[fv * C + c]);
+        }
+        out[i] = (uint16_t)f2bf(s);
+    }
+}
+
+// target t = e / s[b]; per batch T2[b] = sum t^2 (partials [batch][256])
+__global__ __launch_bounds__(256) void k_t2(const float* __restrict__ e, const float* __restrict__ inv, int64_t nper,
+                                            double* __restrict__ part) {
+    __shared__ double sh[4];
+    const int b = blockIdx.y;
+    double s = 0;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nper; i += (int64_t)gridDim.x * 256) {
+        const double t = (double)(e[(size_t)b * nper + i] * inv[b]);
+        s += t * t;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// Loss (1/B) sum_b |z_b - t_b|^2 / T2_b and its gradient through the head:
+// dZ = 2 (z - t) / (B T2_b); G0 = (d0 > 0) hw dZ; partials of dhw, dhb, loss.
+constexpr int kHeadBlocks = 512;
+__global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ z, const float* __restrict__ e,
+                                                  const float* __restrict__ inv, const double* __restrict__ t2,
+                                                  int64_t nper, int batch, const uint16_t* __restrict__ d0,
+                                                  const float* __restrict__ head, uint16_t* __restrict__ g0,
+                                                  double* __restrict__ part) {
+    __shared__ double sh[4][18];
+    double acc[18];
+    for (int q = 0; q < 18; ++q) acc[q] = 0;
+    const int64_t n = nper * batch;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int b = (int)(i / nper);
+        const float t = e[i] * inv[b];
+        const float diff = z[i] - t;
+        const float dz = (float)(2.0 * diff / ((double)batch * t2[b]));
+        acc[17] += (double)diff * diff / t2[b];
+        acc[16] += dz;
+        uint32_t o[8];
+        for (int c8 = 0; c8 < 2; ++c8) {
+            const u32x4_t a4 = *(const u32x4_t*)&d0[i * 16 + 8 * c8];
+            for (int q = 0; q < 4; ++q) {
+                const float lo = bf2f(a4[q] & 0xffffu), hi = bf2f(a4[q] >> 16);
+                const int c = 8 * c8 + 2 * q;
+                acc[c] += (double)lo * dz;
+                acc[c + 1] += (double)hi * dz;
+                const uint32_t glo = lo > 0.f ? f2bf(head[c] * dz) : 0u;
+                const uint32_t ghi = hi > 0.f ? f2bf(head[c + 1] * dz) : 0u;
+                o[4 * c8 + q] = glo | (ghi << 16);
+            }
+        }
+        *(u32x4_t*)&g0[i * 16] = u32x4_t{o[0], o[1], o[2], o[3]};
+        *(u32x4_t*)&g0[i * 16 + 8] = u32x4_t{o[4], o[5], o[6], o[7]};
+    }
+    for (int q = 0; q < 18; ++q) {
+        double s = acc[q];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        acc[q] = s;
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 18; ++q) sh[threadIdx.x >> 6][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 18) {
+        const int q = threadIdx.x;
+        part[blockIdx.x * 18 + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+    }
+}
+// head gradients (dhw[16], dhb) into grad[], loss into loss[0]
+__global__ void k_head_fin(const double* __restrict__ part, int nblk, float* __restrict__ ghead, double* loss,
+                           int batch) {
+    const int q = threadIdx.x;
+    if (q >= 18) return;
+    double s = 0;
+    for (int i = 0; i < nblk; ++i) s += part[i * 18 + q];
+    if (q < 17) ghead[q] = (float)s;
+    else *loss = s / batch;
+}
+
+__global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+}  // namespace nnx
+
+using namespace nnx;
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Layer {
+    int cin, cout;
+    int64_t w_off, b_off;  // offsets in the parameter vector
+};
+// e0, e1, e2, d1, d0, then the head (16 weights + bias)
+const Layer kLayers[5] = {{2, 16, 0, 0}, {16, 32, 0, 0}, {32, 32, 0, 0}, {32, 16, 0, 0}, {16, 16, 0, 0}};
+
+struct Plan {
+    Layer L[5];
+    int64_t head_off, total;
+    Plan() {
+        int64_t o = 0;
+        for (int i = 0; i < 5; ++i) {
+            L[i] = kLayers[i];
+            L[i].w_off = o;
+            o += (int64_t)L[i].cout * 27 * L[i].cin;
+            L[i].b_off = o;
+            o += L[i].cout;
+        }
+        head_off = o;
+        total = o + 17;
+    }
+};
+const Plan& plan() {
+    static Plan p;
+    return p;
+}
+
+}  // namespace
+
+struct nnx_unet {
+    float *p, *grad, *m, *v;     // fp32 parameters, gradient, Adam moments
+    uint16_t* wf[5];             // bf16 forward weights [cout][kpad(cin)]
+    uint16_t* wb[5];             // bf16 flipped weights [cin][kpad(cout)]
+    int64_t step;
+    bool dirty;                  // bf16 copies out of date
+};
+
+namespace {
+
+template <int CIN, int COUT>
+nnx_status launch_conv(nnx_ctx* c, const ConvArgs& a) {
+    constexpr int T = ConvTile<CIN>::T;
+    NNX_CHECK(c, a.nx % 16 == 0 && a.ny % T == 0 && a.nz % T == 0 && a.batch >= 1,
+              "conv %d->%d: dims (%d,%d,%d) must be multiples of (16,%d,%d)", CIN, COUT, a.nx, a.ny, a.nz, T, T);
+    if (a.pool || a.coarse)
+        NNX_CHECK(c, a.nx % 32 == 0 && a.ny % 2 == 0 && a.nz % 2 == 0, "pooled / upsampled conv needs even dims");
+    dim3 grid(a.nx / 16, a.ny / T, (a.nz / T) * a.batch);
+    k_conv3<CIN, COUT><<<grid, 256, 0, c->stream>>>(a);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+nnx_status conv_dispatch(nnx_ctx* c, int cin, int cout, const ConvArgs& a) {
+    if (cin == 2 && cout == 16) return launch_conv<2, 16>(c, a);
+    if (cin == 16 && cout == 16) return launch_conv<16, 16>(c, a);
+    if (cin == 16 && cout == 32) return launch_conv<16, 32>(c, a);
+    if (cin == 32 && cout == 16) return launch_conv<32, 16>(c, a);
+    if (cin == 32 && cout == 32) return launch_conv<32, 32>(c, a);
+    return fail(c, NNX_ERR_UNSUPPORTED, "conv %d->%d not instantiated", cin, cout);
+}
+
+constexpr int kWgMax = 512;
+
+template <int CIN, int COUT>
+nnx_status launch_wgrad(nnx_ctx* c, WgArgs a, float* dw, float* db) {
+    NNX_CHECK(c, a.nx % 16 == 0 && a.ny % 4 == 0 && a.nz % 4 == 0, "wgrad dims must be multiples of (16,4,4)");
+    if (a.coarse) NNX_CHECK(c, a.nx % 32 == 0 && a.ny % 8 == 0 && a.nz % 8 == 0, "upsampled wgrad needs even dims");
+    a.ntiles = (a.nx / 16) * (a.ny / 4) * (a.nz / 4) * a.batch;
+    const int nwg = std::min(a.ntiles, kWgMax);
+    void *part, *bpart;
+    NNX_TRY(scratch(c, "un_wg_part", (size_t)nwg * COUT * kpad(CIN) * 4, false, &part));
+    NNX_TRY(scratch(c, "un_wg_bpart", (size_t)nwg * COUT * 4, false, &bpart));
+    a.part = (float*)part;
+    a.bpart = (float*)bpart;
+    k_wgrad<CIN, COUT><<<nwg, 256, 0, c->stream>>>(a);
+    const int tot = COUT * 27 * CIN + COUT;
+    k_wgrad_reduce<<<ceil_div(tot, 256), 256, 0, c->stream>>>(a.part, a.bpart, nwg, COUT, kpad(CIN), 27 * CIN,
+                                                                 dw, db);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+nnx_status wgrad_dispatch(nnx_ctx* c, int cin, int cout, const WgArgs& a, float* dw, float* db) {
+    if (cin == 2 && cout == 16) return launch_wgrad<2, 16>(c, a, dw, db);
+    if (cin == 16 && cout == 16) return launch_wgrad<16, 16>(c, a, dw, db);
+    if (cin == 16 && cout == 32) return launch_wgrad<16, 32>(c, a, dw, db);
+    if (cin == 32 && cout == 16) return launch_wgrad<32, 16>(c, a, dw, db);
+    if (cin == 32 && cout == 32) return launch_wgrad<32, 32>(c, a, dw, db);
+    return fail(c, NNX_ERR_UNSUPPORTED, "wgrad %d->%d not instantiated", cin, cout);
+}
+
+unsigned eblocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096)); }
+
+nnx_status prep_weights(nnx_ctx* c, nnx_unet* u) {
+    if (!u->dirty) return NNX_OK;
+    const Plan& P = plan();
+    for (int i = 0; i < 5; ++i) {
+        const Layer& L = P.L[i];
+        const int nf = L.cout * kpad(L.cin), nb = L.cin * kpad(L.cout);
+        k_prep_w<<<ceil_div(nf, 256), 256, 0, c->stream>>>(u->p + L.w_off, L.cin, L.cout, 0, u->wf[i]);
+        k_prep_w<<<ceil_div(nb, 256), 256, 0, c->stream>>>(u->p + L.w_off, L.cin, L.cout, 1, u->wb[i]);
+    }
+    NNX_LAUNCHED(c);
+    u->dirty = false;
+    return NNX_OK;
+}
+
+struct Acts {
+    uint16_t *x, *e0, *p0, *e1, *p1, *e2, *d1, *d0;
+    float *s, *inv, *z;
+    double* part;
+};
+
+nnx_status acts(nnx_ctx* c, int64_t n0, int batch, bool train, Acts* A) {
+    const int64_t n1 = n0 / 8, n2 = n0 / 64;
+    void* p;
+    NNX_TRY(scratch(c, "un_x", n0 * 4, false, &p));
+    A->x = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_e0", n0 * 32, false, &p));
+    A->e0 = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_p0", n1 * 32, false, &p));
+    A->p0 = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_e1", n1 * 64, false, &p));
+    A->e1 = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_p1", n2 * 64, false, &p));
+    A->p1 = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_e2", n2 * 64, false, &p));
+    A->e2 = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_d1", n1 * 32, false, &p));
+    A->d1 = (uint16_t*)p;
+    A->d0 = nullptr;
+    A->z = nullptr;
+    if (train) {
+        NNX_TRY(scratch(c, "un_d0", n0 * 32, false, &p));
+        A->d0 = (uint16_t*)p;
+        NNX_TRY(scratch(c, "un_z", n0 * 4, false, &p));
+        A->z = (float*)p;
+    }
+    NNX_TRY(scratch(c, "un_s", 4096, false, &p));
+    A->s = (float*)p;
+    A->inv = (float*)p + 512;
+    NNX_TRY(scratch(c, "un_part", (size_t)std::max(256 * batch, kHeadBlocks * 18) * 8 + 4096, false, &p));
+    A->part = (double*)p;
+    (void)batch;
+    return NNX_OK;
+}
+
+nnx_status check_dims(nnx_ctx* c, int nx, int ny, int nz, int batch) {
+    NNX_CHECK(c, nx > 0 && ny > 0 && nz > 0 && batch >= 1 && batch <= 256, "bad dims or batch");
+    NNX_CHECK(c, nx % 64 == 0 && ny % 32 == 0 && nz % 32 == 0,
+              "U-Net grids need nx %% 64 == 0 and ny, nz %% 32 == 0 (got %d, %d, %d)", nx, ny, nz);
+    NNX_CHECK(c, (int64_t)nx * ny * nz * batch < ((int64_t)1 << 31), "grid too large");
+    return NNX_OK;
+}
+
+// forward pass; z = hscale * head (hscale = rms(r) per batch, or 1 when train)
+nnx_status forward(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* r, const float* feat,
+                   float* z, bool train, Acts& A) {
+    const Plan& P = plan();
+    const int64_t nper = (int64_t)nx * ny * nz, n0 = nper * batch;
+    NNX_TRY(prep_weights(c, u));
+    k_sumsq<<<dim3(256, batch), 256, 0, c->stream>>>(r, nper, A.part);
+    k_rms_fin<<<1, 256, 0, c->stream>>>(A.part, 256, nper, batch, A.s, A.inv, nullptr);
+    k_pack<<<eblocks(n0), 256, 0, c->stream>>>(r, feat, A.inv, nper, batch, (uint32_t*)A.x);
+    NNX_LAUNCHED(c);
+    ConvArgs a{};
+    a.batch = batch;
+    a.relu = 1;
+    // e0 (+ p0)
+    a.nx = nx, a.ny = ny, a.nz = nz;
+    a.in = A.x, a.w = u->wf[0], a.bias = u->p + P.L[0].b_off, a.out = A.e0, a.pool = A.p0;
+    NNX_TRY(conv_dispatch(c, 2, 16, a));
+    // e1 (+ p1)
+    a.nx = nx / 2, a.ny = ny / 2, a.nz = nz / 2;
+    a.in = A.p0, a.w = u->wf[1], a.bias = u->p + P.L[1].b_off, a.out = A.e1, a.pool = A.p1;
+    NNX_TRY(conv_dispatch(c, 16, 32, a));
+    // e2
+    a.nx = nx / 4, a.ny = ny / 4, a.nz = nz / 4;
+    a.in = A.p1, a.w = u->wf[2], a.bias = u->p + P.L[2].b_off, a.out = A.e2, a.pool = nullptr;
+    NNX_TRY(conv_dispatch(c, 32, 32, a));
+    // d1 = relu(conv(e1 + up(e2)))
+    a.nx = nx / 2, a.ny = ny / 2, a.nz = nz / 2;
+    a.in = A.e1, a.coarse = A.e2, a.cscale = 1.f, a.w = u->wf[3], a.bias = u->p + P.L[3].b_off, a.out = A.d1;
+    NNX_TRY(conv_dispatch(c, 32, 16, a));
+    // d0 = relu(conv(e0 + up(d1))) -> head
+    a.nx = nx, a.ny = ny, a.nz = nz;
+    a.in = A.e0, a.coarse = A.d1, a.w = u->wf[4], a.bias = u->p + P.L[4].b_off, a.out = train ? A.d0 : nullptr;
+    a.head = u->p + P.head_off, a.hscale = train ? nullptr : A.s, a.hout = z;
+    NNX_TRY(conv_dispatch(c, 16, 16, a));
+    return NNX_OK;
+}
+
+}  // namespace
+
+nnx_status nnx_unet_forward_internal(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* r,
+                                     const float* feat, float* z) {
+    Acts A;
+    NNX_TRY(acts(c, (int64_t)nx * ny * nz * batch, batch, false, &A));
+    return forward(c, u, nx, ny, nz, batch, r, feat, z, false, A);
+}
+
+extern "C" {
+
+int64_t nnx_unet_param_count(void) { return plan().total; }
+
+nnx_status nnx_unet_create(nnx_ctx* c, const float* h_params, uint64_t seed, nnx_unet** out) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, out, "null out");
+    NNX_HIP(c, hipSetDevice(c->device));
+    const Plan& P = plan();
+    std::vector<float> h(P.total, 0.f);
+    if (h_params) {
+        std::memcpy(h.data(), h_params, P.total * 4);
+    } else {
+        // He-uniform weights from a counter-based hash of (seed, index); zero biases
+        auto unif = [&](uint64_t i) {
+            uint64_t x = (seed + 0x9e3779b97f4a7c15ull) ^ (i * 0xbf58476d1ce4e5b9ull);
+            x ^= x >> 31;
+            x *= 0x94d049bb133111ebull;
+            x ^= x >> 29;
+            x *= 0xbf58476d1ce4e5b9ull;
+            x ^= x >> 32;
+            return (double)(x >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+        };
+        for (int l = 0; l < 5; ++l) {
+            const Layer& L = P.L[l];
+            const double bound = std::sqrt(6.0 / (27.0 * L.cin));
+            for (int64_t i = 0; i < (int64_t)L.cout * 27 * L.cin; ++i) h[L.w_off + i] = (float)(bound * unif(L.w_off + i));
+        }
+        const double bound = std::sqrt(3.0 / 16.0);
+        for (int i = 0; i < 16; ++i) h[P.head_off + i] = (float)(bound * unif(P.head_off + i));
+    }
+    nnx_unet* u = new nnx_unet();
+    size_t bytes = (size_t)P.total * 4;
+    hipError_t e = hipSuccess;
+    e = e ? e : hipMalloc(&u->p, bytes);
+    e = e ? e : hipMalloc(&u->grad, bytes);
+    e = e ? e : hipMalloc(&u->m, bytes);
+    e = e ? e : hipMalloc(&u->v, bytes);
+    for (int l = 0; l < 5 && !e; ++l) {
+        const Layer& L = P.L[l];
+        e = hipMalloc(&u->wf[l], (size_t)L.cout * kpad(L.cin) * 2);
+        if (!e) e = hipMalloc(&u->wb[l], (size_t)L.cin * kpad(L.cout) * 2);
+    }
+    if (!e) e = hipMemcpyAsync(u->p, h.data(), bytes, hipMemcpyHostToDevice, c->stream);
+    if (!e) e = hipMemsetAsync(u->m, 0, bytes, c->stream);
+    if (!e) e = hipMemsetAsync(u->v, 0, bytes, c->stream);
+    if (!e) e = hipMemsetAsync(u->grad, 0, bytes, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) {
+        delete u;
+        return fail(c, NNX_ERR_HIP, "nnx_unet_create: %s", hipGetErrorString(e));
+    }
+    u->step = 0;
+    u->dirty = true;
+    *out = u;
+    return NNX_OK;
+}
+
+nnx_status nnx_unet_destroy(nnx_ctx* c, nnx_unet* u) {
+    if (!u) return NNX_OK;
+    if (c) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(u->p);
+    (void)hipFree(u->grad);
+    (void)hipFree(u->m);
+    (void)hipFree(u->v);
+    for (int l = 0; l < 5; ++l) {
+        (void)hipFree(u->wf[l]);
+        (void)hipFree(u->wb[l]);
+    }
+    delete u;
+    return NNX_OK;
+}
+
+nnx_status nnx_unet_get_params(nnx_ctx* c, const nnx_unet* u, float* h_params) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, u && h_params, "null argument");
+    NNX_HIP(c, hipMemcpyAsync(h_params, u->p, plan().total * 4, hipMemcpyDeviceToHost, c->stream));
+    NNX_HIP(c, hipStreamSynchronize(c->stream));
+    return NNX_OK;
+}
+
+nnx_status nnx_unet_apply(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* d_r,
+                          const float* d_feat, float* d_z) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, u && d_r && d_z, "null argument");
+    NNX_TRY(check_dims(c, nx, ny, nz, batch));
+    return nnx_unet_forward_internal(c, u, nx, ny, nz, batch, d_r, d_feat, d_z);
+}
+
+nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* d_r,
+                               const float* d_feat, const float* d_e, double lr, double* h_loss) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, u && d_r && d_e, "null argument");
+    NNX_TRY(check_dims(c, nx, ny, nz, batch));
+    const Plan& P = plan();
+    const int64_t nper = (int64_t)nx * ny * nz, n0 = nper * batch, n1 = n0 / 8, n2 = n0 / 64;
+    Acts A;
+    NNX_TRY(acts(c, n0, batch, true, &A));
+    NNX_TRY(forward(c, u, nx, ny, nz, batch, d_r, d_feat, A.z, true, A));
+    void *p, *lossp;
+    NNX_TRY(scratch(c, "un_g0", n0 * 32, false, &p));
+    uint16_t* G0 = (uint16_t*)p;  // later: gradient of e0
+    NNX_TRY(scratch(c, "un_u0", n0 * 32, false, &p));
+    uint16_t* U0 = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_l1a", n1 * 64, false, &p));
+    uint16_t* L1a = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_l1b", n1 * 64, false, &p));
+    uint16_t* L1b = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_l1c", n1 * 64, false, &p));
+    uint16_t* L1c = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_l2a", n2 * 64, false, &p));
+    uint16_t* L2a = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_l2b", n2 * 64, false, &p));
+    uint16_t* L2b = (uint16_t*)p;
+    NNX_TRY(scratch(c, "un_loss", 64, false, &lossp));
+    NNX_TRY(scratch(c, "un_t2", 256 * 8, false, &p));
+    double* t2 = (double*)p;
+    // loss and head
+    k_t2<<<dim3(256, batch), 256, 0, c->stream>>>(d_e, A.inv, nper, A.part);
+    k_rms_fin<<<1, 256, 0, c->stream>>>(A.part, 256, 1, batch, A.s + 256, A.inv + 256, t2);
+    k_head_bwd<<<kHeadBlocks, 256, 0, c->stream>>>(A.z, d_e, A.inv, t2, nper, batch, A.d0, u->p + P.head_off, G0,
+                                                   A.part);
+    k_head_fin<<<1, 64, 0, c->stream>>>(A.part, kHeadBlocks, u->grad + P.head_off, (double*)lossp, batch);
+    NNX_LAUNCHED(c);
+    const int nx1 = nx / 2, ny1 = ny / 2, nz1 = nz / 2, nx2 = nx / 4, ny2 = ny / 4, nz2 = nz / 4;
+    auto WG = [&](int nxl, int nyl, int nzl, const uint16_t* in, const uint16_t* coarse, float cs, const uint16_t* g) {
+        WgArgs w{};
+        w.nx = nxl, w.ny = nyl, w.nz = nzl, w.batch = batch, w.in = in, w.coarse = coarse, w.cscale = cs, w.g = g;
+        return w;
+    };
+    auto BD = [&](int nxl, int nyl, int nzl, const uint16_t* g, const uint16_t* w, uint16_t* out) {
+        ConvArgs a{};
+        a.nx = nxl, a.ny = nyl, a.nz = nzl, a.batch = batch, a.in = g, a.w = w, a.out = out;
+        return a;
+    };
+    // d0: input e0 + up(d1), gradient G0
+    NNX_TRY(wgrad_dispatch(c, 16, 16, WG(nx, ny, nz, A.e0, A.d1, 1.f, G0), u->grad + P.L[4].w_off,
+                           u->grad + P.L[4].b_off));
+    NNX_TRY(conv_dispatch(c, 16, 16, BD(nx, ny, nz, G0, u->wb[4], U0)));  // dU0 (skip part of e0's gradient)
+    k_sumpool8<<<eblocks(n1 * 16), 256, 0, c->stream>>>(nx1, ny1, nz1, batch, 16, U0, L1a);  // d(d1)
+    k_grad_combine<<<eblocks(n1 * 2), 256, 0, c->stream>>>(nx1, ny1, nz1, batch, 16, A.d1, L1a, nullptr, 0.f, L1b);
+    NNX_LAUNCHED(c);
+    // d1: input e1 + up(e2), gradient L1b (16 ch)
+    NNX_TRY(wgrad_dispatch(c, 32, 16, WG(nx1, ny1, nz1, A.e1, A.e2, 1.f, L1b), u->grad + P.L[3].w_off,
+                           u->grad + P.L[3].b_off));
+    NNX_TRY(conv_dispatch(c, 16, 32, BD(nx1, ny1, nz1, L1b, u->wb[3], L1c)));  // dU1 (32 ch)
+    k_sumpool8<<<eblocks(n2 * 32), 256, 0, c->stream>>>(nx2, ny2, nz2, batch, 32, L1c, L2a);  // d(e2)
+    k_grad_combine<<<eblocks(n2 * 4), 256, 0, c->stream>>>(nx2, ny2, nz2, batch, 32, A.e2, L2a, nullptr, 0.f, L2b);
+    NNX_LAUNCHED(c);
+    // e2: input p1, gradient L2b
+    NNX_TRY(wgrad_dispatch(c, 32, 32, WG(nx2, ny2, nz2, A.p1, nullptr, 0.f, L2b), u->grad + P.L[2].w_off,
+                           u->grad + P.L[2].b_off));
+    NNX_TRY(conv_dispatch(c, 32, 32, BD(nx2, ny2, nz2, L2b, u->wb[2], L2a)));  // d(p1)
+    // e1 gradient: skip part dU1 + pooled part d(p1)/8
+    k_grad_combine<<<eblocks(n1 * 4), 256, 0, c->stream>>>(nx1, ny1, nz1, batch, 32, A.e1, L1c, L2a, 0.125f, L1a);
+    NNX_LAUNCHED(c);
+    NNX_TRY(wgrad_dispatch(c, 16, 32, WG(nx1, ny1, nz1, A.p0, nullptr, 0.f, L1a), u->grad + P.L[1].w_off,
+                           u->grad + P.L[1].b_off));
+    NNX_TRY(conv_dispatch(c, 32, 16, BD(nx1, ny1, nz1, L1a, u->wb[1], L1b)));  // d(p0) (16 ch)
+    // e0 gradient: dU0 + d(p0)/8
+    k_grad_combine<<<eblocks(n0 * 2), 256, 0, c->stream>>>(nx, ny, nz, batch, 16, A.e0, U0, L1b, 0.125f, G0);
+    NNX_LAUNCHED(c);
+    NNX_TRY(wgrad_dispatch(c, 2, 16, WG(nx, ny, nz, A.x, nullptr, 0.f, G0), u->grad + P.L[0].w_off,
+                           u->grad + P.L[0].b_off));
+    // Adam
+    u->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = (float)(1.0 - std::pow((double)b1, (double)u->step));
+    const float c2 = (float)(1.0 - std::pow((double)b2, (double)u->step));
+    if (lr > 0) {
+        k_adam<<<eblocks(P.total), 256, 0, c->stream>>>(P.total, u->p, u->grad, u->m, u->v, (float)lr, b1, b2, 1e-8f,
+                                                        c1, c2);
+        NNX_LAUNCHED(c);
+        u->dirty = true;
+    }
+    if (h_loss) {
+        NNX_HIP(c, hipMemcpyAsync(c->pinned, lossp, 8, hipMemcpyDeviceToHost, c->stream));
+        NNX_HIP(c, hipStreamSynchronize(c->stream));
+        std::memcpy(h_loss, c->pinned, 8);
+    }
+    return NNX_OK;
+}
+
+nnx_status nnx_unet_get_grad(nnx_ctx* c, const nnx_unet* u, float* h_grad) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, u && h_grad, "null argument");
+    NNX_HIP(c, hipMemcpyAsync(h_grad, u->grad, plan().total * 4, hipMemcpyDeviceToHost, c->stream));
+    NNX_HIP(c, hipStreamSynchronize(c->stream));
+    return NNX_OK;
+}
+
+// single-layer primitives (weights given in fp32 [cout][27 cin])
+nnx_status nnx_conv3_bf16(nnx_ctx* c, int cin, int cout, int nx, int ny, int nz, int batch, const uint16_t* d_in,
+                          const uint16_t* d_coarse, double coarse_scale, const float* d_w, const float* d_bias,
+                          int relu, uint16_t* d_out, uint16_t* d_pool) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, d_in && d_w && (d_out || d_pool), "null argument");
+    NNX_CHECK(c, cin != 2 || !d_coarse, "the 2-channel input layer takes no upsampled term");
+    void* wp;
+    NNX_TRY(scratch(c, "un_prim_w", (size_t)cout * kpad(cin) * 2, false, &wp));
+    k_prep_w<<<ceil_div(cout * kpad(cin), 256), 256, 0, c->stream>>>(d_w, cin, cout, 0, (uint16_t*)wp);
+    NNX_LAUNCHED(c);
+    ConvArgs a{};
+    a.nx = nx, a.ny = ny, a.nz = nz, a.batch = batch, a.in = d_in, a.coarse = d_coarse, a.cscale = (float)coarse_scale;
+    a.w = (const uint16_t*)wp, a.bias = d_bias, a.relu = relu, a.out = d_out, a.pool = d_pool;
+    return conv_dispatch(c, cin, cout, a);
+}
+
+nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* c, int cin, int cout, int nx, int ny, int nz, int batch,
+                                const uint16_t* d_in, const uint16_t* d_coarse, double coarse_scale,
+                                const uint16_t* d_g, float* d_dw, float* d_db) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, d_in && d_g && d_dw && d_db, "null argument");
+    WgArgs w{};
+    w.nx = nx, w.ny = ny, w.nz = nz, w.batch = batch, w.in = d_in, w.coarse = d_coarse,
+    w.cscale = (float)coarse_scale, w.g = d_g;
+    return wgrad_dispatch(c, cin, cout, w, d_dw, d_db);
+}
+
+}  // extern "C"

@@ -131,3 +131,22 @@ def csr_transpose(rows, cols, off, idx, val):
     host().nnx_csr_transpose(rows, cols, _p(off), _p(idx), _p(np.ascontiguousarray(val, np.float32)),
                              _p(t_off), _p(t_idx), _p(t_val))
     return t_off, t_idx, t_val
+
+
+def grf(seed, purpose, nx, ny, nz, corr) -> np.ndarray:
+    """Unit-variance Gaussian random field, covariance exp(-r^2/(2 corr^2)),
+    shape (nz, ny, nx) (x fastest)."""
+    out = np.empty((nz, ny, nx), np.float32)
+    host().nnx_gen_grf(seed, purpose, nx, ny, nz, corr, _p(out))
+    return out
+
+
+def face_coefficients(c):
+    """kx, ky, kz (nnx_grid layout) from per-voxel coefficients c (nz, ny, nx)."""
+    c = np.ascontiguousarray(c, np.float32)
+    nz, ny, nx = c.shape
+    kx = np.empty((nz, ny, nx + 1), np.float32)
+    ky = np.empty((nz, ny + 1, nx), np.float32)
+    kz = np.empty((nz + 1, ny, nx), np.float32)
+    host().nnx_face_coefficients(nx, ny, nz, _p(c), _p(kx), _p(ky), _p(kz))
+    return kx, ky, kz

@@ -439,4 +439,82 @@ void nnx_csr_transpose(int64_t rows, int64_t cols, const int32_t* off, const int
         }
 }
 
+// Gaussian random field on an nx*ny*nz grid (x fastest): standard normal
+// white noise (Philox stream `purpose`) on a grid padded by R = ceil(3 s) on
+// every side, filtered by the separable kernel exp(-d^2 / (2 s^2)) with
+// s = corr / sqrt(2), and scaled by the kernel's L2 norm, so that the field
+// has unit variance and covariance exp(-r^2 / (2 corr^2)).
+void nnx_gen_grf(uint64_t seed, uint64_t purpose, int32_t nx, int32_t ny, int32_t nz, double corr,
+                 float* out) {
+    const double s = corr / std::sqrt(2.0);
+    const int R = (int)std::ceil(3.0 * s);
+    std::vector<double> k(2 * R + 1);
+    double k2 = 0;
+    for (int d = -R; d <= R; ++d) {
+        k[d + R] = std::exp(-(double)d * d / (2.0 * s * s));
+        k2 += k[d + R] * k[d + R];
+    }
+    const double norm = 1.0 / std::pow(k2, 1.5);
+    const int64_t px = nx + 2 * R, py = ny + 2 * R, pz = nz + 2 * R;
+    std::vector<double> w((size_t)(px * py * pz));
+    nnx_gen_normal(seed, purpose, (int64_t)w.size(), 0.0, 1.0, w.data());
+    // x pass: (px, py, pz) -> (nx, py, pz)
+    std::vector<double> a((size_t)(nx * py * pz));
+#pragma omp parallel for schedule(static)
+    for (int64_t row = 0; row < py * pz; ++row)
+        for (int i = 0; i < nx; ++i) {
+            double t = 0;
+            for (int d = 0; d <= 2 * R; ++d) t += k[d] * w[(size_t)(row * px + i + d)];
+            a[(size_t)(row * nx + i)] = t;
+        }
+    std::vector<double>().swap(w);
+    // y pass: (nx, py, pz) -> (nx, ny, pz)
+    std::vector<double> b((size_t)(nx * ny * pz));
+#pragma omp parallel for schedule(static)
+    for (int64_t z = 0; z < pz; ++z)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double t = 0;
+                for (int d = 0; d <= 2 * R; ++d) t += k[d] * a[(size_t)((z * py + j + d) * nx + i)];
+                b[(size_t)((z * ny + j) * nx + i)] = t;
+            }
+    std::vector<double>().swap(a);
+    // z pass: (nx, ny, pz) -> (nx, ny, nz)
+    const int64_t plane = (int64_t)nx * ny;
+#pragma omp parallel for schedule(static)
+    for (int64_t z = 0; z < nz; ++z)
+        for (int64_t e = 0; e < plane; ++e) {
+            double t = 0;
+            for (int d = 0; d <= 2 * R; ++d) t += k[d] * b[(size_t)((z + d) * plane + e)];
+            out[z * plane + e] = (float)(t * std::sqrt(norm));
+        }
+}
+
+// Face coefficients of the variable 7-point operator from per-voxel
+// coefficients c (= dt D / h^2): interior faces take the harmonic mean of
+// their two voxels, boundary faces the voxel's own value.  Layout as
+// nnx_grid's kx / ky / kz.
+void nnx_face_coefficients(int32_t nx, int32_t ny, int32_t nz, const float* c, float* kx, float* ky,
+                           float* kz) {
+    auto hm = [](float a, float b) { return (float)(2.0 * (double)a * b / ((double)a + b)); };
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j) {
+            const float* row = c + (k * ny + j) * nx;
+            float* fx = kx + (k * ny + j) * (nx + 1);
+            fx[0] = row[0];
+            for (int i = 1; i < nx; ++i) fx[i] = hm(row[i - 1], row[i]);
+            fx[nx] = row[nx - 1];
+            for (int i = 0; i < nx; ++i) {
+                const int64_t id = (k * ny + j) * nx + i;
+                const float cy = j == 0 ? c[id] : hm(c[id - nx], c[id]);
+                ky[i + nx * (j + (int64_t)(ny + 1) * k)] = cy;
+                if (j == ny - 1) ky[i + nx * (ny + (int64_t)(ny + 1) * k)] = c[id];
+                const float cz = k == 0 ? c[id] : hm(c[id - (int64_t)nx * ny], c[id]);
+                kz[id] = cz;
+                if (k == nz - 1) kz[id + (int64_t)nx * ny] = c[id];
+            }
+        }
+}
+
 }  // extern "C"

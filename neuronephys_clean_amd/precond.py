@@ -1,0 +1,191 @@
+"""The learned preconditioner of the paracrine solve: a handle on libnnx's
+U-Net (nnx_unet_*), the harvesting of (residual, error) pairs from
+Jacobi-CG solves on random variable-diffusivity fields, and the training
+loop (forward, hand-written backward and Adam all run in libnnx).
+
+No counterpart in the reference: its paracrine solve is unpreconditioned CG
+(src/paracrine.cu:106-129).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib, configs
+from .devmem import DeviceArray as DA
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_c5.f32")
+
+
+def param_count() -> int:
+    return int(_lib.lib().nnx_unet_param_count())
+
+
+class UNet:
+    def __init__(self, ctx: _lib.Context, params: np.ndarray | None = None, seed: int = 0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        arr = None
+        if params is not None:
+            arr = np.ascontiguousarray(params, np.float32)
+            assert arr.size == param_count(), (arr.size, param_count())
+        ctx.call("nnx_unet_create", None if arr is None else arr.ctypes.data_as(C.c_void_p), seed,
+                 C.byref(self.h))
+
+    @classmethod
+    def load(cls, ctx, path: str = DEFAULT_WEIGHTS) -> "UNet":
+        return cls(ctx, np.fromfile(path, np.float32))
+
+    def save(self, path: str):
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        self.params().tofile(path)
+
+    def params(self) -> np.ndarray:
+        out = np.empty(param_count(), np.float32)
+        self.ctx.call("nnx_unet_get_params", self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def grad(self) -> np.ndarray:
+        out = np.empty(param_count(), np.float32)
+        self.ctx.call("nnx_unet_get_grad", self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def apply(self, dims, batch, r: DA, feat: DA | None, z: DA):
+        nx, ny, nz = dims
+        self.ctx.call("nnx_unet_apply", self.h, nx, ny, nz, batch, r.ptr,
+                      None if feat is None else feat.ptr, z.ptr)
+
+    def train_step(self, dims, batch, r: DA, feat: DA | None, e: DA, lr: float, want_loss=True):
+        nx, ny, nz = dims
+        loss = C.c_double()
+        self.ctx.call("nnx_unet_train_step", self.h, nx, ny, nz, batch, r.ptr,
+                      None if feat is None else feat.ptr, e.ptr, lr,
+                      C.byref(loss) if want_loss else None)
+        return loss.value if want_loss else None
+
+    def close(self):
+        if self.h:
+            self.ctx.call("nnx_unet_destroy", self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_field(ctx, f: configs.DiffusionField):
+    """Grid struct and device arrays (kept alive by the returned dict)."""
+    d = {"kx": DA.from_numpy(ctx, f.kx), "ky": DA.from_numpy(ctx, f.ky),
+         "kz": DA.from_numpy(ctx, f.kz), "feat": DA.from_numpy(ctx, f.feat)}
+    g = _lib.Grid(f.n, f.n, f.n, 7, 0.0, 0.0, d["kx"].ptr, d["ky"].ptr, d["kz"].ptr)
+    return g, d
+
+
+class PairSet:
+    """(residual, error) pairs and the feature field of each pair, resident
+    on the device: r[i], e[i] are n^3 fp32 grids, feature of pair i is
+    feats[field_of[i]]."""
+
+    def __init__(self, ctx, n, npairs, nfields):
+        self.ctx, self.n, self.count = ctx, n, 0
+        vox = n ** 3
+        self.r = DA(ctx, (npairs, vox), np.float32)
+        self.e = DA(ctx, (npairs, vox), np.float32)
+        self.feats = DA(ctx, (nfields, vox), np.float32)
+        self.field_of = np.zeros(npairs, np.int64)
+        self.bytes_per = vox * 4
+
+    def _put(self, dst: DA, row: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float32)
+        self.ctx.call("nnx_memcpy_h2d", C.c_void_p(dst.ptr + row * self.bytes_per),
+                      a.ctypes.data_as(C.c_void_p), C.c_size_t(self.bytes_per))
+
+    def add_field(self, idx, feat):
+        self._put(self.feats, idx, feat)
+
+    def add_pair(self, r, e, field):
+        i = self.count
+        self._put(self.r, i, r)
+        self._put(self.e, i, e)
+        self.field_of[i] = field
+        self.count += 1
+
+    def gather(self, idx, r_out: DA, f_out: DA, e_out: DA):
+        for j, i in enumerate(idx):
+            off = j * self.bytes_per
+            for dst, src, row in ((r_out, self.r, i), (e_out, self.e, i),
+                                  (f_out, self.feats, self.field_of[i])):
+                self.ctx.call("nnx_memcpy_d2d", C.c_void_p(dst.ptr + off),
+                              C.c_void_p(src.ptr + int(row) * self.bytes_per), C.c_size_t(self.bytes_per))
+
+
+def _cg(ctx, g, b: DA, x: DA, rtol, maxit, use_x0=0):
+    o = _lib.CgOpts(rtol=rtol, maxit=maxit, precond=_lib.PRECOND_JACOBI, check_every=8,
+                    use_x0=use_x0)
+    r = _lib.CgResult()
+    st = ctx._l.nnx_cg_solve(ctx.h, C.byref(g), b.ptr, x.ptr, C.byref(o), C.byref(r))
+    if st not in (0, 4):  # 4: not converged within maxit (the snapshots)
+        ctx.check(st)
+    return r
+
+
+def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 64), seed=1,
+            log=print) -> PairSet:
+    """Pairs (r_k, e_k = x* - x_k) from Jacobi-CG iterates x_k (x_0 = 0) on
+    `fields` fresh log-normal diffusivity fields (C5 statistics at n^3) and
+    `sources` random point-source right-hand sides per field."""
+    ps = PairSet(ctx, n, fields * sources * len(schedule), fields)
+    vox = n ** 3
+    xs, xk, ax = DA(ctx, vox, np.float32), DA(ctx, vox, np.float32), DA(ctx, vox, np.float32)
+    t0 = time.time()
+    for f in range(fields):
+        fld = configs.c5_field(seed=seed * 100003 + f, n=n)
+        g, keep = device_field(ctx, fld)
+        ps.add_field(f, fld.feat)
+        for s in range(sources):
+            cnt = int(20 + (s * 37 + f * 11) % 380)
+            b = configs.point_sources(seed * 100003 + f * 1009 + s, n, cnt)
+            db = DA.from_numpy(ctx, b)
+            _cg(ctx, g, db, xs, 1e-7, 5000)
+            xstar = xs.numpy()
+            for k in schedule:
+                if k == 0:
+                    ps.add_pair(b, xstar, f)
+                    continue
+                _cg(ctx, g, db, xk, 0.0, k)
+                ctx.call("nnx_stencil_apply", C.byref(g), xk.ptr, ax.ptr)
+                ps.add_pair(b - ax.numpy(), xstar - xk.numpy(), f)
+        if (f + 1) % 20 == 0:
+            log(f"harvest: {f + 1}/{fields} fields, {ps.count} pairs, {time.time() - t0:.1f}s")
+        del keep
+    return ps
+
+
+def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=2e-3, lr_min=1e-4, seed=0, log=print):
+    """Adam with a cosine learning-rate schedule over shuffled pairs."""
+    n, vox = ps.n, ps.n ** 3
+    rb, fb, eb = (DA(ctx, (batch, vox), np.float32) for _ in range(3))
+    steps = epochs * (ps.count // batch)
+    rng = np.random.default_rng(seed)
+    losses, t0, step = [], time.time(), 0
+    for ep in range(epochs):
+        perm = rng.permutation(ps.count)
+        for j in range(ps.count // batch):
+            idx = perm[j * batch:(j + 1) * batch]
+            ps.gather(idx, rb, fb, eb)
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / max(steps, 1)))
+            want = (step % 50 == 0) or step == steps - 1
+            loss = net.train_step((n, n, n), batch, rb, fb, eb, cur, want_loss=want)
+            if want:
+                losses.append((step, loss))
+                if step % 500 == 0:
+                    log(f"train: epoch {ep} step {step}/{steps} lr {cur:.2e} loss {loss:.4f} "
+                        f"{time.time() - t0:.1f}s")
+            step += 1
+    return losses

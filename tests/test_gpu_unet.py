@@ -1,0 +1,311 @@
+"""BF16 MFMA convolution kernels, the U-Net and flexible CG on the GPU.
+
+References are torch's CPU conv3d / autograd in float64 evaluated on the
+same BF16-rounded inputs and weights, with BF16 rounding applied at the
+points where the kernels store BF16 (torch is the checker only).
+Tolerances: a stored BF16 value may differ by one BF16 ulp (2^-8
+relative) where fp32 accumulation order moves a value across a rounding
+boundary; fp32 sums over K terms add K * 2^-24 of the sum of |terms|.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from neuronephys_clean_amd import _lib, configs
+from neuronephys_clean_amd.devmem import DeviceArray as DA
+from neuronephys_clean_amd.precond import UNet, device_field, param_count
+
+pytestmark = pytest.mark.gpu
+torch.set_num_threads(8)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    """round to nearest even BF16, as uint16 bits"""
+    t = torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(torch.bfloat16)
+    return t.view(torch.int16).numpy().view(np.uint16)
+
+
+def bits_f64(u: np.ndarray) -> np.ndarray:
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def to_ncdhw(a, shape):  # channels-last (b, z, y, x, c) -> torch (b, c, z, y, x)
+    b, nz, ny, nx, c = shape
+    return torch.from_numpy(a.reshape(b, nz, ny, nx, c)).permute(0, 4, 1, 2, 3).contiguous()
+
+
+def from_ncdhw(t):
+    return t.permute(0, 2, 3, 4, 1).contiguous().numpy()
+
+
+def torch_w(w, cin, cout):  # [cout][27][cin], tap = dx + 3 dy + 9 dz -> [cout][cin][dz][dy][dx]
+    return torch.from_numpy(w.reshape(cout, 3, 3, 3, cin)).permute(0, 4, 1, 2, 3).contiguous()
+
+
+def rnd_bf16(t):
+    return t.to(torch.float32).to(torch.bfloat16).to(torch.float64)
+
+
+def up2(t):
+    return t.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
+
+
+def close_bf16(got, ref, K, what):
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
+    scale = np.abs(ref).max()
+    tol = 2.0 ** -7 * np.abs(ref) + (K * 2.0 ** -22 + 1e-6) * scale
+    bad = np.abs(got - ref) > tol
+    say(what, "max|d|", np.abs(got - ref).max(), "scale", scale, "bad", int(bad.sum()))
+    assert not bad.any(), what
+
+
+CASES = [(2, 16, False, True), (16, 16, True, False), (16, 32, False, True), (32, 16, True, False),
+         (32, 32, False, False)]
+
+
+@pytest.mark.parametrize("cin,cout,coarse,pool", CASES)
+def test_conv3_forward_matches_torch(ctx, cin, cout, coarse, pool):
+    rng = np.random.default_rng(cin * 100 + cout)
+    B, nz, ny, nx = 2, 16, 16, 32
+    x = bf16_bits(rng.normal(0, 1, (B, nz, ny, nx, cin)).astype(np.float32))
+    cb = bf16_bits(rng.normal(0, 1, (B, nz // 2, ny // 2, nx // 2, cin)).astype(np.float32)) if coarse else None
+    w = (rng.normal(0, 1, (cout, 27, cin)) / np.sqrt(27 * cin)).astype(np.float32)
+    bias = rng.normal(0, 0.1, cout).astype(np.float32)
+    dx, dw, db = DA.from_numpy(ctx, x), DA.from_numpy(ctx, w), DA.from_numpy(ctx, bias)
+    dc = DA.from_numpy(ctx, cb) if coarse else None
+    out = DA.zeros(ctx, (B, nz, ny, nx, cout), np.uint16)
+    pl = DA.zeros(ctx, (B, nz // 2, ny // 2, nx // 2, cout), np.uint16) if pool else None
+    say("conv3", cin, cout, "coarse", coarse, "pool", pool)
+    ctx.call("nnx_conv3_bf16", cin, cout, nx, ny, nz, B, dx.ptr, dc.ptr if dc else None, 0.5,
+             dw.ptr, db.ptr, 1, out.ptr, pl.ptr if pl else None)
+    xin = to_ncdhw(bits_f64(x), (B, nz, ny, nx, cin))
+    if coarse:
+        cin_t = to_ncdhw(bits_f64(cb), (B, nz // 2, ny // 2, nx // 2, cin))
+        xin = (xin.float() + 0.5 * up2(cin_t).float()).to(torch.bfloat16).to(torch.float64)
+    wt = rnd_bf16(torch_w(w.astype(np.float64), cin, cout))
+    y = torch.relu(F.conv3d(xin, wt, torch.from_numpy(bias.astype(np.float64)), padding=1))
+    close_bf16(bits_f64(out.numpy()), from_ncdhw(rnd_bf16(y)), 27 * cin, "out")
+    if pool:
+        p = F.avg_pool3d(y, 2)
+        close_bf16(bits_f64(pl.numpy()), from_ncdhw(rnd_bf16(p)), 27 * cin, "pool")
+
+
+@pytest.mark.parametrize("cin,cout,coarse,pool", CASES)
+def test_conv3_weight_grad_matches_torch(ctx, cin, cout, coarse, pool):
+    rng = np.random.default_rng(7 + cin * 100 + cout)
+    B, nz, ny, nx = 2, 16, 16, 32
+    x = bf16_bits(rng.normal(0, 1, (B, nz, ny, nx, cin)).astype(np.float32))
+    cb = bf16_bits(rng.normal(0, 1, (B, nz // 2, ny // 2, nx // 2, cin)).astype(np.float32)) if coarse else None
+    g = bf16_bits(rng.normal(0, 1, (B, nz, ny, nx, cout)).astype(np.float32))
+    dx, dg = DA.from_numpy(ctx, x), DA.from_numpy(ctx, g)
+    dc = DA.from_numpy(ctx, cb) if coarse else None
+    dW, dB = DA.zeros(ctx, (cout, 27, cin), np.float32), DA.zeros(ctx, cout, np.float32)
+    say("wgrad", cin, cout, "coarse", coarse)
+    ctx.call("nnx_conv3_wgrad_bf16", cin, cout, nx, ny, nz, B, dx.ptr, dc.ptr if dc else None, 0.5,
+             dg.ptr, dW.ptr, dB.ptr)
+    xin = to_ncdhw(bits_f64(x), (B, nz, ny, nx, cin))
+    if coarse:
+        cin_t = to_ncdhw(bits_f64(cb), (B, nz // 2, ny // 2, nx // 2, cin))
+        xin = (xin.float() + 0.5 * up2(cin_t).float()).to(torch.bfloat16).to(torch.float64)
+    gt = to_ncdhw(bits_f64(g), (B, nz, ny, nx, cout))
+    ref = torch.nn.grad.conv3d_weight(xin, (cout, cin, 3, 3, 3), gt, padding=1)
+    ref = ref.permute(0, 2, 3, 4, 1).reshape(cout, 27, cin).numpy()
+    got = dW.numpy()
+    nvox = B * nx * ny * nz
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say("wgrad rel err", err)
+    assert err < 20 * nvox ** 0.5 * 2.0 ** -24 + 1e-6
+    np.testing.assert_allclose(dB.numpy(), gt.sum(dim=(0, 2, 3, 4)).numpy(), rtol=1e-4,
+                               atol=1e-4 * np.abs(gt.numpy()).sum() / cout / nvox ** 0.5)
+
+
+# --------------------------------------------------------------- U-Net ----
+LAYERS = [(2, 16), (16, 32), (32, 32), (32, 16), (16, 16)]
+
+
+def unpack(p):
+    out, o = [], 0
+    for cin, cout in LAYERS:
+        w = p[o:o + cout * 27 * cin].reshape(cout, 27, cin)
+        o += cout * 27 * cin
+        b = p[o:o + cout]
+        o += cout
+        out.append((w, b))
+    return out, p[o:o + 16], p[o + 16]
+
+
+def torch_unet(p, r, feat, dims, B, round_bf16=True):
+    """The network in float64 with BF16 rounding where the kernels store BF16
+    (straight-through for gradients).  r, feat: (B, nz, ny, nx)."""
+    nx, ny, nz = dims
+    layers, hw, hb = unpack(p)
+    R = lambda t: t + (rnd_bf16(t) - t).detach() if round_bf16 else t  # noqa: E731
+    W = [(R(torch_w(w.astype(np.float64), ci, co)), torch.from_numpy(b.astype(np.float64)))
+         for (w, b), (ci, co) in zip(layers, LAYERS)]
+    s = torch.sqrt((r ** 2).mean(dim=(1, 2, 3), keepdim=True))
+    x = torch.stack([R(r / s), R(feat)], dim=1)
+    conv = lambda t, i: torch.relu(F.conv3d(t, W[i][0], W[i][1], padding=1))  # noqa: E731
+    e0f = conv(x, 0)
+    e0, p0 = R(e0f), R(F.avg_pool3d(e0f, 2))
+    e1f = conv(p0, 1)
+    e1, p1 = R(e1f), R(F.avg_pool3d(e1f, 2))
+    e2 = R(conv(p1, 2))
+    d1 = R(conv(R(e1 + up2(e2)), 3))
+    d0 = conv(R(e0 + up2(d1)), 4)
+    hwt = torch.from_numpy(hw.astype(np.float64)).view(1, 16, 1, 1, 1)
+    z = (d0 * hwt).sum(1) + float(hb)
+    return z, s.view(B, 1, 1, 1)
+
+
+def test_unet_forward_matches_torch(ctx):
+    dims, B = (64, 32, 32), 2
+    nx, ny, nz = dims
+    rng = np.random.default_rng(3)
+    net = UNet(ctx, seed=11)
+    p = net.params()
+    assert p.size == param_count() == 63201
+    r = rng.normal(0, 2, (B, nz, ny, nx)).astype(np.float32)
+    feat = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
+    dr, df, dz = DA.from_numpy(ctx, r), DA.from_numpy(ctx, feat), DA.zeros(ctx, r.shape, np.float32)
+    say("unet forward", dims, B)
+    net.apply(dims, B, dr, df, dz)
+    z, s = torch_unet(p, torch.from_numpy(r.astype(np.float64)), torch.from_numpy(feat.astype(np.float64)),
+                      dims, B)
+    ref = (z * s).numpy()
+    got = dz.numpy()
+    err = np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    say("unet forward rel rms err", err)
+    assert err < 1e-2
+
+
+def test_unet_gradient_matches_autograd(ctx):
+    dims, B = (64, 32, 32), 2
+    nx, ny, nz = dims
+    rng = np.random.default_rng(4)
+    net = UNet(ctx, seed=12)
+    p = net.params()
+    r = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
+    feat = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
+    e = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
+    dr, df, de = (DA.from_numpy(ctx, a) for a in (r, feat, e))
+    say("unet gradient", dims, B)
+    loss = net.train_step(dims, B, dr, df, de, 0.0)
+    got = net.grad()
+    pt = torch.from_numpy(p.astype(np.float64)).requires_grad_(True)
+    layers_t = []
+    o = 0
+    for cin, cout in LAYERS:
+        layers_t.append((pt[o:o + cout * 27 * cin], pt[o + cout * 27 * cin:o + cout * 27 * cin + cout]))
+        o += cout * 27 * cin + cout
+
+    # same network as torch_unet, but differentiable in the parameters
+    def fwd():
+        R = lambda t: t + (rnd_bf16(t) - t).detach()  # noqa: E731
+        W = []
+        for (w, b), (ci, co) in zip(layers_t, LAYERS):
+            wt = w.view(co, 3, 3, 3, ci).permute(0, 4, 1, 2, 3)
+            W.append((R(wt), b))
+        rt = torch.from_numpy(r.astype(np.float64))
+        s = torch.sqrt((rt ** 2).mean(dim=(1, 2, 3), keepdim=True))
+        x = torch.stack([R(rt / s), R(torch.from_numpy(feat.astype(np.float64)))], dim=1)
+        conv = lambda t, i: torch.relu(F.conv3d(t, W[i][0], W[i][1], padding=1))  # noqa: E731
+        e0f = conv(x, 0)
+        e0, p0 = R(e0f), R(F.avg_pool3d(e0f, 2))
+        e1f = conv(p0, 1)
+        e1, p1 = R(e1f), R(F.avg_pool3d(e1f, 2))
+        e2 = R(conv(p1, 2))
+        d1 = R(conv(R(e1 + up2(e2)), 3))
+        d0 = conv(R(e0 + up2(d1)), 4)
+        z = (d0 * pt[o:o + 16].view(1, 16, 1, 1, 1)).sum(1) + pt[o + 16]
+        t = torch.from_numpy(e.astype(np.float64)) / s
+        return (((z - t) ** 2).sum(dim=(1, 2, 3)) / (t ** 2).sum(dim=(1, 2, 3))).mean()
+
+    L = fwd()
+    L.backward()
+    ref = pt.grad.numpy()
+    say("loss", loss, float(L.detach()))
+    assert abs(loss - float(L.detach())) <= 2e-2 * abs(float(L.detach()))
+    o = 0
+    for i, (cin, cout) in enumerate(LAYERS + [(16, 1)]):
+        nw = cout * 27 * cin if i < 5 else 17
+        nb = cout if i < 5 else 0
+        seg = slice(o, o + nw + nb)
+        a, b = got[seg], ref[seg]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        say("layer", i, "grad rel err", rel, "cos", cos)
+        assert rel < 5e-2 and cos > 0.998
+        o += nw + nb
+
+
+def test_unet_training_reduces_loss(ctx):
+    dims, B = (64, 64, 64), 4
+    rng = np.random.default_rng(5)
+    f = configs.c5_field(seed=77, n=64)
+    g, keep = device_field(ctx, f)
+    b = np.stack([configs.point_sources(900 + i, 64, 200) for i in range(B)])
+    x = np.zeros_like(b)
+    db, dx = DA.from_numpy(ctx, b[0]), DA.zeros(ctx, 64 ** 3, np.float32)
+    for i in range(B):  # targets: the exact solutions of the four right-hand sides
+        db.upload(b[i])
+        o = _lib.CgOpts(rtol=1e-7, maxit=5000, precond=_lib.PRECOND_JACOBI, check_every=8)
+        ctx.call("nnx_cg_solve", C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(_lib.CgResult()))
+        x[i] = dx.numpy()
+    dr, de = DA.from_numpy(ctx, b), DA.from_numpy(ctx, x)
+    df = DA.from_numpy(ctx, np.broadcast_to(f.feat.ravel(), (B, 64 ** 3)).copy())
+    net = UNet(ctx, seed=1)
+    losses = [net.train_step(dims, B, dr, df, de, 2e-3) for _ in range(60)]
+    say("training losses", losses[0], losses[19], losses[-1])
+    assert all(np.isfinite(losses)) and losses[-1] < 0.5 * losses[0]
+    del rng, keep
+
+
+def _fcg(ctx, g, net, feat, b, precond, rtol=1e-6):
+    x = DA.zeros(ctx, b.size, np.float32)
+    o = _lib.CgOpts(rtol=rtol, maxit=3000, precond=precond, check_every=8, stall_iters=10)
+    r = _lib.CgResult()
+    ctx.call("nnx_fcg_solve", C.byref(g), net.h if net else None, feat.ptr, b.ptr, x.ptr, C.byref(o),
+             C.byref(r))
+    return r, x
+
+
+def test_fcg_jacobi_and_untrained_network_converge(ctx):
+    f = configs.c5_field(seed=5, n=64)
+    g, keep = device_field(ctx, f)
+    b = DA.from_numpy(ctx, configs.point_sources(17, 64, 300))
+    x = DA.zeros(ctx, 64 ** 3, np.float32)
+    o = _lib.CgOpts(rtol=1e-6, maxit=3000, precond=_lib.PRECOND_JACOBI, check_every=1)
+    rc = _lib.CgResult()
+    ctx.call("nnx_cg_solve", C.byref(g), b.ptr, x.ptr, C.byref(o), C.byref(rc))
+    rj, _ = _fcg(ctx, g, None, keep["feat"], b, _lib.PRECOND_JACOBI)
+    say("cg", rc.as_dict(), "fcg jacobi", rj.as_dict())
+    # fp32 vectors: the true residual stalls well above the recursive one at
+    # this conditioning, for CG and FCG alike
+    assert rj.converged and rj.true_relres < 3 * max(rc.true_relres, 1e-6)
+    assert abs(rj.iters - rc.iters) <= max(3, 0.1 * rc.iters)
+    net = UNet(ctx, seed=3)
+    rl, _ = _fcg(ctx, g, net, keep["feat"], b, _lib.PRECOND_LEARNED)
+    say("fcg untrained net", rl.as_dict())
+    assert rl.converged and rl.true_relres < 10 * max(rc.true_relres, 1e-6)
+
+
+def test_fcg_refined_reaches_fp64_tolerance(ctx):
+    f = configs.c5_field(seed=6, n=64)
+    g, keep = device_field(ctx, f)
+    b = configs.point_sources(18, 64, 300).astype(np.float64)
+    db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, 64 ** 3, np.float64)
+    net = UNet(ctx, seed=4)
+    for pc in (_lib.PRECOND_JACOBI, _lib.PRECOND_LEARNED):
+        o = _lib.CgOpts(rtol=1e-8, maxit=3000, precond=pc, check_every=8, stall_iters=10)
+        r = _lib.CgResult()
+        ctx.call("nnx_fcg_solve_refine", C.byref(g), net.h, keep["feat"].ptr, db.ptr, dx.ptr, C.byref(o),
+                 1e-6, 20, C.byref(r))
+        say("fcg refine", pc, r.as_dict())
+        assert r.converged and r.relres <= 1e-8
