@@ -273,7 +273,7 @@ typedef struct nnx_cg_opts {
     int32_t check_every;  /* iterations between host convergence checks (>=1) */
     int32_t use_x0;       /* 1: start from x; 0: start from 0 */
     int32_t stall_iters;  /* flexible CG: fall back to Jacobi after this many non-improving iterations */
-    int32_t pad_;
+    int32_t smooth;       /* learned preconditioner: damped-Jacobi sweeps after the network (0: 1) */
 } nnx_cg_opts;
 
 typedef struct nnx_cg_result {
@@ -347,13 +347,17 @@ nnx_status nnx_csr_spmv(nnx_ctx* ctx, int64_t rows, const int32_t* d_offsets, co
 
 /* A 3-level 3-D convolutional U-Net M(r) ~ A^-1 r, BF16 weights and
  * activations with FP32 accumulation on matrix cores:
- *   e0 = relu(conv3(x; 2->16)), e1 = relu(conv3(pool(e0); 16->32)),
- *   e2 = relu(conv3(pool(e1); 32->32)), d1 = relu(conv3(e1 + up(e2); 32->16)),
- *   d0 = relu(conv3(e0 + up(d1); 16->16)), z = rms(r) (w_h . d0 + b_h),
- * x = (r / rms(r), feature) per voxel, conv3 = 3x3x3 with zero padding,
+ *   e0 = act(conv3(x; 2->16)), e1 = act(conv3(pool(e0); 16->32)),
+ *   e2 = act(conv3(pool(e1); 32->32)), d1 = act(conv3(e1 + up(e2); 32->16)),
+ *   d0 = act(conv3(e0 + up(d1); 16->16)), z = rms(r) (w_h . d0 + b_h),
+ * x = (q, q * feature) per voxel with q = r / rms(r), conv3 = 3x3x3 with
+ * zero padding,
+ * act(v) = v for v > 0, 0.1 v otherwise (leaky ReLU),
  * pool = 2x2x2 mean, up = nearest x2.  Parameters (fp32, in this order):
  * per layer W[cout][27][cin] (tap = dx + 3 dy + 9 dz fastest-x) then b[cout];
  * then the head's 16 weights and bias: nnx_unet_param_count() floats.
+ * Training holds the biases at zero (the seeded init sets them to zero), so
+ * a trained network maps r = 0 to 0 and scales with r.
  * Grids: nx % 64 == 0, ny % 32 == 0, nz % 32 == 0.                          */
 typedef struct nnx_unet nnx_unet;
 int64_t nnx_unet_param_count(void);
@@ -377,7 +381,8 @@ nnx_status nnx_unet_train_step(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int 
 
 /* Single convolution layers (BF16 channels-last tensors as uint16_t, weights
  * fp32 [cout][27 cin]); (cin, cout) in {(2,16), (16,16), (16,32), (32,16),
- * (32,32)}.  out = act(conv3(in + coarse_scale * up(coarse)) + bias), pool =
+ * (32,32)}.  out = act(conv3(in + coarse_scale * up(coarse)) + bias) with
+ * act the leaky ReLU when relu != 0 (identity otherwise), pool =
  * its 2x2x2 mean; wgrad: dw[co][tap cin + ci] = sum_v g[v][co] in'[v + tap][ci]
  * and db[co] = sum_v g[v][co], in' = in + coarse_scale * up(coarse).        */
 nnx_status nnx_conv3_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny, int nz, int batch,
@@ -389,8 +394,13 @@ nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny,
                                 const uint16_t* d_g, float* d_dw, float* d_db);
 
 /* Flexible CG (Notay, one stored direction): fp32 vectors, FP64 reductions,
- * M = the network (precond LEARNED; d_feat is its second input channel) or
- * Jacobi; after o->stall_iters (default 10) iterations without a new best
+ * precond LEARNED: a two-level preconditioner with the network N as the
+ * coarse correction between damped-Jacobi smoothings (w = 2/3, D the
+ * operator's diagonal): z0 = w D^-1 r, z1 = z0 + N(r - A z0), then
+ * o->smooth sweeps z <- z + w D^-1 (r - A z) (at least one); d_feat is N's
+ * feature input.
+ * precond JACOBI: M = D^-1;
+ * after o->stall_iters (default 10) iterations without a new best
  * residual the learned preconditioner is replaced by Jacobi (res reports
  * the preconditioner of the final iterations).                              */
 nnx_status nnx_fcg_solve(nnx_ctx* ctx, const nnx_grid* g, nnx_unet* net, const float* d_feat,

@@ -67,7 +67,7 @@ class Grid(C.Structure):
 class CgOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("maxit", C.c_int32), ("precond", C.c_int32),
                 ("check_every", C.c_int32), ("use_x0", C.c_int32), ("stall_iters", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("smooth", C.c_int32)]
 
 
 class CgResult(C.Structure):

@@ -724,6 +724,39 @@ __global__ __launch_bounds__(256) void k_fcg_jacobi(int64_t n, const float* __re
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) z[i] = dinv[i] * r[i];
 }
 
+// Learned preconditioner, two-level: damped-Jacobi pre-smoothing, the
+// network as the coarse correction of what the smoother leaves, damped-
+// Jacobi post-smoothing:
+//   z0 = w D^-1 r;  z1 = z0 + N(r - A z0);  z = z1 + w D^-1 (r - A z1).
+constexpr float kSmoothW = 2.0f / 3.0f;
+__global__ __launch_bounds__(256) void k_pc_pre(int64_t n, const float* __restrict__ dinv,
+                                                const float* __restrict__ r, float* __restrict__ z0) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        z0[i] = kSmoothW * dinv[i] * r[i];
+}
+// out = r - A z
+__global__ __launch_bounds__(256) void k_pc_res(GridK g, const float* __restrict__ r, const float* __restrict__ z,
+                                                float* __restrict__ out) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
+        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), k = (int)(id / ((int64_t)g.nx * g.ny));
+        out[id] = r[id] - apply_at<float, float>(z, g, i, j, k);
+    }
+}
+__global__ __launch_bounds__(256) void k_pc_add(int64_t n, float* __restrict__ z, const float* __restrict__ u) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) z[i] = z[i] + u[i];
+}
+// zout = z1 + w D^-1 (r - A z1)
+__global__ __launch_bounds__(256) void k_pc_post(GridK g, const float* __restrict__ dinv,
+                                                 const float* __restrict__ r, const float* __restrict__ z1,
+                                                 float* __restrict__ zout) {
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
+        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), k = (int)(id / ((int64_t)g.nx * g.ny));
+        zout[id] = z1[id] + kSmoothW * dinv[id] * (r[id] - apply_at<float, float>(z1, g, i, j, k));
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fcg_dot(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
                                                  double* part) {
     double v[1] = {0};
@@ -928,11 +961,15 @@ double read_scalar(nnx_ctx* c, const double* d, hipError_t* err) {
 // set, Jacobi otherwise or after stall_iters iterations without a new best
 // residual.  The host reads the residual norm after every iteration.
 nnx_status fcg_run(nnx_ctx* c, const GridK& gk, nnx_unet* net, const float* feat, const float* b, float* x,
-                   int use_x0, double rtol, int maxit, int stall_iters, nnx_cg_result* res) {
+                   int use_x0, double rtol, int maxit, int stall_iters, int smooth, nnx_cg_result* res) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
-    void *r, *z, *pa, *pb, *qa, *qb, *dinv, *part, *sc;
+    void *r, *z, *pa, *pb, *qa, *qb, *dinv, *part, *sc, *w1, *w2;
     NNX_TRY(scratch(c, "fcg_r", n * 4, false, &r));
     NNX_TRY(scratch(c, "fcg_z", n * 4, false, &z));
+    if (net) {
+        NNX_TRY(scratch(c, "fcg_w1", n * 4, false, &w1));
+        NNX_TRY(scratch(c, "fcg_w2", n * 4, false, &w2));
+    }
     NNX_TRY(scratch(c, "fcg_pa", n * 4, false, &pa));
     NNX_TRY(scratch(c, "fcg_pb", n * 4, false, &pb));
     NNX_TRY(scratch(c, "fcg_qa", n * 4, false, &qa));
@@ -965,7 +1002,21 @@ nnx_status fcg_run(nnx_ctx* c, const GridK& gk, nnx_unet* net, const float* feat
     while (rel > rtol && it < maxit && rr > 0) {
         const int cur = it & 1, old = cur ^ 1;
         if (learned) {
-            NNX_TRY(nnx_unet_forward_internal(c, net, gk.nx, gk.ny, gk.nz, 1, R, feat, Z));
+            float *W1 = (float*)w1, *W2 = (float*)w2;
+            k_pc_pre<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const float*)dinv, R, W1);
+            k_pc_res<<<kRedBlocks, 256, 0, c->stream>>>(gk, R, W1, W2);
+            NNX_TRY(nnx_unet_forward_internal(c, net, gk.nx, gk.ny, gk.nz, 1, W2, feat, Z));
+            k_pc_add<<<flat_blocks(n), 256, 0, c->stream>>>(n, W1, Z);
+            // post-smoothing sweeps, ping-ponging W1 / Z, ending in Z
+            const int sweeps = smooth > 0 ? smooth : 1;
+            float* src = W1;
+            for (int sw = 0; sw < sweeps; ++sw) {
+                float* dst = ((sweeps - sw) & 1) ? Z : W1;
+                if (dst == src) dst = W2;
+                k_pc_post<<<kRedBlocks, 256, 0, c->stream>>>(gk, (const float*)dinv, R, src, dst);
+                src = dst;
+            }
+            if (src != Z) NNX_HIP(c, hipMemcpyAsync(Z, src, n * 4, hipMemcpyDeviceToDevice, c->stream));
         } else {
             k_fcg_jacobi<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const float*)dinv, R, Z);
         }
@@ -1018,7 +1069,7 @@ nnx_status nnx_fcg_solve(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, const flo
     auto t0 = std::chrono::steady_clock::now();
     nnx_cg_result r{};
     NNX_TRY(fcg_run(c, gk, o->precond == NNX_PRECOND_LEARNED ? net : nullptr, d_feat, b, x, o->use_x0, o->rtol,
-                    o->maxit, o->stall_iters, &r));
+                    o->maxit, o->stall_iters, o->smooth, &r));
     auto t1 = std::chrono::steady_clock::now();
     CgHost h;
     NNX_TRY(cg_buffers(c, gk, NNX_PRECOND_NONE, &h));
@@ -1064,7 +1115,7 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
         const double tol = std::max(inner_rtol, 0.5 * o->rtol / rel);
         nnx_cg_result ir{};
         NNX_TRY(fcg_run(c, gk, o->precond == NNX_PRECOND_LEARNED ? net : nullptr, d_feat, (const float*)rhs,
-                        (float*)d, 0, tol, o->maxit, o->stall_iters, &ir));
+                        (float*)d, 0, tol, o->maxit, o->stall_iters, o->smooth, &ir));
         inner_total += ir.iters;
         if (ir.precond_used == NNX_PRECOND_JACOBI) used = NNX_PRECOND_JACOBI;
         k_axpy_f64<<<flat_blocks(n), 256, 0, c->stream>>>(n, 1.0 / s, (const float*)d, x);

@@ -4,14 +4,17 @@
 // No counterpart in the reference (its paracrine solve is plain CG,
 // src/paracrine.cu:106-129).
 //
-// Network (x = [r / rms(r), feature], channels-last BF16 activations):
-//   e0 = relu(conv3(x; 2->16))            level 0    p0 = avgpool2(e0)
-//   e1 = relu(conv3(p0; 16->32))          level 1    p1 = avgpool2(e1)
-//   e2 = relu(conv3(p1; 32->32))          level 2
-//   d1 = relu(conv3(e1 + up(e2); 32->16)) level 1    (up: nearest, x2)
-//   d0 = relu(conv3(e0 + up(d1); 16->16)) level 0
+// Network (x = [q, q * feature], q = r / rms(r), channels-last BF16
+// activations; no biases: they are held at zero so that N(0) = 0 and
+// N(a r) = a N(r) for a > 0, as a preconditioner needs):
+//   e0 = act(conv3(x; 2->16))            level 0    p0 = avgpool2(e0)
+//   e1 = act(conv3(p0; 16->32))          level 1    p1 = avgpool2(e1)
+//   e2 = act(conv3(p1; 32->32))          level 2
+//   d1 = act(conv3(e1 + up(e2); 32->16)) level 1    (up: nearest, x2)
+//   d0 = act(conv3(e0 + up(d1); 16->16)) level 0
 //   z  = rms(r) * (w_h . d0 + b_h)        1x1 head, fp32
-// conv3 is a 3x3x3 convolution with zero padding.  63,201 parameters.
+// conv3 is a 3x3x3 convolution with zero padding, act the leaky ReLU
+// (slope 0.1 below zero).  63,201 parameters.
 //
 // Convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16: D[co][voxel] +=
 // W[co][k] * X[k][voxel], k = tap*Cin + ci (taps padded to a multiple of 32).
@@ -50,6 +53,7 @@ __device__ __forceinline__ f32x4_t mfma16(u32x4_t a, u32x4_t b, f32x4_t c) {
                                                    __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
 }
 constexpr int kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
+constexpr float kLeak = 0.1f;  // slope of the activation below zero (leaky ReLU)
 __device__ __forceinline__ int tap_dz(int t) { return t / 9; }
 __device__ __forceinline__ int tap_dy(int t) { return (t / 3) % 3; }
 __device__ __forceinline__ int tap_dx(int t) { return t % 3; }
@@ -101,77 +105,134 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
     const int nzt = nz / TZ;
     const int b = blockIdx.z / nzt, z0 = (blockIdx.z % nzt) * TZ, y0 = blockIdx.y * TY,
               x0 = blockIdx.x * 16;
-    for (int i = tid; i < COUT * (KP / 8); i += 256) {
-        const int co = i / (KP / 8), k8 = i - co * (KP / 8);
-        *(u32x4_t*)&s_w[co * KS + 8 * k8] = *(const u32x4_t*)&a.w[(size_t)co * KP + 8 * k8];
+    // Staging: every load of a thread is issued before the first one is
+    // used, so a workgroup pays one memory latency, not one per chunk.
+    {
+        constexpr int NW = COUT * (KP / 8), WI = (NW + 255) / 256;
+        u32x4_t wv[WI];
+#pragma unroll
+        for (int u = 0; u < WI; ++u) {
+            const int i = tid + 256 * u;
+            if (i < NW) {
+                const int co = i / (KP / 8), k8 = i - co * (KP / 8);
+                wv[u] = *(const u32x4_t*)&a.w[(size_t)co * KP + 8 * k8];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < WI; ++u) {
+            const int i = tid + 256 * u;
+            if (i < NW) {
+                const int co = i / (KP / 8), k8 = i - co * (KP / 8);
+                *(u32x4_t*)&s_w[co * KS + 8 * k8] = wv[u];
+            }
+        }
     }
     if (CIN == 2) {
-        for (int hv = tid; hv < HV; hv += 256) {
+        constexpr int IT = (HV + 255) / 256;
+        uint32_t v[IT];
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            const int hv = tid + 256 * u;
+            v[u] = 0;
             const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
             const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-            uint32_t v = 0;
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
-                v = *(const uint32_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * 2];
-            *(uint32_t*)&s_in[hv * 2] = v;
+            if (hv < HV && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+                v[u] = *(const uint32_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * 2];
+        }
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            const int hv = tid + 256 * u;
+            if (hv < HV) *(uint32_t*)&s_in[hv * 2] = v[u];
         }
     } else {
-        constexpr int CH = CIN / 8;
-        for (int i = tid; i < HV * CH; i += 256) {
-            const int hv = i / CH, c8 = i - hv * CH;
-            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-            u32x4_t v = {0u, 0u, 0u, 0u};
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) {
-                v = *(const u32x4_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * CIN + 8 * c8];
-                if (a.coarse) {
-                    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
-                    const u32x4_t cv = *(const u32x4_t*)&a.coarse
-                        [((((size_t)b * cz + (gz >> 1)) * cy + (gy >> 1)) * cx + (gx >> 1)) * CIN + 8 * c8];
-                    v = add_scaled8(v, cv, a.cscale);
+        constexpr int CH = CIN / 8, NCH = HV * CH, IT = (NCH + 255) / 256, GS = 8;
+#pragma unroll
+        for (int u0 = 0; u0 < IT; u0 += GS) {
+            u32x4_t v[GS], cv[GS];
+#pragma unroll
+            for (int u = 0; u < GS; ++u) {
+                const int i = tid + 256 * (u0 + u);
+                v[u] = u32x4_t{0u, 0u, 0u, 0u};
+                cv[u] = u32x4_t{0u, 0u, 0u, 0u};
+                const int hv = i / CH, c8 = i - hv * CH;
+                const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+                const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+                if (u0 + u < IT && i < NCH && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 &&
+                    gz < nz) {
+                    v[u] = *(const u32x4_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * CIN + 8 * c8];
+                    if (a.coarse) {
+                        const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+                        cv[u] = *(const u32x4_t*)&a.coarse
+                            [((((size_t)b * cz + (gz >> 1)) * cy + (gy >> 1)) * cx + (gx >> 1)) * CIN + 8 * c8];
+                    }
                 }
             }
-            *(u32x4_t*)&s_in[hv * CIN + 8 * c8] = v;
+#pragma unroll
+            for (int u = 0; u < GS; ++u) {
+                const int i = tid + 256 * (u0 + u);
+                if (u0 + u < IT && i < NCH)
+                    *(u32x4_t*)&s_in[i * 8] = a.coarse ? add_scaled8(v[u], cv[u], a.cscale) : v[u];
+            }
         }
     }
     __syncthreads();
+    // Rows of this wave: quads of 2x2 (y,z) rows.  Row r sits at
+    // (wy + ryc(r), wz + rzc(r)) in the tile, with ryc / rzc compile-time
+    // constants, so its LDS address is a per-lane base plus an immediate.
+    const int wy = TY == 8 ? 0 : 2 * (wave & 1), wz = TY == 8 ? 2 * wave : 2 * (wave >> 1);
+    const int vbase = ((wz * HY + wy) * HX + xl) * CIN;
+    auto ryc = [](int r) { return (TY == 8 ? 2 * (r >> 2) : 0) + (r & 1); };
+    auto rzc = [](int r) { return (r >> 1) & 1; };
+    auto toff = [&](int tap, int coff) {
+        tap = tap > 26 ? 26 : tap;  // padded k: zero weights
+        return ((tap_dz(tap) * HY + tap_dy(tap)) * HX + tap_dx(tap)) * CIN + coff;
+    };
     f32x4_t acc[MT][RPW];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int r = 0; r < RPW; ++r) acc[mt][r] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
     for (int kb = 0; kb < NKB; ++kb) {
         u32x4_t af[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) af[mt] = *(const u32x4_t*)&s_w[(16 * mt + xl) * KS + 32 * kb + 8 * g];
+        u32x4_t bv[RPW];
+        if (CIN == 2) {
+            int o[4];
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-            const int qd = wave * QPW + (r >> 2), t = r & 3;
-            const int ry = 2 * (qd % (TY / 2)) + (t & 1), rz = 2 * (qd / (TY / 2)) + (t >> 1);
-            u32x4_t bv;
-            if (CIN == 2) {
+            for (int u = 0; u < 4; ++u) o[u] = vbase + toff(16 * kb + 4 * g + u, 0);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    int tap = 16 * kb + 4 * g + u;
-                    tap = tap > 26 ? 26 : tap;  // padded k: zero weights
-                    bv[u] = *(const uint32_t*)&s_in[(((rz + tap_dz(tap)) * HY + ry + tap_dy(tap)) * HX + xl +
-                                                     tap_dx(tap)) * 2];
-                }
-            } else {
-                int tap = CIN == 16 ? 2 * kb + (g >> 1) : kb;
-                tap = tap > 26 ? 26 : tap;
-                const int coff = CIN == 16 ? 8 * (g & 1) : 8 * g;
-                bv = *(const u32x4_t*)&s_in[(((rz + tap_dz(tap)) * HY + ry + tap_dy(tap)) * HX + xl +
-                                             tap_dx(tap)) * CIN + coff];
+            for (int r = 0; r < RPW; ++r) {
+                const int ro = (rzc(r) * HY + ryc(r)) * HX * CIN;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) bv[r][u] = *(const uint32_t*)&s_in[o[u] + ro];
             }
+        } else {
+            const int o = vbase + (CIN == 16 ? toff(2 * kb + (g >> 1), 8 * (g & 1)) : toff(kb, 8 * g));
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) acc[mt][r] = mfma16(af[mt], bv, acc[mt][r]);
+            for (int r = 0; r < RPW; ++r)
+                bv[r] = *(const u32x4_t*)&s_in[o + (rzc(r) * HY + ryc(r)) * HX * CIN];
         }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[mt][r] = mfma16(af[mt], bv[r], acc[mt][r]);
     }
     // epilogue: lane holds voxel x0 + xl, channels 16 mt + 4 g + q
+    float bias_r[MT][4], head_r[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int co = 16 * mt + 4 * g + q;
+            bias_r[mt][q] = a.bias ? a.bias[co] : 0.f;
+            head_r[mt][q] = a.head ? a.head[co] : 0.f;
+        }
+    const float hb = a.head ? a.head[COUT] : 0.f, hs = a.hscale ? a.hscale[b] : 1.f;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-        const int qd = wave * QPW + (r >> 2), t = r & 3;
-        const int y = y0 + 2 * (qd % (TY / 2)) + (t & 1), z = z0 + 2 * (qd / (TY / 2)) + (t >> 1);
+        const int y = y0 + wy + ryc(r), z = z0 + wz + rzc(r);
         const size_t vox = (((size_t)b * nz + z) * ny + y) * nx + x0 + xl;
         float hsum = 0.f;
 #pragma unroll
@@ -179,11 +240,10 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
             f32x4_t v = acc[mt][r];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int co = 16 * mt + 4 * g + q;
-                float s = v[q] + (a.bias ? a.bias[co] : 0.f);
-                if (a.relu) s = fmaxf(s, 0.f);
+                float s = v[q] + bias_r[mt][q];
+                if (a.relu) s = s > 0.f ? s : kLeak * s;
                 v[q] = s;
-                if (a.head) hsum += a.head[co] * s;
+                hsum += head_r[mt][q] * s;
             }
             acc[mt][r] = v;
             if (a.out) {
@@ -196,15 +256,14 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
         if (a.head) {
             hsum += __shfl_xor(hsum, 16, 64);
             hsum += __shfl_xor(hsum, 32, 64);
-            if (g == 0) a.hout[vox] = (a.hscale ? a.hscale[b] : 1.f) * (hsum + a.head[COUT]);
+            if (g == 0) a.hout[vox] = hs * (hsum + hb);
         }
     }
     if (a.pool) {
         const int px = nx >> 1, py = ny >> 1, pz = nz >> 1;
 #pragma unroll
         for (int j = 0; j < QPW; ++j) {
-            const int qd = wave * QPW + j;
-            const int y = (y0 >> 1) + qd % (TY / 2), z = (z0 >> 1) + qd / (TY / 2);
+            const int y = (y0 + wy + ryc(4 * j)) >> 1, z = (z0 + wz + rzc(4 * j)) >> 1;
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 f32x4_t s = ((acc[mt][4 * j] + acc[mt][4 * j + 1]) + acc[mt][4 * j + 2]) + acc[mt][4 * j + 3];
@@ -393,30 +452,40 @@ __global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ x, int6
     __syncthreads();
     if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
-// s[b] = sqrt(sum / nper); inv[b] = 1/s (0 for a zero field); sum[b] kept
-__global__ void k_rms_fin(const double* __restrict__ part, int nblk, int64_t nper, int batch, float* __restrict__ s,
-                          float* __restrict__ inv, double* __restrict__ sum) {
-    const int b = threadIdx.x;
-    if (b >= batch) return;
+// s[b] = sqrt(sum / nper); inv[b] = 1/s (0 for a zero field); sum[b] kept.
+// One 256-thread block per batch member, fixed summation order.
+__global__ __launch_bounds__(256) void k_rms_fin(const double* __restrict__ part, int nblk, int64_t nper,
+                                                 int batch, float* __restrict__ s, float* __restrict__ inv,
+                                                 double* __restrict__ sum) {
+    __shared__ double sh[4];
+    const int b = blockIdx.x;
     double t = 0;
-    for (int i = 0; i < nblk; ++i) t += part[b * nblk + i];
-    const double r = std::sqrt(t / (double)nper);
-    s[b] = (float)r;
-    inv[b] = r > 0 ? (float)(1.0 / r) : 0.f;
-    if (sum) sum[b] = t;
+    for (int i = threadIdx.x; i < nblk; i += 256) t += part[b * nblk + i];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        t = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        const double r = std::sqrt(t / (double)nper);
+        s[b] = (float)r;
+        inv[b] = r > 0 ? (float)(1.0 / r) : 0.f;
+        if (sum) sum[b] = t;
+    }
 }
 
-// x[v] = (bf16(r[v] inv[b]), bf16(feat[v]))
+// x[v] = (bf16(q), bf16(q feat[v])) with q = r[v] inv[b]: both channels are
+// linear in r, so the bias-free network satisfies N(0) = 0
 __global__ void k_pack(const float* __restrict__ r, const float* __restrict__ feat, const float* __restrict__ inv,
                        int64_t nper, int batch, uint32_t* __restrict__ x) {
     const int64_t n = nper * batch;
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const int b = (int)(i / nper);
-        x[i] = f2bf(r[i] * inv[b]) | (f2bf(feat ? feat[i] : 0.f) << 16);
+        const float q = r[i] * inv[b];
+        x[i] = f2bf(q) | (f2bf(feat ? q * feat[i] : 0.f) << 16);
     }
 }
 
-// G = (act > 0) ? skip + cscale * coarse[parent] : 0, channels C (multiple of 8)
+// G = (skip + cscale * coarse[parent]) * (act > 0 ? 1 : kLeak), channels C (multiple of 8)
 __global__ void k_grad_combine(int nx, int ny, int nz, int batch, int C, const uint16_t* __restrict__ act,
                                const uint16_t* __restrict__ skip, const uint16_t* __restrict__ coarse, float cscale,
                                uint16_t* __restrict__ out) {
@@ -436,9 +505,9 @@ __global__ void k_grad_combine(int nx, int ny, int nz, int batch, int C, const u
         u32x4_t o;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const uint32_t lo = bf2f(m[q] & 0xffffu) > 0.f ? (s[q] & 0xffffu) : 0u;
-            const uint32_t hi = bf2f(m[q] >> 16) > 0.f ? (s[q] >> 16) : 0u;
-            o[q] = lo | (hi << 16);
+            const float lo = bf2f(s[q] & 0xffffu), hi = bf2f(s[q] >> 16);
+            o[q] = f2bf(bf2f(m[q] & 0xffffu) > 0.f ? lo : kLeak * lo) |
+                   (f2bf(bf2f(m[q] >> 16) > 0.f ? hi : kLeak * hi) << 16);
         }
         *(u32x4_t*)&out[v * C + 8 * c8] = o;
     }
@@ -481,8 +550,9 @@ __global__ __launch_bounds__(256) void k_t2(const float* __restrict__ e, const f
 }
 
 // Loss (1/B) sum_b |z_b - t_b|^2 / T2_b and its gradient through the head:
-// dZ = 2 (z - t) / (B T2_b); G0 = (d0 > 0) hw dZ; partials of dhw, dhb, loss.
+// dZ = 2 (z - t) / (B T2_b); G0 = hw dZ act'(d0); partials of dhw, dhb, loss.
 constexpr int kHeadBlocks = 512;
+constexpr int kSumBlocks = 1024;  // blocks per batch member of the sum-of-squares reductions
 __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ z, const float* __restrict__ e,
                                                   const float* __restrict__ inv, const double* __restrict__ t2,
                                                   int64_t nper, int batch, const uint16_t* __restrict__ d0,
@@ -507,8 +577,9 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ z, c
                 const int c = 8 * c8 + 2 * q;
                 acc[c] += (double)lo * dz;
                 acc[c + 1] += (double)hi * dz;
-                const uint32_t glo = lo > 0.f ? f2bf(head[c] * dz) : 0u;
-                const uint32_t ghi = hi > 0.f ? f2bf(head[c + 1] * dz) : 0u;
+                const float gl = head[c] * dz, gh = head[c + 1] * dz;
+                const uint32_t glo = f2bf(lo > 0.f ? gl : kLeak * gl);
+                const uint32_t ghi = f2bf(hi > 0.f ? gh : kLeak * gh);
                 o[4 * c8 + q] = glo | (ghi << 16);
             }
         }
@@ -702,7 +773,7 @@ nnx_status acts(nnx_ctx* c, int64_t n0, int batch, bool train, Acts* A) {
     NNX_TRY(scratch(c, "un_s", 4096, false, &p));
     A->s = (float*)p;
     A->inv = (float*)p + 512;
-    NNX_TRY(scratch(c, "un_part", (size_t)std::max(256 * batch, kHeadBlocks * 18) * 8 + 4096, false, &p));
+    NNX_TRY(scratch(c, "un_part", (size_t)std::max(kSumBlocks * batch, kHeadBlocks * 18) * 8 + 4096, false, &p));
     A->part = (double*)p;
     (void)batch;
     return NNX_OK;
@@ -722,8 +793,8 @@ nnx_status forward(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, c
     const Plan& P = plan();
     const int64_t nper = (int64_t)nx * ny * nz, n0 = nper * batch;
     NNX_TRY(prep_weights(c, u));
-    k_sumsq<<<dim3(256, batch), 256, 0, c->stream>>>(r, nper, A.part);
-    k_rms_fin<<<1, 256, 0, c->stream>>>(A.part, 256, nper, batch, A.s, A.inv, nullptr);
+    k_sumsq<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(r, nper, A.part);
+    k_rms_fin<<<batch, 256, 0, c->stream>>>(A.part, kSumBlocks, nper, batch, A.s, A.inv, nullptr);
     k_pack<<<eblocks(n0), 256, 0, c->stream>>>(r, feat, A.inv, nper, batch, (uint32_t*)A.x);
     NNX_LAUNCHED(c);
     ConvArgs a{};
@@ -790,7 +861,8 @@ nnx_status nnx_unet_create(nnx_ctx* c, const float* h_params, uint64_t seed, nnx
             const double bound = std::sqrt(6.0 / (27.0 * L.cin));
             for (int64_t i = 0; i < (int64_t)L.cout * 27 * L.cin; ++i) h[L.w_off + i] = (float)(bound * unif(L.w_off + i));
         }
-        const double bound = std::sqrt(3.0 / 16.0);
+        // a small head: the untrained network's output starts well below the target
+        const double bound = 0.1 * std::sqrt(3.0 / 16.0);
         for (int i = 0; i < 16; ++i) h[P.head_off + i] = (float)(bound * unif(P.head_off + i));
     }
     nnx_unet* u = new nnx_unet();
@@ -880,8 +952,8 @@ nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, 
     NNX_TRY(scratch(c, "un_t2", 256 * 8, false, &p));
     double* t2 = (double*)p;
     // loss and head
-    k_t2<<<dim3(256, batch), 256, 0, c->stream>>>(d_e, A.inv, nper, A.part);
-    k_rms_fin<<<1, 256, 0, c->stream>>>(A.part, 256, 1, batch, A.s + 256, A.inv + 256, t2);
+    k_t2<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(d_e, A.inv, nper, A.part);
+    k_rms_fin<<<batch, 256, 0, c->stream>>>(A.part, kSumBlocks, 1, batch, A.s + 256, A.inv + 256, t2);
     k_head_bwd<<<kHeadBlocks, 256, 0, c->stream>>>(A.z, d_e, A.inv, t2, nper, batch, A.d0, u->p + P.head_off, G0,
                                                    A.part);
     k_head_fin<<<1, 64, 0, c->stream>>>(A.part, kHeadBlocks, u->grad + P.head_off, (double*)lossp, batch);
@@ -926,6 +998,10 @@ nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, 
     NNX_LAUNCHED(c);
     NNX_TRY(wgrad_dispatch(c, 2, 16, WG(nx, ny, nz, A.x, nullptr, 0.f, G0), u->grad + P.L[0].w_off,
                            u->grad + P.L[0].b_off));
+    // the biases are held at zero (the network must map r = 0 to 0)
+    for (int l = 0; l < 5; ++l)
+        NNX_HIP(c, hipMemsetAsync(u->grad + P.L[l].b_off, 0, P.L[l].cout * 4, c->stream));
+    NNX_HIP(c, hipMemsetAsync(u->grad + P.head_off + 16, 0, 4, c->stream));
     // Adam
     u->step += 1;
     const float b1 = 0.9f, b2 = 0.999f;

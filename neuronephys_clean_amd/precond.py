@@ -18,6 +18,7 @@ import numpy as np
 from . import _lib, configs
 from .devmem import DeviceArray as DA
 
+SMOOTH_W = np.float32(2.0 / 3.0)  # damping of the Jacobi smoothings around the network
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_c5.f32")
 
 
@@ -79,6 +80,13 @@ class UNet:
             pass
 
 
+def jacobi_inverse(f: configs.DiffusionField) -> np.ndarray:
+    """1 / diagonal of the variable 7-point operator, flattened (x fastest)."""
+    d = (1.0 + f.kx[:, :, :-1].astype(np.float64) + f.kx[:, :, 1:] + f.ky[:, :-1, :] + f.ky[:, 1:, :]
+         + f.kz[:-1] + f.kz[1:])
+    return (1.0 / d).astype(np.float32).ravel()
+
+
 def device_field(ctx, f: configs.DiffusionField):
     """Grid struct and device arrays (kept alive by the returned dict)."""
     d = {"kx": DA.from_numpy(ctx, f.kx), "ky": DA.from_numpy(ctx, f.ky),
@@ -137,38 +145,55 @@ def _cg(ctx, g, b: DA, x: DA, rtol, maxit, use_x0=0):
 
 def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 64), seed=1,
             log=print) -> PairSet:
-    """Pairs (r_k, e_k = x* - x_k) from Jacobi-CG iterates x_k (x_0 = 0) on
-    `fields` fresh log-normal diffusivity fields (C5 statistics at n^3) and
-    `sources` random point-source right-hand sides per field."""
+    """Pairs (r_k, e_k) from Jacobi-CG iterates x_k (x_0 = 0) on `fields`
+    fresh log-normal diffusivity fields (C5 statistics at n^3) and `sources`
+    random point-source right-hand sides per field: e_k = x* - x_k with x*
+    the FP64-refined solution (relative residual 1e-11), and r_k = A e_k
+    (= b - A x_k up to the residual of x*), so that every pair satisfies
+    its equation to fp32 rounding.  The network is the coarse correction of
+    a two-level preconditioner after damped-Jacobi pre-smoothing
+    (z0 = w D^-1 r, w = 2/3), so the stored pair is what the smoother
+    leaves: e1 = e_k - w D^-1 r_k and r1 = A e1."""
     ps = PairSet(ctx, n, fields * sources * len(schedule), fields)
     vox = n ** 3
-    xs, xk, ax = DA(ctx, vox, np.float32), DA(ctx, vox, np.float32), DA(ctx, vox, np.float32)
+    x64, xk, ek, rk = (DA(ctx, vox, np.float64), DA(ctx, vox, np.float32), DA(ctx, vox, np.float32),
+                       DA(ctx, vox, np.float32))
     t0 = time.time()
     for f in range(fields):
         fld = configs.c5_field(seed=seed * 100003 + f, n=n)
         g, keep = device_field(ctx, fld)
         ps.add_field(f, fld.feat)
+        dinv = jacobi_inverse(fld)
         for s in range(sources):
             cnt = int(20 + (s * 37 + f * 11) % 380)
             b = configs.point_sources(seed * 100003 + f * 1009 + s, n, cnt)
-            db = DA.from_numpy(ctx, b)
-            _cg(ctx, g, db, xs, 1e-7, 5000)
-            xstar = xs.numpy()
+            db, db64 = DA.from_numpy(ctx, b), DA.from_numpy(ctx, b.astype(np.float64))
+            o = _lib.CgOpts(rtol=1e-11, maxit=5000, precond=_lib.PRECOND_JACOBI, check_every=8)
+            ctx.call("nnx_cg_solve_refine", C.byref(g), db64.ptr, x64.ptr, C.byref(o), 1e-6, 30,
+                     C.byref(_lib.CgResult()))
+            xstar = x64.numpy()
             for k in schedule:
                 if k == 0:
-                    ps.add_pair(b, xstar, f)
-                    continue
-                _cg(ctx, g, db, xk, 0.0, k)
-                ctx.call("nnx_stencil_apply", C.byref(g), xk.ptr, ax.ptr)
-                ps.add_pair(b - ax.numpy(), xstar - xk.numpy(), f)
+                    e = xstar.astype(np.float32)
+                else:
+                    _cg(ctx, g, db, xk, 0.0, k)
+                    e = (xstar - xk.numpy()).astype(np.float32)
+                ek.upload(e)
+                ctx.call("nnx_stencil_apply", C.byref(g), ek.ptr, rk.ptr)
+                e1 = (e - SMOOTH_W * dinv * rk.numpy()).astype(np.float32)
+                ek.upload(e1)
+                ctx.call("nnx_stencil_apply", C.byref(g), ek.ptr, rk.ptr)
+                ps.add_pair(rk.numpy(), e1, f)
         if (f + 1) % 20 == 0:
             log(f"harvest: {f + 1}/{fields} fields, {ps.count} pairs, {time.time() - t0:.1f}s")
         del keep
     return ps
 
 
-def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=2e-3, lr_min=1e-4, seed=0, log=print):
-    """Adam with a cosine learning-rate schedule over shuffled pairs."""
+def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=3e-4, lr_min=1e-5, warmup=500, seed=0,
+          log=print):
+    """Adam, linear warm-up then a cosine learning-rate schedule, over
+    shuffled pairs."""
     n, vox = ps.n, ps.n ** 3
     rb, fb, eb = (DA(ctx, (batch, vox), np.float32) for _ in range(3))
     steps = epochs * (ps.count // batch)
@@ -180,6 +205,8 @@ def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=2e-3, lr_min=1e-4, 
             idx = perm[j * batch:(j + 1) * batch]
             ps.gather(idx, rb, fb, eb)
             cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / max(steps, 1)))
+            if step < warmup:
+                cur = lr * (step + 1) / warmup
             want = (step % 50 == 0) or step == steps - 1
             loss = net.train_step((n, n, n), batch, rb, fb, eb, cur, want_loss=want)
             if want:
