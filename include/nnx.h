@@ -341,6 +341,52 @@ nnx_status nnx_csr_spmv(nnx_ctx* ctx, int64_t rows, const int32_t* d_offsets, co
                         const float* d_vals, const float* d_x, float* d_y, int accumulate);
 
 /* ------------------------------------------------------------------------ */
+/* ensembles of independent networks (no counterpart in the reference,      */
+/* which simulates one network, src/main.cu:112-182)                         */
+/* ------------------------------------------------------------------------ */
+
+/* nnet networks in one index space: network i owns neurons
+ * [net_begin[i], net_begin[i] + net_size[i]) (net_begin[i] a multiple of 64,
+ * networks in ascending order), edges only within a network.  The step is
+ * the RK4 model of nnx_run_steps (exponential synapses, 0 mV crossing,
+ * event-driven weighted coupling in ascending source order), fp32.          */
+typedef struct nnx_ens_plan {
+    nnx_hh_state state;       /* fp32; n = the padded total (a multiple of 64) */
+    nnx_hh_params params;
+    nnx_graph graph;          /* block-diagonal CSR, CSC and out_to_in over the same index space */
+    int32_t nnet, pad_;
+    const int64_t* net_begin; /* host arrays [nnet] */
+    const int64_t* net_size;
+    double dt;
+    uint64_t* bits[2];        /* spike bitmaps by step parity (n / 64 words each) */
+    const int32_t* voxel_of;  /* optional: +1 at voxel_counts[voxel_of[i]] per spike of i */
+    int32_t* voxel_counts;
+    int32_t* net_spikes;      /* optional [nnet]: spikes per network, accumulated */
+    int32_t* raster;          /* optional (step, neuron) pairs; within a step unordered */
+    int64_t* raster_len;
+    int64_t raster_cap;
+} nnx_ens_plan;
+
+/* launch shape of the fused step: threads per workgroup (64, 128, 256, 512),
+ * neurons per tile (a multiple of 64), persistent workgroups.              */
+typedef struct nnx_ens_config {
+    int32_t block, tile, grid, pad_;
+} nnx_ens_config;
+
+/* n_steps steps from first_step (first_step == 0 clears the pending
+ * coupling and the spike bitmaps) with one launch per step, optionally
+ * replayed from a captured hipGraph; cfg NULL = {256, 2048, 1024}.  The
+ * result does not depend on cfg.                                            */
+nnx_status nnx_ensemble_run(nnx_ctx* ctx, const nnx_ens_plan* plan, const nnx_ens_config* cfg,
+                            uint64_t first_step, int n_steps, int use_graph);
+/* Coordinate search over block, tile and grid, timing steps_per_candidate
+ * real steps per candidate (the simulation advances by *steps_run steps;
+ * the trajectory is the same as with any configuration).                   */
+nnx_status nnx_ensemble_autotune(nnx_ctx* ctx, const nnx_ens_plan* plan, uint64_t first_step,
+                                 int steps_per_candidate, nnx_ens_config* best, double* best_us_per_step,
+                                 int* steps_run);
+
+/* ------------------------------------------------------------------------ */
 /* learned preconditioner (no counterpart in the reference: its paracrine    */
 /* solve is unpreconditioned CG, src/paracrine.cu:106-129)                   */
 /* ------------------------------------------------------------------------ */

@@ -87,6 +87,20 @@ class StepPlan(C.Structure):
                 ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64)]
 
 
+class EnsPlan(C.Structure):
+    _fields_ = [("state", HHState), ("params", HHParams), ("graph", Graph), ("nnet", C.c_int32),
+                ("pad_", C.c_int32), ("net_begin", vp), ("net_size", vp), ("dt", C.c_double),
+                ("bits", vp * 2), ("voxel_of", vp), ("voxel_counts", vp), ("net_spikes", vp),
+                ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64)]
+
+
+class EnsConfig(C.Structure):
+    _fields_ = [("block", C.c_int32), ("tile", C.c_int32), ("grid", C.c_int32), ("pad_", C.c_int32)]
+
+    def as_dict(self):
+        return {"block": self.block, "tile": self.tile, "grid": self.grid}
+
+
 _P = C.POINTER
 _SIGS = {
     "nnx_create": ([_P(vp), C.c_int, vp], C.c_int),
@@ -141,6 +155,9 @@ _SIGS = {
                         C.c_double, vp, vp, C.c_int, vp, vp], C.c_int),
     "nnx_conv3_wgrad_bf16": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
                               C.c_double, vp, vp, vp], C.c_int),
+    "nnx_ensemble_run": ([vp, _P(EnsPlan), _P(EnsConfig), C.c_uint64, C.c_int, C.c_int], C.c_int),
+    "nnx_ensemble_autotune": ([vp, _P(EnsPlan), C.c_uint64, C.c_int, _P(EnsConfig), _P(C.c_double),
+                               _P(C.c_int)], C.c_int),
     "nnx_fcg_solve": ([vp, _P(Grid), vp, vp, vp, vp, _P(CgOpts), _P(CgResult)], C.c_int),
     "nnx_fcg_solve_refine": ([vp, _P(Grid), vp, vp, vp, vp, _P(CgOpts), C.c_double, C.c_int,
                               _P(CgResult)], C.c_int),

@@ -166,3 +166,64 @@ def c4_member(seed: int, index: int, n: int, lam_deg: float = 50.0) -> Network:
     pos = host.uniform(s, P_POS, 3 * n, 0.0, 64.0).astype(np.float32).reshape(n, 3)
     return Network(n, off, tgt.astype(np.int32), w, V0, m0, h0, n0, iext, pos,
                    grid=(64, 64, 64), h_grid=1.0, c=1.0, name=f"c4_{index}_n{n}")
+
+
+@dataclass
+class Ensemble:
+    """Independent networks in one index space: network i owns neurons
+    [begin[i], begin[i] + size[i]), begin[i] a multiple of 64; arrays are
+    over the padded total n (a multiple of 64)."""
+    n: int
+    begin: np.ndarray
+    size: np.ndarray
+    off: np.ndarray
+    tgt: np.ndarray
+    w: np.ndarray
+    V0: np.ndarray
+    m0: np.ndarray
+    h0: np.ndarray
+    n0: np.ndarray
+    iext: np.ndarray
+    voxel_of: np.ndarray
+    dt: float = 0.01
+    members: list = field(default=None, repr=False)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.off[-1])
+
+
+def c4_ensemble(seed: int = 0, count: int = 256, sizes=None, keep_members: bool = False) -> Ensemble:
+    """The batched-ensemble configuration: `count` networks with
+    log-uniform sizes 2^U(10,16), Poisson(50) out-degree, uniform targets,
+    w ~ U[0, 0.1], I_ext ~ N(mu_i, 2) with mu_i ~ U(6, 14), each with its own
+    64^3 release grid (voxel ids of network i start at i * 64^3)."""
+    sizes = c4_sizes(seed, count) if sizes is None else np.asarray(sizes, np.int64)
+    begin = np.zeros(len(sizes), np.int64)
+    pos = 0
+    for i, s in enumerate(sizes):
+        begin[i] = pos
+        pos = (pos + int(s) + 63) // 64 * 64
+    n = max(pos, 64)
+    deg = np.zeros(n, np.int64)
+    parts_t, parts_w, members = [], [], []
+    V0 = np.full(n, -65.0)
+    m0, h0, n0, iext = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    vox = np.zeros(n, np.int32)
+    for i, s in enumerate(sizes):
+        net = c4_member(seed, i, int(s))
+        b = int(begin[i])
+        deg[b:b + s] = np.diff(net.off)
+        parts_t.append(net.tgt.astype(np.int32) + np.int32(b))
+        parts_w.append(net.w)
+        V0[b:b + s], m0[b:b + s], h0[b:b + s], n0[b:b + s], iext[b:b + s] = (
+            net.V0, net.m0, net.h0, net.n0, net.iext)
+        local = np.minimum(np.floor(net.pos).astype(np.int64), 63)
+        vox[b:b + s] = (i * 64 ** 3 + local[:, 0] + 64 * (local[:, 1] + 64 * local[:, 2])).astype(np.int32)
+        if keep_members:
+            members.append(net)
+    off = np.zeros(n + 1, np.int32)
+    np.cumsum(deg, out=off[1:])
+    return Ensemble(n, begin, sizes.copy(), off, np.concatenate(parts_t) if parts_t else np.zeros(0, np.int32),
+                    np.concatenate(parts_w) if parts_w else np.zeros(0, np.float32), V0, m0, h0, n0, iext, vox,
+                    members=members if keep_members else None)

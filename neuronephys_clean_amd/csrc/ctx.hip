@@ -16,6 +16,12 @@ nnx_status scratch(nnx_ctx* c, const char* name, size_t bytes, bool zero, void**
             NNX_HIP(c, hipFree(b.p));
             b.p = nullptr;
             b.bytes = 0;
+            // cached graphs hold the old address: drop them (re-captured on use)
+            for (auto& g : c->graphs) {
+                if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+                if (g.second.graph) (void)hipGraphDestroy(g.second.graph);
+            }
+            c->graphs.clear();
         }
         size_t alloc = bytes < 256 ? 256 : bytes;
         NNX_HIP(c, hipMalloc(&b.p, alloc));

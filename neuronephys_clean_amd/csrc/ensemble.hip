@@ -1,0 +1,360 @@
+// ensemble.hip - many independent HH networks of uneven size stepped by one
+// persistent fused kernel per timestep.
+//
+// The networks share one index space (network i owns neurons
+// [begin_i, begin_i + size_i), begin_i a multiple of 64, so that no spike
+// word straddles two networks) and one block-diagonal CSR/CSC.  A timestep
+// is one launch of a fixed grid of workgroups that walk a static list of
+// tiles (a network's neurons cut into pieces of `tile`, largest first,
+// dealt round-robin).  For each tile: the event-driven coupling input of
+// the previous step's spikes (bucket entries summed in ascending source
+// order), the RK4 update, the spike word, and the push of the new spikes
+// into the next step's buckets (buckets and spike words ping-pong by step
+// parity, so no workgroup waits for another within a step).  Because every
+// sum has a fixed order, the trajectory is bit-identical for every
+// (block, tile, grid) configuration; the autotuner uses this to time
+// candidate configurations on real steps.
+//
+// No counterpart in the reference, which simulates one network
+// (src/main.cu:112-182).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "coupling_dev.h"
+#include "internal.h"
+
+namespace nnx {
+
+struct EnsK {
+    float *V, *M, *H, *NG, *S;
+    const float* iext;
+    const int32_t *out_off, *out_tgt, *out_to_in, *in_off, *in_src;
+    const float* in_w;
+    int32_t *cnt0, *cnt1, *slot0, *slot1;
+    uint64_t *bits0, *bits1;
+    const int32_t* voxel_of;
+    int32_t* voxel_counts;
+    int32_t* net_spikes;
+    int32_t* raster;
+    unsigned long long* raster_len;
+    int64_t raster_cap;
+    const int4* tiles;  // (first neuron, end, network, 0)
+    int32_t ntiles;
+    int64_t* stepctr;   // step number by parity
+};
+
+template <int B>
+__global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int par) {
+    const int q = par ^ 1;
+    int32_t* __restrict__ cnt_r = q ? d.cnt1 : d.cnt0;  // filled by the previous step
+    const int32_t* __restrict__ slot_r = q ? d.slot1 : d.slot0;
+    int32_t* __restrict__ cnt_w = par ? d.cnt1 : d.cnt0;
+    int32_t* __restrict__ slot_w = par ? d.slot1 : d.slot0;
+    const uint64_t* __restrict__ bits_r = q ? d.bits1 : d.bits0;
+    uint64_t* __restrict__ bits_w = par ? d.bits1 : d.bits0;
+    const int64_t step = d.stepctr[q] + 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.stepctr[par] = step;
+    const int lane = threadIdx.x & 63;
+    for (int t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+        const int4 tl = d.tiles[t];
+        for (int64_t i0 = tl.x; i0 < tl.y; i0 += B) {
+            const int64_t i = i0 + threadIdx.x;
+            const bool live = i < tl.y;
+            // coupling input
+            float g = 0.f;
+            const int32_t c = live ? cnt_r[i] : 0;
+            bool heavy = c > kSortMax;
+            if (!heavy && c > 0) {
+                if (c <= d.in_off[i + 1] - d.in_off[i])
+                    g = gather_one<float>((int32_t)i, c, d.in_off, d.in_src, d.in_w, bits_r, slot_r);
+                else
+                    heavy = true;
+            }
+            uint64_t hm = __ballot(heavy);
+            const int64_t wave0 = i - lane;
+            while (hm) {
+                const int l = __ffsll((unsigned long long)hm) - 1;
+                hm &= hm - 1;
+                const float acc = wave_pull_one<float>(wave0 + l, d.in_off, d.in_src, d.in_w, bits_r);
+                if (lane == l) g = acc;
+            }
+            if (c) cnt_r[i] = 0;
+            // RK4 update and spike
+            bool spike = false;
+            if (live) {
+                float v = d.V[i], m = d.M[i], h = d.H[i], ng = d.NG[i], s = d.S[i];
+                const float v0 = v;
+                rk4(p, v, m, h, ng, s, d.iext[i], g);
+                d.V[i] = v;
+                d.M[i] = m;
+                d.H[i] = h;
+                d.NG[i] = ng;
+                d.S[i] = s;
+                spike = (v0 < p.v_thresh) && (v >= p.v_thresh);
+            }
+            const uint64_t word = __ballot(spike);
+            if (lane == 0) bits_w[wave0 >> 6] = word;
+            if (!word) continue;
+            // push the new spikes into the next step's buckets, one source at
+            // a time across the wavefront
+            uint64_t sm = word;
+            while (sm) {
+                const int l = __ffsll((unsigned long long)sm) - 1;
+                sm &= sm - 1;
+                const int64_t j = wave0 + l;
+                const int32_t e1 = d.out_off[j + 1];
+                for (int32_t e = d.out_off[j] + lane; e < e1; e += 64) {
+                    const int32_t tt = d.out_tgt[e];
+                    const int32_t pos = d.out_to_in[e];
+                    const int32_t qq = atomicAdd(&cnt_w[tt], 1);
+                    const int32_t b = d.in_off[tt];
+                    if (qq < d.in_off[tt + 1] - b) slot_w[b + qq] = pos;
+                }
+            }
+            if (spike && d.voxel_of) atomicAdd(&d.voxel_counts[d.voxel_of[i]], 1);
+            const int k = __popcll(word);
+            if (d.net_spikes && lane == 0) atomicAdd(&d.net_spikes[tl.z], k);
+            if (d.raster) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(d.raster_len, (unsigned long long)k);
+                const uint32_t lo = __shfl((uint32_t)(base & 0xffffffffull), 0, 64);
+                const uint32_t hi = __shfl((uint32_t)(base >> 32), 0, 64);
+                base = ((unsigned long long)hi << 32) | lo;
+                if (spike) {
+                    const int64_t pos = (int64_t)base + __popcll(word & ((1ull << lane) - 1ull));
+                    if (pos < d.raster_cap) {
+                        d.raster[2 * pos] = (int32_t)step;
+                        d.raster[2 * pos + 1] = (int32_t)i;
+                    }
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_ens_begin(int64_t* stepctr, int q, int64_t prev) { stepctr[q] = prev; }
+
+}  // namespace nnx
+
+using namespace nnx;
+
+namespace {
+
+bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
+
+nnx_status check_ens(nnx_ctx* c, const nnx_ens_plan* p) {
+    NNX_CHECK(c, p, "null plan");
+    const nnx_hh_state& s = p->state;
+    NNX_CHECK(c, s.dtype == NNX_F32, "the ensemble runs in fp32");
+    NNX_CHECK(c, s.v && s.m && s.h && s.ng && s.s && s.i_ext, "state needs v, m, h, ng, s, i_ext");
+    NNX_CHECK(c, s.n > 0 && s.n % 64 == 0, "state n must be a positive multiple of 64");
+    NNX_CHECK(c, p->graph.n == s.n, "graph n != state n");
+    NNX_CHECK(c, p->graph.out_offsets && p->graph.out_targets && p->graph.in_offsets && p->graph.in_sources &&
+                     p->graph.in_weights && p->graph.out_to_in,
+              "graph needs CSR, CSC and out_to_in");
+    NNX_CHECK(c, p->bits[0] && p->bits[1], "plan needs bits[2]");
+    NNX_CHECK(c, p->nnet > 0 && p->net_begin && p->net_size, "plan needs networks");
+    int64_t end = 0;
+    for (int i = 0; i < p->nnet; ++i) {
+        NNX_CHECK(c, p->net_begin[i] % 64 == 0 && p->net_begin[i] >= end && p->net_size[i] >= 0,
+                  "network %d: begin must be a multiple of 64 after the previous network", i);
+        end = p->net_begin[i] + p->net_size[i];
+    }
+    NNX_CHECK(c, end <= s.n, "networks exceed the state");
+    if (p->raster) NNX_CHECK(c, p->raster_len && p->raster_cap >= 0, "raster needs raster_len");
+    if (p->voxel_of) NNX_CHECK(c, p->voxel_counts, "voxel_of needs voxel_counts");
+    if (p->params.noise_std != 0.0)
+        return fail(c, NNX_ERR_UNSUPPORTED, "white noise is not implemented (noise_std must be 0)");
+    return NNX_OK;
+}
+
+nnx_status check_cfg(nnx_ctx* c, const nnx_ens_config* f) {
+    NNX_CHECK(c, valid_block(f->block), "block must be 64, 128, 256 or 512");
+    NNX_CHECK(c, f->tile >= 64 && f->tile % 64 == 0 && f->tile <= (1 << 24), "tile must be a multiple of 64");
+    NNX_CHECK(c, f->grid >= 1 && f->grid <= 65536, "grid must be in [1, 65536]");
+    return NNX_OK;
+}
+
+// tiles of `tile` neurons, largest first (ties: network order), in a buffer
+// that belongs to this tile size (captured graphs keep its address)
+nnx_status build_tiles(nnx_ctx* c, const nnx_ens_plan* p, int tile, const int4** out, int* ntiles) {
+    std::vector<int4> t;
+    for (int i = 0; i < p->nnet; ++i)
+        for (int64_t a = p->net_begin[i]; a < p->net_begin[i] + p->net_size[i]; a += tile)
+            t.push_back(make_int4((int)a, (int)std::min<int64_t>(a + tile, p->net_begin[i] + p->net_size[i]), i, 0));
+    std::stable_sort(t.begin(), t.end(), [](const int4& a, const int4& b) { return (a.y - a.x) > (b.y - b.x); });
+    char name[48];
+    std::snprintf(name, sizeof name, "ens_tiles_%d", tile);
+    const size_t cap = (size_t)(p->state.n / 64 + p->nnet + 1) * sizeof(int4);
+    void* buf;
+    NNX_TRY(scratch(c, name, cap, false, &buf));
+    if (!t.empty())
+        NNX_HIP(c, hipMemcpyAsync(buf, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    NNX_HIP(c, hipStreamSynchronize(c->stream));  // t is a host temporary
+    *out = (const int4*)buf;
+    *ntiles = (int)t.size();
+    return NNX_OK;
+}
+
+struct EnsHost {
+    EnsK k;
+    HHConst<float> hc;
+};
+
+nnx_status ens_prepare(nnx_ctx* c, const nnx_ens_plan* p, EnsHost* h) {
+    const int64_t n = p->state.n, nnz = p->graph.nnz;
+    void *cnt, *slot, *ctr;
+    NNX_TRY(scratch(c, "ens_cnt", (size_t)2 * n * 4, true, &cnt));
+    NNX_TRY(scratch(c, "ens_slot", (size_t)2 * std::max<int64_t>(nnz, 1) * 4, false, &slot));
+    NNX_TRY(scratch(c, "ens_ctr", 64, true, &ctr));
+    EnsK& k = h->k;
+    k = EnsK{};
+    const nnx_hh_state& s = p->state;
+    k.V = (float*)s.v, k.M = (float*)s.m, k.H = (float*)s.h, k.NG = (float*)s.ng, k.S = (float*)s.s;
+    k.iext = (const float*)s.i_ext;
+    k.out_off = p->graph.out_offsets, k.out_tgt = p->graph.out_targets, k.out_to_in = p->graph.out_to_in;
+    k.in_off = p->graph.in_offsets, k.in_src = p->graph.in_sources, k.in_w = p->graph.in_weights;
+    k.cnt0 = (int32_t*)cnt, k.cnt1 = (int32_t*)cnt + n;
+    k.slot0 = (int32_t*)slot, k.slot1 = (int32_t*)slot + std::max<int64_t>(nnz, 1);
+    k.bits0 = p->bits[0], k.bits1 = p->bits[1];
+    k.voxel_of = p->voxel_of, k.voxel_counts = p->voxel_counts, k.net_spikes = p->net_spikes;
+    k.raster = p->raster, k.raster_len = (unsigned long long*)p->raster_len, k.raster_cap = p->raster_cap;
+    k.stepctr = (int64_t*)ctr;
+    h->hc = make_const<float>(&p->params, p->dt);
+    return NNX_OK;
+}
+
+nnx_status ens_launch(nnx_ctx* c, const EnsK& k, const HHConst<float>& hc, const nnx_ens_config& f, int par) {
+    switch (f.block) {
+        case 64: k_ens_step<64><<<f.grid, 64, 0, c->stream>>>(k, hc, par); break;
+        case 128: k_ens_step<128><<<f.grid, 128, 0, c->stream>>>(k, hc, par); break;
+        case 256: k_ens_step<256><<<f.grid, 256, 0, c->stream>>>(k, hc, par); break;
+        default: k_ens_step<512><<<f.grid, 512, 0, c->stream>>>(k, hc, par); break;
+    }
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+uint64_t ens_key(const nnx_ens_plan* p, const nnx_ens_config& f) {
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    auto mix = [&](const void* b, size_t n) {
+        const unsigned char* u = (const unsigned char*)b;
+        for (size_t i = 0; i < n; ++i) h = (h ^ u[i]) * 1099511628211ull;
+    };
+    mix(p, sizeof(*p));
+    mix(p->net_begin, sizeof(int64_t) * p->nnet);
+    mix(p->net_size, sizeof(int64_t) * p->nnet);
+    mix(&f, sizeof f);
+    return h ^ 0x454e53454d424c45ull;  // distinct from the single-network step keys
+}
+
+const nnx_ens_config kDefaultCfg = {256, 2048, 1024, 0};
+
+}  // namespace
+
+extern "C" {
+
+nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_config* cfg, uint64_t first_step,
+                            int n_steps, int use_graph) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_ens(c, p));
+    const nnx_ens_config f = cfg ? *cfg : kDefaultCfg;
+    NNX_TRY(check_cfg(c, &f));
+    NNX_CHECK(c, n_steps >= 0, "n_steps < 0");
+    if (n_steps == 0) return NNX_OK;
+    EnsHost h;
+    NNX_TRY(ens_prepare(c, p, &h));
+    NNX_TRY(build_tiles(c, p, f.tile, &h.k.tiles, &h.k.ntiles));
+    if (first_step == 0) {
+        // a new run: no pending bucket entries, no previous spikes
+        NNX_HIP(c, hipMemsetAsync(h.k.cnt0, 0, (size_t)2 * p->state.n * 4, c->stream));
+        NNX_HIP(c, hipMemsetAsync(p->bits[0], 0, (size_t)p->state.n / 8, c->stream));
+        NNX_HIP(c, hipMemsetAsync(p->bits[1], 0, (size_t)p->state.n / 8, c->stream));
+    }
+    const int q0 = (int)((first_step + 1) & 1);
+    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1);
+    NNX_LAUNCHED(c);
+    if (!use_graph) {
+        for (int i = 0; i < n_steps; ++i) NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1)));
+        return NNX_OK;
+    }
+    NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
+    const std::pair<const void*, int> key((const void*)(uintptr_t)ens_key(p, f), n_steps * 2 + (int)(first_step & 1));
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        nnx_ctx::Graph g;
+        NNX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = 1;
+        nnx_status st = NNX_OK;
+        for (int i = 0; i < n_steps && st == NNX_OK; ++i) st = ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1));
+        c->capturing = 0;
+        hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
+        if (st != NNX_OK) {
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+            return st;
+        }
+        if (e != hipSuccess) return fail(c, NNX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        NNX_HIP(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        it = c->graphs.emplace(key, g).first;
+    }
+    NNX_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
+    return NNX_OK;
+}
+
+nnx_status nnx_ensemble_autotune(nnx_ctx* c, const nnx_ens_plan* p, uint64_t first_step, int steps_per_candidate,
+                                 nnx_ens_config* best, double* best_us, int* steps_run) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_ens(c, p));
+    NNX_CHECK(c, best && steps_per_candidate >= 1, "need best and steps_per_candidate >= 1");
+    hipEvent_t e0, e1;
+    NNX_HIP(c, hipEventCreate(&e0));
+    NNX_HIP(c, hipEventCreate(&e1));
+    uint64_t step = first_step;
+    auto time_cfg = [&](const nnx_ens_config& f, double* us) -> nnx_status {
+        NNX_TRY(nnx_ensemble_run(c, p, &f, step, 1, 0));  // untimed: tiles uploaded, caches warm
+        ++step;
+        NNX_HIP(c, hipEventRecord(e0, c->stream));
+        NNX_TRY(nnx_ensemble_run(c, p, &f, step, steps_per_candidate, 0));
+        NNX_HIP(c, hipEventRecord(e1, c->stream));
+        NNX_HIP(c, hipEventSynchronize(e1));
+        float ms = 0;
+        NNX_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        step += steps_per_candidate;
+        *us = 1e3 * ms / steps_per_candidate;
+        return NNX_OK;
+    };
+    nnx_ens_config cur = kDefaultCfg;
+    double cur_us;
+    NNX_TRY(time_cfg(cur, &cur_us));
+    const int blocks[] = {64, 128, 256, 512};
+    const int tiles[] = {512, 1024, 2048, 4096, 8192};
+    const int grids[] = {256, 512, 1024, 2048, 4096};
+    // coordinate search: block, then tile, then grid
+    for (int dim = 0; dim < 3; ++dim) {
+        const int* vals = dim == 0 ? blocks : dim == 1 ? tiles : grids;
+        const int nv = dim == 0 ? 4 : 5;
+        nnx_ens_config base = cur;
+        for (int i = 0; i < nv; ++i) {
+            nnx_ens_config f = base;
+            (dim == 0 ? f.block : dim == 1 ? f.tile : f.grid) = vals[i];
+            if (f.block == cur.block && f.tile == cur.tile && f.grid == cur.grid) continue;
+            if (f.tile < f.block) continue;
+            double us;
+            NNX_TRY(time_cfg(f, &us));
+            if (us < cur_us) {
+                cur = f;
+                cur_us = us;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *best = cur;
+    if (best_us) *best_us = cur_us;
+    if (steps_run) *steps_run = (int)(step - first_step);
+    return NNX_OK;
+}
+
+}  // extern "C"

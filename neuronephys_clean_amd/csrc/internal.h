@@ -23,7 +23,8 @@ struct nnx_ctx {
         size_t bytes = 0;
     };
     std::map<std::string, Buf> bufs;
-    // captured step graphs keyed by (plan address, n_steps)
+    // captured step graphs keyed by (plan hash, n_steps and parity); they
+    // embed scratch addresses, so growing any scratch buffer drops them all
     struct Graph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;

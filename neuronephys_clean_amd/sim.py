@@ -204,3 +204,95 @@ class ReferenceSimulation:
 
     def state(self) -> dict:
         return {k: self.sv[k].numpy() for k in SOA}
+
+
+class EnsembleSim:
+    """An ensemble (configs.Ensemble) stepped by nnx_ensemble_run."""
+
+    def __init__(self, ens, ctx: _lib.Context | None = None, record_cap: int = 0, deposit: bool = False,
+                 params: HHParams | None = None):
+        self.ctx = ctx or _lib.context()
+        self.ens = ens
+        ctx = self.ctx
+        n = ens.n
+        self.params = params or _lib.params_squid()
+        a = self.a = {}
+        for k, v in (("v", ens.V0), ("m", ens.m0), ("h", ens.h0), ("ng", ens.n0), ("s", np.zeros(n)),
+                     ("i_ext", ens.iext)):
+            a[k] = DeviceArray.from_numpy(ctx, np.asarray(v, np.float32))
+        d = {"off": DeviceArray.from_numpy(ctx, ens.off), "tgt": DeviceArray.from_numpy(ctx, ens.tgt),
+             "w": DeviceArray.from_numpy(ctx, ens.w)}
+        nnz = ens.nnz
+        for k, dt in (("in_off", np.int32), ("in_src", np.int32), ("in_w", np.float32), ("o2i", np.int32)):
+            d[k] = DeviceArray(ctx, max(nnz, 1) if k != "in_off" else n + 1, dt)
+        ctx.call("nnx_graph_transpose", n, nnz, d["off"].ptr, d["tgt"].ptr, d["w"].ptr, d["in_off"].ptr,
+                 d["in_src"].ptr, d["in_w"].ptr, d["o2i"].ptr, _lib.F32, None)
+        self.g = d
+        a["bits0"] = DeviceArray.zeros(ctx, n // 64, np.uint64)
+        a["bits1"] = DeviceArray.zeros(ctx, n // 64, np.uint64)
+        a["net_spikes"] = DeviceArray.zeros(ctx, len(ens.size), np.int32)
+        a["raster"] = DeviceArray.zeros(ctx, max(2 * record_cap, 2), np.int32)
+        a["raster_len"] = DeviceArray.zeros(ctx, 1, np.int64)
+        if deposit:
+            a["voxel_counts"] = DeviceArray.zeros(ctx, len(ens.size) * 64 ** 3, np.int32)
+            a["voxel_of"] = DeviceArray.from_numpy(ctx, ens.voxel_of)
+        self._begin = np.ascontiguousarray(ens.begin, np.int64)
+        self._size = np.ascontiguousarray(ens.size, np.int64)
+        p = _lib.EnsPlan()
+        p.state = _lib.HHState(dtype=_lib.F32, n=n, v=a["v"].ptr, m=a["m"].ptr, h=a["h"].ptr, ng=a["ng"].ptr,
+                               s=a["s"].ptr, i_ext=a["i_ext"].ptr)
+        p.params = self.params
+        p.graph = _lib.Graph(n=n, nnz=nnz, out_offsets=d["off"].ptr, out_targets=d["tgt"].ptr,
+                             out_weights=d["w"].ptr, in_offsets=d["in_off"].ptr, in_sources=d["in_src"].ptr,
+                             in_weights=d["in_w"].ptr, out_to_in=d["o2i"].ptr)
+        p.nnet = len(ens.size)
+        p.net_begin = self._begin.ctypes.data
+        p.net_size = self._size.ctypes.data
+        p.dt = ens.dt
+        p.bits[0] = a["bits0"].ptr
+        p.bits[1] = a["bits1"].ptr
+        p.net_spikes = a["net_spikes"].ptr
+        if record_cap:
+            p.raster = a["raster"].ptr
+            p.raster_len = a["raster_len"].ptr
+            p.raster_cap = record_cap
+        if deposit:
+            p.voxel_of = a["voxel_of"].ptr
+            p.voxel_counts = a["voxel_counts"].ptr
+        self.plan = p
+        self.record_cap = record_cap
+        self.step = 0
+        self.cfg = None
+
+    def run(self, n_steps: int, cfg=None, use_graph: bool = False, chunk: int = 100):
+        cfg = cfg or self.cfg
+        pc = C.byref(cfg) if cfg is not None else None
+        if not use_graph:
+            self.ctx.call("nnx_ensemble_run", C.byref(self.plan), pc, self.step, n_steps, 0)
+            self.step += n_steps
+            return
+        done = 0
+        while done < n_steps:
+            k = min(chunk, n_steps - done)
+            self.ctx.call("nnx_ensemble_run", C.byref(self.plan), pc, self.step, k, 1)
+            self.step += k
+            done += k
+
+    def autotune(self, steps_per_candidate: int = 20):
+        best, us, ran = _lib.EnsConfig(), C.c_double(), C.c_int()
+        self.ctx.call("nnx_ensemble_autotune", C.byref(self.plan), self.step, steps_per_candidate,
+                      C.byref(best), C.byref(us), C.byref(ran))
+        self.step += ran.value
+        self.cfg = best
+        return best, us.value, ran.value
+
+    def synchronize(self):
+        self.ctx.synchronize()
+
+    def state(self) -> dict:
+        return {k: self.a[k].numpy() for k in ("v", "m", "h", "ng", "s")}
+
+    def raster(self):
+        ln = int(self.a["raster_len"].numpy()[0])
+        r = self.a["raster"].numpy().reshape(-1, 2)[:min(ln, self.record_cap)]
+        return r[np.lexsort((r[:, 1], r[:, 0]))], ln

@@ -9,6 +9,7 @@ import pytest
 from neuronephys_clean_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nnx.h")
 
 
 def declared(header):
@@ -45,11 +46,32 @@ def test_libraries_load_without_gpu():
     assert (r.ena, r.ek, r.el, r.out_thresh) == (55.0, -72.0, -50.0, 0.1)
 
 
-def test_struct_sizes_match_header_layout():
+STRUCTS = {"nnx_hh_state": "HHState", "nnx_hh_params": "HHParams", "nnx_graph": "Graph",
+           "nnx_grid": "Grid", "nnx_cg_opts": "CgOpts", "nnx_cg_result": "CgResult",
+           "nnx_step_plan": "StepPlan", "nnx_ens_plan": "EnsPlan", "nnx_ens_config": "EnsConfig"}
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof and every field offset of the ctypes mirrors equal what gcc
+    computes from include/nnx.h."""
     import ctypes as C
-    assert C.sizeof(_lib.HHState) == 16 + 19 * 8
-    assert C.sizeof(_lib.HHParams) == 26 * 8
-    assert C.sizeof(_lib.Graph) == 16 + 7 * 8
-    assert C.sizeof(_lib.Grid) == 16 + 16 + 24
-    assert C.sizeof(_lib.CgOpts) == 32
-    assert C.sizeof(_lib.CgResult) == 16 + 24
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no host C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        lines.append(f'printf("{pyname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in getattr(_lib, pyname)._fields_:
+            lines.append(f'printf("{pyname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        pyname, field, val = line.split()
+        cls = getattr(_lib, pyname)
+        got = C.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert got == int(val), (pyname, field, got, val)

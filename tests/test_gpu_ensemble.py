@@ -1,0 +1,97 @@
+"""The batched ensemble (nnx_ensemble_run / nnx_ensemble_autotune) on the
+GPU: every network's trajectory equals the oracle's run of that network
+alone, bit for bit; results do not depend on the launch configuration or on
+graph replay; the autotuner's steps are ordinary steps."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from neuronephys_clean_amd import _lib, configs
+from neuronephys_clean_amd.sim import EnsembleSim
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1000, 3000, 1500, 2048, 1024, 4000, 700, 2500]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ens():
+    return configs.c4_ensemble(seed=3, count=len(SIZES), sizes=SIZES, keep_members=True)
+
+
+def oracle_member(net, steps):
+    T = np.float32
+    V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
+    s = np.zeros(net.n, T)
+    p = oracle.params(_lib.params_squid().as_dict())
+    ras, ln, _, _ = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
+                                   np.asarray(net.iext, T), cap=1 << 20)
+    return ras, ln, {"v": V, "m": m, "h": h, "ng": ng, "s": s}
+
+
+def test_ensemble_matches_oracle_per_network(ctx, ens):
+    steps = 400
+    say("ensemble", len(SIZES), "networks", ens.n, "neurons", ens.nnz, "edges", steps, "steps")
+    sim = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20, deposit=True)
+    sim.run(steps)
+    sim.synchronize()
+    got = sim.state()
+    ras, ln = sim.raster()
+    assert ln == len(ras)
+    total = 0
+    for i, net in enumerate(ens.members):
+        ref, rln, st = oracle_member(net, steps)
+        b, s = int(ens.begin[i]), int(ens.size[i])
+        for k in ("v", "m", "h", "ng", "s"):
+            assert np.array_equal(got[k][b:b + s], st[k]), (i, k)
+        mine = ras[(ras[:, 1] >= b) & (ras[:, 1] < b + s)].copy()
+        mine[:, 1] -= b
+        assert np.array_equal(mine, ref), i
+        total += rln
+    say("spikes", total)
+    assert total == ln and total > 0
+    assert np.array_equal(sim.a["net_spikes"].numpy(),
+                          [np.count_nonzero((ras[:, 1] >= b) & (ras[:, 1] < b + s))
+                           for b, s in zip(ens.begin, ens.size)])
+    vc = sim.a["voxel_counts"].numpy()
+    ref_vc = np.bincount(ens.voxel_of[ras[:, 1]], minlength=vc.size)
+    assert np.array_equal(vc, ref_vc)
+
+
+@pytest.mark.parametrize("cfg", [(64, 64, 7), (512, 8192, 3), (128, 512, 2048)])
+def test_ensemble_result_independent_of_config_and_graph(ctx, ens, cfg):
+    steps = 200
+    a = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
+    a.run(steps)
+    b = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
+    c = _lib.EnsConfig(*cfg, 0)
+    say("config", cfg)
+    b.run(steps, cfg=c, use_graph=True, chunk=100)
+    a.synchronize()
+    b.synchronize()
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    ra, rb = a.raster(), b.raster()
+    assert ra[1] == rb[1] and np.array_equal(ra[0], rb[0])
+
+
+def test_ensemble_autotune_steps_are_ordinary_steps(ctx, ens):
+    t = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
+    best, us, ran = t.autotune(steps_per_candidate=3)
+    say("autotune", best.as_dict(), us, "us/step over", ran, "steps")
+    assert best.block in (64, 128, 256, 512) and best.tile % 64 == 0 and best.grid > 0
+    t.run(100 - ran % 100 + 50, use_graph=True, chunk=50)
+    ref = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
+    ref.run(t.step)
+    t.synchronize()
+    ref.synchronize()
+    for k, v in t.state().items():
+        assert np.array_equal(v, ref.state()[k]), k
+    assert np.array_equal(t.raster()[0], ref.raster()[0])
