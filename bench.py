@@ -167,6 +167,103 @@ def precision_split(ctx, net, steps=(1, 100, 10000)):
     return res
 
 
+def c4_bench(ctx, steps=5000, chunk=100, tune_steps=20):
+    """Batched ensemble: 256 networks (sizes 2^U(10,16), Poisson(50)
+    out-degree), one fused persistent launch per step, 100 steps per
+    hipGraph, launch shape from the autotuner (whose steps are ordinary
+    steps, run before the timed region)."""
+    from neuronephys_clean_amd import configs
+    from neuronephys_clean_amd.sim import EnsembleSim
+    t0 = time.time()
+    ens = configs.c4_ensemble(seed=0, count=256)
+    live = int(ens.size.sum())
+    log(f"bench: C4 ensemble {len(ens.size)} networks, {live} neurons, {ens.nnz} edges, "
+        f"built in {time.time() - t0:.1f}s")
+    sim = EnsembleSim(ens, ctx=ctx, deposit=True)
+    best, tune_us, ran = sim.autotune(tune_steps)
+    log(f"bench: C4 autotune -> {best.as_dict()} ({tune_us:.1f} us/step, {ran} steps)")
+    sim.run(chunk - sim.step % chunk, use_graph=False)  # align the graph chunks
+    sim.run(chunk, use_graph=True, chunk=chunk)         # capture
+    sim.synchronize()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ctx.call("nnx_event_create", C.byref(e0))
+    ctx.call("nnx_event_create", C.byref(e1))
+    spk0 = int(sim.a["net_spikes"].numpy().sum())
+    ctx.call("nnx_event_record", e0)
+    sim.run(steps, use_graph=True, chunk=chunk)
+    ctx.call("nnx_event_record", e1)
+    sim.synchronize()
+    ms = C.c_float()
+    ctx.call("nnx_event_elapsed", e0, e1, C.byref(ms))
+    spikes = int(sim.a["net_spikes"].numpy().sum()) - spk0
+    sec = ms.value * 1e-3
+    bytes_step = 48.125 * live
+    out = {"networks": len(ens.size), "neurons": live, "edges": ens.nnz, "steps": steps,
+           "graph_chunk": chunk, "ms": ms.value, "us_per_step": 1e6 * sec / steps,
+           "neuron_steps_per_s": live * steps / sec, "spikes_per_step": spikes / steps,
+           "config": best.as_dict(), "autotune_us_per_step": tune_us, "autotune_steps": ran,
+           "hbm_GBps_state": bytes_step * steps / sec / 1e9,
+           "frac_hbm_state": bytes_step * steps / sec / HBM_PEAK}
+    del sim
+    return out
+
+
+UNET_FLOP_PER_VOXEL = (2 * 27 * (2 * 16 + 16 * 16)          # level 0: e0, d0
+                       + 2 * 27 * (16 * 32 + 32 * 16) / 8.0    # level 1: e1, d1
+                       + 2 * 27 * (32 * 32) / 64.0)            # level 2: e2
+BF16_PEAK = 2.5e15  # dense BF16 MFMA FLOP/s, MI355X
+
+
+def c5_bench(ctx, counts, smooth=16):
+    """Variable-diffusivity 256^3 solve (log-normal D, log-sigma 1,
+    correlation length 8 voxels, mean dt D / h^2 = 50, harmonic-mean faces)
+    with the C2 network's release counts as source: Jacobi-CG against
+    flexible CG with the trained U-Net preconditioner, both inside FP64
+    refinement to a true 1e-8 relative residual."""
+    from neuronephys_clean_amd import _lib, configs
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+    from neuronephys_clean_amd.precond import UNet, device_field
+    n = 256
+    t0 = time.time()
+    f = configs.c5_field(seed=0, n=n)
+    g, keep = device_field(ctx, f)
+    log(f"bench: C5 field built in {time.time() - t0:.1f}s")
+    b = 0.01 * counts.astype(np.float64)
+    if not b.any():
+        b[len(b) // 2] = 1.0
+    db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, n ** 3, np.float64)
+    net = UNet.load(ctx)
+    out = {"grid": f"{n}^3", "field": "log-normal D, log-sigma 1, corr 8, mean dt D/h^2 50",
+           "rtol_true_fp64": 1e-8, "smooth_sweeps": smooth}
+    for name, pc in (("jacobi_cg", None), ("learned_fcg", _lib.PRECOND_LEARNED)):
+        best = None
+        for _ in range(2):
+            r = _lib.CgResult()
+            o = _lib.CgOpts(rtol=1e-8, maxit=20000, precond=_lib.PRECOND_JACOBI if pc is None else pc,
+                            check_every=8, stall_iters=10, smooth=smooth)
+            if pc is None:
+                ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-6, 30, C.byref(r))
+            else:
+                ctx.call("nnx_fcg_solve_refine", C.byref(g), net.h, keep["feat"].ptr, db.ptr, dx.ptr,
+                         C.byref(o), 1e-6, 30, C.byref(r))
+            if best is None or r.ms < best["ms"]:
+                best = r.as_dict()
+        best["fell_back_to_jacobi"] = pc is not None and best["precond_used"] != _lib.PRECOND_LEARNED
+        out[name] = best
+    out["speedup_learned_vs_plain"] = out["jacobi_cg"]["ms"] / out["learned_fcg"]["ms"]
+    # one network application (the preconditioner's dense part) on the MFMA roofline
+    r32, z32 = DA.from_numpy(ctx, b.astype(np.float32)), DA.zeros(ctx, n ** 3, np.float32)
+
+    def f_apply():
+        net.apply((n, n, n), 1, r32, keep["feat"], z32)
+    ms = timed_events(ctx, f_apply, 10)
+    flop = UNET_FLOP_PER_VOXEL * n ** 3
+    out["unet_apply"] = {"ms": ms, "flop": flop, "achieved_TFLOPs": flop / (ms * 1e-3) / 1e12,
+                         "peak_TFLOPs": BF16_PEAK / 1e12, "mfma_frac": flop / (ms * 1e-3) / BF16_PEAK}
+    net.close()
+    return out
+
+
 def stencil_roofline(ctx, n=256, reps=50):
     """y = A x on an n^3 grid (constant 7-point): 8 algorithmic bytes per
     voxel (read x, write y), HIP events on the library stream."""
@@ -224,6 +321,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
@@ -306,6 +405,16 @@ def main():
         cg = cg_bench(ctx, net.grid[0], b)
         log(f"bench: CG {cg}")
 
+    c5 = None
+    if not args.no_c5:
+        c5 = c5_bench(ctx, sim.a["voxel_counts"].numpy())
+        log(f"bench: C5 {c5}")
+
+    c4 = None
+    if not args.no_c4:
+        c4 = c4_bench(ctx)
+        log(f"bench: C4 {c4}")
+
     c3 = None
     if not args.no_c3:
         c3 = c3_bench(ctx)
@@ -329,7 +438,7 @@ def main():
                       "graph_chunk": args.chunk, "spikes_timed": int(spikes),
                       "spikes_per_step": spikes / args.steps},
            "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
-           "c3": c3,
+           "c3": c3, "c4": c4, "c5": c5,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},
            "event_ms": ev_ms.value}

@@ -418,11 +418,15 @@ nnx_status nnx_unet_get_grad(nnx_ctx* ctx, const nnx_unet* net, float* h_grad);
 nnx_status nnx_unet_apply(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int nz, int batch,
                           const float* d_r, const float* d_feat, float* d_z);
 /* One Adam step (lr; lr = 0 only computes the gradient) on the loss
- * (1/B) sum_b |N(r_b / s_b) - e_b / s_b|^2 / |e_b / s_b|^2, s_b = rms(r_b),
- * N the network without the output scale.  *h_loss (if not NULL) receives
- * the loss before the step (the call then synchronises).                    */
+ * (1/B) sum_b |d_b|^2 / |t_b|^2 with t_b = e_b / s_b, d_b = N(r_b / s_b) - t_b,
+ * s_b = rms(r_b), N the network without the output scale; with face
+ * coefficients (d_kx/ky/kz: each sample's nnx_grid kx/ky/kz arrays, one
+ * after another) the A-norm loss (1/B) sum_b d_b.A_b d_b / t_b.A_b t_b
+ * instead.  *h_loss (if not NULL) receives the loss before the step (the
+ * call then synchronises).                                                  */
 nnx_status nnx_unet_train_step(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int nz, int batch,
-                               const float* d_r, const float* d_feat, const float* d_e, double lr,
+                               const float* d_r, const float* d_feat, const float* d_e,
+                               const float* d_kx, const float* d_ky, const float* d_kz, double lr,
                                double* h_loss);
 
 /* Single convolution layers (BF16 channels-last tensors as uint16_t, weights

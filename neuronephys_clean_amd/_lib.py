@@ -149,8 +149,8 @@ _SIGS = {
     "nnx_unet_get_params": ([vp, vp, vp], C.c_int),
     "nnx_unet_get_grad": ([vp, vp, vp], C.c_int),
     "nnx_unet_apply": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
-    "nnx_unet_train_step": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_double,
-                             _P(C.c_double)], C.c_int),
+    "nnx_unet_train_step": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp,
+                             C.c_double, _P(C.c_double)], C.c_int),
     "nnx_conv3_bf16": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
                         C.c_double, vp, vp, C.c_int, vp, vp], C.c_int),
     "nnx_conv3_wgrad_bf16": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,

@@ -549,12 +549,57 @@ __global__ __launch_bounds__(256) void k_t2(const float* __restrict__ e, const f
     if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+// A-norm loss: Ad = A_b (z - t) for every voxel of sample b (variable
+// 7-point operator with the sample's own faces, nnx_grid layout, Dirichlet
+// 0), and partials of t . A_b t per sample ([batch][gridDim.x]).
+__global__ __launch_bounds__(256) void k_anorm(int nx, int ny, int nz, const float* __restrict__ z,
+                                               const float* __restrict__ e, const float* __restrict__ inv,
+                                               const float* __restrict__ kx, const float* __restrict__ ky,
+                                               const float* __restrict__ kz, float* __restrict__ Ad,
+                                               double* __restrict__ part) {
+    __shared__ double sh[4];
+    const int b = blockIdx.y;
+    const int64_t nper = (int64_t)nx * ny * nz;
+    const float sb = inv[b];
+    const float* zb = z + b * nper;
+    const float* eb = e + b * nper;
+    const float* fx = kx + (int64_t)b * (nx + 1) * ny * nz;
+    const float* fy = ky + (int64_t)b * nx * (ny + 1) * nz;
+    const float* fz = kz + (int64_t)b * nx * ny * (nz + 1);
+    double acc = 0;
+    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < nper; id += (int64_t)gridDim.x * 256) {
+        const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((int64_t)nx * ny));
+        const int64_t ix = i + (int64_t)(nx + 1) * (j + (int64_t)ny * k), iy = i + (int64_t)nx * (j + (int64_t)(ny + 1) * k);
+        const float kxm = fx[ix], kxp = fx[ix + 1], kym = fy[iy], kyp = fy[iy + nx], kzm = fz[id],
+                    kzp = fz[id + (int64_t)nx * ny];
+        const float dg = 1.f + kxm + kxp + kym + kyp + kzm + kzp;
+        auto tv = [&](int64_t q) { return eb[q] * sb; };
+        auto dv = [&](int64_t q) { return zb[q] - eb[q] * sb; };
+        float sd = 0.f, st = 0.f;
+        if (i > 0) { sd += kxm * dv(id - 1); st += kxm * tv(id - 1); }
+        if (i < nx - 1) { sd += kxp * dv(id + 1); st += kxp * tv(id + 1); }
+        if (j > 0) { sd += kym * dv(id - nx); st += kym * tv(id - nx); }
+        if (j < ny - 1) { sd += kyp * dv(id + nx); st += kyp * tv(id + nx); }
+        if (k > 0) { sd += kzm * dv(id - (int64_t)nx * ny); st += kzm * tv(id - (int64_t)nx * ny); }
+        if (k < nz - 1) { sd += kzp * dv(id + (int64_t)nx * ny); st += kzp * tv(id + (int64_t)nx * ny); }
+        Ad[b * nper + id] = dg * dv(id) - sd;
+        const float t = tv(id);
+        acc += (double)t * (dg * t - st);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
 // Loss (1/B) sum_b |z_b - t_b|^2 / T2_b and its gradient through the head:
 // dZ = 2 (z - t) / (B T2_b); G0 = hw dZ act'(d0); partials of dhw, dhb, loss.
 constexpr int kHeadBlocks = 512;
 constexpr int kSumBlocks = 1024;  // blocks per batch member of the sum-of-squares reductions
+// With Ad (A-norm loss): (1/B) sum_b d_b.A_b d_b / t_b.A_b t_b, dZ = 2 A_b d_b / (B t_b.A_b t_b).
 __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ z, const float* __restrict__ e,
                                                   const float* __restrict__ inv, const double* __restrict__ t2,
+                                                  const float* __restrict__ Ad,
                                                   int64_t nper, int batch, const uint16_t* __restrict__ d0,
                                                   const float* __restrict__ head, uint16_t* __restrict__ g0,
                                                   double* __restrict__ part) {
@@ -566,8 +611,9 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ z, c
         const int b = (int)(i / nper);
         const float t = e[i] * inv[b];
         const float diff = z[i] - t;
-        const float dz = (float)(2.0 * diff / ((double)batch * t2[b]));
-        acc[17] += (double)diff * diff / t2[b];
+        const float gd = Ad ? Ad[i] : diff;
+        const float dz = (float)(2.0 * gd / ((double)batch * t2[b]));
+        acc[17] += (double)diff * gd / t2[b];
         acc[16] += dz;
         uint32_t o[8];
         for (int c8 = 0; c8 < 2; ++c8) {
@@ -924,7 +970,8 @@ nnx_status nnx_unet_apply(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int b
 }
 
 nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* d_r,
-                               const float* d_feat, const float* d_e, double lr, double* h_loss) {
+                               const float* d_feat, const float* d_e, const float* d_kx, const float* d_ky,
+                               const float* d_kz, double lr, double* h_loss) {
     if (!c) return NNX_ERR_ARG;
     NNX_CHECK(c, u && d_r && d_e, "null argument");
     NNX_TRY(check_dims(c, nx, ny, nz, batch));
@@ -952,10 +999,20 @@ nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, 
     NNX_TRY(scratch(c, "un_t2", 256 * 8, false, &p));
     double* t2 = (double*)p;
     // loss and head
-    k_t2<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(d_e, A.inv, nper, A.part);
+    const bool anorm = d_kx != nullptr;
+    NNX_CHECK(c, !anorm || (d_ky && d_kz), "A-norm loss needs kx, ky and kz");
+    float* Ad = nullptr;
+    if (anorm) {
+        NNX_TRY(scratch(c, "un_ad", n0 * 4, false, &p));
+        Ad = (float*)p;
+        k_anorm<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(nx, ny, nz, A.z, d_e, A.inv, d_kx, d_ky, d_kz, Ad,
+                                                                A.part);
+    } else {
+        k_t2<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(d_e, A.inv, nper, A.part);
+    }
     k_rms_fin<<<batch, 256, 0, c->stream>>>(A.part, kSumBlocks, 1, batch, A.s + 256, A.inv + 256, t2);
-    k_head_bwd<<<kHeadBlocks, 256, 0, c->stream>>>(A.z, d_e, A.inv, t2, nper, batch, A.d0, u->p + P.head_off, G0,
-                                                   A.part);
+    k_head_bwd<<<kHeadBlocks, 256, 0, c->stream>>>(A.z, d_e, A.inv, t2, Ad, nper, batch, A.d0, u->p + P.head_off,
+                                                   G0, A.part);
     k_head_fin<<<1, 64, 0, c->stream>>>(A.part, kHeadBlocks, u->grad + P.head_off, (double*)lossp, batch);
     NNX_LAUNCHED(c);
     const int nx1 = nx / 2, ny1 = ny / 2, nz1 = nz / 2, nx2 = nx / 4, ny2 = ny / 4, nz2 = nz / 4;

@@ -60,11 +60,15 @@ class UNet:
         self.ctx.call("nnx_unet_apply", self.h, nx, ny, nz, batch, r.ptr,
                       None if feat is None else feat.ptr, z.ptr)
 
-    def train_step(self, dims, batch, r: DA, feat: DA | None, e: DA, lr: float, want_loss=True):
+    def train_step(self, dims, batch, r: DA, feat: DA | None, e: DA, lr: float, want_loss=True,
+                   faces=None):
+        """faces: (kx, ky, kz) device arrays of the batch's operators for the
+        A-norm loss, or None for the plain relative MSE."""
         nx, ny, nz = dims
         loss = C.c_double()
+        kx, ky, kz = (None, None, None) if faces is None else (f.ptr for f in faces)
         self.ctx.call("nnx_unet_train_step", self.h, nx, ny, nz, batch, r.ptr,
-                      None if feat is None else feat.ptr, e.ptr, lr,
+                      None if feat is None else feat.ptr, e.ptr, kx, ky, kz, lr,
                       C.byref(loss) if want_loss else None)
         return loss.value if want_loss else None
 
@@ -106,6 +110,9 @@ class PairSet:
         self.r = DA(ctx, (npairs, vox), np.float32)
         self.e = DA(ctx, (npairs, vox), np.float32)
         self.feats = DA(ctx, (nfields, vox), np.float32)
+        nf = (n + 1) * n * n
+        self.faces = [DA(ctx, (nfields, nf), np.float32) for _ in range(3)]
+        self.face_bytes = nf * 4
         self.field_of = np.zeros(npairs, np.int64)
         self.bytes_per = vox * 4
 
@@ -114,8 +121,13 @@ class PairSet:
         self.ctx.call("nnx_memcpy_h2d", C.c_void_p(dst.ptr + row * self.bytes_per),
                       a.ctypes.data_as(C.c_void_p), C.c_size_t(self.bytes_per))
 
-    def add_field(self, idx, feat):
+    def add_field(self, idx, feat, faces=None):
         self._put(self.feats, idx, feat)
+        if faces is not None:
+            for dst, a in zip(self.faces, faces):
+                a = np.ascontiguousarray(a, np.float32)
+                self.ctx.call("nnx_memcpy_h2d", C.c_void_p(dst.ptr + idx * self.face_bytes),
+                              a.ctypes.data_as(C.c_void_p), C.c_size_t(self.face_bytes))
 
     def add_pair(self, r, e, field):
         i = self.count
@@ -124,13 +136,18 @@ class PairSet:
         self.field_of[i] = field
         self.count += 1
 
-    def gather(self, idx, r_out: DA, f_out: DA, e_out: DA):
+    def gather(self, idx, r_out: DA, f_out: DA, e_out: DA, faces_out=None):
         for j, i in enumerate(idx):
             off = j * self.bytes_per
             for dst, src, row in ((r_out, self.r, i), (e_out, self.e, i),
                                   (f_out, self.feats, self.field_of[i])):
                 self.ctx.call("nnx_memcpy_d2d", C.c_void_p(dst.ptr + off),
                               C.c_void_p(src.ptr + int(row) * self.bytes_per), C.c_size_t(self.bytes_per))
+            if faces_out is not None:
+                fo = int(self.field_of[i]) * self.face_bytes
+                for dst, src in zip(faces_out, self.faces):
+                    self.ctx.call("nnx_memcpy_d2d", C.c_void_p(dst.ptr + j * self.face_bytes),
+                                  C.c_void_p(src.ptr + fo), C.c_size_t(self.face_bytes))
 
 
 def _cg(ctx, g, b: DA, x: DA, rtol, maxit, use_x0=0):
@@ -162,7 +179,7 @@ def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 
     for f in range(fields):
         fld = configs.c5_field(seed=seed * 100003 + f, n=n)
         g, keep = device_field(ctx, fld)
-        ps.add_field(f, fld.feat)
+        ps.add_field(f, fld.feat, (fld.kx, fld.ky, fld.kz))
         dinv = jacobi_inverse(fld)
         for s in range(sources):
             cnt = int(20 + (s * 37 + f * 11) % 380)
@@ -191,11 +208,13 @@ def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 
 
 
 def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=3e-4, lr_min=1e-5, warmup=500, seed=0,
-          log=print):
+          anorm=True, log=print):
     """Adam, linear warm-up then a cosine learning-rate schedule, over
-    shuffled pairs."""
+    shuffled pairs; the A-norm loss by default (errors the operator
+    amplifies - rough ones - weigh most, as they do for the solver)."""
     n, vox = ps.n, ps.n ** 3
     rb, fb, eb = (DA(ctx, (batch, vox), np.float32) for _ in range(3))
+    fcs = [DA(ctx, (batch, (n + 1) * n * n), np.float32) for _ in range(3)] if anorm else None
     steps = epochs * (ps.count // batch)
     rng = np.random.default_rng(seed)
     losses, t0, step = [], time.time(), 0
@@ -203,12 +222,12 @@ def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=3e-4, lr_min=1e-5, 
         perm = rng.permutation(ps.count)
         for j in range(ps.count // batch):
             idx = perm[j * batch:(j + 1) * batch]
-            ps.gather(idx, rb, fb, eb)
+            ps.gather(idx, rb, fb, eb, fcs)
             cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / max(steps, 1)))
             if step < warmup:
                 cur = lr * (step + 1) / warmup
             want = (step % 50 == 0) or step == steps - 1
-            loss = net.train_step((n, n, n), batch, rb, fb, eb, cur, want_loss=want)
+            loss = net.train_step((n, n, n), batch, rb, fb, eb, cur, want_loss=want, faces=fcs)
             if want:
                 losses.append((step, loss))
                 if step % 500 == 0:

@@ -185,7 +185,61 @@ def test_unet_forward_matches_torch(ctx):
     assert err < 1e-2
 
 
-def test_unet_gradient_matches_autograd(ctx):
+def apply_A(u, kx, ky, kz):
+    """variable 7-point operator (nnx_grid face layout), batched, Dirichlet 0"""
+    dg = 1 + kx[..., :-1] + kx[..., 1:] + ky[:, :, :-1, :] + ky[:, :, 1:, :] + kz[:, :-1] + kz[:, 1:]
+    s = torch.zeros_like(u)
+    s[..., 1:] = s[..., 1:] + kx[..., 1:-1] * u[..., :-1]
+    s[..., :-1] = s[..., :-1] + kx[..., 1:-1] * u[..., 1:]
+    s[:, :, 1:, :] = s[:, :, 1:, :] + ky[:, :, 1:-1, :] * u[:, :, :-1, :]
+    s[:, :, :-1, :] = s[:, :, :-1, :] + ky[:, :, 1:-1, :] * u[:, :, 1:, :]
+    s[:, 1:] = s[:, 1:] + kz[:, 1:-1] * u[:, :-1]
+    s[:, :-1] = s[:, :-1] + kz[:, 1:-1] * u[:, 1:]
+    return dg * u - s
+
+
+def reference_grad(p, r, feat, e, faces=None):
+    """Loss and parameter gradient by autograd through the float64 network
+    with BF16 rounding (straight-through); biases held at zero."""
+    pt = torch.from_numpy(p.astype(np.float64)).requires_grad_(True)
+    R = lambda t: t + (rnd_bf16(t) - t).detach()  # noqa: E731
+    W, o = [], 0
+    for cin, cout in LAYERS:
+        w = pt[o:o + cout * 27 * cin].view(cout, 3, 3, 3, cin).permute(0, 4, 1, 2, 3)
+        W.append((R(w), pt[o + cout * 27 * cin:o + cout * 27 * cin + cout]))
+        o += cout * 27 * cin + cout
+    rt = torch.from_numpy(r.astype(np.float64))
+    s = torch.sqrt((rt ** 2).mean(dim=(1, 2, 3), keepdim=True))
+    x = torch.stack([R(rt / s), R(rt / s * torch.from_numpy(feat.astype(np.float64)))], dim=1)
+    conv = lambda t, i: F.leaky_relu(F.conv3d(t, W[i][0], W[i][1], padding=1), 0.1)  # noqa: E731
+    e0f = conv(x, 0)
+    e0, p0 = R(e0f), R(F.avg_pool3d(e0f, 2))
+    e1f = conv(p0, 1)
+    e1, p1 = R(e1f), R(F.avg_pool3d(e1f, 2))
+    e2 = R(conv(p1, 2))
+    d1 = R(conv(R(e1 + up2(e2)), 3))
+    d0 = conv(R(e0 + up2(d1)), 4)
+    z = (d0 * pt[o:o + 16].view(1, 16, 1, 1, 1)).sum(1) + pt[o + 16]
+    t = torch.from_numpy(e.astype(np.float64)) / s
+    d = z - t
+    if faces is None:
+        L = ((d ** 2).sum(dim=(1, 2, 3)) / (t ** 2).sum(dim=(1, 2, 3))).mean()
+    else:
+        K = [torch.from_numpy(a.astype(np.float64)) for a in faces]
+        L = ((d * apply_A(d, *K)).sum(dim=(1, 2, 3)) / (t * apply_A(t, *K)).sum(dim=(1, 2, 3))).mean()
+    L.backward()
+    g = pt.grad.numpy().copy()
+    o = 0
+    for cin, cout in LAYERS:
+        o += cout * 27 * cin
+        g[o:o + cout] = 0
+        o += cout
+    g[o + 16] = 0
+    return float(L.detach()), g
+
+
+@pytest.mark.parametrize("anorm", [False, True])
+def test_unet_gradient_matches_autograd(ctx, anorm):
     dims, B = (64, 32, 32), 2
     nx, ny, nz = dims
     rng = np.random.default_rng(4)
@@ -194,50 +248,18 @@ def test_unet_gradient_matches_autograd(ctx):
     r = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
     feat = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
     e = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
+    faces = None
+    if anorm:
+        faces = [rng.uniform(1, 60, sh).astype(np.float32)
+                 for sh in ((B, nz, ny, nx + 1), (B, nz, ny + 1, nx), (B, nz + 1, ny, nx))]
     dr, df, de = (DA.from_numpy(ctx, a) for a in (r, feat, e))
-    say("unet gradient", dims, B)
-    loss = net.train_step(dims, B, dr, df, de, 0.0)
+    dfc = [DA.from_numpy(ctx, a) for a in faces] if anorm else None
+    say("unet gradient", dims, B, "A-norm" if anorm else "MSE")
+    loss = net.train_step(dims, B, dr, df, de, 0.0, faces=dfc)
     got = net.grad()
-    pt = torch.from_numpy(p.astype(np.float64)).requires_grad_(True)
-    layers_t = []
-    o = 0
-    for cin, cout in LAYERS:
-        layers_t.append((pt[o:o + cout * 27 * cin], pt[o + cout * 27 * cin:o + cout * 27 * cin + cout]))
-        o += cout * 27 * cin + cout
-
-    # same network as torch_unet, but differentiable in the parameters
-    def fwd():
-        R = lambda t: t + (rnd_bf16(t) - t).detach()  # noqa: E731
-        W = []
-        for (w, b), (ci, co) in zip(layers_t, LAYERS):
-            wt = w.view(co, 3, 3, 3, ci).permute(0, 4, 1, 2, 3)
-            W.append((R(wt), b))
-        rt = torch.from_numpy(r.astype(np.float64))
-        s = torch.sqrt((rt ** 2).mean(dim=(1, 2, 3), keepdim=True))
-        x = torch.stack([R(rt / s), R(rt / s * torch.from_numpy(feat.astype(np.float64)))], dim=1)
-        conv = lambda t, i: F.leaky_relu(F.conv3d(t, W[i][0], W[i][1], padding=1), 0.1)  # noqa: E731
-        e0f = conv(x, 0)
-        e0, p0 = R(e0f), R(F.avg_pool3d(e0f, 2))
-        e1f = conv(p0, 1)
-        e1, p1 = R(e1f), R(F.avg_pool3d(e1f, 2))
-        e2 = R(conv(p1, 2))
-        d1 = R(conv(R(e1 + up2(e2)), 3))
-        d0 = conv(R(e0 + up2(d1)), 4)
-        z = (d0 * pt[o:o + 16].view(1, 16, 1, 1, 1)).sum(1) + pt[o + 16]
-        t = torch.from_numpy(e.astype(np.float64)) / s
-        return (((z - t) ** 2).sum(dim=(1, 2, 3)) / (t ** 2).sum(dim=(1, 2, 3))).mean()
-
-    L = fwd()
-    L.backward()
-    ref = pt.grad.numpy().copy()
-    o = 0
-    for cin, cout in LAYERS:  # biases are held at zero by the training step
-        o += cout * 27 * cin
-        ref[o:o + cout] = 0
-        o += cout
-    ref[o + 16] = 0
-    say("loss", loss, float(L.detach()))
-    assert abs(loss - float(L.detach())) <= 2e-2 * abs(float(L.detach()))
+    ref_loss, ref = reference_grad(p, r, feat, e, faces)
+    say("loss", loss, ref_loss)
+    assert abs(loss - ref_loss) <= 2e-2 * abs(ref_loss)
     o = 0
     for i, (cin, cout) in enumerate(LAYERS + [(16, 1)]):
         nw = cout * 27 * cin if i < 5 else 17
