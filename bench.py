@@ -214,7 +214,7 @@ UNET_FLOP_PER_VOXEL = (2 * 27 * (2 * 16 + 16 * 16)          # level 0: e0, d0
 BF16_PEAK = 2.5e15  # dense BF16 MFMA FLOP/s, MI355X
 
 
-def c5_bench(ctx, counts, smooth=16):
+def c5_bench(ctx, counts, smooth=2):
     """Variable-diffusivity 256^3 solve (log-normal D, log-sigma 1,
     correlation length 8 voxels, mean dt D / h^2 = 50, harmonic-mean faces)
     with the C2 network's release counts as source: Jacobi-CG against

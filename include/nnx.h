@@ -411,6 +411,13 @@ int64_t nnx_unet_param_count(void);
 nnx_status nnx_unet_create(nnx_ctx* ctx, const float* h_params, uint64_t seed, nnx_unet** out);
 nnx_status nnx_unet_destroy(nnx_ctx* ctx, nnx_unet* net);
 nnx_status nnx_unet_get_params(nnx_ctx* ctx, const nnx_unet* net, float* h_params);
+/* Two-grid mode (coarse != 0): the network runs on the 2x coarser grid;
+ * its input is the mean over 2x2x2 blocks of (q, q * feature) and its
+ * output is prolonged back by cell-centred trilinear interpolation (weights
+ * 3/4 and 1/4 per axis, zero outside the box).  Grids then need
+ * nx % 128 == 0 and ny, nz % 64 == 0.  The mode is not part of the
+ * parameters; training and use must agree.                                 */
+nnx_status nnx_unet_set_coarse(nnx_ctx* ctx, nnx_unet* net, int coarse);
 /* gradient of the last nnx_unet_train_step (same layout as the parameters) */
 nnx_status nnx_unet_get_grad(nnx_ctx* ctx, const nnx_unet* net, float* h_grad);
 /* z = M(r) for `batch` grids stored one after another (fp32; d_feat may be
@@ -422,12 +429,14 @@ nnx_status nnx_unet_apply(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int nz, i
  * s_b = rms(r_b), N the network without the output scale; with face
  * coefficients (d_kx/ky/kz: each sample's nnx_grid kx/ky/kz arrays, one
  * after another) the A-norm loss (1/B) sum_b d_b.A_b d_b / t_b.A_b t_b
- * instead.  *h_loss (if not NULL) receives the loss before the step (the
- * call then synchronises).                                                  */
+ * instead, and with smooth = m > 0 (at most 4) the smoothed A-norm loss
+ * (1/B) sum_b |S_b^m d_b|_A^2 / t_b.A_b t_b, S_b = I - (2/3) D_b^-1 A_b (the
+ * error left after m damped-Jacobi post-smoothing sweeps).  *h_loss (if
+ * not NULL) receives the loss before the step (the call then synchronises). */
 nnx_status nnx_unet_train_step(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int nz, int batch,
                                const float* d_r, const float* d_feat, const float* d_e,
-                               const float* d_kx, const float* d_ky, const float* d_kz, double lr,
-                               double* h_loss);
+                               const float* d_kx, const float* d_ky, const float* d_kz, int smooth,
+                               double lr, double* h_loss);
 
 /* Single convolution layers (BF16 channels-last tensors as uint16_t, weights
  * fp32 [cout][27 cin]); (cin, cout) in {(2,16), (16,16), (16,32), (32,16),

@@ -147,10 +147,11 @@ _SIGS = {
     "nnx_unet_create": ([vp, vp, C.c_uint64, _P(vp)], C.c_int),
     "nnx_unet_destroy": ([vp, vp], C.c_int),
     "nnx_unet_get_params": ([vp, vp, vp], C.c_int),
+    "nnx_unet_set_coarse": ([vp, vp, C.c_int], C.c_int),
     "nnx_unet_get_grad": ([vp, vp, vp], C.c_int),
     "nnx_unet_apply": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
     "nnx_unet_train_step": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp,
-                             C.c_double, _P(C.c_double)], C.c_int),
+                             C.c_int, C.c_double, _P(C.c_double)], C.c_int),
     "nnx_conv3_bf16": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
                         C.c_double, vp, vp, C.c_int, vp, vp], C.c_int),
     "nnx_conv3_wgrad_bf16": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,

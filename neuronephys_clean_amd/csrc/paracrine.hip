@@ -1062,9 +1062,6 @@ nnx_status nnx_fcg_solve(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, const flo
     NNX_CHECK(c, o->rtol >= 0 && o->maxit >= 0, "bad rtol/maxit");
     NNX_CHECK(c, o->precond == NNX_PRECOND_LEARNED || o->precond == NNX_PRECOND_JACOBI, "FCG precond: LEARNED or JACOBI");
     NNX_CHECK(c, o->precond != NNX_PRECOND_LEARNED || net, "learned preconditioner needs a network");
-    if (o->precond == NNX_PRECOND_LEARNED)
-        NNX_CHECK(c, g->nx % 64 == 0 && g->ny % 32 == 0 && g->nz % 32 == 0,
-                  "learned preconditioner needs nx %% 64 == 0 and ny, nz %% 32 == 0");
     const GridK gk = make_grid(g);
     auto t0 = std::chrono::steady_clock::now();
     nnx_cg_result r{};
@@ -1087,9 +1084,6 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
     NNX_CHECK(c, b && x && o, "null b, x or opts");
     NNX_CHECK(c, o->precond == NNX_PRECOND_LEARNED || o->precond == NNX_PRECOND_JACOBI, "FCG precond: LEARNED or JACOBI");
     NNX_CHECK(c, o->precond != NNX_PRECOND_LEARNED || net, "learned preconditioner needs a network");
-    if (o->precond == NNX_PRECOND_LEARNED)
-        NNX_CHECK(c, g->nx % 64 == 0 && g->ny % 32 == 0 && g->nz % 32 == 0,
-                  "learned preconditioner needs nx %% 64 == 0 and ny, nz %% 32 == 0");
     const GridK gk = make_grid(g);
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
     auto t0 = std::chrono::steady_clock::now();

@@ -549,6 +549,92 @@ __global__ __launch_bounds__(256) void k_t2(const float* __restrict__ e, const f
     if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
+// Two-grid mode: the network runs on the 2x coarser grid.  Input: the mean
+// over each 2x2x2 block of q = r inv[b] and of q feat; output: cell-centred
+// trilinear prolongation (weights 3/4, 1/4 per axis, zero outside the box).
+__global__ void k_restrict_pack(const float* __restrict__ r, const float* __restrict__ feat,
+                                const float* __restrict__ inv, int nx, int ny, int nz, int batch,
+                                uint32_t* __restrict__ x) {
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const int64_t nc = (int64_t)cx * cy * cz, nf = (int64_t)nx * ny * nz;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nc * batch; i += (int64_t)gridDim.x * 256) {
+        const int b = (int)(i / nc);
+        const int64_t v = i - b * nc;
+        const int X = (int)(v % cx), Y = (int)((v / cx) % cy), Z = (int)(v / ((int64_t)cx * cy));
+        float q = 0.f, qf = 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const int64_t f = (int64_t)b * nf + ((int64_t)(2 * Z + (d >> 2)) * ny + 2 * Y + ((d >> 1) & 1)) * nx +
+                              2 * X + (d & 1);
+            const float qq = r[f] * inv[b];
+            q += qq;
+            qf += feat ? qq * feat[f] : 0.f;
+        }
+        x[i] = f2bf(0.125f * q) | (f2bf(0.125f * qf) << 16);
+    }
+}
+
+// 1-D prolongation stencil of fine index i: coarse indices and weights
+__device__ __forceinline__ void prol1(int i, int nc, int& i0, float& w0, int& i1, float& w1) {
+    const int I = i >> 1;
+    if (i & 1) { i0 = I; w0 = 0.75f; i1 = I + 1; w1 = 0.25f; }
+    else { i0 = I - 1; w0 = 0.25f; i1 = I; w1 = 0.75f; }
+    if (i0 < 0 || i0 >= nc) { i0 = 0; w0 = 0.f; }
+    if (i1 < 0 || i1 >= nc) { i1 = 0; w1 = 0.f; }
+}
+
+__global__ void k_prolong(const float* __restrict__ zc, int nx, int ny, int nz, int batch,
+                          const float* __restrict__ scale, float* __restrict__ z) {
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const int64_t nc = (int64_t)cx * cy * cz, nf = (int64_t)nx * ny * nz;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nf * batch; i += (int64_t)gridDim.x * 256) {
+        const int b = (int)(i / nf);
+        const int64_t v = i - b * nf;
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), zz = (int)(v / ((int64_t)nx * ny));
+        int xi[2], yi[2], zi[2];
+        float xw[2], yw[2], zw[2];
+        prol1(x, cx, xi[0], xw[0], xi[1], xw[1]);
+        prol1(y, cy, yi[0], yw[0], yi[1], yw[1]);
+        prol1(zz, cz, zi[0], zw[0], zi[1], zw[1]);
+        const float* zb = zc + b * nc;
+        float acc = 0.f;
+        for (int c = 0; c < 2; ++c)
+            for (int bb = 0; bb < 2; ++bb)
+                for (int a = 0; a < 2; ++a)
+                    acc += (zw[c] * yw[bb] * xw[a]) * zb[((int64_t)zi[c] * cy + yi[bb]) * cx + xi[a]];
+        z[i] = (scale ? scale[b] : 1.f) * acc;
+    }
+}
+
+// adjoint of k_prolong (without the scale): dzc[V] = sum_v w(v, V) dz[v]
+__global__ void k_prolong_adj(const float* __restrict__ dz, int nx, int ny, int nz, int batch,
+                              float* __restrict__ dzc) {
+    const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+    const int64_t nc = (int64_t)cx * cy * cz, nf = (int64_t)nx * ny * nz;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nc * batch; i += (int64_t)gridDim.x * 256) {
+        const int b = (int)(i / nc);
+        const int64_t v = i - b * nc;
+        const int X = (int)(v % cx), Y = (int)((v / cx) % cy), Z = (int)(v / ((int64_t)cx * cy));
+        // fine indices 2I-1 .. 2I+2 with weights 1/4, 3/4, 3/4, 1/4
+        const float w4[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+        const float* db = dz + b * nf;
+        float acc = 0.f;
+        for (int c = 0; c < 4; ++c) {
+            const int fz = 2 * Z - 1 + c;
+            if (fz < 0 || fz >= nz) continue;
+            for (int bb = 0; bb < 4; ++bb) {
+                const int fy = 2 * Y - 1 + bb;
+                if (fy < 0 || fy >= ny) continue;
+                for (int a = 0; a < 4; ++a) {
+                    const int fx = 2 * X - 1 + a;
+                    if (fx < 0 || fx >= nx) continue;
+                    acc += (w4[c] * w4[bb] * w4[a]) * db[((int64_t)fz * ny + fy) * nx + fx];
+                }
+            }
+        }
+        dzc[i] = acc;
+    }
+}
+
 // A-norm loss: Ad = A_b (z - t) for every voxel of sample b (variable
 // 7-point operator with the sample's own faces, nnx_grid layout, Dirichlet
 // 0), and partials of t . A_b t per sample ([batch][gridDim.x]).
@@ -592,28 +678,86 @@ __global__ __launch_bounds__(256) void k_anorm(int nx, int ny, int nz, const flo
     if (threadIdx.x == 0) part[b * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-// Loss (1/B) sum_b |z_b - t_b|^2 / T2_b and its gradient through the head:
-// dZ = 2 (z - t) / (B T2_b); G0 = hw dZ act'(d0); partials of dhw, dhb, loss.
+// Per-sample operator pieces for the smoothed A-norm loss (faces of sample
+// b in nnx_grid layout, Dirichlet 0, D = 1 + the six faces):
+//   mode 0: out = A x;  mode 1: out = x - w D^-1 A x  (S x);
+//   mode 2: out = D^-1 x;  mode 3: out = x2 - w A x  (with x = D^-1 x2: S^T x2).
+constexpr float kSmoothWTrain = 2.0f / 3.0f;
+__global__ __launch_bounds__(256) void k_sample_op(int nx, int ny, int nz, int batch, int mode,
+                                                   const float* __restrict__ x, const float* __restrict__ x2,
+                                                   const float* __restrict__ kx, const float* __restrict__ ky,
+                                                   const float* __restrict__ kz, float* __restrict__ out) {
+    const int64_t nper = (int64_t)nx * ny * nz;
+    for (int64_t g = blockIdx.x * 256ll + threadIdx.x; g < nper * batch; g += (int64_t)gridDim.x * 256) {
+        const int b = (int)(g / nper);
+        const int64_t id = g - b * nper;
+        const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((int64_t)nx * ny));
+        const float* fx = kx + (int64_t)b * (nx + 1) * ny * nz;
+        const float* fy = ky + (int64_t)b * nx * (ny + 1) * nz;
+        const float* fz = kz + (int64_t)b * nx * ny * (nz + 1);
+        const int64_t ix = i + (int64_t)(nx + 1) * (j + (int64_t)ny * k), iy = i + (int64_t)nx * (j + (int64_t)(ny + 1) * k);
+        const float kxm = fx[ix], kxp = fx[ix + 1], kym = fy[iy], kyp = fy[iy + nx], kzm = fz[id],
+                    kzp = fz[id + (int64_t)nx * ny];
+        const float dg = 1.f + kxm + kxp + kym + kyp + kzm + kzp;
+        const float* xb = x + b * nper;
+        if (mode == 2) {
+            out[g] = xb[id] / dg;
+            continue;
+        }
+        float sn = 0.f;
+        if (i > 0) sn += kxm * xb[id - 1];
+        if (i < nx - 1) sn += kxp * xb[id + 1];
+        if (j > 0) sn += kym * xb[id - nx];
+        if (j < ny - 1) sn += kyp * xb[id + nx];
+        if (k > 0) sn += kzm * xb[id - (int64_t)nx * ny];
+        if (k < nz - 1) sn += kzp * xb[id + (int64_t)nx * ny];
+        const float ax = dg * xb[id] - sn;
+        if (mode == 0) out[g] = ax;
+        else if (mode == 1) out[g] = xb[id] - kSmoothWTrain * ax / dg;
+        else out[g] = x2[g] - kSmoothWTrain * ax;
+    }
+}
+// d = z - e inv[b]
+__global__ void k_diff(const float* __restrict__ z, const float* __restrict__ e, const float* __restrict__ inv,
+                       int64_t nper, int batch, float* __restrict__ d) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nper * batch; i += (int64_t)gridDim.x * 256)
+        d[i] = z[i] - e[i] * inv[i / nper];
+}
+
+// Loss (1/B) sum_b |d_b|^2 / T2_b (or, with Ad, d_b.A_b d_b / t_b.A_b t_b)
+// with d = z - e inv[b], and dZ = 2 d / (B T2_b) (or 2 A_b d_b / (B t_b.A_b t_b));
+// one loss partial per block.
 constexpr int kHeadBlocks = 512;
 constexpr int kSumBlocks = 1024;  // blocks per batch member of the sum-of-squares reductions
-// With Ad (A-norm loss): (1/B) sum_b d_b.A_b d_b / t_b.A_b t_b, dZ = 2 A_b d_b / (B t_b.A_b t_b).
-__global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ z, const float* __restrict__ e,
-                                                  const float* __restrict__ inv, const double* __restrict__ t2,
-                                                  const float* __restrict__ Ad,
-                                                  int64_t nper, int batch, const uint16_t* __restrict__ d0,
-                                                  const float* __restrict__ head, uint16_t* __restrict__ g0,
-                                                  double* __restrict__ part) {
-    __shared__ double sh[4][18];
-    double acc[18];
-    for (int q = 0; q < 18; ++q) acc[q] = 0;
+__global__ __launch_bounds__(256) void k_loss_grad(const float* __restrict__ z, const float* __restrict__ e,
+                                                   const float* __restrict__ inv, const double* __restrict__ t2,
+                                                   const float* __restrict__ Ad, int64_t nper, int batch,
+                                                   float* __restrict__ dz, double* __restrict__ lpart) {
+    __shared__ double sh[4];
+    double acc = 0;
     const int64_t n = nper * batch;
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const int b = (int)(i / nper);
-        const float t = e[i] * inv[b];
-        const float diff = z[i] - t;
+        const float diff = z[i] - e[i] * inv[b];
         const float gd = Ad ? Ad[i] : diff;
-        const float dz = (float)(2.0 * gd / ((double)batch * t2[b]));
-        acc[17] += (double)diff * gd / t2[b];
+        dz[i] = (float)(2.0 * gd / ((double)batch * t2[b]));
+        acc += (double)diff * gd / t2[b];
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) lpart[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// Through the head: G0 = hw dZ act'(d0); partials of dhw (16) and dhb.
+__global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dzv, int64_t n,
+                                                  const uint16_t* __restrict__ d0, const float* __restrict__ head,
+                                                  uint16_t* __restrict__ g0, double* __restrict__ part) {
+    __shared__ double sh[4][17];
+    double acc[17];
+    for (int q = 0; q < 17; ++q) acc[q] = 0;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float dz = dzv[i];
         acc[16] += dz;
         uint32_t o[8];
         for (int c8 = 0; c8 < 2; ++c8) {
@@ -632,28 +776,32 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ z, c
         *(u32x4_t*)&g0[i * 16] = u32x4_t{o[0], o[1], o[2], o[3]};
         *(u32x4_t*)&g0[i * 16 + 8] = u32x4_t{o[4], o[5], o[6], o[7]};
     }
-    for (int q = 0; q < 18; ++q) {
+    for (int q = 0; q < 17; ++q) {
         double s = acc[q];
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         acc[q] = s;
     }
     if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 18; ++q) sh[threadIdx.x >> 6][q] = acc[q];
+        for (int q = 0; q < 17; ++q) sh[threadIdx.x >> 6][q] = acc[q];
     __syncthreads();
-    if (threadIdx.x < 18) {
+    if (threadIdx.x < 17) {
         const int q = threadIdx.x;
-        part[blockIdx.x * 18 + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+        part[blockIdx.x * 17 + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
     }
 }
-// head gradients (dhw[16], dhb) into grad[], loss into loss[0]
-__global__ void k_head_fin(const double* __restrict__ part, int nblk, float* __restrict__ ghead, double* loss,
-                           int batch) {
+// head gradients (dhw[16], dhb) into grad[], the loss into loss[0]
+__global__ void k_head_fin(const double* __restrict__ part, const double* __restrict__ lpart, int nblk,
+                           float* __restrict__ ghead, double* loss, int batch) {
     const int q = threadIdx.x;
-    if (q >= 18) return;
-    double s = 0;
-    for (int i = 0; i < nblk; ++i) s += part[i * 18 + q];
-    if (q < 17) ghead[q] = (float)s;
-    else *loss = s / batch;
+    if (q < 17) {
+        double s = 0;
+        for (int i = 0; i < nblk; ++i) s += part[i * 17 + q];
+        ghead[q] = (float)s;
+    } else if (q == 17) {
+        double s = 0;
+        for (int i = 0; i < nblk; ++i) s += lpart[i];
+        *loss = s / batch;
+    }
 }
 
 __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -713,6 +861,7 @@ struct nnx_unet {
     uint16_t* wb[5];             // bf16 flipped weights [cin][kpad(cout)]
     int64_t step;
     bool dirty;                  // bf16 copies out of date
+    int coarse;                  // 1: two-grid mode (network on the 2x coarser grid)
 };
 
 namespace {
@@ -787,11 +936,12 @@ nnx_status prep_weights(nnx_ctx* c, nnx_unet* u) {
 
 struct Acts {
     uint16_t *x, *e0, *p0, *e1, *p1, *e2, *d1, *d0;
-    float *s, *inv, *z;
+    float *s, *inv, *z, *zc;
     double* part;
 };
 
-nnx_status acts(nnx_ctx* c, int64_t n0, int batch, bool train, Acts* A) {
+// n0: voxels at the network's finest level; nf: voxels of the caller's grid
+nnx_status acts(nnx_ctx* c, int64_t n0, int64_t nf, int batch, bool train, bool coarse, Acts* A) {
     const int64_t n1 = n0 / 8, n2 = n0 / 64;
     void* p;
     NNX_TRY(scratch(c, "un_x", n0 * 4, false, &p));
@@ -810,63 +960,83 @@ nnx_status acts(nnx_ctx* c, int64_t n0, int batch, bool train, Acts* A) {
     A->d1 = (uint16_t*)p;
     A->d0 = nullptr;
     A->z = nullptr;
+    A->zc = nullptr;
     if (train) {
         NNX_TRY(scratch(c, "un_d0", n0 * 32, false, &p));
         A->d0 = (uint16_t*)p;
-        NNX_TRY(scratch(c, "un_z", n0 * 4, false, &p));
+        NNX_TRY(scratch(c, "un_z", nf * 4, false, &p));
         A->z = (float*)p;
+    }
+    if (coarse) {
+        NNX_TRY(scratch(c, "un_zc", n0 * 4, false, &p));
+        A->zc = (float*)p;
     }
     NNX_TRY(scratch(c, "un_s", 4096, false, &p));
     A->s = (float*)p;
     A->inv = (float*)p + 512;
     NNX_TRY(scratch(c, "un_part", (size_t)std::max(kSumBlocks * batch, kHeadBlocks * 18) * 8 + 4096, false, &p));
     A->part = (double*)p;
-    (void)batch;
     return NNX_OK;
 }
 
-nnx_status check_dims(nnx_ctx* c, int nx, int ny, int nz, int batch) {
+nnx_status check_dims(nnx_ctx* c, const nnx_unet* u, int nx, int ny, int nz, int batch) {
     NNX_CHECK(c, nx > 0 && ny > 0 && nz > 0 && batch >= 1 && batch <= 256, "bad dims or batch");
-    NNX_CHECK(c, nx % 64 == 0 && ny % 32 == 0 && nz % 32 == 0,
-              "U-Net grids need nx %% 64 == 0 and ny, nz %% 32 == 0 (got %d, %d, %d)", nx, ny, nz);
+    const int f = u->coarse ? 2 : 1;
+    NNX_CHECK(c, nx % (64 * f) == 0 && ny % (32 * f) == 0 && nz % (32 * f) == 0,
+              "U-Net grids need nx %% %d == 0 and ny, nz %% %d == 0 (got %d, %d, %d)", 64 * f, 32 * f, nx, ny,
+              nz);
     NNX_CHECK(c, (int64_t)nx * ny * nz * batch < ((int64_t)1 << 31), "grid too large");
     return NNX_OK;
 }
 
-// forward pass; z = hscale * head (hscale = rms(r) per batch, or 1 when train)
+// Forward pass on the caller's grid (nx, ny, nz); z = rms(r) * N(r / rms(r))
+// (train: z unscaled).  In two-grid mode the network runs on the 2x coarser
+// grid between restriction and prolongation.
 nnx_status forward(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* r, const float* feat,
                    float* z, bool train, Acts& A) {
     const Plan& P = plan();
-    const int64_t nper = (int64_t)nx * ny * nz, n0 = nper * batch;
+    const int64_t nper = (int64_t)nx * ny * nz;
+    const int f = u->coarse ? 2 : 1;
+    const int mx = nx / f, my = ny / f, mz = nz / f;  // the network's finest level
+    const int64_t n0 = (int64_t)mx * my * mz * batch;
     NNX_TRY(prep_weights(c, u));
     k_sumsq<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(r, nper, A.part);
     k_rms_fin<<<batch, 256, 0, c->stream>>>(A.part, kSumBlocks, nper, batch, A.s, A.inv, nullptr);
-    k_pack<<<eblocks(n0), 256, 0, c->stream>>>(r, feat, A.inv, nper, batch, (uint32_t*)A.x);
+    if (u->coarse)
+        k_restrict_pack<<<eblocks(n0), 256, 0, c->stream>>>(r, feat, A.inv, nx, ny, nz, batch, (uint32_t*)A.x);
+    else
+        k_pack<<<eblocks(n0), 256, 0, c->stream>>>(r, feat, A.inv, nper, batch, (uint32_t*)A.x);
     NNX_LAUNCHED(c);
     ConvArgs a{};
     a.batch = batch;
     a.relu = 1;
     // e0 (+ p0)
-    a.nx = nx, a.ny = ny, a.nz = nz;
+    a.nx = mx, a.ny = my, a.nz = mz;
     a.in = A.x, a.w = u->wf[0], a.bias = u->p + P.L[0].b_off, a.out = A.e0, a.pool = A.p0;
     NNX_TRY(conv_dispatch(c, 2, 16, a));
     // e1 (+ p1)
-    a.nx = nx / 2, a.ny = ny / 2, a.nz = nz / 2;
+    a.nx = mx / 2, a.ny = my / 2, a.nz = mz / 2;
     a.in = A.p0, a.w = u->wf[1], a.bias = u->p + P.L[1].b_off, a.out = A.e1, a.pool = A.p1;
     NNX_TRY(conv_dispatch(c, 16, 32, a));
     // e2
-    a.nx = nx / 4, a.ny = ny / 4, a.nz = nz / 4;
+    a.nx = mx / 4, a.ny = my / 4, a.nz = mz / 4;
     a.in = A.p1, a.w = u->wf[2], a.bias = u->p + P.L[2].b_off, a.out = A.e2, a.pool = nullptr;
     NNX_TRY(conv_dispatch(c, 32, 32, a));
-    // d1 = relu(conv(e1 + up(e2)))
-    a.nx = nx / 2, a.ny = ny / 2, a.nz = nz / 2;
+    // d1 = act(conv(e1 + up(e2)))
+    a.nx = mx / 2, a.ny = my / 2, a.nz = mz / 2;
     a.in = A.e1, a.coarse = A.e2, a.cscale = 1.f, a.w = u->wf[3], a.bias = u->p + P.L[3].b_off, a.out = A.d1;
     NNX_TRY(conv_dispatch(c, 32, 16, a));
-    // d0 = relu(conv(e0 + up(d1))) -> head
-    a.nx = nx, a.ny = ny, a.nz = nz;
+    // d0 = act(conv(e0 + up(d1))) -> head
+    a.nx = mx, a.ny = my, a.nz = mz;
     a.in = A.e0, a.coarse = A.d1, a.w = u->wf[4], a.bias = u->p + P.L[4].b_off, a.out = train ? A.d0 : nullptr;
-    a.head = u->p + P.head_off, a.hscale = train ? nullptr : A.s, a.hout = z;
+    a.head = u->p + P.head_off;
+    a.hscale = (train || u->coarse) ? nullptr : A.s;
+    a.hout = u->coarse ? A.zc : z;
     NNX_TRY(conv_dispatch(c, 16, 16, a));
+    if (u->coarse) {
+        k_prolong<<<eblocks(nper * batch), 256, 0, c->stream>>>(A.zc, nx, ny, nz, batch, train ? nullptr : A.s, z);
+        NNX_LAUNCHED(c);
+    }
     return NNX_OK;
 }
 
@@ -874,8 +1044,11 @@ nnx_status forward(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, c
 
 nnx_status nnx_unet_forward_internal(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* r,
                                      const float* feat, float* z) {
+    NNX_TRY(check_dims(c, u, nx, ny, nz, batch));
+    const int f = u->coarse ? 2 : 1;
+    const int64_t nf = (int64_t)nx * ny * nz * batch;
     Acts A;
-    NNX_TRY(acts(c, (int64_t)nx * ny * nz * batch, batch, false, &A));
+    NNX_TRY(acts(c, nf / (f * f * f), nf, batch, false, u->coarse != 0, &A));
     return forward(c, u, nx, ny, nz, batch, r, feat, z, false, A);
 }
 
@@ -934,6 +1107,7 @@ nnx_status nnx_unet_create(nnx_ctx* c, const float* h_params, uint64_t seed, nnx
     }
     u->step = 0;
     u->dirty = true;
+    u->coarse = 0;
     *out = u;
     return NNX_OK;
 }
@@ -965,20 +1139,30 @@ nnx_status nnx_unet_apply(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int b
                           const float* d_feat, float* d_z) {
     if (!c) return NNX_ERR_ARG;
     NNX_CHECK(c, u && d_r && d_z, "null argument");
-    NNX_TRY(check_dims(c, nx, ny, nz, batch));
     return nnx_unet_forward_internal(c, u, nx, ny, nz, batch, d_r, d_feat, d_z);
+}
+
+nnx_status nnx_unet_set_coarse(nnx_ctx* c, nnx_unet* u, int coarse) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, u, "null network");
+    u->coarse = coarse ? 1 : 0;
+    return NNX_OK;
 }
 
 nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, const float* d_r,
                                const float* d_feat, const float* d_e, const float* d_kx, const float* d_ky,
-                               const float* d_kz, double lr, double* h_loss) {
+                               const float* d_kz, int smooth, double lr, double* h_loss) {
     if (!c) return NNX_ERR_ARG;
     NNX_CHECK(c, u && d_r && d_e, "null argument");
-    NNX_TRY(check_dims(c, nx, ny, nz, batch));
+    NNX_TRY(check_dims(c, u, nx, ny, nz, batch));
     const Plan& P = plan();
-    const int64_t nper = (int64_t)nx * ny * nz, n0 = nper * batch, n1 = n0 / 8, n2 = n0 / 64;
+    const int64_t nper = (int64_t)nx * ny * nz, nf = nper * batch;
+    // the network's levels (the caller's grid, or the 2x coarser one)
+    const int f = u->coarse ? 2 : 1;
+    const int mx = nx / f, my = ny / f, mz = nz / f;
+    const int64_t n0 = nf / (f * f * f), n1 = n0 / 8, n2 = n0 / 64;
     Acts A;
-    NNX_TRY(acts(c, n0, batch, true, &A));
+    NNX_TRY(acts(c, n0, nf, batch, true, u->coarse != 0, &A));
     NNX_TRY(forward(c, u, nx, ny, nz, batch, d_r, d_feat, A.z, true, A));
     void *p, *lossp;
     NNX_TRY(scratch(c, "un_g0", n0 * 32, false, &p));
@@ -1003,19 +1187,58 @@ nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, 
     NNX_CHECK(c, !anorm || (d_ky && d_kz), "A-norm loss needs kx, ky and kz");
     float* Ad = nullptr;
     if (anorm) {
-        NNX_TRY(scratch(c, "un_ad", n0 * 4, false, &p));
+        NNX_TRY(scratch(c, "un_ad", nf * 4, false, &p));
         Ad = (float*)p;
         k_anorm<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(nx, ny, nz, A.z, d_e, A.inv, d_kx, d_ky, d_kz, Ad,
                                                                 A.part);
+        if (smooth > 0) {
+            // smoothed A-norm: |S^m d|_A^2 with S = I - w D^-1 A; gradient
+            // 2 (S^T)^m A S^m d (replaces Ad; the loss kernel's d . Ad is
+            // then |S^m d|_A^2)
+            NNX_CHECK(c, smooth <= 4, "at most 4 smoothing sweeps in the loss");
+            void *q1, *q2;
+            NNX_TRY(scratch(c, "un_sm1", nf * 4, false, &q1));
+            NNX_TRY(scratch(c, "un_sm2", nf * 4, false, &q2));
+            float *U = (float*)q1, *V = (float*)q2;
+            const unsigned eb = eblocks(nf);
+            k_diff<<<eb, 256, 0, c->stream>>>(A.z, d_e, A.inv, nper, batch, Ad);
+            float *src = Ad, *dst = U;
+            for (int m = 0; m < smooth; ++m) {  // src <- S src
+                k_sample_op<<<eb, 256, 0, c->stream>>>(nx, ny, nz, batch, 1, src, nullptr, d_kx, d_ky, d_kz, dst);
+                std::swap(src, dst);
+            }
+            k_sample_op<<<eb, 256, 0, c->stream>>>(nx, ny, nz, batch, 0, src, nullptr, d_kx, d_ky, d_kz, dst);
+            std::swap(src, dst);  // src = A S^m d
+            for (int m = 0; m < smooth; ++m) {  // src <- S^T src
+                k_sample_op<<<eb, 256, 0, c->stream>>>(nx, ny, nz, batch, 2, src, nullptr, d_kx, d_ky, d_kz, dst);
+                k_sample_op<<<eb, 256, 0, c->stream>>>(nx, ny, nz, batch, 3, dst, src, d_kx, d_ky, d_kz, V);
+                std::swap(src, V);
+                if (V == Ad) std::swap(V, dst);
+            }
+            if (src != Ad) NNX_HIP(c, hipMemcpyAsync(Ad, src, nf * 4, hipMemcpyDeviceToDevice, c->stream));
+            NNX_LAUNCHED(c);
+        }
     } else {
         k_t2<<<dim3(kSumBlocks, batch), 256, 0, c->stream>>>(d_e, A.inv, nper, A.part);
     }
     k_rms_fin<<<batch, 256, 0, c->stream>>>(A.part, kSumBlocks, 1, batch, A.s + 256, A.inv + 256, t2);
-    k_head_bwd<<<kHeadBlocks, 256, 0, c->stream>>>(A.z, d_e, A.inv, t2, Ad, nper, batch, A.d0, u->p + P.head_off,
-                                                   G0, A.part);
-    k_head_fin<<<1, 64, 0, c->stream>>>(A.part, kHeadBlocks, u->grad + P.head_off, (double*)lossp, batch);
+    // dZ on the caller's grid, then (two-grid) through the prolongation
+    void *dzp, *dzcp, *lpart;
+    NNX_TRY(scratch(c, "un_dz", nf * 4, false, &dzp));
+    NNX_TRY(scratch(c, "un_lpart", kHeadBlocks * 8, false, &lpart));
+    k_loss_grad<<<kHeadBlocks, 256, 0, c->stream>>>(A.z, d_e, A.inv, t2, Ad, nper, batch, (float*)dzp,
+                                                    (double*)lpart);
+    float* dzn = (float*)dzp;
+    if (u->coarse) {
+        NNX_TRY(scratch(c, "un_dzc", n0 * 4, false, &dzcp));
+        k_prolong_adj<<<eblocks(n0), 256, 0, c->stream>>>((const float*)dzp, nx, ny, nz, batch, (float*)dzcp);
+        dzn = (float*)dzcp;
+    }
+    k_head_bwd<<<kHeadBlocks, 256, 0, c->stream>>>(dzn, n0, A.d0, u->p + P.head_off, G0, A.part);
+    k_head_fin<<<1, 64, 0, c->stream>>>(A.part, (const double*)lpart, kHeadBlocks, u->grad + P.head_off,
+                                        (double*)lossp, batch);
     NNX_LAUNCHED(c);
-    const int nx1 = nx / 2, ny1 = ny / 2, nz1 = nz / 2, nx2 = nx / 4, ny2 = ny / 4, nz2 = nz / 4;
+    const int nx1 = mx / 2, ny1 = my / 2, nz1 = mz / 2, nx2 = mx / 4, ny2 = my / 4, nz2 = mz / 4;
     auto WG = [&](int nxl, int nyl, int nzl, const uint16_t* in, const uint16_t* coarse, float cs, const uint16_t* g) {
         WgArgs w{};
         w.nx = nxl, w.ny = nyl, w.nz = nzl, w.batch = batch, w.in = in, w.coarse = coarse, w.cscale = cs, w.g = g;
@@ -1027,9 +1250,9 @@ nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, 
         return a;
     };
     // d0: input e0 + up(d1), gradient G0
-    NNX_TRY(wgrad_dispatch(c, 16, 16, WG(nx, ny, nz, A.e0, A.d1, 1.f, G0), u->grad + P.L[4].w_off,
+    NNX_TRY(wgrad_dispatch(c, 16, 16, WG(mx, my, mz, A.e0, A.d1, 1.f, G0), u->grad + P.L[4].w_off,
                            u->grad + P.L[4].b_off));
-    NNX_TRY(conv_dispatch(c, 16, 16, BD(nx, ny, nz, G0, u->wb[4], U0)));  // dU0 (skip part of e0's gradient)
+    NNX_TRY(conv_dispatch(c, 16, 16, BD(mx, my, mz, G0, u->wb[4], U0)));  // dU0 (skip part of e0's gradient)
     k_sumpool8<<<eblocks(n1 * 16), 256, 0, c->stream>>>(nx1, ny1, nz1, batch, 16, U0, L1a);  // d(d1)
     k_grad_combine<<<eblocks(n1 * 2), 256, 0, c->stream>>>(nx1, ny1, nz1, batch, 16, A.d1, L1a, nullptr, 0.f, L1b);
     NNX_LAUNCHED(c);
@@ -1051,9 +1274,9 @@ nnx_status nnx_unet_train_step(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, 
                            u->grad + P.L[1].b_off));
     NNX_TRY(conv_dispatch(c, 32, 16, BD(nx1, ny1, nz1, L1a, u->wb[1], L1b)));  // d(p0) (16 ch)
     // e0 gradient: dU0 + d(p0)/8
-    k_grad_combine<<<eblocks(n0 * 2), 256, 0, c->stream>>>(nx, ny, nz, batch, 16, A.e0, U0, L1b, 0.125f, G0);
+    k_grad_combine<<<eblocks(n0 * 2), 256, 0, c->stream>>>(mx, my, mz, batch, 16, A.e0, U0, L1b, 0.125f, G0);
     NNX_LAUNCHED(c);
-    NNX_TRY(wgrad_dispatch(c, 2, 16, WG(nx, ny, nz, A.x, nullptr, 0.f, G0), u->grad + P.L[0].w_off,
+    NNX_TRY(wgrad_dispatch(c, 2, 16, WG(mx, my, mz, A.x, nullptr, 0.f, G0), u->grad + P.L[0].w_off,
                            u->grad + P.L[0].b_off));
     // the biases are held at zero (the network must map r = 0 to 0)
     for (int l = 0; l < 5; ++l)

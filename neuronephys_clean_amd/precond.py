@@ -9,6 +9,7 @@ No counterpart in the reference: its paracrine solve is unpreconditioned CG
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
 import os
 import time
@@ -27,7 +28,11 @@ def param_count() -> int:
 
 
 class UNet:
-    def __init__(self, ctx: _lib.Context, params: np.ndarray | None = None, seed: int = 0):
+    """coarse: two-grid mode (the network on the 2x coarser grid between
+    block-mean restriction and trilinear prolongation)."""
+
+    def __init__(self, ctx: _lib.Context, params: np.ndarray | None = None, seed: int = 0,
+                 coarse: bool = False):
         self.ctx = ctx
         self.h = C.c_void_p()
         arr = None
@@ -36,10 +41,18 @@ class UNet:
             assert arr.size == param_count(), (arr.size, param_count())
         ctx.call("nnx_unet_create", None if arr is None else arr.ctypes.data_as(C.c_void_p), seed,
                  C.byref(self.h))
+        self.coarse = bool(coarse)
+        ctx.call("nnx_unet_set_coarse", self.h, int(self.coarse))
 
     @classmethod
     def load(cls, ctx, path: str = DEFAULT_WEIGHTS) -> "UNet":
-        return cls(ctx, np.fromfile(path, np.float32))
+        """Parameters from `path`; the mode from the report beside it."""
+        coarse = False
+        meta = os.path.splitext(path)[0] + ".json"
+        if os.path.exists(meta):
+            with open(meta) as fh:
+                coarse = bool(json.load(fh).get("coarse", False))
+        return cls(ctx, np.fromfile(path, np.float32), coarse=coarse)
 
     def save(self, path: str):
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
@@ -61,14 +74,15 @@ class UNet:
                       None if feat is None else feat.ptr, z.ptr)
 
     def train_step(self, dims, batch, r: DA, feat: DA | None, e: DA, lr: float, want_loss=True,
-                   faces=None):
+                   faces=None, smooth=0):
         """faces: (kx, ky, kz) device arrays of the batch's operators for the
-        A-norm loss, or None for the plain relative MSE."""
+        A-norm loss (smooth > 0: after that many Jacobi post-smoothing
+        sweeps), or None for the plain relative MSE."""
         nx, ny, nz = dims
         loss = C.c_double()
         kx, ky, kz = (None, None, None) if faces is None else (f.ptr for f in faces)
         self.ctx.call("nnx_unet_train_step", self.h, nx, ny, nz, batch, r.ptr,
-                      None if feat is None else feat.ptr, e.ptr, kx, ky, kz, lr,
+                      None if feat is None else feat.ptr, e.ptr, kx, ky, kz, int(smooth), lr,
                       C.byref(loss) if want_loss else None)
         return loss.value if want_loss else None
 
@@ -208,7 +222,7 @@ def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 
 
 
 def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=3e-4, lr_min=1e-5, warmup=500, seed=0,
-          anorm=True, log=print):
+          anorm=True, smooth=0, log=print):
     """Adam, linear warm-up then a cosine learning-rate schedule, over
     shuffled pairs; the A-norm loss by default (errors the operator
     amplifies - rough ones - weigh most, as they do for the solver)."""
@@ -227,7 +241,7 @@ def train(ctx, net: UNet, ps: PairSet, batch=8, epochs=3, lr=3e-4, lr_min=1e-5, 
             if step < warmup:
                 cur = lr * (step + 1) / warmup
             want = (step % 50 == 0) or step == steps - 1
-            loss = net.train_step((n, n, n), batch, rb, fb, eb, cur, want_loss=want, faces=fcs)
+            loss = net.train_step((n, n, n), batch, rb, fb, eb, cur, want_loss=want, faces=fcs, smooth=smooth)
             if want:
                 losses.append((step, loss))
                 if step % 500 == 0:

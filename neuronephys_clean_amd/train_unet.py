@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import os
 import sys
 import time
 
@@ -29,7 +30,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def compare(ctx, net, n, seed, sources=400, rtol=1e-8):
+def compare(ctx, net, n, seed, sources=400, rtol=1e-8, smooth=1):
     """Jacobi-CG vs learned FCG, both inside FP64 refinement, on one field."""
     f = configs.c5_field(seed=seed, n=n)
     g, keep = device_field(ctx, f)
@@ -41,7 +42,7 @@ def compare(ctx, net, n, seed, sources=400, rtol=1e-8):
         for _ in range(2):
             r = _lib.CgResult()
             o = _lib.CgOpts(rtol=rtol, maxit=20000, precond=_lib.PRECOND_JACOBI if pc is None else pc,
-                            check_every=8, stall_iters=10)
+                            check_every=8, stall_iters=10, smooth=smooth)
             if pc is None:
                 ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-6, 30, C.byref(r))
             else:
@@ -78,16 +79,23 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--init", default=None, help="start from these parameters")
     ap.add_argument("--eval-n", type=int, nargs="*", default=[64, 256])
+    ap.add_argument("--n", type=int, default=64, help="training grid size")
+    ap.add_argument("--coarse", action="store_true", help="two-grid mode")
+    ap.add_argument("--smooth", type=int, default=1, help="post-smoothing sweeps in the evaluation")
+    ap.add_argument("--loss", choices=["mse", "anorm", "smoothed"], default="smoothed",
+                    help="smoothed: A-norm of the error left after --smooth post-smoothing sweeps")
     args = ap.parse_args()
     ctx = _lib.context(0)
     t0 = time.time()
-    ps = harvest(ctx, n=64, fields=args.fields, sources=args.sources, seed=args.seed, log=log)
+    ps = harvest(ctx, n=args.n, fields=args.fields, sources=args.sources, seed=args.seed, log=log)
     ctx.synchronize()
     t_h = time.time() - t0
     log(f"harvested {ps.count} pairs in {t_h:.1f}s")
-    net = UNet.load(ctx, args.init) if args.init else UNet(ctx, seed=args.seed)
+    net = UNet(ctx, np.fromfile(args.init, np.float32) if args.init else None, seed=args.seed,
+               coarse=args.coarse)
     t0 = time.time()
-    losses = train(ctx, net, ps, batch=args.batch, epochs=args.epochs, lr=args.lr, log=log)
+    losses = train(ctx, net, ps, batch=args.batch, epochs=args.epochs, lr=args.lr, anorm=args.loss != "mse",
+                   smooth=args.smooth if args.loss == "smoothed" else 0, log=log)
     ctx.synchronize()
     t_t = time.time() - t0
     steps = args.epochs * (ps.count // args.batch)
@@ -95,13 +103,14 @@ def main():
         f"loss {losses[0][1]:.4f} -> {losses[-1][1]:.4f}")
     net.save(args.out)
     del ps
-    report = {"pairs": args.fields * args.sources * 8, "harvest_s": t_h, "train_s": t_t,
+    report = {"pairs": args.fields * args.sources * 8, "n": args.n, "coarse": args.coarse,
+              "smooth": args.smooth, "loss": args.loss, "harvest_s": t_h, "train_s": t_t,
               "steps": steps, "batch": args.batch, "epochs": args.epochs,
               "loss_first": losses[0][1], "loss_last": losses[-1][1], "losses": losses, "eval": {}}
     for n in args.eval_n:
-        report["eval"][str(n)] = compare(ctx, net, n, seed=424242 + n)
+        report["eval"][str(n)] = compare(ctx, net, n, seed=424242 + n, smooth=args.smooth)
         log(f"eval {n}^3: {report['eval'][str(n)]}")
-    with open(args.out + ".json", "w") as fh:
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as fh:
         json.dump(report, fh, indent=1)
 
 

@@ -198,9 +198,14 @@ def apply_A(u, kx, ky, kz):
     return dg * u - s
 
 
-def reference_grad(p, r, feat, e, faces=None):
+def diag_A(kx, ky, kz):
+    return 1 + kx[..., :-1] + kx[..., 1:] + ky[:, :, :-1, :] + ky[:, :, 1:, :] + kz[:, :-1] + kz[:, 1:]
+
+
+def reference_grad(p, r, feat, e, faces=None, coarse=False, smooth=0):
     """Loss and parameter gradient by autograd through the float64 network
-    with BF16 rounding (straight-through); biases held at zero."""
+    with BF16 rounding (straight-through); biases held at zero; coarse: the
+    two-grid mode (block-mean restriction, network, trilinear prolongation)."""
     pt = torch.from_numpy(p.astype(np.float64)).requires_grad_(True)
     R = lambda t: t + (rnd_bf16(t) - t).detach()  # noqa: E731
     W, o = [], 0
@@ -210,7 +215,10 @@ def reference_grad(p, r, feat, e, faces=None):
         o += cout * 27 * cin + cout
     rt = torch.from_numpy(r.astype(np.float64))
     s = torch.sqrt((rt ** 2).mean(dim=(1, 2, 3), keepdim=True))
-    x = torch.stack([R(rt / s), R(rt / s * torch.from_numpy(feat.astype(np.float64)))], dim=1)
+    q, qf = rt / s, rt / s * torch.from_numpy(feat.astype(np.float64))
+    if coarse:
+        q, qf = (F.avg_pool3d(a.unsqueeze(1), 2)[:, 0] for a in (q, qf))
+    x = torch.stack([R(q), R(qf)], dim=1)
     conv = lambda t, i: F.leaky_relu(F.conv3d(t, W[i][0], W[i][1], padding=1), 0.1)  # noqa: E731
     e0f = conv(x, 0)
     e0, p0 = R(e0f), R(F.avg_pool3d(e0f, 2))
@@ -220,13 +228,18 @@ def reference_grad(p, r, feat, e, faces=None):
     d1 = R(conv(R(e1 + up2(e2)), 3))
     d0 = conv(R(e0 + up2(d1)), 4)
     z = (d0 * pt[o:o + 16].view(1, 16, 1, 1, 1)).sum(1) + pt[o + 16]
+    if coarse:
+        z = prolong(z)
     t = torch.from_numpy(e.astype(np.float64)) / s
     d = z - t
     if faces is None:
         L = ((d ** 2).sum(dim=(1, 2, 3)) / (t ** 2).sum(dim=(1, 2, 3))).mean()
     else:
         K = [torch.from_numpy(a.astype(np.float64)) for a in faces]
-        L = ((d * apply_A(d, *K)).sum(dim=(1, 2, 3)) / (t * apply_A(t, *K)).sum(dim=(1, 2, 3))).mean()
+        u = d
+        for _ in range(smooth):
+            u = u - (2.0 / 3.0) * apply_A(u, *K) / diag_A(*K)
+        L = ((u * apply_A(u, *K)).sum(dim=(1, 2, 3)) / (t * apply_A(t, *K)).sum(dim=(1, 2, 3))).mean()
     L.backward()
     g = pt.grad.numpy().copy()
     o = 0
@@ -238,12 +251,13 @@ def reference_grad(p, r, feat, e, faces=None):
     return float(L.detach()), g
 
 
-@pytest.mark.parametrize("anorm", [False, True])
-def test_unet_gradient_matches_autograd(ctx, anorm):
-    dims, B = (64, 32, 32), 2
+@pytest.mark.parametrize("anorm,coarse,smooth", [(False, False, 0), (True, False, 0), (True, True, 0),
+                                                  (True, False, 2)])
+def test_unet_gradient_matches_autograd(ctx, anorm, coarse, smooth):
+    dims, B = ((128, 64, 64), 1) if coarse else ((64, 32, 32), 2)
     nx, ny, nz = dims
     rng = np.random.default_rng(4)
-    net = UNet(ctx, seed=12)
+    net = UNet(ctx, seed=12, coarse=coarse)
     p = net.params()
     r = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
     feat = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
@@ -254,10 +268,10 @@ def test_unet_gradient_matches_autograd(ctx, anorm):
                  for sh in ((B, nz, ny, nx + 1), (B, nz, ny + 1, nx), (B, nz + 1, ny, nx))]
     dr, df, de = (DA.from_numpy(ctx, a) for a in (r, feat, e))
     dfc = [DA.from_numpy(ctx, a) for a in faces] if anorm else None
-    say("unet gradient", dims, B, "A-norm" if anorm else "MSE")
-    loss = net.train_step(dims, B, dr, df, de, 0.0, faces=dfc)
+    say("unet gradient", dims, B, "A-norm" if anorm else "MSE", "two-grid" if coarse else "", "smooth", smooth)
+    loss = net.train_step(dims, B, dr, df, de, 0.0, faces=dfc, smooth=smooth)
     got = net.grad()
-    ref_loss, ref = reference_grad(p, r, feat, e, faces)
+    ref_loss, ref = reference_grad(p, r, feat, e, faces, coarse, smooth)
     say("loss", loss, ref_loss)
     assert abs(loss - ref_loss) <= 2e-2 * abs(ref_loss)
     o = 0
@@ -271,6 +285,66 @@ def test_unet_gradient_matches_autograd(ctx, anorm):
         say("layer", i, "grad rel err", rel, "cos", cos)
         assert rel < 5e-2 and cos > 0.998
         o += nw + nb
+
+
+def prolong_axis(t, dim):
+    """cell-centred linear prolongation x2 along dim, zero outside"""
+    n = t.shape[dim]
+    tp = F.pad(t.movedim(dim, -1), (1, 1)).movedim(-1, dim)
+    sl = lambda a, b: tp.narrow(dim, a, b)  # noqa: E731
+    even = 0.25 * sl(0, n) + 0.75 * sl(1, n)
+    odd = 0.75 * sl(1, n) + 0.25 * sl(2, n)
+    return torch.stack([even, odd], dim=dim + 1).flatten(dim, dim + 1)
+
+
+def prolong(t):  # (B, nz, ny, nx)
+    return prolong_axis(prolong_axis(prolong_axis(t, 1), 2), 3)
+
+
+def two_grid_input(r, feat):
+    s = torch.sqrt((r ** 2).mean(dim=(1, 2, 3), keepdim=True))
+    q = r / s
+    x = torch.stack([rnd_bf16(F.avg_pool3d(q.unsqueeze(1), 2)[:, 0]),
+                     rnd_bf16(F.avg_pool3d((q * feat).unsqueeze(1), 2)[:, 0])], dim=1)
+    return x, s
+
+
+def core(p, x):
+    """the five layers and the head on a 2-channel input (B, 2, z, y, x)"""
+    layers, hw, hb = unpack(p)
+    W = [(rnd_bf16(torch_w(w.astype(np.float64), ci, co)), torch.from_numpy(b.astype(np.float64)))
+         for (w, b), (ci, co) in zip(layers, LAYERS)]
+    conv = lambda t, i: F.leaky_relu(F.conv3d(t, W[i][0], W[i][1], padding=1), 0.1)  # noqa: E731
+    e0f = conv(x, 0)
+    e0, p0 = rnd_bf16(e0f), rnd_bf16(F.avg_pool3d(e0f, 2))
+    e1f = conv(p0, 1)
+    e1, p1 = rnd_bf16(e1f), rnd_bf16(F.avg_pool3d(e1f, 2))
+    e2 = rnd_bf16(conv(p1, 2))
+    d1 = rnd_bf16(conv(rnd_bf16(e1 + up2(e2)), 3))
+    d0 = conv(rnd_bf16(e0 + up2(d1)), 4)
+    return (d0 * torch.from_numpy(hw.astype(np.float64)).view(1, 16, 1, 1, 1)).sum(1) + float(hb)
+
+
+def test_unet_two_grid_forward_matches_torch(ctx):
+    dims, B = (128, 64, 64), 2
+    nx, ny, nz = dims
+    rng = np.random.default_rng(9)
+    net = UNet(ctx, seed=21, coarse=True)
+    p = net.params()
+    r = rng.normal(0, 2, (B, nz, ny, nx)).astype(np.float32)
+    feat = rng.normal(0, 1, (B, nz, ny, nx)).astype(np.float32)
+    dr, df, dz = DA.from_numpy(ctx, r), DA.from_numpy(ctx, feat), DA.zeros(ctx, r.shape, np.float32)
+    say("two-grid forward", dims, B)
+    net.apply(dims, B, dr, df, dz)
+    x, s = two_grid_input(torch.from_numpy(r.astype(np.float64)), torch.from_numpy(feat.astype(np.float64)))
+    ref = (s * prolong(core(p, x))).numpy()
+    got = dz.numpy()
+    err = np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean(ref ** 2))
+    say("two-grid forward rel rms err", err)
+    assert err < 1e-2
+    # prolongation of a constant coarse field away from the boundary is that constant
+    with pytest.raises(Exception):
+        net.apply((64, 32, 32), 1, dr, df, dz)  # two-grid mode needs (128, 64, 64) multiples
 
 
 def test_unet_training_reduces_loss(ctx):
