@@ -284,10 +284,31 @@ def stencil_roofline(ctx, n=256, reps=50):
 
 
 def profile_pass(ctx, sim, net):
-    from neuronephys_clean_amd import _lib
+    from neuronephys_clean_amd import _lib, configs
     from neuronephys_clean_amd.devmem import DeviceArray as DA
+    from neuronephys_clean_amd.precond import UNet
+    from neuronephys_clean_amd.sim import EnsembleSim
     sim.run(20)  # eager: one dispatch per kernel per step
     sim.synchronize()
+    # the HH kernel alone (the roofline measurement of the benchmark)
+    st, p = sim.plan.state, sim.params
+    for _ in range(20):
+        ctx.call("nnx_hh_step", C.byref(st), C.byref(p), _lib.INT_RK4, _lib.SPIKE_CROSSING, net.dt, 0,
+                 sim.a["g_in"].ptr, sim.a["bits0"].ptr)
+    ctx.synchronize()
+    # 20 steps of the 256-network ensemble (default launch shape)
+    ens = EnsembleSim(configs.c4_ensemble(seed=0, count=256), ctx=ctx)
+    ens.run(20)
+    ens.synchronize()
+    del ens
+    # 5 applications of the preconditioner network at 256^3
+    u = UNet.load(ctx)
+    rr = DA.from_numpy(ctx, np.random.default_rng(0).normal(0, 1, 256 ** 3).astype(np.float32))
+    zz = DA.zeros(ctx, 256 ** 3, np.float32)
+    for _ in range(5):
+        u.apply((256, 256, 256), 1, rr, None, zz)
+    ctx.synchronize()
+    u.close()
     stencil_roofline(ctx, 256, 20)
     vc = sim.a["voxel_counts"].numpy().astype(np.float32)
     b = 0.01 * vc
@@ -301,8 +322,8 @@ def profile_pass(ctx, sim, net):
     src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
-    log("profile pass done: 20 steps, 20 stencil applies, 1 CG solve "
-        f"({r.iters} iterations), 256 MiB copy")
+    log("profile pass done: 20 steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
+        f"20 stencil applies, 1 CG solve ({r.iters} iterations), 256 MiB copy")
 
 
 def main():

@@ -137,10 +137,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sums NV values over the block (256 or 512 threads); every thread receives
+// Sums NV values over the block (up to 1024 threads); every thread receives
 // the total: lane 0's butterfly result of each wave, waves added in order.
 template <int NV> __device__ __forceinline__ void block_sum_all(double (&v)[NV]) {
-    __shared__ double sh[8][NV];
+    __shared__ double sh[16][NV];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
 #pragma unroll
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 }
 
 // Constant 7-point stencil, 2.5D LDS tiling with four voxels per thread.
-// A block of 512 threads owns a 256 (x) x 8 (y) tile and marches kZc planes
+// A block of 1024 threads owns a 256 (x) x 16 (y) tile and marches kZ4 planes
 // in z; thread (lx, ly) holds x = 4lx .. 4lx+3 of row ly as a float4.  Each
 // plane (plus its one-voxel halo in x and y) goes into a double-buffered LDS
 // tile; the column's z neighbours stay in registers; the next plane's loads
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 //   FUSED = false: y = A x (summation order of the CPU restatement).
 //   FUSED = true : CG iteration k - p_new = dinv r + beta p_old at every
 //                  point, q = A p_new, p_new stored, pq partials.
-constexpr int kTX4 = 256, kTY4 = 8, kZ4 = 16, kT4 = 512, kRow4 = kTX4 + 8;
+constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
 template <bool FUSED>
 __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int nparts,
                                                     const float* __restrict__ src,
@@ -313,21 +313,19 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
     const int ntiles = tx * ty * tz;
     const int64_t sxy = (int64_t)nx * ny;
     const int tt = threadIdx.x, lx = tt & 63, ly = tt >> 6;
-    // halo roles: 0..63 row y=-1, 64..127 row y=kTY4 (float4 each), 128..135
-    // column x=-1, 136..143 column x=kTX4 (one voxel each)
-    const bool hrow = tt < 128, hcol = tt >= 128 && tt < 144;
-    const int hy = tt < 64 ? -1 : tt < 128 ? kTY4 : (tt < 136 ? tt - 128 : tt - 136);
+    // halo roles: 0..63 row y=-1, 64..127 row y=kTY4 (float4 each), the next
+    // kTY4 threads column x=-1, the kTY4 after them column x=kTX4 (one voxel each)
+    const bool hrow = tt < 128, hcol = tt >= 128 && tt < 128 + 2 * kTY4;
+    const int hy = tt < 64 ? -1 : tt < 128 ? kTY4 : (tt < 128 + kTY4 ? tt - 128 : tt - 128 - kTY4);
     const int hl = tt < 64 ? tt : tt - 64;
-    const int hslot = tt < 136 ? 3 : 4 + kTX4;
+    const int hslot = tt < 128 + kTY4 ? 3 : 4 + kTX4;
     const float cdiag = g.diag, cc = g.c;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     double v[1] = {0};
-    // workgroups go to the 8 XCDs round-robin (block b on XCD b % 8): give
-    // each XCD a contiguous run of tiles so neighbouring tiles' halo rows and
-    // planes are read through the same L2
+    // one tile per workgroup when the grid covers them (the usual case);
+    // a grid-stride loop otherwise
     const int nb = gridDim.x;
-    const int lb = (nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
-    for (int tile = lb; tile < ntiles; tile += nb) {
+    for (int tile = blockIdx.x; tile < ntiles; tile += nb) {
         const int bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
         const int x0 = bx * kTX4, k0 = bz * kZ4, kend = min(k0 + kZ4, nz);
         const int j = by * kTY4 + ly;
@@ -335,7 +333,7 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
         const bool own = xo < nx && j < ny;
         const int gj = by * kTY4 + hy;
         const bool hr_ok = hrow && gj >= 0 && gj < ny && x0 + 4 * hl < nx;
-        const int hx = tt < 136 ? x0 - 1 : x0 + kTX4;
+        const int hx = tt < 128 + kTY4 ? x0 - 1 : x0 + kTX4;
         const bool hc_ok = hcol && gj < ny && hx >= 0 && hx < nx;
         const int64_t ocol = xo + (int64_t)nx * j;
         const int64_t rcol = x0 + 4 * hl + (int64_t)nx * gj;
@@ -357,7 +355,7 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
         *reinterpret_cast<float4*>(&t[0][ly + 1][4 + 4 * lx]) = pc;
         if (hrow) *reinterpret_cast<float4*>(&t[0][hy + 1][4 + 4 * hl]) = val4(hr_ok, rcol + sxy * k0);
         if (hcol) t[0][hy + 1][hslot] = val1(hc_ok, ccol + sxy * k0);
-        for (int kz = k0; kz < kend; ++kz) {
+        auto plane = [&](int kz) {
             const int b = (kz - k0) & 1;
             const bool nxt = kz + 1 < nz;
             const float4 pp = val4(own && nxt, ocol + sxy * (kz + 1));
@@ -388,6 +386,15 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
             if (hcol) t[b ^ 1][hy + 1][hslot] = hc;
             pm = pc;
             pc = pp;
+        };
+        // a full tile's march has a compile-time trip count, so it unrolls
+        // and the next plane's loads overlap the current one (a fifth of the
+        // bandwidth otherwise)
+        if (kend - k0 == kZ4) {
+#pragma unroll
+            for (int q = 0; q < kZ4; ++q) plane(k0 + q);
+        } else {
+            for (int kz = k0; kz < kend; ++kz) plane(kz);
         }
         __syncthreads();  // the next tile reuses both buffers
     }
@@ -399,6 +406,7 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
 
 static int stencil4_blocks(const GridK& g) {
     const int64_t nt = ceil_div(g.nx, kTX4) * ceil_div(g.ny, kTY4) * ceil_div(g.nz, kZ4);
+    // at most 2048: the fused kernel's partials region holds 2048 entries
     return (int)std::max<int64_t>(1, std::min<int64_t>(nt, 2048));
 }
 
