@@ -290,6 +290,12 @@ def profile_pass(ctx, sim, net):
     from neuronephys_clean_amd.sim import EnsembleSim
     sim.run(20)  # eager: one dispatch per kernel per step
     sim.synchronize()
+    if sim.plan.couple_mode == _lib.COUPLE_WEIGHTED_FUSED:
+        # and 20 steps of the unfused event path (push, HH + gather, compaction)
+        sim.plan.couple_mode = _lib.COUPLE_WEIGHTED_EVENT
+        sim.run(20)
+        sim.synchronize()
+        sim.plan.couple_mode = _lib.COUPLE_WEIGHTED_FUSED
     # the HH kernel alone (the roofline measurement of the benchmark)
     st, p = sim.plan.state, sim.params
     for _ in range(20):
@@ -322,8 +328,36 @@ def profile_pass(ctx, sim, net):
     src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
-    log("profile pass done: 20 steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
+    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
         f"20 stencil applies, 1 CG solve ({r.iters} iterations), 256 MiB copy")
+
+
+def tune_fused(ctx, sim, chunk):
+    """Picks the fused step's launch shape by timing whole graph-replayed
+    chunks of real steps (untimed by the benchmark; results do not depend on
+    the shape)."""
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ctx.call("nnx_event_create", C.byref(e0))
+    ctx.call("nnx_event_create", C.byref(e1))
+    best, steps = None, 0
+    for block in (256, 512):
+        for tile in (512, 1024, 2048):
+            for grid in (1024, 2048):
+                sim.set_fused(block, tile, grid)
+                sim.run(chunk, use_graph=True, chunk=chunk)  # capture
+                ctx.call("nnx_event_record", e0)
+                sim.run(2 * chunk, use_graph=True, chunk=chunk)
+                ctx.call("nnx_event_record", e1)
+                sim.synchronize()
+                ms = C.c_float()
+                ctx.call("nnx_event_elapsed", e0, e1, C.byref(ms))
+                us = ms.value * 1e3 / (2 * chunk)
+                steps += 3 * chunk
+                if best is None or us < best["us_per_step"]:
+                    best = {"block": block, "tile": tile, "grid": grid, "us_per_step": us}
+    sim.set_fused(best["block"], best["tile"], best["grid"])
+    best["tune_steps"] = steps
+    return best
 
 
 def main():
@@ -333,7 +367,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--couple", default="event", choices=["event", "pull"])
+    ap.add_argument("--couple", default="fused", choices=["fused", "event", "pull"])
     ap.add_argument("--cpu-steps", type=int, default=1500)
     ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
                     help="profile: a short eager sequence for rocprofv3 --pmc passes (20 steps, "
@@ -371,8 +405,12 @@ def main():
 
     sim.run(args.warmup, use_graph=True, chunk=args.chunk)
     sim.synchronize()
+    log(f"bench: warmup {args.warmup} steps done, {sim.raster()[1]} spikes")
+    tune = None
+    if args.couple == "fused":
+        tune = tune_fused(ctx, sim, args.chunk)
+        log(f"bench: fused launch shape {tune}")
     len0 = sim.raster()[1]
-    log(f"bench: warmup {args.warmup} steps done, {len0} spikes")
     e0, e1 = C.c_void_p(), C.c_void_p()
     ctx.call("nnx_event_create", C.byref(e0))
     ctx.call("nnx_event_create", C.byref(e1))
@@ -391,8 +429,22 @@ def main():
     log(f"bench: {args.steps} steps in {wall * 1e3:.1f} ms (events {ev_ms.value:.1f} ms), "
         f"{spikes} spikes, {value:.3e} neuron-steps/s")
 
-    # dominant kernel: the HH RK4 step (one launch per step), timed alone on
-    # the library stream over the same state
+    # the fused step kernel is the only per-step launch: its average duration
+    # is the timed region's event time over the steps (an upper bound: the
+    # per-call list push and compaction are included)
+    fused_roof = None
+    if args.couple == "fused":
+        ev_per_step = spikes / args.steps * net.nnz / net.n
+        fb = 48.125 * net.n + 24.0 * ev_per_step
+        us = ev_ms.value * 1e3 / args.steps
+        fused_roof = {"kernel": "k_ens_step<%d> (fused gather + RK4 + push)" % tune["block"], "bound": "hbm",
+                      "achieved": fb / (us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                      "frac": fb / (us * 1e-6) / HBM_PEAK, "traffic": None, "avg_launch_us": us,
+                      "alg_bytes_per_launch": fb,
+                      "alg_bytes_rule": "48.125 B per neuron + 24 B per delivered event"}
+
+    # the HH RK4 kernel of the unfused path, timed alone on the library
+    # stream over the same state
     st = sim.plan.state
     bits = sim.a["bits0"].ptr
     gin = sim.a["g_in"].ptr
@@ -403,14 +455,15 @@ def main():
                  net.dt, 0, gin, bits)
     hh_ms = timed_events(ctx, hh, 200)
     hh_bytes = 48.125 * net.n
-    roof = {"kernel": "k_hh_rk4<float>", "bound": "hbm", "achieved": hh_bytes / (hh_ms * 1e-3) / 1e9,
-            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hh_bytes / (hh_ms * 1e-3) / HBM_PEAK,
-            "traffic": None, "avg_launch_us": hh_ms * 1e3, "alg_bytes_per_launch": hh_bytes}
+    roof_hh = {"kernel": "k_hh_rk4<float>", "bound": "hbm", "achieved": hh_bytes / (hh_ms * 1e-3) / 1e9,
+               "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hh_bytes / (hh_ms * 1e-3) / HBM_PEAK,
+               "traffic": None, "avg_launch_us": hh_ms * 1e3, "alg_bytes_per_launch": hh_bytes}
+    roof = fused_roof or roof_hh
 
     # per-step share of the coupling: eager per-kernel timing of one step's
     # coupling on the current spikes
     def couple():
-        ctx.call("nnx_couple", C.byref(sim.graph), 1 if args.couple == "event" else 0, _lib.F32,
+        ctx.call("nnx_couple", C.byref(sim.graph), 0 if args.couple == "pull" else 1, _lib.F32,
                  sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
     couple_ms = timed_events(ctx, couple, 50)
 
@@ -455,10 +508,11 @@ def main():
            "config": {"workload": "c2: 1,048,576 squid HH neurons, RK4 dt 0.01 ms, ~210M-edge "
                                   "exp(-d/10) CSR, log-normal weights, event-driven deterministic "
                                   "coupling, release deposit on 256^3",
-                      "n": net.n, "nnz": net.nnz, "dt": net.dt, "couple": args.couple,
+                      "n": net.n, "nnz": net.nnz, "dt": net.dt, "couple": args.couple, "fused_shape": tune,
                       "graph_chunk": args.chunk, "spikes_timed": int(spikes),
                       "spikes_per_step": spikes / args.steps},
            "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
+           "roofline_hh_unfused": roof_hh,
            "c3": c3, "c4": c4, "c5": c5,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},

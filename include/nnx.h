@@ -60,9 +60,13 @@ enum { NNX_SPIKE_LEVEL = 0, NNX_SPIKE_CROSSING = 1 };
  *                    the out-edges of spiking sources (event-driven).
  *   COUNT          : +1 per spiking source (src/coupling.cu:34-48).
  *   SIGNAL         : +output[source] per spiking source
- *                    (src_wholebrain/coupling.cu:12-33).                    */
+ *                    (src_wholebrain/coupling.cu:12-33).
+ *   WEIGHTED_FUSED : (nnx_run_steps only; fp32 RK4 with crossing spikes)
+ *                    the WEIGHTED_EVENT sum, bit-identical, computed inside
+ *                    one persistent launch per step that also performs the
+ *                    RK4 update and pushes the new spikes to their targets. */
 enum { NNX_COUPLE_WEIGHTED_PULL = 0, NNX_COUPLE_WEIGHTED_EVENT = 1,
-       NNX_COUPLE_COUNT = 2, NNX_COUPLE_SIGNAL = 3 };
+       NNX_COUPLE_COUNT = 2, NNX_COUPLE_SIGNAL = 3, NNX_COUPLE_WEIGHTED_FUSED = 4 };
 
 /* stencils / time schemes for the paracrine grid */
 enum { NNX_STENCIL_7 = 7, NNX_STENCIL_27 = 27 };
@@ -494,11 +498,22 @@ typedef struct nnx_step_plan {
     int32_t* raster;         /* 2*raster_cap */
     int64_t* raster_len;
     int64_t raster_cap;
+    /* WEIGHTED_FUSED launch shape (0 = default): workgroup size (64, 128,
+     * 256 or 512), tile (neurons per work item, a multiple of 64), grid
+     * (workgroups); and the burst threshold: when a step fires more than
+     * `fused_burst` neurons (0 = max(1024, n/16), < 0 = never) the next step
+     * pulls every target's in-edges instead of reading its event bucket.
+     * None of these changes any result.                                     */
+    int32_t fused_block, fused_tile, fused_grid, fused_burst;
 } nnx_step_plan;
 
 /* Runs n_steps timesteps starting at global step `first_step`.  With
  * use_graph = 1 the step sequence is captured once into a hipGraph per
- * (plan, n_steps) and replayed; plans must not move between calls.         */
+ * (plan, n_steps) and replayed; plans must not move between calls.
+ * WEIGHTED_FUSED: g_in is not used; each call starts from the previous
+ * step's list/count and leaves the last step's list/count, so calls with
+ * other coupling modes can be interleaved; raster entries of one step are
+ * appended in unspecified order (steps in order).                          */
 nnx_status nnx_run_steps(nnx_ctx* ctx, nnx_step_plan* plan, uint64_t first_step, int n_steps,
                          int use_graph);
 

@@ -16,7 +16,7 @@ HOST_LIB_PATH = os.path.join(_HERE, "libnnx_host.so")
 F32, F64 = 0, 1
 INT_LEAPFROG, INT_RK4 = 0, 1
 SPIKE_LEVEL, SPIKE_CROSSING = 0, 1
-COUPLE_WEIGHTED_PULL, COUPLE_WEIGHTED_EVENT, COUPLE_COUNT, COUPLE_SIGNAL = 0, 1, 2, 3
+COUPLE_WEIGHTED_PULL, COUPLE_WEIGHTED_EVENT, COUPLE_COUNT, COUPLE_SIGNAL, COUPLE_WEIGHTED_FUSED = 0, 1, 2, 3, 4
 STENCIL_7, STENCIL_27 = 7, 27
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_LEARNED = 0, 1, 2
 
@@ -84,7 +84,9 @@ class StepPlan(C.Structure):
                 ("integrator", C.c_int32), ("spike_rule", C.c_int32), ("couple_mode", C.c_int32),
                 ("record", C.c_int32), ("dt", C.c_double), ("g_in", vp), ("bits", vp * 2),
                 ("list", vp), ("count", vp), ("voxel_of", vp), ("voxel_counts", vp),
-                ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64)]
+                ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64),
+                ("fused_block", C.c_int32), ("fused_tile", C.c_int32), ("fused_grid", C.c_int32),
+                ("fused_burst", C.c_int32)]
 
 
 class EnsPlan(C.Structure):

@@ -273,6 +273,16 @@ nnx_status push_event_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list
     return NNX_OK;
 }
 
+nnx_status push_list_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list, const int32_t* count,
+                            int32_t* cnt, int32_t* slot) {
+    void* dn;
+    NNX_TRY(scratch(c, "event_dense", 256, true, &dn));
+    k_push_event<<<2048, 256, 0, c->stream>>>(list, count, g->out_offsets, g->out_targets, g->out_to_in,
+                                              g->in_offsets, cnt, slot, -1, (int32_t*)dn);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
 nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, const uint64_t* bits,
                          const int32_t* list, const int32_t* count, const void* outputs,
                          void* g_in) {
@@ -316,9 +326,10 @@ nnx_status check_graph(nnx_ctx* c, const nnx_graph* g, int mode) {
     NNX_CHECK(c, g->n >= 0 && g->n < (int64_t)1 << 31, "graph n out of range");
     NNX_CHECK(c, g->nnz >= 0 && g->nnz < (int64_t)1 << 31, "graph nnz out of int32 range");
     NNX_CHECK(c, g->in_offsets && (g->in_sources || g->nnz == 0), "graph needs the CSC (in_offsets, in_sources)");
-    if (mode == NNX_COUPLE_WEIGHTED_PULL || mode == NNX_COUPLE_WEIGHTED_EVENT)
+    const bool event = mode == NNX_COUPLE_WEIGHTED_EVENT || mode == NNX_COUPLE_WEIGHTED_FUSED;
+    if (mode == NNX_COUPLE_WEIGHTED_PULL || event)
         NNX_CHECK(c, g->in_weights || g->nnz == 0, "weighted coupling needs in_weights");
-    if (mode == NNX_COUPLE_WEIGHTED_EVENT)
+    if (event)
         NNX_CHECK(c, g->out_offsets && (g->nnz == 0 || (g->out_targets && g->out_to_in)),
                   "event coupling needs out_offsets, out_targets and out_to_in");
     return NNX_OK;

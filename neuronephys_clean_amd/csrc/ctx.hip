@@ -26,6 +26,7 @@ nnx_status scratch(nnx_ctx* c, const char* name, size_t bytes, bool zero, void**
         size_t alloc = bytes < 256 ? 256 : bytes;
         NNX_HIP(c, hipMalloc(&b.p, alloc));
         b.bytes = alloc;
+        b.tag = 0;
         if (zero) NNX_HIP(c, hipMemsetAsync(b.p, 0, alloc, c->stream));
     }
     *out = b.p;

@@ -43,10 +43,19 @@ struct EnsK {
     const int4* tiles;  // (first neuron, end, network, 0)
     int32_t ntiles;
     int64_t* stepctr;   // step number by parity
+    int32_t* spk3;      // spikes of each step, by step mod 3
+    int32_t burst;      // above this many spikes the next step pulls densely (< 0: never)
 };
 
+// par: parity of this step; push = 0 leaves the new spikes unpushed (the
+// caller pushes them from the compacted list before the next step).
+// Burst steps: every wavefront adds its spike count to spk3[step mod 3] and
+// pushes only while the running total stays within `burst`; if the final
+// total exceeds it, some pushes were skipped, and the next step pulls every
+// target over its in-edges with the spike bitmap instead (same sums, same
+// order), clearing the partly filled buckets.
 template <int B>
-__global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int par) {
+__global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int par, int push) {
     const int q = par ^ 1;
     int32_t* __restrict__ cnt_r = q ? d.cnt1 : d.cnt0;  // filled by the previous step
     const int32_t* __restrict__ slot_r = q ? d.slot1 : d.slot0;
@@ -55,7 +64,12 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
     const uint64_t* __restrict__ bits_r = q ? d.bits1 : d.bits0;
     uint64_t* __restrict__ bits_w = par ? d.bits1 : d.bits0;
     const int64_t step = d.stepctr[q] + 1;
-    if (blockIdx.x == 0 && threadIdx.x == 0) d.stepctr[par] = step;
+    const int cur3 = (int)(step % 3);
+    const bool dense = d.burst >= 0 && d.spk3[(cur3 + 2) % 3] > d.burst;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.stepctr[par] = step;
+        d.spk3[(cur3 + 1) % 3] = 0;  // the next step's counter (last read by the previous step)
+    }
     const int lane = threadIdx.x & 63;
     for (int t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         const int4 tl = d.tiles[t];
@@ -65,7 +79,7 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
             // coupling input
             float g = 0.f;
             const int32_t c = live ? cnt_r[i] : 0;
-            bool heavy = c > kSortMax;
+            bool heavy = live && (dense || c > kSortMax);
             if (!heavy && c > 0) {
                 if (c <= d.in_off[i + 1] - d.in_off[i])
                     g = gather_one<float>((int32_t)i, c, d.in_off, d.in_src, d.in_w, bits_r, slot_r);
@@ -95,11 +109,22 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
                 spike = (v0 < p.v_thresh) && (v >= p.v_thresh);
             }
             const uint64_t word = __ballot(spike);
-            if (lane == 0) bits_w[wave0 >> 6] = word;
+            // a wavefront past the tile's end (last chunk shorter than B) has no
+            // word of its own: the next word may belong to another tile or lie
+            // past the bitmap
+            if (lane == 0 && wave0 < tl.y) bits_w[wave0 >> 6] = word;
             if (!word) continue;
+            const int k = __popcll(word);
+            bool do_push = push != 0;
+            if (d.burst >= 0) {
+                int old = 0;
+                if (lane == 0) old = atomicAdd(&d.spk3[cur3], k);
+                old = __shfl(old, 0, 64);
+                do_push = do_push && old + k <= d.burst;
+            }
             // push the new spikes into the next step's buckets, one source at
             // a time across the wavefront
-            uint64_t sm = word;
+            uint64_t sm = do_push ? word : 0ull;
             while (sm) {
                 const int l = __ffsll((unsigned long long)sm) - 1;
                 sm &= sm - 1;
@@ -114,7 +139,6 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
                 }
             }
             if (spike && d.voxel_of) atomicAdd(&d.voxel_counts[d.voxel_of[i]], 1);
-            const int k = __popcll(word);
             if (d.net_spikes && lane == 0) atomicAdd(&d.net_spikes[tl.z], k);
             if (d.raster) {
                 unsigned long long base = 0;
@@ -134,7 +158,17 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
     }
 }
 
-__global__ void k_ens_begin(int64_t* stepctr, int q, int64_t prev) { stepctr[q] = prev; }
+// run set-up: the step counter of the step before the first; fresh = 1
+// (buckets empty or about to be filled by a full push of the previous
+// step's list) also marks the previous step as no burst
+__global__ void k_ens_begin(int64_t* stepctr, int q, int64_t prev, int32_t* spk3, int fresh) {
+    stepctr[q] = prev;
+    if (fresh) {
+        const int p3 = (int)((prev % 3 + 3) % 3);
+        spk3[p3] = 0;
+        spk3[(p3 + 1) % 3] = 0;
+    }
+}
 
 }  // namespace nnx
 
@@ -178,21 +212,29 @@ nnx_status check_cfg(nnx_ctx* c, const nnx_ens_config* f) {
 }
 
 // tiles of `tile` neurons, largest first (ties: network order), in a buffer
-// that belongs to this tile size (captured graphs keep its address)
-nnx_status build_tiles(nnx_ctx* c, const nnx_ens_plan* p, int tile, const int4** out, int* ntiles) {
+// that belongs to this tile size and `kind` (captured graphs keep its
+// address); the upload is skipped when the buffer already holds this list
+nnx_status build_tiles(nnx_ctx* c, const char* kind, int64_t n, int nnet, const int64_t* begin,
+                       const int64_t* size, int tile, const int4** out, int* ntiles) {
     std::vector<int4> t;
-    for (int i = 0; i < p->nnet; ++i)
-        for (int64_t a = p->net_begin[i]; a < p->net_begin[i] + p->net_size[i]; a += tile)
-            t.push_back(make_int4((int)a, (int)std::min<int64_t>(a + tile, p->net_begin[i] + p->net_size[i]), i, 0));
+    for (int i = 0; i < nnet; ++i)
+        for (int64_t a = begin[i]; a < begin[i] + size[i]; a += tile)
+            t.push_back(make_int4((int)a, (int)std::min<int64_t>(a + tile, begin[i] + size[i]), i, 0));
     std::stable_sort(t.begin(), t.end(), [](const int4& a, const int4& b) { return (a.y - a.x) > (b.y - b.x); });
     char name[48];
-    std::snprintf(name, sizeof name, "ens_tiles_%d", tile);
-    const size_t cap = (size_t)(p->state.n / 64 + p->nnet + 1) * sizeof(int4);
+    std::snprintf(name, sizeof name, "%s_tiles_%d", kind, tile);
+    const size_t cap = (size_t)(n / 64 + nnet + 1) * sizeof(int4);
     void* buf;
     NNX_TRY(scratch(c, name, cap, false, &buf));
-    if (!t.empty())
+    uint64_t h = 0x7469006c65730000ull ^ t.size();
+    for (const int4& v : t) h = (h ^ ((uint64_t)(uint32_t)v.x << 32 | (uint32_t)v.y)) * 1099511628211ull;
+    h |= 1;  // never 0 (0 marks an empty buffer)
+    nnx_ctx::Buf& b = c->bufs[name];
+    if (b.tag != h && !t.empty()) {
         NNX_HIP(c, hipMemcpyAsync(buf, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
-    NNX_HIP(c, hipStreamSynchronize(c->stream));  // t is a host temporary
+        NNX_HIP(c, hipStreamSynchronize(c->stream));  // t is a host temporary
+    }
+    b.tag = h;
     *out = (const int4*)buf;
     *ntiles = (int)t.size();
     return NNX_OK;
@@ -203,35 +245,42 @@ struct EnsHost {
     HHConst<float> hc;
 };
 
-nnx_status ens_prepare(nnx_ctx* c, const nnx_ens_plan* p, EnsHost* h) {
-    const int64_t n = p->state.n, nnz = p->graph.nnz;
+// kernel arguments shared by the ensemble and the fused single-network step;
+// `kind` names the bucket, counter and tile buffers of each
+nnx_status prepare(nnx_ctx* c, const char* kind, const nnx_hh_state& s, const nnx_hh_params& prm,
+                   const nnx_graph& g, double dt, uint64_t* const bits[2], EnsHost* h) {
+    const int64_t n = s.n, nnz = g.nnz;
+    char nc[48], ns[48], nt[48];
+    std::snprintf(nc, sizeof nc, "%s_cnt", kind);
+    std::snprintf(ns, sizeof ns, "%s_slot", kind);
+    std::snprintf(nt, sizeof nt, "%s_ctr", kind);
     void *cnt, *slot, *ctr;
-    NNX_TRY(scratch(c, "ens_cnt", (size_t)2 * n * 4, true, &cnt));
-    NNX_TRY(scratch(c, "ens_slot", (size_t)2 * std::max<int64_t>(nnz, 1) * 4, false, &slot));
-    NNX_TRY(scratch(c, "ens_ctr", 64, true, &ctr));
+    NNX_TRY(scratch(c, nc, (size_t)2 * n * 4, true, &cnt));
+    NNX_TRY(scratch(c, ns, (size_t)2 * std::max<int64_t>(nnz, 1) * 4, false, &slot));
+    NNX_TRY(scratch(c, nt, 64, true, &ctr));
     EnsK& k = h->k;
     k = EnsK{};
-    const nnx_hh_state& s = p->state;
     k.V = (float*)s.v, k.M = (float*)s.m, k.H = (float*)s.h, k.NG = (float*)s.ng, k.S = (float*)s.s;
     k.iext = (const float*)s.i_ext;
-    k.out_off = p->graph.out_offsets, k.out_tgt = p->graph.out_targets, k.out_to_in = p->graph.out_to_in;
-    k.in_off = p->graph.in_offsets, k.in_src = p->graph.in_sources, k.in_w = p->graph.in_weights;
+    k.out_off = g.out_offsets, k.out_tgt = g.out_targets, k.out_to_in = g.out_to_in;
+    k.in_off = g.in_offsets, k.in_src = g.in_sources, k.in_w = g.in_weights;
     k.cnt0 = (int32_t*)cnt, k.cnt1 = (int32_t*)cnt + n;
     k.slot0 = (int32_t*)slot, k.slot1 = (int32_t*)slot + std::max<int64_t>(nnz, 1);
-    k.bits0 = p->bits[0], k.bits1 = p->bits[1];
-    k.voxel_of = p->voxel_of, k.voxel_counts = p->voxel_counts, k.net_spikes = p->net_spikes;
-    k.raster = p->raster, k.raster_len = (unsigned long long*)p->raster_len, k.raster_cap = p->raster_cap;
-    k.stepctr = (int64_t*)ctr;
-    h->hc = make_const<float>(&p->params, p->dt);
+    k.bits0 = bits[0], k.bits1 = bits[1];
+    k.stepctr = (int64_t*)ctr;                       // bytes 0..15
+    k.spk3 = (int32_t*)((char*)ctr + 16);            // bytes 16..27
+    k.burst = (int32_t)std::max<int64_t>(1024, n / 16);
+    h->hc = make_const<float>(&prm, dt);
     return NNX_OK;
 }
 
-nnx_status ens_launch(nnx_ctx* c, const EnsK& k, const HHConst<float>& hc, const nnx_ens_config& f, int par) {
+nnx_status ens_launch(nnx_ctx* c, const EnsK& k, const HHConst<float>& hc, const nnx_ens_config& f, int par,
+                      int push) {
     switch (f.block) {
-        case 64: k_ens_step<64><<<f.grid, 64, 0, c->stream>>>(k, hc, par); break;
-        case 128: k_ens_step<128><<<f.grid, 128, 0, c->stream>>>(k, hc, par); break;
-        case 256: k_ens_step<256><<<f.grid, 256, 0, c->stream>>>(k, hc, par); break;
-        default: k_ens_step<512><<<f.grid, 512, 0, c->stream>>>(k, hc, par); break;
+        case 64: k_ens_step<64><<<f.grid, 64, 0, c->stream>>>(k, hc, par, push); break;
+        case 128: k_ens_step<128><<<f.grid, 128, 0, c->stream>>>(k, hc, par, push); break;
+        case 256: k_ens_step<256><<<f.grid, 256, 0, c->stream>>>(k, hc, par, push); break;
+        default: k_ens_step<512><<<f.grid, 512, 0, c->stream>>>(k, hc, par, push); break;
     }
     NNX_LAUNCHED(c);
     return NNX_OK;
@@ -251,44 +300,18 @@ uint64_t ens_key(const nnx_ens_plan* p, const nnx_ens_config& f) {
 }
 
 const nnx_ens_config kDefaultCfg = {256, 2048, 1024, 0};
+const nnx_ens_config kFusedDefault = {256, 1024, 1024, 0};
 
-}  // namespace
-
-extern "C" {
-
-nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_config* cfg, uint64_t first_step,
-                            int n_steps, int use_graph) {
-    if (!c) return NNX_ERR_ARG;
-    NNX_TRY(check_ens(c, p));
-    const nnx_ens_config f = cfg ? *cfg : kDefaultCfg;
-    NNX_TRY(check_cfg(c, &f));
-    NNX_CHECK(c, n_steps >= 0, "n_steps < 0");
-    if (n_steps == 0) return NNX_OK;
-    EnsHost h;
-    NNX_TRY(ens_prepare(c, p, &h));
-    NNX_TRY(build_tiles(c, p, f.tile, &h.k.tiles, &h.k.ntiles));
-    if (first_step == 0) {
-        // a new run: no pending bucket entries, no previous spikes
-        NNX_HIP(c, hipMemsetAsync(h.k.cnt0, 0, (size_t)2 * p->state.n * 4, c->stream));
-        NNX_HIP(c, hipMemsetAsync(p->bits[0], 0, (size_t)p->state.n / 8, c->stream));
-        NNX_HIP(c, hipMemsetAsync(p->bits[1], 0, (size_t)p->state.n / 8, c->stream));
-    }
-    const int q0 = (int)((first_step + 1) & 1);
-    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1);
-    NNX_LAUNCHED(c);
-    if (!use_graph) {
-        for (int i = 0; i < n_steps; ++i) NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1)));
-        return NNX_OK;
-    }
+// Captures `body` (which enqueues on c->stream) into a graph cached under
+// `key`, or replays the cached one.
+template <class F> nnx_status capture_or_replay(nnx_ctx* c, std::pair<const void*, int> key, F body) {
     NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
-    const std::pair<const void*, int> key((const void*)(uintptr_t)ens_key(p, f), n_steps * 2 + (int)(first_step & 1));
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         nnx_ctx::Graph g;
         NNX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         c->capturing = 1;
-        nnx_status st = NNX_OK;
-        for (int i = 0; i < n_steps && st == NNX_OK; ++i) st = ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1));
+        const nnx_status st = body();
         c->capturing = 0;
         hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
         if (st != NNX_OK) {
@@ -301,6 +324,97 @@ nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_con
     }
     NNX_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
     return NNX_OK;
+}
+
+}  // namespace
+
+// The WEIGHTED_FUSED timestep of nnx_run_steps: one network stepped by the
+// ensemble kernel.  Each call is self-contained: the previous step's spike
+// list is pushed into the buckets first, the last step pushes nothing, and
+// the last step's bitmap is compacted into list/count at the end.
+nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t first_step, int n_steps,
+                                int use_graph) {
+    const nnx_hh_state& s = p->state;
+    NNX_CHECK(c, s.dtype == NNX_F32 && p->integrator == NNX_INT_RK4 && p->spike_rule == NNX_SPIKE_CROSSING,
+              "WEIGHTED_FUSED needs an fp32 RK4 plan with crossing spikes");
+    NNX_CHECK(c, s.v && s.m && s.h && s.ng && s.s && s.i_ext, "state needs v, m, h, ng, s, i_ext");
+    nnx_ens_config f = kFusedDefault;
+    if (p->fused_block) f.block = p->fused_block;
+    if (p->fused_tile) f.tile = p->fused_tile;
+    if (p->fused_grid) f.grid = p->fused_grid;
+    NNX_TRY(check_cfg(c, &f));
+    EnsHost h;
+    uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
+    NNX_TRY(prepare(c, "fstep", s, p->params, p->graph, p->dt, bits, &h));
+    h.k.burst = p->fused_burst == 0 ? h.k.burst : p->fused_burst < 0 ? -1 : p->fused_burst;
+    if (p->voxel_of) h.k.voxel_of = p->voxel_of, h.k.voxel_counts = p->voxel_counts;
+    if (p->record) {
+        h.k.raster = p->raster;
+        h.k.raster_len = (unsigned long long*)p->raster_len;
+        h.k.raster_cap = p->raster_cap;
+    }
+    const int64_t zero = 0;
+    NNX_TRY(build_tiles(c, "fstep", s.n, 1, &zero, &s.n, f.tile, &h.k.tiles, &h.k.ntiles));
+    void* dn;
+    NNX_TRY(scratch(c, "event_dense", 256, true, &dn));  // used by the push; allocated before any capture
+    const int q0 = (int)((first_step + 1) & 1);
+    int32_t* cnt_q0 = q0 ? h.k.cnt1 : h.k.cnt0;
+    int32_t* slot_q0 = q0 ? h.k.slot1 : h.k.slot0;
+    // the step number is set outside the captured sequence (a captured
+    // kernel argument would replay the first capture's number)
+    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, 1);
+    NNX_LAUNCHED(c);
+    auto body = [&]() -> nnx_status {
+        NNX_TRY(push_list_launch(c, &p->graph, p->list, p->count, cnt_q0, slot_q0));
+        for (int i = 0; i < n_steps; ++i)
+            NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1), i + 1 < n_steps));
+        CompactExtras ex{};
+        return compact_launch(c, s.n, p->bits[(first_step + n_steps - 1) & 1], p->list, p->count, ex);
+    };
+    if (!use_graph) return body();
+    uint64_t key = 0xcbf29ce484222325ull;
+    const unsigned char* u = (const unsigned char*)p;
+    for (size_t i = 0; i < sizeof(*p); ++i) key = (key ^ u[i]) * 1099511628211ull;
+    key ^= 0x4655534544535450ull;
+    return capture_or_replay(c, {(const void*)(uintptr_t)key, n_steps * 2 + (int)(first_step & 1)}, body);
+}
+
+extern "C" {
+
+nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_config* cfg, uint64_t first_step,
+                            int n_steps, int use_graph) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_ens(c, p));
+    const nnx_ens_config f = cfg ? *cfg : kDefaultCfg;
+    NNX_TRY(check_cfg(c, &f));
+    NNX_CHECK(c, n_steps >= 0, "n_steps < 0");
+    if (n_steps == 0) return NNX_OK;
+    EnsHost h;
+    uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
+    NNX_TRY(prepare(c, "ens", p->state, p->params, p->graph, p->dt, bits, &h));
+    h.k.voxel_of = p->voxel_of, h.k.voxel_counts = p->voxel_counts, h.k.net_spikes = p->net_spikes;
+    h.k.raster = p->raster, h.k.raster_len = (unsigned long long*)p->raster_len, h.k.raster_cap = p->raster_cap;
+    NNX_TRY(build_tiles(c, "ens", p->state.n, p->nnet, p->net_begin, p->net_size, f.tile, &h.k.tiles,
+                        &h.k.ntiles));
+    const bool fresh = first_step == 0;
+    if (fresh) {
+        // a new run: no pending bucket entries, no previous spikes
+        NNX_HIP(c, hipMemsetAsync(h.k.cnt0, 0, (size_t)2 * p->state.n * 4, c->stream));
+        NNX_HIP(c, hipMemsetAsync(p->bits[0], 0, (size_t)p->state.n / 8, c->stream));
+        NNX_HIP(c, hipMemsetAsync(p->bits[1], 0, (size_t)p->state.n / 8, c->stream));
+    }
+    // a continuing run reads the buckets and burst counter the previous
+    // call's last step left
+    const int q0 = (int)((first_step + 1) & 1);
+    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, fresh ? 1 : 0);
+    NNX_LAUNCHED(c);
+    auto body = [&]() -> nnx_status {
+        for (int i = 0; i < n_steps; ++i) NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1), 1));
+        return NNX_OK;
+    };
+    if (!use_graph) return body();
+    return capture_or_replay(c, {(const void*)(uintptr_t)ens_key(p, f), n_steps * 2 + (int)(first_step & 1)},
+                             body);
 }
 
 nnx_status nnx_ensemble_autotune(nnx_ctx* c, const nnx_ens_plan* p, uint64_t first_step, int steps_per_candidate,

@@ -38,9 +38,10 @@ static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     NNX_TRY(check_graph(c, &p->graph, p->couple_mode));
     NNX_CHECK(c, p->bits[0] && p->bits[1] && p->list && p->count, "plan needs bits[2], list, count");
     if (p->integrator == NNX_INT_RK4) {
-        NNX_CHECK(c, p->g_in, "RK4 plan needs g_in");
-        NNX_CHECK(c, p->couple_mode == NNX_COUPLE_WEIGHTED_PULL || p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT,
+        NNX_CHECK(c, p->couple_mode == NNX_COUPLE_WEIGHTED_PULL || p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT ||
+                         p->couple_mode == NNX_COUPLE_WEIGHTED_FUSED,
                   "RK4 plan needs weighted coupling");
+        NNX_CHECK(c, p->g_in || p->couple_mode == NNX_COUPLE_WEIGHTED_FUSED, "RK4 plan needs g_in");
     } else {
         NNX_CHECK(c, s.in_local, "leapfrog plan writes the coupling into state.in_local");
         NNX_CHECK(c, p->couple_mode == NNX_COUPLE_COUNT || p->couple_mode == NNX_COUPLE_SIGNAL,
@@ -96,6 +97,7 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
     NNX_TRY(check_plan(c, p));
     NNX_CHECK(c, n_steps >= 0, "n_steps < 0");
     if (n_steps == 0 || p->state.n == 0) return NNX_OK;
+    if (p->couple_mode == NNX_COUPLE_WEIGHTED_FUSED) return fused_run_steps(c, p, first_step, n_steps, use_graph);
     // everything a step touches is allocated before any capture
     if (p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT) {
         int32_t *a, *b;

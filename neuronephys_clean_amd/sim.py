@@ -17,7 +17,7 @@ from .configs import Network
 from .devmem import DeviceArray
 
 COUPLE = {"pull": _lib.COUPLE_WEIGHTED_PULL, "event": _lib.COUPLE_WEIGHTED_EVENT,
-          "count": _lib.COUPLE_COUNT, "signal": _lib.COUPLE_SIGNAL}
+          "count": _lib.COUPLE_COUNT, "signal": _lib.COUPLE_SIGNAL, "fused": _lib.COUPLE_WEIGHTED_FUSED}
 
 
 def upload_graph(ctx, net: Network, device_csc: bool = False):
@@ -51,7 +51,9 @@ class Simulation:
 
     def __init__(self, net: Network, dtype=np.float32, couple: str = "event",
                  record_cap: int = 0, deposit: bool = False, params: HHParams | None = None,
-                 ctx: _lib.Context | None = None, device_csc: bool = False):
+                 ctx: _lib.Context | None = None, device_csc: bool = False, fused=None):
+        """couple "fused": one persistent launch per step; `fused` =
+        (block, tile, grid, burst) of that launch (0 = default)."""
         self.ctx = ctx or _lib.context()
         self.net = net
         self.T = np.dtype(dtype)
@@ -100,6 +102,8 @@ class Simulation:
         p.raster = a["raster"].ptr
         p.raster_len = a["raster_len"].ptr
         p.raster_cap = record_cap
+        if fused:
+            p.fused_block, p.fused_tile, p.fused_grid, p.fused_burst = fused
         self.plan = p
         self.step = 0
 
@@ -118,10 +122,20 @@ class Simulation:
     def synchronize(self):
         self.ctx.synchronize()
 
+    def set_fused(self, block=0, tile=0, grid=0, burst=None):
+        """Launch shape of the fused step (results do not depend on it)."""
+        p = self.plan
+        p.fused_block, p.fused_tile, p.fused_grid = block, tile, grid
+        if burst is not None:
+            p.fused_burst = burst
+
     def raster(self) -> np.ndarray:
+        """(step, neuron) rows, ordered by step then neuron."""
         ln = int(self.a["raster_len"].numpy()[0])
-        r = self.a["raster"].numpy().reshape(-1, 2)
-        return r[:min(ln, self.record_cap)], ln
+        r = self.a["raster"].numpy().reshape(-1, 2)[:min(ln, self.record_cap)]
+        if self.plan.couple_mode == _lib.COUPLE_WEIGHTED_FUSED and len(r):
+            r = r[np.lexsort((r[:, 1], r[:, 0]))]
+        return r, ln
 
     def state(self) -> dict:
         return {k: self.a[k].numpy() for k in ("v", "m", "h", "ng", "s")}

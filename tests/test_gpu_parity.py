@@ -371,14 +371,14 @@ def _oracle_rk4(net, T, steps, cap, deposit=True):
     return ras, ln, vc, dict(v=V, m=m, h=h, ng=ng, s=s)
 
 
-@pytest.mark.parametrize("T", [np.float64, np.float32])
-def test_sim_c1_100ms_rasters_bitexact(ctx, T):
+@pytest.mark.parametrize("T,couple", [(np.float64, "event"), (np.float32, "event"), (np.float32, "fused")])
+def test_sim_c1_100ms_rasters_bitexact(ctx, T, couple):
     """C1 (1,024 neurons, 100 out-edges) for 100 ms = 10,000 RK4 steps:
     rasters, final state and release counts identical to the oracle."""
     net = configs.c1()
     steps = 10000
-    say("sim c1", T, steps)
-    sim = Simulation(net, T, couple="event", record_cap=2_000_000, deposit=True)
+    say("sim c1", T, couple, steps)
+    sim = Simulation(net, T, couple=couple, record_cap=2_000_000, deposit=True)
     sim.run(steps, use_graph=True, chunk=500)
     sim.synchronize()
     ras, ln = sim.raster()
@@ -412,7 +412,7 @@ def test_sim_fp32_vs_fp64_report(ctx):
     assert agree > 0.5
 
 
-@pytest.mark.parametrize("couple", ["pull", "event"])
+@pytest.mark.parametrize("couple", ["pull", "event", "fused"])
 def test_graph_replay_equals_eager(ctx, couple):
     net = configs.c1(seed=3, n=777, k=40)
     a = Simulation(net, np.float32, couple=couple, record_cap=100000)
@@ -455,12 +455,13 @@ def test_reference_leapfrog_sim_bitexact(ctx, mode, T):
         assert np.array_equal(got[k], a), k
 
 
-def test_sim_c2_scaled_bitexact(ctx):
+@pytest.mark.parametrize("couple", ["event", "fused"])
+def test_sim_c2_scaled_bitexact(ctx, couple):
     """C2's generator at 1/16 scale (65,536 neurons, ~13M edges,
     distance-dependent targets): 300 steps identical to the oracle."""
     net = configs.c2(n=1 << 16)
-    say("c2 scaled", net.n, net.nnz)
-    sim = Simulation(net, np.float32, couple="event", record_cap=2_000_000, deposit=True)
+    say("c2 scaled", couple, net.n, net.nnz)
+    sim = Simulation(net, np.float32, couple=couple, record_cap=2_000_000, deposit=True)
     sim.run(300, use_graph=True, chunk=100)
     sim.synchronize()
     ras, ln = sim.raster()
@@ -468,3 +469,47 @@ def test_sim_c2_scaled_bitexact(ctx):
     assert ln == rln and np.array_equal(ras, ref)
     assert np.array_equal(sim.state()["v"], st["v"])
     assert np.array_equal(sim.a["voxel_counts"].numpy(), vc)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 7, 3), (512, 4096, 3, 0), (128, 512, 2048, -1), (256, 1024, 64, 40)])
+def test_fused_step_independent_of_shape_and_bursts(ctx, shape):
+    """The fused step on a ragged network (777 neurons): identical to the
+    event path for any launch shape, and when a small burst threshold makes
+    steps fall back to the dense pull (burst 3 and 40 fire on most steps of
+    the initial transient)."""
+    net = configs.c1(seed=3, n=777, k=40)
+    a = Simulation(net, np.float32, couple="event", record_cap=100000, deposit=True)
+    b = Simulation(net, np.float32, couple="fused", record_cap=100000, deposit=True, fused=shape)
+    say("fused shape", shape)
+    a.run(500)
+    b.run(500, use_graph=True, chunk=100)
+    a.synchronize()
+    b.synchronize()
+    ra, rb = a.raster(), b.raster()
+    assert ra[1] == rb[1] and np.array_equal(ra[0], rb[0])
+    for k, v in a.state().items():
+        assert np.array_equal(v, b.state()[k]), k
+    assert np.array_equal(a.a["voxel_counts"].numpy(), b.a["voxel_counts"].numpy())
+    assert a.spike_count_last() == b.spike_count_last()
+    assert np.array_equal(a.a["list"].numpy()[:a.spike_count_last()], b.a["list"].numpy()[:b.spike_count_last()])
+
+
+def test_fused_interleaves_with_event_calls(ctx):
+    """Calls in WEIGHTED_FUSED and WEIGHTED_EVENT mode on one plan can
+    alternate (each hands the next the previous step's spike list)."""
+    net = configs.c1(seed=4, n=1024, k=60)
+    ref = Simulation(net, np.float32, couple="event", record_cap=100000)
+    ref.run(400)
+    mix = Simulation(net, np.float32, couple="event", record_cap=100000)
+    for i, k in enumerate((37, 63, 100, 1, 99, 100)):
+        mix.plan.couple_mode = _lib.COUPLE_WEIGHTED_FUSED if i % 2 == 0 else _lib.COUPLE_WEIGHTED_EVENT
+        mix.run(k, use_graph=(i % 3 == 0))
+    ref.synchronize()
+    mix.synchronize()
+    say("interleaved", mix.step, "steps")
+    for k, v in ref.state().items():
+        assert np.array_equal(v, mix.state()[k]), k
+    r0 = ref.raster()[0]
+    r1 = mix.raster()[0]
+    r1 = r1[np.lexsort((r1[:, 1], r1[:, 0]))]
+    assert np.array_equal(r0, r1)
