@@ -217,12 +217,13 @@ BF16_PEAK = 2.5e15  # dense BF16 MFMA FLOP/s, MI355X
 def c5_bench(ctx, counts, smooth=2):
     """Variable-diffusivity 256^3 solve (log-normal D, log-sigma 1,
     correlation length 8 voxels, mean dt D / h^2 = 50, harmonic-mean faces)
-    with the C2 network's release counts as source: Jacobi-CG against
-    flexible CG with the trained U-Net preconditioner, both inside FP64
-    refinement to a true 1e-8 relative residual."""
+    with the C2 network's release counts as source, every solver inside FP64
+    refinement to a true 1e-8 relative residual: Jacobi-CG (the plain
+    baseline), flexible CG with the U-Net on Jacobi smoothing, with the
+    multigrid V-cycle, and with the U-Net correcting the V-cycle."""
     from neuronephys_clean_amd import _lib, configs
     from neuronephys_clean_amd.devmem import DeviceArray as DA
-    from neuronephys_clean_amd.precond import UNet, device_field
+    from neuronephys_clean_amd.precond import MG_WEIGHTS, UNet, device_field
     n = 256
     t0 = time.time()
     f = configs.c5_field(seed=0, n=n)
@@ -233,24 +234,34 @@ def c5_bench(ctx, counts, smooth=2):
         b[len(b) // 2] = 1.0
     db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, n ** 3, np.float64)
     net = UNet.load(ctx)
+    net_mg = UNet.load(ctx, MG_WEIGHTS) if os.path.exists(MG_WEIGHTS) else None
     out = {"grid": f"{n}^3", "field": "log-normal D, log-sigma 1, corr 8, mean dt D/h^2 50",
            "rtol_true_fp64": 1e-8, "smooth_sweeps": smooth}
-    for name, pc in (("jacobi_cg", None), ("learned_fcg", _lib.PRECOND_LEARNED)):
+    solvers = [("jacobi_cg", None, None), ("learned_fcg", _lib.PRECOND_LEARNED, net),
+               ("mg_fcg", _lib.PRECOND_MG, None)]
+    if net_mg is not None:
+        solvers.append(("mg_learned_fcg", _lib.PRECOND_MG_LEARNED, net_mg))
+    for name, pc, nw in solvers:
         best = None
         for _ in range(2):
             r = _lib.CgResult()
             o = _lib.CgOpts(rtol=1e-8, maxit=20000, precond=_lib.PRECOND_JACOBI if pc is None else pc,
-                            check_every=8, stall_iters=10, smooth=smooth)
+                            check_every=8, stall_iters=10, smooth=smooth if pc == _lib.PRECOND_LEARNED else 1)
             if pc is None:
                 ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-6, 30, C.byref(r))
             else:
-                ctx.call("nnx_fcg_solve_refine", C.byref(g), net.h, keep["feat"].ptr, db.ptr, dx.ptr,
-                         C.byref(o), 1e-6, 30, C.byref(r))
+                ctx.call("nnx_fcg_solve_refine", C.byref(g), nw.h if nw else None, keep["feat"].ptr, db.ptr,
+                         dx.ptr, C.byref(o), 1e-6, 30, C.byref(r))
             if best is None or r.ms < best["ms"]:
                 best = r.as_dict()
-        best["fell_back_to_jacobi"] = pc is not None and best["precond_used"] != _lib.PRECOND_LEARNED
+        if pc in (_lib.PRECOND_LEARNED, _lib.PRECOND_MG_LEARNED):
+            best["fell_back"] = best["precond_used"] != pc
         out[name] = best
-    out["speedup_learned_vs_plain"] = out["jacobi_cg"]["ms"] / out["learned_fcg"]["ms"]
+        log(f"bench: C5 {name} {best}")
+    for name, _, _ in solvers[1:]:
+        out["speedup_" + name] = out["jacobi_cg"]["ms"] / out[name]["ms"]
+    out["speedup_learned_vs_plain"] = max(out["jacobi_cg"]["ms"] / out[k]["ms"]
+                                          for k in ("learned_fcg", "mg_learned_fcg") if k in out)
     # one network application (the preconditioner's dense part) on the MFMA roofline
     r32, z32 = DA.from_numpy(ctx, b.astype(np.float32)), DA.zeros(ctx, n ** 3, np.float32)
 
@@ -260,7 +271,13 @@ def c5_bench(ctx, counts, smooth=2):
     flop = UNET_FLOP_PER_VOXEL * n ** 3
     out["unet_apply"] = {"ms": ms, "flop": flop, "achieved_TFLOPs": flop / (ms * 1e-3) / 1e12,
                          "peak_TFLOPs": BF16_PEAK / 1e12, "mfma_frac": flop / (ms * 1e-3) / BF16_PEAK}
+
+    def f_mg():
+        ctx.call("nnx_mg_apply", C.byref(g), 0, r32.ptr, z32.ptr)
+    out["mg_vcycle_ms"] = timed_events(ctx, f_mg, 10)
     net.close()
+    if net_mg is not None:
+        net_mg.close()
     return out
 
 

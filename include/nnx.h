@@ -70,7 +70,8 @@ enum { NNX_COUPLE_WEIGHTED_PULL = 0, NNX_COUPLE_WEIGHTED_EVENT = 1,
 
 /* stencils / time schemes for the paracrine grid */
 enum { NNX_STENCIL_7 = 7, NNX_STENCIL_27 = 27 };
-enum { NNX_PRECOND_NONE = 0, NNX_PRECOND_JACOBI = 1, NNX_PRECOND_LEARNED = 2 };
+enum { NNX_PRECOND_NONE = 0, NNX_PRECOND_JACOBI = 1, NNX_PRECOND_LEARNED = 2, NNX_PRECOND_MG = 3,
+       NNX_PRECOND_MG_LEARNED = 4 };
 
 /* ------------------------------------------------------------------------ */
 /* context                                                                   */
@@ -278,6 +279,8 @@ typedef struct nnx_cg_opts {
     int32_t use_x0;       /* 1: start from x; 0: start from 0 */
     int32_t stall_iters;  /* flexible CG: fall back to Jacobi after this many non-improving iterations */
     int32_t smooth;       /* learned preconditioner: damped-Jacobi sweeps after the network (0: 1) */
+    int32_t mg_nu;        /* multigrid: damped-Jacobi sweeps before and after each level (0: 4) */
+    int32_t reserved_;
 } nnx_cg_opts;
 
 typedef struct nnx_cg_result {
@@ -456,16 +459,29 @@ nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny,
                                 const uint16_t* d_in, const uint16_t* d_coarse, double coarse_scale,
                                 const uint16_t* d_g, float* d_dw, float* d_db);
 
+/* z = V(r): one multigrid V-cycle from a zero initial guess for the
+ * variable-coefficient 7-point operator of g (kx, ky, kz set).  Cell-centred
+ * 2x2x2 coarsening while every dimension is even and >= 8; a coarse face
+ * carries 1/16 of the sum of the 4 fine faces it covers; block-mean
+ * restriction of the residual; trilinear prolongation (zero beyond the
+ * boundary); nu damped-Jacobi sweeps (w = 0.8) before and after each level
+ * (nu = 0: 4), 16 on the coarsest.  No counterpart in the reference.        */
+nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, const float* d_r, float* d_z);
+
 /* Flexible CG (Notay, one stored direction): fp32 vectors, FP64 reductions,
  * precond LEARNED: a two-level preconditioner with the network N as the
  * coarse correction between damped-Jacobi smoothings (w = 2/3, D the
  * operator's diagonal): z0 = w D^-1 r, z1 = z0 + N(r - A z0), then
  * o->smooth sweeps z <- z + w D^-1 (r - A z) (at least one); d_feat is N's
  * feature input.
+ * precond MG: one multigrid V-cycle (nnx_mg_apply with o->mg_nu).
+ * precond MG_LEARNED: the V-cycle in place of z0 above: z0 = V(r),
+ * z1 = z0 + N(r - A z0), then o->smooth damped-Jacobi sweeps.
  * precond JACOBI: M = D^-1;
  * after o->stall_iters (default 10) iterations without a new best
- * residual the learned preconditioner is replaced by Jacobi (res reports
- * the preconditioner of the final iterations).                              */
+ * residual a learned preconditioner is replaced by its base (Jacobi or the
+ * V-cycle; res reports the preconditioner of the final iterations).
+ * MG and MG_LEARNED need the variable-coefficient grid (kx, ky, kz).        */
 nnx_status nnx_fcg_solve(nnx_ctx* ctx, const nnx_grid* g, nnx_unet* net, const float* d_feat,
                          const float* d_b, float* d_x, const nnx_cg_opts* o, nnx_cg_result* res);
 /* FP64 iterative refinement around nnx_fcg_solve (as nnx_cg_solve_refine).  */

@@ -273,16 +273,6 @@ nnx_status push_event_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list
     return NNX_OK;
 }
 
-nnx_status push_list_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list, const int32_t* count,
-                            int32_t* cnt, int32_t* slot) {
-    void* dn;
-    NNX_TRY(scratch(c, "event_dense", 256, true, &dn));
-    k_push_event<<<2048, 256, 0, c->stream>>>(list, count, g->out_offsets, g->out_targets, g->out_to_in,
-                                              g->in_offsets, cnt, slot, -1, (int32_t*)dn);
-    NNX_LAUNCHED(c);
-    return NNX_OK;
-}
-
 nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, const uint64_t* bits,
                          const int32_t* list, const int32_t* count, const void* outputs,
                          void* g_in) {

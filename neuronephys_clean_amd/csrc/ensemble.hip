@@ -7,13 +7,19 @@
 // is one launch of a fixed grid of workgroups that walk a static list of
 // tiles (a network's neurons cut into pieces of `tile`, largest first,
 // dealt round-robin).  For each tile: the event-driven coupling input of
-// the previous step's spikes (bucket entries summed in ascending source
-// order), the RK4 update, the spike word, and the push of the new spikes
-// into the next step's buckets (buckets and spike words ping-pong by step
-// parity, so no workgroup waits for another within a step).  Because every
-// sum has a fixed order, the trajectory is bit-identical for every
-// (block, tile, grid) configuration; the autotuner uses this to time
-// candidate configurations on real steps.
+// the previous step's spikes, the RK4 update, the spike word, and the push
+// of the new spikes for the next step (event storage and spike words
+// ping-pong by step parity, so no workgroup waits for another within a
+// step).
+//
+// Event storage is one bit per in-edge, at the edge's CSC position, plus a
+// per-target flag: a push sets the bit (atomic OR, no return value) and the
+// flag (plain store), so it never waits on memory.  A flagged target reads
+// the bit words of its CSC range and adds the weights of the set bits in
+// ascending position - ascending source - order, clearing the bits as it
+// goes.  Because every sum has a fixed order, the trajectory is
+// bit-identical for every (block, tile, grid) configuration; the autotuner
+// uses this to time candidate configurations on real steps.
 //
 // No counterpart in the reference, which simulates one network
 // (src/main.cu:112-182).
@@ -32,7 +38,8 @@ struct EnsK {
     const float* iext;
     const int32_t *out_off, *out_tgt, *out_to_in, *in_off, *in_src;
     const float* in_w;
-    int32_t *cnt0, *cnt1, *slot0, *slot1;
+    int32_t *flag0, *flag1;  // per target: events pending
+    uint32_t *eb0, *eb1;     // per in-edge (CSC position): event bit
     uint64_t *bits0, *bits1;
     const int32_t* voxel_of;
     int32_t* voxel_counts;
@@ -47,20 +54,72 @@ struct EnsK {
     int32_t burst;      // above this many spikes the next step pulls densely (< 0: never)
 };
 
+// Sum of the pending events of the target whose CSC range is [e0, e1)
+// (e1 > e0): weights of the set bits of eb in that range, in ascending
+// position.  The bits are cleared: words that hold only this target's edges
+// by a store, the first and last word (which may hold a neighbour's edges,
+// read and cleared concurrently) by an atomic AND of the bits read.
+__device__ __forceinline__ float gather_bits(int32_t e0, int32_t e1, uint32_t* __restrict__ eb,
+                                             const float* __restrict__ in_w) {
+    float acc = 0.f;
+    const int32_t wf = e0 >> 5, wl = (e1 - 1) >> 5;
+    const uint32_t mf = ~0u << (e0 & 31);
+    const uint32_t ml = 0xffffffffu >> (31 - ((e1 - 1) & 31));
+    for (int32_t w0 = wf; w0 <= wl; w0 += 8) {
+        uint32_t m[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m[k] = (w0 + k <= wl) ? eb[w0 + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t w = w0 + k;
+            uint32_t x = m[k];
+            if (w == wf) x &= mf;
+            if (w == wl) x &= ml;
+            if (x) {
+                if (w == wf || w == wl)
+                    atomicAnd(&eb[w], ~x);
+                else
+                    eb[w] = 0u;
+                do {
+                    const int b = __ffs(x) - 1;
+                    x &= x - 1;
+                    acc = acc + in_w[w * 32 + b];
+                } while (x);
+            }
+        }
+    }
+    return acc;
+}
+
+// Clears the event bits of the CSC range [e0, e1) (e1 > e0) without
+// reading them (a dense step recomputes the sums from the spike bitmap).
+__device__ __forceinline__ void clear_bits(int32_t e0, int32_t e1, uint32_t* __restrict__ eb) {
+    const int32_t wf = e0 >> 5, wl = (e1 - 1) >> 5;
+    const uint32_t mf = ~0u << (e0 & 31);
+    const uint32_t ml = 0xffffffffu >> (31 - ((e1 - 1) & 31));
+    if (wf == wl) {
+        atomicAnd(&eb[wf], ~(mf & ml));
+        return;
+    }
+    atomicAnd(&eb[wf], ~mf);
+    for (int32_t w = wf + 1; w < wl; ++w) eb[w] = 0u;
+    atomicAnd(&eb[wl], ~ml);
+}
+
 // par: parity of this step; push = 0 leaves the new spikes unpushed (the
 // caller pushes them from the compacted list before the next step).
 // Burst steps: every wavefront adds its spike count to spk3[step mod 3] and
 // pushes only while the running total stays within `burst`; if the final
 // total exceeds it, some pushes were skipped, and the next step pulls every
 // target over its in-edges with the spike bitmap instead (same sums, same
-// order), clearing the partly filled buckets.
+// order), clearing the partly set event bits.
 template <int B>
 __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int par, int push) {
     const int q = par ^ 1;
-    int32_t* __restrict__ cnt_r = q ? d.cnt1 : d.cnt0;  // filled by the previous step
-    const int32_t* __restrict__ slot_r = q ? d.slot1 : d.slot0;
-    int32_t* __restrict__ cnt_w = par ? d.cnt1 : d.cnt0;
-    int32_t* __restrict__ slot_w = par ? d.slot1 : d.slot0;
+    int32_t* __restrict__ flag_r = q ? d.flag1 : d.flag0;  // set by the previous step
+    uint32_t* __restrict__ eb_r = q ? d.eb1 : d.eb0;
+    int32_t* __restrict__ flag_w = par ? d.flag1 : d.flag0;
+    uint32_t* __restrict__ eb_w = par ? d.eb1 : d.eb0;
     const uint64_t* __restrict__ bits_r = q ? d.bits1 : d.bits0;
     uint64_t* __restrict__ bits_w = par ? d.bits1 : d.bits0;
     const int64_t step = d.stepctr[q] + 1;
@@ -73,28 +132,30 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
     const int lane = threadIdx.x & 63;
     for (int t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
         const int4 tl = d.tiles[t];
-        for (int64_t i0 = tl.x; i0 < tl.y; i0 += B) {
-            const int64_t i = i0 + threadIdx.x;
+        for (int32_t i0 = tl.x; i0 < tl.y; i0 += B) {
+            const int32_t i = i0 + (int32_t)threadIdx.x;
             const bool live = i < tl.y;
             // coupling input
             float g = 0.f;
-            const int32_t c = live ? cnt_r[i] : 0;
-            bool heavy = live && (dense || c > kSortMax);
-            if (!heavy && c > 0) {
-                if (c <= d.in_off[i + 1] - d.in_off[i])
-                    g = gather_one<float>((int32_t)i, c, d.in_off, d.in_src, d.in_w, bits_r, slot_r);
-                else
-                    heavy = true;
+            const int32_t wave0 = i - lane;
+            if (!dense) {
+                if (live && flag_r[i]) {
+                    g = gather_bits(d.in_off[i], d.in_off[i + 1], eb_r, d.in_w);
+                    flag_r[i] = 0;
+                }
+            } else {
+                if (live && flag_r[i]) {
+                    clear_bits(d.in_off[i], d.in_off[i + 1], eb_r);
+                    flag_r[i] = 0;
+                }
+                uint64_t hm = __ballot(live);
+                while (hm) {
+                    const int l = __ffsll((unsigned long long)hm) - 1;
+                    hm &= hm - 1;
+                    const float acc = wave_pull_one<float>(wave0 + l, d.in_off, d.in_src, d.in_w, bits_r);
+                    if (lane == l) g = acc;
+                }
             }
-            uint64_t hm = __ballot(heavy);
-            const int64_t wave0 = i - lane;
-            while (hm) {
-                const int l = __ffsll((unsigned long long)hm) - 1;
-                hm &= hm - 1;
-                const float acc = wave_pull_one<float>(wave0 + l, d.in_off, d.in_src, d.in_w, bits_r);
-                if (lane == l) g = acc;
-            }
-            if (c) cnt_r[i] = 0;
             // RK4 update and spike
             bool spike = false;
             if (live) {
@@ -128,14 +189,12 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
             while (sm) {
                 const int l = __ffsll((unsigned long long)sm) - 1;
                 sm &= sm - 1;
-                const int64_t j = wave0 + l;
+                const int32_t j = wave0 + l;
                 const int32_t e1 = d.out_off[j + 1];
                 for (int32_t e = d.out_off[j] + lane; e < e1; e += 64) {
-                    const int32_t tt = d.out_tgt[e];
                     const int32_t pos = d.out_to_in[e];
-                    const int32_t qq = atomicAdd(&cnt_w[tt], 1);
-                    const int32_t b = d.in_off[tt];
-                    if (qq < d.in_off[tt + 1] - b) slot_w[b + qq] = pos;
+                    atomicOr(&eb_w[pos >> 5], 1u << (pos & 31));
+                    flag_w[d.out_tgt[e]] = 1;
                 }
             }
             if (spike && d.voxel_of) atomicAdd(&d.voxel_counts[d.voxel_of[i]], 1);
@@ -154,6 +213,27 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
                     }
                 }
             }
+        }
+    }
+}
+
+// events of every spike of `list` (all `count` of them), one wavefront per
+// source
+__global__ __launch_bounds__(256) void k_push_bits(const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+                                                   const int32_t* __restrict__ out_off,
+                                                   const int32_t* __restrict__ out_tgt,
+                                                   const int32_t* __restrict__ out_to_in, uint32_t* __restrict__ eb,
+                                                   int32_t* __restrict__ flag) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    const int32_t K = *count;
+    for (int64_t k = blockIdx.x * 4ll + (threadIdx.x >> 6); k < K; k += nwaves) {
+        const int32_t j = list[k];
+        const int32_t e1 = out_off[j + 1];
+        for (int32_t e = out_off[j] + lane; e < e1; e += 64) {
+            const int32_t pos = out_to_in[e];
+            atomicOr(&eb[pos >> 5], 1u << (pos & 31));
+            flag[out_tgt[e]] = 1;
         }
     }
 }
@@ -243,6 +323,7 @@ nnx_status build_tiles(nnx_ctx* c, const char* kind, int64_t n, int nnet, const 
 struct EnsHost {
     EnsK k;
     HHConst<float> hc;
+    size_t ebits_bytes;
 };
 
 // kernel arguments shared by the ensemble and the fused single-network step;
@@ -251,12 +332,13 @@ nnx_status prepare(nnx_ctx* c, const char* kind, const nnx_hh_state& s, const nn
                    const nnx_graph& g, double dt, uint64_t* const bits[2], EnsHost* h) {
     const int64_t n = s.n, nnz = g.nnz;
     char nc[48], ns[48], nt[48];
-    std::snprintf(nc, sizeof nc, "%s_cnt", kind);
-    std::snprintf(ns, sizeof ns, "%s_slot", kind);
+    std::snprintf(nc, sizeof nc, "%s_flag", kind);
+    std::snprintf(ns, sizeof ns, "%s_ebits", kind);
     std::snprintf(nt, sizeof nt, "%s_ctr", kind);
-    void *cnt, *slot, *ctr;
-    NNX_TRY(scratch(c, nc, (size_t)2 * n * 4, true, &cnt));
-    NNX_TRY(scratch(c, ns, (size_t)2 * std::max<int64_t>(nnz, 1) * 4, false, &slot));
+    void *flag, *eb, *ctr;
+    const int64_t nw = nnz / 32 + 2;  // event-bit words per parity
+    NNX_TRY(scratch(c, nc, (size_t)2 * n * 4, true, &flag));
+    NNX_TRY(scratch(c, ns, (size_t)2 * nw * 4, true, &eb));
     NNX_TRY(scratch(c, nt, 64, true, &ctr));
     EnsK& k = h->k;
     k = EnsK{};
@@ -264,8 +346,9 @@ nnx_status prepare(nnx_ctx* c, const char* kind, const nnx_hh_state& s, const nn
     k.iext = (const float*)s.i_ext;
     k.out_off = g.out_offsets, k.out_tgt = g.out_targets, k.out_to_in = g.out_to_in;
     k.in_off = g.in_offsets, k.in_src = g.in_sources, k.in_w = g.in_weights;
-    k.cnt0 = (int32_t*)cnt, k.cnt1 = (int32_t*)cnt + n;
-    k.slot0 = (int32_t*)slot, k.slot1 = (int32_t*)slot + std::max<int64_t>(nnz, 1);
+    k.flag0 = (int32_t*)flag, k.flag1 = (int32_t*)flag + n;
+    k.eb0 = (uint32_t*)eb, k.eb1 = (uint32_t*)eb + nw;
+    h->ebits_bytes = (size_t)2 * nw * 4;
     k.bits0 = bits[0], k.bits1 = bits[1];
     k.stepctr = (int64_t*)ctr;                       // bytes 0..15
     k.spk3 = (int32_t*)((char*)ctr + 16);            // bytes 16..27
@@ -355,17 +438,17 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     }
     const int64_t zero = 0;
     NNX_TRY(build_tiles(c, "fstep", s.n, 1, &zero, &s.n, f.tile, &h.k.tiles, &h.k.ntiles));
-    void* dn;
-    NNX_TRY(scratch(c, "event_dense", 256, true, &dn));  // used by the push; allocated before any capture
     const int q0 = (int)((first_step + 1) & 1);
-    int32_t* cnt_q0 = q0 ? h.k.cnt1 : h.k.cnt0;
-    int32_t* slot_q0 = q0 ? h.k.slot1 : h.k.slot0;
+    int32_t* flag_q0 = q0 ? h.k.flag1 : h.k.flag0;
+    uint32_t* eb_q0 = q0 ? h.k.eb1 : h.k.eb0;
     // the step number is set outside the captured sequence (a captured
     // kernel argument would replay the first capture's number)
     k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, 1);
     NNX_LAUNCHED(c);
     auto body = [&]() -> nnx_status {
-        NNX_TRY(push_list_launch(c, &p->graph, p->list, p->count, cnt_q0, slot_q0));
+        k_push_bits<<<2048, 256, 0, c->stream>>>(p->list, p->count, p->graph.out_offsets, p->graph.out_targets,
+                                                 p->graph.out_to_in, eb_q0, flag_q0);
+        NNX_LAUNCHED(c);
         for (int i = 0; i < n_steps; ++i)
             NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1), i + 1 < n_steps));
         CompactExtras ex{};
@@ -399,7 +482,8 @@ nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_con
     const bool fresh = first_step == 0;
     if (fresh) {
         // a new run: no pending bucket entries, no previous spikes
-        NNX_HIP(c, hipMemsetAsync(h.k.cnt0, 0, (size_t)2 * p->state.n * 4, c->stream));
+        NNX_HIP(c, hipMemsetAsync(h.k.flag0, 0, (size_t)2 * p->state.n * 4, c->stream));
+        NNX_HIP(c, hipMemsetAsync(h.k.eb0, 0, h.ebits_bytes, c->stream));
         NNX_HIP(c, hipMemsetAsync(p->bits[0], 0, (size_t)p->state.n / 8, c->stream));
         NNX_HIP(c, hipMemsetAsync(p->bits[1], 0, (size_t)p->state.n / 8, c->stream));
     }

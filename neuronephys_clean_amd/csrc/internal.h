@@ -112,9 +112,6 @@ nnx_status push_event_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list
 nnx_status hh_rk4_event_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q,
                                double dt, const EventIn& ev, uint64_t* bits);
 nnx_status event_scratch(nnx_ctx* c, int64_t n, int64_t nnz, int32_t** cnt, int32_t** slot);
-// pushes every spike of `list` (all `count` of them) into the given buckets
-nnx_status push_list_launch(nnx_ctx* c, const nnx_graph* g, const int32_t* list, const int32_t* count,
-                            int32_t* cnt, int32_t* slot);
 // nnx_run_steps for WEIGHTED_FUSED plans (ensemble.hip); plan already checked
 nnx_status fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t first_step, int n_steps,
                            int use_graph);

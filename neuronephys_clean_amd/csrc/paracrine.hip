@@ -15,100 +15,11 @@
 #include <cmath>
 #include <cstring>
 
+#include "grid_dev.h"
 #include "internal.h"
+#include "mg.h"
 
 namespace nnx {
-
-struct GridK {
-    int32_t nx, ny, nz, kind;  // kind: 0 const 7-pt, 1 variable 7-pt, 2 27-pt
-    float c, diag, a0, w;      // fp32 constants (cast once on the host)
-    double cd, diagd, a0d, wd; // fp64 constants
-    const float *kx, *ky, *kz;
-};
-
-static GridK make_grid(const nnx_grid* g) {
-    GridK k;
-    k.nx = g->nx;
-    k.ny = g->ny;
-    k.nz = g->nz;
-    k.kind = g->stencil == NNX_STENCIL_27 ? 2 : (g->kx ? 1 : 0);
-    k.cd = g->c;
-    k.diagd = 1.0 + 6.0 * g->c;
-    k.a0d = 1.0 + 0.5 * g->decay_dt;
-    k.wd = g->c / 60.0;
-    k.c = (float)k.cd;
-    k.diag = (float)k.diagd;
-    k.a0 = (float)k.a0d;
-    k.w = (float)k.wd;
-    k.kx = g->kx;
-    k.ky = g->ky;
-    k.kz = g->kz;
-    return k;
-}
-
-template <class T> __device__ __forceinline__ T cst(const GridK& g, float f, double d);
-template <> __device__ __forceinline__ float cst<float>(const GridK&, float f, double) { return f; }
-template <> __device__ __forceinline__ double cst<double>(const GridK&, float, double d) { return d; }
-
-template <class T>
-__device__ __forceinline__ T at(const T* __restrict__ x, const GridK& g, int i, int j, int k) {
-    if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return T(0);
-    return x[i + (int64_t)g.nx * (j + (int64_t)g.ny * k)];
-}
-
-// (A x)_ijk in T with the operation order of the CPU restatement.
-template <class T, class U>
-__device__ __forceinline__ T apply_at(const U* __restrict__ x, const GridK& g, int i, int j, int k) {
-    const int64_t id = i + (int64_t)g.nx * (j + (int64_t)g.ny * k);
-    const T xc = T(x[id]);
-    if (g.kind == 0) {
-        T s = T(at(x, g, i - 1, j, k)) + T(at(x, g, i + 1, j, k));
-        s = s + T(at(x, g, i, j - 1, k));
-        s = s + T(at(x, g, i, j + 1, k));
-        s = s + T(at(x, g, i, j, k - 1));
-        s = s + T(at(x, g, i, j, k + 1));
-        return cst<T>(g, g.diag, g.diagd) * xc - cst<T>(g, g.c, g.cd) * s;
-    }
-    if (g.kind == 1) {
-        const int64_t nx = g.nx, ny = g.ny;
-        const int64_t fx = i + (nx + 1) * (j + ny * k), fy = i + nx * (j + (ny + 1) * k);
-        const T kxm = T(g.kx[fx]), kxp = T(g.kx[fx + 1]);
-        const T kym = T(g.ky[fy]), kyp = T(g.ky[fy + nx]);
-        const T kzm = T(g.kz[id]), kzp = T(g.kz[id + nx * ny]);
-        const T d = T(1) + kxm + kxp + kym + kyp + kzm + kzp;
-        T s = kxm * T(at(x, g, i - 1, j, k)) + kxp * T(at(x, g, i + 1, j, k));
-        s = s + kym * T(at(x, g, i, j - 1, k));
-        s = s + kyp * T(at(x, g, i, j + 1, k));
-        s = s + kzm * T(at(x, g, i, j, k - 1));
-        s = s + kzp * T(at(x, g, i, j, k + 1));
-        return d * xc - s;
-    }
-    T corners = T(0), edges = T(0), faces = T(0);
-    for (int dk = -1; dk <= 1; ++dk)
-        for (int dj = -1; dj <= 1; ++dj)
-            for (int di = -1; di <= 1; ++di) {
-                const int nzc = (di != 0) + (dj != 0) + (dk != 0);
-                if (nzc == 0) continue;
-                const T v = T(at(x, g, i + di, j + dj, k + dk));
-                if (nzc == 3) corners = corners + v;
-                else if (nzc == 2) edges = edges + v;
-                else faces = faces + v;
-            }
-    const T s27 = ((corners + T(3) * edges) + T(14) * faces) - T(128) * xc;
-    return cst<T>(g, g.a0, g.a0d) * xc - cst<T>(g, g.w, g.wd) * s27;
-}
-
-// 1/diag for Jacobi
-__device__ __forceinline__ float dinv_at(const GridK& g, int64_t id) {
-    if (g.kind == 0) return (float)(1.0 / g.diagd);
-    if (g.kind == 2) return (float)(1.0 / (g.a0d + 128.0 * g.wd));
-    const int64_t nx = g.nx, ny = g.ny;
-    const int64_t i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
-    const int64_t fx = i + (nx + 1) * (j + ny * k), fy = i + nx * (j + (ny + 1) * k);
-    const double d = 1.0 + (double)g.kx[fx] + g.kx[fx + 1] + g.ky[fy] + g.ky[fy + nx] + g.kz[id] +
-                     g.kz[id + nx * ny];
-    return (float)(1.0 / d);
-}
 
 template <class T, class U>
 __global__ __launch_bounds__(256) void k_apply(GridK g, const U* __restrict__ x, T* __restrict__ y) {
@@ -425,14 +336,12 @@ __global__ __launch_bounds__(256) void k_cg_spmv_gen(GridK g, CgDev d, int k,
                                                      const float* __restrict__ p) {
     double rz, rzp, rr;
     if (!cg_scalars(d, k, false, rz, rzp, rr)) return;
-    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     double v[1] = {0};
-    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
-        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), kk = (int)(id / ((int64_t)g.nx * g.ny));
+    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int kk, int64_t id) {
         const float qv = apply_at<float, float>(p, g, i, j, kk);
         d.q[id] = qv;
         v[0] += (double)p[id] * qv;
-    }
+    });
     put_partial<1>(v, d.ppq);
 }
 
@@ -745,11 +654,9 @@ __global__ __launch_bounds__(256) void k_pc_pre(int64_t n, const float* __restri
 // out = r - A z
 __global__ __launch_bounds__(256) void k_pc_res(GridK g, const float* __restrict__ r, const float* __restrict__ z,
                                                 float* __restrict__ out) {
-    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
-        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), k = (int)(id / ((int64_t)g.nx * g.ny));
+    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
         out[id] = r[id] - apply_at<float, float>(z, g, i, j, k);
-    }
+    });
 }
 __global__ __launch_bounds__(256) void k_pc_add(int64_t n, float* __restrict__ z, const float* __restrict__ u) {
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) z[i] = z[i] + u[i];
@@ -758,11 +665,9 @@ __global__ __launch_bounds__(256) void k_pc_add(int64_t n, float* __restrict__ z
 __global__ __launch_bounds__(256) void k_pc_post(GridK g, const float* __restrict__ dinv,
                                                  const float* __restrict__ r, const float* __restrict__ z1,
                                                  float* __restrict__ zout) {
-    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
-        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), k = (int)(id / ((int64_t)g.nx * g.ny));
+    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
         zout[id] = z1[id] + kSmoothW * dinv[id] * (r[id] - apply_at<float, float>(z1, g, i, j, k));
-    }
+    });
 }
 
 __global__ __launch_bounds__(256) void k_fcg_dot(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
@@ -791,15 +696,13 @@ __global__ __launch_bounds__(256) void k_fcg_dir(int64_t n, const float* __restr
 // q = A p; partials (p.q, p.r)
 __global__ __launch_bounds__(256) void k_fcg_spmv(GridK g, const float* __restrict__ p, const float* __restrict__ r,
                                                   float* __restrict__ q, double* ppq) {
-    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
     double v[2] = {0, 0};
-    for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
-        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), kk = (int)(id / ((int64_t)g.nx * g.ny));
+    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int kk, int64_t id) {
         const float qv = apply_at<float, float>(p, g, i, j, kk);
         q[id] = qv;
         v[0] += (double)p[id] * qv;
         v[1] += (double)p[id] * r[id];
-    }
+    });
     put_partial<2>(v, ppq);
 }
 
@@ -968,9 +871,17 @@ double read_scalar(nnx_ctx* c, const double* d, hipError_t* err) {
 // FCG on A x = b (fp32 vectors, FP64 reductions), M = the U-Net when net is
 // set, Jacobi otherwise or after stall_iters iterations without a new best
 // residual.  The host reads the residual norm after every iteration.
-nnx_status fcg_run(nnx_ctx* c, const GridK& gk, nnx_unet* net, const float* feat, const float* b, float* x,
-                   int use_x0, double rtol, int maxit, int stall_iters, int smooth, nnx_cg_result* res) {
+// pc: JACOBI, LEARNED (net), MG or MG_LEARNED (net); the learned variants
+// fall back to their base (Jacobi, MG) after stall_iters iterations without
+// a new best residual.
+nnx_status fcg_run(nnx_ctx* c, const GridK& gk, int pc, int mg_nu, nnx_unet* net, const float* feat,
+                   const float* b, float* x, int use_x0, double rtol, int maxit, int stall_iters, int smooth,
+                   nnx_cg_result* res) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
+    const bool use_mg = pc == NNX_PRECOND_MG || pc == NNX_PRECOND_MG_LEARNED;
+    if (pc != NNX_PRECOND_LEARNED && pc != NNX_PRECOND_MG_LEARNED) net = nullptr;
+    Mg mg;
+    if (use_mg) NNX_TRY(mg_setup(c, gk, mg_nu, &mg));
     void *r, *z, *pa, *pb, *qa, *qb, *dinv, *part, *sc, *w1, *w2;
     NNX_TRY(scratch(c, "fcg_r", n * 4, false, &r));
     NNX_TRY(scratch(c, "fcg_z", n * 4, false, &z));
@@ -1011,7 +922,10 @@ nnx_status fcg_run(nnx_ctx* c, const GridK& gk, nnx_unet* net, const float* feat
         const int cur = it & 1, old = cur ^ 1;
         if (learned) {
             float *W1 = (float*)w1, *W2 = (float*)w2;
-            k_pc_pre<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const float*)dinv, R, W1);
+            if (use_mg)
+                NNX_TRY(mg_vcycle(c, mg, R, W1));
+            else
+                k_pc_pre<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const float*)dinv, R, W1);
             k_pc_res<<<kRedBlocks, 256, 0, c->stream>>>(gk, R, W1, W2);
             NNX_TRY(nnx_unet_forward_internal(c, net, gk.nx, gk.ny, gk.nz, 1, W2, feat, Z));
             k_pc_add<<<flat_blocks(n), 256, 0, c->stream>>>(n, W1, Z);
@@ -1025,6 +939,8 @@ nnx_status fcg_run(nnx_ctx* c, const GridK& gk, nnx_unet* net, const float* feat
                 src = dst;
             }
             if (src != Z) NNX_HIP(c, hipMemcpyAsync(Z, src, n * 4, hipMemcpyDeviceToDevice, c->stream));
+        } else if (use_mg) {
+            NNX_TRY(mg_vcycle(c, mg, R, Z));
         } else {
             k_fcg_jacobi<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const float*)dinv, R, Z);
         }
@@ -1053,8 +969,22 @@ nnx_status fcg_run(nnx_ctx* c, const GridK& gk, nnx_unet* net, const float* feat
         res->iters = it;
         res->relres = rel;
         res->converged = rel <= rtol;
-        res->precond_used = learned ? NNX_PRECOND_LEARNED : NNX_PRECOND_JACOBI;
+        res->precond_used = use_mg ? (learned ? NNX_PRECOND_MG_LEARNED : NNX_PRECOND_MG)
+                                   : (learned ? NNX_PRECOND_LEARNED : NNX_PRECOND_JACOBI);
     }
+    return NNX_OK;
+}
+
+nnx_status check_fcg(nnx_ctx* c, const nnx_grid* g, const nnx_unet* net, const nnx_cg_opts* o) {
+    const int pc = o->precond;
+    NNX_CHECK(c, pc == NNX_PRECOND_JACOBI || pc == NNX_PRECOND_LEARNED || pc == NNX_PRECOND_MG ||
+                     pc == NNX_PRECOND_MG_LEARNED,
+              "FCG precond: JACOBI, LEARNED, MG or MG_LEARNED");
+    NNX_CHECK(c, (pc != NNX_PRECOND_LEARNED && pc != NNX_PRECOND_MG_LEARNED) || net,
+              "learned preconditioner needs a network");
+    NNX_CHECK(c, (pc != NNX_PRECOND_MG && pc != NNX_PRECOND_MG_LEARNED) || (g->kx && g->ky && g->kz),
+              "multigrid needs the variable-coefficient operator (kx, ky, kz)");
+    NNX_CHECK(c, o->mg_nu >= 0 && o->mg_nu <= 16, "mg_nu must be in [0, 16]");
     return NNX_OK;
 }
 
@@ -1068,13 +998,12 @@ nnx_status nnx_fcg_solve(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, const flo
     NNX_TRY(check_grid(c, g));
     NNX_CHECK(c, b && x && o, "null b, x or opts");
     NNX_CHECK(c, o->rtol >= 0 && o->maxit >= 0, "bad rtol/maxit");
-    NNX_CHECK(c, o->precond == NNX_PRECOND_LEARNED || o->precond == NNX_PRECOND_JACOBI, "FCG precond: LEARNED or JACOBI");
-    NNX_CHECK(c, o->precond != NNX_PRECOND_LEARNED || net, "learned preconditioner needs a network");
+    NNX_TRY(check_fcg(c, g, net, o));
     const GridK gk = make_grid(g);
     auto t0 = std::chrono::steady_clock::now();
     nnx_cg_result r{};
-    NNX_TRY(fcg_run(c, gk, o->precond == NNX_PRECOND_LEARNED ? net : nullptr, d_feat, b, x, o->use_x0, o->rtol,
-                    o->maxit, o->stall_iters, o->smooth, &r));
+    NNX_TRY(fcg_run(c, gk, o->precond, o->mg_nu, net, d_feat, b, x, o->use_x0, o->rtol, o->maxit, o->stall_iters,
+                    o->smooth, &r));
     auto t1 = std::chrono::steady_clock::now();
     CgHost h;
     NNX_TRY(cg_buffers(c, gk, NNX_PRECOND_NONE, &h));
@@ -1090,8 +1019,7 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
     if (!c) return NNX_ERR_ARG;
     NNX_TRY(check_grid(c, g));
     NNX_CHECK(c, b && x && o, "null b, x or opts");
-    NNX_CHECK(c, o->precond == NNX_PRECOND_LEARNED || o->precond == NNX_PRECOND_JACOBI, "FCG precond: LEARNED or JACOBI");
-    NNX_CHECK(c, o->precond != NNX_PRECOND_LEARNED || net, "learned preconditioner needs a network");
+    NNX_TRY(check_fcg(c, g, net, o));
     const GridK gk = make_grid(g);
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
     auto t0 = std::chrono::steady_clock::now();
@@ -1116,10 +1044,10 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
         NNX_LAUNCHED(c);
         const double tol = std::max(inner_rtol, 0.5 * o->rtol / rel);
         nnx_cg_result ir{};
-        NNX_TRY(fcg_run(c, gk, o->precond == NNX_PRECOND_LEARNED ? net : nullptr, d_feat, (const float*)rhs,
-                        (float*)d, 0, tol, o->maxit, o->stall_iters, o->smooth, &ir));
+        NNX_TRY(fcg_run(c, gk, o->precond, o->mg_nu, net, d_feat, (const float*)rhs, (float*)d, 0, tol, o->maxit,
+                        o->stall_iters, o->smooth, &ir));
         inner_total += ir.iters;
-        if (ir.precond_used == NNX_PRECOND_JACOBI) used = NNX_PRECOND_JACOBI;
+        if (ir.precond_used != o->precond) used = ir.precond_used;
         k_axpy_f64<<<flat_blocks(n), 256, 0, c->stream>>>(n, 1.0 / s, (const float*)d, x);
         NNX_LAUNCHED(c);
         ++outer;

@@ -21,6 +21,8 @@ from .devmem import DeviceArray as DA
 
 SMOOTH_W = np.float32(2.0 / 3.0)  # damping of the Jacobi smoothings around the network
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_c5.f32")
+# weights of the network that corrects the multigrid V-cycle (PRECOND_MG_LEARNED)
+MG_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_c5_mg.f32")
 
 
 def param_count() -> int:
@@ -164,18 +166,23 @@ class PairSet:
                                   C.c_void_p(src.ptr + fo), C.c_size_t(self.face_bytes))
 
 
-def _cg(ctx, g, b: DA, x: DA, rtol, maxit, use_x0=0):
-    o = _lib.CgOpts(rtol=rtol, maxit=maxit, precond=_lib.PRECOND_JACOBI, check_every=8,
-                    use_x0=use_x0)
+def _cg(ctx, g, b: DA, x: DA, rtol, maxit, use_x0=0, mg=False):
+    """maxit iterations of Jacobi-CG (or of multigrid-preconditioned
+    flexible CG) from x = 0: the iterate snapshots of the pair harvest."""
+    o = _lib.CgOpts(rtol=rtol, maxit=maxit, precond=_lib.PRECOND_MG if mg else _lib.PRECOND_JACOBI,
+                    check_every=8, use_x0=use_x0)
     r = _lib.CgResult()
-    st = ctx._l.nnx_cg_solve(ctx.h, C.byref(g), b.ptr, x.ptr, C.byref(o), C.byref(r))
+    if mg:
+        st = ctx._l.nnx_fcg_solve(ctx.h, C.byref(g), None, None, b.ptr, x.ptr, C.byref(o), C.byref(r))
+    else:
+        st = ctx._l.nnx_cg_solve(ctx.h, C.byref(g), b.ptr, x.ptr, C.byref(o), C.byref(r))
     if st not in (0, 4):  # 4: not converged within maxit (the snapshots)
         ctx.check(st)
     return r
 
 
 def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 64), seed=1,
-            log=print) -> PairSet:
+            log=print, base: str = "jacobi") -> PairSet:
     """Pairs (r_k, e_k) from Jacobi-CG iterates x_k (x_0 = 0) on `fields`
     fresh log-normal diffusivity fields (C5 statistics at n^3) and `sources`
     random point-source right-hand sides per field: e_k = x* - x_k with x*
@@ -184,11 +191,14 @@ def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 
     its equation to fp32 rounding.  The network is the coarse correction of
     a two-level preconditioner after damped-Jacobi pre-smoothing
     (z0 = w D^-1 r, w = 2/3), so the stored pair is what the smoother
-    leaves: e1 = e_k - w D^-1 r_k and r1 = A e1."""
+    leaves: e1 = e_k - w D^-1 r_k and r1 = A e1.
+    base "mg": the iterates are those of multigrid-preconditioned flexible
+    CG and the network corrects one V-cycle: e1 = e_k - V(r_k)."""
     ps = PairSet(ctx, n, fields * sources * len(schedule), fields)
     vox = n ** 3
-    x64, xk, ek, rk = (DA(ctx, vox, np.float64), DA(ctx, vox, np.float32), DA(ctx, vox, np.float32),
-                       DA(ctx, vox, np.float32))
+    x64, xk, ek, rk, zk = (DA(ctx, vox, np.float64), DA(ctx, vox, np.float32), DA(ctx, vox, np.float32),
+                           DA(ctx, vox, np.float32), DA(ctx, vox, np.float32))
+    mg = base == "mg"
     t0 = time.time()
     for f in range(fields):
         fld = configs.c5_field(seed=seed * 100003 + f, n=n)
@@ -207,11 +217,15 @@ def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 
                 if k == 0:
                     e = xstar.astype(np.float32)
                 else:
-                    _cg(ctx, g, db, xk, 0.0, k)
+                    _cg(ctx, g, db, xk, 0.0, k, mg=mg)
                     e = (xstar - xk.numpy()).astype(np.float32)
                 ek.upload(e)
                 ctx.call("nnx_stencil_apply", C.byref(g), ek.ptr, rk.ptr)
-                e1 = (e - SMOOTH_W * dinv * rk.numpy()).astype(np.float32)
+                if mg:
+                    ctx.call("nnx_mg_apply", C.byref(g), 0, rk.ptr, zk.ptr)
+                    e1 = (e - zk.numpy()).astype(np.float32)
+                else:
+                    e1 = (e - SMOOTH_W * dinv * rk.numpy()).astype(np.float32)
                 ek.upload(e1)
                 ctx.call("nnx_stencil_apply", C.byref(g), ek.ptr, rk.ptr)
                 ps.add_pair(rk.numpy(), e1, f)

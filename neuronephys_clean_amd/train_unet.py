@@ -30,14 +30,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def compare(ctx, net, n, seed, sources=400, rtol=1e-8, smooth=1):
-    """Jacobi-CG vs learned FCG, both inside FP64 refinement, on one field."""
+def compare(ctx, net, n, seed, sources=400, rtol=1e-8, smooth=1, base="jacobi"):
+    """Jacobi-CG vs learned FCG (and, for base "mg", multigrid FCG without
+    the network), all inside FP64 refinement, on one field."""
     f = configs.c5_field(seed=seed, n=n)
     g, keep = device_field(ctx, f)
     b = configs.point_sources(seed + 7, n, sources).astype(np.float64)
     db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, n ** 3, np.float64)
     out = {}
-    for name, pc in (("jacobi_cg", None), ("learned_fcg", _lib.PRECOND_LEARNED)):
+    solvers = [("jacobi_cg", None), ("learned_fcg", _lib.PRECOND_LEARNED)]
+    if base == "mg":
+        solvers = [("jacobi_cg", None), ("mg_fcg", _lib.PRECOND_MG), ("learned_fcg", _lib.PRECOND_MG_LEARNED)]
+    for name, pc in solvers:
         best = None
         for _ in range(2):
             r = _lib.CgResult()
@@ -52,6 +56,8 @@ def compare(ctx, net, n, seed, sources=400, rtol=1e-8, smooth=1):
                 best = r.as_dict()
         out[name] = best
     out["speedup"] = out["jacobi_cg"]["ms"] / out["learned_fcg"]["ms"]
+    if base == "mg":
+        out["speedup_vs_mg"] = out["mg_fcg"]["ms"] / out["learned_fcg"]["ms"]
     # cost of one network application at this size
     r32, z32 = DA.from_numpy(ctx, b.astype(np.float32)), DA.zeros(ctx, n ** 3, np.float32)
     e0, e1 = C.c_void_p(), C.c_void_p()
@@ -84,10 +90,14 @@ def main():
     ap.add_argument("--smooth", type=int, default=1, help="post-smoothing sweeps in the evaluation")
     ap.add_argument("--loss", choices=["mse", "anorm", "smoothed"], default="smoothed",
                     help="smoothed: A-norm of the error left after --smooth post-smoothing sweeps")
+    ap.add_argument("--base", choices=["jacobi", "mg"], default="jacobi",
+                    help="what the network corrects: damped-Jacobi smoothing or a multigrid V-cycle")
     args = ap.parse_args()
     ctx = _lib.context(0)
     t0 = time.time()
-    ps = harvest(ctx, n=args.n, fields=args.fields, sources=args.sources, seed=args.seed, log=log)
+    sched = (0, 1, 2, 3, 4, 6, 8, 12) if args.base == "mg" else (0, 1, 2, 4, 8, 16, 32, 64)
+    ps = harvest(ctx, n=args.n, fields=args.fields, sources=args.sources, seed=args.seed, log=log,
+                 base=args.base, schedule=sched)
     ctx.synchronize()
     t_h = time.time() - t0
     log(f"harvested {ps.count} pairs in {t_h:.1f}s")
@@ -103,12 +113,12 @@ def main():
         f"loss {losses[0][1]:.4f} -> {losses[-1][1]:.4f}")
     net.save(args.out)
     del ps
-    report = {"pairs": args.fields * args.sources * 8, "n": args.n, "coarse": args.coarse,
+    report = {"pairs": args.fields * args.sources * 8, "n": args.n, "coarse": args.coarse, "base": args.base,
               "smooth": args.smooth, "loss": args.loss, "harvest_s": t_h, "train_s": t_t,
               "steps": steps, "batch": args.batch, "epochs": args.epochs,
               "loss_first": losses[0][1], "loss_last": losses[-1][1], "losses": losses, "eval": {}}
     for n in args.eval_n:
-        report["eval"][str(n)] = compare(ctx, net, n, seed=424242 + n, smooth=args.smooth)
+        report["eval"][str(n)] = compare(ctx, net, n, seed=424242 + n, smooth=args.smooth, base=args.base)
         log(f"eval {n}^3: {report['eval'][str(n)]}")
     with open(os.path.splitext(args.out)[0] + ".json", "w") as fh:
         json.dump(report, fh, indent=1)

@@ -1,0 +1,73 @@
+"""numpy restatement of the multigrid V-cycle for the variable-coefficient
+7-point backward-Euler operator (the checker of nnx_mg_apply), written
+from the definitions: arrays are indexed [k, j, i] (x fastest in memory),
+face arrays kx [nz, ny, nx+1], ky [nz, ny+1, nx], kz [nz+1, ny, nx]."""
+import numpy as np
+
+
+def apply(x, kx, ky, kz):
+    """(A x)_v = x_v + sum over the 6 faces of k_f (x_v - x_nbr), x = 0
+    beyond the boundary."""
+    p = np.pad(x, 1)
+    c = p[1:-1, 1:-1, 1:-1]
+    d = 1.0 + kx[:, :, :-1] + kx[:, :, 1:] + ky[:, :-1, :] + ky[:, 1:, :] + kz[:-1] + kz[1:]
+    s = (kx[:, :, :-1] * p[1:-1, 1:-1, :-2] + kx[:, :, 1:] * p[1:-1, 1:-1, 2:]
+         + ky[:, :-1, :] * p[1:-1, :-2, 1:-1] + ky[:, 1:, :] * p[1:-1, 2:, 1:-1]
+         + kz[:-1] * p[:-2, 1:-1, 1:-1] + kz[1:] * p[2:, 1:-1, 1:-1])
+    return d * c - s
+
+
+def diag(kx, ky, kz):
+    return 1.0 + kx[:, :, :-1] + kx[:, :, 1:] + ky[:, :-1, :] + ky[:, 1:, :] + kz[:-1] + kz[1:]
+
+
+def coarsen(kx, ky, kz):
+    """Coarse face = 1/16 of the sum of the 4 fine faces it covers."""
+    cx = (kx[0::2, 0::2, 0::2] + kx[0::2, 1::2, 0::2] + kx[1::2, 0::2, 0::2] + kx[1::2, 1::2, 0::2]) / 16
+    cy = (ky[0::2, 0::2, 0::2] + ky[0::2, 0::2, 1::2] + ky[1::2, 0::2, 0::2] + ky[1::2, 0::2, 1::2]) / 16
+    cz = (kz[0::2, 0::2, 0::2] + kz[0::2, 0::2, 1::2] + kz[0::2, 1::2, 0::2] + kz[0::2, 1::2, 1::2]) / 16
+    return cx, cy, cz
+
+
+def restrict(r):
+    """Block mean over 2 x 2 x 2 children."""
+    nz, ny, nx = r.shape
+    return r.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).mean(axis=(1, 3, 5))
+
+
+def _prolong_axis(e, axis):
+    """Cell-centred linear interpolation by 2 along one axis, zero beyond
+    the ends: fine 2I gets 3/4 e_I + 1/4 e_{I-1}, fine 2I+1 gets
+    3/4 e_I + 1/4 e_{I+1}."""
+    e = np.moveaxis(e, axis, 0)
+    z = np.zeros_like(e[:1])
+    lo = np.concatenate([z, e[:-1]])
+    hi = np.concatenate([e[1:], z])
+    out = np.empty((2 * e.shape[0],) + e.shape[1:], e.dtype)
+    out[0::2] = 0.75 * e + 0.25 * lo
+    out[1::2] = 0.75 * e + 0.25 * hi
+    return np.moveaxis(out, 0, axis)
+
+
+def prolong(e):
+    return _prolong_axis(_prolong_axis(_prolong_axis(e, 2), 1), 0)
+
+
+def vcycle(r, kx, ky, kz, nu=2, w=0.8, coarse_sweeps=16):
+    """One V-cycle from zero: nu damped-Jacobi sweeps before and after each
+    level, coarsening while every dimension is even and >= 8."""
+    nz, ny, nx = r.shape
+    dinv = 1.0 / diag(kx, ky, kz)
+    can = nx % 2 == 0 and ny % 2 == 0 and nz % 2 == 0 and min(nx, ny, nz) >= 8
+    sweeps = nu if can else coarse_sweeps
+    x = w * dinv * r
+    for _ in range(sweeps - 1):
+        x = x + w * dinv * (r - apply(x, kx, ky, kz))
+    if not can:
+        return x
+    rc = restrict(r - apply(x, kx, ky, kz))
+    ec = vcycle(rc, *coarsen(kx, ky, kz), nu=nu, w=w, coarse_sweeps=coarse_sweeps)
+    x = x + prolong(ec)
+    for _ in range(nu):
+        x = x + w * dinv * (r - apply(x, kx, ky, kz))
+    return x
