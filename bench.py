@@ -25,6 +25,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -333,6 +334,13 @@ def profile_pass(ctx, sim, net):
     ctx.synchronize()
     u.close()
     stencil_roofline(ctx, 256, 20)
+    # 3 multigrid V-cycles (default sweeps) on the 256^3 variable-diffusivity operator
+    from neuronephys_clean_amd.precond import device_field
+    fg, keep = device_field(ctx, configs.c5_field(seed=0, n=256))
+    for _ in range(3):
+        ctx.call("nnx_mg_apply", C.byref(fg), 0, rr.ptr, zz.ptr)
+    ctx.synchronize()
+    del keep
     vc = sim.a["voxel_counts"].numpy().astype(np.float32)
     b = 0.01 * vc
     if not b.any():
@@ -346,7 +354,20 @@ def profile_pass(ctx, sim, net):
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
     log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
-        f"20 stencil applies, 1 CG solve ({r.iters} iterations), 256 MiB copy")
+        f"20 stencil applies, 3 V-cycles, 1 CG solve ({r.iters} iterations), 256 MiB copy")
+
+
+def profiled_traffic(label):
+    """HBM bytes per launch of a kernel as measured by the FETCH_SIZE and
+    WRITE_SIZE passes summarised in profiles/hbm_traffic_profile_mode.md
+    (None when the summary is absent)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_profile_mode.md")
+    try:
+        with open(path) as fh:
+            m = re.search(re.escape(label) + r" traffic per launch: ([0-9.eE+-]+) bytes", fh.read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
 
 
 def tune_fused(ctx, sim, chunk):
@@ -387,8 +408,9 @@ def main():
     ap.add_argument("--couple", default="fused", choices=["fused", "event", "pull"])
     ap.add_argument("--cpu-steps", type=int, default=1500)
     ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
-                    help="profile: a short eager sequence for rocprofv3 --pmc passes (20 steps, "
-                         "20 stencil applies and one CG solve on the 256^3 grid, one 256 MiB "
+                    help="profile: a short eager sequence for rocprofv3 --pmc passes (20 fused and 20 "
+                         "unfused C2 steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
+                         "20 stencil applies, 3 V-cycles and one CG solve on 256^3 grids, one 256 MiB "
                          "device copy to calibrate FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
@@ -456,7 +478,10 @@ def main():
         us = ev_ms.value * 1e3 / args.steps
         fused_roof = {"kernel": "k_ens_step<%d> (fused gather + RK4 + push)" % tune["block"], "bound": "hbm",
                       "achieved": fb / (us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                      "frac": fb / (us * 1e-6) / HBM_PEAK, "traffic": None, "avg_launch_us": us,
+                      "frac": fb / (us * 1e-6) / HBM_PEAK, "traffic": profiled_traffic("fused-step"),
+                      "traffic_source": "profiles/hbm_traffic_profile_mode.md (FETCH_SIZE + WRITE_SIZE, "
+                                        "calibrated on a 256 MiB copy, quiet-step median)",
+                      "avg_launch_us": us,
                       "alg_bytes_per_launch": fb,
                       "alg_bytes_rule": "48.125 B per neuron + 24 B per delivered event"}
 

@@ -517,8 +517,9 @@ typedef struct nnx_step_plan {
     /* WEIGHTED_FUSED launch shape (0 = default): workgroup size (64, 128,
      * 256 or 512), tile (neurons per work item, a multiple of 64), grid
      * (workgroups); and the burst threshold: when a step fires more than
-     * `fused_burst` neurons (0 = max(1024, n/16), < 0 = never) the next step
-     * pulls every target's in-edges instead of reading its event bucket.
+     * `fused_burst` neurons (<= 0: never) the next step pulls every target's
+     * in-edges instead of reading its pending events (pushing events is the
+     * cheaper of the two until a large part of the network fires at once).
      * None of these changes any result.                                     */
     int32_t fused_block, fused_tile, fused_grid, fused_burst;
 } nnx_step_plan;

@@ -352,7 +352,7 @@ nnx_status prepare(nnx_ctx* c, const char* kind, const nnx_hh_state& s, const nn
     k.bits0 = bits[0], k.bits1 = bits[1];
     k.stepctr = (int64_t*)ctr;                       // bytes 0..15
     k.spk3 = (int32_t*)((char*)ctr + 16);            // bytes 16..27
-    k.burst = (int32_t)std::max<int64_t>(1024, n / 16);
+    k.burst = -1;  // no dense fallback unless asked for (event pushes are cheaper up to near-synchronous firing)
     h->hc = make_const<float>(&prm, dt);
     return NNX_OK;
 }
@@ -383,7 +383,7 @@ uint64_t ens_key(const nnx_ens_plan* p, const nnx_ens_config& f) {
 }
 
 const nnx_ens_config kDefaultCfg = {256, 2048, 1024, 0};
-const nnx_ens_config kFusedDefault = {256, 1024, 1024, 0};
+const nnx_ens_config kFusedDefault = {512, 1024, 2048, 0};
 
 // Captures `body` (which enqueues on c->stream) into a graph cached under
 // `key`, or replays the cached one.
@@ -429,7 +429,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     EnsHost h;
     uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
     NNX_TRY(prepare(c, "fstep", s, p->params, p->graph, p->dt, bits, &h));
-    h.k.burst = p->fused_burst == 0 ? h.k.burst : p->fused_burst < 0 ? -1 : p->fused_burst;
+    h.k.burst = p->fused_burst > 0 ? p->fused_burst : -1;
     if (p->voxel_of) h.k.voxel_of = p->voxel_of, h.k.voxel_counts = p->voxel_counts;
     if (p->record) {
         h.k.raster = p->raster;

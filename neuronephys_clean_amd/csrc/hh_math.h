@@ -19,8 +19,7 @@ namespace nnx {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float dexp(float x) {
     if (x != x) return x;
-    x = x > 88.0f ? 88.0f : x;
-    x = x < -80.0f ? -80.0f : x;
+    x = __builtin_amdgcn_fmed3f(x, -80.0f, 88.0f);  // the clamp to [-80, 88] (x is not NaN here)
     const float k = __builtin_rintf(x * 0x1.715476p+0f);
     float r = __builtin_fmaf(-k, 0x1.62e4p-1f, x);
     r = __builtin_fmaf(-k, 0x1.7f7d1cp-20f, r);
