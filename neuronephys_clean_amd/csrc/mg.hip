@@ -11,9 +11,9 @@
 // Restriction is the block mean of the residual, prolongation trilinear
 // between cell centres with zero beyond the boundary (the fine operator's
 // own boundary treatment), smoothing damped Jacobi (nu sweeps before and
-// after; 4 by default), the coarsest level 16 sweeps.  Block-mean restriction with
-// trilinear prolongation is not a symmetric pair, so the cycle is used
-// inside flexible CG.
+// after; 4 by default), the coarsest level 16 sweeps (one workgroup when
+// the level is small).  Block-mean restriction with trilinear prolongation
+// is not a symmetric pair, so the cycle is used inside flexible CG.
 //
 // No counterpart in the reference, whose CG is unpreconditioned
 // (src/paracrine.cu:106-129).
@@ -27,6 +27,9 @@ namespace nnx {
 namespace {
 
 unsigned grid_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
+// row kernels: enough workgroups for one wavefront per x-row on small levels
+unsigned row_blocks(int ny, int nz) { return (unsigned)std::max(1, std::min((ny * nz + 3) / 4, 4096)); }
+constexpr int kCoarseOneBlock = 4096;  // coarsest levels up to this size are solved by one workgroup
 
 __global__ __launch_bounds__(256) void k_mg_dinv(GridK g, float* __restrict__ dinv) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
@@ -92,6 +95,29 @@ __global__ __launch_bounds__(256) void k_mg_restrict(GridK g, const float* __res
         }
         bc[q] = 0.125f * s;
     });
+}
+
+// all `sweeps` damped-Jacobi sweeps of a small level from zero in one
+// workgroup (x and t ping-pong, the result ends in x or t as returned by
+// the host's parity bookkeeping: sweep s writes x for even s, t for odd s)
+__global__ __launch_bounds__(1024) void k_mg_coarse(GridK g, const float* __restrict__ dinv,
+                                                    const float* __restrict__ b, float* __restrict__ x,
+                                                    float* __restrict__ t, float w, int sweeps) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int n = nx * ny * nz;
+    for (int s = 0; s < sweeps; ++s) {
+        const float* src = (s & 1) ? x : t;
+        float* dst = (s & 1) ? t : x;
+        for (int id = threadIdx.x; id < n; id += blockDim.x) {
+            if (s == 0) {
+                dst[id] = w * dinv[id] * b[id];
+            } else {
+                const int i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
+                dst[id] = src[id] + w * dinv[id] * (b[id] - apply_at<float, float>(src, g, i, j, k));
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // cell-centred linear weights along one axis: fine cell i sits at coarse
@@ -186,7 +212,8 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
         const int64_t n = n_of(l);
         float *src = nullptr, *dst = first;
         for (int s = 0; s < count; ++s) {
-            k_mg_smooth<<<grid_blocks(n), 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], src, dst, m.w, s == 0);
+            k_mg_smooth<<<s == 0 ? grid_blocks(n) : row_blocks(m.g[l].ny, m.g[l].nz), 256, 0, c->stream>>>(
+                m.g[l], m.dinv[l], B[l], src, dst, m.w, s == 0);
             NNX_LAUNCHED(c);
             src = dst;
             dst = (dst == first) ? other : first;
@@ -195,10 +222,10 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
         return NNX_OK;
     };
     auto more = [&](int l, int count) -> nnx_status {
-        const int64_t n = n_of(l);
         float* other = (cur[l] == m.t[l]) ? (l ? m.x[l] : z) : m.t[l];
         for (int s = 0; s < count; ++s) {
-            k_mg_smooth<<<grid_blocks(n), 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], cur[l], other, m.w, 0);
+            k_mg_smooth<<<row_blocks(m.g[l].ny, m.g[l].nz), 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], cur[l],
+                                                                                 other, m.w, 0);
             NNX_LAUNCHED(c);
             std::swap(cur[l], other);
         }
@@ -211,16 +238,21 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
     for (int l = 0; l < L; ++l) {
         NNX_TRY(sweeps(l, m.nu, m.t[l], l ? m.x[l] : z));
         const GridK& g = m.g[l];
-        const int64_t nc = n_of(l + 1);
-        k_mg_restrict<<<grid_blocks(nc), 256, 0, c->stream>>>(g, B[l], cur[l], m.b[l + 1], m.g[l + 1].nx,
-                                                              m.g[l + 1].ny, m.g[l + 1].nz);
+        k_mg_restrict<<<row_blocks(m.g[l + 1].ny, m.g[l + 1].nz), 256, 0, c->stream>>>(
+            g, B[l], cur[l], m.b[l + 1], m.g[l + 1].nx, m.g[l + 1].ny, m.g[l + 1].nz);
         NNX_LAUNCHED(c);
     }
-    NNX_TRY(sweeps(L, m.coarse_sweeps, m.x[L] ? m.x[L] : z, m.t[L]));
+    if (L > 0 && n_of(L) <= kCoarseOneBlock) {
+        k_mg_coarse<<<1, 1024, 0, c->stream>>>(m.g[L], m.dinv[L], B[L], m.x[L], m.t[L], m.w, m.coarse_sweeps);
+        NNX_LAUNCHED(c);
+        cur[L] = (m.coarse_sweeps & 1) ? m.x[L] : m.t[L];
+    } else {
+        NNX_TRY(sweeps(L, m.coarse_sweeps, m.x[L] ? m.x[L] : z, m.t[L]));
+    }
     // up
     for (int l = L - 1; l >= 0; --l) {
         const GridK& g = m.g[l];
-        k_mg_prolong_add<<<grid_blocks(n_of(l)), 256, 0, c->stream>>>(g.nx, g.ny, g.nz, cur[l + 1], m.g[l + 1].nx,
+        k_mg_prolong_add<<<row_blocks(g.ny, g.nz), 256, 0, c->stream>>>(g.nx, g.ny, g.nz, cur[l + 1], m.g[l + 1].nx,
                                                                       m.g[l + 1].ny, m.g[l + 1].nz, cur[l]);
         NNX_LAUNCHED(c);
         NNX_TRY(more(l, m.nu));
