@@ -474,7 +474,7 @@ def main():
     fused_roof = None
     if args.couple == "fused":
         ev_per_step = spikes / args.steps * net.nnz / net.n
-        fb = 48.125 * net.n + 24.0 * ev_per_step
+        fb = 48.125 * net.n + 32.0 * ev_per_step
         us = ev_ms.value * 1e3 / args.steps
         fused_roof = {"kernel": "k_ens_step<%d> (fused gather + RK4 + push)" % tune["block"], "bound": "hbm",
                       "achieved": fb / (us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -483,7 +483,7 @@ def main():
                                         "calibrated on a 256 MiB copy, quiet-step median)",
                       "avg_launch_us": us,
                       "alg_bytes_per_launch": fb,
-                      "alg_bytes_rule": "48.125 B per neuron + 24 B per delivered event"}
+                      "alg_bytes_rule": "48.125 B per neuron + 32 B per delivered event"}
 
     # the HH RK4 kernel of the unfused path, timed alone on the library
     # stream over the same state
