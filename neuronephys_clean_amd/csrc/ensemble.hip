@@ -135,6 +135,10 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
         for (int32_t i0 = tl.x; i0 < tl.y; i0 += B) {
             const int32_t i = i0 + (int32_t)threadIdx.x;
             const bool live = i < tl.y;
+            // the state is loaded first, so that its latency overlaps the
+            // event gather's
+            float v = 0.f, m = 0.f, h = 0.f, ng = 0.f, s = 0.f, ie = 0.f;
+            if (live) v = d.V[i], m = d.M[i], h = d.H[i], ng = d.NG[i], s = d.S[i], ie = d.iext[i];
             // coupling input
             float g = 0.f;
             const int32_t wave0 = i - lane;
@@ -159,9 +163,8 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
             // RK4 update and spike
             bool spike = false;
             if (live) {
-                float v = d.V[i], m = d.M[i], h = d.H[i], ng = d.NG[i], s = d.S[i];
                 const float v0 = v;
-                rk4(p, v, m, h, ng, s, d.iext[i], g);
+                rk4(p, v, m, h, ng, s, ie, g);
                 d.V[i] = v;
                 d.M[i] = m;
                 d.H[i] = h;
