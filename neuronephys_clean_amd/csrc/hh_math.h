@@ -63,18 +63,14 @@ template <> __device__ __forceinline__ float dfma(float a, float b, float c) { r
 template <> __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <class T> __device__ __forceinline__ T dabs(T a) { return a < T(0) ? -a : a; }
 
-// a / (1 - e^-a) given ea = e^-a; for |a| < 1/2 the series 1 + a/2 + a^2/12
-// - a^4/720 + a^6/30240 - a^8/1209600 (Bernoulli numbers of x/(e^x - 1) at
-// x = -a) is selected instead, which removes the 0/0 at a = 0.  Both are
-// evaluated (no divergent branch).
-template <class T> __device__ __forceinline__ T vtrap(T a, T ea) {
+// The Bernoulli series of a / (1 - e^-a): 1 + a/2 + a^2/12 - a^4/720 +
+// a^6/30240 - a^8/1209600, used for |a| < 1/2 (it removes the 0/0 at a = 0).
+template <class T> __device__ __forceinline__ T vtrap_ser(T a) {
     const T a2 = a * a;
     T q = dfma(a2, T(-1.0 / 1209600.0), T(1.0 / 30240.0));
     q = dfma(a2, q, T(-1.0 / 720.0));
     q = dfma(a2, q, T(1.0 / 12.0));
-    const T ser = dfma(a2, q, dfma(a, T(0.5), T(1)));
-    const T dir = a / (T(1) - ea);
-    return dabs(a) < T(0.5) ? ser : dir;
+    return dfma(a2, q, dfma(a, T(0.5), T(1)));
 }
 
 template <class T> struct Rates { T am, bm, ah, bh, an, bn; };
@@ -85,14 +81,33 @@ template <class T> struct Rates { T am, bm, ah, bh, an, bn; };
 //   an = 0.1 b/(1-e^-b); bh = 1/(e^(-3-0.1u) + 1) = 1/(ea e^0.5 + 1);
 //   w = u + 60:  bn = 0.125 e^(-w/80), ah = 0.07 e^(-u/20-3) = 0.07 (e^(-w/80))^4,
 //   bm = 4 e^(-w/18).
+// The three quotients share one division: with d1 = 1 - e^-a, d2 = 1 - e^-b,
+// d3 = ea e^0.5 + 1 (a denominator replaced by 1 where its series is used)
+// and inv = 1 / (d1 d2 d3): am = a (d2 d3) inv, an = 0.1 b (d1 d3) inv,
+// bh = (d1 d2) inv.  Below a = -20 (V near -240 mV and lower) the product
+// could overflow, and three divisions are used.
 template <class T> __device__ __forceinline__ Rates<T> rates(T u) {
     Rates<T> R;
     const T a = T(3.5) + T(0.1) * u;
     const T ea = dexp(-a);
-    R.am = vtrap(a, ea);
     const T b = a + T(1.5);
-    R.an = T(0.1) * vtrap(b, ea * T(0x1.c8f87724b5c1dp-3));
-    R.bh = T(1) / (ea * T(0x1.a61298e1e069cp+0) + T(1));
+    const T d1 = T(1) - ea;
+    const T d2 = T(1) - ea * T(0x1.c8f87724b5c1dp-3);
+    const T d3 = ea * T(0x1.a61298e1e069cp+0) + T(1);
+    const bool sa = dabs(a) < T(0.5), sb = dabs(b) < T(0.5);
+    if (a > T(-20)) {
+        const T e1 = sa ? T(1) : d1, e2 = sb ? T(1) : d2;
+        const T e12 = e1 * e2;
+        const T inv = T(1) / (e12 * d3);
+        const T t1 = e2 * d3, t2 = e1 * d3;
+        R.am = sa ? vtrap_ser(a) : (a * t1) * inv;
+        R.an = T(0.1) * (sb ? vtrap_ser(b) : (b * t2) * inv);
+        R.bh = e12 * inv;
+    } else {
+        R.am = a / d1;
+        R.an = T(0.1) * (b / d2);
+        R.bh = T(1) / d3;
+    }
     const T w = u + T(60);
     const T e80 = dexp(w * T(-0.0125));
     const T e40 = e80 * e80;
