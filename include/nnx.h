@@ -72,6 +72,8 @@ enum { NNX_COUPLE_WEIGHTED_PULL = 0, NNX_COUPLE_WEIGHTED_EVENT = 1,
 enum { NNX_STENCIL_7 = 7, NNX_STENCIL_27 = 27 };
 enum { NNX_PRECOND_NONE = 0, NNX_PRECOND_JACOBI = 1, NNX_PRECOND_LEARNED = 2, NNX_PRECOND_MG = 3,
        NNX_PRECOND_MG_LEARNED = 4 };
+/* multigrid smoothers */
+enum { NNX_MG_CHEBYSHEV = 0, NNX_MG_JACOBI = 1 };
 
 /* ------------------------------------------------------------------------ */
 /* context                                                                   */
@@ -279,8 +281,8 @@ typedef struct nnx_cg_opts {
     int32_t use_x0;       /* 1: start from x; 0: start from 0 */
     int32_t stall_iters;  /* flexible CG: fall back to Jacobi after this many non-improving iterations */
     int32_t smooth;       /* learned preconditioner: damped-Jacobi sweeps after the network (0: 1) */
-    int32_t mg_nu;        /* multigrid: damped-Jacobi sweeps before and after each level (0: 4) */
-    int32_t reserved_;
+    int32_t mg_nu;        /* multigrid: smoothing sweeps before and after each level (0: 3) */
+    int32_t mg_smoother;  /* NNX_MG_CHEBYSHEV (0) or NNX_MG_JACOBI */
 } nnx_cg_opts;
 
 typedef struct nnx_cg_result {
@@ -464,9 +466,11 @@ nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny,
  * 2x2x2 coarsening while every dimension is even and >= 8; a coarse face
  * carries 1/16 of the sum of the 4 fine faces it covers; block-mean
  * restriction of the residual; trilinear prolongation (zero beyond the
- * boundary); nu damped-Jacobi sweeps (w = 0.8) before and after each level
- * (nu = 0: 4), 16 on the coarsest.  No counterpart in the reference.        */
-nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, const float* d_r, float* d_z);
+ * boundary); nu smoothing sweeps before and after each level (nu = 0: 3):
+ * smoother NNX_MG_CHEBYSHEV, a Chebyshev polynomial in D^-1 A on [1/8, 2],
+ * or NNX_MG_JACOBI, damped Jacobi (w = 0.8); 16 damped-Jacobi sweeps on the
+ * coarsest level.  No counterpart in the reference.                         */
+nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, int smoother, const float* d_r, float* d_z);
 
 /* Flexible CG (Notay, one stored direction): fp32 vectors, FP64 reductions,
  * precond LEARNED: a two-level preconditioner with the network N as the

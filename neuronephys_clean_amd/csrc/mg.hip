@@ -10,8 +10,10 @@
 // of the 8 times larger cell), with the 4 parallel fine conductances added.
 // Restriction is the block mean of the residual, prolongation trilinear
 // between cell centres with zero beyond the boundary (the fine operator's
-// own boundary treatment), smoothing damped Jacobi (nu sweeps before and
-// after; 4 by default), the coarsest level 16 sweeps (one workgroup when
+// own boundary treatment), smoothing nu sweeps before and after (3 by
+// default) of a Chebyshev polynomial in D^-1 A on [1/8, 2] (2 bounds the
+// spectrum of D^-1 A for these diagonally dominant operators) or of damped
+// Jacobi, the coarsest level 16 damped-Jacobi sweeps (one workgroup when
 // the level is small).  Block-mean restriction with trilinear prolongation
 // is not a symmetric pair, so the cycle is used inside flexible CG.
 //
@@ -77,6 +79,29 @@ __global__ __launch_bounds__(256) void k_mg_smooth(GridK g, const float* __restr
     }
     for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
         xout[id] = xin[id] + w * dinv[id] * (b[id] - apply_at<float, float>(xin, g, i, j, k));
+    });
+}
+
+// Chebyshev sweep: d = c1 d + c2 D^-1 (b - A xin), xout = xin + d; xin NULL
+// means x = 0 (and c1 = 0 means d is not read)
+__global__ __launch_bounds__(256) void k_mg_cheb(GridK g, const float* __restrict__ dinv, const float* __restrict__ b,
+                                                 const float* __restrict__ xin, float* __restrict__ xout,
+                                                 float* __restrict__ d, float c1, float c2) {
+    if (!xin) {
+        const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+        for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
+            const float dn = c2 * dinv[id] * b[id];
+            d[id] = dn;
+            xout[id] = dn;
+        }
+        return;
+    }
+    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
+        const float res = b[id] - apply_at<float, float>(xin, g, i, j, k);
+        const float t = c2 * dinv[id] * res;
+        const float dn = c1 != 0.0f ? c1 * d[id] + t : t;
+        d[id] = dn;
+        xout[id] = xin[id] + dn;
     });
 }
 
@@ -153,9 +178,27 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(int nx, int ny, int nz, 
 
 }  // namespace
 
-nnx_status mg_setup(nnx_ctx* c, const GridK& g0, int nu, Mg* m) {
+nnx_status mg_setup(nnx_ctx* c, const GridK& g0, int nu, int smoother, Mg* m) {
     NNX_CHECK(c, g0.kind == 1, "multigrid needs the variable-coefficient 7-point operator");
-    m->nu = nu > 0 ? nu : 4;
+    NNX_CHECK(c, nu >= 0 && nu <= 16, "nu must be in [0, 16]");
+    NNX_CHECK(c, smoother == NNX_MG_CHEBYSHEV || smoother == NNX_MG_JACOBI, "unknown smoother");
+    m->nu = nu > 0 ? nu : 3;
+    m->smoother = smoother;
+    // Chebyshev polynomial on [lmax/16, lmax] with lmax = 2, the bound of
+    // D^-1 A for these diagonally dominant operators (Gershgorin)
+    {
+        const double lmax = 2.0, lmin = lmax / 16.0;
+        const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sig = th / de;
+        double rho = 1.0 / sig;
+        m->c1[0] = 0.0f;
+        m->c2[0] = (float)(1.0 / th);
+        for (int k = 1; k < 16; ++k) {
+            const double r1 = 1.0 / (2.0 * sig - rho);
+            m->c1[k] = (float)(r1 * rho);
+            m->c2[k] = (float)(2.0 * r1 / de);
+            rho = r1;
+        }
+    }
     m->g[0] = g0;
     int l = 0;
     while (l + 1 < kMgMaxLevels) {
@@ -179,12 +222,13 @@ nnx_status mg_setup(nnx_ctx* c, const GridK& g0, int nu, Mg* m) {
         char name[32];
         void* p;
         std::snprintf(name, sizeof name, "mg_v%d", q);
-        NNX_TRY(scratch(c, name, (size_t)(q == 0 ? 2 : 4) * n * 4, false, &p));
+        NNX_TRY(scratch(c, name, (size_t)(q == 0 ? 3 : 5) * n * 4, false, &p));
         float* f = (float*)p;
         m->dinv[q] = f;
         m->t[q] = f + n;
-        m->x[q] = q ? f + 2 * n : nullptr;
-        m->b[q] = q ? f + 3 * n : nullptr;
+        m->d[q] = f + 2 * n;
+        m->x[q] = q ? f + 3 * n : nullptr;
+        m->b[q] = q ? f + 4 * n : nullptr;
     }
     for (int q = 1; q < m->nlev; ++q) {
         const GridK& f = m->g[q - 1];
@@ -207,26 +251,32 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
     const float* B[kMgMaxLevels];
     float* cur[kMgMaxLevels];
     auto n_of = [&](int l) { return (int64_t)m.g[l].nx * m.g[l].ny * m.g[l].nz; };
+    // one smoothing sweep number s of a phase: xin NULL means x = 0
+    auto sweep = [&](int l, int s, const float* xin, float* xout, bool jacobi) -> nnx_status {
+        const unsigned blocks = xin ? row_blocks(m.g[l].ny, m.g[l].nz) : grid_blocks(n_of(l));
+        if (jacobi)
+            k_mg_smooth<<<blocks, 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], xin, xout, m.w, xin == nullptr);
+        else
+            k_mg_cheb<<<blocks, 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], xin, xout, m.d[l], m.c1[s], m.c2[s]);
+        NNX_LAUNCHED(c);
+        return NNX_OK;
+    };
     // sweeps from zero: the first writes `first`, later ones ping-pong
-    auto sweeps = [&](int l, int count, float* first, float* other) -> nnx_status {
-        const int64_t n = n_of(l);
+    auto sweeps = [&](int l, int count, float* first, float* other, bool jacobi) -> nnx_status {
         float *src = nullptr, *dst = first;
         for (int s = 0; s < count; ++s) {
-            k_mg_smooth<<<s == 0 ? grid_blocks(n) : row_blocks(m.g[l].ny, m.g[l].nz), 256, 0, c->stream>>>(
-                m.g[l], m.dinv[l], B[l], src, dst, m.w, s == 0);
-            NNX_LAUNCHED(c);
+            NNX_TRY(sweep(l, s, src, dst, jacobi));
             src = dst;
             dst = (dst == first) ? other : first;
         }
         cur[l] = src;
         return NNX_OK;
     };
+    // sweeps from the current iterate
     auto more = [&](int l, int count) -> nnx_status {
         float* other = (cur[l] == m.t[l]) ? (l ? m.x[l] : z) : m.t[l];
         for (int s = 0; s < count; ++s) {
-            k_mg_smooth<<<row_blocks(m.g[l].ny, m.g[l].nz), 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], cur[l],
-                                                                                 other, m.w, 0);
-            NNX_LAUNCHED(c);
+            NNX_TRY(sweep(l, s, cur[l], other, m.smoother == NNX_MG_JACOBI));
             std::swap(cur[l], other);
         }
         return NNX_OK;
@@ -236,7 +286,7 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
     const int L = m.nlev - 1;
     // down: at level 0 start in t so that 2 nu - 1 swaps end in z
     for (int l = 0; l < L; ++l) {
-        NNX_TRY(sweeps(l, m.nu, m.t[l], l ? m.x[l] : z));
+        NNX_TRY(sweeps(l, m.nu, m.t[l], l ? m.x[l] : z, m.smoother == NNX_MG_JACOBI));
         const GridK& g = m.g[l];
         k_mg_restrict<<<row_blocks(m.g[l + 1].ny, m.g[l + 1].nz), 256, 0, c->stream>>>(
             g, B[l], cur[l], m.b[l + 1], m.g[l + 1].nx, m.g[l + 1].ny, m.g[l + 1].nz);
@@ -247,7 +297,7 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
         NNX_LAUNCHED(c);
         cur[L] = (m.coarse_sweeps & 1) ? m.x[L] : m.t[L];
     } else {
-        NNX_TRY(sweeps(L, m.coarse_sweeps, m.x[L] ? m.x[L] : z, m.t[L]));
+        NNX_TRY(sweeps(L, m.coarse_sweeps, m.x[L] ? m.x[L] : z, m.t[L], true));
     }
     // up
     for (int l = L - 1; l >= 0; --l) {
@@ -265,12 +315,13 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
 
 using namespace nnx;
 
-extern "C" nnx_status nnx_mg_apply(nnx_ctx* c, const nnx_grid* g, int nu, const float* d_r, float* d_z) {
+extern "C" nnx_status nnx_mg_apply(nnx_ctx* c, const nnx_grid* g, int nu, int smoother, const float* d_r,
+                                   float* d_z) {
     if (!c) return NNX_ERR_ARG;
     NNX_CHECK(c, g && d_r && d_z, "null argument");
     NNX_CHECK(c, g->stencil == NNX_STENCIL_7 && g->kx && g->ky && g->kz, "multigrid needs a 7-point grid with kx, ky, kz");
     NNX_CHECK(c, g->nx > 0 && g->ny > 0 && g->nz > 0, "bad dims");
     Mg m;
-    NNX_TRY(mg_setup(c, make_grid(g), nu, &m));
+    NNX_TRY(mg_setup(c, make_grid(g), nu, smoother, &m));
     return mg_vcycle(c, m, d_r, d_z);
 }

@@ -53,21 +53,49 @@ def prolong(e):
     return _prolong_axis(_prolong_axis(_prolong_axis(e, 2), 1), 0)
 
 
-def vcycle(r, kx, ky, kz, nu=2, w=0.8, coarse_sweeps=16):
-    """One V-cycle from zero: nu damped-Jacobi sweeps before and after each
-    level, coarsening while every dimension is even and >= 8."""
+def chebyshev_coefficients(m, lmax=2.0, lmin=2.0 / 16):
+    """Three-term recurrence of the Chebyshev polynomial on [lmin, lmax]:
+    sweep s sets d = c1 d + c2 D^-1 r and x = x + d."""
+    th, de = (lmax + lmin) / 2, (lmax - lmin) / 2
+    sig = th / de
+    rho = 1 / sig
+    out = [(0.0, 1 / th)]
+    for _ in range(1, m):
+        r1 = 1 / (2 * sig - rho)
+        out.append((r1 * rho, 2 * r1 / de))
+        rho = r1
+    return out
+
+
+def _smooth(x, r, kx, ky, kz, dinv, nu, w, smoother):
+    """nu sweeps from x (None: from zero)."""
+    if smoother == "jacobi":
+        if x is None:
+            x = w * dinv * r
+            nu -= 1
+        for _ in range(nu):
+            x = x + w * dinv * (r - apply(x, kx, ky, kz))
+        return x
+    d = None
+    for c1, c2 in chebyshev_coefficients(nu):
+        res = r if x is None else r - apply(x, kx, ky, kz)
+        d = c2 * dinv * res if d is None else c1 * d + c2 * dinv * res
+        x = d if x is None else x + d
+    return x
+
+
+def vcycle(r, kx, ky, kz, nu=2, w=0.8, coarse_sweeps=16, smoother="jacobi"):
+    """One V-cycle from zero: nu sweeps of the smoother (damped Jacobi or
+    Chebyshev) before and after each level, coarsening while every
+    dimension is even and >= 8, coarse_sweeps damped-Jacobi sweeps on the
+    coarsest level."""
     nz, ny, nx = r.shape
     dinv = 1.0 / diag(kx, ky, kz)
     can = nx % 2 == 0 and ny % 2 == 0 and nz % 2 == 0 and min(nx, ny, nz) >= 8
-    sweeps = nu if can else coarse_sweeps
-    x = w * dinv * r
-    for _ in range(sweeps - 1):
-        x = x + w * dinv * (r - apply(x, kx, ky, kz))
     if not can:
-        return x
+        return _smooth(None, r, kx, ky, kz, dinv, coarse_sweeps, w, "jacobi")
+    x = _smooth(None, r, kx, ky, kz, dinv, nu, w, smoother)
     rc = restrict(r - apply(x, kx, ky, kz))
-    ec = vcycle(rc, *coarsen(kx, ky, kz), nu=nu, w=w, coarse_sweeps=coarse_sweeps)
+    ec = vcycle(rc, *coarsen(kx, ky, kz), nu=nu, w=w, coarse_sweeps=coarse_sweeps, smoother=smoother)
     x = x + prolong(ec)
-    for _ in range(nu):
-        x = x + w * dinv * (r - apply(x, kx, ky, kz))
-    return x
+    return _smooth(x, r, kx, ky, kz, dinv, nu, w, smoother)

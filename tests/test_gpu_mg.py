@@ -33,8 +33,9 @@ def const_field(n, k):
 
 
 @pytest.mark.parametrize("dims", [(32, 32, 32), (32, 16, 24), (12, 10, 8), (64, 64, 64)])
-@pytest.mark.parametrize("nu", [1, 2])
-def test_vcycle_matches_numpy_restatement(ctx, dims, nu):
+@pytest.mark.parametrize("nu", [1, 2, 4])
+@pytest.mark.parametrize("smoother", ["chebyshev", "jacobi"])
+def test_vcycle_matches_numpy_restatement(ctx, dims, nu, smoother):
     """nnx_mg_apply against tests/mg_ref.py (FP64) on log-normal faces,
     including grids whose coarsening stops early (odd or small dims)."""
     nx, ny, nz = dims
@@ -46,17 +47,19 @@ def test_vcycle_matches_numpy_restatement(ctx, dims, nu):
     d = [DA.from_numpy(ctx, a.ravel()) for a in (kx, ky, kz, r)]
     z = DA.zeros(ctx, r.size, np.float32)
     g = _lib.Grid(nx, ny, nz, 7, 0.0, 0.0, d[0].ptr, d[1].ptr, d[2].ptr)
-    ctx.call("nnx_mg_apply", C.byref(g), nu, d[3].ptr, z.ptr)
+    sm = _lib.MG_CHEBYSHEV if smoother == "chebyshev" else _lib.MG_JACOBI
+    ctx.call("nnx_mg_apply", C.byref(g), nu, sm, d[3].ptr, z.ptr)
     ref = mg_ref.vcycle(r.astype(np.float64), kx.astype(np.float64), ky.astype(np.float64),
-                        kz.astype(np.float64), nu=nu)
+                        kz.astype(np.float64), nu=nu, smoother=smoother)
     got = z.numpy().reshape(nz, ny, nx)
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    say("dims", dims, "nu", nu, "max rel diff", err)
+    say("dims", dims, "nu", nu, smoother, "max rel diff", err)
     assert err < 1e-4
 
 
 @pytest.mark.parametrize("kind", ["smooth", "rough"])
-def test_vcycle_contracts_energy_norm(ctx, kind):
+@pytest.mark.parametrize("smoother", [0, 1])
+def test_vcycle_contracts_energy_norm(ctx, kind, smoother):
     n = 64
     f = configs.c5_field(seed=8, n=n)
     g, keep = device_field(ctx, f)
@@ -72,10 +75,10 @@ def test_vcycle_contracts_energy_norm(ctx, kind):
     Ae = apply_A(ctx, g, e)
     r, z = DA.from_numpy(ctx, Ae.astype(np.float32)), DA.zeros(ctx, n ** 3, np.float32)
     for nu in (1, 2):
-        ctx.call("nnx_mg_apply", C.byref(g), nu, r.ptr, z.ptr)
+        ctx.call("nnx_mg_apply", C.byref(g), nu, smoother, r.ptr, z.ptr)
         e1 = e - z.numpy().astype(np.float64)
         ratio = np.sqrt(e1 @ apply_A(ctx, g, e1) / (e @ Ae))
-        say(kind, "nu", nu, "energy-norm contraction", ratio)
+        say(kind, "smoother", smoother, "nu", nu, "energy-norm contraction", ratio)
         assert ratio < (0.6 if nu == 1 else 0.45)
 
 

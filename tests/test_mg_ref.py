@@ -65,3 +65,37 @@ def test_vcycle_converges_on_lognormal_field():
     rate = _rate(kx, ky, kz, 32)
     print("log-normal rate", rate)
     assert rate < 0.5
+
+
+def test_chebyshev_recurrence_is_the_chebyshev_residual_polynomial():
+    """m sweeps from zero on a scalar equation lambda x = 1 leave the
+    residual 1 - lambda x = T_m((theta - lambda)/delta) / T_m(theta/delta)
+    (theta, delta: centre and half-width of [lmax/16, lmax])."""
+    lmax, lmin = 2.0, 2.0 / 16
+    th, de = (lmax + lmin) / 2, (lmax - lmin) / 2
+    for m in (1, 2, 4, 7):
+        co = mg_ref.chebyshev_coefficients(m)
+        for lam in np.linspace(lmin, lmax, 9):
+            x, d = 0.0, None
+            for c1, c2 in co:
+                res = 1.0 - lam * x
+                d = c2 * res if d is None else c1 * d + c2 * res
+                x = x + d
+            T = np.polynomial.chebyshev.Chebyshev.basis(m)
+            assert np.isclose(1.0 - lam * x, T((th - lam) / de) / T(th / de), rtol=1e-9, atol=1e-12)
+
+
+def test_chebyshev_vcycle_rate():
+    rates = {}
+    for n in (16, 32):
+        k = faces(n, 50.0)
+        rng = np.random.default_rng(1)
+        b = rng.normal(size=(n, n, n))
+        x = np.zeros_like(b)
+        res = [np.linalg.norm(b)]
+        for _ in range(6):
+            x = x + mg_ref.vcycle(b - mg_ref.apply(x, *k), *k, nu=4, smoother="chebyshev")
+            res.append(np.linalg.norm(b - mg_ref.apply(x, *k)))
+        rates[n] = (res[-1] / res[2]) ** 0.25
+    print("chebyshev rates", rates)
+    assert max(rates.values()) < 0.2
