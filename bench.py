@@ -168,12 +168,12 @@ def precision_split(ctx, net, steps=(1, 100, 10000)):
     return res
 
 
-def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=1000):
+def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=0):
     """Batched ensemble: 256 networks (sizes 2^U(10,16), Poisson(50)
     out-degree), one fused persistent launch per step, 100 steps per
     hipGraph, launch shape from the autotuner (whose steps are ordinary
-    steps, run after `warmup` steps - once the networks fire - and before
-    the timed region)."""
+    steps, run after `warmup` steps and before the timed region: the timed
+    5,000 steps are the simulation's first after tuning)."""
     from neuronephys_clean_amd import configs
     from neuronephys_clean_amd.sim import EnsembleSim
     t0 = time.time()
@@ -182,7 +182,8 @@ def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=1000):
     log(f"bench: C4 ensemble {len(ens.size)} networks, {live} neurons, {ens.nnz} edges, "
         f"built in {time.time() - t0:.1f}s")
     sim = EnsembleSim(ens, ctx=ctx, deposit=True)
-    sim.run(warmup, use_graph=True, chunk=chunk)
+    if warmup:
+        sim.run(warmup, use_graph=True, chunk=chunk)
     best, tune_us, ran = sim.autotune(tune_steps)
     log(f"bench: C4 autotune -> {best.as_dict()} ({tune_us:.1f} us/step, {ran} steps)")
     sim.run(chunk - sim.step % chunk, use_graph=False)  # align the graph chunks
