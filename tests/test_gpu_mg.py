@@ -138,3 +138,22 @@ def test_fcg_mg_refined_and_mg_learned(ctx):
         rel = np.linalg.norm(res) / np.linalg.norm(b)
         say("host FP64 residual", rel)
         assert rel < 2e-10
+
+
+def test_multigrid_argument_errors(ctx):
+    """Multigrid needs the variable-coefficient operator and a known
+    smoother; the errors come back as status codes, not faults."""
+    n = 16
+    b, x = DA.from_numpy(ctx, np.ones(n ** 3, np.float32)), DA.zeros(ctx, n ** 3, np.float32)
+    g = _lib.Grid(n, n, n, 7, 1.0, 0.0, None, None, None)
+    r = _lib.CgResult()
+    o = _lib.CgOpts(rtol=1e-6, maxit=100, precond=_lib.PRECOND_MG, check_every=1)
+    st = ctx._l.nnx_fcg_solve(ctx.h, C.byref(g), None, None, b.ptr, x.ptr, C.byref(o), C.byref(r))
+    assert st == 1 and "variable-coefficient" in ctx._l.nnx_last_error(ctx.h).decode()
+    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(g), 2, 0, b.ptr, x.ptr) == 1
+    k = DA.from_numpy(ctx, const_field(n, 5.0))
+    gv = _lib.Grid(n, n, n, 7, 0.0, 0.0, k.ptr, k.ptr, k.ptr)
+    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(gv), 2, 7, b.ptr, x.ptr) == 1
+    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(gv), 17, 0, b.ptr, x.ptr) == 1
+    o2 = _lib.CgOpts(rtol=1e-6, maxit=100, precond=_lib.PRECOND_MG, check_every=1, mg_smoother=5)
+    assert ctx._l.nnx_fcg_solve(ctx.h, C.byref(gv), None, None, b.ptr, x.ptr, C.byref(o2), C.byref(r)) == 1
