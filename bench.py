@@ -277,7 +277,7 @@ def c5_bench(ctx, counts, smooth=2):
                          "peak_TFLOPs": BF16_PEAK / 1e12, "mfma_frac": flop / (ms * 1e-3) / BF16_PEAK}
 
     def f_mg():
-        ctx.call("nnx_mg_apply", C.byref(g), 0, 0, r32.ptr, z32.ptr)
+        ctx.call("nnx_mg_apply", C.byref(g), 0, 0, 0, r32.ptr, z32.ptr)
     out["mg_vcycle_ms"] = timed_events(ctx, f_mg, 10)
     net.close()
     if net_mg is not None:
@@ -341,7 +341,7 @@ def profile_pass(ctx, sim, net):
     from neuronephys_clean_amd.precond import device_field
     fg, keep = device_field(ctx, configs.c5_field(seed=0, n=256))
     for _ in range(3):
-        ctx.call("nnx_mg_apply", C.byref(fg), 0, 0, rr.ptr, zz.ptr)
+        ctx.call("nnx_mg_apply", C.byref(fg), 0, 0, 0, rr.ptr, zz.ptr)
     ctx.synchronize()
     del keep
     vc = sim.a["voxel_counts"].numpy().astype(np.float32)

@@ -283,6 +283,8 @@ typedef struct nnx_cg_opts {
     int32_t smooth;       /* learned preconditioner: damped-Jacobi sweeps after the network (0: 1) */
     int32_t mg_nu;        /* multigrid: smoothing sweeps before and after each level (0: 3) */
     int32_t mg_smoother;  /* NNX_MG_CHEBYSHEV (0) or NNX_MG_JACOBI */
+    int32_t mg_cycle;     /* multigrid: visits of each coarser level, 1 V-cycle, 2 (or 0) W-cycle */
+    int32_t reserved_;
 } nnx_cg_opts;
 
 typedef struct nnx_cg_result {
@@ -461,7 +463,7 @@ nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny,
                                 const uint16_t* d_in, const uint16_t* d_coarse, double coarse_scale,
                                 const uint16_t* d_g, float* d_dw, float* d_db);
 
-/* z = V(r): one multigrid V-cycle from a zero initial guess for the
+/* z = M(r): one multigrid cycle from a zero initial guess for the
  * variable-coefficient 7-point operator of g (kx, ky, kz set).  Cell-centred
  * 2x2x2 coarsening while every dimension is even and >= 8; a coarse face
  * carries 1/16 of the sum of the 4 fine faces it covers; block-mean
@@ -469,8 +471,11 @@ nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny,
  * boundary); nu smoothing sweeps before and after each level (nu = 0: 3):
  * smoother NNX_MG_CHEBYSHEV, a Chebyshev polynomial in D^-1 A on [1/8, 2],
  * or NNX_MG_JACOBI, damped Jacobi (w = 0.8); 16 damped-Jacobi sweeps on the
- * coarsest level.  No counterpart in the reference.                         */
-nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, int smoother, const float* d_r, float* d_z);
+ * coarsest level.  cycle 1: V-cycle; 2 (or 0): W-cycle (each coarser level
+ * visited twice per visit of its parent, the second time on the residual
+ * the first visit leaves).  No counterpart in the reference.               */
+nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, int smoother, int cycle, const float* d_r,
+                        float* d_z);
 
 /* Flexible CG (Notay, one stored direction): fp32 vectors, FP64 reductions,
  * precond LEARNED: a two-level preconditioner with the network N as the
@@ -478,7 +483,7 @@ nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, int smoother, c
  * operator's diagonal): z0 = w D^-1 r, z1 = z0 + N(r - A z0), then
  * o->smooth sweeps z <- z + w D^-1 (r - A z) (at least one); d_feat is N's
  * feature input.
- * precond MG: one multigrid V-cycle (nnx_mg_apply with o->mg_nu).
+ * precond MG: one multigrid cycle (nnx_mg_apply with o->mg_nu, mg_smoother, mg_cycle).
  * precond MG_LEARNED: the V-cycle in place of z0 above: z0 = V(r),
  * z1 = z0 + N(r - A z0), then o->smooth damped-Jacobi sweeps.
  * precond JACOBI: M = D^-1;

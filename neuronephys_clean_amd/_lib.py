@@ -68,7 +68,8 @@ class Grid(C.Structure):
 class CgOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("maxit", C.c_int32), ("precond", C.c_int32),
                 ("check_every", C.c_int32), ("use_x0", C.c_int32), ("stall_iters", C.c_int32),
-                ("smooth", C.c_int32), ("mg_nu", C.c_int32), ("mg_smoother", C.c_int32)]
+                ("smooth", C.c_int32), ("mg_nu", C.c_int32), ("mg_smoother", C.c_int32),
+                ("mg_cycle", C.c_int32), ("reserved_", C.c_int32)]
 
 
 class CgResult(C.Structure):
@@ -163,7 +164,7 @@ _SIGS = {
     "nnx_ensemble_autotune": ([vp, _P(EnsPlan), C.c_uint64, C.c_int, _P(EnsConfig), _P(C.c_double),
                                _P(C.c_int)], C.c_int),
     "nnx_fcg_solve": ([vp, _P(Grid), vp, vp, vp, vp, _P(CgOpts), _P(CgResult)], C.c_int),
-    "nnx_mg_apply": ([vp, _P(Grid), C.c_int, C.c_int, vp, vp], C.c_int),
+    "nnx_mg_apply": ([vp, _P(Grid), C.c_int, C.c_int, C.c_int, vp, vp], C.c_int),
     "nnx_fcg_solve_refine": ([vp, _P(Grid), vp, vp, vp, vp, _P(CgOpts), C.c_double, C.c_int,
                               _P(CgResult)], C.c_int),
 }

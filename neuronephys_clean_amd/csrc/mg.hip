@@ -21,6 +21,7 @@
 // (src/paracrine.cu:106-129).
 #include <algorithm>
 #include <cstdio>
+#include <vector>
 
 #include "mg.h"
 
@@ -31,7 +32,6 @@ namespace {
 unsigned grid_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
 // row kernels: enough workgroups for one wavefront per x-row on small levels
 unsigned row_blocks(int ny, int nz) { return (unsigned)std::max(1, std::min((ny * nz + 3) / 4, 4096)); }
-constexpr int kCoarseOneBlock = 4096;  // coarsest levels up to this size are solved by one workgroup
 
 __global__ __launch_bounds__(256) void k_mg_dinv(GridK g, float* __restrict__ dinv) {
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
@@ -67,84 +67,33 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(GridK f, float* __restrict__
     }
 }
 
-// xout = zero ? w D^-1 b : xin + w D^-1 (b - A xin)
-__global__ __launch_bounds__(256) void k_mg_smooth(GridK g, const float* __restrict__ dinv,
-                                                   const float* __restrict__ b, const float* __restrict__ xin,
-                                                   float* __restrict__ xout, float w, int zero) {
-    if (zero) {
-        const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-        for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256)
-            xout[id] = w * dinv[id] * b[id];
-        return;
-    }
-    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
-        xout[id] = xin[id] + w * dinv[id] * (b[id] - apply_at<float, float>(xin, g, i, j, k));
-    });
+// ---- the steps of a cycle, per cell ----
+__device__ __forceinline__ void op_cheb(const GridK& g, const MgOp& o, int i, int j, int k, int64_t id) {
+    const float res = o.xin ? o.rhs[id] - apply_at<float, float>(o.xin, g, i, j, k) : o.rhs[id];
+    const float t = o.c2 * o.dinv[id] * res;
+    const float dn = o.c1 != 0.0f ? o.c1 * o.dvec[id] + t : t;
+    o.dvec[id] = dn;
+    o.xout[id] = o.xin ? o.xin[id] + dn : dn;
 }
-
-// Chebyshev sweep: d = c1 d + c2 D^-1 (b - A xin), xout = xin + d; xin NULL
-// means x = 0 (and c1 = 0 means d is not read)
-__global__ __launch_bounds__(256) void k_mg_cheb(GridK g, const float* __restrict__ dinv, const float* __restrict__ b,
-                                                 const float* __restrict__ xin, float* __restrict__ xout,
-                                                 float* __restrict__ d, float c1, float c2) {
-    if (!xin) {
-        const int64_t n = (int64_t)g.nx * g.ny * g.nz;
-        for (int64_t id = blockIdx.x * 256ll + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
-            const float dn = c2 * dinv[id] * b[id];
-            d[id] = dn;
-            xout[id] = dn;
-        }
-        return;
-    }
-    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
-        const float res = b[id] - apply_at<float, float>(xin, g, i, j, k);
-        const float t = c2 * dinv[id] * res;
-        const float dn = c1 != 0.0f ? c1 * d[id] + t : t;
-        d[id] = dn;
-        xout[id] = xin[id] + dn;
-    });
+__device__ __forceinline__ void op_jac(const GridK& g, const MgOp& o, int i, int j, int k, int64_t id) {
+    o.xout[id] = o.xin ? o.xin[id] + o.c1 * o.dinv[id] * (o.rhs[id] - apply_at<float, float>(o.xin, g, i, j, k))
+                       : o.c1 * o.dinv[id] * o.rhs[id];
 }
-
-// bc = block mean of (b - A x) over each coarse cell's 8 children
-__global__ __launch_bounds__(256) void k_mg_restrict(GridK g, const float* __restrict__ b,
-                                                     const float* __restrict__ x, float* __restrict__ bc, int NX,
-                                                     int NY, int NZ) {
+// coarse cell (I, J, K): block mean of the fine residual over its 8 children
+__device__ __forceinline__ void op_restrict(const GridK& g, const MgOp& o, int I, int J, int K, int64_t q) {
     const int64_t nx = g.nx, ny = g.ny;
-    for_rows(NX, NY, NZ, [&](int I, int J, int K, int64_t q) {
-        float s = 0.f;
+    float s = 0.f;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int i = 2 * I + (c & 1), j = 2 * J + ((c >> 1) & 1), k = 2 * K + (c >> 2);
-            const int64_t id = i + nx * (j + ny * k);
-            s = s + (b[id] - apply_at<float, float>(x, g, i, j, k));
-        }
-        bc[q] = 0.125f * s;
-    });
-}
-
-// all `sweeps` damped-Jacobi sweeps of a small level from zero in one
-// workgroup (x and t ping-pong, the result ends in x or t as returned by
-// the host's parity bookkeeping: sweep s writes x for even s, t for odd s)
-__global__ __launch_bounds__(1024) void k_mg_coarse(GridK g, const float* __restrict__ dinv,
-                                                    const float* __restrict__ b, float* __restrict__ x,
-                                                    float* __restrict__ t, float w, int sweeps) {
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int n = nx * ny * nz;
-    for (int s = 0; s < sweeps; ++s) {
-        const float* src = (s & 1) ? x : t;
-        float* dst = (s & 1) ? t : x;
-        for (int id = threadIdx.x; id < n; id += blockDim.x) {
-            if (s == 0) {
-                dst[id] = w * dinv[id] * b[id];
-            } else {
-                const int i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
-                dst[id] = src[id] + w * dinv[id] * (b[id] - apply_at<float, float>(src, g, i, j, k));
-            }
-        }
-        __syncthreads();
+    for (int c = 0; c < 8; ++c) {
+        const int i = 2 * I + (c & 1), j = 2 * J + ((c >> 1) & 1), k = 2 * K + (c >> 2);
+        const int64_t id = i + nx * (j + ny * k);
+        s = s + (o.rhs[id] - apply_at<float, float>(o.xin, g, i, j, k));
     }
+    o.xout[q] = 0.125f * s;
 }
-
+__device__ __forceinline__ void op_resid(const GridK& g, const MgOp& o, int i, int j, int k, int64_t id) {
+    o.xout[id] = o.xout[id] - apply_at<float, float>(o.xin, g, i, j, k);
+}
 // cell-centred linear weights along one axis: fine cell i sits at coarse
 // coordinate i/2 - 1/4: 3/4 on its parent, 1/4 on the parent's neighbour
 // toward it (none beyond the boundary)
@@ -154,36 +103,145 @@ __device__ __forceinline__ void lin_w(int i, int N, int& a, int& b, float& wb) {
     wb = (b >= 0 && b < N) ? 0.25f : 0.0f;
     if (b < 0 || b >= N) b = a;
 }
+// x += P e (trilinear, zero outside); g fine, (NX, NY, NZ) coarse dims
+__device__ __forceinline__ void op_prolong(const MgOp& o, int NX, int NY, int NZ, int i, int j, int k, int64_t id) {
+    int ia, ib, ja, jb, ka, kb;
+    float wi, wj, wk;
+    lin_w(i, NX, ia, ib, wi);
+    lin_w(j, NY, ja, jb, wj);
+    lin_w(k, NZ, ka, kb, wk);
+    const float* e = o.xin;
+    auto E = [&](int I, int J, int K) { return e[I + (int64_t)NX * (J + (int64_t)NY * K)]; };
+    const float ai = 0.75f, aj = 0.75f, ak = 0.75f;
+    const float v0 = ai * E(ia, ja, ka) + wi * E(ib, ja, ka);
+    const float v1 = ai * E(ia, jb, ka) + wi * E(ib, jb, ka);
+    const float v2 = ai * E(ia, ja, kb) + wi * E(ib, ja, kb);
+    const float v3 = ai * E(ia, jb, kb) + wi * E(ib, jb, kb);
+    const float u0 = aj * v0 + wj * v1;
+    const float u1 = aj * v2 + wj * v3;
+    o.xout[id] = o.xout[id] + (ak * u0 + wk * u1);
+}
 
-// x += P e (trilinear, zero outside)
-__global__ __launch_bounds__(256) void k_mg_prolong_add(int nx, int ny, int nz, const float* __restrict__ e, int NX,
-                                                        int NY, int NZ, float* __restrict__ x) {
-    for_rows(nx, ny, nz, [&](int i, int j, int k, int64_t id) {
-        int ia, ib, ja, jb, ka, kb;
-        float wi, wj, wk;
-        lin_w(i, NX, ia, ib, wi);
-        lin_w(j, NY, ja, jb, wj);
-        lin_w(k, NZ, ka, kb, wk);
-        auto E = [&](int I, int J, int K) { return e[I + (int64_t)NX * (J + (int64_t)NY * K)]; };
-        const float ai = 0.75f, aj = 0.75f, ak = 0.75f;
-        const float v0 = ai * E(ia, ja, ka) + wi * E(ib, ja, ka);
-        const float v1 = ai * E(ia, jb, ka) + wi * E(ib, jb, ka);
-        const float v2 = ai * E(ia, ja, kb) + wi * E(ib, ja, kb);
-        const float v3 = ai * E(ia, jb, kb) + wi * E(ib, jb, kb);
-        const float u0 = aj * v0 + wj * v1;
-        const float u1 = aj * v2 + wj * v3;
-        x[id] = x[id] + (ak * u0 + wk * u1);
-    });
+template <int T> __device__ __forceinline__ void op_at(const GridK& g, const GridK& gc, const MgOp& o, int i, int j,
+                                                       int k, int64_t id) {
+    if (T == MG_CHEB) op_cheb(g, o, i, j, k, id);
+    if (T == MG_JAC) op_jac(g, o, i, j, k, id);
+    if (T == MG_RESTRICT) op_restrict(g, o, i, j, k, id);
+    if (T == MG_RESID) op_resid(g, o, i, j, k, id);
+    if (T == MG_PROLONG) op_prolong(o, gc.nx, gc.ny, gc.nz, i, j, k, id);
+    if (T == MG_ADD) o.xout[id] = o.xout[id] + o.xin[id];
+}
+
+// one step over a large level: a wavefront per x-row (RESTRICT: per coarse row)
+template <int T> __global__ __launch_bounds__(256) void k_mg_op(MgOp o, GridK g, GridK gc) {
+    const GridK& d = T == MG_RESTRICT ? gc : g;
+    for_rows(d.nx, d.ny, d.nz, [&](int i, int j, int k, int64_t id) { op_at<T>(g, gc, o, i, j, k, id); });
+}
+
+// a whole sequence of steps over small levels by one workgroup
+__global__ __launch_bounds__(1024) void k_mg_sub(const MgOp* __restrict__ ops, int nops, const GridK* __restrict__ gk) {
+    for (int q = 0; q < nops; ++q) {
+        const MgOp o = ops[q];
+        const GridK g = gk[o.level];
+        const GridK gc = gk[o.level + 1 < kMgMaxLevels ? o.level + 1 : o.level];
+        const GridK& d = o.type == MG_RESTRICT ? gc : g;
+        const int nx = d.nx, ny = d.ny, n = d.nx * d.ny * d.nz;
+        for (int id = threadIdx.x; id < n; id += blockDim.x) {
+            const int i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
+            switch (o.type) {
+                case MG_CHEB: op_at<MG_CHEB>(g, gc, o, i, j, k, id); break;
+                case MG_JAC: op_at<MG_JAC>(g, gc, o, i, j, k, id); break;
+                case MG_RESTRICT: op_at<MG_RESTRICT>(g, gc, o, i, j, k, id); break;
+                case MG_RESID: op_at<MG_RESID>(g, gc, o, i, j, k, id); break;
+                case MG_PROLONG: op_at<MG_PROLONG>(g, gc, o, i, j, k, id); break;
+                default: op_at<MG_ADD>(g, gc, o, i, j, k, id); break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The steps of one visit of level l (right-hand side rhs, buffer set `set`),
+// appended to ops; returns the buffer that holds the visit's result.  With
+// `host`, a visit of the level cycled in one workgroup is one MG_SUB marker
+// (type -1, c1 = set).  Level 0 starts its sweeps in t so that an even
+// number of sweeps ends in `z0`.
+float* gen_visit(const Mg& m, int l, const float* rhs, int set, std::vector<MgOp>& ops, bool host, float* z0) {
+    if (host && l == m.sub) {
+        MgOp o{};
+        o.type = -1, o.level = l, o.c1 = (float)set;
+        ops.push_back(o);
+        return m.sub_res[set];
+    }
+    float* X = l == 0 ? z0 : m.x[set][l];
+    float* T = m.t[set][l];
+    float *cur = nullptr, *oth = nullptr;
+    auto sweep = [&](int s, bool jacobi) {
+        MgOp o{};
+        o.level = l, o.rhs = rhs, o.dinv = m.dinv[l];
+        if (!cur) {
+            o.xin = nullptr;
+            o.xout = l == 0 ? T : X;
+            oth = l == 0 ? X : T;
+        } else {
+            o.xin = cur;
+            o.xout = oth;
+            oth = cur;
+        }
+        if (jacobi) {
+            o.type = MG_JAC;
+            o.c1 = m.w;
+        } else {
+            o.type = MG_CHEB;
+            o.dvec = m.d[l];
+            o.c1 = m.c1[s];
+            o.c2 = m.c2[s];
+        }
+        ops.push_back(o);
+        cur = o.xout;
+    };
+    if (l == m.nlev - 1) {
+        for (int s = 0; s < m.coarse_sweeps; ++s) sweep(s, true);
+        return cur;
+    }
+    const bool jac = m.smoother == NNX_MG_JACOBI;
+    for (int s = 0; s < m.nu; ++s) sweep(s, jac);
+    {
+        MgOp o{};
+        o.type = MG_RESTRICT, o.level = l, o.rhs = rhs, o.xin = cur, o.xout = m.b[l + 1];
+        ops.push_back(o);
+    }
+    float* e = gen_visit(m, l + 1, m.b[l + 1], 0, ops, host, z0);
+    // the W-cycle's second visit only for levels above the one-workgroup ones
+    const int gam = (m.sub < 0 || l + 1 < m.sub) ? m.gamma : 1;
+    for (int v = 1; v < gam; ++v) {
+        MgOp r{};
+        r.type = MG_RESID, r.level = l + 1, r.xin = e, r.xout = m.b[l + 1];
+        ops.push_back(r);
+        float* e2 = gen_visit(m, l + 1, m.b[l + 1], 1, ops, host, z0);
+        MgOp a{};
+        a.type = MG_ADD, a.level = l + 1, a.xin = e2, a.xout = e;
+        ops.push_back(a);
+    }
+    {
+        MgOp o{};
+        o.type = MG_PROLONG, o.level = l, o.xin = e, o.xout = cur;
+        ops.push_back(o);
+    }
+    for (int s = 0; s < m.nu; ++s) sweep(s, jac);
+    return cur;
 }
 
 }  // namespace
 
-nnx_status mg_setup(nnx_ctx* c, const GridK& g0, int nu, int smoother, Mg* m) {
+nnx_status mg_setup(nnx_ctx* c, const GridK& g0, int nu, int smoother, int gamma, Mg* m) {
     NNX_CHECK(c, g0.kind == 1, "multigrid needs the variable-coefficient 7-point operator");
     NNX_CHECK(c, nu >= 0 && nu <= 16, "nu must be in [0, 16]");
     NNX_CHECK(c, smoother == NNX_MG_CHEBYSHEV || smoother == NNX_MG_JACOBI, "unknown smoother");
+    NNX_CHECK(c, gamma >= 0 && gamma <= 2, "cycle must be 0 (W), 1 (V) or 2 (W)");
     m->nu = nu > 0 ? nu : 3;
     m->smoother = smoother;
+    m->gamma = gamma > 0 ? gamma : 2;
     // Chebyshev polynomial on [lmax/16, lmax] with lmax = 2, the bound of
     // D^-1 A for these diagonally dominant operators (Gershgorin)
     {
@@ -217,18 +275,21 @@ nnx_status mg_setup(nnx_ctx* c, const GridK& g0, int nu, int smoother, Mg* m) {
         ++l;
     }
     m->nlev = l + 1;
+    auto n_of = [&](int q) { return (int64_t)m->g[q].nx * m->g[q].ny * m->g[q].nz; };
     for (int q = 0; q < m->nlev; ++q) {
-        const int64_t n = (int64_t)m->g[q].nx * m->g[q].ny * m->g[q].nz;
+        const int64_t n = n_of(q);
         char name[32];
         void* p;
         std::snprintf(name, sizeof name, "mg_v%d", q);
-        NNX_TRY(scratch(c, name, (size_t)(q == 0 ? 3 : 5) * n * 4, false, &p));
+        NNX_TRY(scratch(c, name, (size_t)(q == 0 ? 3 : 7) * n * 4, false, &p));
         float* f = (float*)p;
         m->dinv[q] = f;
-        m->t[q] = f + n;
-        m->d[q] = f + 2 * n;
-        m->x[q] = q ? f + 3 * n : nullptr;
+        m->d[q] = f + n;
+        m->t[0][q] = f + 2 * n;
+        m->x[0][q] = q ? f + 3 * n : nullptr;
         m->b[q] = q ? f + 4 * n : nullptr;
+        m->x[1][q] = q ? f + 5 * n : nullptr;
+        m->t[1][q] = q ? f + 6 * n : nullptr;
     }
     for (int q = 1; q < m->nlev; ++q) {
         const GridK& f = m->g[q - 1];
@@ -240,74 +301,65 @@ nnx_status mg_setup(nnx_ctx* c, const GridK& g0, int nu, int smoother, Mg* m) {
         NNX_LAUNCHED(c);
     }
     for (int q = 0; q < m->nlev; ++q) {
-        const int64_t n = (int64_t)m->g[q].nx * m->g[q].ny * m->g[q].nz;
-        k_mg_dinv<<<grid_blocks(n), 256, 0, c->stream>>>(m->g[q], m->dinv[q]);
+        k_mg_dinv<<<grid_blocks(n_of(q)), 256, 0, c->stream>>>(m->g[q], m->dinv[q]);
         NNX_LAUNCHED(c);
+    }
+    // the small levels: the first level >= 1 with at most kMgSubCells cells
+    m->sub = -1;
+    for (int q = 1; q < m->nlev; ++q)
+        if (n_of(q) <= kMgSubCells) {
+            m->sub = q;
+            break;
+        }
+    if (m->sub > 0) {
+        std::vector<MgOp> lists[2];
+        for (int set = 0; set < 2; ++set)
+            m->sub_res[set] = gen_visit(*m, m->sub, m->b[m->sub], set, lists[set], false, nullptr);
+        const size_t n0 = lists[0].size(), n1 = lists[1].size();
+        void *ops, *gk;
+        NNX_TRY(scratch(c, "mg_ops", (n0 + n1) * sizeof(MgOp), false, &ops));
+        NNX_TRY(scratch(c, "mg_gk", sizeof(m->g), false, &gk));
+        std::vector<MgOp> all(lists[0]);
+        all.insert(all.end(), lists[1].begin(), lists[1].end());
+        NNX_HIP(c, hipMemcpyAsync(ops, all.data(), all.size() * sizeof(MgOp), hipMemcpyHostToDevice, c->stream));
+        NNX_HIP(c, hipMemcpyAsync(gk, m->g, sizeof(m->g), hipMemcpyHostToDevice, c->stream));
+        NNX_HIP(c, hipStreamSynchronize(c->stream));  // host temporaries
+        m->sub_ops[0] = (const MgOp*)ops;
+        m->sub_ops[1] = (const MgOp*)ops + n0;
+        m->sub_nops[0] = (int)n0;
+        m->sub_nops[1] = (int)n1;
+        m->d_gk = (const GridK*)gk;
     }
     return NNX_OK;
 }
 
 nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
-    const float* B[kMgMaxLevels];
-    float* cur[kMgMaxLevels];
-    auto n_of = [&](int l) { return (int64_t)m.g[l].nx * m.g[l].ny * m.g[l].nz; };
-    // one smoothing sweep number s of a phase: xin NULL means x = 0
-    auto sweep = [&](int l, int s, const float* xin, float* xout, bool jacobi) -> nnx_status {
-        const unsigned blocks = xin ? row_blocks(m.g[l].ny, m.g[l].nz) : grid_blocks(n_of(l));
-        if (jacobi)
-            k_mg_smooth<<<blocks, 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], xin, xout, m.w, xin == nullptr);
-        else
-            k_mg_cheb<<<blocks, 256, 0, c->stream>>>(m.g[l], m.dinv[l], B[l], xin, xout, m.d[l], m.c1[s], m.c2[s]);
-        NNX_LAUNCHED(c);
-        return NNX_OK;
-    };
-    // sweeps from zero: the first writes `first`, later ones ping-pong
-    auto sweeps = [&](int l, int count, float* first, float* other, bool jacobi) -> nnx_status {
-        float *src = nullptr, *dst = first;
-        for (int s = 0; s < count; ++s) {
-            NNX_TRY(sweep(l, s, src, dst, jacobi));
-            src = dst;
-            dst = (dst == first) ? other : first;
+    std::vector<MgOp> ops;
+    ops.reserve(256);
+    float* res = gen_visit(m, 0, r, 0, ops, true, z);
+    for (const MgOp& o : ops) {
+        if (o.type < 0) {
+            const int set = (int)o.c1;
+            k_mg_sub<<<1, 1024, 0, c->stream>>>(m.sub_ops[set], m.sub_nops[set], m.d_gk);
+            NNX_LAUNCHED(c);
+            continue;
         }
-        cur[l] = src;
-        return NNX_OK;
-    };
-    // sweeps from the current iterate
-    auto more = [&](int l, int count) -> nnx_status {
-        float* other = (cur[l] == m.t[l]) ? (l ? m.x[l] : z) : m.t[l];
-        for (int s = 0; s < count; ++s) {
-            NNX_TRY(sweep(l, s, cur[l], other, m.smoother == NNX_MG_JACOBI));
-            std::swap(cur[l], other);
+        const GridK& g = m.g[o.level];
+        const GridK& gc = m.g[o.level + 1 < m.nlev ? o.level + 1 : o.level];
+        const GridK& d = o.type == MG_RESTRICT ? gc : g;
+        const unsigned blocks = row_blocks(d.ny, d.nz);
+        switch (o.type) {
+            case MG_CHEB: k_mg_op<MG_CHEB><<<blocks, 256, 0, c->stream>>>(o, g, gc); break;
+            case MG_JAC: k_mg_op<MG_JAC><<<blocks, 256, 0, c->stream>>>(o, g, gc); break;
+            case MG_RESTRICT: k_mg_op<MG_RESTRICT><<<blocks, 256, 0, c->stream>>>(o, g, gc); break;
+            case MG_RESID: k_mg_op<MG_RESID><<<blocks, 256, 0, c->stream>>>(o, g, gc); break;
+            case MG_PROLONG: k_mg_op<MG_PROLONG><<<blocks, 256, 0, c->stream>>>(o, g, gc); break;
+            default: k_mg_op<MG_ADD><<<blocks, 256, 0, c->stream>>>(o, g, gc); break;
         }
-        return NNX_OK;
-    };
-    B[0] = r;
-    for (int l = 1; l < m.nlev; ++l) B[l] = m.b[l];
-    const int L = m.nlev - 1;
-    // down: at level 0 start in t so that 2 nu - 1 swaps end in z
-    for (int l = 0; l < L; ++l) {
-        NNX_TRY(sweeps(l, m.nu, m.t[l], l ? m.x[l] : z, m.smoother == NNX_MG_JACOBI));
-        const GridK& g = m.g[l];
-        k_mg_restrict<<<row_blocks(m.g[l + 1].ny, m.g[l + 1].nz), 256, 0, c->stream>>>(
-            g, B[l], cur[l], m.b[l + 1], m.g[l + 1].nx, m.g[l + 1].ny, m.g[l + 1].nz);
         NNX_LAUNCHED(c);
     }
-    if (L > 0 && n_of(L) <= kCoarseOneBlock) {
-        k_mg_coarse<<<1, 1024, 0, c->stream>>>(m.g[L], m.dinv[L], B[L], m.x[L], m.t[L], m.w, m.coarse_sweeps);
-        NNX_LAUNCHED(c);
-        cur[L] = (m.coarse_sweeps & 1) ? m.x[L] : m.t[L];
-    } else {
-        NNX_TRY(sweeps(L, m.coarse_sweeps, m.x[L] ? m.x[L] : z, m.t[L], true));
-    }
-    // up
-    for (int l = L - 1; l >= 0; --l) {
-        const GridK& g = m.g[l];
-        k_mg_prolong_add<<<row_blocks(g.ny, g.nz), 256, 0, c->stream>>>(g.nx, g.ny, g.nz, cur[l + 1], m.g[l + 1].nx,
-                                                                      m.g[l + 1].ny, m.g[l + 1].nz, cur[l]);
-        NNX_LAUNCHED(c);
-        NNX_TRY(more(l, m.nu));
-    }
-    if (cur[0] != z) NNX_HIP(c, hipMemcpyAsync(z, cur[0], n_of(0) * 4, hipMemcpyDeviceToDevice, c->stream));
+    if (res != z) NNX_HIP(c, hipMemcpyAsync(z, res, m.g[0].nx * (size_t)m.g[0].ny * m.g[0].nz * 4,
+                                            hipMemcpyDeviceToDevice, c->stream));
     return NNX_OK;
 }
 
@@ -315,13 +367,13 @@ nnx_status mg_vcycle(nnx_ctx* c, Mg& m, const float* r, float* z) {
 
 using namespace nnx;
 
-extern "C" nnx_status nnx_mg_apply(nnx_ctx* c, const nnx_grid* g, int nu, int smoother, const float* d_r,
-                                   float* d_z) {
+extern "C" nnx_status nnx_mg_apply(nnx_ctx* c, const nnx_grid* g, int nu, int smoother, int cycle,
+                                   const float* d_r, float* d_z) {
     if (!c) return NNX_ERR_ARG;
     NNX_CHECK(c, g && d_r && d_z, "null argument");
     NNX_CHECK(c, g->stencil == NNX_STENCIL_7 && g->kx && g->ky && g->kz, "multigrid needs a 7-point grid with kx, ky, kz");
     NNX_CHECK(c, g->nx > 0 && g->ny > 0 && g->nz > 0, "bad dims");
     Mg m;
-    NNX_TRY(mg_setup(c, make_grid(g), nu, smoother, &m));
+    NNX_TRY(mg_setup(c, make_grid(g), nu, smoother, cycle, &m));
     return mg_vcycle(c, m, d_r, d_z);
 }

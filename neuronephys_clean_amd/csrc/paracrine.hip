@@ -874,14 +874,15 @@ double read_scalar(nnx_ctx* c, const double* d, hipError_t* err) {
 // pc: JACOBI, LEARNED (net), MG or MG_LEARNED (net); the learned variants
 // fall back to their base (Jacobi, MG) after stall_iters iterations without
 // a new best residual.
-nnx_status fcg_run(nnx_ctx* c, const GridK& gk, int pc, int mg_nu, int mg_smoother, nnx_unet* net, const float* feat,
+nnx_status fcg_run(nnx_ctx* c, const GridK& gk, int pc, int mg_nu, int mg_smoother, int mg_cycle, nnx_unet* net,
+                   const float* feat,
                    const float* b, float* x, int use_x0, double rtol, int maxit, int stall_iters, int smooth,
                    nnx_cg_result* res) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
     const bool use_mg = pc == NNX_PRECOND_MG || pc == NNX_PRECOND_MG_LEARNED;
     if (pc != NNX_PRECOND_LEARNED && pc != NNX_PRECOND_MG_LEARNED) net = nullptr;
     Mg mg;
-    if (use_mg) NNX_TRY(mg_setup(c, gk, mg_nu, mg_smoother, &mg));
+    if (use_mg) NNX_TRY(mg_setup(c, gk, mg_nu, mg_smoother, mg_cycle, &mg));
     void *r, *z, *pa, *pb, *qa, *qb, *dinv, *part, *sc, *w1, *w2;
     NNX_TRY(scratch(c, "fcg_r", n * 4, false, &r));
     NNX_TRY(scratch(c, "fcg_z", n * 4, false, &z));
@@ -986,6 +987,7 @@ nnx_status check_fcg(nnx_ctx* c, const nnx_grid* g, const nnx_unet* net, const n
               "multigrid needs the variable-coefficient operator (kx, ky, kz)");
     NNX_CHECK(c, o->mg_nu >= 0 && o->mg_nu <= 16, "mg_nu must be in [0, 16]");
     NNX_CHECK(c, o->mg_smoother == NNX_MG_CHEBYSHEV || o->mg_smoother == NNX_MG_JACOBI, "unknown mg_smoother");
+    NNX_CHECK(c, o->mg_cycle >= 0 && o->mg_cycle <= 2, "mg_cycle must be 0 (W), 1 (V) or 2 (W)");
     return NNX_OK;
 }
 
@@ -1003,7 +1005,8 @@ nnx_status nnx_fcg_solve(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, const flo
     const GridK gk = make_grid(g);
     auto t0 = std::chrono::steady_clock::now();
     nnx_cg_result r{};
-    NNX_TRY(fcg_run(c, gk, o->precond, o->mg_nu, o->mg_smoother, net, d_feat, b, x, o->use_x0, o->rtol, o->maxit,
+    NNX_TRY(fcg_run(c, gk, o->precond, o->mg_nu, o->mg_smoother, o->mg_cycle, net, d_feat, b, x, o->use_x0, o->rtol,
+                    o->maxit,
                     o->stall_iters,
                     o->smooth, &r));
     auto t1 = std::chrono::steady_clock::now();
@@ -1046,7 +1049,8 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
         NNX_LAUNCHED(c);
         const double tol = std::max(inner_rtol, 0.5 * o->rtol / rel);
         nnx_cg_result ir{};
-        NNX_TRY(fcg_run(c, gk, o->precond, o->mg_nu, o->mg_smoother, net, d_feat, (const float*)rhs, (float*)d, 0, tol,
+        NNX_TRY(fcg_run(c, gk, o->precond, o->mg_nu, o->mg_smoother, o->mg_cycle, net, d_feat, (const float*)rhs,
+                        (float*)d, 0, tol,
                         o->maxit,
                         o->stall_iters, o->smooth, &ir));
         inner_total += ir.iters;

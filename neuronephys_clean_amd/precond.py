@@ -222,7 +222,7 @@ def harvest(ctx, n=64, fields=160, sources=16, schedule=(0, 1, 2, 4, 8, 16, 32, 
                 ek.upload(e)
                 ctx.call("nnx_stencil_apply", C.byref(g), ek.ptr, rk.ptr)
                 if mg:
-                    ctx.call("nnx_mg_apply", C.byref(g), 0, _lib.MG_CHEBYSHEV, rk.ptr, zk.ptr)
+                    ctx.call("nnx_mg_apply", C.byref(g), 0, _lib.MG_CHEBYSHEV, 0, rk.ptr, zk.ptr)
                     e1 = (e - zk.numpy()).astype(np.float32)
                 else:
                     e1 = (e - SMOOTH_W * dinv * rk.numpy()).astype(np.float32)

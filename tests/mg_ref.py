@@ -84,11 +84,13 @@ def _smooth(x, r, kx, ky, kz, dinv, nu, w, smoother):
     return x
 
 
-def vcycle(r, kx, ky, kz, nu=2, w=0.8, coarse_sweeps=16, smoother="jacobi"):
-    """One V-cycle from zero: nu sweeps of the smoother (damped Jacobi or
+def vcycle(r, kx, ky, kz, nu=2, w=0.8, coarse_sweeps=16, smoother="jacobi", gamma=1, v_cells=4096):
+    """One cycle from zero: nu sweeps of the smoother (damped Jacobi or
     Chebyshev) before and after each level, coarsening while every
     dimension is even and >= 8, coarse_sweeps damped-Jacobi sweeps on the
-    coarsest level."""
+    coarsest level; each coarser level with more than v_cells cells visited
+    gamma times (1: V-cycle, 2: W-cycle, the later visits on the residual
+    the earlier ones leave), smaller ones once."""
     nz, ny, nx = r.shape
     dinv = 1.0 / diag(kx, ky, kz)
     can = nx % 2 == 0 and ny % 2 == 0 and nz % 2 == 0 and min(nx, ny, nz) >= 8
@@ -96,6 +98,10 @@ def vcycle(r, kx, ky, kz, nu=2, w=0.8, coarse_sweeps=16, smoother="jacobi"):
         return _smooth(None, r, kx, ky, kz, dinv, coarse_sweeps, w, "jacobi")
     x = _smooth(None, r, kx, ky, kz, dinv, nu, w, smoother)
     rc = restrict(r - apply(x, kx, ky, kz))
-    ec = vcycle(rc, *coarsen(kx, ky, kz), nu=nu, w=w, coarse_sweeps=coarse_sweeps, smoother=smoother)
+    kc = coarsen(kx, ky, kz)
+    kw = dict(nu=nu, w=w, coarse_sweeps=coarse_sweeps, smoother=smoother, gamma=gamma, v_cells=v_cells)
+    ec = vcycle(rc, *kc, **kw)
+    for _ in range(gamma - 1 if rc.size > v_cells else 0):
+        ec = ec + vcycle(rc - apply(ec, *kc), *kc, **kw)
     x = x + prolong(ec)
     return _smooth(x, r, kx, ky, kz, dinv, nu, w, smoother)

@@ -35,7 +35,8 @@ def const_field(n, k):
 @pytest.mark.parametrize("dims", [(32, 32, 32), (32, 16, 24), (12, 10, 8), (64, 64, 64)])
 @pytest.mark.parametrize("nu", [1, 2, 4])
 @pytest.mark.parametrize("smoother", ["chebyshev", "jacobi"])
-def test_vcycle_matches_numpy_restatement(ctx, dims, nu, smoother):
+@pytest.mark.parametrize("gamma", [1, 2])
+def test_vcycle_matches_numpy_restatement(ctx, dims, nu, smoother, gamma):
     """nnx_mg_apply against tests/mg_ref.py (FP64) on log-normal faces,
     including grids whose coarsening stops early (odd or small dims)."""
     nx, ny, nz = dims
@@ -48,12 +49,12 @@ def test_vcycle_matches_numpy_restatement(ctx, dims, nu, smoother):
     z = DA.zeros(ctx, r.size, np.float32)
     g = _lib.Grid(nx, ny, nz, 7, 0.0, 0.0, d[0].ptr, d[1].ptr, d[2].ptr)
     sm = _lib.MG_CHEBYSHEV if smoother == "chebyshev" else _lib.MG_JACOBI
-    ctx.call("nnx_mg_apply", C.byref(g), nu, sm, d[3].ptr, z.ptr)
+    ctx.call("nnx_mg_apply", C.byref(g), nu, sm, gamma, d[3].ptr, z.ptr)
     ref = mg_ref.vcycle(r.astype(np.float64), kx.astype(np.float64), ky.astype(np.float64),
-                        kz.astype(np.float64), nu=nu, smoother=smoother)
+                        kz.astype(np.float64), nu=nu, smoother=smoother, gamma=gamma)
     got = z.numpy().reshape(nz, ny, nx)
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    say("dims", dims, "nu", nu, smoother, "max rel diff", err)
+    say("dims", dims, "nu", nu, smoother, "gamma", gamma, "max rel diff", err)
     assert err < 1e-4
 
 
@@ -75,7 +76,7 @@ def test_vcycle_contracts_energy_norm(ctx, kind, smoother):
     Ae = apply_A(ctx, g, e)
     r, z = DA.from_numpy(ctx, Ae.astype(np.float32)), DA.zeros(ctx, n ** 3, np.float32)
     for nu in (1, 2):
-        ctx.call("nnx_mg_apply", C.byref(g), nu, smoother, r.ptr, z.ptr)
+        ctx.call("nnx_mg_apply", C.byref(g), nu, smoother, 1, r.ptr, z.ptr)
         e1 = e - z.numpy().astype(np.float64)
         ratio = np.sqrt(e1 @ apply_A(ctx, g, e1) / (e @ Ae))
         say(kind, "smoother", smoother, "nu", nu, "energy-norm contraction", ratio)
@@ -150,10 +151,11 @@ def test_multigrid_argument_errors(ctx):
     o = _lib.CgOpts(rtol=1e-6, maxit=100, precond=_lib.PRECOND_MG, check_every=1)
     st = ctx._l.nnx_fcg_solve(ctx.h, C.byref(g), None, None, b.ptr, x.ptr, C.byref(o), C.byref(r))
     assert st == 1 and "variable-coefficient" in ctx._l.nnx_last_error(ctx.h).decode()
-    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(g), 2, 0, b.ptr, x.ptr) == 1
+    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(g), 2, 0, 0, b.ptr, x.ptr) == 1
     k = DA.from_numpy(ctx, const_field(n, 5.0))
     gv = _lib.Grid(n, n, n, 7, 0.0, 0.0, k.ptr, k.ptr, k.ptr)
-    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(gv), 2, 7, b.ptr, x.ptr) == 1
-    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(gv), 17, 0, b.ptr, x.ptr) == 1
+    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(gv), 2, 7, 0, b.ptr, x.ptr) == 1
+    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(gv), 17, 0, 0, b.ptr, x.ptr) == 1
+    assert ctx._l.nnx_mg_apply(ctx.h, C.byref(gv), 2, 0, 3, b.ptr, x.ptr) == 1
     o2 = _lib.CgOpts(rtol=1e-6, maxit=100, precond=_lib.PRECOND_MG, check_every=1, mg_smoother=5)
     assert ctx._l.nnx_fcg_solve(ctx.h, C.byref(gv), None, None, b.ptr, x.ptr, C.byref(o2), C.byref(r)) == 1

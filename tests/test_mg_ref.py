@@ -99,3 +99,23 @@ def test_chebyshev_vcycle_rate():
         rates[n] = (res[-1] / res[2]) ** 0.25
     print("chebyshev rates", rates)
     assert max(rates.values()) < 0.2
+
+
+def test_wcycle_converges_faster_than_vcycle():
+    """On the log-normal field the W-cycle (each coarser level visited twice)
+    contracts more per cycle than the V-cycle (at 64^3: five levels)."""
+    n = 64
+    f = configs.c5_field(seed=2, n=n)
+    k = (f.kx.astype(np.float64), f.ky.astype(np.float64), f.kz.astype(np.float64))
+    rng = np.random.default_rng(1)
+    b = rng.normal(size=(n, n, n))
+    out = {}
+    for gamma in (1, 2):
+        x = np.zeros_like(b)
+        res = [np.linalg.norm(b)]
+        for _ in range(6):
+            x = x + mg_ref.vcycle(b - mg_ref.apply(x, *k), *k, nu=3, smoother="chebyshev", gamma=gamma)
+            res.append(np.linalg.norm(b - mg_ref.apply(x, *k)))
+        out[gamma] = (res[-1] / res[2]) ** 0.25
+    print("V/W rates", out)
+    assert out[2] < 0.8 * out[1]
