@@ -472,8 +472,9 @@ nnx_status nnx_conv3_wgrad_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny,
  * smoother NNX_MG_CHEBYSHEV, a Chebyshev polynomial in D^-1 A on [1/8, 2],
  * or NNX_MG_JACOBI, damped Jacobi (w = 0.8); 16 damped-Jacobi sweeps on the
  * coarsest level.  cycle 1: V-cycle; 2 (or 0): W-cycle (each coarser level
- * visited twice per visit of its parent, the second time on the residual
- * the first visit leaves).  No counterpart in the reference.               */
+ * of more than 4096 cells visited twice per visit of its parent, the second
+ * time on the residual the first visit leaves; smaller levels once).  No
+ * counterpart in the reference.                                             */
 nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, int smoother, int cycle, const float* d_r,
                         float* d_z);
 

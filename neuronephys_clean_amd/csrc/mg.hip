@@ -1,4 +1,4 @@
-// mg.hip - geometric multigrid V-cycle for the variable-coefficient 7-point
+// mg.hip - geometric multigrid for the variable-coefficient 7-point
 // backward-Euler operator, the preconditioner of flexible CG for the
 // variable-diffusivity paracrine solve and the base the learned
 // preconditioner corrects.
@@ -13,9 +13,18 @@
 // own boundary treatment), smoothing nu sweeps before and after (3 by
 // default) of a Chebyshev polynomial in D^-1 A on [1/8, 2] (2 bounds the
 // spectrum of D^-1 A for these diagonally dominant operators) or of damped
-// Jacobi, the coarsest level 16 damped-Jacobi sweeps (one workgroup when
-// the level is small).  Block-mean restriction with trilinear prolongation
-// is not a symmetric pair, so the cycle is used inside flexible CG.
+// Jacobi, the coarsest level 16 damped-Jacobi sweeps.  A W-cycle (each
+// coarser level visited twice, the second time on the residual the first
+// visit leaves) on the levels of more than 4096 cells, a V-cycle below.
+// Block-mean restriction with trilinear prolongation is not a symmetric
+// pair, so the cycle is used inside flexible CG.
+//
+// One generator produces the cycle as a sequence of steps (sweep, restrict,
+// residual, prolong-add, add).  Steps on large levels are kernel launches;
+// the steps of a visit of the first level of <= 4096 cells and below are
+// built once per hierarchy and run by a single workgroup in one launch
+// (tens of steps of a few microseconds' latency each would otherwise be
+// tens of launches).
 //
 // No counterpart in the reference, whose CG is unpreconditioned
 // (src/paracrine.cu:106-129).
