@@ -77,16 +77,22 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(GridK f, float* __restrict__
 }
 
 // ---- the steps of a cycle, per cell ----
+// (every load of a step precedes its stores: the buffers of one step never
+// alias, but the compiler cannot know that)
 __device__ __forceinline__ void op_cheb(const GridK& g, const MgOp& o, int i, int j, int k, int64_t id) {
-    const float res = o.xin ? o.rhs[id] - apply_at<float, float>(o.xin, g, i, j, k) : o.rhs[id];
+    const float* __restrict__ xin = o.xin;
+    const float xc = xin ? xin[id] : 0.0f;
+    const float res = xin ? o.rhs[id] - apply_at<float, float>(xin, g, i, j, k) : o.rhs[id];
+    const float dold = o.c1 != 0.0f ? o.dvec[id] : 0.0f;
     const float t = o.c2 * o.dinv[id] * res;
-    const float dn = o.c1 != 0.0f ? o.c1 * o.dvec[id] + t : t;
+    const float dn = o.c1 != 0.0f ? o.c1 * dold + t : t;
     o.dvec[id] = dn;
-    o.xout[id] = o.xin ? o.xin[id] + dn : dn;
+    o.xout[id] = xin ? xc + dn : dn;
 }
 __device__ __forceinline__ void op_jac(const GridK& g, const MgOp& o, int i, int j, int k, int64_t id) {
-    o.xout[id] = o.xin ? o.xin[id] + o.c1 * o.dinv[id] * (o.rhs[id] - apply_at<float, float>(o.xin, g, i, j, k))
-                       : o.c1 * o.dinv[id] * o.rhs[id];
+    const float* __restrict__ xin = o.xin;
+    o.xout[id] = xin ? xin[id] + o.c1 * o.dinv[id] * (o.rhs[id] - apply_at<float, float>(xin, g, i, j, k))
+                     : o.c1 * o.dinv[id] * o.rhs[id];
 }
 // coarse cell (I, J, K): block mean of the fine residual over its 8 children
 __device__ __forceinline__ void op_restrict(const GridK& g, const MgOp& o, int I, int J, int K, int64_t q) {
