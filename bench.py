@@ -382,8 +382,10 @@ def tune_fused(ctx, sim, chunk):
     ctx.call("nnx_event_create", C.byref(e1))
     best, steps = None, 0
     for block in (256, 512):
-        for tile in (512, 1024, 2048):
-            for grid in (1024, 2048):
+        for tile in (256, 512, 1024, 2048):
+            if tile < block:
+                continue
+            for grid in (1024, 2048, 4096):
                 sim.set_fused(block, tile, grid)
                 sim.run(chunk, use_graph=True, chunk=chunk)  # capture
                 ctx.call("nnx_event_record", e0)

@@ -386,7 +386,7 @@ uint64_t ens_key(const nnx_ens_plan* p, const nnx_ens_config& f) {
 }
 
 const nnx_ens_config kDefaultCfg = {256, 2048, 1024, 0};
-const nnx_ens_config kFusedDefault = {512, 1024, 2048, 0};
+const nnx_ens_config kFusedDefault = {256, 256, 4096, 0};
 
 // Captures `body` (which enqueues on c->stream) into a graph cached under
 // `key`, or replays the cached one.
