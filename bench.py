@@ -412,6 +412,9 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--couple", default="fused", choices=["fused", "event", "pull"])
     ap.add_argument("--cpu-steps", type=int, default=1500)
+    ap.add_argument("--tune", action="store_true",
+                    help="pick the fused step's launch shape by timing 21 shapes on real steps before the timed "
+                         "region (the timed window then starts later in the simulation)")
     ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
                     help="profile: a short eager sequence for rocprofv3 --pmc passes (20 fused and 20 "
                          "unfused C2 steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
@@ -452,7 +455,11 @@ def main():
     log(f"bench: warmup {args.warmup} steps done, {sim.raster()[1]} spikes")
     tune = None
     if args.couple == "fused":
-        tune = tune_fused(ctx, sim, args.chunk)
+        if args.tune:
+            tune = tune_fused(ctx, sim, args.chunk)
+        else:  # the library's default shape (one 256-neuron tile per workgroup, 4096 workgroups)
+            tune = {"block": 256, "tile": 256, "grid": 4096, "tune_steps": 0}
+        sim.set_fused(tune["block"], tune["tile"], tune["grid"])
         log(f"bench: fused launch shape {tune}")
     len0 = sim.raster()[1]
     e0, e1 = C.c_void_p(), C.c_void_p()

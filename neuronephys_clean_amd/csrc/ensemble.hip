@@ -530,12 +530,12 @@ nnx_status nnx_ensemble_autotune(nnx_ctx* c, const nnx_ens_plan* p, uint64_t fir
     double cur_us;
     NNX_TRY(time_cfg(cur, &cur_us));
     const int blocks[] = {64, 128, 256, 512};
-    const int tiles[] = {512, 1024, 2048, 4096, 8192};
-    const int grids[] = {256, 512, 1024, 2048, 4096};
+    const int tiles[] = {256, 512, 1024, 2048, 4096, 8192};
+    const int grids[] = {256, 512, 1024, 2048, 4096, 8192};
     // coordinate search: block, then tile, then grid
     for (int dim = 0; dim < 3; ++dim) {
         const int* vals = dim == 0 ? blocks : dim == 1 ? tiles : grids;
-        const int nv = dim == 0 ? 4 : 5;
+        const int nv = dim == 0 ? 4 : 6;
         nnx_ens_config base = cur;
         for (int i = 0; i < nv; ++i) {
             nnx_ens_config f = base;
