@@ -500,6 +500,71 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* ctx, const nnx_grid* g, nnx_unet* net, 
                                 double inner_rtol, int max_outer, nnx_cg_result* res);
 
 /* ------------------------------------------------------------------------ */
+/* electrode readout                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Regions of an electrode net, an adjacency list as the reference's
+ * (ElectrodeNet's d_regions, src/electrode_net.cuh:6-22): region r holds the
+ * neurons idx[offsets[r] .. offsets[r+1]), in summation order.  nnz =
+ * offsets[n_regions] (a host copy; it bounds every idx read).  Entries
+ * outside [0, n) make the region's mean NaN and count as not firing;
+ * offsets are clamped to [0, nnz] (a region with end <= start is empty).   */
+typedef struct nnx_regions {
+    int64_t n_regions;
+    int64_t nnz;
+    const int32_t* offsets;   /* n_regions+1, device */
+    const int32_t* idx;       /* nnz, device */
+} nnx_regions;
+
+/* Mean V of each region and the number of its neurons whose spike bit is
+ * set.  Replaces ElectrodeNet::DoElectrodeMeasurements and its
+ * electrodes_measure_kernel (src/electrode_net.cu:3-36,
+ * src_wholebrain/electrode_net.cu:3-49).  avg[r] (dtype) = the sum of V over
+ * the region in idx order, one add at a time in dtype, divided by the
+ * region's size: bit-identical to the reference's sequential loop, NaN for
+ * an empty region as there.  d_bits: spike bitmap (ceil(n/64) words; the
+ * reference's is_firing flags), NULL = no counts (d_firing unused).  Both
+ * outputs stay on the device (the reference copies them to the host).     */
+nnx_status nnx_electrode_measure(nnx_ctx* ctx, const nnx_regions* r, int dtype, int64_t n,
+                                 const void* d_v, const uint64_t* d_bits, void* d_avg,
+                                 int32_t* d_firing);
+
+/* Mean of V over all n neurons into *d_mean (dtype, device).  Replaces
+ * ElectrodeNet::MeasureAverageEntireNetwork (src/electrode_net.cu:44-47),
+ * whose fp32 thrust::reduce has an unspecified order; here the sum is taken
+ * in FP64 in a fixed order (n < 2^31), divided by n and rounded once.       */
+nnx_status nnx_network_mean(nnx_ctx* ctx, int dtype, int64_t n, const void* d_v, void* d_mean);
+
+/* Synchrony statistics of one region (SynchronyNet,
+ * src_wholebrain/electrode_net.cu:54-126): per call (one timestep),
+ *   vi_sum[j] += V[idx[j]],  vi2_sum[j] += V[idx[j]]*V[idx[j]]  (product in
+ *   dtype, sums in FP64), j < n_idx;
+ *   for each sub-population s (the first sub_counts[s] entries of idx):
+ *   Vsum = their sum (FP64 in a fixed order, rounded to dtype; the reference
+ *   uses an fp32 thrust::reduce), Vavg = Vsum / count in dtype, widened to
+ *   FP64; vavg_sum[s] += Vavg, vavg2_sum[s] += Vavg*Vavg.
+ * Sums are caller-owned device FP64 arrays (n_idx, n_idx, n_sub, n_sub),
+ * zeroed by the caller before the first step; 1 <= sub_counts[s] <= n_idx
+ * (device int32 array) is checked on the device (a count out of range
+ * leaves that sub-population's sums NaN).                                   */
+nnx_status nnx_synchrony_accumulate(nnx_ctx* ctx, int dtype, int64_t n, const void* d_v, int64_t n_idx,
+                                    const int32_t* d_idx, int32_t n_sub, const int32_t* d_sub_counts,
+                                    double* d_vi_sum, double* d_vi2_sum, double* d_vavg_sum,
+                                    double* d_vavg2_sum);
+
+/* chi^2 of each sub-population after n_steps accumulated timesteps
+ * (SynchronyNet::GetSynchronyMeasures, src_wholebrain/electrode_net.cu:
+ * 99-126), the reference's host arithmetic in the same order:
+ *   chi[s] = count * var(Vavg) / sum_{j<count} var(V_j),
+ *   var(Vavg) = (vavg2 - vavg^2 / T) / T, var(V_j) = vi2/T - (vi/T)^2 with
+ *   T = n_steps (the per-neuron division skipped when T = 0, as there).
+ * d_chi: n_sub doubles on the device.                                       */
+nnx_status nnx_synchrony_measures(nnx_ctx* ctx, int64_t n_idx, int32_t n_sub, const int32_t* d_sub_counts,
+                                  const double* d_vi_sum, const double* d_vi2_sum,
+                                  const double* d_vavg_sum, const double* d_vavg2_sum, int64_t n_steps,
+                                  double* d_chi);
+
+/* ------------------------------------------------------------------------ */
 /* composite timestep                                                        */
 /* ------------------------------------------------------------------------ */
 
@@ -532,6 +597,21 @@ typedef struct nnx_step_plan {
      * cheaper of the two until a large part of the network fires at once).
      * None of these changes any result.                                     */
     int32_t fused_block, fused_tile, fused_grid, fused_burst;
+    /* electrode readout (optional; readout_every <= 0 = off), as the
+     * reference's print step (src/main.cu:133-141) and ElectrodesExample
+     * (src_wholebrain/main.cu:36-47): before each step s with
+     * s % readout_every == 0, row s / readout_every (rows >= readout_cap are
+     * dropped) receives nnx_electrode_measure of the state entering step s -
+     * n_regions means in readout_avg (state dtype), the counts over the
+     * spike bitmap the previous step's coupling read (the reference's
+     * is_firing) in readout_firing (NULL = not kept) - and
+     * nnx_network_mean in readout_mean (one value per row, NULL = not
+     * kept).  The rows are written on the device; nothing synchronises.     */
+    nnx_regions regions;
+    int64_t readout_every, readout_cap;
+    void* readout_avg;         /* readout_cap * n_regions */
+    int32_t* readout_firing;   /* readout_cap * n_regions */
+    void* readout_mean;        /* readout_cap */
 } nnx_step_plan;
 
 /* Runs n_steps timesteps starting at global step `first_step`.  With

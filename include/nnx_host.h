@@ -105,6 +105,19 @@ void nnx_gen_grf(uint64_t seed, uint64_t purpose, int32_t nx, int32_t ny, int32_
 void nnx_face_coefficients(int32_t nx, int32_t ny, int32_t nz, const float* c, float* kx, float* ky,
                            float* kz);
 
+/* Electrode regions (Morphology::GenSphericalRegions, src/morphology.cpp:
+ * 48-97, src_wholebrain/morphology.cpp): region r = the k neurons nearest to
+ * query r, nearest first, by squared Euclidean distance computed in FP64
+ * from the float positions (pos: 3n, x,y,z interleaved), ties by lower index.
+ * Queries are neuron indices (center_idx, a neuron is its own nearest) or
+ * points (center_pos: 3 n_regions floats; center_idx NULL).  idx: n_regions*k
+ * (the reference's knn order, from its exact ANN search), offsets:
+ * n_regions+1 = r*k.  The reference copies k+1 entries per region into k
+ * slots (the last region writes one past the end); here exactly k.
+ * Returns 0, or -1 for k outside [1, n] or a center index outside [0, n).  */
+int nnx_spherical_regions(int64_t n, const float* pos, int64_t n_regions, const int32_t* center_idx,
+                          const float* center_pos, int32_t k, int32_t* offsets, int32_t* idx);
+
 #ifdef __cplusplus
 }
 #endif

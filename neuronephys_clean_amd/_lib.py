@@ -81,6 +81,10 @@ class CgResult(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Regions(C.Structure):
+    _fields_ = [("n_regions", C.c_int64), ("nnz", C.c_int64), ("offsets", vp), ("idx", vp)]
+
+
 class StepPlan(C.Structure):
     _fields_ = [("state", HHState), ("params", HHParams), ("graph", Graph),
                 ("integrator", C.c_int32), ("spike_rule", C.c_int32), ("couple_mode", C.c_int32),
@@ -88,7 +92,9 @@ class StepPlan(C.Structure):
                 ("list", vp), ("count", vp), ("voxel_of", vp), ("voxel_counts", vp),
                 ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64),
                 ("fused_block", C.c_int32), ("fused_tile", C.c_int32), ("fused_grid", C.c_int32),
-                ("fused_burst", C.c_int32)]
+                ("fused_burst", C.c_int32), ("regions", Regions), ("readout_every", C.c_int64),
+                ("readout_cap", C.c_int64), ("readout_avg", vp), ("readout_firing", vp),
+                ("readout_mean", vp)]
 
 
 class EnsPlan(C.Structure):
@@ -147,6 +153,11 @@ _SIGS = {
     "nnx_rhs_from_counts": ([vp, C.c_int64, vp, vp, C.c_double, vp], C.c_int),
     "nnx_csr_spmv": ([vp, C.c_int64, vp, vp, vp, vp, vp, C.c_int], C.c_int),
     "nnx_run_steps": ([vp, _P(StepPlan), C.c_uint64, C.c_int, C.c_int], C.c_int),
+    "nnx_electrode_measure": ([vp, _P(Regions), C.c_int, C.c_int64, vp, vp, vp, vp], C.c_int),
+    "nnx_network_mean": ([vp, C.c_int, C.c_int64, vp, vp], C.c_int),
+    "nnx_synchrony_accumulate": ([vp, C.c_int, C.c_int64, vp, C.c_int64, vp, C.c_int32, vp, vp, vp, vp,
+                                  vp], C.c_int),
+    "nnx_synchrony_measures": ([vp, C.c_int64, C.c_int32, vp, vp, vp, vp, vp, C.c_int64, vp], C.c_int),
     "nnx_unet_param_count": ([], C.c_int64),
     "nnx_unet_create": ([vp, vp, C.c_uint64, _P(vp)], C.c_int),
     "nnx_unet_destroy": ([vp, vp], C.c_int),
@@ -190,6 +201,7 @@ _HOST_SIGS = {
     "nnx_gen_grf": ([C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_double, vp],
                     None),
     "nnx_face_coefficients": ([C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp], None),
+    "nnx_spherical_regions": ([C.c_int64, vp, C.c_int64, vp, vp, C.c_int32, vp, vp], C.c_int),
 }
 
 _lib = None

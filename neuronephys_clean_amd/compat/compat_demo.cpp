@@ -4,12 +4,18 @@
 // check the run against the oracle.
 //
 //   compat_demo DIR nincs nx ny nz h D decay par_dt timescale coup_strength
+//               [print_incs render_N k]
 //
 // DIR/off.i32 (N+1), DIR/tgt.i32 (nnz), DIR/sv.f32 (17 x N, SoA in the
 // SIM_PARAMS order), DIR/pos.f32 (3 x N).  Writes DIR/sv_out.f32,
 // DIR/raster.i32 ((step, neuron) pairs of the post-step firing),
 // DIR/nfiring.i32 (per-increment return of the coupling), DIR/g_par.f32,
-// DIR/density.f32.
+// DIR/density.f32.  With print_incs, render_N electrodes at neurons
+// (N / render_N) * e, each the mean of its k nearest neurons
+// (src/main.cu:279-285), are read at every print increment before its
+// coupling (src/main.cu:133-141): DIR/electrodes.f32 holds one row per print
+// increment, the render_N means then the network mean; DIR/regions.i32 the
+// region indices.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -50,9 +56,10 @@ static void record(const AtomicCoupling& c, int n_firing, int step, std::vector<
 }
 
 int main(int argc, char** argv) {
-    if (argc != 12) {
+    if (argc != 12 && argc != 15) {
         std::fprintf(stderr,
-                     "usage: %s DIR nincs nx ny nz h D decay par_dt timescale coup_strength\n",
+                     "usage: %s DIR nincs nx ny nz h D decay par_dt timescale coup_strength "
+                     "[print_incs render_N k]\n",
                      argv[0]);
         return 2;
     }
@@ -92,8 +99,32 @@ int main(int argc, char** argv) {
         ParacrineModule paracrine(g_par, N, nx, ny, nz, par_dt, timescale, coup, h, D, decay,
                                   px.data(), py.data(), pz.data());
 
+        const int print_incs = argc == 15 ? std::atoi(argv[12]) : 0;
+        std::unique_ptr<ElectrodeNet> electrodes;
+        int render_N = 0;
+        std::vector<Float> erows;
+        if (print_incs > 0) {
+            render_N = std::atoi(argv[13]);
+            const int k = std::atoi(argv[14]);
+            std::vector<float> xyz(3 * (size_t)N);
+            for (int i = 0; i < N; ++i)
+                for (int d = 0; d < 3; ++d) xyz[3 * (size_t)i + d] = pos[(size_t)d * N + i];
+            Morphology morphology(xyz);
+            std::vector<int> electrode_inds(render_N), h_regions, h_regions_inds;
+            for (int e = 0; e < render_N; ++e) electrode_inds[e] = (N / render_N) * e;
+            morphology.GenSphericalRegions(h_regions, h_regions_inds, electrode_inds, k);
+            electrodes = std::make_unique<ElectrodeNet>(h_regions, h_regions_inds);
+            save(dir + "/regions.i32", h_regions);
+        }
+
         std::vector<int32_t> raster, nfiring;
         for (int i = 0; i < nincs; ++i) {
+            if (electrodes && i % print_incs == 0) {
+                electrodes->DoElectrodeMeasurements(ptrs[SIM_V]);
+                const Float* avg = electrodes->ElectrodeOutputHandle();
+                erows.insert(erows.end(), avg, avg + render_N);
+                erows.push_back(ElectrodeNet::MeasureAverageEntireNetwork(ptrs[SIM_V], N));
+            }
             const int n_firing = coupling(list, ptrs[SIM_LOCAL_IN], ptrs[SIM_LOCAL_OUTPUT]);
             nfiring.push_back(n_firing);
             record(coupling, n_firing, i - 1, raster);
@@ -111,6 +142,7 @@ int main(int argc, char** argv) {
         // the firing of the last increment (the coupling input it writes is
         // not part of sv_out)
         record(coupling, coupling(list, ptrs[SIM_LOCAL_IN], ptrs[SIM_LOCAL_OUTPUT]), nincs - 1, raster);
+        if (electrodes) save(dir + "/electrodes.f32", erows);
         save(dir + "/sv_out.f32", out);
         save(dir + "/raster.i32", raster);
         save(dir + "/nfiring.i32", nfiring);

@@ -13,6 +13,9 @@
 //   Density_on_grid_3d                          src/paracrine.cuh:121-173
 //   Scatter_grid_converter_3d                   src/paracrine.cuh:184-209
 //   ParacrineModule                             src/paracrine.cuh:212-267
+//   ElectrodeNet, SynchronyNet                  src/electrode_net.cuh:6-38,
+//                                               src_wholebrain/electrode_net.cuh:6-92
+//   Morphology::GenSphericalRegions             src/morphology.cpp:48-97
 //
 // Deliberate differences (also listed in the design notes):
 //   - thrust::device_vector members become nnx_compat::DeviceVector (same
@@ -24,8 +27,14 @@
 //   - update_on_fine_timescale adds the release to the firing neurons (the
 //     reference's transform_if uses the compacted index list as its stencil,
 //     src/paracrine.cuh:250-252, which selects list positions, not neurons).
+//   - AtomicCoupling::is_firing is the spike bitmap (one bit per neuron), not
+//     an int per neuron; ElectrodeNet's wholebrain overload takes it as is.
+//   - host_vector arguments are std::vector; electrode positions are
+//     std::array<float, 3> (no glm); Morphology holds positions only.
 #pragma once
 
+#include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -231,16 +240,15 @@ class HHEphys : public Ephys {
 // the source's output.
 class AtomicCoupling {
     int N;
-    DeviceVector<uint64_t> bits_;
     DeviceVector<int32_t> count_;
     Float* downstream_inputs = nullptr;
     Float* neuron_outputs = nullptr;
 
     int couple(const AdjacencyList& l, Float* spikes, const Float* outputs, int mode) {
-        check(nnx_spike_threshold(ctx(), NNX_F32, N, outputs, 0.1, bits_.data()));
-        check(nnx_spike_compact(ctx(), N, bits_.data(), firing_neurons.data(), count_.data()));
+        check(nnx_spike_threshold(ctx(), NNX_F32, N, outputs, 0.1, is_firing.data()));
+        check(nnx_spike_compact(ctx(), N, is_firing.data(), firing_neurons.data(), count_.data()));
         const nnx_graph g = l.graph();
-        check(nnx_couple(ctx(), &g, mode, NNX_F32, bits_.data(), firing_neurons.data(),
+        check(nnx_couple(ctx(), &g, mode, NNX_F32, is_firing.data(), firing_neurons.data(),
                          count_.data(), outputs, spikes));
         int32_t k = 0;
         check(nnx_read_count(ctx(), count_.data(), &k));  // the reference returns the count
@@ -249,10 +257,11 @@ class AtomicCoupling {
 
    public:
     AdjacencyList list;                    // src_wholebrain form
+    DeviceVector<uint64_t> is_firing;      // bit i: neuron i's output exceeded the level
     DeviceVector<int32_t> firing_neurons;  // ascending; first count entries valid
 
     explicit AtomicCoupling(int const N_)
-        : N(N_), bits_((N_ + 63) / 64 + 1), count_(1), firing_neurons(N_ + 1) {}
+        : N(N_), count_(1), is_firing((N_ + 63) / 64 + 1), firing_neurons(N_ + 1) {}
     AtomicCoupling(int const N_, AdjacencyList& list_, Float* downstream_inputs_, Float* neuron_outputs_)
         : AtomicCoupling(N_) {
         std::swap(list, list_);  // the reference moves the list in (src_wholebrain/coupling.cu:6-9)
@@ -381,6 +390,143 @@ class ParacrineModule {
                              density_at_neurons.size() * sizeof(Float)));
     }
     const Density_on_grid_3d& get_grid() const { return grid; }
+};
+
+// ----------------------------------------------------------- readout ----
+// Largest index + 1 of a region list: the neuron count the device bounds
+// its reads by (the reference's classes are not told N).
+inline int64_t index_bound(const std::vector<int>& idx) {
+    int64_t m = 0;
+    for (int i : idx) m = std::max<int64_t>(m, (int64_t)i + 1);
+    return m;
+}
+
+// Electrodes measuring the mean voltage (and, wholebrain form, the number
+// of firing neurons) of regions stored as an adjacency list.
+class ElectrodeNet {
+    DeviceVector<int32_t> inds_, offs_;
+    int N_regions;
+    int64_t nnz_, n_;
+    DeviceVector<Float> d_avg_voltages;
+    DeviceVector<int32_t> d_firing_per_region;
+    std::vector<Float> h_avg_voltages;
+    std::vector<int> h_firing_per_region;
+
+    nnx_regions regions() const {
+        nnx_regions r{};
+        r.n_regions = N_regions;
+        r.nnz = nnz_;
+        r.offsets = offs_.data();
+        r.idx = inds_.data();
+        return r;
+    }
+
+   public:
+    ElectrodeNet(const std::vector<int>& regions_inds, const std::vector<int>& regions_offsets)
+        : inds_(regions_inds.empty() ? std::vector<int>(1, 0) : regions_inds), offs_(regions_offsets),
+          N_regions((int)regions_offsets.size() - 1), nnz_((int64_t)regions_inds.size()),
+          n_(index_bound(regions_inds)), d_avg_voltages(std::max(N_regions, 1)),
+          d_firing_per_region(std::max(N_regions, 1)), h_avg_voltages(std::max(N_regions, 0), 0),
+          h_firing_per_region(std::max(N_regions, 0), 0) {}
+
+    // mean V per region, copied to the host (src/electrode_net.cu:23-36)
+    void DoElectrodeMeasurements(const Float* V_ptr) {
+        const nnx_regions r = regions();
+        check(nnx_electrode_measure(ctx(), &r, NNX_F32, n_, V_ptr, nullptr, d_avg_voltages.data(), nullptr));
+        if (N_regions > 0)
+            check(nnx_memcpy_d2h(ctx(), h_avg_voltages.data(), d_avg_voltages.data(), N_regions * sizeof(Float)));
+    }
+    // and the firing count per region (src_wholebrain/electrode_net.cu:32-49)
+    void DoElectrodeMeasurements(const Float* V_ptr, const DeviceVector<uint64_t>& is_firing) {
+        const nnx_regions r = regions();
+        check(nnx_electrode_measure(ctx(), &r, NNX_F32, n_, V_ptr, is_firing.data(), d_avg_voltages.data(),
+                                    d_firing_per_region.data()));
+        if (N_regions > 0) {
+            check(nnx_memcpy_d2h(ctx(), h_avg_voltages.data(), d_avg_voltages.data(), N_regions * sizeof(Float)));
+            check(nnx_memcpy_d2h(ctx(), h_firing_per_region.data(), d_firing_per_region.data(),
+                                 N_regions * sizeof(int)));
+        }
+    }
+    static Float MeasureAverageEntireNetwork(const Float* V_ptr, int const N) {
+        static DeviceVector<Float> out(1);
+        check(nnx_network_mean(ctx(), NNX_F32, N, V_ptr, out.data()));
+        Float m = 0;
+        check(nnx_memcpy_d2h(ctx(), &m, out.data(), sizeof(Float)));
+        return m;
+    }
+    const Float* ElectrodeOutputHandle() const { return h_avg_voltages.data(); }
+    const int* FiringPerRegionHandle() const { return h_firing_per_region.data(); }
+};
+
+// chi^2 synchrony of sub-populations (the first sub_sample_counts[s]
+// neurons of the region) over the timesteps measured so far.
+class SynchronyNet {
+    std::vector<int> h_region_inds;
+    DeviceVector<int32_t> d_region_inds;
+    std::vector<int> sub_sample_counts;
+    DeviceVector<int32_t> d_counts;
+    DeviceVector<double> d_Vi_sum, d_Vi2_sum, d_Vavg_sum, d_Vavg2_sum, d_chi;
+    int64_t n_;
+    int N_temporal_steps = 0;
+
+   public:
+    SynchronyNet(std::vector<int>&& h_region_inds_, std::vector<int> sub_sample_rates_)
+        : h_region_inds(std::move(h_region_inds_)),
+          d_region_inds(h_region_inds.empty() ? std::vector<int>(1, 0) : h_region_inds),
+          sub_sample_counts(std::move(sub_sample_rates_)),
+          d_counts(sub_sample_counts.empty() ? std::vector<int>(1, 0) : sub_sample_counts),
+          d_Vi_sum(std::max<size_t>(h_region_inds.size(), 1)), d_Vi2_sum(std::max<size_t>(h_region_inds.size(), 1)),
+          d_Vavg_sum(std::max<size_t>(sub_sample_counts.size(), 1)),
+          d_Vavg2_sum(std::max<size_t>(sub_sample_counts.size(), 1)),
+          d_chi(std::max<size_t>(sub_sample_counts.size(), 1)), n_(index_bound(h_region_inds)) {}
+
+    void MeasureVariances(const Float* V_ptr) {
+        check(nnx_synchrony_accumulate(ctx(), NNX_F32, n_, V_ptr, (int64_t)h_region_inds.size(),
+                                       d_region_inds.data(), (int32_t)sub_sample_counts.size(), d_counts.data(),
+                                       d_Vi_sum.data(), d_Vi2_sum.data(), d_Vavg_sum.data(), d_Vavg2_sum.data()));
+        N_temporal_steps += 1;
+    }
+    std::vector<double> GetSynchronyMeasures() {
+        const int32_t ns = (int32_t)sub_sample_counts.size();
+        check(nnx_synchrony_measures(ctx(), (int64_t)h_region_inds.size(), ns, d_counts.data(), d_Vi_sum.data(),
+                                     d_Vi2_sum.data(), d_Vavg_sum.data(), d_Vavg2_sum.data(), N_temporal_steps,
+                                     d_chi.data()));
+        std::vector<double> chi(ns);
+        if (ns) check(nnx_memcpy_d2h(ctx(), chi.data(), d_chi.data(), ns * sizeof(double)));
+        return chi;
+    }
+};
+
+// Neuron positions and the spherical electrode regions built from them.
+struct Morphology {
+    std::vector<float> xyz;  // 3 per neuron, interleaved
+
+    explicit Morphology(std::vector<float> xyz_) : xyz(std::move(xyz_)) {}
+    int64_t size() const { return (int64_t)xyz.size() / 3; }
+
+    // region n = the k neurons nearest to neuron electrode_inds[n]
+    void GenSphericalRegions(std::vector<int>& h_regions, std::vector<int>& h_region_inds,
+                             const std::vector<int>& electrode_inds, int const k) const {
+        const int64_t nr = (int64_t)electrode_inds.size();
+        h_regions.assign(nr * k, 0);
+        h_region_inds.assign(nr + 1, 0);
+        if (nnx_spherical_regions(size(), xyz.data(), nr, electrode_inds.data(), nullptr, k, h_region_inds.data(),
+                                  h_regions.data()) != 0)
+            throw Error(NNX_ERR_ARG, "GenSphericalRegions: k or an electrode index out of range");
+    }
+    // region n = the k neurons nearest to the point electrode_positions[n]
+    void GenSphericalRegions(std::vector<int>& h_regions, std::vector<int>& h_region_inds,
+                             const std::vector<std::array<float, 3>>& electrode_positions, int const k) const {
+        const int64_t nr = (int64_t)electrode_positions.size();
+        std::vector<float> c(3 * nr);
+        for (int64_t r = 0; r < nr; ++r)
+            for (int d = 0; d < 3; ++d) c[3 * r + d] = electrode_positions[r][d];
+        h_regions.assign(nr * k, 0);
+        h_region_inds.assign(nr + 1, 0);
+        if (nnx_spherical_regions(size(), xyz.data(), nr, nullptr, c.data(), k, h_region_inds.data(),
+                                  h_regions.data()) != 0)
+            throw Error(NNX_ERR_ARG, "GenSphericalRegions: k out of range");
+    }
 };
 
 }  // namespace nnx_compat

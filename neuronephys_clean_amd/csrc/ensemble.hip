@@ -448,12 +448,19 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     // kernel argument would replay the first capture's number)
     k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, 1);
     NNX_LAUNCHED(c);
+    NNX_TRY(readout_prepare(c, p));
     auto body = [&]() -> nnx_status {
         k_push_bits<<<2048, 256, 0, c->stream>>>(p->list, p->count, p->graph.out_offsets, p->graph.out_targets,
                                                  p->graph.out_to_in, eb_q0, flag_q0);
         NNX_LAUNCHED(c);
-        for (int i = 0; i < n_steps; ++i)
-            NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1), i + 1 < n_steps));
+        for (int i = 0; i < n_steps; ++i) {
+            const uint64_t st = first_step + i;
+            // print step (as in nnx_run_steps): the step counter slot of the
+            // previous step holds st - 1
+            if (p->readout_every > 0 && st % (uint64_t)p->readout_every == 0)
+                NNX_TRY(readout_launch(c, p, h.k.stepctr + ((st + 1) & 1), 1, p->bits[st & 1]));
+            NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)(st & 1), i + 1 < n_steps));
+        }
         CompactExtras ex{};
         return compact_launch(c, s.n, p->bits[(first_step + n_steps - 1) & 1], p->list, p->count, ex);
     };
@@ -462,6 +469,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     const unsigned char* u = (const unsigned char*)p;
     for (size_t i = 0; i < sizeof(*p); ++i) key = (key ^ u[i]) * 1099511628211ull;
     key ^= 0x4655534544535450ull;
+    if (p->readout_every > 0) key ^= (first_step % (uint64_t)p->readout_every) * 0x9e3779b97f4a7c15ull;
     return capture_or_replay(c, {(const void*)(uintptr_t)key, n_steps * 2 + (int)(first_step & 1)}, body);
 }
 

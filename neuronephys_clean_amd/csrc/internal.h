@@ -116,6 +116,14 @@ nnx_status event_scratch(nnx_ctx* c, int64_t n, int64_t nnz, int32_t** cnt, int3
 nnx_status fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t first_step, int n_steps,
                            int use_graph);
 
+// electrode readout inside nnx_run_steps (readout.hip): plan fields checked,
+// scratch allocated before a capture, and the launches of one print step
+// (row = (*stepp + add) / readout_every, from the device step counter)
+nnx_status check_readout(nnx_ctx* c, const nnx_step_plan* p);
+nnx_status readout_prepare(nnx_ctx* c, const nnx_step_plan* p);
+nnx_status readout_launch(nnx_ctx* c, const nnx_step_plan* p, const int64_t* stepp, int64_t add,
+                          const uint64_t* bits);
+
 }  // namespace nnx
 
 // z = M(r) of the U-Net preconditioner (unet.hip); dims already checked

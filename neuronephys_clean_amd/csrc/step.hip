@@ -52,13 +52,18 @@ static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     if (p->voxel_of) NNX_CHECK(c, p->voxel_counts, "voxel_of needs voxel_counts");
     if (p->params.noise_std != 0.0)
         return fail(c, NNX_ERR_UNSUPPORTED, "white noise is not implemented (noise_std must be 0)");
-    return NNX_OK;
+    return check_readout(c, p);
 }
 
 static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, int64_t* ctr) {
     const uint64_t* prev = p->bits[(s + 1) & 1];
     uint64_t* cur = p->bits[s & 1];
     const bool rk4 = p->integrator == NNX_INT_RK4;
+    // print step: the state entering step s; cur still holds the spikes of
+    // step s-2, which the coupling of step s-1 read (the reference's
+    // is_firing at its measurement); the row comes from the device step
+    if (p->readout_every > 0 && s % (uint64_t)p->readout_every == 0)
+        NNX_TRY(readout_launch(c, p, ctr + 2 + (s & 1), 0, cur));
     if (rk4 && p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT) {
         // push the previous step's spikes into the targets' buckets; the
         // ephys kernel sums each bucket in source order before its update
@@ -103,6 +108,7 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
         int32_t *a, *b;
         NNX_TRY(event_scratch(c, p->graph.n, p->graph.nnz, &a, &b));
     }
+    NNX_TRY(readout_prepare(c, p));
     void* ctrp;
     NNX_TRY(scratch(c, "step_ctr", 64, true, &ctrp));
     int64_t* ctr = (int64_t*)ctrp;  // [0..1] raster length, [2..3] step, by parity
@@ -124,7 +130,9 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
 
 static nnx_status replay(nnx_ctx* c, nnx_step_plan* p, uint64_t first_step, int n_steps, int64_t* ctr) {
     NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
-    const std::pair<const void*, int> key((const void*)(uintptr_t)plan_hash(p),
+    // the print steps are fixed in the captured sequence: key on their phase
+    const uint64_t phase = p->readout_every > 0 ? first_step % (uint64_t)p->readout_every : 0;
+    const std::pair<const void*, int> key((const void*)(uintptr_t)(plan_hash(p) ^ (phase * 0x9e3779b97f4a7c15ull)),
                                           n_steps * 2 + (int)(first_step & 1));
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {

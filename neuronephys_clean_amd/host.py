@@ -150,3 +150,19 @@ def face_coefficients(c):
     kz = np.empty((nz + 1, ny, nx), np.float32)
     host().nnx_face_coefficients(nx, ny, nz, _p(c), _p(kx), _p(ky), _p(kz))
     return kx, ky, kz
+
+
+def spherical_regions(pos, k, center_idx=None, center_pos=None):
+    """Electrode regions: the k nearest neurons of each center (neuron indices
+    or points), nearest first, ties by lower index.  Returns (offsets, idx)."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    ci = None if center_idx is None else np.ascontiguousarray(center_idx, np.int32)
+    cp = None if center_pos is None else np.ascontiguousarray(center_pos, np.float32).reshape(-1, 3)
+    nreg = len(ci) if ci is not None else len(cp)
+    off = np.empty(nreg + 1, np.int32)
+    idx = np.empty(nreg * int(k), np.int32)
+    rc = host().nnx_spherical_regions(pos.shape[0], _p(pos), nreg, _p(ci), _p(cp), int(k), _p(off), _p(idx))
+    if rc != 0:
+        raise ValueError(f"nnx_spherical_regions: bad arguments (k={k}, n={pos.shape[0]})")
+    return off, idx
+

@@ -517,4 +517,109 @@ void nnx_face_coefficients(int32_t nx, int32_t ny, int32_t nz, const float* c, f
         }
 }
 
+// k nearest neurons of each query point by squared Euclidean distance
+// (FP64 from the float coordinates), ties by lower index, nearest first.
+// A uniform cell grid over the bounding box (about 4 neurons per cell) is
+// searched in Chebyshev shells around the query's cell until the k-th
+// distance is below the distance to the searched block's boundary.
+int nnx_spherical_regions(int64_t n, const float* pos, int64_t n_regions, const int32_t* center_idx,
+                          const float* center_pos, int32_t k, int32_t* offsets, int32_t* idx) {
+    if (k < 1 || k > n || n_regions < 0 || (!center_idx && !center_pos && n_regions > 0)) return -1;
+    if (center_idx)
+        for (int64_t r = 0; r < n_regions; ++r)
+            if (center_idx[r] < 0 || center_idx[r] >= n) return -1;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = pos[d];
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], (double)pos[3 * i + d]);
+            hi[d] = std::max(hi[d], (double)pos[3 * i + d]);
+        }
+    double ext = 0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d]);
+    ext = ext > 0 ? ext : 1.0;
+    // cell edge from the box volume (degenerate axes count as one cell)
+    double vol = 1;
+    for (int d = 0; d < 3; ++d) vol *= std::max(hi[d] - lo[d], ext * 1e-6);
+    const double cs = std::max(std::cbrt(vol * 4.0 / (double)n), ext * 1e-6);
+    int64_t nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = std::min<int64_t>((int64_t)((hi[d] - lo[d]) / cs) + 1, 1024);
+    auto cell_of = [&](double x, int d) {
+        int64_t c = (int64_t)std::floor((x - lo[d]) / cs);
+        return std::min<int64_t>(std::max<int64_t>(c, 0), nc[d] - 1);
+    };
+    const int64_t ncell = nc[0] * nc[1] * nc[2];
+    std::vector<int64_t> start(ncell + 1, 0);
+    std::vector<int32_t> member(n);
+    std::vector<int64_t> cid(n);
+    for (int64_t i = 0; i < n; ++i) {
+        cid[i] = cell_of(pos[3 * i], 0) + nc[0] * (cell_of(pos[3 * i + 1], 1) + nc[1] * cell_of(pos[3 * i + 2], 2));
+        ++start[cid[i] + 1];
+    }
+    for (int64_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
+    {
+        std::vector<int64_t> fillp(start.begin(), start.end() - 1);
+        for (int64_t i = 0; i < n; ++i) member[fillp[cid[i]]++] = (int32_t)i;  // ascending within a cell
+    }
+    for (int64_t r = 0; r <= n_regions; ++r) offsets[r] = (int32_t)(r * k);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < n_regions; ++r) {
+        double q[3];
+        for (int d = 0; d < 3; ++d)
+            q[d] = center_idx ? (double)pos[3 * (int64_t)center_idx[r] + d] : (double)center_pos[3 * r + d];
+        int64_t qc[3];
+        for (int d = 0; d < 3; ++d) qc[d] = cell_of(q[d], d);
+        // max-heap of the best k as (distance, index), lexicographic
+        std::vector<std::pair<double, int32_t>> heap;
+        heap.reserve(k + 1);
+        for (int64_t ring = 0;; ++ring) {
+            bool any = false;
+            for (int64_t cz = qc[2] - ring; cz <= qc[2] + ring; ++cz) {
+                if (cz < 0 || cz >= nc[2]) continue;
+                for (int64_t cy = qc[1] - ring; cy <= qc[1] + ring; ++cy) {
+                    if (cy < 0 || cy >= nc[1]) continue;
+                    for (int64_t cx = qc[0] - ring; cx <= qc[0] + ring; ++cx) {
+                        if (cx < 0 || cx >= nc[0]) continue;
+                        const int64_t cheb = std::max({std::llabs(cx - qc[0]), std::llabs(cy - qc[1]),
+                                                       std::llabs(cz - qc[2])});
+                        if (cheb != ring) continue;
+                        any = true;
+                        const int64_t c = cx + nc[0] * (cy + nc[1] * cz);
+                        for (int64_t t = start[c]; t < start[c + 1]; ++t) {
+                            const int32_t i = member[t];
+                            double d2 = 0;
+                            for (int d = 0; d < 3; ++d) {
+                                const double dd = (double)pos[3 * (int64_t)i + d] - q[d];
+                                d2 += dd * dd;
+                            }
+                            const std::pair<double, int32_t> e(d2, i);
+                            if ((int32_t)heap.size() < k) {
+                                heap.push_back(e);
+                                std::push_heap(heap.begin(), heap.end());
+                            } else if (e < heap.front()) {
+                                std::pop_heap(heap.begin(), heap.end());
+                                heap.back() = e;
+                                std::push_heap(heap.begin(), heap.end());
+                            }
+                        }
+                    }
+                }
+            }
+            // every point outside the searched block is farther than this
+            double bound = INFINITY;
+            bool whole = true;
+            for (int d = 0; d < 3; ++d) {
+                if (qc[d] - ring > 0) bound = std::min(bound, q[d] - (lo[d] + (qc[d] - ring) * cs)), whole = false;
+                if (qc[d] + ring < nc[d] - 1)
+                    bound = std::min(bound, lo[d] + (qc[d] + ring + 1) * cs - q[d]), whole = false;
+            }
+            if (whole || (!any && ring > 2 * 1024)) break;
+            if ((int32_t)heap.size() == k && bound > 0 && heap.front().first < bound * bound) break;
+        }
+        std::sort_heap(heap.begin(), heap.end());
+        for (int32_t t = 0; t < k; ++t) idx[r * k + t] = heap[t].second;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -122,6 +122,15 @@ class Simulation:
     def synchronize(self):
         self.ctx.synchronize()
 
+    def set_readout(self, offsets, idx, every: int, cap: int, firing: bool = True, mean: bool = True):
+        """Electrode readout every `every` steps into device rows (readout.attach)."""
+        from . import readout
+        readout.attach(self, offsets, idx, every, cap, firing, mean)
+
+    def readout(self) -> dict:
+        from . import readout
+        return readout.rows(self)
+
     def set_fused(self, block=0, tile=0, grid=0, burst=None):
         """Launch shape of the fused step (results do not depend on it)."""
         p = self.plan
@@ -211,6 +220,15 @@ class ReferenceSimulation:
     def run(self, n_steps: int, use_graph: bool = False):
         self.ctx.call("nnx_run_steps", C.byref(self.plan), self.step, n_steps, int(use_graph))
         self.step += n_steps
+
+    def set_readout(self, offsets, idx, every: int, cap: int, firing: bool = True, mean: bool = True):
+        """Electrode readout every `every` steps into device rows (readout.attach)."""
+        from . import readout
+        readout.attach(self, offsets, idx, every, cap, firing, mean)
+
+    def readout(self) -> dict:
+        from . import readout
+        return readout.rows(self)
 
     def raster(self):
         ln = int(self.a["raster_len"].numpy()[0])

@@ -176,3 +176,46 @@ def run_leapfrog(p, dt, steps, off, tgt, mode, sv, cap=1 << 20):
         C.c_int64(sv[0].size), C.byref(p), C.c_double(dt), C.c_int64(steps), _p(off), _p(tgt),
         mode, arr, _p(raster), C.c_int64(cap))
     return raster.reshape(-1, 2)[:min(ln, cap)], ln
+
+
+def electrode_measure(off, idx, V, fire=None):
+    """Per-region (mean V, firing count) as ElectrodeNet::DoElectrodeMeasurements
+    (src_wholebrain/electrode_net.cu:3-30); fire: 0/1 per neuron or None."""
+    off = np.ascontiguousarray(off, np.int32)
+    idx = np.ascontiguousarray(idx, np.int32)
+    V = np.ascontiguousarray(V)
+    nreg = len(off) - 1
+    avg = np.empty(nreg, V.dtype)
+    cnt = np.zeros(nreg, np.int32)
+    f = None if fire is None else np.ascontiguousarray(fire, np.int32)
+    getattr(lib(), "orc_electrode_measure_" + _sfx(V.dtype))(
+        C.c_int64(nreg), C.c_int64(len(idx)), _p(off), _p(idx), C.c_int64(len(V)), _p(V), _p(f), _p(avg),
+        _p(cnt))
+    return avg, (cnt if fire is not None else None)
+
+
+def network_mean(V):
+    """Mean of V in the device's fixed FP64 summation order, rounded to V's dtype."""
+    V = np.ascontiguousarray(V)
+    fn = getattr(lib(), "orc_network_mean_" + _sfx(V.dtype))
+    fn.restype = C.c_float if V.dtype == np.float32 else C.c_double
+    return V.dtype.type(fn(C.c_int64(len(V)), _p(V)))
+
+
+def sync_accumulate(V, idx, counts, sums):
+    """One SynchronyNet::MeasureVariances step; sums = (vi, vi2, vavg, vavg2) FP64 arrays, updated."""
+    V = np.ascontiguousarray(V)
+    idx = np.ascontiguousarray(idx, np.int32)
+    counts = np.ascontiguousarray(counts, np.int32)
+    getattr(lib(), "orc_sync_accumulate_" + _sfx(V.dtype))(
+        C.c_int64(len(V)), _p(V), C.c_int64(len(idx)), _p(idx), C.c_int(len(counts)), _p(counts),
+        *(_p(a) for a in sums))
+
+
+def sync_measures(n_idx, counts, sums, steps):
+    """chi^2 per sub-population (SynchronyNet::GetSynchronyMeasures)."""
+    counts = np.ascontiguousarray(counts, np.int32)
+    chi = np.empty(len(counts), np.float64)
+    lib().orc_sync_measures(C.c_int64(n_idx), C.c_int(len(counts)), _p(counts), *(_p(a) for a in sums),
+                            C.c_int64(steps), _p(chi))
+    return chi

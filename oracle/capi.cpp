@@ -299,4 +299,28 @@ int64_t orc_run_leapfrog_f64(int64_t n, const orc_params* q, double dt, int64_t 
     return run_leapfrog<double>(n, q, dt, steps, off, tgt, mode, sv, raster, cap);
 }
 
+// Electrode readout, network mean and synchrony (see oracle.hpp).
+void orc_electrode_measure_f32(int64_t nreg, int64_t nnz, const int32_t* off, const int32_t* idx, int64_t n,
+                               const float* V, const int32_t* fire, float* avg, int32_t* cnt) {
+    electrode_measure<float>(nreg, nnz, off, idx, n, V, fire, avg, cnt);
+}
+void orc_electrode_measure_f64(int64_t nreg, int64_t nnz, const int32_t* off, const int32_t* idx, int64_t n,
+                               const double* V, const int32_t* fire, double* avg, int32_t* cnt) {
+    electrode_measure<double>(nreg, nnz, off, idx, n, V, fire, avg, cnt);
+}
+float orc_network_mean_f32(int64_t n, const float* V) { return network_mean<float>(n, V); }
+double orc_network_mean_f64(int64_t n, const double* V) { return network_mean<double>(n, V); }
+void orc_sync_accumulate_f32(int64_t n, const float* V, int64_t n_idx, const int32_t* idx, int n_sub,
+                             const int32_t* counts, double* vi, double* vi2, double* va, double* va2) {
+    synchrony_accumulate<float>(n, V, n_idx, idx, n_sub, counts, vi, vi2, va, va2);
+}
+void orc_sync_accumulate_f64(int64_t n, const double* V, int64_t n_idx, const int32_t* idx, int n_sub,
+                             const int32_t* counts, double* vi, double* vi2, double* va, double* va2) {
+    synchrony_accumulate<double>(n, V, n_idx, idx, n_sub, counts, vi, vi2, va, va2);
+}
+void orc_sync_measures(int64_t n_idx, int n_sub, const int32_t* counts, const double* vi, const double* vi2,
+                       const double* va, const double* va2, int64_t steps, double* chi) {
+    synchrony_measures(n_idx, n_sub, counts, vi, vi2, va, va2, steps, chi);
+}
+
 }  // extern "C"

@@ -17,11 +17,14 @@
 //   src/network_factory.cu:5-27 CSR + in-degree
 //   src/paracrine.cu:19-134     27-point CN-CG
 //   src/main.cu:166-181         step order coupling -> paracrine -> ephys
+//   src/electrode_net.cu:3-47, src_wholebrain/electrode_net.cu:3-126
+//                               electrode readout and synchrony
 #pragma once
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace oracle {
@@ -457,6 +460,126 @@ inline int update_density_reference(int nx, int ny, int nz, double h, double D, 
     std::memcpy(data, xg.data(), n * sizeof(float));
     (void)L;
     return it;
+}
+
+// --------------------------------------------------------------------------
+// Electrode readout (src_wholebrain/electrode_net.cu:3-30, which adds the
+// firing count to src/electrode_net.cu:3-21): one region at a time, its
+// voltages added in index order in T, divided by the region's size (NaN for
+// an empty region, 0/0 as there).  Outside the reference's domain, and as the
+// device defines it: an index outside [0, n) contributes NaN and does not
+// count as firing; offsets are clamped to [0, nnz].  fire: 0/1 per neuron
+// (the reference's is_firing), NULL = no counts.
+// --------------------------------------------------------------------------
+template <class T>
+void electrode_measure(int64_t nreg, int64_t nnz, const int32_t* off, const int32_t* idx, int64_t n,
+                       const T* V, const int32_t* fire, T* avg, int32_t* cnt) {
+    for (int64_t r = 0; r < nreg; ++r) {
+        int64_t b = std::min<int64_t>(std::max<int64_t>(off[r], 0), nnz);
+        int64_t e = std::min<int64_t>(std::max<int64_t>(off[r + 1], b), nnz);
+        T s = 0;
+        int32_t c = 0;
+        for (int64_t j = b; j < e; ++j) {
+            const int32_t i = idx[j];
+            if (i >= 0 && i < n) {
+                s += V[i];
+                if (fire) c += fire[i];
+            } else {
+                s += std::numeric_limits<T>::quiet_NaN();
+            }
+        }
+        avg[r] = s / T(e - b);
+        if (fire) cnt[r] = c;
+    }
+}
+
+// Mean of V (ElectrodeNet::MeasureAverageEntireNetwork,
+// src/electrode_net.cu:44-47).  The reference's fp32 thrust::reduce has no
+// specified order; this is the order the device uses: 2^18 lanes (1024
+// blocks of 256), lane L adding elements L, L + 2^18, ... in FP64; a block
+// total is the butterfly of each 64-lane wave (pairs (l, l+o) for o = 32 ..
+// 1) followed by its 4 waves in order; the 1024 block totals are summed the
+// same way by 256 lanes taking t, t+256, t+512, t+768; mean = total / n,
+// rounded to T.
+inline double wave_tree64(double* v) {
+    for (int o = 32; o > 0; o >>= 1)
+        for (int l = 0; l < o; ++l) v[l] = v[l] + v[l + o];
+    return v[0];
+}
+inline double block_total(const double* lanes, int nt) {
+    std::vector<double> w(64);
+    double t = 0;
+    for (int wv = 0; wv < nt / 64; ++wv) {
+        std::copy(lanes + 64 * wv, lanes + 64 * wv + 64, w.begin());
+        const double x = wave_tree64(w.data());
+        t = wv == 0 ? x : t + x;
+    }
+    return t;
+}
+template <class T> T network_mean(int64_t n, const T* V) {
+    const int NB = 1024, NT = 256;
+    const int64_t stride = (int64_t)NB * NT;
+    std::vector<double> lanes(stride, 0.0), part(NB), l2(NT, 0.0);
+    for (int64_t i = 0; i < n; ++i) lanes[i % stride] += (double)V[i];
+    for (int b = 0; b < NB; ++b) part[b] = block_total(&lanes[(size_t)b * NT], NT);
+    for (int t = 0; t < NT; ++t)
+        for (int k = t; k < NB; k += NT) l2[t] += part[k];
+    return (T)(block_total(l2.data(), NT) / (double)n);
+}
+
+// Synchrony accumulation of one timestep (SynchronyNet::MeasureVariances,
+// src_wholebrain/electrode_net.cu:54-97): per-neuron sums with the square
+// formed in T; each sub-population's Vsum is an FP64 sum in the device's
+// order (1024 lanes, lane L adding entries L, L+1024, ...; block total as
+// in network_mean with 16 waves) rounded to T (the reference: an fp32
+// thrust::reduce of unspecified order), Vavg = Vsum / count in T.
+template <class T>
+void synchrony_accumulate(int64_t n, const T* V, int64_t n_idx, const int32_t* idx, int n_sub,
+                          const int32_t* counts, double* vi, double* vi2, double* vavg, double* vavg2) {
+    auto get = [&](int32_t i) { return (i >= 0 && i < n) ? V[i] : std::numeric_limits<T>::quiet_NaN(); };
+    for (int64_t j = 0; j < n_idx; ++j) {
+        const T x = get(idx[j]);
+        vi2[j] += (double)(x * x);
+        vi[j] += (double)x;
+    }
+    for (int s = 0; s < n_sub; ++s) {
+        const int32_t cnt = counts[s];
+        double d = std::numeric_limits<double>::quiet_NaN();
+        if (cnt >= 1 && cnt <= n_idx) {
+            std::vector<double> lanes(1024, 0.0);
+            for (int32_t j = 0; j < cnt; ++j) lanes[j % 1024] += (double)get(idx[j]);
+            const T vs = (T)block_total(lanes.data(), 1024);
+            d = (double)(vs / (T)cnt);
+        }
+        vavg[s] += d;
+        vavg2[s] += d * d;
+    }
+}
+
+// chi^2 per sub-population (SynchronyNet::GetSynchronyMeasures,
+// src_wholebrain/electrode_net.cu:99-126), the same operations in order.
+inline void synchrony_measures(int64_t n_idx, int n_sub, const int32_t* counts, const double* vi,
+                               const double* vi2, const double* vavg, const double* vavg2, int64_t steps,
+                               double* chi) {
+    const double T_ = (double)steps;
+    for (int s = 0; s < n_sub; ++s) {
+        const int32_t cnt = counts[s];
+        if (cnt < 1 || cnt > n_idx) {
+            chi[s] = std::numeric_limits<double>::quiet_NaN();
+            continue;
+        }
+        double denom = 0;
+        for (int32_t j = 0; j < cnt; ++j) {
+            double a2 = vi2[j], a1 = vi[j];
+            if (steps > 0) {
+                a2 /= T_;
+                a1 /= T_;
+            }
+            denom += a2 - a1 * a1;
+        }
+        const double sig = (vavg2[s] - vavg[s] * vavg[s] / T_) / T_;
+        chi[s] = (double)cnt * sig / denom;
+    }
 }
 
 }  // namespace oracle

@@ -59,16 +59,37 @@ def test_compat_loop_matches_oracle(tmp_path):
     net.tgt.astype(np.int32).tofile(tmp_path / "tgt.i32")
     np.concatenate(sv).tofile(tmp_path / "sv.f32")
     pos.tofile(tmp_path / "pos.f32")
+    print_incs, render_N, kreg = 25, 12, 30
     args = [DEMO, str(tmp_path), str(nincs), str(nx), str(ny), str(nz), repr(h), repr(D),
-            repr(decay), repr(par_dt), str(ts), repr(coup)]
+            repr(decay), repr(par_dt), str(ts), repr(coup), str(print_incs), str(render_N), str(kreg)]
     print("compat_demo", " ".join(args[1:]), flush=True)
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
 
-    # ephys + coupling: identical to the oracle's leapfrog run
+    # electrodes (src/main.cu:133-141, 279-285): regions by brute-force kNN,
+    # each print row the oracle's region means and network mean of the state
+    # entering that increment
     q = _lib.params_reference()
     q.noise_std = 0.0
+    xyz = np.ascontiguousarray(pos.T)
+    centers = (net.n // render_N) * np.arange(render_N)
+    regions = np.fromfile(tmp_path / "regions.i32", np.int32)
+    for e, c in enumerate(centers):
+        d2 = ((xyz.astype(np.float64) - xyz[c].astype(np.float64)) ** 2).sum(1)
+        assert regions[e * kreg:(e + 1) * kreg].tolist() == np.lexsort((np.arange(net.n), d2))[:kreg].tolist()
+    rows = np.fromfile(tmp_path / "electrodes.f32", np.float32).reshape(-1, render_N + 1)
+    assert len(rows) == nincs // print_incs
+    sv_e = [a.copy() for a in sv]
+    roff = np.arange(0, (render_N + 1) * kreg, kreg, dtype=np.int32)
+    for r in range(len(rows)):
+        if r:
+            oracle.run_leapfrog(oracle.params(q.as_dict()), 0.4, print_incs, net.off, net.tgt, 2, sv_e, cap=1)
+        avg, _ = oracle.electrode_measure(roff, regions, sv_e[0])
+        assert np.array_equal(rows[r, :render_N], avg), r
+        assert rows[r, render_N] == oracle.network_mean(sv_e[0]), r
+
+    # ephys + coupling: identical to the oracle's leapfrog run
     ref, rln = oracle.run_leapfrog(oracle.params(q.as_dict()), 0.4, nincs, net.off, net.tgt, 2, sv,
                                    cap=1_000_000)
     ras = np.fromfile(tmp_path / "raster.i32", np.int32).reshape(-1, 2)
