@@ -323,6 +323,16 @@ def profile_pass(ctx, sim, net):
         ctx.call("nnx_hh_step", C.byref(st), C.byref(p), _lib.INT_RK4, _lib.SPIKE_CROSSING, net.dt, 0,
                  sim.a["g_in"].ptr, sim.a["bits0"].ptr)
     ctx.synchronize()
+    # 20 electrode readouts (1,000 electrodes x 50 nearest neurons) and network means
+    from neuronephys_clean_amd import host, readout
+    off, idx = host.spherical_regions(net.pos, 50, center_idx=(net.n // 1000) * np.arange(1000))
+    enet = readout.ElectrodeNet(ctx, idx, off)
+    avg, mean = DA(ctx, 1000, np.float32), DA(ctx, 1, np.float32)
+    for _ in range(20):
+        ctx.call("nnx_electrode_measure", C.byref(enet.r), _lib.F32, net.n, sim.a["v"].ptr, sim.a["bits0"].ptr,
+                 avg.ptr, enet.d_firing.ptr)
+        ctx.call("nnx_network_mean", _lib.F32, net.n, sim.a["v"].ptr, mean.ptr)
+    ctx.synchronize()
     # 20 steps of the 256-network ensemble (default launch shape)
     ens = EnsembleSim(configs.c4_ensemble(seed=0, count=256), ctx=ctx)
     ens.run(20)
@@ -356,7 +366,8 @@ def profile_pass(ctx, sim, net):
     src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
-    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
+    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 electrode readouts, 20 ensemble steps, "
+        "5 network applications, "
         f"20 stencil applies, 3 V-cycles, 1 CG solve ({r.iters} iterations), 256 MiB copy")
 
 
@@ -371,6 +382,100 @@ def profiled_traffic(label):
     except OSError:
         return None
     return float(m.group(1)) if m else None
+
+
+SNAP_KEYS = ("v", "m", "h", "ng", "s", "bits0", "bits1", "list", "count", "voxel_counts", "raster_len")
+
+
+def snapshot(ctx, sim):
+    """Device copy of everything a fused C2 run reads between calls."""
+    from neuronephys_clean_amd.devmem import DeviceArray
+    snap = {}
+    for kk in SNAP_KEYS:
+        snap[kk] = DeviceArray(ctx, sim.a[kk].shape, sim.a[kk].dtype)
+        snap[kk].copy_from(sim.a[kk])
+    return snap, sim.step
+
+
+def readout_bench(ctx, sim, net, chunk, snap, steps=2000, every=40, n_elec=1000, k=50):
+    """Electrode readout on the C2 run: 1,000 electrodes, each the mean of
+    its 50 nearest neurons, plus the network mean, read every 40 steps (the
+    reference's 2.5 kHz sampling, src/main.cu:547,574-575, at dt 0.01 ms).
+    The benchmark's timed window is run three more times from the state it
+    started from (the run is deterministic): without readout, with it every
+    40 steps and every step; the time differences are the readout's cost
+    inside the step graph.  Graphs are captured before each timed window."""
+    from neuronephys_clean_amd import host
+
+    centers = (net.n // n_elec) * np.arange(n_elec)
+    off, idx = host.spherical_regions(net.pos, k, center_idx=centers)
+    keys = SNAP_KEYS
+    snap, step0 = snap
+
+    def restore():
+        for kk in keys:
+            sim.a[kk].copy_from(snap[kk])
+        sim.step = step0
+
+    def window():
+        restore()
+        sim.run(2 * chunk, use_graph=True, chunk=chunk)  # captures this plan's graphs
+        restore()
+        sim.synchronize()
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        ctx.call("nnx_event_create", C.byref(e0))
+        ctx.call("nnx_event_create", C.byref(e1))
+        ctx.call("nnx_event_record", e0)
+        sim.run(steps, use_graph=True, chunk=chunk)
+        ctx.call("nnx_event_record", e1)
+        sim.synchronize()
+        msv = C.c_float()
+        ctx.call("nnx_event_elapsed", e0, e1, C.byref(msv))
+        ctx.call("nnx_event_destroy", e0)
+        ctx.call("nnx_event_destroy", e1)
+        ms = msv.value
+        v = sim.a["v"].numpy()
+        spk = int(sim.a["raster_len"].numpy()[0]) - int(snap["raster_len"].numpy()[0])
+        return ms, (float(np.sum(v.astype(np.float64))), spk), v.view(np.uint32)[::4099].copy()
+
+    t_none, sum0, probe0 = window()
+    sim.set_readout(off, idx, every, steps // every + 1)
+    t_40, sum40, probe40 = window()
+    rows = sim.readout()
+    sim.set_readout(off, idx, 1, steps + 1)
+    t_1, sum1, probe1 = window()
+    sim.plan.readout_every = 0
+    same = sum0 == sum40 == sum1 and np.array_equal(probe0, probe40) and np.array_equal(probe0, probe1)
+    per_read_us = (t_1 - t_none) * 1e3 / steps
+    # the kernels alone, eager launches (an upper bound: includes launch gaps)
+    r = sim.plan.regions
+    fire = sim.readout()["firing"]
+
+    def f_measure():
+        ctx.call("nnx_electrode_measure", C.byref(r), _lib_f32(), net.n, sim.a["v"].ptr, sim.a["bits0"].ptr,
+                 sim.a["ro_avg"].ptr, sim.a["ro_firing"].ptr)
+
+    def f_mean():
+        ctx.call("nnx_network_mean", _lib_f32(), net.n, sim.a["v"].ptr, sim.a["ro_mean"].ptr)
+    m_us = timed_events(ctx, f_measure, 200) * 1e3
+    mean_us = timed_events(ctx, f_mean, 200) * 1e3
+    alg = (8.125 * len(idx) + 12 * n_elec)  # idx + V + spike bit per index; offset, mean, count per region
+    return {"workload": f"{n_elec} electrodes x {k} nearest neurons + network mean, every {every} steps, "
+                        f"the {steps}-step timed C2 window replayed from its saved start",
+            "window_ms": {"no_readout": t_none, f"every_{every}": t_40, "every_step": t_1},
+            "overhead_frac_every_%d" % every: (t_40 - t_none) / t_none,
+            "per_readout_us_in_graph": per_read_us,
+            "eager_launch_us": {"electrode_measure": m_us, "network_mean": mean_us},
+            "electrode_measure_alg_bytes": alg,
+            "rows": int(len(rows["avg"]) - 1), "firing_in_rows": int(rows["firing"].sum()),
+            "firing_every_step_rows": int(fire.sum()),
+            "spikes_in_window": sum0[1],
+            "simulation_unchanged": bool(same)}
+
+
+def _lib_f32():
+    from neuronephys_clean_amd import _lib
+    return _lib.F32
 
 
 def tune_fused(ctx, sim, chunk):
@@ -417,7 +522,7 @@ def main():
                          "region (the timed window then starts later in the simulation)")
     ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
                     help="profile: a short eager sequence for rocprofv3 --pmc passes (20 fused and 20 "
-                         "unfused C2 steps, 20 HH launches, 20 ensemble steps, 5 network applications, "
+                         "unfused C2 steps, 20 HH launches, 20 electrode readouts, 20 ensemble steps, 5 network applications, "
                          "20 stencil applies, 3 V-cycles and one CG solve on 256^3 grids, one 256 MiB "
                          "device copy to calibrate FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--no-cpu", action="store_true")
@@ -425,6 +530,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-readout", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
@@ -462,6 +568,7 @@ def main():
         sim.set_fused(tune["block"], tune["tile"], tune["grid"])
         log(f"bench: fused launch shape {tune}")
     len0 = sim.raster()[1]
+    snap = snapshot(ctx, sim) if args.couple == "fused" and not args.no_readout else None
     e0, e1 = C.c_void_p(), C.c_void_p()
     ctx.call("nnx_event_create", C.byref(e0))
     ctx.call("nnx_event_create", C.byref(e1))
@@ -521,6 +628,11 @@ def main():
                  sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
     couple_ms = timed_events(ctx, couple, 50)
 
+    ro = None
+    if snap is not None:
+        ro = readout_bench(ctx, sim, net, args.chunk, snap, steps=args.steps)
+        log(f"bench: readout {ro}")
+
     roof_stencil = stencil_roofline(ctx)
     log(f"bench: stencil {roof_stencil}")
 
@@ -567,7 +679,7 @@ def main():
                       "spikes_per_step": spikes / args.steps},
            "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
            "roofline_hh_unfused": roof_hh,
-           "c3": c3, "c4": c4, "c5": c5,
+           "c3": c3, "c4": c4, "c5": c5, "readout": ro,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},
            "event_ms": ev_ms.value}
