@@ -565,6 +565,42 @@ nnx_status nnx_synchrony_measures(nnx_ctx* ctx, int64_t n_idx, int32_t n_sub, co
                                   double* d_chi);
 
 /* ------------------------------------------------------------------------ */
+/* Hebbian plasticity                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Hebbian learning of per-synapse weights (Hebbian, src_wholebrain/
+ * hebbian.cuh:3-27, hebbian.cu:3-69).  The reference keeps a countdown per
+ * synapse, restarts it at `window` whenever the source fires, and on every
+ * call decrements every running countdown: at zero the weight goes down by
+ * heb_inc * exp(-|w - 1|); if the target fires first it goes up by as much
+ * and the countdown stops.  The same rule is applied here event by event: a
+ * call touches the out-synapses of the sources firing now, the in-synapses
+ * of the targets firing now and the out-synapses of the sources whose
+ * window ends now, instead of every synapse; weights and which synapses are
+ * counting down are identical to the reference's after every call.
+ * State (device arrays the caller owns; nnx_hebbian_init clears it):       */
+typedef struct nnx_hebbian {
+    int32_t window;          /* the reference's window_duration (>= 1); 0 in a step plan = off */
+    float heb_inc;
+    float* weights;          /* nnz, CSR order (AdjacencyList::weights) */
+    uint8_t* armed;          /* nnz: 1 while the synapse's countdown runs */
+    int32_t* last_fire;      /* n: step of each neuron's last firing seen */
+    uint64_t* ring;          /* window * ceil(n/64) words: the last `window` firing bitmaps */
+    int32_t* in_to_out;      /* nnz: CSR position of each CSC entry (built by init) */
+} nnx_hebbian;
+
+/* Clears armed, last_fire and ring, and builds in_to_out from the graph's
+ * out_to_in.  Weights are left as they are.                                 */
+nnx_status nnx_hebbian_init(nnx_ctx* ctx, const nnx_graph* g, const nnx_hebbian* h);
+
+/* One call of Hebbian::UpdateWeights with the firing set d_firing_bits
+ * (the bitmap the coupling read: the reference's is_firing and
+ * firing_inds).  `step` numbers the calls: consecutive calls must pass
+ * consecutive steps (the countdowns advance by one per call).               */
+nnx_status nnx_hebbian_update(nnx_ctx* ctx, const nnx_graph* g, const nnx_hebbian* h, int64_t step,
+                              const uint64_t* d_firing_bits);
+
+/* ------------------------------------------------------------------------ */
 /* composite timestep                                                        */
 /* ------------------------------------------------------------------------ */
 
@@ -612,6 +648,13 @@ typedef struct nnx_step_plan {
     void* readout_avg;         /* readout_cap * n_regions */
     int32_t* readout_firing;   /* readout_cap * n_regions */
     void* readout_mean;        /* readout_cap */
+    /* Hebbian learning (optional; hebbian.window = 0 = off): after the
+     * coupling of each step s, nnx_hebbian_update(step s) with the spike
+     * bitmap that coupling read, as the reference's HebbianLearningExample
+     * calls UpdateWeights after each Simulate() (src_wholebrain/main.cu:
+     * 158-168).  The learned weights are not the coupling's weights (the
+     * reference's coupling does not read them either).                    */
+    nnx_hebbian hebbian;
 } nnx_step_plan;
 
 /* Runs n_steps timesteps starting at global step `first_step`.  With

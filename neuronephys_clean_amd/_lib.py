@@ -85,6 +85,11 @@ class Regions(C.Structure):
     _fields_ = [("n_regions", C.c_int64), ("nnz", C.c_int64), ("offsets", vp), ("idx", vp)]
 
 
+class Hebbian(C.Structure):
+    _fields_ = [("window", C.c_int32), ("heb_inc", C.c_float), ("weights", vp), ("armed", vp),
+                ("last_fire", vp), ("ring", vp), ("in_to_out", vp)]
+
+
 class StepPlan(C.Structure):
     _fields_ = [("state", HHState), ("params", HHParams), ("graph", Graph),
                 ("integrator", C.c_int32), ("spike_rule", C.c_int32), ("couple_mode", C.c_int32),
@@ -94,7 +99,7 @@ class StepPlan(C.Structure):
                 ("fused_block", C.c_int32), ("fused_tile", C.c_int32), ("fused_grid", C.c_int32),
                 ("fused_burst", C.c_int32), ("regions", Regions), ("readout_every", C.c_int64),
                 ("readout_cap", C.c_int64), ("readout_avg", vp), ("readout_firing", vp),
-                ("readout_mean", vp)]
+                ("readout_mean", vp), ("hebbian", Hebbian)]
 
 
 class EnsPlan(C.Structure):
@@ -158,6 +163,8 @@ _SIGS = {
     "nnx_synchrony_accumulate": ([vp, C.c_int, C.c_int64, vp, C.c_int64, vp, C.c_int32, vp, vp, vp, vp,
                                   vp], C.c_int),
     "nnx_synchrony_measures": ([vp, C.c_int64, C.c_int32, vp, vp, vp, vp, vp, C.c_int64, vp], C.c_int),
+    "nnx_hebbian_init": ([vp, _P(Graph), _P(Hebbian)], C.c_int),
+    "nnx_hebbian_update": ([vp, _P(Graph), _P(Hebbian), C.c_int64, vp], C.c_int),
     "nnx_unet_param_count": ([], C.c_int64),
     "nnx_unet_create": ([vp, vp, C.c_uint64, _P(vp)], C.c_int),
     "nnx_unet_destroy": ([vp, vp], C.c_int),

@@ -459,6 +459,9 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
             // previous step holds st - 1
             if (p->readout_every > 0 && st % (uint64_t)p->readout_every == 0)
                 NNX_TRY(readout_launch(c, p, h.k.stepctr + ((st + 1) & 1), 1, p->bits[st & 1]));
+            if (p->hebbian.window > 0)
+                NNX_TRY(hebbian_launch(c, &p->graph, &p->hebbian, h.k.stepctr + ((st + 1) & 1), 1, 0,
+                                       p->bits[(st + 1) & 1]));
             NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)(st & 1), i + 1 < n_steps));
         }
         CompactExtras ex{};

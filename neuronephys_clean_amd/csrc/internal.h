@@ -124,6 +124,12 @@ nnx_status readout_prepare(nnx_ctx* c, const nnx_step_plan* p);
 nnx_status readout_launch(nnx_ctx* c, const nnx_step_plan* p, const int64_t* stepp, int64_t add,
                           const uint64_t* bits);
 
+// Hebbian learning inside nnx_run_steps (hebbian.hip): step = *stepp + add
+// (stepp NULL: step)
+nnx_status hebbian_launch(nnx_ctx* c, const nnx_graph* g, const nnx_hebbian* h, const int64_t* stepp, int64_t add,
+                          int64_t step, const uint64_t* fire);
+nnx_status check_hebbian_plan(nnx_ctx* c, const nnx_step_plan* p);
+
 }  // namespace nnx
 
 // z = M(r) of the U-Net preconditioner (unet.hip); dims already checked

@@ -52,6 +52,7 @@ static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     if (p->voxel_of) NNX_CHECK(c, p->voxel_counts, "voxel_of needs voxel_counts");
     if (p->params.noise_std != 0.0)
         return fail(c, NNX_ERR_UNSUPPORTED, "white noise is not implemented (noise_std must be 0)");
+    NNX_TRY(check_hebbian_plan(c, p));
     return check_readout(c, p);
 }
 
@@ -64,6 +65,8 @@ static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, i
     // is_firing at its measurement); the row comes from the device step
     if (p->readout_every > 0 && s % (uint64_t)p->readout_every == 0)
         NNX_TRY(readout_launch(c, p, ctr + 2 + (s & 1), 0, cur));
+    // Hebbian learning on the spikes this step's coupling reads
+    if (p->hebbian.window > 0) NNX_TRY(hebbian_launch(c, &p->graph, &p->hebbian, ctr + 2 + (s & 1), 0, 0, prev));
     if (rk4 && p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT) {
         // push the previous step's spikes into the targets' buckets; the
         // ephys kernel sums each bucket in source order before its update

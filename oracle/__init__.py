@@ -219,3 +219,13 @@ def sync_measures(n_idx, counts, sums, steps):
     lib().orc_sync_measures(C.c_int64(n_idx), C.c_int(len(counts)), _p(counts), *(_p(a) for a in sums),
                             C.c_int64(steps), _p(chi))
     return chi
+
+
+def hebbian_update(off, tgt, fire, window, heb_inc, timers, w):
+    """One Hebbian::UpdateWeights call; timers (int32) and w (float32) updated in place."""
+    off = np.ascontiguousarray(off, np.int32)
+    tgt = np.ascontiguousarray(tgt, np.int32)
+    f = np.ascontiguousarray(fire, np.uint8)
+    assert timers.dtype == np.int32 and w.dtype == np.float32
+    lib().orc_hebbian_update(C.c_int64(len(off) - 1), _p(off), _p(tgt), _p(f), C.c_int32(window),
+                             C.c_float(heb_inc), _p(timers), _p(w))

@@ -323,4 +323,10 @@ void orc_sync_measures(int64_t n_idx, int n_sub, const int32_t* counts, const do
     synchrony_measures(n_idx, n_sub, counts, vi, vi2, va, va2, steps, chi);
 }
 
+// One Hebbian::UpdateWeights call (see oracle.hpp).
+void orc_hebbian_update(int64_t n, const int32_t* off, const int32_t* tgt, const uint8_t* fire, int32_t window,
+                        float heb_inc, int32_t* timers, float* w) {
+    hebbian_update(n, off, tgt, fire, window, heb_inc, timers, w);
+}
+
 }  // extern "C"

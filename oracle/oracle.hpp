@@ -19,6 +19,7 @@
 //   src/main.cu:166-181         step order coupling -> paracrine -> ephys
 //   src/electrode_net.cu:3-47, src_wholebrain/electrode_net.cu:3-126
 //                               electrode readout and synchrony
+//   src_wholebrain/hebbian.cu:3-69   Hebbian learning
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -579,6 +580,33 @@ inline void synchrony_measures(int64_t n_idx, int n_sub, const int32_t* counts, 
         }
         const double sig = (vavg2[s] - vavg[s] * vavg[s] / T_) / T_;
         chi[s] = (double)cnt * sig / denom;
+    }
+}
+
+// --------------------------------------------------------------------------
+// Hebbian::UpdateWeights (src_wholebrain/hebbian.cu:3-69), as written: the
+// countdown of every out-synapse of a firing source restarts at `window`
+// (set_timers), then every synapse with a running countdown decrements it;
+// at zero its weight goes down by heb_inc * exp(-|w - 1|), otherwise, if its
+// target fires, it goes up by as much and the countdown stops.  The exp is
+// this file's oexp (the reference calls CUDA's expf).  fire: 0/1 per neuron.
+// --------------------------------------------------------------------------
+inline void hebbian_update(int64_t n, const int32_t* off, const int32_t* tgt, const uint8_t* fire, int32_t window,
+                           float heb_inc, int32_t* timers, float* w) {
+    for (int64_t p = 0; p < n; ++p)
+        if (fire[p])
+            for (int32_t e = off[p]; e < off[p + 1]; ++e) timers[e] = window;
+    const int64_t nnz = off[n];
+    for (int64_t e = 0; e < nnz; ++e) {
+        if (timers[e] == 0) continue;
+        timers[e] -= 1;
+        const float alpha = oexp(-std::fabs(w[e] - 1.0f));
+        if (timers[e] == 0)
+            w[e] -= heb_inc * alpha;
+        else if (fire[tgt[e]]) {
+            w[e] += heb_inc * alpha;
+            timers[e] = 0;
+        }
     }
 }
 
