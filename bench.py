@@ -397,6 +397,60 @@ def snapshot(ctx, sim):
     return snap, sim.step
 
 
+def replay_window(ctx, sim, snap, chunk, steps):
+    """Restores the saved state, captures the plan's graphs on 2 chunks,
+    restores again and times `steps` steps (HIP events).  Returns the time,
+    (sum of V, spikes) after the window and a sample of V's bits."""
+    snapd, step0 = snap
+
+    def restore():
+        for kk in SNAP_KEYS:
+            sim.a[kk].copy_from(snapd[kk])
+        sim.step = step0
+
+    restore()
+    sim.run(2 * chunk, use_graph=True, chunk=chunk)
+    restore()
+    sim.synchronize()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ctx.call("nnx_event_create", C.byref(e0))
+    ctx.call("nnx_event_create", C.byref(e1))
+    ctx.call("nnx_event_record", e0)
+    sim.run(steps, use_graph=True, chunk=chunk)
+    ctx.call("nnx_event_record", e1)
+    sim.synchronize()
+    msv = C.c_float()
+    ctx.call("nnx_event_elapsed", e0, e1, C.byref(msv))
+    ctx.call("nnx_event_destroy", e0)
+    ctx.call("nnx_event_destroy", e1)
+    v = sim.a["v"].numpy()
+    spk = int(sim.a["raster_len"].numpy()[0]) - int(snapd["raster_len"].numpy()[0])
+    return msv.value, (float(np.sum(v.astype(np.float64))), spk), v.view(np.uint32)[::4099].copy()
+
+
+def hebbian_bench(ctx, sim, net, chunk, snap, steps=2000, window=300, inc=0.1):
+    """Hebbian learning on every synapse of C2 during the timed window, with
+    the reference's HebbianLearningExample settings (window 300 steps,
+    heb_inc 0.1, src_wholebrain/main.cu:150): the window replayed from its
+    saved start without and with the rule."""
+    from neuronephys_clean_amd import _lib, hebbian
+
+    t_none, s0, p0 = replay_window(ctx, sim, snap, chunk, steps)
+    st = hebbian.attach(sim, window, inc, net.w)
+    t_h, s1, p1 = replay_window(ctx, sim, snap, chunk, steps)
+    changed = int((st.weights() != net.w).sum())
+    armed = int(st.armed().astype(np.int64).sum())
+    sim.plan.hebbian = _lib.Hebbian()
+    del sim.hebbian, st
+    return {"workload": f"Hebbian rule (window {window}, heb_inc {inc}) on all {net.nnz} synapses, "
+                        f"the {steps}-step timed C2 window replayed from its saved start",
+            "window_ms": {"no_hebbian": t_none, "hebbian": t_h},
+            "per_step_us": (t_h - t_none) * 1e3 / steps,
+            "weights_changed": changed, "armed_after": armed,
+            "dense_rule_bytes_per_step": 8 * net.nnz,
+            "simulation_unchanged": bool(s0 == s1 and np.array_equal(p0, p1))}
+
+
 def readout_bench(ctx, sim, net, chunk, snap, steps=2000, every=40, n_elec=1000, k=50):
     """Electrode readout on the C2 run: 1,000 electrodes, each the mean of
     its 50 nearest neurons, plus the network mean, read every 40 steps (the
@@ -409,34 +463,9 @@ def readout_bench(ctx, sim, net, chunk, snap, steps=2000, every=40, n_elec=1000,
 
     centers = (net.n // n_elec) * np.arange(n_elec)
     off, idx = host.spherical_regions(net.pos, k, center_idx=centers)
-    keys = SNAP_KEYS
-    snap, step0 = snap
-
-    def restore():
-        for kk in keys:
-            sim.a[kk].copy_from(snap[kk])
-        sim.step = step0
 
     def window():
-        restore()
-        sim.run(2 * chunk, use_graph=True, chunk=chunk)  # captures this plan's graphs
-        restore()
-        sim.synchronize()
-        e0, e1 = C.c_void_p(), C.c_void_p()
-        ctx.call("nnx_event_create", C.byref(e0))
-        ctx.call("nnx_event_create", C.byref(e1))
-        ctx.call("nnx_event_record", e0)
-        sim.run(steps, use_graph=True, chunk=chunk)
-        ctx.call("nnx_event_record", e1)
-        sim.synchronize()
-        msv = C.c_float()
-        ctx.call("nnx_event_elapsed", e0, e1, C.byref(msv))
-        ctx.call("nnx_event_destroy", e0)
-        ctx.call("nnx_event_destroy", e1)
-        ms = msv.value
-        v = sim.a["v"].numpy()
-        spk = int(sim.a["raster_len"].numpy()[0]) - int(snap["raster_len"].numpy()[0])
-        return ms, (float(np.sum(v.astype(np.float64))), spk), v.view(np.uint32)[::4099].copy()
+        return replay_window(ctx, sim, snap, chunk, steps)
 
     t_none, sum0, probe0 = window()
     sim.set_readout(off, idx, every, steps // every + 1)
@@ -628,10 +657,12 @@ def main():
                  sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
     couple_ms = timed_events(ctx, couple, 50)
 
-    ro = None
+    ro = heb = None
     if snap is not None:
         ro = readout_bench(ctx, sim, net, args.chunk, snap, steps=args.steps)
         log(f"bench: readout {ro}")
+        heb = hebbian_bench(ctx, sim, net, args.chunk, snap, steps=args.steps)
+        log(f"bench: hebbian {heb}")
 
     roof_stencil = stencil_roofline(ctx)
     log(f"bench: stencil {roof_stencil}")
@@ -679,7 +710,7 @@ def main():
                       "spikes_per_step": spikes / args.steps},
            "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
            "roofline_hh_unfused": roof_hh,
-           "c3": c3, "c4": c4, "c5": c5, "readout": ro,
+           "c3": c3, "c4": c4, "c5": c5, "readout": ro, "hebbian": heb,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},
            "event_ms": ev_ms.value}

@@ -15,7 +15,10 @@
 // (src/main.cu:279-285), are read at every print increment before its
 // coupling (src/main.cu:133-141): DIR/electrodes.f32 holds one row per print
 // increment, the render_N means then the network mean; DIR/regions.i32 the
-// region indices.
+// region indices.  If DIR/w.f32 (nnz weights) exists, Hebbian learning
+// (window 20, heb_inc 0.05) updates them after every increment's ephys, on
+// the firing set of that increment's coupling (src_wholebrain/main.cu:
+// 158-168); the result is DIR/w_out.f32.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -81,6 +84,15 @@ int main(int argc, char** argv) {
             throw std::runtime_error("input sizes do not match");
 
         AdjacencyList list(off, tgt);
+        std::unique_ptr<Hebbian> heb;
+        {
+            FILE* wf = std::fopen((dir + "/w.f32").c_str(), "rb");
+            if (wf) {
+                std::fclose(wf);
+                list.weights = DeviceVector<Float>(load<Float>(dir + "/w.f32"));
+                heb = std::make_unique<Hebbian>(20, 0.05f, (int)tgt.size());
+            }
+        }
         std::vector<DeviceVector<Float>> vars;
         std::vector<Float*> ptrs;
         for (int k = 0; k < NUM_SIM; ++k) {
@@ -133,6 +145,7 @@ int main(int argc, char** argv) {
             else
                 paracrine.update_on_fine_timescale(coupling);
             ephys.SimulateEphys(sim_vars.data());
+            if (heb) heb->UpdateWeights(list, coupling.is_firing, coupling.firing_neurons);
         }
         std::vector<Float> out;
         for (int k = 0; k < NUM_SIM; ++k) {
@@ -143,6 +156,7 @@ int main(int argc, char** argv) {
         // not part of sv_out)
         record(coupling, coupling(list, ptrs[SIM_LOCAL_IN], ptrs[SIM_LOCAL_OUTPUT]), nincs - 1, raster);
         if (electrodes) save(dir + "/electrodes.f32", erows);
+        if (heb) save(dir + "/w_out.f32", list.weights.host());
         save(dir + "/sv_out.f32", out);
         save(dir + "/raster.i32", raster);
         save(dir + "/nfiring.i32", nfiring);

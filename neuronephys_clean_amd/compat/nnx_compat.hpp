@@ -16,6 +16,7 @@
 //   ElectrodeNet, SynchronyNet                  src/electrode_net.cuh:6-38,
 //                                               src_wholebrain/electrode_net.cuh:6-92
 //   Morphology::GenSphericalRegions             src/morphology.cpp:48-97
+//   Hebbian                                     src_wholebrain/hebbian.cuh:3-27
 //
 // Deliberate differences (also listed in the design notes):
 //   - thrust::device_vector members become nnx_compat::DeviceVector (same
@@ -494,6 +495,54 @@ class SynchronyNet {
         std::vector<double> chi(ns);
         if (ns) check(nnx_memcpy_d2h(ctx(), chi.data(), d_chi.data(), ns * sizeof(double)));
         return chi;
+    }
+};
+
+// ------------------------------------------------------------ hebbian ----
+// Hebbian learning of connectivity.weights (CSR order), one call per
+// timestep.  The firing set is read from is_firing (the coupling's bitmap);
+// firing_inds names the same neurons and is not needed.
+class Hebbian {
+    int Nconns;
+    int64_t calls_ = 0;
+    const void* bound_ = nullptr;  // the list the state was built for
+    DeviceVector<uint8_t> armed_;
+    DeviceVector<int32_t> last_, in_to_out_;
+    DeviceVector<uint64_t> ring_;
+
+   public:
+    int const window_duration;
+    Float const heb_inc;
+
+    Hebbian(int window_duration_, Float heb_inc_, int Nconns_)
+        : Nconns(Nconns_), window_duration(window_duration_), heb_inc(heb_inc_) {}
+
+    void UpdateWeights(AdjacencyList& connectivity, const DeviceVector<uint64_t>& is_firing,
+                       const DeviceVector<int32_t>& firing_inds) {
+        (void)firing_inds;
+        const nnx_graph g = connectivity.graph();
+        if ((int64_t)connectivity.weights.size() != g.nnz || g.nnz != Nconns)
+            throw Error(NNX_ERR_ARG, "Hebbian: the list needs one weight per synapse (Nconns)");
+        nnx_hebbian h{};
+        h.window = window_duration;
+        h.heb_inc = heb_inc;
+        h.weights = connectivity.weights.data();
+        if (bound_ != &connectivity) {
+            armed_ = DeviceVector<uint8_t>(std::max<int64_t>(g.nnz, 1));
+            in_to_out_ = DeviceVector<int32_t>(std::max<int64_t>(g.nnz, 1));
+            last_ = DeviceVector<int32_t>(std::max<int64_t>(g.n, 1));
+            ring_ = DeviceVector<uint64_t>(std::max<int64_t>((int64_t)window_duration * ((g.n + 63) / 64), 1));
+        }
+        h.armed = armed_.data();
+        h.last_fire = last_.data();
+        h.ring = ring_.data();
+        h.in_to_out = in_to_out_.data();
+        if (bound_ != &connectivity) {
+            check(nnx_hebbian_init(ctx(), &g, &h));
+            bound_ = &connectivity;
+            calls_ = 0;
+        }
+        check(nnx_hebbian_update(ctx(), &g, &h, calls_++, is_firing.data()));
     }
 };
 

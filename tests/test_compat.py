@@ -59,6 +59,8 @@ def test_compat_loop_matches_oracle(tmp_path):
     net.tgt.astype(np.int32).tofile(tmp_path / "tgt.i32")
     np.concatenate(sv).tofile(tmp_path / "sv.f32")
     pos.tofile(tmp_path / "pos.f32")
+    w0 = rng.uniform(0, 2, net.nnz).astype(np.float32)
+    w0.tofile(tmp_path / "w.f32")
     print_incs, render_N, kreg = 25, 12, 30
     args = [DEMO, str(tmp_path), str(nincs), str(nx), str(ny), str(nz), repr(h), repr(D),
             repr(decay), repr(par_dt), str(ts), repr(coup), str(print_incs), str(render_N), str(kreg)]
@@ -102,6 +104,19 @@ def test_compat_loop_matches_oracle(tmp_path):
     counts = np.bincount(ref[:, 0], minlength=nincs)
     nfiring = np.fromfile(tmp_path / "nfiring.i32", np.int32)
     assert nfiring[0] == first and np.array_equal(nfiring[1:], counts[:-1])
+
+    # Hebbian learning after each increment, on the firing set its coupling read
+    timers = np.zeros(net.nnz, np.int32)
+    w = w0.copy()
+    for i in range(nincs):
+        flags = np.zeros(net.n, np.uint8)
+        if i == 0:
+            flags[np.asarray(sv0["LOCAL_OUTPUT"], np.float32) > 0.1] = 1
+        else:
+            flags[ref[ref[:, 0] == i - 1, 1]] = 1
+        oracle.hebbian_update(net.off, net.tgt, flags, 20, 0.05, timers, w)
+    got_w = np.fromfile(tmp_path / "w_out.f32", np.float32)
+    assert (w != w0).sum() > 0 and np.array_equal(got_w, w)
 
     # paracrine chain
     fired = [np.zeros(0, np.int64)] + [ref[ref[:, 0] == s, 1] for s in range(nincs - 1)]
