@@ -118,6 +118,48 @@ void nnx_face_coefficients(int32_t nx, int32_t ny, int32_t nz, const float* c, f
 int nnx_spherical_regions(int64_t n, const float* pos, int64_t n_regions, const int32_t* center_idx,
                           const float* center_pos, int32_t k, int32_t* offsets, int32_t* idx);
 
+/* ---- files ---------------------------------------------------------------
+ * HDF5 datasets without the HDF5 library (a reader of the published file
+ * format: superblocks 0-3, symbol-table and compact groups, integer and IEEE
+ * float datasets of either byte order, compact / contiguous / chunked
+ * storage with version-1 B-tree, single-chunk, implicit or fixed-array chunk
+ * indexes, deflate / shuffle / fletcher32).  `name` is a path such as
+ * "/row_ptr" or "grp/x".  Errors return -1 with a message in err.         */
+enum { NNX_H5_I32 = 0, NNX_H5_F32 = 1, NNX_H5_F64 = 2, NNX_H5_I64 = 3 };
+/* number of elements of the dataset */
+int64_t nnx_h5_count(const char* path, const char* name, char* err, size_t errlen);
+/* the whole dataset, row-major, converted to out_type as a C cast would;
+ * returns the element count (at most cap) */
+int64_t nnx_h5_read(const char* path, const char* name, int out_type, void* out, int64_t cap, char* err,
+                    size_t errlen);
+
+/* FileNetwork::GenerateFromHDF's sampling (src/network_factory.cu:171-212):
+ * with sparsity = numpnts / N, neuron i is row i*sparsity of the file's CSR
+ * (row_ptr: numpnts+1 entries) and keeps the targets divisible by sparsity,
+ * divided by it.  Targets that land at N or above (numpnts not a multiple of
+ * N; the reference keeps them, out of range) are dropped and counted in
+ * *dropped.  offsets: N+1; targets: room for row_ptr[numpnts] entries.
+ * Returns the number of edges, or -1 if N < 1 or N > numpnts (the
+ * reference prints an error and divides by zero).                         */
+int64_t nnx_csr_subsample(int64_t numpnts, const int32_t* row_ptr, const int32_t* col_idx, int64_t N,
+                          int32_t* offsets, int32_t* targets, int64_t* dropped);
+
+/* FileMorphology::GenerateFromHDF's placement (src_wholebrain/
+ * morphology.cpp:301-370): neuron i at point (numpnts/N)*i of the axes,
+ * mapped per axis to [-1, 1] by the bounding box of all numpnts points
+ * (p - min, times 2/(max - min), minus 1, in float).  pos: 3N interleaved.
+ * Returns 0, or -1 if N < 1 or N > numpnts.                                */
+int nnx_morphology_from_axes(int64_t numpnts, const float* x, const float* y, const float* z, int64_t N,
+                             float* pos);
+
+/* Csr_matrix::input_from_connection_file (src/paracrine.cuh:49-108): the
+ * text format "rows nnz" / rowPtr (rows+1) / colIdx (nnz) / val (nnz), one
+ * line each.  With row_ptr NULL only *rows and *nnz are set.  Returns 0, or
+ * -1 (message in err) for a missing file or a short line (the reference
+ * prints a message and leaves the entries zero).                          */
+int64_t nnx_read_connection_file(const char* path, int64_t* rows, int64_t* nnz, int32_t* row_ptr, int32_t* col,
+                                 float* val, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -209,6 +209,12 @@ _HOST_SIGS = {
                     None),
     "nnx_face_coefficients": ([C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp], None),
     "nnx_spherical_regions": ([C.c_int64, vp, C.c_int64, vp, vp, C.c_int32, vp, vp], C.c_int),
+    "nnx_h5_count": ([C.c_char_p, C.c_char_p, vp, C.c_size_t], C.c_int64),
+    "nnx_h5_read": ([C.c_char_p, C.c_char_p, C.c_int, vp, C.c_int64, vp, C.c_size_t], C.c_int64),
+    "nnx_csr_subsample": ([C.c_int64, vp, vp, C.c_int64, vp, vp, _P(C.c_int64)], C.c_int64),
+    "nnx_morphology_from_axes": ([C.c_int64, vp, vp, vp, C.c_int64, vp], C.c_int),
+    "nnx_read_connection_file": ([C.c_char_p, _P(C.c_int64), _P(C.c_int64), vp, vp, vp, vp, C.c_size_t],
+                                 C.c_int64),
 }
 
 _lib = None

@@ -17,6 +17,9 @@
 //                                               src_wholebrain/electrode_net.cuh:6-92
 //   Morphology::GenSphericalRegions             src/morphology.cpp:48-97
 //   Hebbian                                     src_wholebrain/hebbian.cuh:3-27
+//   Network, FileNetwork                        src/network_factory.cuh:5-40,
+//                                               src/network_factory.cu:5-27,155-212
+//   FileMorphology                              src_wholebrain/morphology.cpp:255-370
 //
 // Deliberate differences (also listed in the design notes):
 //   - thrust::device_vector members become nnx_compat::DeviceVector (same
@@ -32,6 +35,10 @@
 //     an int per neuron; ElectrodeNet's wholebrain overload takes it as is.
 //   - host_vector arguments are std::vector; electrode positions are
 //     std::array<float, 3> (no glm); Morphology holds positions only.
+//   - FileNetwork / FileMorphology open the path as given (the reference
+//     prepends "../"); FileNetwork keeps exactly N rows and only the kept
+//     edges (the reference keeps numpnts+1 offsets and zero-filled index
+//     slots, which ToAdjacencyList then counts as in-edges of neuron 0).
 #pragma once
 
 #include <algorithm>
@@ -546,10 +553,74 @@ class Hebbian {
     }
 };
 
+// ------------------------------------------------------------- files ----
+// Host CSR of out-edges, as the reference's Network base class.
+class Network {
+   protected:
+    std::vector<int> indices, offset_in_indices;
+
+   public:
+    Network() = default;
+    virtual ~Network() = default;
+    virtual bool Generate(int const N) = 0;
+    // the list on the device, and the in-degree of each neuron written into
+    // the caller's device array Tcount (N floats), as src/network_factory.cu:5-27
+    AdjacencyList ToAdjacencyList(Float*& Tcount) const {
+        AdjacencyList list(offset_in_indices, indices);
+        const int64_t N = (int64_t)offset_in_indices.size() - 1;
+        if (N > 0) check(nnx_memcpy_d2d(ctx(), Tcount, list.in_degree(), N * sizeof(Float)));
+        return list;
+    }
+    const std::vector<int>& GetCpuIndices() const { return indices; }
+    const std::vector<int>& GetCpuOffsetsIntoIndices() const { return offset_in_indices; }
+};
+
+// Connectivity from an HDF5 file holding /row_ptr and /col_idx, sampled to
+// N neurons as FileNetwork::GenerateFromHDF does.
+class FileNetwork : public Network {
+    std::string fl_name, format;
+
+   public:
+    int64_t dropped = 0;  // sampled targets >= N (numpnts not a multiple of N)
+    explicit FileNetwork(const std::string& fl_name_)
+        : fl_name(fl_name_), format(fl_name_.substr(fl_name_.find_last_of('.') + 1)) {}
+    bool Generate(int const N) override {
+        if (format != "h5" && format != "H5") {
+            std::fprintf(stderr, "Unsupported format %s\n", format.c_str());
+            return false;
+        }
+        char err[512];
+        const int64_t nr = nnx_h5_count(fl_name.c_str(), "/row_ptr", err, sizeof err);
+        const int64_t nc = nnx_h5_count(fl_name.c_str(), "/col_idx", err, sizeof err);
+        if (nr < 2 || nc < 0) {
+            std::fprintf(stderr, "ERROR : Failed to read in coupling (%s)\n", err);
+            return false;
+        }
+        std::vector<int32_t> rp(nr), ci(std::max<int64_t>(nc, 1));
+        if (nnx_h5_read(fl_name.c_str(), "/row_ptr", NNX_H5_I32, rp.data(), nr, err, sizeof err) < 0 ||
+            nnx_h5_read(fl_name.c_str(), "/col_idx", NNX_H5_I32, ci.data(), nc, err, sizeof err) < 0) {
+            std::fprintf(stderr, "ERROR : Failed to read in coupling (%s)\n", err);
+            return false;
+        }
+        offset_in_indices.assign(N + 1, 0);
+        indices.assign(std::max<int64_t>(nc, 1), 0);
+        const int64_t k = nnx_csr_subsample(nr - 1, rp.data(), ci.data(), N, offset_in_indices.data(),
+                                            indices.data(), &dropped);
+        if (k < 0) {
+            std::fprintf(stderr, "Not enough points in file %s: N = %d, %lld available\n", fl_name.c_str(), N,
+                         (long long)(nr - 1));
+            return false;
+        }
+        indices.resize(k);
+        return true;
+    }
+};
+
 // Neuron positions and the spherical electrode regions built from them.
 struct Morphology {
     std::vector<float> xyz;  // 3 per neuron, interleaved
 
+    Morphology() = default;
     explicit Morphology(std::vector<float> xyz_) : xyz(std::move(xyz_)) {}
     int64_t size() const { return (int64_t)xyz.size() / 3; }
 
@@ -575,6 +646,46 @@ struct Morphology {
         if (nnx_spherical_regions(size(), xyz.data(), nr, nullptr, c.data(), k, h_region_inds.data(),
                                   h_regions.data()) != 0)
             throw Error(NNX_ERR_ARG, "GenSphericalRegions: k out of range");
+    }
+};
+
+// Positions (and region ids) from an HDF5 file holding /x, /y, /z (and
+// /region_id), sampled to N neurons and mapped to [-1, 1]^3 as
+// FileMorphology::GenerateFromHDF does.
+struct FileMorphology : Morphology {
+    std::string fl_name, format;
+    bool read_regions;
+    std::vector<int> regions, unique_regions;
+
+    explicit FileMorphology(const std::string& fl_name_, bool read_regions_ = false)
+        : fl_name(fl_name_), format(fl_name_.substr(fl_name_.find_last_of('.') + 1)), read_regions(read_regions_) {}
+    bool Generate(int const N) {
+        if (format != "h5" && format != "H5") return false;
+        const char* ids[4] = {"/x", "/y", "/z", "/region_id"};
+        std::vector<float> ax[4];
+        char err[512];
+        for (int d = 0; d < (read_regions ? 4 : 3); ++d) {
+            const int64_t n = nnx_h5_count(fl_name.c_str(), ids[d], err, sizeof err);
+            if (n < 1) return false;
+            ax[d].resize(n);
+            if (nnx_h5_read(fl_name.c_str(), ids[d], NNX_H5_F32, ax[d].data(), n, err, sizeof err) < 0) return false;
+        }
+        const int64_t np = (int64_t)ax[0].size();
+        if (ax[1].size() != (size_t)np || ax[2].size() != (size_t)np) return false;
+        xyz.assign(3 * (size_t)N, 0.0f);
+        if (nnx_morphology_from_axes(np, ax[0].data(), ax[1].data(), ax[2].data(), N, xyz.data()) != 0) {
+            std::fprintf(stderr, "Not enough points in file %s\n", fl_name.c_str());
+            return false;
+        }
+        if (read_regions) {
+            const int64_t stride = np / N;
+            regions.resize(N);
+            for (int i = 0; i < N; ++i) regions[i] = (int)ax[3][stride * i];
+            unique_regions = regions;
+            std::sort(unique_regions.begin(), unique_regions.end());
+            unique_regions.erase(std::unique(unique_regions.begin(), unique_regions.end()), unique_regions.end());
+        }
+        return true;
     }
 };
 

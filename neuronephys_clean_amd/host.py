@@ -166,3 +166,76 @@ def spherical_regions(pos, k, center_idx=None, center_pos=None):
         raise ValueError(f"nnx_spherical_regions: bad arguments (k={k}, n={pos.shape[0]})")
     return off, idx
 
+
+H5_TYPES = {np.dtype(np.int32): 0, np.dtype(np.float32): 1, np.dtype(np.float64): 2, np.dtype(np.int64): 3}
+
+
+def h5_read(path, name, dtype=np.float32) -> np.ndarray:
+    """A whole HDF5 dataset, flattened row-major, converted to dtype (int32,
+    int64, float32 or float64)."""
+    err = C.create_string_buffer(512)
+    n = host().nnx_h5_count(str(path).encode(), name.encode(), err, 512)
+    if n < 0:
+        raise IOError(err.value.decode())
+    out = np.empty(max(n, 1), dtype)
+    k = host().nnx_h5_read(str(path).encode(), name.encode(), H5_TYPES[np.dtype(dtype)], _p(out), n, err, 512)
+    if k < 0:
+        raise IOError(err.value.decode())
+    return out[:k]
+
+
+def csr_subsample(row_ptr, col_idx, N):
+    """FileNetwork::GenerateFromHDF's sampling of N neurons: (offsets, targets, dropped)."""
+    row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+    col_idx = np.ascontiguousarray(col_idx, np.int32)
+    off = np.empty(N + 1, np.int32)
+    tgt = np.empty(max(len(col_idx), 1), np.int32)
+    dropped = C.c_int64()
+    k = host().nnx_csr_subsample(len(row_ptr) - 1, _p(row_ptr), _p(col_idx), N, _p(off), _p(tgt),
+                                 C.byref(dropped))
+    if k < 0:
+        raise ValueError(f"cannot sample {N} neurons from {len(row_ptr) - 1}")
+    return off, tgt[:k].copy(), dropped.value
+
+
+def morphology_from_axes(x, y, z, N):
+    """FileMorphology::GenerateFromHDF's placement: (N, 3) positions in [-1, 1]."""
+    ax = [np.ascontiguousarray(a, np.float32) for a in (x, y, z)]
+    pos = np.empty((N, 3), np.float32)
+    if host().nnx_morphology_from_axes(len(ax[0]), *(_p(a) for a in ax), N, _p(pos)) != 0:
+        raise ValueError(f"cannot place {N} neurons from {len(ax[0])} points")
+    return pos
+
+
+def read_connection_file(path):
+    """Csr_matrix::input_from_connection_file: (row_ptr, col, val)."""
+    err = C.create_string_buffer(512)
+    rows, nnz = C.c_int64(), C.c_int64()
+    if host().nnx_read_connection_file(str(path).encode(), C.byref(rows), C.byref(nnz), None, None, None, err,
+                                       512) < 0:
+        raise IOError(err.value.decode())
+    rp = np.empty(rows.value + 1, np.int32)
+    col = np.empty(max(nnz.value, 1), np.int32)
+    val = np.empty(max(nnz.value, 1), np.float32)
+    if host().nnx_read_connection_file(str(path).encode(), C.byref(rows), C.byref(nnz), _p(rp), _p(col), _p(val),
+                                       err, 512) < 0:
+        raise IOError(err.value.decode())
+    return rp, col[:nnz.value], val[:nnz.value]
+
+
+def load_hdf5_network(path, N):
+    """The reference's FileNetwork on an HDF5 file with /row_ptr and /col_idx."""
+    return csr_subsample(h5_read(path, "/row_ptr", np.int32), h5_read(path, "/col_idx", np.int32), N)
+
+
+def load_hdf5_morphology(path, N, regions=False):
+    """The reference's FileMorphology: positions (N, 3) and, with regions,
+    the region id of each neuron (float ids truncated to int, as there)."""
+    x, y, z = (h5_read(path, a, np.float32) for a in ("/x", "/y", "/z"))
+    pos = morphology_from_axes(x, y, z, N)
+    if not regions:
+        return pos
+    rid = h5_read(path, "/region_id", np.float32)
+    stride = len(x) // N
+    return pos, rid[stride * np.arange(N)].astype(np.int32)
+
