@@ -185,6 +185,42 @@ nnx_status nnx_hh_step(nnx_ctx* ctx, const nnx_hh_state* st, const nnx_hh_params
                        int integrator, int spike_rule, double dt, uint64_t step,
                        const void* g_in, uint64_t* spike_bits);
 
+/* Diekman suprachiasmatic-nucleus neuron (DiekmanEphys, src/ephys.cuh,
+ * Leapfrog_diekman, src/ephys.cu:250-347): Hodgkin-Huxley-type Na, K,
+ * L-type and non-L-type Ca, Ca-activated K, persistent Na, two Ca pools and
+ * a kinetic GABA synapse, all gates by the trapezoid rule, the submembrane
+ * Ca by the root of the trapezoid rule's quadratic, V semi-implicitly.
+ * Arrays are device arrays of n elements of dtype, in the reference's
+ * DIEKMAN_VAR order (src/defines.cuh:153-173).                             */
+typedef struct nnx_diekman_state {
+    int32_t dtype;
+    int32_t pad_;
+    int64_t n;
+    void *v, *m, *h, *ng, *rl, *rnl, *fnl, *s, *p, *y, *cas, *cac;  /* updated */
+    void* iin;                /* DIEKMAN_VAR_IIN (coupling input, written by the coupling) */
+    const void* icount;       /* DIEKMAN_VAR_ICOUNT (in-degree) */
+    void* ioutput;            /* DIEKMAN_VAR_IOUTPUT, written after the update */
+    const void *egaba, *gkca, *gkleak;  /* per-neuron parameters */
+} nnx_diekman_state;
+
+/* The constants of diekman_params (src/defines.cuh), and the ones
+ * Leapfrog_diekman fixes in its body (gnaP 2.85, tau_Rl = tau_Rnl = 3.1,
+ * tau_P 100 ms).  The reference stores them as float.                      */
+typedef struct nnx_diekman_params {
+    double gna, gnaleak, gk, gcal, gcanl, ena, ek, eca, k1, k2, kcas, kcac, taucas, taucac, bcas, bcac;
+    double ar, ad, tmax, vt, kp, g_gaba, appcur, gnap, tau_rl, tau_rnl, tau_p;
+    double out_thresh;        /* level rule on IOUTPUT (0.1, src/coupling.cu:103-109) */
+} nnx_diekman_params;
+void nnx_diekman_params_reference(nnx_diekman_params* p);
+
+/* One step of every neuron (DiekmanEphys::SimulateEphys).  spike_bits
+ * (optional): bit i = IOUTPUT of neuron i after the step > out_thresh.
+ * fp32 follows the reference's mix of float and double operations; the
+ * exponentials are this library's (see the HH step), powf(x, 0.25) is
+ * sqrt(sqrt(x)), powf(x, 1.5) is x sqrt(x) and powf(x, 4) is (x^2)^2.    */
+nnx_status nnx_diekman_step(nnx_ctx* ctx, const nnx_diekman_state* st, const nnx_diekman_params* p, double dt,
+                            uint64_t* spike_bits);
+
 /* Returns NNX_ERR_NONFINITE if any V is NaN/Inf (synchronous).             */
 nnx_status nnx_check_finite(nnx_ctx* ctx, int dtype, int64_t n, const void* d_x);
 

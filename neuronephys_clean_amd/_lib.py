@@ -81,6 +81,25 @@ class CgResult(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+DIEKMAN_PARAM_FIELDS = ("gna", "gnaleak", "gk", "gcal", "gcanl", "ena", "ek", "eca", "k1", "k2", "kcas", "kcac",
+                        "taucas", "taucac", "bcas", "bcac", "ar", "ad", "tmax", "vt", "kp", "g_gaba", "appcur",
+                        "gnap", "tau_rl", "tau_rnl", "tau_p", "out_thresh")
+DIEKMAN_STATE = ("v", "m", "h", "ng", "rl", "rnl", "fnl", "s", "p", "y", "cas", "cac")
+
+
+class DiekmanParams(C.Structure):
+    _fields_ = [(f, C.c_double) for f in DIEKMAN_PARAM_FIELDS]
+
+    def consts(self):
+        """The 27 model constants (everything but out_thresh), in order."""
+        return [getattr(self, f) for f in DIEKMAN_PARAM_FIELDS[:27]]
+
+
+class DiekmanState(C.Structure):
+    _fields_ = ([("dtype", C.c_int32), ("pad_", C.c_int32), ("n", C.c_int64)] + [(f, vp) for f in DIEKMAN_STATE] +
+                [("iin", vp), ("icount", vp), ("ioutput", vp), ("egaba", vp), ("gkca", vp), ("gkleak", vp)])
+
+
 class Regions(C.Structure):
     _fields_ = [("n_regions", C.c_int64), ("nnz", C.c_int64), ("offsets", vp), ("idx", vp)]
 
@@ -164,6 +183,8 @@ _SIGS = {
                                   vp], C.c_int),
     "nnx_synchrony_measures": ([vp, C.c_int64, C.c_int32, vp, vp, vp, vp, vp, C.c_int64, vp], C.c_int),
     "nnx_hebbian_init": ([vp, _P(Graph), _P(Hebbian)], C.c_int),
+    "nnx_diekman_params_reference": ([_P(DiekmanParams)], None),
+    "nnx_diekman_step": ([vp, _P(DiekmanState), _P(DiekmanParams), C.c_double, vp], C.c_int),
     "nnx_hebbian_update": ([vp, _P(Graph), _P(Hebbian), C.c_int64, vp], C.c_int),
     "nnx_unet_param_count": ([], C.c_int64),
     "nnx_unet_create": ([vp, vp, C.c_uint64, _P(vp)], C.c_int),
@@ -304,6 +325,12 @@ def context(device: int = 0) -> Context:
 def params_reference() -> HHParams:
     p = HHParams()
     lib().nnx_hh_params_reference(C.byref(p))
+    return p
+
+
+def params_diekman() -> DiekmanParams:
+    p = DiekmanParams()
+    lib().nnx_diekman_params_reference(C.byref(p))
     return p
 
 

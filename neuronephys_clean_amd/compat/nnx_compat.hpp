@@ -6,6 +6,7 @@
 //
 // Replaced interfaces (neuronephys_clean sources):
 //   Ephys / HHEphys::SimulateEphys(Float**)     src/ephys.cuh:6-31, src/ephys.cu:98-103
+//   DiekmanEphys::SimulateEphys(Float**)        src/ephys.cu:250-353
 //   AtomicCoupling(int), operator(), GetFiringNeurons()   src/coupling.cuh:17-38
 //   AtomicCoupling(int, AdjacencyList&, Float*, Float*), PerformCoupling()
 //                                               src_wholebrain/coupling.cuh:26-43
@@ -238,6 +239,45 @@ class HHEphys : public Ephys {
         check(nnx_hh_step(ctx(), &st, &p_, NNX_INT_LEAPFROG, NNX_SPIKE_LEVEL, leapfrog_dt, 0, nullptr,
                           nullptr));
         check(nnx_synchronize(ctx()));  // the reference synchronises after the launch
+    }
+};
+
+// SoA indices of the Diekman model (src/defines.cuh:153-173)
+enum DiekmanVars {
+    DIEKMAN_VAR_V = 0, DIEKMAN_VAR_M, DIEKMAN_VAR_H, DIEKMAN_VAR_N, DIEKMAN_VAR_RL, DIEKMAN_VAR_RNL, DIEKMAN_VAR_FNL,
+    DIEKMAN_VAR_S, DIEKMAN_VAR_P, DIEKMAN_VAR_Y, DIEKMAN_VAR_CAS, DIEKMAN_VAR_CAC, DIEKMAN_VAR_G_BIOCHEM,
+    DIEKMAN_VAR_IIN, DIEKMAN_VAR_ICOUNT, DIEKMAN_VAR_IOUTPUT, DIEKMAN_VAR_EGABA, DIEKMAN_VAR_PARACRINE,
+    DIEKMAN_VAR_GKCA, DIEKMAN_VAR_GKLEAK, NUM_DIEKMAN_VAR
+};
+
+// Leapfrog_diekman at hh_params::leapfrog_dt (0.4 ms), as
+// DiekmanEphys::SimulateEphys launches it; sim_vars is a device array of
+// NUM_DIEKMAN_VAR device pointers.
+class DiekmanEphys : public Ephys {
+    std::map<Float**, std::vector<Float*>> tables_;
+    nnx_diekman_params p_{};
+
+   public:
+    explicit DiekmanEphys(int const N_) : Ephys(N_) { nnx_diekman_params_reference(&p_); }
+    nnx_diekman_params& params() { return p_; }
+    void SimulateEphys(Float** sim_vars) override {
+        auto it = tables_.find(sim_vars);
+        if (it == tables_.end()) {
+            std::vector<Float*> h(NUM_DIEKMAN_VAR);
+            check(nnx_memcpy_d2h(ctx(), h.data(), sim_vars, NUM_DIEKMAN_VAR * sizeof(Float*)));
+            it = tables_.emplace(sim_vars, h).first;
+        }
+        const std::vector<Float*>& a = it->second;
+        nnx_diekman_state st{};
+        st.dtype = NNX_F32;
+        st.n = N;
+        st.v = a[DIEKMAN_VAR_V], st.m = a[DIEKMAN_VAR_M], st.h = a[DIEKMAN_VAR_H], st.ng = a[DIEKMAN_VAR_N];
+        st.rl = a[DIEKMAN_VAR_RL], st.rnl = a[DIEKMAN_VAR_RNL], st.fnl = a[DIEKMAN_VAR_FNL], st.s = a[DIEKMAN_VAR_S];
+        st.p = a[DIEKMAN_VAR_P], st.y = a[DIEKMAN_VAR_Y], st.cas = a[DIEKMAN_VAR_CAS], st.cac = a[DIEKMAN_VAR_CAC];
+        st.iin = a[DIEKMAN_VAR_IIN], st.icount = a[DIEKMAN_VAR_ICOUNT], st.ioutput = a[DIEKMAN_VAR_IOUTPUT];
+        st.egaba = a[DIEKMAN_VAR_EGABA], st.gkca = a[DIEKMAN_VAR_GKCA], st.gkleak = a[DIEKMAN_VAR_GKLEAK];
+        check(nnx_diekman_step(ctx(), &st, &p_, HHEphys::leapfrog_dt, nullptr));
+        check(nnx_synchronize(ctx()));
     }
 };
 

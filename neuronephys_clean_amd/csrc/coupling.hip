@@ -21,10 +21,10 @@ namespace nnx {
 // level rule (CheckFiringOperator, src/coupling.cu:103-109)
 // ------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_threshold(int64_t n, const T* __restrict__ out, T thr,
+__global__ __launch_bounds__(256) void k_threshold(int64_t n, const T* __restrict__ out, double thr,
                                                    uint64_t* __restrict__ bits, int64_t nwords) {
     const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-    const bool f = i < n && out[i] > thr;
+    const bool f = i < n && (double)out[i] > thr;  // Float > 0.1 compares in double
     const uint64_t w = __ballot(f);
     if ((threadIdx.x & 63) == 0 && (i >> 6) < nwords) bits[i >> 6] = w;
 }
@@ -337,7 +337,7 @@ nnx_status nnx_spike_threshold(nnx_ctx* c, int dtype, int64_t n, const void* out
     const int64_t nwords = ceil_div(n, 64);
     const unsigned blocks = (unsigned)ceil_div(n, 256);
     if (dtype == NNX_F32)
-        k_threshold<float><<<blocks, 256, 0, c->stream>>>(n, (const float*)out, (float)thr, bits, nwords);
+        k_threshold<float><<<blocks, 256, 0, c->stream>>>(n, (const float*)out, thr, bits, nwords);
     else if (dtype == NNX_F64)
         k_threshold<double><<<blocks, 256, 0, c->stream>>>(n, (const double*)out, thr, bits, nwords);
     else

@@ -58,7 +58,7 @@ template <class T> HHConst<T> make_const(const nnx_hh_params* q, double dt) {
     p.e_e = T(q->e_e);
     p.noise_mean = T(q->noise_mean);
     p.v_thresh = T(q->v_thresh);
-    p.out_thresh = T(q->out_thresh);
+    p.out_thresh = q->out_thresh;
     p.dt = T(dt);
     p.hdt = T(dt / 2.0);
     p.dt6 = T(dt / 6.0);

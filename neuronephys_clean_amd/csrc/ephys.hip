@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_hh_leapfrog(int64_t n, HHConst<T> p,
         if (a.out_l) a.out_l[i] = out;
         if (a.out_i) a.out_i[i] = out;
         if (a.out_e) a.out_e[i] = out;
-        spike = level ? (out > p.out_thresh) : (V < p.v_thresh && Vn >= p.v_thresh);
+        spike = level ? ((double)out > p.out_thresh) : (V < p.v_thresh && Vn >= p.v_thresh);
     }
     const uint64_t word = __ballot(spike);
     if (bits && (threadIdx.x & 63) == 0) {

@@ -121,7 +121,10 @@ template <class T> __device__ __forceinline__ Rates<T> rates(T u) {
 template <class T> struct HHConst {
     T C, invC, gna, gk, gl, ena, ek, el, vshift, esyn;
     T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean;
-    T v_thresh, out_thresh;
+    T v_thresh;
+    // the level rule compares the output, widened, with this double (the
+    // reference's CheckFiringOperator tests a Float against the literal 0.1)
+    double out_thresh;
     // step constants: dt, dt/2, dt/6, exp(-dt/(2 tau)), exp(-dt/tau)
     T dt, hdt, dt6, eh, ef;
 };

@@ -229,3 +229,26 @@ def hebbian_update(off, tgt, fire, window, heb_inc, timers, w):
     assert timers.dtype == np.int32 and w.dtype == np.float32
     lib().orc_hebbian_update(C.c_int64(len(off) - 1), _p(off), _p(tgt), _p(f), C.c_int32(window),
                              C.c_float(heb_inc), _p(timers), _p(w))
+
+
+DIEKMAN_STATE = ("v", "m", "h", "ng", "rl", "rnl", "fnl", "s", "p", "y", "cas", "cac")
+DIEKMAN_IN = ("iin", "icount", "egaba", "gkca", "gkleak")
+
+
+def diekman_step(consts, dt, x, inputs, thr=0.1):
+    """One Leapfrog_diekman step; x: dict of the 12 state arrays (updated in
+    place), inputs: dict of the 5 input arrays; consts: the 27 constants in
+    nnx_diekman_params order.  Returns (IOUTPUT, spike flags)."""
+    a = [x[k] for k in DIEKMAN_STATE]
+    dt_ = a[0].dtype
+    assert all(v.dtype == dt_ and v.flags.c_contiguous for v in a)
+    ins = [np.ascontiguousarray(inputs[k], dt_) for k in DIEKMAN_IN]
+    n = len(a[0])
+    out = np.empty(n, dt_)
+    sp = np.empty(n, np.uint8)
+    c = np.ascontiguousarray(consts, np.float64)
+    xs = (C.c_void_p * 12)(*[v.ctypes.data for v in a])
+    ii = (C.c_void_p * 5)(*[v.ctypes.data for v in ins])
+    getattr(lib(), "orc_diekman_" + _sfx(dt_))(C.c_int64(n), _p(c), C.c_double(dt), C.c_double(thr), xs, ii,
+                                               _p(out), _p(sp))
+    return out, sp

@@ -54,7 +54,7 @@ template <class T> static P<T> cast(const orc_params* q) {
     p.e_e = T(q->e_e);
     p.noise_mean = T(q->noise_mean);
     p.v_thresh = T(q->v_thresh);
-    p.out_thresh = T(q->out_thresh);
+    p.out_thresh = q->out_thresh;
     return p;
 }
 
@@ -119,7 +119,7 @@ static void leapfrog_arrays(int64_t n, const orc_params* q, double dt, T* const*
         sv[7][i] = s.out;
         sv[10][i] = s.out;
         sv[13][i] = s.out;
-        if (spiked) spiked[i] = s.out > p.out_thresh ? 1 : 0;
+        if (spiked) spiked[i] = (double)s.out > p.out_thresh ? 1 : 0;
     }
 }
 
@@ -271,7 +271,7 @@ static int64_t run_leapfrog(int64_t n, const orc_params* q, double dt, int64_t s
     std::vector<int32_t> list;
     int64_t len = 0;
     // the first coupling reads the firing of the initial outputs
-    for (int64_t i = 0; i < n; ++i) fire[i] = sv[7][i] > T(q->out_thresh) ? 1 : 0;
+    for (int64_t i = 0; i < n; ++i) fire[i] = (double)sv[7][i] > q->out_thresh ? 1 : 0;
     compact(n, fire.data(), list);
     for (int64_t k = 0; k < steps; ++k) {
         couple_push<T>(n, off, tgt, nullptr, list, mode, sv[7], sv[8]);
@@ -327,6 +327,38 @@ void orc_sync_measures(int64_t n_idx, int n_sub, const int32_t* counts, const do
 void orc_hebbian_update(int64_t n, const int32_t* off, const int32_t* tgt, const uint8_t* fire, int32_t window,
                         float heb_inc, int32_t* timers, float* w) {
     hebbian_update(n, off, tgt, fire, window, heb_inc, timers, w);
+}
+
+}  // extern "C"
+
+// Diekman step over arrays: x = 12 state arrays (DIEKMAN_VAR order V..Cac),
+// in = 5 input arrays (Tin, num_in, egaba, gkca, gkleak), out = IOUTPUT;
+// c = 27 constants in nnx_diekman_params order; spiked (optional):
+// IOUTPUT > thr with the output widened to double.
+template <class T>
+static void diekman_arrays(int64_t n, const double* c, double dt, double thr, T* const* x, const T* const* in, T* out,
+                           uint8_t* spiked) {
+    DkC<T> k;
+    T* f = &k.gna;
+    for (int i = 0; i < 27; ++i) f[i] = (T)c[i];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        T s[12], v[5];
+        for (int q = 0; q < 12; ++q) s[q] = x[q][i];
+        for (int q = 0; q < 5; ++q) v[q] = in[q][i];
+        out[i] = diekman_step<T>(k, (T)dt, s, v);
+        for (int q = 0; q < 12; ++q) x[q][i] = s[q];
+        if (spiked) spiked[i] = (double)out[i] > thr ? 1 : 0;
+    }
+}
+extern "C" {
+void orc_diekman_f32(int64_t n, const double* c, double dt, double thr, float* const* x, const float* const* in,
+                     float* out, uint8_t* spiked) {
+    diekman_arrays<float>(n, c, dt, thr, x, in, out, spiked);
+}
+void orc_diekman_f64(int64_t n, const double* c, double dt, double thr, double* const* x, const double* const* in,
+                     double* out, uint8_t* spiked) {
+    diekman_arrays<double>(n, c, dt, thr, x, in, out, spiked);
 }
 
 }  // extern "C"

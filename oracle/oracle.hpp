@@ -20,6 +20,7 @@
 //   src/electrode_net.cu:3-47, src_wholebrain/electrode_net.cu:3-126
 //                               electrode readout and synchrony
 //   src_wholebrain/hebbian.cu:3-69   Hebbian learning
+//   src/ephys.cu:250-347        Leapfrog_diekman (SCN neuron)
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -134,7 +135,8 @@ template <class T> inline Rates<T> rates(T u) {
 template <class T> struct P {
     T C, invC, gna, gk, gl, ena, ek, el, vshift, esyn;
     T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean;
-    T v_thresh, out_thresh;
+    T v_thresh;
+    double out_thresh;  // the level rule widens the output (CheckFiringOperator: Float > 0.1)
 };
 
 // dy/dt of the RK4 model (V, m, h, n) with synaptic conductance s.
@@ -608,6 +610,82 @@ inline void hebbian_update(int64_t n, const int32_t* off, const int32_t* tgt, co
             timers[e] = 0;
         }
     }
+}
+
+// --------------------------------------------------------------------------
+// Diekman SCN neuron, one step (Leapfrog_diekman, src/ephys.cu:250-347).  In
+// fp32 the reference's operations keep their C++ types: its double literals
+// widen parts of the arithmetic to double (written here as W), expf takes a
+// float argument, exp a double one; powf(x, 0.25), powf(x, 1.5) and
+// powf(x, 4) are sqrt(sqrt(x)), x sqrt(x) and (x^2)^2.  fp64 is double
+// throughout.  c: the constants (fp32: rounded to float as diekman_params
+// stores them).  x: V, M, H, N, Rl, Rnl, Fnl, S, P, Y, Cas, Cac (updated);
+// in: Tin, num_in, egaba, gkca, gkleak; returns IOUTPUT.
+// --------------------------------------------------------------------------
+template <class T> struct DkC {
+    T gna, gnaleak, gk, gcal, gcanl, ena, ek, eca, k1, k2, kcas, kcac, taucas, taucac, bcas, bcac;
+    T ar, ad, tmax, vt, kp, g_gaba, appcur, gnap, tau_rl, tau_rnl, tau_p;
+};
+inline float dk_exf(float, double a) { return oexp((float)a); }
+inline double dk_exf(double, double a) { return oexp(a); }
+
+template <class T> T diekman_step(const DkC<T>& c, T dt, T* x, const T* in) {
+    using W = double;
+    const T V = x[0], M = x[1], H = x[2], N = x[3], Rl = x[4], Rnl = x[5], Fnl = x[6], S = x[7], P = x[8], Y = x[9];
+    const T Cas = x[10], Cac = x[11];
+    const T Tin = in[0], num_in = in[1], egaba = in[2], gkca = in[3], gkleak = in[4];
+    const T z = 0;
+    const T M_inf = (T)(1.0 / (1.0 + (W)dk_exf(z, -((W)V + 35.2) / 8.1)));
+    const T H_inf = (T)(1.0 / (1.0 + (W)dk_exf(z, ((W)V + 62.0) / 2.0)));
+    const T nb = (T)(1.0 + (W)dk_exf(z, ((W)V - 14.0) / (-17.0)));
+    const T N_inf = (T)(1.0 / (W)std::sqrt(std::sqrt(nb)));
+    const T Rl_inf = (T)(1.0 / (1.0 + (W)dk_exf(z, -((W)V + 36.0) / 5.1)));
+    const T Rnl_inf = (T)(1.0 / (1.0 + (W)dk_exf(z, -((W)V + 21.6) / 6.7)));
+    const T Fnl_inf = (T)(1.0 / (1.0 + oexp(((W)V + 260.0) / 65.0)));
+    const W cs = 1e7 * (W)Cas * (W)Cas;
+    const T S_inf = (T)(cs / (cs + 5.6));
+    const T pb = (T)(1.0 + (W)dk_exf(z, -((W)V + 25.0) / 7.4));
+    const T P_inf = (T)(1.0 / (W)(pb * std::sqrt(pb)));
+    const T tM = dk_exf(z, -((W)V + 286.0) / 160.0);
+    const T tH = (T)(0.51 + (W)dk_exf(z, -((W)V + 26.6) / 7.1));
+    const T tN = dk_exf(z, -((W)V - 67.0) / 68.0);
+    const T tF = dk_exf(z, -((W)V - 444.0) / 220.0);
+    const T tS = (T)(500.0 / (cs + 5.6));
+    auto gate = [&](T tau, T xi, T xo) -> T {
+        return (T)(2.0 * (W)dt / (2.0 * (W)tau + (W)dt) * (W)xi + (2.0 * (W)tau - (W)dt) / (2.0 * (W)tau + (W)dt) * (W)xo);
+    };
+    const T nM = gate(tM, M_inf, M), nH = gate(tH, H_inf, H), nN = gate(tN, N_inf, N);
+    const T nRl = gate(c.tau_rl, Rl_inf, Rl), nRnl = gate(c.tau_rnl, Rnl_inf, Rnl), nFnl = gate(tF, Fnl_inf, Fnl);
+    const T nP = gate(c.tau_p, P_inf, P);
+    const T dv = V - c.eca, h = dt / T(2);
+    const T ex = -c.kcas * (c.gcal * Rl * c.k1 / (c.k2 + Cas) + c.gcanl * Rnl * Fnl) * dv - Cas / c.taucas + c.bcas;
+    const T dn = T(1) + h / c.taucas;
+    const T B = (c.k2 - Cas - h * ex + h * c.kcas * c.gcanl * nRnl * nFnl * dv + h / c.taucas * c.k2 - c.bcas * dt / T(2)) / dn;
+    const T C = (-c.k2 * Cas - c.k2 * dt / T(2) * ex + h * c.kcas * c.gcal * nRl * c.k1 * dv +
+                 h * c.kcas * c.gcanl * nRnl * nFnl * dv * c.k2 - c.bcas * dt / T(2) * c.k2) / dn;
+    const T nCas = (std::sqrt(B * B - T(4) * C) - B) / T(2);
+    const W ncs = 1e7 * (W)nCas * (W)nCas;
+    const T S_inf2 = (T)(ncs / (ncs + 5.6));
+    const T tS2 = (T)(500.0 / (ncs + 5.6));
+    const T nS = (T)(1.0 / (1.0 + (W)dt / (2.0 * (W)tS2)) *
+                     ((W)S * (1.0 - (W)dt / (2.0 * (W)tS)) + (W)dt / 2.0 * (W)(S_inf / tS + S_inf2 / tS2)));
+    const T sum = c.gcal * nRl * (c.k1 / (c.k2 + Cac)) + c.gcanl * nRl * nFnl + c.gcal * Rl * (c.k1 / (c.k2 + Cas)) +
+                  c.gcanl * Rnl * Fnl;
+    const T nCac = (T)(1.0 / (1.0 + (W)dt / (2.0 * (W)c.taucac)) *
+                       ((W)Cac * (1.0 - (W)dt / (2.0 * (W)c.taucac)) + (W)(c.bcac * dt) -
+                        (W)(dt * c.kcac) / 2.0 * (W)sum * (W)dv));
+    const T Ta = num_in == T(0) ? T(0) : Tin / num_in;
+    const T ya = c.ar * Ta + c.ad;
+    const T nY = (c.ar * Ta * dt + (T(1) - h * ya) * Y) / (h * ya + T(1));
+    const T n2 = nN * nN, n4 = n2 * n2, fL = c.k1 / (c.k2 + nCas), gn = c.gna * nM * nM * nM * nH;
+    const T G = gn + c.gk * n4 + c.gcal * nRl * fL + c.gcanl * nRnl * nFnl + gkca * nS * nS + gkleak + c.gnaleak +
+                c.gnap * nP + nY;
+    const T E = c.ena * (gn + c.gnaleak + c.gnap * nP) + c.ek * (c.gk * n4 + gkca * nS * nS + gkleak) +
+                c.eca * (c.gcal * nRl * fL + c.gcanl * nRnl * nFnl) + c.g_gaba * nY * egaba;
+    const T nV = (c.appcur * dt + E * dt + (T(1) - h * G) * V) / (T(1) + h * G);
+    const T nx[12] = {nV, nM, nH, nN, nRl, nRnl, nFnl, nS, nP, nY, nCas, nCac};
+    for (int k = 0; k < 12; ++k) x[k] = nx[k];
+    return (T)((W)c.tmax / (1.0 + (W)oexp(-(nV - c.vt) / c.kp)));
 }
 
 }  // namespace oracle

@@ -49,7 +49,8 @@ def test_libraries_load_without_gpu():
 STRUCTS = {"nnx_hh_state": "HHState", "nnx_hh_params": "HHParams", "nnx_graph": "Graph",
            "nnx_grid": "Grid", "nnx_cg_opts": "CgOpts", "nnx_cg_result": "CgResult",
            "nnx_step_plan": "StepPlan", "nnx_ens_plan": "EnsPlan", "nnx_ens_config": "EnsConfig",
-           "nnx_regions": "Regions", "nnx_hebbian": "Hebbian"}
+           "nnx_regions": "Regions", "nnx_hebbian": "Hebbian",
+           "nnx_diekman_state": "DiekmanState", "nnx_diekman_params": "DiekmanParams"}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):

@@ -120,6 +120,18 @@ def test_check_finite(ctx):
 
 
 # ------------------------------------------------------- spikes, coupling --
+def test_threshold_compares_in_double(ctx):
+    """CheckFiringOperator (src/coupling.cu:103-109) compares a Float output
+    with the double literal 0.1: the float nearest 0.1 (just above it)
+    fires, the next float below does not."""
+    t = np.float32(0.1)
+    out = np.array([t, np.nextafter(t, np.float32(0)), np.nextafter(t, np.float32(1)), 0.0], np.float32)
+    d = DA.from_numpy(ctx, out)
+    bits = DA.zeros(ctx, 1, np.uint64)
+    ctx.call("nnx_spike_threshold", _lib.F32, 4, d.ptr, 0.1, bits.ptr)
+    assert bits_to_flags(bits.numpy(), 4).tolist() == [1, 0, 1, 0]
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 4097, 300001])
 @pytest.mark.parametrize("frac", [0.0, 0.01, 1.0])
 def test_threshold_and_compact(ctx, n, frac):
