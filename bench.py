@@ -397,6 +397,50 @@ def snapshot(ctx, sim):
     return snap, sim.step
 
 
+def model_bench(ctx, n, reps=100):
+    """The reference's other neuron models at n neurons, fp32: the Diekman
+    SCN step (the reference's default program, src/main.cu:570-577) and the
+    SAN step, each timed over `reps` back-to-back launches from a
+    physiological start, with its HBM roofline (algorithmic bytes: every
+    array read and written once, plus the spike bit)."""
+    from neuronephys_clean_amd import _lib
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+
+    rng = np.random.default_rng(0)
+    V = rng.uniform(-70, -40, n)
+    e = np.exp
+    dk = {"v": V, "m": 1 / (1 + e(-(V + 35.2) / 8.1)), "h": 1 / (1 + e((V + 62) / 2)),
+          "ng": (1 + e((V - 14) / -17.0)) ** -0.25, "rl": 1 / (1 + e(-(V + 36) / 5.1)),
+          "rnl": 1 / (1 + e(-(V + 21.6) / 6.7)), "fnl": 1 / (1 + e((V + 260) / 65)), "s": np.zeros(n),
+          "p": (1 + e(-(V + 25) / 7.4)) ** -1.5, "y": np.zeros(n), "cas": np.full(n, 1e-4), "cac": np.full(n, 1e-4),
+          "iin": np.zeros(n), "icount": np.zeros(n), "ioutput": np.zeros(n), "egaba": np.full(n, -60.0),
+          "gkca": rng.normal(100, 0.5, n), "gkleak": rng.normal(0.12, 0.01, n)}
+    d = {k: DA.from_numpy(ctx, np.asarray(v, np.float32)) for k, v in dk.items()}
+    bits = DA.zeros(ctx, (n + 63) // 64, np.uint64)
+    st = _lib.DiekmanState(dtype=_lib.F32, n=n, **{k: d[k].ptr for k in dk})
+    pd = _lib.params_diekman()
+
+    def f_dk():
+        ctx.call("nnx_diekman_step", C.byref(st), C.byref(pd), 0.4, bits.ptr)
+    us_dk = timed_events(ctx, f_dk, reps) * 1e3
+    sn = {"v": rng.uniform(-70, -40, n), "nk": rng.uniform(0, 0.3, n), "ca": rng.uniform(1, 60, n),
+          "ye": np.zeros(n), "yi": np.zeros(n), "ein": np.zeros(n), "ecount": np.zeros(n), "iin": np.zeros(n),
+          "icount": np.zeros(n), "eoutput": np.zeros(n), "ioutput": np.zeros(n)}
+    d2 = {k: DA.from_numpy(ctx, np.asarray(v, np.float32)) for k, v in sn.items()}
+    st2 = _lib.SanState(dtype=_lib.F32, n=n, **{k: d2[k].ptr for k in sn})
+    ps = _lib.params_san()
+
+    def f_san():
+        ctx.call("nnx_san_step", C.byref(st2), C.byref(ps), 0.4, bits.ptr)
+    us_san = timed_events(ctx, f_san, reps) * 1e3
+    out = {}
+    for name, us, b, kern in (("diekman", us_dk, 120.125, "k_diekman<float>"), ("san", us_san, 64.125, "k_san<float>")):
+        ach = b * n / (us * 1e-6)
+        out[name] = {"kernel": kern, "neurons": n, "avg_launch_us": us, "neuron_steps_per_s": n / (us * 1e-6),
+                     "alg_bytes_per_neuron": b, "bound": "hbm", "achieved_GBs": ach / 1e9, "frac": ach / HBM_PEAK}
+    return out
+
+
 def replay_window(ctx, sim, snap, chunk, steps):
     """Restores the saved state, captures the plan's graphs on 2 chunks,
     restores again and times `steps` steps (HIP events).  Returns the time,
@@ -649,6 +693,8 @@ def main():
                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hh_bytes / (hh_ms * 1e-3) / HBM_PEAK,
                "traffic": None, "avg_launch_us": hh_ms * 1e3, "alg_bytes_per_launch": hh_bytes}
     roof = fused_roof or roof_hh
+    models = model_bench(ctx, net.n)
+    log(f"bench: models {models}")
 
     # per-step share of the coupling: eager per-kernel timing of one step's
     # coupling on the current spikes
@@ -710,7 +756,7 @@ def main():
                       "spikes_per_step": spikes / args.steps},
            "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
            "roofline_hh_unfused": roof_hh,
-           "c3": c3, "c4": c4, "c5": c5, "readout": ro, "hebbian": heb,
+           "c3": c3, "c4": c4, "c5": c5, "readout": ro, "hebbian": heb, "models": models,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},
            "event_ms": ev_ms.value}

@@ -221,6 +221,35 @@ void nnx_diekman_params_reference(nnx_diekman_params* p);
 nnx_status nnx_diekman_step(nnx_ctx* ctx, const nnx_diekman_state* st, const nnx_diekman_params* p, double dt,
                             uint64_t* spike_bits);
 
+/* Slow-wave "AN" neuron with excitatory and inhibitory synapses (SANEphys,
+ * src/ephys.cuh:34-39: Euler_an_ie_model with dxdt_an_ie_model,
+ * src/ephys.cu:105-141,203-245): V, K activation nK, Ca, AMPA (Ye) and GABA
+ * (Yi) gates, one forward-Euler step; the white-noise drive is its mean
+ * (the reference draws it from cuRAND).  Arrays in the AN_IE_VAR order
+ * (src/defines.cuh:107-117).                                               */
+typedef struct nnx_san_state {
+    int32_t dtype;
+    int32_t pad_;
+    int64_t n;
+    void *v, *nk, *ca, *ye, *yi;             /* updated */
+    const void *ein, *ecount, *iin, *icount; /* synaptic inputs and in-degrees */
+    void *eoutput, *ioutput;                 /* written after the update (both the same sigmoid) */
+} nnx_san_state;
+
+/* an_ie_params (src/defines.cuh:118-149); stored as float there.           */
+typedef struct nnx_san_params {
+    double c, a, v_l, v_na, v_k, v_ca, kd, v_ampa, v_gaba, alpha_ca, tau_ampa, tau_gaba;
+    double g_l, g_k, g_nap, g_ca, g_kca, g_ampa, g_gaba, tau_ca, noise_mean, tmax, vt, kp;
+    double out_thresh;                       /* level rule on the outputs (0.1) */
+} nnx_san_params;
+void nnx_san_params_reference(nnx_san_params* p);
+
+/* One Euler step of every neuron (SANEphys::SimulateEphys).  spike_bits
+ * (optional): bit i = output of neuron i after the step > out_thresh.
+ * powf(x, 3.5) is x^3 sqrt(x) and powf(x, 4) is (x^2)^2.                  */
+nnx_status nnx_san_step(nnx_ctx* ctx, const nnx_san_state* st, const nnx_san_params* p, double dt,
+                        uint64_t* spike_bits);
+
 /* Returns NNX_ERR_NONFINITE if any V is NaN/Inf (synchronous).             */
 nnx_status nnx_check_finite(nnx_ctx* ctx, int dtype, int64_t n, const void* d_x);
 

@@ -100,6 +100,24 @@ class DiekmanState(C.Structure):
                 [("iin", vp), ("icount", vp), ("ioutput", vp), ("egaba", vp), ("gkca", vp), ("gkleak", vp)])
 
 
+SAN_PARAM_FIELDS = ("c", "a", "v_l", "v_na", "v_k", "v_ca", "kd", "v_ampa", "v_gaba", "alpha_ca", "tau_ampa",
+                    "tau_gaba", "g_l", "g_k", "g_nap", "g_ca", "g_kca", "g_ampa", "g_gaba", "tau_ca", "noise_mean",
+                    "tmax", "vt", "kp", "out_thresh")
+
+
+class SanParams(C.Structure):
+    _fields_ = [(f, C.c_double) for f in SAN_PARAM_FIELDS]
+
+    def consts(self):
+        """The 24 model constants (everything but out_thresh), in order."""
+        return [getattr(self, f) for f in SAN_PARAM_FIELDS[:24]]
+
+
+class SanState(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("pad_", C.c_int32), ("n", C.c_int64)] + [
+        (f, vp) for f in ("v", "nk", "ca", "ye", "yi", "ein", "ecount", "iin", "icount", "eoutput", "ioutput")]
+
+
 class Regions(C.Structure):
     _fields_ = [("n_regions", C.c_int64), ("nnz", C.c_int64), ("offsets", vp), ("idx", vp)]
 
@@ -185,6 +203,8 @@ _SIGS = {
     "nnx_hebbian_init": ([vp, _P(Graph), _P(Hebbian)], C.c_int),
     "nnx_diekman_params_reference": ([_P(DiekmanParams)], None),
     "nnx_diekman_step": ([vp, _P(DiekmanState), _P(DiekmanParams), C.c_double, vp], C.c_int),
+    "nnx_san_params_reference": ([_P(SanParams)], None),
+    "nnx_san_step": ([vp, _P(SanState), _P(SanParams), C.c_double, vp], C.c_int),
     "nnx_hebbian_update": ([vp, _P(Graph), _P(Hebbian), C.c_int64, vp], C.c_int),
     "nnx_unet_param_count": ([], C.c_int64),
     "nnx_unet_create": ([vp, vp, C.c_uint64, _P(vp)], C.c_int),
@@ -331,6 +351,12 @@ def params_reference() -> HHParams:
 def params_diekman() -> DiekmanParams:
     p = DiekmanParams()
     lib().nnx_diekman_params_reference(C.byref(p))
+    return p
+
+
+def params_san() -> SanParams:
+    p = SanParams()
+    lib().nnx_san_params_reference(C.byref(p))
     return p
 
 

@@ -7,6 +7,7 @@
 // Replaced interfaces (neuronephys_clean sources):
 //   Ephys / HHEphys::SimulateEphys(Float**)     src/ephys.cuh:6-31, src/ephys.cu:98-103
 //   DiekmanEphys::SimulateEphys(Float**)        src/ephys.cu:250-353
+//   SANEphys::SimulateEphys(Float**)            src/ephys.cu:203-248
 //   AtomicCoupling(int), operator(), GetFiringNeurons()   src/coupling.cuh:17-38
 //   AtomicCoupling(int, AdjacencyList&, Float*, Float*), PerformCoupling()
 //                                               src_wholebrain/coupling.cuh:26-43
@@ -277,6 +278,40 @@ class DiekmanEphys : public Ephys {
         st.iin = a[DIEKMAN_VAR_IIN], st.icount = a[DIEKMAN_VAR_ICOUNT], st.ioutput = a[DIEKMAN_VAR_IOUTPUT];
         st.egaba = a[DIEKMAN_VAR_EGABA], st.gkca = a[DIEKMAN_VAR_GKCA], st.gkleak = a[DIEKMAN_VAR_GKLEAK];
         check(nnx_diekman_step(ctx(), &st, &p_, HHEphys::leapfrog_dt, nullptr));
+        check(nnx_synchronize(ctx()));
+    }
+};
+
+// SoA indices of the SAN model (src/defines.cuh:107-117)
+enum AnIeVars {
+    AN_IE_VAR_V = 0, AN_IE_VAR_NK, AN_IE_VAR_CA, AN_IE_VAR_YE, AN_IE_VAR_YI, AN_IE_VAR_EIN, AN_IE_VAR_ECOUNT,
+    AN_IE_VAR_EOUTPUT, AN_IE_VAR_IIN, AN_IE_VAR_ICOUNT, AN_IE_VAR_IOUTPUT, NUM_AN_IE_VAR
+};
+
+// Euler_an_ie_model at hh_params::leapfrog_dt (0.4 ms), as
+// SANEphys::SimulateEphys launches it (white noise replaced by its mean).
+class SANEphys : public Ephys {
+    std::map<Float**, std::vector<Float*>> tables_;
+    nnx_san_params p_{};
+
+   public:
+    explicit SANEphys(int const N_) : Ephys(N_) { nnx_san_params_reference(&p_); }
+    nnx_san_params& params() { return p_; }
+    void SimulateEphys(Float** sim_vars) override {
+        auto it = tables_.find(sim_vars);
+        if (it == tables_.end()) {
+            std::vector<Float*> h(NUM_AN_IE_VAR);
+            check(nnx_memcpy_d2h(ctx(), h.data(), sim_vars, NUM_AN_IE_VAR * sizeof(Float*)));
+            it = tables_.emplace(sim_vars, h).first;
+        }
+        const std::vector<Float*>& a = it->second;
+        nnx_san_state st{};
+        st.dtype = NNX_F32;
+        st.n = N;
+        st.v = a[AN_IE_VAR_V], st.nk = a[AN_IE_VAR_NK], st.ca = a[AN_IE_VAR_CA], st.ye = a[AN_IE_VAR_YE];
+        st.yi = a[AN_IE_VAR_YI], st.ein = a[AN_IE_VAR_EIN], st.ecount = a[AN_IE_VAR_ECOUNT], st.iin = a[AN_IE_VAR_IIN];
+        st.icount = a[AN_IE_VAR_ICOUNT], st.eoutput = a[AN_IE_VAR_EOUTPUT], st.ioutput = a[AN_IE_VAR_IOUTPUT];
+        check(nnx_san_step(ctx(), &st, &p_, HHEphys::leapfrog_dt, nullptr));
         check(nnx_synchronize(ctx()));
     }
 };

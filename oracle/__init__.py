@@ -252,3 +252,26 @@ def diekman_step(consts, dt, x, inputs, thr=0.1):
     getattr(lib(), "orc_diekman_" + _sfx(dt_))(C.c_int64(n), _p(c), C.c_double(dt), C.c_double(thr), xs, ii,
                                                _p(out), _p(sp))
     return out, sp
+
+
+SAN_STATE = ("v", "nk", "ca", "ye", "yi")
+SAN_IN = ("ein", "ecount", "iin", "icount")
+
+
+def san_step(consts, dt, x, inputs, thr=0.1):
+    """One Euler_an_ie_model step (noise = its mean); x: dict of the 5 state
+    arrays (updated in place), inputs: dict of the 4 input arrays; consts:
+    the 24 constants in nnx_san_params order.  Returns (output, spike flags)."""
+    a = [x[k] for k in SAN_STATE]
+    dt_ = a[0].dtype
+    assert all(v.dtype == dt_ and v.flags.c_contiguous for v in a)
+    ins = [np.ascontiguousarray(inputs[k], dt_) for k in SAN_IN]
+    n = len(a[0])
+    out = np.empty(n, dt_)
+    sp = np.empty(n, np.uint8)
+    c = np.ascontiguousarray(consts, np.float64)
+    xs = (C.c_void_p * 5)(*[v.ctypes.data for v in a])
+    ii = (C.c_void_p * 4)(*[v.ctypes.data for v in ins])
+    getattr(lib(), "orc_san_" + _sfx(dt_))(C.c_int64(n), _p(c), C.c_double(dt), C.c_double(thr), xs, ii,
+                                           _p(out), _p(sp))
+    return out, sp

@@ -362,3 +362,34 @@ void orc_diekman_f64(int64_t n, const double* c, double dt, double thr, double* 
 }
 
 }  // extern "C"
+
+// SAN step over arrays: x = 5 state arrays (V, nK, Ca, Ye, Yi), in = 4
+// input arrays (Ein, Ecount, Iin, Icount), out = the output; c = 24
+// constants in nnx_san_params order; spiked (optional): out > thr widened.
+template <class T>
+static void san_arrays(int64_t n, const double* c, double dt, double thr, T* const* x, const T* const* in, T* out,
+                       uint8_t* spiked) {
+    T k[24];
+    for (int i = 0; i < 24; ++i) k[i] = (T)c[i];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        T s[5], v[4];
+        for (int q = 0; q < 5; ++q) s[q] = x[q][i];
+        for (int q = 0; q < 4; ++q) v[q] = in[q][i];
+        out[i] = san_step<T>(k, (T)dt, s, v);
+        for (int q = 0; q < 5; ++q) x[q][i] = s[q];
+        if (spiked) spiked[i] = (double)out[i] > thr ? 1 : 0;
+    }
+}
+
+extern "C" {
+void orc_san_f32(int64_t n, const double* c, double dt, double thr, float* const* x, const float* const* in, float* out,
+                 uint8_t* spiked) {
+    san_arrays<float>(n, c, dt, thr, x, in, out, spiked);
+}
+void orc_san_f64(int64_t n, const double* c, double dt, double thr, double* const* x, const double* const* in,
+                 double* out, uint8_t* spiked) {
+    san_arrays<double>(n, c, dt, thr, x, in, out, spiked);
+}
+
+}  // extern "C"

@@ -21,6 +21,7 @@
 //                               electrode readout and synchrony
 //   src_wholebrain/hebbian.cu:3-69   Hebbian learning
 //   src/ephys.cu:250-347        Leapfrog_diekman (SCN neuron)
+//   src/ephys.cu:105-141,203-245  Euler_an_ie_model (SAN neuron)
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -686,6 +687,46 @@ template <class T> T diekman_step(const DkC<T>& c, T dt, T* x, const T* in) {
     const T nx[12] = {nV, nM, nH, nN, nRl, nRnl, nFnl, nS, nP, nY, nCas, nCac};
     for (int k = 0; k < 12; ++k) x[k] = nx[k];
     return (T)((W)c.tmax / (1.0 + (W)oexp(-(nV - c.vt) / c.kp)));
+}
+
+// --------------------------------------------------------------------------
+// SAN ("AN" slow-wave) neuron, one forward-Euler step (Euler_an_ie_model
+// with dxdt_an_ie_model, src/ephys.cu:105-141,203-245); the white noise is
+// its mean.  fp32 keeps the reference's types: expf of float arguments, exp
+// of double ones, double literals (W); powf(x, 3.5) = x^3 sqrt(x),
+// powf(x, 4) = (x^2)^2.  c: 24 constants in nnx_san_params order.  x: V, nK,
+// Ca, Ye, Yi (updated); in: Ein, Ecount, Iin, Icount; returns the output.
+// --------------------------------------------------------------------------
+template <class T> T san_step(const T* c, T dt, T* x, const T* in) {
+    using W = double;
+    const T C = c[0], A = c[1], vL = c[2], vNa = c[3], vK = c[4], vCa = c[5], kD = c[6], vAMPA = c[7], vGABA = c[8];
+    const T alphaCa = c[9], tauAMPA = c[10], tauGABA = c[11], gL = c[12], gK = c[13], gNaP = c[14], gCa = c[15];
+    const T gKCa = c[16], gAMPA = c[17], gGABA = c[18], tauCa = c[19], noise = c[20], Tmax = c[21], Vt = c[22];
+    const T Kp = c[23];
+    const T V = x[0], nK = x[1], Ca = x[2], Ye = x[3], Yi = x[4];
+    const T eT = in[1] == T(0) ? T(0) : in[0] / in[1];
+    const T iT = in[3] == T(0) ? T(0) : in[2] / in[3];
+    const T mCa = T(1) / (T(1) + oexp(-(V + T(20)) / T(9)));
+    const T r = kD / Ca;
+    const T mKCa = T(1) / (T(1) + r * r * r * std::sqrt(r));
+    const T mNaP = (T)(1.0 / (1.0 + (W)oexp(-((W)V + 55.7) / 7.7)));
+    const T an = (T)(V == T(-34) ? 0.1 : 0.01 * (W)(V + T(34)) / (W)(T(1) - oexp(-(V + T(34)) / T(10))));
+    const T bn = (T)(0.125 * (W)oexp(-(V + T(44)) / T(25)));
+    const T dn = T(4) * (an * (T(1) - nK) - bn * nK);
+    const T dye = (T)(3.48 * (W)eT - (W)(Ye / tauAMPA));
+    const T dyi = iT - Yi / tauGABA;
+    const T ICa = gCa * mCa * mCa * (V - vCa);
+    const T dca = -alphaCa * (T(10) * A * ICa) - Ca / tauCa;
+    const T n2 = nK * nK;
+    T dv = noise - gK * (n2 * n2) * (V - vK) - ICa - gKCa * mKCa * (V - vK) - gNaP * mNaP * mNaP * mNaP * (V - vNa) -
+           gL * (V - vL) - gAMPA * Ye * (V - vAMPA) - gGABA * Yi * (V - vGABA);
+    dv = dv / C;
+    x[0] = V + dt * dv;
+    x[1] = nK + dt * dn;
+    x[2] = Ca + dt * dca;
+    x[3] = Ye + dt * dye;
+    x[4] = Yi + dt * dyi;
+    return (T)((W)Tmax / (1.0 + (W)oexp(-(x[0] - Vt) / Kp)));
 }
 
 }  // namespace oracle

@@ -50,7 +50,8 @@ STRUCTS = {"nnx_hh_state": "HHState", "nnx_hh_params": "HHParams", "nnx_graph": 
            "nnx_grid": "Grid", "nnx_cg_opts": "CgOpts", "nnx_cg_result": "CgResult",
            "nnx_step_plan": "StepPlan", "nnx_ens_plan": "EnsPlan", "nnx_ens_config": "EnsConfig",
            "nnx_regions": "Regions", "nnx_hebbian": "Hebbian",
-           "nnx_diekman_state": "DiekmanState", "nnx_diekman_params": "DiekmanParams"}
+           "nnx_diekman_state": "DiekmanState", "nnx_diekman_params": "DiekmanParams",
+           "nnx_san_state": "SanState", "nnx_san_params": "SanParams"}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
