@@ -333,6 +333,8 @@ def profile_pass(ctx, sim, net):
                  avg.ptr, enet.d_firing.ptr)
         ctx.call("nnx_network_mean", _lib.F32, net.n, sim.a["v"].ptr, mean.ptr)
     ctx.synchronize()
+    # 21 launches each of the Diekman and SAN steps at the same size
+    model_bench(ctx, net.n, reps=20)
     # 20 steps of the 256-network ensemble (default launch shape)
     ens = EnsembleSim(configs.c4_ensemble(seed=0, count=256), ctx=ctx)
     ens.run(20)
@@ -366,7 +368,8 @@ def profile_pass(ctx, sim, net):
     src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
-    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 electrode readouts, 20 ensemble steps, "
+    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 electrode readouts, 21 Diekman and SAN "
+        "steps, 20 ensemble steps, "
         "5 network applications, "
         f"20 stencil applies, 3 V-cycles, 1 CG solve ({r.iters} iterations), 256 MiB copy")
 
@@ -595,7 +598,8 @@ def main():
                          "region (the timed window then starts later in the simulation)")
     ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
                     help="profile: a short eager sequence for rocprofv3 --pmc passes (20 fused and 20 "
-                         "unfused C2 steps, 20 HH launches, 20 electrode readouts, 20 ensemble steps, 5 network applications, "
+                         "unfused C2 steps, 20 HH launches, 20 electrode readouts, 21 Diekman and 21 SAN steps, "
+                         "20 ensemble steps, 5 network applications, "
                          "20 stencil applies, 3 V-cycles and one CG solve on 256^3 grids, one 256 MiB "
                          "device copy to calibrate FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--no-cpu", action="store_true")
