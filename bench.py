@@ -335,6 +335,14 @@ def profile_pass(ctx, sim, net):
     ctx.synchronize()
     # 21 launches each of the Diekman and SAN steps at the same size
     model_bench(ctx, net.n, reps=20)
+    # 20 Hebbian updates (window 300) on the firing of 20 more fused steps
+    from neuronephys_clean_amd import hebbian
+    hs = hebbian.HebbianState(ctx, sim.graph, 300, 0.1, net.w)
+    for _ in range(20):
+        sim.run(1)
+        hs.update(sim.step, sim.a["bits0"] if sim.step % 2 else sim.a["bits1"])  # the step just run
+    ctx.synchronize()
+    del hs
     # 20 steps of the 256-network ensemble (default launch shape)
     ens = EnsembleSim(configs.c4_ensemble(seed=0, count=256), ctx=ctx)
     ens.run(20)
@@ -369,7 +377,7 @@ def profile_pass(ctx, sim, net):
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
     log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 electrode readouts, 21 Diekman and SAN "
-        "steps, 20 ensemble steps, "
+        "steps, 20 Hebbian updates, 20 ensemble steps, "
         "5 network applications, "
         f"20 stencil applies, 3 V-cycles, 1 CG solve ({r.iters} iterations), 256 MiB copy")
 
@@ -599,7 +607,7 @@ def main():
     ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
                     help="profile: a short eager sequence for rocprofv3 --pmc passes (20 fused and 20 "
                          "unfused C2 steps, 20 HH launches, 20 electrode readouts, 21 Diekman and 21 SAN steps, "
-                         "20 ensemble steps, 5 network applications, "
+                         "20 Hebbian updates on 20 more C2 steps, 20 ensemble steps, 5 network applications, "
                          "20 stencil applies, 3 V-cycles and one CG solve on 256^3 grids, one 256 MiB "
                          "device copy to calibrate FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--no-cpu", action="store_true")
