@@ -168,13 +168,13 @@ def precision_split(ctx, net, steps=(1, 100, 10000)):
     return res
 
 
-def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=0):
+def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=0, tune=False):
     """Batched ensemble: 256 networks (sizes 2^U(10,16), Poisson(50)
     out-degree), one fused persistent launch per step, 100 steps per
-    hipGraph, launch shape from the autotuner (whose steps are ordinary
-    steps, run after `warmup` steps and before the timed region: the timed
-    5,000 steps are the simulation's first after tuning)."""
-    from neuronephys_clean_amd import configs
+    hipGraph, the library's default launch shape (with tune, the
+    autotuner's, whose steps are ordinary steps run after `warmup` steps and
+    before the timed region)."""
+    from neuronephys_clean_amd import _lib, configs
     from neuronephys_clean_amd.sim import EnsembleSim
     t0 = time.time()
     ens = configs.c4_ensemble(seed=0, count=256)
@@ -184,8 +184,12 @@ def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=0):
     sim = EnsembleSim(ens, ctx=ctx, deposit=True)
     if warmup:
         sim.run(warmup, use_graph=True, chunk=chunk)
-    best, tune_us, ran = sim.autotune(tune_steps)
-    log(f"bench: C4 autotune -> {best.as_dict()} ({tune_us:.1f} us/step, {ran} steps)")
+    if tune:
+        best, tune_us, ran = sim.autotune(tune_steps)
+        log(f"bench: C4 autotune -> {best.as_dict()} ({tune_us:.1f} us/step, {ran} steps)")
+    else:
+        best, tune_us, ran = _lib.EnsConfig(block=256, tile=256, grid=8192), None, 0
+        sim.cfg = best
     sim.run(chunk - sim.step % chunk, use_graph=False)  # align the graph chunks
     sim.run(chunk, use_graph=True, chunk=chunk)         # capture
     sim.synchronize()
@@ -741,7 +745,7 @@ def main():
 
     c4 = None
     if not args.no_c4:
-        c4 = c4_bench(ctx)
+        c4 = c4_bench(ctx, tune=args.tune)
         log(f"bench: C4 {c4}")
 
     c3 = None

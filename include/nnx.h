@@ -451,7 +451,7 @@ typedef struct nnx_ens_config {
 
 /* n_steps steps from first_step (first_step == 0 clears the pending
  * coupling and the spike bitmaps) with one launch per step, optionally
- * replayed from a captured hipGraph; cfg NULL = {256, 2048, 1024}.  The
+ * replayed from a captured hipGraph; cfg NULL = {256, 256, 8192}.  The
  * result does not depend on cfg.                                            */
 nnx_status nnx_ensemble_run(nnx_ctx* ctx, const nnx_ens_plan* plan, const nnx_ens_config* cfg,
                             uint64_t first_step, int n_steps, int use_graph);
