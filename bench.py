@@ -719,6 +719,23 @@ def main():
                  sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
     couple_ms = timed_events(ctx, couple, 50)
 
+    # the dense pull coupling (WEIGHTED_PULL: every target reads all its
+    # in-edges and the spike bitmap), timed alone; with the HH kernel it is
+    # the step whose bytes the 60%-of-HBM target counts: 4(N+1) + 8E + N/8 +
+    # 4N for the pull plus 48.125 B per neuron for HH
+    def pull():
+        ctx.call("nnx_couple", C.byref(sim.graph), _lib.COUPLE_WEIGHTED_PULL, _lib.F32,
+                 sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
+    pull_ms = timed_events(ctx, pull, 20)
+    pull_bytes = 4.0 * (net.n + 1) + 8.0 * net.nnz + net.n / 8.0 + 4.0 * net.n
+    roof_pull = {"kernel": "k_pull<float> (dense pull over the CSC, ascending-source sums)", "bound": "hbm",
+                 "avg_launch_us": pull_ms * 1e3, "alg_bytes_per_launch": pull_bytes,
+                 "achieved": pull_bytes / (pull_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                 "frac": pull_bytes / (pull_ms * 1e-3) / HBM_PEAK,
+                 "hh_plus_pull_frac": (pull_bytes + hh_bytes) / ((pull_ms + hh_ms) * 1e-3) / HBM_PEAK,
+                 "hh_plus_pull_neuron_steps_per_s": net.n / ((pull_ms + hh_ms) * 1e-3)}
+    log(f"bench: pull coupling {roof_pull}")
+
     ro = heb = None
     if snap is not None:
         ro = readout_bench(ctx, sim, net, args.chunk, snap, steps=args.steps)
@@ -771,7 +788,7 @@ def main():
                       "graph_chunk": args.chunk, "spikes_timed": int(spikes),
                       "spikes_per_step": spikes / args.steps},
            "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
-           "roofline_hh_unfused": roof_hh,
+           "roofline_hh_unfused": roof_hh, "roofline_pull_coupling": roof_pull,
            "c3": c3, "c4": c4, "c5": c5, "readout": ro, "hebbian": heb, "models": models,
            "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
                                  "step_graph": ms_step * 1e3},

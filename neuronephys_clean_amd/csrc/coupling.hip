@@ -134,40 +134,185 @@ __global__ __launch_bounds__(kCB) void k_compact(int64_t nwords, const uint64_t*
     }
 }
 
+// value of lane l (wave-uniform) of x: v_readlane, a scalar read of one lane
+__device__ __forceinline__ int rdl(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ float rdl(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double rdl(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // ------------------------------------------------------------------------
-// dense pull over the CSC: one wavefront per target, lanes read 64 in-edges
-// at a time (coalesced), and the contributions of the spiking lanes are
-// added in ascending lane (= source) order by every lane in lockstep.
+// dense pull over the CSC: one wavefront per target.  Each lane reads four
+// consecutive in-edges with one 16-byte load (a wavefront covers 256 edges
+// per load, from the target's range rounded down to a multiple of four), tests
+// their four spike bits, loads the values of the spiking ones, and the
+// contributions are added in ascending edge (= source) order - lanes in
+// order, the four edges of a lane in order - by every lane in lockstep.  The
+// next target's range is loaded while the current one is summed.
+// VEC = 0: in_src not 16-byte aligned, one edge per lane at a time.
 // mode: 0 weighted, 2 count, 3 signal
 // ------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_pull(int64_t n, int mode, const int32_t* __restrict__ in_off,
+template <class T, bool VEC>
+__global__ __launch_bounds__(256) void k_pull(int64_t n, int64_t nnz, int mode, const int32_t* __restrict__ in_off,
                                               const int32_t* __restrict__ in_src,
                                               const float* __restrict__ in_w,
                                               const uint64_t* __restrict__ bits,
                                               const T* __restrict__ outputs, T* __restrict__ g) {
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = (int64_t)gridDim.x * 4;
-    for (int64_t t = blockIdx.x * 4ll + (threadIdx.x >> 6); t < n; t += nwaves) {
-        const int32_t e0 = in_off[t], e1 = in_off[t + 1];
+    int64_t t = blockIdx.x * 4ll + (threadIdx.x >> 6);
+    int32_t e0 = t < n ? in_off[t] : 0, e1 = t < n ? in_off[t + 1] : 0;
+    for (; t < n; t += nwaves) {
+        const int64_t tn = t + nwaves;
+        const int32_t ne0 = tn < n ? in_off[tn] : 0, ne1 = tn < n ? in_off[tn + 1] : 0;
         T acc = T(0);
-        for (int32_t b = e0; b < e1; b += 64) {
-            const int32_t e = b + lane;
-            bool act = false;
-            T val = T(0);
-            if (e < e1) {
-                const int32_t s = in_src[e];
-                act = bit_of(bits, s);
-                if (act) val = mode == 0 ? T(in_w[e]) : mode == 2 ? T(1) : outputs[s];
+        if (VEC) {
+            for (int32_t base = e0 & ~3; base < e1; base += 256) {
+                const int32_t q = base + 4 * lane;
+                int32_t s4[4] = {-1, -1, -1, -1};
+                if (q < e1) {
+                    if ((int64_t)q + 3 < nnz) {
+                        const int4 v = *reinterpret_cast<const int4*>(in_src + q);
+                        s4[0] = v.x, s4[1] = v.y, s4[2] = v.z, s4[3] = v.w;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if ((int64_t)q + j < nnz) s4[j] = in_src[q + j];
+                    }
+                }
+                int am = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int32_t e = q + j;
+                    if (e >= e0 && e < e1 && bit_of(bits, s4[j])) am |= 1 << j;
+                }
+                T v[4] = {T(0), T(0), T(0), T(0)};
+                if (am) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if ((am >> j) & 1)
+                            v[j] = mode == 0 ? T(in_w[q + j]) : mode == 2 ? T(1) : outputs[s4[j]];
+                }
+                uint64_t m = __ballot(am != 0);
+                while (m) {
+                    const int l = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const int ml = rdl(am, l);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if ((ml >> j) & 1) acc = acc + rdl(v[j], l);
+                }
             }
-            uint64_t m = __ballot(act);
-            while (m) {
-                const int l = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                acc = acc + __shfl(val, l, 64);
+        } else {
+            for (int32_t b = e0; b < e1; b += 64) {
+                const int32_t e = b + lane;
+                bool act = false;
+                T val = T(0);
+                if (e < e1) {
+                    const int32_t src = in_src[e];
+                    act = bit_of(bits, src);
+                    if (act) val = mode == 0 ? T(in_w[e]) : mode == 2 ? T(1) : outputs[src];
+                }
+                uint64_t m = __ballot(act);
+                while (m) {
+                    const int l = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    acc = acc + rdl(val, l);
+                }
             }
         }
         if (lane == 0) g[t] = acc;
+        e0 = ne0, e1 = ne1;
+    }
+}
+
+// Networks of up to kLdsBitsMax neurons: the same pull with the spike bitmap
+// copied into LDS first (128 KiB at 1,048,576 neurons), so that the bit test
+// of every in-edge is an LDS read instead of a random L2 transaction.  One
+// workgroup of 1024 threads per CU walks the targets; each lane still reads
+// four consecutive in-edges with one 16-byte load.
+constexpr int64_t kLdsBitsMax = 1 << 20;
+
+// this lane's four in-edges q..q+3 (entries past nnz read as -1)
+__device__ __forceinline__ void load4(const int32_t* __restrict__ in_src, int64_t nnz, int32_t q, int32_t e1,
+                                      int32_t (&s4)[4]) {
+    s4[0] = s4[1] = s4[2] = s4[3] = -1;
+    if (q < e1) {
+        if ((int64_t)q + 3 < nnz) {
+            const int4 v = *reinterpret_cast<const int4*>(in_src + q);
+            s4[0] = v.x, s4[1] = v.y, s4[2] = v.z, s4[3] = v.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((int64_t)q + j < nnz) s4[j] = in_src[q + j];
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_pull_lds(int64_t n, int64_t nnz, int mode,
+                                                   const int32_t* __restrict__ in_off,
+                                                   const int32_t* __restrict__ in_src,
+                                                   const float* __restrict__ in_w,
+                                                   const uint64_t* __restrict__ bits,
+                                                   const T* __restrict__ outputs, T* __restrict__ g) {
+    __shared__ uint32_t sb[kLdsBitsMax / 32];
+    const int64_t nw32 = (n + 31) >> 5;
+    const uint32_t* b32 = reinterpret_cast<const uint32_t*>(bits);
+    for (int64_t i = threadIdx.x; i < nw32; i += 1024) sb[i] = b32[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * 16;
+    int64_t t = blockIdx.x * 16ll + (threadIdx.x >> 6);
+    // software pipeline: the next target's range and first 256 in-edges are
+    // in flight while the current target is summed
+    int32_t e0 = t < n ? in_off[t] : 0, e1 = t < n ? in_off[t + 1] : 0;
+    int32_t s4[4];
+    load4(in_src, nnz, (e0 & ~3) + 4 * lane, e1, s4);
+    int64_t tn = t + nwaves;
+    int32_t ne0 = tn < n ? in_off[tn] : 0, ne1 = tn < n ? in_off[tn + 1] : 0;
+    for (; t < n; t += nwaves) {
+        int32_t s4n[4];
+        load4(in_src, nnz, (ne0 & ~3) + 4 * lane, ne1, s4n);
+        const int64_t tnn = tn + nwaves;
+        const int32_t nne0 = tnn < n ? in_off[tnn] : 0, nne1 = tnn < n ? in_off[tnn + 1] : 0;
+        T acc = T(0);
+        for (int32_t base = e0 & ~3; base < e1; base += 256) {
+            const int32_t q = base + 4 * lane;
+            if (base != (e0 & ~3)) load4(in_src, nnz, q, e1, s4);
+            int am = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t e = q + j;
+                if (e >= e0 && e < e1 && ((sb[s4[j] >> 5] >> (s4[j] & 31)) & 1u)) am |= 1 << j;
+            }
+            T v[4] = {T(0), T(0), T(0), T(0)};
+            if (am) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if ((am >> j) & 1) v[j] = mode == 0 ? T(in_w[q + j]) : mode == 2 ? T(1) : outputs[s4[j]];
+            }
+            uint64_t m = __ballot(am != 0);
+            while (m) {
+                const int l = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const int ml = rdl(am, l);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if ((ml >> j) & 1) acc = acc + rdl(v[j], l);
+            }
+        }
+        if (lane == 0) g[t] = acc;
+        e0 = ne0, e1 = ne1;
+        ne0 = nne0, ne1 = nne1;
+        tn = tnn;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s4[j] = s4n[j];
     }
 }
 
@@ -299,14 +444,30 @@ nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, co
     }
     const int m = mode == NNX_COUPLE_WEIGHTED_PULL ? 0 : mode == NNX_COUPLE_COUNT ? 2 : 3;
     const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 4), 65536);
-    if (dtype == NNX_F32)
-        k_pull<float><<<blocks, 256, 0, c->stream>>>(n, m, g->in_offsets, g->in_sources,
-                                                     g->in_weights, bits, (const float*)outputs,
-                                                     (float*)g_in);
-    else
-        k_pull<double><<<blocks, 256, 0, c->stream>>>(n, m, g->in_offsets, g->in_sources,
-                                                      g->in_weights, bits, (const double*)outputs,
-                                                      (double*)g_in);
+    const bool vec = ((uintptr_t)g->in_sources & 15) == 0;
+    if (vec && n <= kLdsBitsMax) {
+        // one 1024-thread workgroup per CU (the bitmap copy takes 128 KiB of LDS)
+        const unsigned nb = (unsigned)std::min<int64_t>(ceil_div(n, 16), 256);
+        if (dtype == NNX_F32)
+            k_pull_lds<float><<<nb, 1024, 0, c->stream>>>(n, g->nnz, m, g->in_offsets, g->in_sources, g->in_weights,
+                                                          bits, (const float*)outputs, (float*)g_in);
+        else
+            k_pull_lds<double><<<nb, 1024, 0, c->stream>>>(n, g->nnz, m, g->in_offsets, g->in_sources,
+                                                           g->in_weights, bits, (const double*)outputs, (double*)g_in);
+        NNX_LAUNCHED(c);
+        return NNX_OK;
+    }
+#define NNX_PULL(T, V)                                                                                    \
+    k_pull<T, V><<<blocks, 256, 0, c->stream>>>(n, g->nnz, m, g->in_offsets, g->in_sources, g->in_weights, \
+                                                bits, (const T*)outputs, (T*)g_in)
+    if (dtype == NNX_F32) {
+        if (vec) NNX_PULL(float, true);
+        else NNX_PULL(float, false);
+    } else {
+        if (vec) NNX_PULL(double, true);
+        else NNX_PULL(double, false);
+    }
+#undef NNX_PULL
     NNX_LAUNCHED(c);
     return NNX_OK;
 }
