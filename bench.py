@@ -327,6 +327,15 @@ def profile_pass(ctx, sim, net):
         ctx.call("nnx_hh_step", C.byref(st), C.byref(p), _lib.INT_RK4, _lib.SPIKE_CROSSING, net.dt, 0,
                  sim.a["g_in"].ptr, sim.a["bits0"].ptr)
     ctx.synchronize()
+    # 20 dense pulls (WEIGHTED_PULL) on a seeded bitmap of 1% of the neurons
+    fire = np.random.default_rng(11).random(net.n) < 0.01
+    pb = DA.from_numpy(ctx, np.packbits(np.concatenate([fire, np.zeros(-net.n % 64, bool)]),
+                                        bitorder="little").view(np.uint64))
+    for _ in range(20):
+        ctx.call("nnx_couple", C.byref(sim.graph), _lib.COUPLE_WEIGHTED_PULL, _lib.F32, pb.ptr, None, None, None,
+                 sim.a["g_in"].ptr)
+    ctx.synchronize()
+    log(f"profile: dense pull spiking in-edges E_s={int(np.diff(net.off.astype(np.int64))[fire].sum())}")
     # 20 electrode readouts (1,000 electrodes x 50 nearest neurons) and network means
     from neuronephys_clean_amd import host, readout
     off, idx = host.spherical_regions(net.pos, 50, center_idx=(net.n // 1000) * np.arange(1000))
@@ -380,7 +389,8 @@ def profile_pass(ctx, sim, net):
     src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
-    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 electrode readouts, 21 Diekman and SAN "
+    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 dense pulls, 20 electrode readouts, "
+        "21 Diekman and SAN "
         "steps, 20 Hebbian updates, 20 ensemble steps, "
         "5 network applications, "
         f"20 stencil applies, 3 V-cycles, 1 CG solve ({r.iters} iterations), 256 MiB copy")
@@ -720,20 +730,37 @@ def main():
     couple_ms = timed_events(ctx, couple, 50)
 
     # the dense pull coupling (WEIGHTED_PULL: every target reads all its
-    # in-edges and the spike bitmap), timed alone; with the HH kernel it is
-    # the step whose bytes the 60%-of-HBM target counts: 4(N+1) + 8E + N/8 +
-    # 4N for the pull plus 48.125 B per neuron for HH
-    def pull():
-        ctx.call("nnx_couple", C.byref(sim.graph), _lib.COUPLE_WEIGHTED_PULL, _lib.F32,
-                 sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
-    pull_ms = timed_events(ctx, pull, 20)
-    pull_bytes = 4.0 * (net.n + 1) + 8.0 * net.nnz + net.n / 8.0 + 4.0 * net.n
-    roof_pull = {"kernel": "k_pull<float> (dense pull over the CSC, ascending-source sums)", "bound": "hbm",
-                 "avg_launch_us": pull_ms * 1e3, "alg_bytes_per_launch": pull_bytes,
-                 "achieved": pull_bytes / (pull_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                 "frac": pull_bytes / (pull_ms * 1e-3) / HBM_PEAK,
-                 "hh_plus_pull_frac": (pull_bytes + hh_bytes) / ((pull_ms + hh_ms) * 1e-3) / HBM_PEAK,
-                 "hh_plus_pull_neuron_steps_per_s": net.n / ((pull_ms + hh_ms) * 1e-3)}
+    # in-edges, the spike bitmap and the weights of its spiking in-edges),
+    # timed alone on seeded spike bitmaps of 0.1%, 1% and 3% of the neurons.
+    # Algorithmic bytes: 4(N+1) + 4E + 4E_s + N/8 + 4N (offsets, sources,
+    # weights of the E_s spiking in-edges, bitmap, g_in); with the HH kernel
+    # (48.125 B per neuron) beside it, the pair is an unfused HH + coupling
+    # step whose bytes are counted against the HBM peak.
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+    outdeg = np.diff(net.off.astype(np.int64))
+    prng = np.random.default_rng(11)
+    pbits = DA.zeros(ctx, (net.n + 63) // 64, np.uint64)
+    pull_cases = []
+    for frac in (0.001, 0.01, 0.03):
+        fire = prng.random(net.n) < frac
+        pbits.upload(np.packbits(np.concatenate([fire, np.zeros(-net.n % 64, bool)]),
+                                 bitorder="little").view(np.uint64))
+        e_s = int(outdeg[fire].sum())
+
+        def pull():
+            ctx.call("nnx_couple", C.byref(sim.graph), _lib.COUPLE_WEIGHTED_PULL, _lib.F32,
+                     pbits.ptr, None, None, None, gin)
+        ms = timed_events(ctx, pull, 20)
+        b = 4.0 * (net.n + 1) + 4.0 * net.nnz + 4.0 * e_s + net.n / 8.0 + 4.0 * net.n
+        pull_cases.append({"firing": frac, "spiking_in_edges": e_s, "avg_launch_us": ms * 1e3,
+                           "alg_bytes_per_launch": b, "achieved": b / (ms * 1e-3) / 1e9,
+                           "frac": b / (ms * 1e-3) / HBM_PEAK,
+                           "hh_plus_pull_frac": (b + hh_bytes) / ((ms + hh_ms) * 1e-3) / HBM_PEAK,
+                           "hh_plus_pull_neuron_steps_per_s": net.n / ((ms + hh_ms) * 1e-3)})
+    roof_pull = dict(pull_cases[1])
+    roof_pull.update({"kernel": "k_pull_lds<float,0> (dense pull over the CSC, spike bitmap in LDS, "
+                                "ascending-source sums)", "bound": "hbm", "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                      "alg_bytes_rule": "4(N+1) + 4E + 4E_s + N/8 + 4N", "cases": pull_cases})
     log(f"bench: pull coupling {roof_pull}")
 
     ro = heb = None

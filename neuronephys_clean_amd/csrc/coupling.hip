@@ -231,88 +231,133 @@ __global__ __launch_bounds__(256) void k_pull(int64_t n, int64_t nnz, int mode, 
     }
 }
 
-// Networks of up to kLdsBitsMax neurons: the same pull with the spike bitmap
-// copied into LDS first (128 KiB at 1,048,576 neurons), so that the bit test
-// of every in-edge is an LDS read instead of a random L2 transaction.  One
-// workgroup of 1024 threads per CU walks the targets; each lane still reads
-// four consecutive in-edges with one 16-byte load.
+// Networks of up to kLdsBitsMax neurons (and at least 8 in-edges): the same
+// sums, with the spike bitmap copied into LDS first (128 KiB at 1,048,576
+// neurons) so that the bit test of every in-edge is an LDS read instead of a
+// random L2 transaction.  One workgroup of 1024 threads per CU.  A wavefront
+// takes 64 consecutive targets, whose in-edges are one contiguous slab of
+// the CSC, and streams the slab in chunks of 256 in-edges (a 16-byte load
+// per lane).  Every load of the loop is unconditional - addresses past the
+// slab are clamped to its last whole 16-byte group, and a weight (or output)
+// load of a lane without a spiking edge reads entry 0 - so that the source
+// loads of the next three chunks and the value loads of the last two stay in
+// flight while the current chunk is tested and the chunk before last is
+// summed.  The sums run over the spiking in-edges in ascending
+// position, lanes in order and a lane's four edges in order, so each target
+// is summed in ascending source order exactly as in k_pull; target j's sum
+// is kept by lane j and the 64 sums are stored together.  Entries of the
+// last partial 16-byte group of the CSC (nnz not a multiple of four) are read
+// once at the start.
 constexpr int64_t kLdsBitsMax = 1 << 20;
 
-// this lane's four in-edges q..q+3 (entries past nnz read as -1)
-__device__ __forceinline__ void load4(const int32_t* __restrict__ in_src, int64_t nnz, int32_t q, int32_t e1,
-                                      int32_t (&s4)[4]) {
-    s4[0] = s4[1] = s4[2] = s4[3] = -1;
-    if (q < e1) {
-        if ((int64_t)q + 3 < nnz) {
-            const int4 v = *reinterpret_cast<const int4*>(in_src + q);
-            s4[0] = v.x, s4[1] = v.y, s4[2] = v.z, s4[3] = v.w;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if ((int64_t)q + j < nnz) s4[j] = in_src[q + j];
-        }
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(1024) void k_pull_lds(int64_t n, int64_t nnz, int mode,
-                                                   const int32_t* __restrict__ in_off,
+template <class T, int MODE>
+__global__ __launch_bounds__(1024) void k_pull_lds(int64_t n, int64_t nnz, const int32_t* __restrict__ in_off,
                                                    const int32_t* __restrict__ in_src,
                                                    const float* __restrict__ in_w,
                                                    const uint64_t* __restrict__ bits,
                                                    const T* __restrict__ outputs, T* __restrict__ g) {
+    // the value type each chunk loads: weights are float, outputs are T
+    using V = typename std::conditional<MODE == 3, T, float>::type;
     __shared__ uint32_t sb[kLdsBitsMax / 32];
-    const int64_t nw32 = (n + 31) >> 5;
-    const uint32_t* b32 = reinterpret_cast<const uint32_t*>(bits);
-    for (int64_t i = threadIdx.x; i < nw32; i += 1024) sb[i] = b32[i];
+    // the bitmap copy: eight 8-byte loads per thread in flight at once
+    const int64_t nw64 = (n + 63) >> 6;
+#pragma unroll 8
+    for (int64_t i = threadIdx.x; i < nw64; i += 1024) {
+        const uint64_t w = bits[i];
+        sb[2 * i] = (uint32_t)w, sb[2 * i + 1] = (uint32_t)(w >> 32);
+    }
+    const int32_t tail0 = (int32_t)(nnz & ~3ll), qmax = tail0 - 4;
+    const int32_t tv0 = tail0 < nnz ? in_src[tail0] : 0, tv1 = tail0 + 1 < nnz ? in_src[tail0 + 1] : 0,
+                  tv2 = tail0 + 2 < nnz ? in_src[tail0 + 2] : 0;
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = (int64_t)gridDim.x * 16;
-    int64_t t = blockIdx.x * 16ll + (threadIdx.x >> 6);
-    // software pipeline: the next target's range and first 256 in-edges are
-    // in flight while the current target is summed
-    int32_t e0 = t < n ? in_off[t] : 0, e1 = t < n ? in_off[t + 1] : 0;
-    int32_t s4[4];
-    load4(in_src, nnz, (e0 & ~3) + 4 * lane, e1, s4);
-    int64_t tn = t + nwaves;
-    int32_t ne0 = tn < n ? in_off[tn] : 0, ne1 = tn < n ? in_off[tn + 1] : 0;
-    for (; t < n; t += nwaves) {
-        int32_t s4n[4];
-        load4(in_src, nnz, (ne0 & ~3) + 4 * lane, ne1, s4n);
-        const int64_t tnn = tn + nwaves;
-        const int32_t nne0 = tnn < n ? in_off[tnn] : 0, nne1 = tnn < n ? in_off[tnn + 1] : 0;
-        T acc = T(0);
-        for (int32_t base = e0 & ~3; base < e1; base += 256) {
-            const int32_t q = base + 4 * lane;
-            if (base != (e0 & ~3)) load4(in_src, nnz, q, e1, s4);
-            int am = 0;
+    for (int64_t t0 = (blockIdx.x * 16ll + (threadIdx.x >> 6)) * 64; t0 < n; t0 += nwaves * 64) {
+        const int cnt = (int)(n - t0 < 64 ? n - t0 : 64);
+        // o1: end of lane j's target range (past the group: never reached)
+        const int32_t o1 = lane < cnt ? in_off[t0 + lane + 1] : INT32_MAX;
+        const int32_t E0 = in_off[t0], E1 = rdl(o1, cnt - 1), S = E0 & ~3;
+        const int32_t nch = E1 > S ? (E1 - S + 255) >> 8 : 0;
+        int32_t qlim = (E1 - 1) & ~3;
+        qlim = qlim < qmax ? qlim : qmax;
+        qlim = qlim > 0 ? qlim : 0;
+        const bool tailgrp = E1 > tail0;
+        auto load = [&](int32_t c) {
+            int32_t q = S + c * 256 + 4 * lane;
+            q = q < qlim ? q : qlim;
+            q = q > 0 ? q : 0;
+            return *reinterpret_cast<const int4*>(in_src + q);
+        };
+        // the loop starts at chunk -4 (no edges: its tests fail and its value
+        // loads read entry 0), so that it enters its steady state - three
+        // chunks of sources and two of values in flight - before chunk 0
+        int4 ring[4];
+        int msk[4];
+        V wv[4][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int32_t e = q + j;
-                if (e >= e0 && e < e1 && ((sb[s4[j] >> 5] >> (s4[j] & 31)) & 1u)) am |= 1 << j;
-            }
-            T v[4] = {T(0), T(0), T(0), T(0)};
-            if (am) {
+        for (int u = 0; u < 4; ++u) ring[u] = make_int4(0, 0, 0, 0), msk[u] = 0;
+        T acc = T(0), res = T(0);
+        int cur_t = 0;
+        int32_t cur_end = INT32_MIN;
+        for (int32_t c0 = -4; c0 < nch + 2; c0 += 4) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if ((am >> j) & 1) v[j] = mode == 0 ? T(in_w[q + j]) : mode == 2 ? T(1) : outputs[s4[j]];
-            }
-            uint64_t m = __ballot(am != 0);
-            while (m) {
-                const int l = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                const int ml = rdl(am, l);
+            for (int u = 0; u < 4; ++u) {
+                const int32_t c = c0 + u;
+                ring[(u + 3) & 3] = load(c + 3);
+                // chunk c: spike bits of its in-edges, then their values
+                const int32_t q = S + c * 256 + 4 * lane;
+                int32_t src[4] = {ring[u].x, ring[u].y, ring[u].z, ring[u].w};
+                if (tailgrp) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if ((ml >> j) & 1) acc = acc + rdl(v[j], l);
+                    for (int k = 0; k < 4; ++k) {
+                        const int32_t e = q + k;
+                        if (e >= tail0) src[k] = e == tail0 ? tv0 : e == tail0 + 1 ? tv1 : tv2;
+                    }
+                }
+                // every source read here is an entry of the CSC, so the LDS
+                // read is in range for the edges outside the slab too
+                unsigned am = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int32_t e = q + k;
+                    const unsigned in = (unsigned)(e >= E0) & (unsigned)(e < E1);
+                    am |= (in & (sb[src[k] >> 5] >> (src[k] & 31))) << k;
+                }
+                msk[u] = (int)am;
+                if (MODE != 2) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const bool f = (am >> k) & 1;
+                        if (MODE == 0) wv[u][k] = in_w[f ? q + k : 0];
+                        else wv[u][k] = outputs[f ? src[k] : 0];
+                    }
+                }
+                // chunk c - 2: ordered sums
+                const int ps = (u + 2) & 3;
+                const int32_t pq = S + (c - 2) * 256;
+                uint64_t m = __ballot(msk[ps] != 0);
+                while (m) {
+                    const int l = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const int ml = rdl(msk[ps], l);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if ((ml >> k) & 1) {
+                            const int32_t e = pq + 4 * l + k;
+                            if (e >= cur_end) {  // the first spiking in-edge of another target
+                                if (lane == cur_t) res = acc;
+                                acc = T(0);
+                                cur_t = __popcll(__ballot(o1 <= e));
+                                cur_end = rdl(o1, cur_t);
+                            }
+                            acc = acc + (MODE == 2 ? T(1) : T(rdl(wv[ps][k], l)));
+                        }
+                    }
+                }
             }
         }
-        if (lane == 0) g[t] = acc;
-        e0 = ne0, e1 = ne1;
-        ne0 = nne0, ne1 = nne1;
-        tn = tnn;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) s4[j] = s4n[j];
+        if (lane == cur_t) res = acc;
+        if (lane < cnt) g[t0 + lane] = res;
     }
 }
 
@@ -445,15 +490,22 @@ nnx_status couple_launch(nnx_ctx* c, const nnx_graph* g, int mode, int dtype, co
     const int m = mode == NNX_COUPLE_WEIGHTED_PULL ? 0 : mode == NNX_COUPLE_COUNT ? 2 : 3;
     const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 4), 65536);
     const bool vec = ((uintptr_t)g->in_sources & 15) == 0;
-    if (vec && n <= kLdsBitsMax) {
+    if (vec && n <= kLdsBitsMax && g->nnz >= 8 && g->nnz < (1ll << 31) - 4096) {
         // one 1024-thread workgroup per CU (the bitmap copy takes 128 KiB of LDS)
-        const unsigned nb = (unsigned)std::min<int64_t>(ceil_div(n, 16), 256);
-        if (dtype == NNX_F32)
-            k_pull_lds<float><<<nb, 1024, 0, c->stream>>>(n, g->nnz, m, g->in_offsets, g->in_sources, g->in_weights,
-                                                          bits, (const float*)outputs, (float*)g_in);
-        else
-            k_pull_lds<double><<<nb, 1024, 0, c->stream>>>(n, g->nnz, m, g->in_offsets, g->in_sources,
-                                                           g->in_weights, bits, (const double*)outputs, (double*)g_in);
+        const unsigned nb = (unsigned)std::min<int64_t>(ceil_div(n, 1024), 256);
+#define NNX_PULL_LDS(T, M)                                                                                  \
+    k_pull_lds<T, M><<<nb, 1024, 0, c->stream>>>(n, g->nnz, g->in_offsets, g->in_sources, g->in_weights, bits, \
+                                                 (const T*)outputs, (T*)g_in)
+        if (dtype == NNX_F32) {
+            if (m == 0) NNX_PULL_LDS(float, 0);
+            else if (m == 2) NNX_PULL_LDS(float, 2);
+            else NNX_PULL_LDS(float, 3);
+        } else {
+            if (m == 0) NNX_PULL_LDS(double, 0);
+            else if (m == 2) NNX_PULL_LDS(double, 2);
+            else NNX_PULL_LDS(double, 3);
+        }
+#undef NNX_PULL_LDS
         NNX_LAUNCHED(c);
         return NNX_OK;
     }

@@ -185,6 +185,57 @@ def test_couple_all_modes_bitexact(ctx, T, frac):
     assert np.array_equal(gi.numpy(), res["event"])
 
 
+def _skewed_graph(n, nnz, seed):
+    """nnz edges, targets drawn as n u^3 (u uniform) and made even in the
+    upper half: the low targets receive thousands of in-edges, the odd ones of
+    the upper half none."""
+    rng = np.random.default_rng(seed)
+    src = np.sort(rng.integers(0, n, nnz))
+    tgt = np.minimum((n * rng.random(nnz) ** 3).astype(np.int64), n - 1)
+    tgt = np.where(tgt > n // 2, tgt & ~1, tgt)
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.int32)
+    # ascending targets within each source's out-edges
+    order = np.lexsort((tgt, src))
+    tgt = tgt[order].astype(np.int32)
+    w = host.lognormal(seed, 9, nnz, -3, 1).astype(np.float32)
+    return configs.Network(n, off, tgt, w, *(np.zeros(n),) * 5, np.zeros((n, 3), np.float32))
+
+
+@pytest.mark.parametrize("n,nnz,shift", [(37, 9, 0), (4099, 150_001, 0), (4099, 150_002, 0), (4099, 150_003, 0),
+                                         (70_000, 3_000_000, 0), (4099, 150_003, 1), ((1 << 20) + 77, 2_000_001, 0)])
+def test_pull_ragged_graphs(ctx, n, nnz, shift):
+    """The dense pull on skewed in-degrees (hubs of thousands of in-edges,
+    empty targets), a CSC length of every residue mod 4, fewer than 64
+    targets, a CSC array that is not 16-byte aligned (shift 1) and a network
+    above the LDS bitmap's size: all modes identical to the oracle."""
+    say("pull ragged", n, nnz, shift)
+    net = _skewed_graph(n, nnz, n + nnz)
+    g, keep = upload_graph(ctx, net)
+    if shift:
+        in_src = net.ensure_csc()[1]
+        pad = DA.from_numpy(ctx, np.concatenate([np.zeros(shift, np.int32), in_src]))
+        keep["pad"] = pad
+        g.in_sources = pad.ptr + 4 * shift
+    indeg = np.diff(net.ensure_csc()[0])
+    assert indeg.max() > 600 and (indeg == 0).any() or n < 64
+    rng = np.random.default_rng(7)
+    for T in (np.float32, np.float64):
+        for frac in (0.0, 0.03, 0.5):
+            fire = (rng.random(n) < frac).astype(np.uint8)
+            out = rng.random(n).astype(T)
+            words = np.packbits(np.concatenate([fire, np.zeros(-n % 64, np.uint8)]),
+                                bitorder="little").view(np.uint64)
+            bits = DA.from_numpy(ctx, words)
+            dout = DA.from_numpy(ctx, out)
+            for mode in (0, 2, 3):
+                gi = DA.from_numpy(ctx, np.full(n, np.nan, T))
+                ctx.call("nnx_couple", C.byref(g), mode, DT[T], bits.ptr, None, None, dout.ptr, gi.ptr)
+                ref = oracle.couple(net.off, net.tgt, net.w, fire, mode, out=out, dtype=T)
+                assert np.array_equal(gi.numpy(), ref), (T, frac, mode)
+
+
 def test_graph_transpose_device_equals_host(ctx):
     rng = np.random.default_rng(5)
     n = 20000
