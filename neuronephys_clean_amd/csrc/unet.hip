@@ -88,11 +88,17 @@ struct ConvArgs {
     float* hout;             // [b][z][y][x] fp32 head output
 };
 
-template <int CIN> struct ConvTile { static constexpr int T = CIN == 32 ? 4 : 8; };
+// output tile 16 x TY x TZ.  TZ = 4 keeps the 16-channel layers at three
+// workgroups per CU (input halo 34.5 KiB + weights 14.3 KiB of LDS) and the
+// 2-channel layer at half the registers of TZ = 8; the kernel is bound by
+// each workgroup's staging -> MFMA -> epilogue latency, so occupancy pays.
+template <int CIN> struct ConvTile {
+    static constexpr int TY = CIN == 32 ? 4 : 8, TZ = 4;
+};
 
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
-    constexpr int TY = ConvTile<CIN>::T, TZ = TY;
+    constexpr int TY = ConvTile<CIN>::TY, TZ = ConvTile<CIN>::TZ;
     constexpr int KP = kpad(CIN), NKB = KP / 32, KS = KP + 8;
     constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2, HV = HX * HY * HZ;
     constexpr int MT = COUT / 16;
@@ -179,9 +185,12 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
     // Rows of this wave: quads of 2x2 (y,z) rows.  Row r sits at
     // (wy + ryc(r), wz + rzc(r)) in the tile, with ryc / rzc compile-time
     // constants, so its LDS address is a per-lane base plus an immediate.
-    const int wy = TY == 8 ? 0 : 2 * (wave & 1), wz = TY == 8 ? 2 * wave : 2 * (wave >> 1);
+    // a wave owns RY x 2 (y, z) rows; the four waves tile TY x TZ
+    constexpr int RY = RPW / 2, NYB = TY / RY;
+    static_assert(NYB * (TZ / 2) == 4, "four waves per tile");
+    const int wy = RY * (wave % NYB), wz = 2 * (wave / NYB);
     const int vbase = ((wz * HY + wy) * HX + xl) * CIN;
-    auto ryc = [](int r) { return (TY == 8 ? 2 * (r >> 2) : 0) + (r & 1); };
+    auto ryc = [](int r) { return 2 * (r >> 2) + (r & 1); };
     auto rzc = [](int r) { return (r >> 1) & 1; };
     auto toff = [&](int tap, int coff) {
         tap = tap > 26 ? 26 : tap;  // padded k: zero weights
@@ -868,12 +877,12 @@ namespace {
 
 template <int CIN, int COUT>
 nnx_status launch_conv(nnx_ctx* c, const ConvArgs& a) {
-    constexpr int T = ConvTile<CIN>::T;
-    NNX_CHECK(c, a.nx % 16 == 0 && a.ny % T == 0 && a.nz % T == 0 && a.batch >= 1,
-              "conv %d->%d: dims (%d,%d,%d) must be multiples of (16,%d,%d)", CIN, COUT, a.nx, a.ny, a.nz, T, T);
+    constexpr int TY = ConvTile<CIN>::TY, TZ = ConvTile<CIN>::TZ;
+    NNX_CHECK(c, a.nx % 16 == 0 && a.ny % TY == 0 && a.nz % TZ == 0 && a.batch >= 1,
+              "conv %d->%d: dims (%d,%d,%d) must be multiples of (16,%d,%d)", CIN, COUT, a.nx, a.ny, a.nz, TY, TZ);
     if (a.pool || a.coarse)
         NNX_CHECK(c, a.nx % 32 == 0 && a.ny % 2 == 0 && a.nz % 2 == 0, "pooled / upsampled conv needs even dims");
-    dim3 grid(a.nx / 16, a.ny / T, (a.nz / T) * a.batch);
+    dim3 grid(a.nx / 16, a.ny / TY, (a.nz / TZ) * a.batch);
     k_conv3<CIN, COUT><<<grid, 256, 0, c->stream>>>(a);
     NNX_LAUNCHED(c);
     return NNX_OK;
