@@ -111,17 +111,49 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
     const int nzt = nz / TZ;
     const int b = blockIdx.z / nzt, z0 = (blockIdx.z % nzt) * TZ, y0 = blockIdx.y * TY,
               x0 = blockIdx.x * 16;
-    // Staging: every load of a thread is issued before the first one is
-    // used, so a workgroup pays one memory latency, not one per chunk.
-    {
-        constexpr int NW = COUT * (KP / 8), WI = (NW + 255) / 256;
-        u32x4_t wv[WI];
+    // Staging: every load of a thread (weights, input halo, coarse input) is
+    // issued before the first one is used, so a workgroup pays one memory
+    // latency, not one per chunk.
+    constexpr int NW = COUT * (KP / 8), WI = (NW + 255) / 256;
+    u32x4_t wv[WI];
 #pragma unroll
-        for (int u = 0; u < WI; ++u) {
-            const int i = tid + 256 * u;
-            if (i < NW) {
-                const int co = i / (KP / 8), k8 = i - co * (KP / 8);
-                wv[u] = *(const u32x4_t*)&a.w[(size_t)co * KP + 8 * k8];
+    for (int u = 0; u < WI; ++u) {
+        const int i = min(tid + 256 * u, NW - 1);
+        const int co = i / (KP / 8), k8 = i - co * (KP / 8);
+        wv[u] = *(const u32x4_t*)&a.w[(size_t)co * KP + 8 * k8];
+    }
+    // The input halo.  Every load is unconditional (a voxel outside the grid
+    // reads the nearest inside one and is zeroed when stored): a load under a
+    // condition made the compiler wait for it at the end of the condition,
+    // one memory latency per chunk.
+    {
+        constexpr int CH = CIN == 2 ? 1 : CIN / 8, NCH = HV * CH, IT = (NCH + 255) / 256;
+        using SV = typename std::conditional<CIN == 2, uint32_t, u32x4_t>::type;
+        SV v[IT], cv[IT];
+        uint32_t inside = 0;
+        size_t fo[IT];
+        int coff[IT];
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            int i = tid + 256 * u;
+            i = i < NCH ? i : NCH - 1;
+            const int hv = i / CH, c8 = i - hv * CH;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz) inside |= 1u << u;
+            const int cx = min(max(gx, 0), nx - 1), cy = min(max(gy, 0), ny - 1), cz = min(max(gz, 0), nz - 1);
+            fo[u] = ((((size_t)b * nz + cz) * ny + cy) * nx + cx) * CIN + 8 * c8;
+            coff[u] = (((cz >> 1) * (ny >> 1) + (cy >> 1)) * (nx >> 1) + (cx >> 1)) * CIN + 8 * c8;
+            if constexpr (CIN == 2)
+                v[u] = *(const uint32_t*)&a.in[fo[u]];
+            else
+                v[u] = *(const u32x4_t*)&a.in[fo[u]];
+        }
+        if constexpr (CIN != 2) {
+            if (a.coarse) {
+                const size_t cb = (size_t)b * (nz >> 1) * (ny >> 1) * (nx >> 1) * CIN;
+#pragma unroll
+                for (int u = 0; u < IT; ++u) cv[u] = *(const u32x4_t*)&a.coarse[cb + coff[u]];
             }
         }
 #pragma unroll
@@ -132,52 +164,17 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
                 *(u32x4_t*)&s_w[co * KS + 8 * k8] = wv[u];
             }
         }
-    }
-    if (CIN == 2) {
-        constexpr int IT = (HV + 255) / 256;
-        uint32_t v[IT];
 #pragma unroll
         for (int u = 0; u < IT; ++u) {
-            const int hv = tid + 256 * u;
-            v[u] = 0;
-            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-            if (hv < HV && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
-                v[u] = *(const uint32_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * 2];
-        }
-#pragma unroll
-        for (int u = 0; u < IT; ++u) {
-            const int hv = tid + 256 * u;
-            if (hv < HV) *(uint32_t*)&s_in[hv * 2] = v[u];
-        }
-    } else {
-        constexpr int CH = CIN / 8, NCH = HV * CH, IT = (NCH + 255) / 256, GS = 8;
-#pragma unroll
-        for (int u0 = 0; u0 < IT; u0 += GS) {
-            u32x4_t v[GS], cv[GS];
-#pragma unroll
-            for (int u = 0; u < GS; ++u) {
-                const int i = tid + 256 * (u0 + u);
-                v[u] = u32x4_t{0u, 0u, 0u, 0u};
-                cv[u] = u32x4_t{0u, 0u, 0u, 0u};
-                const int hv = i / CH, c8 = i - hv * CH;
-                const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-                const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-                if (u0 + u < IT && i < NCH && gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 &&
-                    gz < nz) {
-                    v[u] = *(const u32x4_t*)&a.in[((((size_t)b * nz + gz) * ny + gy) * nx + gx) * CIN + 8 * c8];
-                    if (a.coarse) {
-                        const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
-                        cv[u] = *(const u32x4_t*)&a.coarse
-                            [((((size_t)b * cz + (gz >> 1)) * cy + (gy >> 1)) * cx + (gx >> 1)) * CIN + 8 * c8];
-                    }
+            const int i = tid + 256 * u;
+            if (i < NCH) {
+                const bool in = (inside >> u) & 1u;
+                if constexpr (CIN == 2) {
+                    *(uint32_t*)&s_in[i * 2] = in ? v[u] : 0u;
+                } else {
+                    const u32x4_t z = u32x4_t{0u, 0u, 0u, 0u};
+                    *(u32x4_t*)&s_in[i * 8] = !in ? z : a.coarse ? add_scaled8(v[u], cv[u], a.cscale) : v[u];
                 }
-            }
-#pragma unroll
-            for (int u = 0; u < GS; ++u) {
-                const int i = tid + 256 * (u0 + u);
-                if (u0 + u < IT && i < NCH)
-                    *(u32x4_t*)&s_in[i * 8] = a.coarse ? add_scaled8(v[u], cv[u], a.cscale) : v[u];
             }
         }
     }
