@@ -203,13 +203,15 @@ def _skewed_graph(n, nnz, seed):
     return configs.Network(n, off, tgt, w, *(np.zeros(n),) * 5, np.zeros((n, 3), np.float32))
 
 
-@pytest.mark.parametrize("n,nnz,shift", [(37, 9, 0), (4099, 150_001, 0), (4099, 150_002, 0), (4099, 150_003, 0),
-                                         (70_000, 3_000_000, 0), (4099, 150_003, 1), ((1 << 20) + 77, 2_000_001, 0)])
+@pytest.mark.parametrize("n,nnz,shift", [(20, 5, 0), (37, 9, 0), (4099, 150_001, 0), (4099, 150_002, 0),
+                                         (4099, 150_003, 0), (70_000, 3_000_000, 0), (4099, 150_003, 1),
+                                         (1 << 20, 2_000_003, 0), ((1 << 20) + 77, 2_000_001, 0)])
 def test_pull_ragged_graphs(ctx, n, nnz, shift):
     """The dense pull on skewed in-degrees (hubs of thousands of in-edges,
     empty targets), a CSC length of every residue mod 4, fewer than 64
-    targets, a CSC array that is not 16-byte aligned (shift 1) and a network
-    above the LDS bitmap's size: all modes identical to the oracle."""
+    targets, fewer than 8 in-edges, a CSC array that is not 16-byte aligned
+    (shift 1), a network of exactly the LDS bitmap's size and one above it:
+    all modes identical to the oracle."""
     say("pull ragged", n, nnz, shift)
     net = _skewed_graph(n, nnz, n + nnz)
     g, keep = upload_graph(ctx, net)
