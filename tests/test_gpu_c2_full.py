@@ -73,3 +73,38 @@ def test_c2_full_scale_pull_and_event_1000_steps(ctx):
             assert np.array_equal(st[k], a), (couple, k)
         assert np.array_equal(sim.a["voxel_counts"].numpy(), rvc), couple
         del sim
+
+
+def test_c2_full_scale_fp64_equals_oracle_and_fp32_tolerance(ctx):
+    """Precision split at full scale: the fp64 device run of the first 100
+    steps equals the FP64 oracle bit for bit (raster and state), and the fp32
+    run stays within the stated tolerance of it: |dV| <= 1e-4 mV after one
+    step and <= 1e-3 mV after 100, gating variables within 1e-5."""
+    net = configs.c2()
+    steps, cap = 100, 4_000_000
+    dev = {}
+    for T in (np.float64, np.float32):
+        sim = Simulation(net, T, couple="event", record_cap=cap, ctx=ctx)
+        sim.run(1)
+        sim.synchronize()
+        v1 = sim.state()["v"].astype(np.float64)
+        sim.run(steps - 1, use_graph=True, chunk=99)
+        sim.synchronize()
+        dev[T] = (v1, {k: a.astype(np.float64) for k, a in sim.state().items() if k in ("v", "m", "h", "ng")},
+                  sim.raster())
+        del sim
+    T = np.float64
+    V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
+    s = np.zeros(net.n, T)
+    p = oracle.params(_lib.params_squid().as_dict())
+    ref, rln, _, sec = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
+                                      np.asarray(net.iext, T), cap=cap)
+    ras, ln = dev[np.float64][2]
+    assert ln == rln and np.array_equal(ras, ref)
+    for k, a in dict(v=V, m=m, h=h, ng=ng).items():
+        assert np.array_equal(dev[np.float64][1][k], a), k
+    d1 = np.abs(dev[np.float32][0] - dev[np.float64][0]).max()
+    d = {k: np.abs(dev[np.float32][1][k] - dev[np.float64][1][k]).max() for k in ("v", "m", "h", "ng")}
+    say("fp32 vs fp64: max|dV| after 1 step", d1, "after 100:", d)
+    assert d1 <= 1e-4 and d["v"] <= 1e-3
+    assert max(d["m"], d["h"], d["ng"]) <= 1e-5
