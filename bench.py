@@ -609,7 +609,7 @@ def tune_fused(ctx, sim, chunk):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=10000)  # the configuration's 100 ms
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--n", type=int, default=1 << 20)
