@@ -77,37 +77,38 @@ def cpu_baseline(net, steps):
                       f"OpenMP over neurons, {sec:.2f} s"}
 
 
-def cg_bench(ctx, grid_n, b_host, rtol=1e-8):
+def cg_bench(ctx, grid_n, b_host, rtol=1e-8, c=1.0):
+    """The configuration's paracrine solve on its own: 256^3, dt D / h^2 = 1,
+    b = the fixed release field; Jacobi-CG with FP64 refinement to a true
+    (FP64) relative residual of 1e-8 - the headline CG milliseconds - median
+    of 3; beside it the fp32 CG to a recursive 1e-8 and its true residual."""
     from neuronephys_clean_amd import _lib
     from neuronephys_clean_amd.devmem import DeviceArray as DA
     nvox = grid_n ** 3
-    g = _lib.Grid(grid_n, grid_n, grid_n, 7, 1.0, 0.0, None, None, None)
-    db = DA.from_numpy(ctx, b_host.astype(np.float32))
-    dx = DA.zeros(ctx, nvox, np.float32)
-    o = _lib.CgOpts(rtol=rtol, maxit=2000, precond=_lib.PRECOND_JACOBI, check_every=8)
-    r = _lib.CgResult()
-    ctx._l.nnx_cg_solve(ctx.h, C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(r))  # warm
-    runs = []
-    for _ in range(3):
-        r = _lib.CgResult()
-        st = ctx._l.nnx_cg_solve(ctx.h, C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(r))
-        runs.append((r.ms, r.iters, r.relres, r.true_relres, st))
-    runs.sort()
-    ms, it, rel, true_rel, st = runs[1]
-    # mixed precision: FP64 outer residual to a true 1e-8
-    db64 = DA.from_numpy(ctx, b_host.astype(np.float64))
+    g = _lib.Grid(grid_n, grid_n, grid_n, 7, c, 0.0, None, None, None)
+    o = _lib.CgOpts(rtol=rtol, maxit=4000, precond=_lib.PRECOND_JACOBI, check_every=8)
+    db64 = DA.from_numpy(ctx, np.asarray(b_host, np.float64))
     dx64 = DA.zeros(ctx, nvox, np.float64)
-    r2 = _lib.CgResult()
-    ctx._l.nnx_cg_solve_refine(ctx.h, C.byref(g), db64.ptr, dx64.ptr, C.byref(o), 1e-6, 20,
-                               C.byref(r2))
-    # dominant CG kernels: per-iteration time from the solve
-    return {"grid": f"{grid_n}^3", "c": 1.0, "rtol": rtol, "precond": "jacobi",
-            "plain_ms": ms, "iters": it, "relres": rel, "true_relres_fp64": true_rel,
-            "ms_per_iter": ms / max(it, 1),
-            "bytes_per_iter_alg": 44 * nvox,
-            "refined_ms": r2.ms, "refined_outer": r2.outer, "refined_inner_iters": r2.iters,
-            "refined_relres_fp64": r2.relres, "learned_pc_ms": None,
-            "status": int(st)}
+    runs = []
+    for _ in range(4):
+        r2 = _lib.CgResult()
+        ctx.call("nnx_cg_solve_refine", C.byref(g), db64.ptr, dx64.ptr, C.byref(o), 1e-6, 20, C.byref(r2))
+        runs.append((r2.ms, r2.iters, r2.outer, r2.relres, r2.converged))
+    runs = sorted(runs[1:])
+    ms, it, outer, rel, conv = runs[1]
+    db = DA.from_numpy(ctx, np.asarray(b_host, np.float32))
+    dx = DA.zeros(ctx, nvox, np.float32)
+    r = _lib.CgResult()
+    st = ctx._l.nnx_cg_solve(ctx.h, C.byref(g), db.ptr, dx.ptr, C.byref(o), C.byref(r))
+    return {"grid": f"{grid_n}^3", "c": c, "rtol": rtol, "precond": "jacobi",
+            "rhs": f"0.01 x the per-voxel spike counts of the first {FIXED_RELEASE_STEPS} C2 steps",
+            "plain_ms": ms, "plain_iters": it, "plain_outer": outer, "plain_true_relres_fp64": rel,
+            "plain_converged": bool(conv),
+            "method": "Jacobi-CG, fp32 vectors, FP64 dots, FP64 iterative refinement (inner 1e-6)",
+            "fp32_only": {"ms": r.ms, "iters": r.iters, "relres": r.relres, "true_relres_fp64": r.true_relres,
+                          "status": int(st)},
+            "learned_pc_ms": None,
+            "bytes_per_iter_alg": 40 * nvox}
 
 
 def c3_bench(ctx):
@@ -135,16 +136,37 @@ def c3_bench(ctx):
             "bytes_per_inner_iter_alg": 44 * n ** 3}
 
 
+def raster_agreement(r32, r64, n):
+    """fp32 raster against the FP64 one: the fraction of (step, neuron)
+    pairs they share, the fraction of fp64 spikes the fp32 run has within
+    one step, and the first step where they differ."""
+    k32 = r32[:, 0].astype(np.int64) * n + r32[:, 1]
+    k64 = r64[:, 0].astype(np.int64) * n + r64[:, 1]
+    both = np.intersect1d(k32, k64, assume_unique=True)
+    near = np.isin(k64, k32) | np.isin(k64 - n, k32) | np.isin(k64 + n, k32)
+    diff = np.setxor1d(k32, k64, assume_unique=True)
+    first = None
+    if diff.size:
+        d0 = int(diff.min())
+        first = {"step": d0 // n, "neuron": d0 % n, "in_fp32": bool(np.isin(d0, k32)),
+                 "in_fp64": bool(np.isin(d0, k64))}
+    return {"spikes_fp32": int(k32.size), "spikes_fp64": int(k64.size),
+            "match_fraction": both.size / max(k32.size, k64.size, 1),
+            "within_one_step_fraction": float(near.mean()) if k64.size else 1.0,
+            "first_divergence": first}
+
+
 def precision_split(ctx, net, steps=(1, 100, 10000)):
     """max |V_fp32 - V_fp64| of the same network after 1, 100 and 10,000
-    steps.  The fp64 device run is checked bit for bit against the FP64
-    oracle over the first 100 steps; later steps use it as the FP64
-    trajectory."""
+    steps, and the fp32 raster against the fp64 one over the whole run.
+    The fp64 device run is checked bit for bit against the FP64 oracle over
+    the first 100 steps; later steps use it as the FP64 trajectory."""
     import oracle
     from neuronephys_clean_amd import _lib
     from neuronephys_clean_amd.sim import Simulation
-    s32 = Simulation(net, np.float32, couple="event", ctx=ctx, device_csc=True)
-    s64 = Simulation(net, np.float64, couple="event", ctx=ctx, device_csc=True)
+    cap = 1 << 23
+    s32 = Simulation(net, np.float32, couple="event", ctx=ctx, device_csc=True, record_cap=cap)
+    s64 = Simulation(net, np.float64, couple="event", ctx=ctx, device_csc=True, record_cap=cap)
     out, done = {}, 0
     v64_100 = None
     for k in steps:
@@ -156,6 +178,9 @@ def precision_split(ctx, net, steps=(1, 100, 10000)):
         if k == 100:
             v64_100 = v64
     res = {"max_abs_dV_mV": out}
+    (r32, l32), (r64, l64) = s32.raster(), s64.raster()
+    if l32 <= cap and l64 <= cap:
+        res["raster_fp32_vs_fp64"] = raster_agreement(r32, r64, net.n)
     if v64_100 is not None:
         T = np.float64
         V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
@@ -238,8 +263,6 @@ def c5_bench(ctx, counts, smooth=2):
     g, keep = device_field(ctx, f)
     log(f"bench: C5 field built in {time.time() - t0:.1f}s")
     b = 0.01 * counts.astype(np.float64)
-    if not b.any():
-        b[len(b) // 2] = 1.0
     db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, n ** 3, np.float64)
     net = UNet.load(ctx)
     net_mg = UNet.load(ctx, MG_WEIGHTS) if os.path.exists(MG_WEIGHTS) else None
@@ -313,6 +336,16 @@ def profile_pass(ctx, sim, net):
     from neuronephys_clean_amd.devmem import DeviceArray as DA
     from neuronephys_clean_amd.precond import UNet
     from neuronephys_clean_amd.sim import EnsembleSim
+    # 1,000 steps in graphs, then one step at step 1,000 with the in-step
+    # paracrine solve on the releases of those steps (eager dispatches)
+    every = sim.plan.paracrine.every
+    sim.plan.paracrine.every = 0
+    sim.run(1000, use_graph=True, chunk=100)
+    sim.plan.paracrine.every = every
+    sim.run(1)
+    sim.synchronize()
+    log(f"profile: in-step solve {sim.paracrine_stats()}")
+    sim.plan.paracrine.every = 0
     sim.run(20)  # eager: one dispatch per kernel per step
     sim.synchronize()
     if sim.plan.couple_mode == _lib.COUPLE_WEIGHTED_FUSED:
@@ -389,7 +422,7 @@ def profile_pass(ctx, sim, net):
     src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
-    log("profile pass done: 20 fused + 20 unfused steps, 20 HH launches, 20 dense pulls, 20 electrode readouts, "
+    log("profile pass done: 1,000 graph steps, 1 step with the in-step solve, 20 fused + 20 unfused steps, 20 HH launches, 20 dense pulls, 20 electrode readouts, "
         "21 Diekman and SAN "
         "steps, 20 Hebbian updates, 20 ensemble steps, "
         "5 network applications, "
@@ -412,14 +445,50 @@ def profiled_traffic(label):
 SNAP_KEYS = ("v", "m", "h", "ng", "s", "bits0", "bits1", "list", "count", "voxel_counts", "raster_len")
 
 
-def snapshot(ctx, sim):
-    """Device copy of everything a fused C2 run reads between calls."""
+def snapshot(ctx, sim, extra=()):
+    """Device copy of everything a fused C2 run reads between calls (and
+    the `extra` arrays, e.g. the paracrine field and its statistics)."""
     from neuronephys_clean_amd.devmem import DeviceArray
     snap = {}
-    for kk in SNAP_KEYS:
+    for kk in SNAP_KEYS + tuple(extra):
         snap[kk] = DeviceArray(ctx, sim.a[kk].shape, sim.a[kk].dtype)
         snap[kk].copy_from(sim.a[kk])
     return snap, sim.step
+
+
+def restore_snapshot(sim, snap):
+    snapd, step0 = snap
+    for kk, arr in snapd.items():
+        sim.a[kk].copy_from(arr)
+    sim.step = step0
+
+
+def cg_iteration_roofline(ctx, n, b32, c, k1=10, k2=40):
+    """One Jacobi-CG iteration on the n^3 constant 7-point grid (the fused
+    direction + stencil kernel and the update kernel): the difference of two
+    capturable solves of k2 and k1 iterations with rtol 0 (every iteration
+    works), HIP events on the library stream.  Algorithmic bytes per
+    iteration: 40 per voxel (read r, p_old; write p, q; read x, p, r, q;
+    write x, r)."""
+    from neuronephys_clean_amd import _lib
+    from neuronephys_clean_amd.devmem import DeviceArray as DA
+    g = _lib.Grid(n, n, n, 7, c, 0.0, None, None, None)
+    db, dx = DA.from_numpy(ctx, b32), DA.zeros(ctx, n ** 3, np.float32)
+    res = DA.zeros(ctx, 4, np.float64)
+
+    def solve(k):
+        o = _lib.CgOpts(rtol=0.0, maxit=k, precond=_lib.PRECOND_JACOBI, check_every=1)
+        ctx.call("nnx_cg_solve_async", C.byref(g), db.ptr, dx.ptr, C.byref(o), res.ptr)
+    t1 = timed_events(ctx, lambda: solve(k1), 5)
+    t2 = timed_events(ctx, lambda: solve(k2), 5)
+    us = (t2 - t1) * 1e3 / (k2 - k1)
+    b = 40.0 * n ** 3
+    return {"kernel": "k_stencil7x4<true> + k_cg_update (one Jacobi-CG iteration, 256^3 constant 7-point)",
+            "bound": "hbm", "achieved": b / (us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": b / (us * 1e-6) / HBM_PEAK, "traffic": profiled_traffic("cg-iteration"),
+            "avg_iteration_us": us, "alg_bytes_per_launch": b,
+            "alg_bytes_rule": "40 B per voxel per iteration (two launches)",
+            "time_source": f"HIP events: ({k2}-iteration solve - {k1}-iteration solve) / {k2 - k1}"}
 
 
 def model_bench(ctx, n, reps=100):
@@ -606,24 +675,92 @@ def tune_fused(ctx, sim, chunk):
     return best
 
 
+C2_STEPS = 10000      # the configuration's 100 ms at dt 0.01 ms
+FIXED_RELEASE_STEPS = 1000
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_full(net, every, b_fixed, all_steps=300, one_steps=20, reps=3):
+    """The oracle (fp32 restatement of the same step: RK4, push coupling in
+    ascending source order, release counting) on the host cores, on a
+    bounded sample of the same workload: the first `all_steps` steps on all
+    threads and the first `one_steps` on one thread, median of `reps` runs
+    each, plus one FP64-refined Jacobi-CG solve of the 256^3 grid to 1e-8 on
+    the fixed release field (all threads).  The loop rate is
+    N / (t_step + t_solve / every)."""
+    import oracle
+    from neuronephys_clean_amd import _lib
+    T = np.float32
+    p = oracle.params(_lib.params_squid().as_dict())
+    nthreads = oracle.set_threads(0)
+
+    def step_time(steps):
+        ts = []
+        for _ in range(reps):
+            V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
+            s = np.zeros(net.n, T)
+            _, _, _, sec = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
+                                          np.asarray(net.iext, T), cap=1, voxel_of=net.ensure_voxel(),
+                                          nvox=int(np.prod(net.grid)))
+            ts.append(sec / steps)
+        return float(np.median(ts))
+
+    t_all = step_time(all_steps)
+    oracle.set_threads(1)
+    t_one = step_time(one_steps)
+    oracle.set_threads(0)
+    g = oracle.grid(*net.grid, 7, net.c)
+    t0 = time.perf_counter()
+    x, outer = oracle.refine(g, b_fixed, 1e-8, 1e-6)
+    t_solve = time.perf_counter() - t0
+    per_step = t_all + (t_solve / every if every > 0 else 0.0)
+    return {"value": net.n / per_step, "unit": "neuron-steps/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle fp32 step (RK4 + push coupling in ascending source order + release counting) over "
+                      f"the first {all_steps} steps of the same network and initial state, median of {reps} runs "
+                      f"on {nthreads} threads ({t_all * 1e3:.2f} ms/step), plus one FP64-refined Jacobi-CG solve of "
+                      f"the 256^3 grid to 1e-8 on the fixed release field ({t_solve:.2f} s, {outer} outer steps) "
+                      f"every {every} steps",
+            "one_core": {"value": net.n / (t_one + (t_solve * nthreads / every if every > 0 else 0.0)),
+                         "step_ms": t_one * 1e3, "steps": one_steps,
+                         "note": "first steps on 1 thread, median of 3; the solve share scaled from the "
+                                 "all-thread time by the thread count"},
+            "all_threads_step_ms": t_all * 1e3, "solve_s": t_solve, "cpu_model": cpu_model(),
+            "nproc": os.cpu_count()}
+
+
+SNAP_PARA = ("conc", "para_stats")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)  # the configuration's 100 ms
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=C2_STEPS,
+                    help="requested timed steps; the timed window is always at least the configuration's "
+                         "10,000 steps from step 0")
+    ap.add_argument("--warmup", type=int, default=200,
+                    help="untimed steps from step 0 before the timed window (at least two graph chunks); the "
+                         "state is restored to step 0 afterwards")
     ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--para-every", type=int, default=100,
+                    help="steps between paracrine solves inside the step (the reference's timescale)")
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--couple", default="fused", choices=["fused", "event", "pull"])
-    ap.add_argument("--cpu-steps", type=int, default=1500)
     ap.add_argument("--tune", action="store_true",
                     help="pick the fused step's launch shape by timing 21 shapes on real steps before the timed "
-                         "region (the timed window then starts later in the simulation)")
-    ap.add_argument("--mode", default="bench", choices=["bench", "profile"],
-                    help="profile: a short eager sequence for rocprofv3 --pmc passes (20 fused and 20 "
-                         "unfused C2 steps, 20 HH launches, 20 electrode readouts, 21 Diekman and 21 SAN steps, "
-                         "20 Hebbian updates on 20 more C2 steps, 20 ensemble steps, 5 network applications, "
-                         "20 stencil applies, 3 V-cycles and one CG solve on 256^3 grids, one 256 MiB "
-                         "device copy to calibrate FETCH_SIZE/WRITE_SIZE); prints no JSON")
+                         "region")
+    ap.add_argument("--mode", default="bench", choices=["bench", "profile", "loop"],
+                    help="profile: a short eager sequence for rocprofv3 --pmc passes; loop: only the timed C2 "
+                         "windows (for a kernel trace); neither prints the JSON line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
@@ -635,8 +772,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(multi-GPU sharding is design only)"}))
         sys.exit(2)
-    if args.chunk < 1 or args.steps % args.chunk or args.warmup % args.chunk:
-        args.chunk = 1 if args.steps % 2 or args.warmup % 2 else 2
+    chunk = max(1, args.chunk)
+    steps = max(args.steps, C2_STEPS)
+    steps = (steps + chunk - 1) // chunk * chunk
+    warm = max(args.warmup, 2 * chunk)
+    warm = (warm + chunk - 1) // chunk * chunk
 
     from neuronephys_clean_amd import _lib, configs
     from neuronephys_clean_amd.sim import Simulation
@@ -648,6 +788,8 @@ def main():
     t0 = time.time()
     sim = Simulation(net, np.float32, couple=args.couple, record_cap=1 << 24, deposit=True,
                      ctx=ctx, device_csc=True)
+    every = args.para_every
+    sim.set_paracrine(max(every, 1), rtol=1e-8, inner_rtol=1e-6, max_inner=40, max_outer=2)
     sim.synchronize()
     log(f"bench: uploaded + device CSC in {time.time() - t0:.1f}s")
 
@@ -655,53 +797,112 @@ def main():
         profile_pass(ctx, sim, net)
         return
 
-    sim.run(args.warmup, use_graph=True, chunk=args.chunk)
-    sim.synchronize()
-    log(f"bench: warmup {args.warmup} steps done, {sim.raster()[1]} spikes")
     tune = None
     if args.couple == "fused":
         if args.tune:
-            tune = tune_fused(ctx, sim, args.chunk)
+            tune = tune_fused(ctx, sim, chunk)
+            sim.step = 0
         else:  # the library's default shape (one 256-neuron tile per workgroup, 4096 workgroups)
             tune = {"block": 256, "tile": 256, "grid": 4096, "tune_steps": 0}
         sim.set_fused(tune["block"], tune["tile"], tune["grid"])
-        log(f"bench: fused launch shape {tune}")
-    len0 = sim.raster()[1]
-    snap = snapshot(ctx, sim) if args.couple == "fused" and not args.no_readout else None
-    e0, e1 = C.c_void_p(), C.c_void_p()
-    ctx.call("nnx_event_create", C.byref(e0))
-    ctx.call("nnx_event_create", C.byref(e1))
-    sim.synchronize()
-    wall0 = time.perf_counter()
-    ctx.call("nnx_event_record", e0)
-    sim.run(args.steps, use_graph=True, chunk=args.chunk)
-    ctx.call("nnx_event_record", e1)
-    sim.synchronize()
-    wall = time.perf_counter() - wall0
-    ev_ms = C.c_float()
-    ctx.call("nnx_event_elapsed", e0, e1, C.byref(ev_ms))
-    spikes = sim.raster()[1] - len0
-    ms_step = wall * 1e3 / args.steps
-    value = net.n * args.steps / wall
-    log(f"bench: {args.steps} steps in {wall * 1e3:.1f} ms (events {ev_ms.value:.1f} ms), "
-        f"{spikes} spikes, {value:.3e} neuron-steps/s")
+    snap = snapshot(ctx, sim, extra=SNAP_PARA)
 
-    # the fused step kernel is the only per-step launch: its average duration
-    # is the timed region's event time over the steps (an upper bound: the
-    # per-call list push and compaction are included)
+    def restore():
+        restore_snapshot(sim, snap)
+
+    def para(on):
+        sim.plan.paracrine.every = every if (on and every > 0) else 0
+
+    # a fixed release field: the per-voxel spike counts of the first 1,000
+    # steps (the right-hand side of the standalone CG and C5 solves)
+    para(False)
+    sim.run(FIXED_RELEASE_STEPS, use_graph=True, chunk=chunk)
+    counts_fixed = sim.a["voxel_counts"].numpy().astype(np.float64)
+    b_fixed = net.amount * counts_fixed
+    log(f"bench: fixed release field: {int(counts_fixed.sum())} releases in {FIXED_RELEASE_STEPS} steps")
+
+    def window(on):
+        """The timed window from step 0: `steps` steps in graphs of `chunk`
+        (a solve at the first step of every chunk when on), warmed up on
+        the same graphs first; returns wall and event milliseconds."""
+        para(on)
+        restore()
+        sim.run(warm, use_graph=True, chunk=chunk)
+        restore()
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        ctx.call("nnx_event_create", C.byref(e0))
+        ctx.call("nnx_event_create", C.byref(e1))
+        sim.synchronize()
+        w0 = time.perf_counter()
+        ctx.call("nnx_event_record", e0)
+        sim.run(steps, use_graph=True, chunk=chunk)
+        ctx.call("nnx_event_record", e1)
+        sim.synchronize()
+        wall = time.perf_counter() - w0
+        ev = C.c_float()
+        ctx.call("nnx_event_elapsed", e0, e1, C.byref(ev))
+        ctx.call("nnx_event_destroy", e0)
+        ctx.call("nnx_event_destroy", e1)
+        return wall * 1e3, ev.value
+
+    if args.mode == "loop":
+        for on in (True, False):
+            wall_ms, ev_ms = window(on)
+            log(f"bench loop: paracrine={'on' if on else 'off'} {steps} steps {wall_ms:.1f} ms")
+        return
+
+    # the headline: HH + coupling + release + paracrine solve every `every` steps
+    wall_ms, ev_ms = window(True)
+    spikes = sim.raster()[1]
+    pst = sim.paracrine_stats()
+    ras_full = sim.raster()[0]
+    conc_sum = float(sim.a["conc"].numpy().sum())
+    value = net.n * steps / (wall_ms * 1e-3)
+    ms_step = wall_ms / steps
+    log(f"bench: {steps} steps with paracrine every {every}: {wall_ms:.1f} ms (events {ev_ms:.1f}), {spikes} spikes, "
+        f"{value:.3e} neuron-steps/s, solves {pst}")
+    # the same window without the solves
+    wall0_ms, ev0_ms = window(False)
+    same_raster = bool(np.array_equal(sim.raster()[0], ras_full))
+    steps_only = net.n * steps / (wall0_ms * 1e-3)
+    log(f"bench: steps only {wall0_ms:.1f} ms (events {ev0_ms:.1f}), {steps_only:.3e} neuron-steps/s")
+    n_solves = int(pst["solves"])
+    paracrine = {"every": every, "solves": n_solves, "grid": f"{net.grid[0]}^3", "c": net.c,
+                 "method": "Jacobi-CG (fp32 inner, FP64 dots) with FP64 iterative refinement, warm start from the "
+                           "previous field, captured in the step graphs (40 inner iterations x 2 refinement "
+                           "steps enqueued per solve, kernels after convergence return at once)",
+                 "rtol_true_fp64": 1e-8, "max_relres": pst["max_relres"], "unconverged": int(pst["unconverged"]),
+                 "inner_iters_per_solve": pst["inner_iters"] / max(n_solves, 1),
+                 "outer_per_solve": pst["outer_steps"] / max(n_solves, 1),
+                 "ms_per_solve": (ev_ms - ev0_ms) / max(n_solves, 1),
+                 "share_of_window": (ev_ms - ev0_ms) / ev_ms, "field_sum": conc_sum,
+                 "rasters_unchanged_by_solves": same_raster}
+
+    # kernel rooflines: the fused step (the steps-only window over its steps:
+    # the step kernel plus its share of one push and one compaction per chunk)
     fused_roof = None
     if args.couple == "fused":
-        ev_per_step = spikes / args.steps * net.nnz / net.n
+        ev_per_step = spikes / steps * net.nnz / net.n
         fb = 48.125 * net.n + 32.0 * ev_per_step
-        us = ev_ms.value * 1e3 / args.steps
+        us = ev0_ms * 1e3 / steps
         fused_roof = {"kernel": "k_ens_step<%d> (fused gather + RK4 + push)" % tune["block"], "bound": "hbm",
                       "achieved": fb / (us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                       "frac": fb / (us * 1e-6) / HBM_PEAK, "traffic": profiled_traffic("fused-step"),
                       "traffic_source": "profiles/hbm_traffic_profile_mode.md (FETCH_SIZE + WRITE_SIZE, "
                                         "calibrated on a 256 MiB copy, quiet-step median)",
-                      "avg_launch_us": us,
+                      "avg_step_us": us,
+                      "time_source": "HIP events around the steps-only window on the library stream / steps "
+                                     "(the step kernel plus its share of one push and one compaction per "
+                                     "100-step graph)",
                       "alg_bytes_per_launch": fb,
                       "alg_bytes_rule": "48.125 B per neuron + 32 B per delivered event"}
+    cg_iter = cg_iteration_roofline(ctx, net.grid[0], b_fixed.astype(np.float32), net.c)
+    log(f"bench: CG iteration {cg_iter}")
+    t_cg_iters = cg_iter["avg_iteration_us"] * 1e-3 * pst["inner_iters"]
+    t_steps = ev0_ms
+    roof = fused_roof if (fused_roof and t_steps >= t_cg_iters) else cg_iter
+    roof = dict(roof)
+    roof["share_of_timed_window"] = (t_steps if roof is fused_roof else t_cg_iters) / ev_ms
 
     # the HH RK4 kernel of the unfused path, timed alone on the library
     # stream over the same state
@@ -718,16 +919,8 @@ def main():
     roof_hh = {"kernel": "k_hh_rk4<float>", "bound": "hbm", "achieved": hh_bytes / (hh_ms * 1e-3) / 1e9,
                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": hh_bytes / (hh_ms * 1e-3) / HBM_PEAK,
                "traffic": None, "avg_launch_us": hh_ms * 1e3, "alg_bytes_per_launch": hh_bytes}
-    roof = fused_roof or roof_hh
     models = model_bench(ctx, net.n)
     log(f"bench: models {models}")
-
-    # per-step share of the coupling: eager per-kernel timing of one step's
-    # coupling on the current spikes
-    def couple():
-        ctx.call("nnx_couple", C.byref(sim.graph), 0 if args.couple == "pull" else 1, _lib.F32,
-                 sim.a["bits1"].ptr, sim.a["list"].ptr, sim.a["count"].ptr, None, gin)
-    couple_ms = timed_events(ctx, couple, 50)
 
     # the dense pull coupling (WEIGHTED_PULL: every target reads all its
     # in-edges, the spike bitmap and the weights of its spiking in-edges),
@@ -764,10 +957,12 @@ def main():
     log(f"bench: pull coupling {roof_pull}")
 
     ro = heb = None
-    if snap is not None:
-        ro = readout_bench(ctx, sim, net, args.chunk, snap, steps=args.steps)
+    if args.couple == "fused" and not args.no_readout:
+        para(False)
+        rsnap = (snap[0], 0)
+        ro = readout_bench(ctx, sim, net, chunk, rsnap, steps=2000)
         log(f"bench: readout {ro}")
-        heb = hebbian_bench(ctx, sim, net, args.chunk, snap, steps=args.steps)
+        heb = hebbian_bench(ctx, sim, net, chunk, rsnap, steps=2000)
         log(f"bench: hebbian {heb}")
 
     roof_stencil = stencil_roofline(ctx)
@@ -775,16 +970,12 @@ def main():
 
     cg = None
     if not args.no_cg:
-        vc = sim.a["voxel_counts"].numpy().astype(np.float32)
-        b = 0.01 * vc
-        if not b.any():
-            b[len(b) // 2] = 1.0
-        cg = cg_bench(ctx, net.grid[0], b)
+        cg = cg_bench(ctx, net.grid[0], b_fixed)
         log(f"bench: CG {cg}")
 
     c5 = None
     if not args.no_c5:
-        c5 = c5_bench(ctx, sim.a["voxel_counts"].numpy())
+        c5 = c5_bench(ctx, counts_fixed)
         log(f"bench: C5 {c5}")
 
     c4 = None
@@ -801,25 +992,30 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(net, args.cpu_steps)
+        cpu = cpu_baseline_full(net, every, b_fixed)
         log(f"bench: cpu {cpu}")
-
+    n_graphs = ctx.graph_cache()
     out = {"metric": METRIC, "value": value, "unit": "neuron-steps/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+           "steps": steps, "warmup": warm, "steps_requested": args.steps, "warmup_requested": args.warmup,
+           "ms_per_step": ms_step,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (Philox4x64-10, seed 0)",
            "config": {"workload": "c2: 1,048,576 squid HH neurons, RK4 dt 0.01 ms, ~210M-edge "
                                   "exp(-d/10) CSR, log-normal weights, event-driven deterministic "
-                                  "coupling, release deposit on 256^3",
+                                  "coupling, release deposit on 256^3 with the paracrine Jacobi-CG solve to a "
+                                  "true 1e-8 every 100 steps inside the step graphs; the configuration's "
+                                  "10,000 steps from step 0",
                       "n": net.n, "nnz": net.nnz, "dt": net.dt, "couple": args.couple, "fused_shape": tune,
-                      "graph_chunk": args.chunk, "spikes_timed": int(spikes),
-                      "spikes_per_step": spikes / args.steps},
-           "roofline": roof, "cpu_baseline": cpu, "cg": cg, "roofline_stencil": roof_stencil,
+                      "graph_chunk": chunk, "spikes_timed": int(spikes), "spikes_per_step": spikes / steps,
+                      "paracrine_every": every},
+           "roofline": roof, "cpu_baseline": cpu,
+           "steps_only": {"neuron_steps_per_s": steps_only, "ms": wall0_ms, "ms_per_step": wall0_ms / steps},
+           "paracrine": paracrine, "cg": cg, "roofline_fused_step": fused_roof,
+           "roofline_cg_iteration": cg_iter, "roofline_stencil": roof_stencil,
            "roofline_hh_unfused": roof_hh, "roofline_pull_coupling": roof_pull,
            "c3": c3, "c4": c4, "c5": c5, "readout": ro, "hebbian": heb, "models": models,
-           "step_breakdown_us": {"hh_rk4": hh_ms * 1e3, "coupling_eager": couple_ms * 1e3,
-                                 "step_graph": ms_step * 1e3},
-           "event_ms": ev_ms.value}
+           "graphs": {"cached": n_graphs[0], "instantiated": n_graphs[1]},
+           "event_ms": ev_ms}
     print(json.dumps(out), flush=True)
 
 

@@ -372,6 +372,18 @@ nnx_status nnx_stencil_apply(nnx_ctx* ctx, const nnx_grid* g, const float* d_x, 
 nnx_status nnx_cg_solve(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float* d_x,
                         const nnx_cg_opts* o, nnx_cg_result* res);
 
+/* The same Jacobi (or unpreconditioned) CG with no host read and no host
+ * synchronisation, so that it can be captured into a hipGraph: the init and
+ * exactly o->maxit iterations are enqueued; the iterations after
+ * convergence are launched but return at once (the stop test runs on the
+ * device).  d_result (device, 4 doubles) receives the iterations performed,
+ * the final relative recursive residual and 1.0 if it met o->rtol (else
+ * 0.0).  check_every, stall_iters, smooth and the mg_* fields are unused.
+ * No counterpart in the reference (its CG reads every scalar back to the
+ * host, src/paracrine.cu:106-129).                                           */
+nnx_status nnx_cg_solve_async(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float* d_x,
+                              const nnx_cg_opts* o, double* d_result);
+
 /* Mixed-precision iterative refinement: FP64 b and x, the outer residual
  * r = b - A x recomputed in FP64, inner fp32 CG on A d = r, x += d, until
  * ||r||/||b|| <= o->rtol.  Each inner solve stops at the relative residual
@@ -669,6 +681,40 @@ nnx_status nnx_hebbian_update(nnx_ctx* ctx, const nnx_graph* g, const nnx_hebbia
 /* composite timestep                                                        */
 /* ------------------------------------------------------------------------ */
 
+/* The paracrine field solved on its own cadence inside the timestep, as
+ * ParacrineModule::update_on_own_timescale does every `timescale` steps
+ * (src/main.cu:171-177, src/paracrine.cuh:255-265): before each step s with
+ * s % every == 0 the releases counted since the last solve become the
+ * source,
+ *     b = conc + release * voxel_counts,   voxel_counts = 0,
+ * and conc is replaced by the solution of A conc = b (A the grid's
+ * operator), warm-started from the previous field.  The solve is Jacobi-CG
+ * with FP64 iterative refinement: the residual b - A conc in FP64, an inner
+ * fp32 CG (FP64 dots) on it, conc += correction, until the FP64 relative
+ * residual ||b - A conc|| / ||b|| <= rtol.  Nothing is read back: max_outer
+ * refinement steps of max_inner CG iterations each are enqueued (captured
+ * with the steps), and every kernel after convergence returns at once.
+ * Releases: every spike of the steps before s (the reference's
+ * update_on_fine_timescale skips the spikes its coupling reads on a solve
+ * step; here none is skipped).                                             */
+typedef struct nnx_paracrine {
+    int64_t every;            /* the reference's timescale in steps (<= 0: off) */
+    nnx_grid grid;            /* nx*ny*nz = the number of voxel_counts entries */
+    double release;           /* source added per spike at its voxel (0.01) */
+    double rtol;              /* FP64 relative residual of each solve (1e-8) */
+    double inner_rtol;        /* tightest inner fp32 CG tolerance (0: 1e-6) */
+    int32_t max_inner;        /* CG iterations enqueued per refinement step (0: 64) */
+    int32_t max_outer;        /* refinement steps enqueued (0: 3) */
+    double* conc;             /* device, nvox FP64: the field (in: previous, out: new) */
+    float* at_neurons;        /* optional device n floats: conc at each neuron's voxel
+                               * after each solve (the write-back into d_g_paracrine,
+                               * src/paracrine.cuh:261-264, at the neuron's voxel) */
+    double* stats;            /* optional device 8 doubles, accumulated (the caller
+                               * zeroes them): solves, inner iterations, refinement
+                               * steps, max relres, solves above rtol, and the last
+                               * solve's relres, inner iterations, refinement steps */
+} nnx_paracrine;
+
 /* Everything one timestep touches, in the reference's order
  * coupling -> paracrine -> ephys (src/main.cu:166-181,
  * src_wholebrain/mouse_cortex.cu:104-120).                                 */
@@ -720,17 +766,34 @@ typedef struct nnx_step_plan {
      * 158-168).  The learned weights are not the coupling's weights (the
      * reference's coupling does not read them either).                    */
     nnx_hebbian hebbian;
+    /* paracrine solve on its cadence (optional; paracrine.every <= 0 = off;
+     * needs voxel_of and voxel_counts)                                     */
+    nnx_paracrine paracrine;
 } nnx_step_plan;
 
 /* Runs n_steps timesteps starting at global step `first_step`.  With
  * use_graph = 1 the step sequence is captured once into a hipGraph per
- * (plan, n_steps) and replayed; plans must not move between calls.
+ * (plan contents, n_steps, phase) and replayed; plans must not move between
+ * calls.  The context keeps at most NNX_GRAPH_CACHE executable graphs
+ * (least recently used ones are destroyed first).
  * WEIGHTED_FUSED: g_in is not used; each call starts from the previous
  * step's list/count and leaves the last step's list/count, so calls with
  * other coupling modes can be interleaved; raster entries of one step are
  * appended in unspecified order (steps in order).                          */
 nnx_status nnx_run_steps(nnx_ctx* ctx, nnx_step_plan* plan, uint64_t first_step, int n_steps,
                          int use_graph);
+
+/* Executable graphs kept per context by nnx_run_steps and
+ * nnx_ensemble_run.                                                         */
+#define NNX_GRAPH_CACHE 32
+/* Destroys every cached graph (after the context's stream has drained);
+ * a caller frees a plan's buffers after this or after nnx_destroy.  The
+ * reference has no graphs (every launch is synchronised,
+ * src/main.cu:112-182).                                                     */
+nnx_status nnx_graph_cache_clear(nnx_ctx* ctx);
+/* Number of cached executable graphs and the number instantiated since the
+ * context was created (either pointer may be NULL).                        */
+nnx_status nnx_graph_cache_stats(const nnx_ctx* ctx, int* cached, int64_t* instantiated);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,16 @@ class Hebbian(C.Structure):
                 ("last_fire", vp), ("ring", vp), ("in_to_out", vp)]
 
 
+class Paracrine(C.Structure):
+    _fields_ = [("every", C.c_int64), ("grid", Grid), ("release", C.c_double), ("rtol", C.c_double),
+                ("inner_rtol", C.c_double), ("max_inner", C.c_int32), ("max_outer", C.c_int32),
+                ("conc", vp), ("at_neurons", vp), ("stats", vp)]
+
+
+PARA_STATS = ("solves", "inner_iters", "outer_steps", "max_relres", "unconverged", "last_relres",
+              "last_inner_iters", "last_outer_steps")
+
+
 class StepPlan(C.Structure):
     _fields_ = [("state", HHState), ("params", HHParams), ("graph", Graph),
                 ("integrator", C.c_int32), ("spike_rule", C.c_int32), ("couple_mode", C.c_int32),
@@ -136,7 +146,7 @@ class StepPlan(C.Structure):
                 ("fused_block", C.c_int32), ("fused_tile", C.c_int32), ("fused_grid", C.c_int32),
                 ("fused_burst", C.c_int32), ("regions", Regions), ("readout_every", C.c_int64),
                 ("readout_cap", C.c_int64), ("readout_avg", vp), ("readout_firing", vp),
-                ("readout_mean", vp), ("hebbian", Hebbian)]
+                ("readout_mean", vp), ("hebbian", Hebbian), ("paracrine", Paracrine)]
 
 
 class EnsPlan(C.Structure):
@@ -187,6 +197,9 @@ _SIGS = {
     "nnx_cg_solve": ([vp, _P(Grid), vp, vp, _P(CgOpts), _P(CgResult)], C.c_int),
     "nnx_cg_solve_refine": ([vp, _P(Grid), vp, vp, _P(CgOpts), C.c_double, C.c_int,
                              _P(CgResult)], C.c_int),
+    "nnx_cg_solve_async": ([vp, _P(Grid), vp, vp, _P(CgOpts), vp], C.c_int),
+    "nnx_graph_cache_clear": ([vp], C.c_int),
+    "nnx_graph_cache_stats": ([vp, _P(C.c_int), _P(C.c_int64)], C.c_int),
     "nnx_update_density_reference": ([vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_int, vp,
                                       _P(CgResult)], C.c_int),
@@ -319,6 +332,15 @@ class Context:
 
     def synchronize(self):
         self.call("nnx_synchronize")
+
+    def graph_cache(self):
+        """(cached executable graphs, graphs instantiated so far)."""
+        n, tot = C.c_int(), C.c_int64()
+        self.call("nnx_graph_cache_stats", C.byref(n), C.byref(tot))
+        return n.value, tot.value
+
+    def clear_graphs(self):
+        self.call("nnx_graph_cache_clear")
 
     def close(self):
         if getattr(self, "h", None):

@@ -12,16 +12,11 @@ nnx_status scratch(nnx_ctx* c, const char* name, size_t bytes, bool zero, void**
             return fail(c, NNX_ERR_STATE, "scratch '%s' must grow to %zu bytes during graph capture",
                         name, bytes);
         if (b.p) {
-            NNX_HIP(c, hipStreamSynchronize(c->stream));
+            // cached graphs hold the old address: drop them (re-captured on use)
+            NNX_TRY(graph_cache_clear(c));
             NNX_HIP(c, hipFree(b.p));
             b.p = nullptr;
             b.bytes = 0;
-            // cached graphs hold the old address: drop them (re-captured on use)
-            for (auto& g : c->graphs) {
-                if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
-                if (g.second.graph) (void)hipGraphDestroy(g.second.graph);
-            }
-            c->graphs.clear();
         }
         size_t alloc = bytes < 256 ? 256 : bytes;
         NNX_HIP(c, hipMalloc(&b.p, alloc));
@@ -30,6 +25,58 @@ nnx_status scratch(nnx_ctx* c, const char* name, size_t bytes, bool zero, void**
         if (zero) NNX_HIP(c, hipMemsetAsync(b.p, 0, alloc, c->stream));
     }
     *out = b.p;
+    return NNX_OK;
+}
+
+nnx_status graph_cache_clear(nnx_ctx* c) {
+    if (c->graphs.empty()) return NNX_OK;
+    NNX_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& g : c->graphs) {
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+        if (g.second.graph) (void)hipGraphDestroy(g.second.graph);
+    }
+    c->graphs.clear();
+    return NNX_OK;
+}
+
+nnx_status graph_launch_cached(nnx_ctx* c, GraphKey key, bool* found) {
+    auto it = c->graphs.find(key);
+    *found = it != c->graphs.end();
+    if (!*found) return NNX_OK;
+    it->second.last_use = ++c->graph_clock;
+    NNX_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
+    return NNX_OK;
+}
+
+nnx_status graph_insert_launch(nnx_ctx* c, GraphKey key, hipGraph_t graph) {
+    // evict the least recently launched graphs first (after the stream has
+    // drained: an evicted graph may still be running)
+    if ((int)c->graphs.size() >= NNX_GRAPH_CACHE) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(c, NNX_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+        }
+        while ((int)c->graphs.size() >= NNX_GRAPH_CACHE) {
+            auto lru = c->graphs.begin();
+            for (auto it = c->graphs.begin(); it != c->graphs.end(); ++it)
+                if (it->second.last_use < lru->second.last_use) lru = it;
+            if (lru->second.exec) (void)hipGraphExecDestroy(lru->second.exec);
+            if (lru->second.graph) (void)hipGraphDestroy(lru->second.graph);
+            c->graphs.erase(lru);
+        }
+    }
+    nnx_ctx::Graph g;
+    g.graph = graph;
+    hipError_t e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(c, NNX_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    ++c->graphs_instantiated;
+    g.last_use = ++c->graph_clock;
+    c->graphs[key] = g;
+    NNX_HIP(c, hipGraphLaunch(g.exec, c->stream));
     return NNX_OK;
 }
 
@@ -126,15 +173,25 @@ nnx_status nnx_destroy(nnx_ctx* c) {
     if (!c) return NNX_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (auto& g : c->graphs) {
-        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
-        if (g.second.graph) (void)hipGraphDestroy(g.second.graph);
-    }
+    (void)graph_cache_clear(c);
     for (auto& b : c->bufs)
         if (b.second.p) (void)hipFree(b.second.p);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return NNX_OK;
+}
+
+nnx_status nnx_graph_cache_clear(nnx_ctx* c) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, !c->capturing, "graph cache cleared during a capture");
+    return graph_cache_clear(c);
+}
+
+nnx_status nnx_graph_cache_stats(const nnx_ctx* c, int* cached, int64_t* instantiated) {
+    if (!c) return NNX_ERR_ARG;
+    if (cached) *cached = (int)c->graphs.size();
+    if (instantiated) *instantiated = c->graphs_instantiated;
     return NNX_OK;
 }
 

@@ -391,30 +391,6 @@ uint64_t ens_key(const nnx_ens_plan* p, const nnx_ens_config& f) {
 const nnx_ens_config kDefaultCfg = {256, 256, 8192, 0};
 const nnx_ens_config kFusedDefault = {256, 256, 4096, 0};
 
-// Captures `body` (which enqueues on c->stream) into a graph cached under
-// `key`, or replays the cached one.
-template <class F> nnx_status capture_or_replay(nnx_ctx* c, std::pair<const void*, int> key, F body) {
-    NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
-    auto it = c->graphs.find(key);
-    if (it == c->graphs.end()) {
-        nnx_ctx::Graph g;
-        NNX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        c->capturing = 1;
-        const nnx_status st = body();
-        c->capturing = 0;
-        hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
-        if (st != NNX_OK) {
-            if (g.graph) (void)hipGraphDestroy(g.graph);
-            return st;
-        }
-        if (e != hipSuccess) return fail(c, NNX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-        NNX_HIP(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
-        it = c->graphs.emplace(key, g).first;
-    }
-    NNX_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
-    return NNX_OK;
-}
-
 }  // namespace
 
 // The WEIGHTED_FUSED timestep of nnx_run_steps: one network stepped by the
@@ -452,6 +428,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, 1);
     NNX_LAUNCHED(c);
     NNX_TRY(readout_prepare(c, p));
+    NNX_TRY(para_prepare(c, p));
     auto body = [&]() -> nnx_status {
         k_push_bits<<<2048, 256, 0, c->stream>>>(p->list, p->count, p->graph.out_offsets, p->graph.out_targets,
                                                  p->graph.out_to_in, eb_q0, flag_q0);
@@ -465,6 +442,8 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
             if (p->hebbian.window > 0)
                 NNX_TRY(hebbian_launch(c, &p->graph, &p->hebbian, h.k.stepctr + ((st + 1) & 1), 1, 0,
                                        p->bits[(st + 1) & 1]));
+            // paracrine solve on its cadence (releases of the steps before st)
+            if (p->paracrine.every > 0 && st % (uint64_t)p->paracrine.every == 0) NNX_TRY(para_launch(c, p));
             NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)(st & 1), i + 1 < n_steps));
         }
         CompactExtras ex{};
@@ -474,8 +453,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     uint64_t key = 0xcbf29ce484222325ull;
     const unsigned char* u = (const unsigned char*)p;
     for (size_t i = 0; i < sizeof(*p); ++i) key = (key ^ u[i]) * 1099511628211ull;
-    key ^= 0x4655534544535450ull;
-    if (p->readout_every > 0) key ^= (first_step % (uint64_t)p->readout_every) * 0x9e3779b97f4a7c15ull;
+    key ^= 0x4655534544535450ull ^ phase_key(p, first_step);
     return capture_or_replay(c, {(const void*)(uintptr_t)key, n_steps * 2 + (int)(first_step & 1)}, body);
 }
 

@@ -25,12 +25,16 @@ struct nnx_ctx {
     };
     std::map<std::string, Buf> bufs;
     // captured step graphs keyed by (plan hash, n_steps and parity); they
-    // embed scratch addresses, so growing any scratch buffer drops them all
+    // embed scratch addresses, so growing any scratch buffer drops them all;
+    // at most NNX_GRAPH_CACHE are kept (least recently launched evicted)
     struct Graph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        uint64_t last_use = 0;
     };
     std::map<std::pair<const void*, int>, Graph> graphs;
+    uint64_t graph_clock = 0;
+    int64_t graphs_instantiated = 0;
     int capturing = 0;
 };
 
@@ -73,6 +77,36 @@ inline nnx_status fail(nnx_ctx* c, nnx_status s, const char* fmt, ...) {
 nnx_status scratch(nnx_ctx* c, const char* name, size_t bytes, bool zero, void** out);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Destroys every cached graph (waits for the stream first).
+nnx_status graph_cache_clear(nnx_ctx* c);
+
+// Launches the graph cached under `key`, capturing `body` (which enqueues on
+// c->stream) into a new one first if there is none; evicts the least
+// recently launched graph beyond NNX_GRAPH_CACHE.
+typedef std::pair<const void*, int> GraphKey;
+template <class F> nnx_status capture_or_replay(nnx_ctx* c, GraphKey key, F body);
+nnx_status graph_insert_launch(nnx_ctx* c, GraphKey key, hipGraph_t graph);
+nnx_status graph_launch_cached(nnx_ctx* c, GraphKey key, bool* found);
+
+template <class F> nnx_status capture_or_replay(nnx_ctx* c, GraphKey key, F body) {
+    NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
+    bool found = false;
+    NNX_TRY(graph_launch_cached(c, key, &found));
+    if (found) return NNX_OK;
+    hipGraph_t graph = nullptr;
+    NNX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = 1;
+    const nnx_status st = body();
+    c->capturing = 0;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (st != NNX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return fail(c, NNX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    return graph_insert_launch(c, key, graph);
+}
 
 template <class T> HHConst<T> make_const(const nnx_hh_params* q, double dt);
 
@@ -129,6 +163,14 @@ nnx_status readout_launch(nnx_ctx* c, const nnx_step_plan* p, const int64_t* ste
 nnx_status hebbian_launch(nnx_ctx* c, const nnx_graph* g, const nnx_hebbian* h, const int64_t* stepp, int64_t add,
                           int64_t step, const uint64_t* fire);
 nnx_status check_hebbian_plan(nnx_ctx* c, const nnx_step_plan* p);
+
+// the paracrine solve inside nnx_run_steps (paracrine.hip): plan check,
+// scratch set-up before a capture, and the launches of one solve
+nnx_status para_check(nnx_ctx* c, const nnx_step_plan* p);
+nnx_status para_prepare(nnx_ctx* c, const nnx_step_plan* p);
+nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p);
+// graph key component for the print and paracrine phases of first_step
+uint64_t phase_key(const nnx_step_plan* p, uint64_t first_step);
 
 }  // namespace nnx
 

@@ -458,36 +458,65 @@ struct CgHost {
     double* out2;
 };
 
-static nnx_status cg_buffers(nnx_ctx* c, const GridK& gk, int precond, CgHost* h) {
+// Device memory of one CG working set: four n-vectors (r, p ping-pong, q),
+// the Jacobi inverse diagonal (n floats, variable coefficients only), 64
+// scalars, 64 flags and the reduction partials (kCgPartDoubles).
+struct CgMem {
+    float *r, *pa, *pb, *q, *dinv;
+    double* sc;
+    int32_t* fl;
+    double* part;
+};
+constexpr size_t kCgPartDoubles = (size_t)kRedBlocks * 16;
+
+static bool cg_needs_dinv(const GridK& gk, int precond) { return precond == NNX_PRECOND_JACOBI && gk.kind == 1; }
+
+// the working set from the context's named scratch buffers `prefix`_*
+static nnx_status cg_mem_scratch(nnx_ctx* c, const char* prefix, const GridK& gk, int precond, CgMem* m) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
-    void *r, *pa, *pb, *q, *sc, *fl, *part;
-    NNX_TRY(scratch(c, "cg_r", n * 4, false, &r));
-    NNX_TRY(scratch(c, "cg_pa", n * 4, false, &pa));
-    NNX_TRY(scratch(c, "cg_pb", n * 4, false, &pb));
-    NNX_TRY(scratch(c, "cg_q", n * 4, false, &q));
-    NNX_TRY(scratch(c, "cg_sc", 64 * 8, true, &sc));
-    NNX_TRY(scratch(c, "cg_fl", 64 * 4, true, &fl));
-    NNX_TRY(scratch(c, "cg_part", (size_t)kRedBlocks * 16 * 8, false, &part));
-    double* P = (double*)part;
-    h->d.r = (float*)r;
-    h->d.q = (float*)q;
-    h->pa = (float*)pa;
-    h->pb = (float*)pb;
-    h->d.sc = (double*)sc;
-    h->d.fl = (int32_t*)fl;
+    char nm[64];
+    void* p[8] = {};
+    const char* names[] = {"r", "pa", "pb", "q", "sc", "fl", "part"};
+    const size_t bytes[] = {(size_t)n * 4, (size_t)n * 4, (size_t)n * 4, (size_t)n * 4, 64 * 8, 64 * 4,
+                            kCgPartDoubles * 8};
+    for (int i = 0; i < 7; ++i) {
+        std::snprintf(nm, sizeof nm, "%s_%s", prefix, names[i]);
+        NNX_TRY(scratch(c, nm, bytes[i], i == 4 || i == 5, &p[i]));
+    }
+    m->dinv = nullptr;
+    if (cg_needs_dinv(gk, precond)) {
+        std::snprintf(nm, sizeof nm, "%s_dinv", prefix);
+        void* dvp;
+        NNX_TRY(scratch(c, nm, n * 4, false, &dvp));
+        m->dinv = (float*)dvp;
+    }
+    m->r = (float*)p[0], m->pa = (float*)p[1], m->pb = (float*)p[2], m->q = (float*)p[3];
+    m->sc = (double*)p[4], m->fl = (int32_t*)p[5], m->part = (double*)p[6];
+    return NNX_OK;
+}
+
+// Points h at the working set m; fills the inverse diagonal when the grid
+// needs one (a launch on the context's stream).
+static nnx_status cg_bind(nnx_ctx* c, const GridK& gk, int precond, const CgMem& m, CgHost* h) {
+    const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
+    double* P = m.part;
+    h->d.r = m.r;
+    h->d.q = m.q;
+    h->pa = m.pa;
+    h->pb = m.pb;
+    h->d.sc = m.sc;
+    h->d.fl = m.fl;
     h->d.pu0 = P;                     // 2 x kRedBlocks
     h->d.pu1 = P + 2 * kRedBlocks;    // 2 x kRedBlocks
     h->d.ppq = P + 4 * kRedBlocks;    // up to 2048 operator blocks
     h->d.pb = P + 6 * kRedBlocks;     // kRedBlocks
-    h->tr_part = P + 7 * kRedBlocks;  // 2 x kRedBlocks
-    h->out2 = (double*)sc + 32;
+    h->tr_part = P + 7 * kRedBlocks;  // 2 x kRedBlocks (up to 9 x kRedBlocks used)
+    h->out2 = m.sc + 32;
     h->d.dinv = nullptr;
     h->d.dinv_c = 1.0f;
     if (precond == NNX_PRECOND_JACOBI) {
         if (gk.kind == 1) {
-            void* dvp;
-            NNX_TRY(scratch(c, "cg_dinv", n * 4, false, &dvp));
-            h->d.dinv = (float*)dvp;
+            h->d.dinv = m.dinv;
             k_dinv<<<flat_blocks(n), 256, 0, c->stream>>>(gk, h->d.dinv);
             NNX_LAUNCHED(c);
         } else {
@@ -495,6 +524,12 @@ static nnx_status cg_buffers(nnx_ctx* c, const GridK& gk, int precond, CgHost* h
         }
     }
     return NNX_OK;
+}
+
+static nnx_status cg_buffers(nnx_ctx* c, const GridK& gk, int precond, CgHost* h) {
+    CgMem m;
+    NNX_TRY(cg_mem_scratch(c, "cg", gk, precond, &m));
+    return cg_bind(c, gk, precond, m, h);
 }
 
 static nnx_status cg_readback(nnx_ctx* c, CgHost& h, int32_t* fl, double* sc) {
@@ -506,12 +541,28 @@ static nnx_status cg_readback(nnx_ctx* c, CgHost& h, int32_t* fl, double* sc) {
     return NNX_OK;
 }
 
+// Iteration k of CG on the prepared buffers: the new direction and q = A p
+// (one fused kernel for the constant 7-point stencil), then the update of x
+// and r.  Both kernels return at once when the device flag says the solve
+// is over, so iterations can be enqueued past convergence.
+static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float* x, int vec, bool fast, int nb4) {
+    const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
+    float* pold = (k & 1) ? h.pb : h.pa;
+    float* pnew = (k & 1) ? h.pa : h.pb;
+    if (fast) {
+        k_stencil7x4<true><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nb4, nullptr, pold, pnew, h.d.q);
+    } else {
+        k_cg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pold, pnew);
+        k_cg_spmv_gen<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, k, pnew);
+    }
+    k_cg_update<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pnew, x, vec);
+}
+
 // Runs CG on the prepared buffers; x holds the guess when use_x0.  The host
 // enqueues check_every iterations, then one status kernel and a read-back.
 static nnx_status cg_run(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b, float* x,
                          int use_x0, double tol, int maxit, int abs_mode, int check_every,
                          nnx_cg_result* res, int* initdone = nullptr) {
-    const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
     k_cg_init<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, b, x, use_x0, h.pa);
     k_cg_init_fin<<<1, 256, 0, c->stream>>>(h.d, tol, abs_mode, maxit);
     NNX_LAUNCHED(c);
@@ -527,17 +578,7 @@ static nnx_status cg_run(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b,
     if (initdone) *initdone = fl[FL_INITDONE];
     int enq = 0;
     while (!fl[FL_DONE] && enq < maxit) {
-        for (int q = 0; q < check_every && enq < maxit; ++q, ++enq) {
-            float* pold = (enq & 1) ? h.pb : h.pa;
-            float* pnew = (enq & 1) ? h.pa : h.pb;
-            if (fast) {
-                k_stencil7x4<true><<<nb4, kT4, 0, c->stream>>>(gk, h.d, enq, nb4, nullptr, pold, pnew, h.d.q);
-            } else {
-                k_cg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, enq, pold, pnew);
-                k_cg_spmv_gen<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, enq, pnew);
-            }
-            k_cg_update<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, enq, pnew, x, vec);
-        }
+        for (int q = 0; q < check_every && enq < maxit; ++q, ++enq) cg_iter_launch(c, gk, h, enq, x, vec, fast, nb4);
         k_cg_status<<<1, 256, 0, c->stream>>>(h.d);
         NNX_LAUNCHED(c);
         NNX_TRY(cg_readback(c, h, fl, sc));
@@ -566,6 +607,149 @@ static nnx_status true_residual(nnx_ctx* c, const GridK& gk, CgHost& h, const U*
         rr_out[1] = o[1];
     }
     return NNX_OK;
+}
+
+// ---- solves enqueued without host reads (capturable) ----
+
+// Outcome of the CG run on h after its last executed update, for the caller
+// (device doubles): iterations, relative recursive residual, converged.
+__global__ __launch_bounds__(256) void k_cg_result(CgDev d, double* out) {
+    const int it = d.fl[FL_ITERS];
+    double u[2];
+    sum_partials<2>((it & 1) ? d.pu1 : d.pu0, kRedBlocks, u);
+    if (threadIdx.x == 0) {
+        const double thr = d.sc[SC_THR], bb = d.sc[SC_BB];
+        out[0] = (double)it;
+        out[1] = u[1] > 0 ? sqrt(u[1] / (bb > 0 ? bb : 1.0)) : 0.0;
+        out[2] = u[1] <= thr ? 1.0 : 0.0;
+        out[3] = 0.0;
+    }
+}
+
+// Init and exactly maxit iterations, no host read: the iterations after
+// convergence are launched but return at once.
+static void cg_enqueue_fixed(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b, float* x, int use_x0,
+                             double tol, int maxit) {
+    k_cg_init<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, b, x, use_x0, h.pa);
+    k_cg_init_fin<<<1, 256, 0, c->stream>>>(h.d, tol, 0, maxit);
+    const int vec = ((uintptr_t)x % 16) == 0;
+    const bool fast = gk.kind == 0 && gk.nx % 4 == 0;
+    const int nb4 = stencil4_blocks(gk);
+    h.d.npq = fast ? nb4 : kRedBlocks;
+    for (int k = 0; k < maxit; ++k) cg_iter_launch(c, gk, h, k, x, vec, fast, nb4);
+}
+
+// The paracrine solve of a timestep (nnx_paracrine): rhs, then max_outer
+// refinement steps - FP64 residual b - A c, inner fp32 Jacobi-CG on it of
+// at most max_inner iterations, c += correction - and a final FP64 check.
+// Refinement flags (pfl): [0] done, [1] refinement steps, [2] inner iterations.
+
+// b = conc + amount * counts in FP64, counts = 0; partials of ||b||^2
+__global__ __launch_bounds__(256) void k_para_rhs(int64_t n, const double* __restrict__ conc,
+                                                  int32_t* __restrict__ vc, double amount,
+                                                  double* __restrict__ b, double* pbb, int32_t* pfl) {
+    double v[1] = {0};
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int32_t k = vc[i];
+        const double bi = conc[i] + amount * (double)k;
+        b[i] = bi;
+        if (k) vc[i] = 0;
+        v[0] += bi * bi;
+    }
+    put_partial<1>(v, pbb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) pfl[0] = pfl[1] = pfl[2] = 0;
+}
+
+// r = b - A x in FP64 (partials of ||r||^2); unless FINAL also the inner
+// CG's start on r32 = (float) r: residual r32, correction 0, p0 = 0, and
+// the partials k_cg_init would leave.
+template <bool FINAL>
+__global__ __launch_bounds__(256) void k_para_res(GridK g, const double* __restrict__ x, const double* __restrict__ b,
+                                                  CgDev d, float* __restrict__ xin, float* __restrict__ p0,
+                                                  double* prr, const int32_t* __restrict__ pfl) {
+    if (pfl[0]) return;
+    double vr[1] = {0}, vb[1] = {0}, vu[2] = {0, 0};
+    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
+        const double r = b[id] - apply_at<double, double>(x, g, i, j, k);
+        vr[0] += r * r;
+        if (!FINAL) {
+            const float r32 = (float)r;
+            d.r[id] = r32;
+            xin[id] = 0.0f;
+            p0[id] = 0.0f;
+            vb[0] += (double)r32 * r32;
+            vu[0] += (double)r32 * (r32 * dv(d, id));
+            vu[1] += (double)r32 * r32;
+        }
+    });
+    put_partial<1>(vr, prr);
+    if (!FINAL) {
+        put_partial<1>(vb, d.pb);
+        put_partial<2>(vu, d.pu0);
+    }
+}
+
+// Refinement control after k_para_res: relres = ||r|| / ||b||.  Done when
+// relres <= rtol or at the final check (the outcome goes to stats);
+// otherwise the inner CG's state for the tolerance max(inner_rtol,
+// rtol / (2 relres)).
+__global__ __launch_bounds__(256) void k_para_ctl(CgDev d, const double* prr, const double* pbb, double rtol,
+                                                  double inner_rtol, int maxit, int32_t* pfl, double* stats,
+                                                  int final_check) {
+    if (pfl[0]) return;
+    double rr[1], bb[1], bin[1] = {0}, u[2] = {0, 0};
+    sum_partials<1>(prr, kRedBlocks, rr);
+    sum_partials<1>(pbb, kRedBlocks, bb);
+    if (!final_check) {
+        sum_partials<1>(d.pb, kRedBlocks, bin);
+        sum_partials<2>(d.pu0, kRedBlocks, u);
+    }
+    if (threadIdx.x != 0) return;
+    const double rel = rr[0] <= 0 ? 0.0 : (bb[0] > 0 ? sqrt(rr[0] / bb[0]) : INFINITY);
+    if (rel <= rtol || final_check) {
+        pfl[0] = 1;
+        d.fl[FL_DONE] = 1;
+        if (stats) {
+            stats[0] += 1.0;
+            stats[1] += (double)pfl[2];
+            stats[2] += (double)pfl[1];
+            stats[3] = fmax(stats[3], rel);
+            if (!(rel <= rtol)) stats[4] += 1.0;
+            stats[5] = rel;
+            stats[6] = (double)pfl[2];
+            stats[7] = (double)pfl[1];
+        }
+        return;
+    }
+    const double tol = fmax(inner_rtol, 0.5 * rtol / rel);
+    const double thr = tol * tol * bin[0];
+    d.sc[SC_BB] = bin[0];
+    d.sc[SC_THR] = thr;
+    d.sc[SC_RZ] = u[0];
+    d.sc[SC_RR] = u[1];
+    d.fl[FL_ABS] = 0;
+    d.fl[FL_INITDONE] = u[1] <= thr;
+    d.fl[FL_DONE] = (u[1] <= thr) || maxit <= 0;
+    d.fl[FL_ITERS] = 0;
+}
+
+// x += correction; counts the refinement step and its inner iterations
+__global__ __launch_bounds__(256) void k_para_axpy(int64_t n, const float* __restrict__ xin, double* __restrict__ x,
+                                                   const int32_t* __restrict__ cgfl, int32_t* pfl) {
+    if (pfl[0]) return;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        x[i] = x[i] + (double)xin[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pfl[1] += 1;
+        pfl[2] += cgfl[FL_ITERS];
+    }
+}
+
+// the field at each neuron's voxel (the write-back of the solve step)
+__global__ void k_para_neurons(int64_t n, const int32_t* __restrict__ voxel_of, const double* __restrict__ conc,
+                               float* __restrict__ out) {
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        out[i] = (float)conc[voxel_of[i]];
 }
 
 // ---- refinement kernels ----
@@ -857,6 +1041,119 @@ nnx_status nnx_cg_solve_refine(nnx_ctx* c, const nnx_grid* g, const double* b, d
 }
 
 } // extern "C"
+
+// ---- the paracrine solve inside nnx_run_steps ----
+namespace nnx {
+
+struct ParaHost {
+    CgHost h;
+    double *b, *prr, *pbb;
+    float* xin;
+    int32_t* pfl;
+};
+
+nnx_status para_check(nnx_ctx* c, const nnx_step_plan* p) {
+    const nnx_paracrine& q = p->paracrine;
+    if (q.every <= 0) return NNX_OK;
+    NNX_TRY(check_grid(c, &q.grid));
+    NNX_CHECK(c, p->voxel_of && p->voxel_counts, "paracrine needs the release deposit (voxel_of, voxel_counts)");
+    NNX_CHECK(c, q.conc, "paracrine needs conc");
+    NNX_CHECK(c, q.rtol > 0 && q.inner_rtol >= 0 && q.release >= 0, "paracrine: rtol > 0, inner_rtol, release >= 0");
+    NNX_CHECK(c, q.max_inner >= 0 && q.max_outer >= 0 && q.max_inner <= 100000 && q.max_outer <= 64,
+              "paracrine: max_inner in [0, 100000], max_outer in [0, 64]");
+    return NNX_OK;
+}
+
+static nnx_status para_host(nnx_ctx* c, const nnx_paracrine& q, const GridK& gk, ParaHost* ph) {
+    const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
+    CgMem m;
+    NNX_TRY(cg_mem_scratch(c, "para", gk, NNX_PRECOND_JACOBI, &m));
+    void *b, *xin, *fl;
+    NNX_TRY(scratch(c, "para_b", (size_t)n * 8, false, &b));
+    NNX_TRY(scratch(c, "para_xin", (size_t)n * 4, false, &xin));
+    NNX_TRY(scratch(c, "para_pfl", 64, true, &fl));
+    ph->b = (double*)b;
+    ph->xin = (float*)xin;
+    ph->pfl = (int32_t*)fl;
+    ph->prr = m.part + 9 * kRedBlocks;
+    ph->pbb = m.part + 10 * kRedBlocks;
+    // the inverse diagonal of a variable-coefficient grid is computed once
+    // per grid (it is not captured with the steps)
+    if (m.dinv) {
+        nnx_ctx::Buf& bd = c->bufs["para_dinv"];
+        uint64_t tag = 0xcbf29ce484222325ull;
+        const unsigned char* u = (const unsigned char*)&q.grid;
+        for (size_t i = 0; i < sizeof q.grid; ++i) tag = (tag ^ u[i]) * 1099511628211ull;
+        tag |= 1;
+        if (bd.tag != tag) {
+            NNX_CHECK(c, !c->capturing, "paracrine: grid changed during capture");
+            NNX_TRY(cg_bind(c, gk, NNX_PRECOND_JACOBI, m, &ph->h));
+            bd.tag = tag;
+            return NNX_OK;
+        }
+        m.dinv = nullptr;  // already filled: bind without relaunching
+        NNX_TRY(cg_bind(c, gk, NNX_PRECOND_NONE, m, &ph->h));
+        ph->h.d.dinv = (float*)bd.p;
+        return NNX_OK;
+    }
+    return cg_bind(c, gk, NNX_PRECOND_JACOBI, m, &ph->h);
+}
+
+nnx_status para_prepare(nnx_ctx* c, const nnx_step_plan* p) {
+    if (p->paracrine.every <= 0) return NNX_OK;
+    const GridK gk = make_grid(&p->paracrine.grid);
+    ParaHost ph;
+    return para_host(c, p->paracrine, gk, &ph);
+}
+
+// One solve: b = conc + release * counts, counts = 0, then the refinement.
+nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
+    const nnx_paracrine& q = p->paracrine;
+    const GridK gk = make_grid(&q.grid);
+    const int64_t nv = (int64_t)gk.nx * gk.ny * gk.nz;
+    ParaHost ph;
+    NNX_TRY(para_host(c, q, gk, &ph));
+    CgHost& h = ph.h;
+    const int max_inner = q.max_inner > 0 ? q.max_inner : 64;
+    const int max_outer = q.max_outer > 0 ? q.max_outer : 3;
+    const double inner_rtol = q.inner_rtol > 0 ? q.inner_rtol : 1e-6;
+    const bool fast = gk.kind == 0 && gk.nx % 4 == 0;
+    const int nb4 = stencil4_blocks(gk);
+    h.d.npq = fast ? nb4 : kRedBlocks;
+    k_para_rhs<<<kRedBlocks, 256, 0, c->stream>>>(nv, q.conc, p->voxel_counts, q.release, ph.b, ph.pbb, ph.pfl);
+    for (int o = 0; o < max_outer; ++o) {
+        k_para_res<false><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
+        k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, max_inner, ph.pfl, q.stats, 0);
+        for (int k = 0; k < max_inner; ++k) cg_iter_launch(c, gk, h, k, ph.xin, 1, fast, nb4);
+        k_para_axpy<<<flat_blocks(nv), 256, 0, c->stream>>>(nv, ph.xin, q.conc, h.d.fl, ph.pfl);
+    }
+    k_para_res<true><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
+    k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, max_inner, ph.pfl, q.stats, 1);
+    if (q.at_neurons)
+        k_para_neurons<<<flat_blocks(p->state.n), 256, 0, c->stream>>>(p->state.n, p->voxel_of, q.conc, q.at_neurons);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
+}  // namespace nnx
+
+extern "C" nnx_status nnx_cg_solve_async(nnx_ctx* c, const nnx_grid* g, const float* b, float* x,
+                                         const nnx_cg_opts* o, double* d_result) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(check_grid(c, g));
+    NNX_CHECK(c, b && x && o && d_result, "null b, x, opts or result");
+    NNX_CHECK(c, o->rtol >= 0 && o->maxit >= 0 && o->maxit <= 100000, "bad rtol/maxit");
+    NNX_CHECK(c, o->precond == NNX_PRECOND_NONE || o->precond == NNX_PRECOND_JACOBI, "precond must be NONE or JACOBI");
+    const GridK gk = make_grid(g);
+    CgMem m;
+    NNX_TRY(cg_mem_scratch(c, "cga", gk, o->precond, &m));
+    CgHost h;
+    NNX_TRY(cg_bind(c, gk, o->precond, m, &h));
+    cg_enqueue_fixed(c, gk, h, b, x, o->use_x0, o->rtol, o->maxit);
+    k_cg_result<<<1, 256, 0, c->stream>>>(h.d, d_result);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
 
 namespace {
 

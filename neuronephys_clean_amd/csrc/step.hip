@@ -30,6 +30,15 @@ static uint64_t plan_hash(const nnx_step_plan* p) {
     return h;
 }
 
+// The captured sequence fixes which steps print and which solve the
+// paracrine field: graphs are keyed on the phase of both cadences.
+uint64_t phase_key(const nnx_step_plan* p, uint64_t first_step) {
+    uint64_t k = 0;
+    if (p->readout_every > 0) k ^= (first_step % (uint64_t)p->readout_every) * 0x9e3779b97f4a7c15ull;
+    if (p->paracrine.every > 0) k ^= (first_step % (uint64_t)p->paracrine.every + 1) * 0xc2b2ae3d27d4eb4full;
+    return k;
+}
+
 static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     NNX_CHECK(c, p, "null plan");
     const nnx_hh_state& s = p->state;
@@ -53,6 +62,7 @@ static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     if (p->params.noise_std != 0.0)
         return fail(c, NNX_ERR_UNSUPPORTED, "white noise is not implemented (noise_std must be 0)");
     NNX_TRY(check_hebbian_plan(c, p));
+    NNX_TRY(para_check(c, p));
     return check_readout(c, p);
 }
 
@@ -67,6 +77,8 @@ static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, i
         NNX_TRY(readout_launch(c, p, ctr + 2 + (s & 1), 0, cur));
     // Hebbian learning on the spikes this step's coupling reads
     if (p->hebbian.window > 0) NNX_TRY(hebbian_launch(c, &p->graph, &p->hebbian, ctr + 2 + (s & 1), 0, 0, prev));
+    // paracrine solve on its cadence, from the releases of the steps before s
+    if (p->paracrine.every > 0 && s % (uint64_t)p->paracrine.every == 0) NNX_TRY(para_launch(c, p));
     if (rk4 && p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT) {
         // push the previous step's spikes into the targets' buckets; the
         // ephys kernel sums each bucket in source order before its update
@@ -112,6 +124,7 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
         NNX_TRY(event_scratch(c, p->graph.n, p->graph.nnz, &a, &b));
     }
     NNX_TRY(readout_prepare(c, p));
+    NNX_TRY(para_prepare(c, p));
     void* ctrp;
     NNX_TRY(scratch(c, "step_ctr", 64, true, &ctrp));
     int64_t* ctr = (int64_t*)ctrp;  // [0..1] raster length, [2..3] step, by parity
@@ -132,28 +145,10 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
 }
 
 static nnx_status replay(nnx_ctx* c, nnx_step_plan* p, uint64_t first_step, int n_steps, int64_t* ctr) {
-    NNX_CHECK(c, c->stream != nullptr, "graph replay needs a non-null stream");
-    // the print steps are fixed in the captured sequence: key on their phase
-    const uint64_t phase = p->readout_every > 0 ? first_step % (uint64_t)p->readout_every : 0;
-    const std::pair<const void*, int> key((const void*)(uintptr_t)(plan_hash(p) ^ (phase * 0x9e3779b97f4a7c15ull)),
-                                          n_steps * 2 + (int)(first_step & 1));
-    auto it = c->graphs.find(key);
-    if (it == c->graphs.end()) {
-        nnx_ctx::Graph g;
-        NNX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        c->capturing = 1;
-        nnx_status st = NNX_OK;
-        for (int i = 0; i < n_steps && st == NNX_OK; ++i) st = enqueue_step(c, p, first_step + i, ctr);
-        c->capturing = 0;
-        hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
-        if (st != NNX_OK) {
-            if (g.graph) (void)hipGraphDestroy(g.graph);
-            return st;
-        }
-        if (e != hipSuccess) return fail(c, NNX_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-        NNX_HIP(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
-        it = c->graphs.emplace(key, g).first;
-    }
-    NNX_HIP(c, hipGraphLaunch(it->second.exec, c->stream));
-    return NNX_OK;
+    const GraphKey key((const void*)(uintptr_t)(plan_hash(p) ^ phase_key(p, first_step)),
+                       n_steps * 2 + (int)(first_step & 1));
+    return capture_or_replay(c, key, [&]() -> nnx_status {
+        for (int i = 0; i < n_steps; ++i) NNX_TRY(enqueue_step(c, p, first_step + i, ctr));
+        return NNX_OK;
+    });
 }

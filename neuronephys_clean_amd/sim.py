@@ -131,6 +131,42 @@ class Simulation:
         from . import readout
         return readout.rows(self)
 
+    def set_paracrine(self, every: int, rtol: float = 1e-8, inner_rtol: float = 1e-6, max_inner: int = 0,
+                      max_outer: int = 0, at_neurons: bool = False, conc0=None, grid: _lib.Grid | None = None):
+        """Solves the network's paracrine field inside the step every `every`
+        steps (nnx_paracrine; needs deposit=True): b = conc + amount * counts,
+        conc = A^-1 b to an FP64 relative residual rtol.  `every` <= 0 turns
+        it off."""
+        net, a, ctx = self.net, self.a, self.ctx
+        q = self.plan.paracrine
+        if every <= 0:
+            q.every = 0
+            return
+        if "voxel_counts" not in a:
+            raise ValueError("the paracrine solve needs the release deposit (deposit=True)")
+        nvox = int(np.prod(net.grid))
+        c0 = np.zeros(nvox) if conc0 is None else np.asarray(conc0, np.float64)
+        a["conc"] = DeviceArray.from_numpy(ctx, c0)
+        a["para_stats"] = DeviceArray.zeros(ctx, 8, np.float64)
+        if at_neurons:
+            a["at_neurons"] = DeviceArray.zeros(ctx, net.n, np.float32)
+        g = grid or _lib.Grid(net.grid[0], net.grid[1], net.grid[2], _lib.STENCIL_7, net.c, 0.0, None, None, None)
+        self._para_grid = g
+        q.every = every
+        q.grid = g
+        q.release = net.amount
+        q.rtol = rtol
+        q.inner_rtol = inner_rtol
+        q.max_inner = max_inner
+        q.max_outer = max_outer
+        q.conc = a["conc"].ptr
+        q.at_neurons = a["at_neurons"].ptr if at_neurons else None
+        q.stats = a["para_stats"].ptr
+
+    def paracrine_stats(self) -> dict:
+        v = self.a["para_stats"].numpy()
+        return dict(zip(_lib.PARA_STATS, (float(x) for x in v)))
+
     def set_fused(self, block=0, tile=0, grid=0, burst=None):
         """Launch shape of the fused step (results do not depend on it)."""
         p = self.plan

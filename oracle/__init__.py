@@ -55,6 +55,13 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def set_threads(n: int) -> int:
+    """Sets the OpenMP thread count of the oracle's loops (n <= 0: all
+    processors); returns the count now in effect."""
+    lib().orc_set_threads(int(n))
+    return int(lib().orc_threads())
+
+
 def params(d) -> Params:
     """Params from a dict or any object with the PARAM_FIELDS attributes."""
     p = Params()

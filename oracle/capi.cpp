@@ -2,6 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and bench.py's
 // CPU-baseline leg, never by the product.
+#include <omp.h>
+
 #include <chrono>
 #include <cstdint>
 #include <vector>
@@ -392,4 +394,10 @@ void orc_san_f64(int64_t n, const double* c, double dt, double thr, double* cons
     san_arrays<double>(n, c, dt, thr, x, in, out, spiked);
 }
 
+}  // extern "C"
+
+extern "C" {
+// OpenMP threads of the oracle's parallel loops (n <= 0: the default)
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+int orc_threads(void) { return omp_get_max_threads(); }
 }  // extern "C"

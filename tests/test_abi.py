@@ -51,7 +51,7 @@ STRUCTS = {"nnx_hh_state": "HHState", "nnx_hh_params": "HHParams", "nnx_graph": 
            "nnx_step_plan": "StepPlan", "nnx_ens_plan": "EnsPlan", "nnx_ens_config": "EnsConfig",
            "nnx_regions": "Regions", "nnx_hebbian": "Hebbian",
            "nnx_diekman_state": "DiekmanState", "nnx_diekman_params": "DiekmanParams",
-           "nnx_san_state": "SanState", "nnx_san_params": "SanParams"}
+           "nnx_san_state": "SanState", "nnx_san_params": "SanParams", "nnx_paracrine": "Paracrine"}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
