@@ -158,7 +158,7 @@ typedef struct nnx_hh_params {
     /* leapfrog: kinetic synapses + sigmoid output (src/defines.cuh:94-101) */
     double tmax, vt, kp, ar, ad;
     double g_local, e_local, g_i, e_i, g_e, e_e;
-    double noise_mean, noise_std;   /* white noise added to the drive; std 0 = deterministic */
+    double noise_mean, noise_std;   /* white noise added to the drive (leapfrog); std 0 = deterministic */
     uint64_t noise_seed;
     /* spike rules */
     double v_thresh;                /* CROSSING threshold, mV */
@@ -180,7 +180,12 @@ void nnx_hh_params_squid(nnx_hh_params* p);
  *              (coupling output), dtype of the state; NULL = none.
  *   spike_bits out: bit i of word i/64 = neuron i spiked at this step under
  *              `spike_rule`; ceil(n/64) uint64 words; NULL = not wanted.
- *   step       global step index (keys the counter-based noise).          */
+ *   step       global step index (keys the counter-based white noise).
+ * White noise (noise_std != 0): the leapfrog step adds
+ * noise_std * z + noise_mean to the drive, z = nnx_noise_normal's draw of
+ * (noise_seed, step, neuron) (the reference's curand_normal,
+ * src/ephys.cu:82); RK4 returns NNX_ERR_UNSUPPORTED for it (the benchmark
+ * model's drive is static).                                               */
 nnx_status nnx_hh_step(nnx_ctx* ctx, const nnx_hh_state* st, const nnx_hh_params* p,
                        int integrator, int spike_rule, double dt, uint64_t step,
                        const void* g_in, uint64_t* spike_bits);
@@ -249,6 +254,18 @@ void nnx_san_params_reference(nnx_san_params* p);
  * powf(x, 3.5) is x^3 sqrt(x) and powf(x, 4) is (x^2)^2.                  */
 nnx_status nnx_san_step(nnx_ctx* ctx, const nnx_san_state* st, const nnx_san_params* p, double dt,
                         uint64_t* spike_bits);
+
+/* Standard normal draws of the leapfrog step's white noise: d_out[i] (FP64)
+ * = the draw of neuron first + i at `step` with `seed`: Philox4x64-10 at
+ * counter {step, neuron, 0x4e4f495345, 0} with key {seed, 0}, words 0 and 1
+ * as u1 = ((w0 >> 11) + 1) 2^-53 and u2 = (w1 >> 11) 2^-53, then
+ * sqrt(-2 ln u1) cos(2 pi u2) with this library's ln and cos and a correctly
+ * rounded sqrt.  The step draws noise_std * z + noise_mean (in the state's
+ * type) per neuron.  Replaces init_random + curand_normal
+ * (src/ephys.cu:4-10,82): a stateless counter-based stream instead of 48
+ * bytes of XORWOW state per neuron.                                        */
+nnx_status nnx_noise_normal(nnx_ctx* ctx, uint64_t seed, uint64_t step, int64_t first, int64_t n,
+                            double* d_out);
 
 /* Returns NNX_ERR_NONFINITE if any V is NaN/Inf (synchronous).             */
 nnx_status nnx_check_finite(nnx_ctx* ctx, int dtype, int64_t n, const void* d_x);
@@ -383,6 +400,7 @@ nnx_status nnx_cg_solve(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float
  * host, src/paracrine.cu:106-129).                                           */
 nnx_status nnx_cg_solve_async(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float* d_x,
                               const nnx_cg_opts* o, double* d_result);
+
 
 /* Mixed-precision iterative refinement: FP64 b and x, the outer residual
  * r = b - A x recomputed in FP64, inner fp32 CG on A d = r, x += d, until

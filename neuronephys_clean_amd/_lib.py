@@ -187,6 +187,7 @@ _SIGS = {
     "nnx_hh_step": ([vp, _P(HHState), _P(HHParams), C.c_int, C.c_int, C.c_double, C.c_uint64,
                      vp, vp], C.c_int),
     "nnx_check_finite": ([vp, C.c_int, C.c_int64, vp], C.c_int),
+    "nnx_noise_normal": ([vp, C.c_uint64, C.c_uint64, C.c_int64, C.c_int64, vp], C.c_int),
     "nnx_spike_threshold": ([vp, C.c_int, C.c_int64, vp, C.c_double, vp], C.c_int),
     "nnx_spike_compact": ([vp, C.c_int64, vp, vp, vp], C.c_int),
     "nnx_read_count": ([vp, vp, _P(C.c_int32)], C.c_int),

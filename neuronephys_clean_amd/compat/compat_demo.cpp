@@ -1,7 +1,11 @@
 // compat_demo.cpp - the reference's simulation loop (src/main.cu:112-181:
-// coupling -> paracrine -> ephys per increment) written against
-// nnx_compat.hpp, with inputs and outputs as raw files so that a test can
-// check the run against the oracle.
+// coupling -> paracrine -> ephys per increment, coupling through the
+// reference's own CouplingStruct over a ChunkedAdjacencyList,
+// src/main.cu:22-33,264-265) written against nnx_compat.hpp, with inputs and
+// outputs as raw files so that a test can check the run against the oracle.
+// The ephys step draws the reference's white noise (mean 3, std 1.5) unless
+// the environment sets COMPAT_DEMO_NOISE_STD (e.g. 0 for the deterministic
+// mode).
 //
 //   compat_demo DIR nincs nx ny nz h D decay par_dt timescale coup_strength
 //               [print_incs render_N k]
@@ -49,6 +53,33 @@ template <class T> static void save(const std::string& path, const std::vector<T
     std::fclose(f);
 }
 
+// src/main.cu:22-33, unchanged
+struct CouplingStruct 
+{
+    ChunkedAdjacencyList& list;
+    AtomicCoupling& coupling;
+    Float* downstream_inputs;
+    Float* neuron_outputs;
+    
+    // Performs coupling and stores results in downstream_inputs. 
+    // Returns number of neurons firing.
+    int operator() () 
+    {
+        return coupling(list, downstream_inputs, neuron_outputs);
+    }
+};
+
+// the network of the input files (the reference generates one on the host,
+// e.g. UniformRandomNetwork, src/network_factory.cu:96-131)
+class CsrFileNetwork : public Network {
+   public:
+    CsrFileNetwork(const std::vector<int>& off, const std::vector<int>& tgt) {
+        offset_in_indices = off;
+        indices = tgt;
+    }
+    bool Generate(int const) override { return true; }
+};
+
 static void record(const AtomicCoupling& c, int n_firing, int step, std::vector<int32_t>& raster) {
     if (step < 0 || n_firing == 0) return;
     std::vector<int32_t> l = c.GetFiringNeurons().host();
@@ -83,7 +114,17 @@ int main(int argc, char** argv) {
         if (N <= 0 || sv.size() != (size_t)NUM_SIM * N || pos.size() != 3 * (size_t)N)
             throw std::runtime_error("input sizes do not match");
 
-        AdjacencyList list(off, tgt);
+        std::vector<DeviceVector<Float>> vars;
+        std::vector<Float*> ptrs;
+        for (int k = 0; k < NUM_SIM; ++k) {
+            vars.emplace_back(std::vector<Float>(sv.begin() + (size_t)k * N, sv.begin() + (size_t)(k + 1) * N));
+            ptrs.push_back(vars.back().data());
+        }
+        DeviceVector<Float*> sim_vars(ptrs);
+        // connectivity and in-degree as src/main.cu:264-265 builds them
+        CsrFileNetwork network(off, tgt);
+        ChunkedAdjacencyList chunked_list = network.ToChunkedAdjacencyList(ptrs[SIM_LOCAL_COUNT]);
+        AdjacencyList& list = chunked_list.list;  // the unchunked list, for Hebbian learning
         std::unique_ptr<Hebbian> heb;
         {
             FILE* wf = std::fopen((dir + "/w.f32").c_str(), "rb");
@@ -93,13 +134,6 @@ int main(int argc, char** argv) {
                 heb = std::make_unique<Hebbian>(20, 0.05f, (int)tgt.size());
             }
         }
-        std::vector<DeviceVector<Float>> vars;
-        std::vector<Float*> ptrs;
-        for (int k = 0; k < NUM_SIM; ++k) {
-            vars.emplace_back(std::vector<Float>(sv.begin() + (size_t)k * N, sv.begin() + (size_t)(k + 1) * N));
-            ptrs.push_back(vars.back().data());
-        }
-        DeviceVector<Float*> sim_vars(ptrs);
         DeviceVector<Float> px(std::vector<Float>(pos.begin(), pos.begin() + N));
         DeviceVector<Float> py(std::vector<Float>(pos.begin() + N, pos.begin() + 2 * N));
         DeviceVector<Float> pz(std::vector<Float>(pos.begin() + 2 * N, pos.end()));
@@ -107,7 +141,9 @@ int main(int argc, char** argv) {
         Float* g_par = g_par_vec.data();
 
         HHEphys ephys(N);
+        if (const char* ns = std::getenv("COMPAT_DEMO_NOISE_STD")) ephys.params().noise_std = std::atof(ns);
         AtomicCoupling coupling(N);
+        std::vector<CouplingStruct> coupling_structs{{chunked_list, coupling, ptrs[SIM_LOCAL_IN], ptrs[SIM_LOCAL_OUTPUT]}};
         ParacrineModule paracrine(g_par, N, nx, ny, nz, par_dt, timescale, coup, h, D, decay,
                                   px.data(), py.data(), pz.data());
 
@@ -137,7 +173,10 @@ int main(int argc, char** argv) {
                 erows.insert(erows.end(), avg, avg + render_N);
                 erows.push_back(ElectrodeNet::MeasureAverageEntireNetwork(ptrs[SIM_V], N));
             }
-            const int n_firing = coupling(list, ptrs[SIM_LOCAL_IN], ptrs[SIM_LOCAL_OUTPUT]);
+            // Perform coupling. The number of neurons firing is returned by each of these calls.
+            int n_firing;
+            for(auto& coup_struct: coupling_structs)
+                n_firing = coup_struct();
             nfiring.push_back(n_firing);
             record(coupling, n_firing, i - 1, raster);
             if (i % paracrine.timescale == 0)
@@ -154,7 +193,7 @@ int main(int argc, char** argv) {
         }
         // the firing of the last increment (the coupling input it writes is
         // not part of sv_out)
-        record(coupling, coupling(list, ptrs[SIM_LOCAL_IN], ptrs[SIM_LOCAL_OUTPUT]), nincs - 1, raster);
+        record(coupling, coupling_structs[0](), nincs - 1, raster);
         if (electrodes) save(dir + "/electrodes.f32", erows);
         if (heb) save(dir + "/w_out.f32", list.weights.host());
         save(dir + "/sv_out.f32", out);

@@ -12,6 +12,8 @@
 //   AtomicCoupling(int, AdjacencyList&, Float*, Float*), PerformCoupling()
 //                                               src_wholebrain/coupling.cuh:26-43
 //   AdjacencyList                               src/coupling.cuh:4-8, src_wholebrain/coupling.cuh:11-16
+//   ChunkedAdjacencyList, chunked operator()    src/coupling.cuh:10-14,37, src/coupling.cu:143-178
+//   Network::ToChunkedAdjacencyList(Float*&)    src/network_factory.cuh:38, src/network_factory.cu:29-85
 //   Density_on_grid_3d                          src/paracrine.cuh:121-173
 //   Scatter_grid_converter_3d                   src/paracrine.cuh:184-209
 //   ParacrineModule                             src/paracrine.cuh:212-267
@@ -28,8 +30,13 @@
 //     data()/size() use); ParacrineModule takes coordinate arrays as device
 //     pointers; density() returns the device pointer of the field.
 //   - errors throw nnx_compat::Error (the reference prints and exits).
-//   - the white noise of Leapfrog_HH (cuRAND, src/ephys.cu:82) is replaced
-//     by its mean: the step is deterministic.
+//   - the white noise of Leapfrog_HH (cuRAND XORWOW, src/ephys.cu:82) is
+//     drawn by the library's counter-based generator (Philox4x64-10 keyed by
+//     seed, call number and neuron; Box-Muller): same distribution, other
+//     numbers; noise_std = 0 gives the deterministic mode.
+//   - the chunked coupling is computed without chunks (same counts; see
+//     AtomicCoupling), so it has no N_PER_CHUNK shared-memory limit of its
+//     own, but ToChunkedAdjacencyList keeps the reference's layout and cap.
 //   - update_on_fine_timescale adds the release to the firing neurons (the
 //     reference's transform_if uses the compacted index list as its stencil,
 //     src/paracrine.cuh:250-252, which selects list positions, not neurons).
@@ -47,7 +54,6 @@
 #include <array>
 #include <cstdio>
 #include <cstring>
-#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -184,6 +190,24 @@ struct AdjacencyList {
     }
 };
 
+// Launch and chunking constants of the reference (src/defines.cuh:81-84);
+// the chunked list's layout caps N at NCHUNKS * N_PER_CHUNK as there.
+constexpr int NBLOCKS = 32, NTHREADS = 64, NCHUNKS = 2, N_PER_CHUNK = 7319;
+
+// The reference's chunked out-edge lists (src/coupling.cuh:10-14): for each
+// target chunk c (target / N_PER_CHUNK), a CSR over all sources whose
+// entries are the targets of that chunk relative to its first neuron.
+// indices_chunks / offset_in_indices_chunks are device tables of device
+// pointers, as there.  Network::ToChunkedAdjacencyList builds it and also
+// keeps the unchunked list (`list`), which the library's deterministic
+// coupling reads.
+struct ChunkedAdjacencyList {
+    DeviceVector<int*> indices_chunks;
+    DeviceVector<int*> offset_in_indices_chunks;
+    std::vector<DeviceVector<int>> chunk_indices, chunk_offsets;  // the arrays the tables point to
+    AdjacencyList list;
+};
+
 // --------------------------------------------------------------- ephys ----
 class Ephys {
    protected:
@@ -198,25 +222,24 @@ class Ephys {
 // Leapfrog_HH (src/ephys.cu:28-96) at hh_params::leapfrog_dt = 0.4 ms.
 // sim_vars is a device array of NUM_SIM device pointers, as the reference's
 // thrust::raw_pointer_cast(sim_vars.data()).
+// The white noise white_noise_intensity * N(0,1) + white_noise_mean
+// (src/ephys.cu:82, src/defines.cuh:99-101) is drawn per (seed, call,
+// neuron) by the library's counter-based generator; params().noise_std = 0
+// gives the deterministic mode.
 class HHEphys : public Ephys {
-    std::map<Float**, std::vector<Float*>> tables_;
     nnx_hh_params p_{};
+    uint64_t step_ = 0;
 
    public:
     static constexpr Float leapfrog_dt = 0.4f;  // src/defines.cuh:89
-    explicit HHEphys(int const N_) : Ephys(N_) {
-        nnx_hh_params_reference(&p_);
-        p_.noise_std = 0.0;
-    }
+    explicit HHEphys(int const N_) : Ephys(N_) { nnx_hh_params_reference(&p_); }
     nnx_hh_params& params() { return p_; }
+    uint64_t steps_done() const { return step_; }
     void SimulateEphys(Float** sim_vars) override {
-        auto it = tables_.find(sim_vars);
-        if (it == tables_.end()) {
-            std::vector<Float*> h(NUM_SIM);
-            check(nnx_memcpy_d2h(ctx(), h.data(), sim_vars, NUM_SIM * sizeof(Float*)));
-            it = tables_.emplace(sim_vars, h).first;
-        }
-        const std::vector<Float*>& a = it->second;
+        // the pointer table is read on every call, as the reference's kernel
+        // reads it on every launch
+        std::vector<Float*> a(NUM_SIM);
+        check(nnx_memcpy_d2h(ctx(), a.data(), sim_vars, NUM_SIM * sizeof(Float*)));
         nnx_hh_state st{};
         st.dtype = NNX_F32;
         st.n = N;
@@ -237,7 +260,7 @@ class HHEphys : public Ephys {
         st.out_local = a[SIM_LOCAL_OUTPUT];
         st.out_i = a[SIM_IOUTPUT];
         st.out_e = a[SIM_EOUTPUT];
-        check(nnx_hh_step(ctx(), &st, &p_, NNX_INT_LEAPFROG, NNX_SPIKE_LEVEL, leapfrog_dt, 0, nullptr,
+        check(nnx_hh_step(ctx(), &st, &p_, NNX_INT_LEAPFROG, NNX_SPIKE_LEVEL, leapfrog_dt, step_++, nullptr,
                           nullptr));
         check(nnx_synchronize(ctx()));  // the reference synchronises after the launch
     }
@@ -255,20 +278,16 @@ enum DiekmanVars {
 // DiekmanEphys::SimulateEphys launches it; sim_vars is a device array of
 // NUM_DIEKMAN_VAR device pointers.
 class DiekmanEphys : public Ephys {
-    std::map<Float**, std::vector<Float*>> tables_;
     nnx_diekman_params p_{};
 
    public:
     explicit DiekmanEphys(int const N_) : Ephys(N_) { nnx_diekman_params_reference(&p_); }
     nnx_diekman_params& params() { return p_; }
     void SimulateEphys(Float** sim_vars) override {
-        auto it = tables_.find(sim_vars);
-        if (it == tables_.end()) {
-            std::vector<Float*> h(NUM_DIEKMAN_VAR);
-            check(nnx_memcpy_d2h(ctx(), h.data(), sim_vars, NUM_DIEKMAN_VAR * sizeof(Float*)));
-            it = tables_.emplace(sim_vars, h).first;
-        }
-        const std::vector<Float*>& a = it->second;
+        // the pointer table is read on every call (the reference's kernel
+        // reads it on every launch)
+        std::vector<Float*> a(NUM_DIEKMAN_VAR);
+        check(nnx_memcpy_d2h(ctx(), a.data(), sim_vars, NUM_DIEKMAN_VAR * sizeof(Float*)));
         nnx_diekman_state st{};
         st.dtype = NNX_F32;
         st.n = N;
@@ -291,20 +310,16 @@ enum AnIeVars {
 // Euler_an_ie_model at hh_params::leapfrog_dt (0.4 ms), as
 // SANEphys::SimulateEphys launches it (white noise replaced by its mean).
 class SANEphys : public Ephys {
-    std::map<Float**, std::vector<Float*>> tables_;
     nnx_san_params p_{};
 
    public:
     explicit SANEphys(int const N_) : Ephys(N_) { nnx_san_params_reference(&p_); }
     nnx_san_params& params() { return p_; }
     void SimulateEphys(Float** sim_vars) override {
-        auto it = tables_.find(sim_vars);
-        if (it == tables_.end()) {
-            std::vector<Float*> h(NUM_AN_IE_VAR);
-            check(nnx_memcpy_d2h(ctx(), h.data(), sim_vars, NUM_AN_IE_VAR * sizeof(Float*)));
-            it = tables_.emplace(sim_vars, h).first;
-        }
-        const std::vector<Float*>& a = it->second;
+        // the pointer table is read on every call (the reference's kernel
+        // reads it on every launch)
+        std::vector<Float*> a(NUM_AN_IE_VAR);
+        check(nnx_memcpy_d2h(ctx(), a.data(), sim_vars, NUM_AN_IE_VAR * sizeof(Float*)));
         nnx_san_state st{};
         st.dtype = NNX_F32;
         st.n = N;
@@ -353,6 +368,18 @@ class AtomicCoupling {
     }
     int operator()(const AdjacencyList& connectivity, Float* spikes, const Float* outputs) {
         return couple(connectivity, spikes, outputs, NNX_COUPLE_COUNT);
+    }
+    // The chunked form (src/coupling.cu:143-178) merges per-block shared-
+    // memory histograms of the firing sources' targets into `spikes`: the
+    // number of firing sources of each target, the counts the unchunked
+    // form produces (integer sums do not depend on the order).  Here they
+    // come from the deterministic COUNT coupling over the unchunked list,
+    // without the reference's shared-memory race (src/coupling.cu:63-83).
+    // `spikes` is overwritten on every call; the reference leaves it
+    // untouched until the first call that finds firing neurons and adds that
+    // call's counts to it (the same when it starts at zero).
+    int operator()(const ChunkedAdjacencyList& connectivity, Float* spikes, const Float* outputs) {
+        return couple(connectivity.list, spikes, outputs, NNX_COUPLE_COUNT);
     }
     int PerformCoupling() { return couple(list, downstream_inputs, neuron_outputs, NNX_COUPLE_SIGNAL); }
     const DeviceVector<int32_t>& GetFiringNeurons() const { return firing_neurons; }
@@ -645,6 +672,41 @@ class Network {
         const int64_t N = (int64_t)offset_in_indices.size() - 1;
         if (N > 0) check(nnx_memcpy_d2d(ctx(), Tcount, list.in_degree(), N * sizeof(Float)));
         return list;
+    }
+    // The chunked lists of src/network_factory.cu:29-85 (targets split by
+    // target / N_PER_CHUNK, each chunk's entries relative to its first
+    // neuron, one offset array of N+1 per chunk), the in-degree written into
+    // Tcount as ToAdjacencyList does.
+    ChunkedAdjacencyList ToChunkedAdjacencyList(Float*& Tcount) const {
+        ChunkedAdjacencyList cl;
+        const int64_t N = (int64_t)offset_in_indices.size() - 1;
+        if (N < 0) {
+            std::fprintf(stderr, "Connectivity is empty. Returning blank list.\n");
+            return cl;
+        }
+        std::vector<std::vector<int>> ind(NCHUNKS), off(NCHUNKS, std::vector<int>(N + 1, 0));
+        for (int64_t i = 0; i < N; ++i) {
+            for (auto& o : off) o[i + 1] = o[i];
+            for (int j = offset_in_indices[i]; j < offset_in_indices[i + 1]; ++j) {
+                const int t = indices[j], c = t / N_PER_CHUNK;
+                if (t < 0 || c >= NCHUNKS)
+                    throw Error(NNX_ERR_ARG, "target " + std::to_string(t) + " lies beyond the " +
+                                                 std::to_string(NCHUNKS) + " chunks of " + std::to_string(N_PER_CHUNK));
+                ind[c].push_back(t % N_PER_CHUNK);
+                off[c][i + 1] += 1;
+            }
+        }
+        std::vector<int*> pi(NCHUNKS), po(NCHUNKS);
+        for (int c = 0; c < NCHUNKS; ++c) {
+            cl.chunk_indices.emplace_back(ind[c].empty() ? std::vector<int>(1, 0) : ind[c]);
+            cl.chunk_offsets.emplace_back(off[c]);
+            pi[c] = cl.chunk_indices.back().data();
+            po[c] = cl.chunk_offsets.back().data();
+        }
+        cl.indices_chunks = DeviceVector<int*>(pi);
+        cl.offset_in_indices_chunks = DeviceVector<int*>(po);
+        cl.list = ToAdjacencyList(Tcount);
+        return cl;
     }
     const std::vector<int>& GetCpuIndices() const { return indices; }
     const std::vector<int>& GetCpuOffsetsIntoIndices() const { return offset_in_indices; }

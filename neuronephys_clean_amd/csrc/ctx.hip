@@ -104,6 +104,7 @@ template <class T> HHConst<T> make_const(const nnx_hh_params* q, double dt) {
     p.g_e = T(q->g_e);
     p.e_e = T(q->e_e);
     p.noise_mean = T(q->noise_mean);
+    p.noise_std = T(q->noise_std);
     p.v_thresh = T(q->v_thresh);
     p.out_thresh = q->out_thresh;
     p.dt = T(dt);

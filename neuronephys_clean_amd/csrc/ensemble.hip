@@ -283,7 +283,7 @@ nnx_status check_ens(nnx_ctx* c, const nnx_ens_plan* p) {
     if (p->raster) NNX_CHECK(c, p->raster_len && p->raster_cap >= 0, "raster needs raster_len");
     if (p->voxel_of) NNX_CHECK(c, p->voxel_counts, "voxel_of needs voxel_counts");
     if (p->params.noise_std != 0.0)
-        return fail(c, NNX_ERR_UNSUPPORTED, "white noise is not implemented (noise_std must be 0)");
+        return fail(c, NNX_ERR_UNSUPPORTED, "white noise (noise_std != 0) is drawn by the leapfrog step only");
     return NNX_OK;
 }
 

@@ -8,6 +8,7 @@
 // the grid covers N (4096 blocks of 256 at N = 1M) and nothing synchronises.
 #include "coupling_dev.h"
 #include "internal.h"
+#include "noise.h"
 
 namespace nnx {
 
@@ -75,14 +76,30 @@ template <class T> struct LeapPtr {
 
 template <class T> __device__ __forceinline__ T ld0(const T* p, int64_t i) { return p ? p[i] : T(0); }
 
-// src/ephys.cu:57-94 (deterministic: the white noise is its mean).
+// White noise of the step: noise_std z + noise_mean with z the standard
+// normal of (seed, step, neuron) (noise.h); the mean alone when noise_std
+// is 0 (the deterministic mode).  The step number comes from *stepp when
+// given (the device step counter of a captured step sequence), else `step`.
+struct NoiseArg {
+    uint64_t seed;
+    uint64_t step;
+    const int64_t* stepp;
+    int on;
+};
+
+// src/ephys.cu:57-94
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_hh_leapfrog(int64_t n, HHConst<T> p, LeapPtr<T> a,
                                                         int level, uint64_t* __restrict__ bits,
-                                                        int64_t nwords) {
+                                                        int64_t nwords, NoiseArg nz) {
     const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     bool spike = false;
     if (i < n) {
+        T wn = p.noise_mean;
+        if (nz.on) {
+            const uint64_t step = nz.stepp ? (uint64_t)*nz.stepp : nz.step;
+            wn = p.noise_std * T(noise_normal(nz.seed, step, (uint64_t)i)) + p.noise_mean;
+        }
         const T V = a.V[i], M = a.M[i], NG = a.NG[i], H = a.H[i];
         const Rates<T> R = rates(V + p.vshift);
         const T dt = p.dt, hdt = p.hdt;
@@ -98,8 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_hh_leapfrog(int64_t n, HHConst<T> p,
         const T G = m3h + n4 + p.gl + p.g_local * nYl + p.g_i * nYi + p.g_e * nYe;
         const T E = m3h * p.ena + n4 * p.ek + p.gl * p.el + p.g_local * nYl * p.e_local +
                     p.g_i * nYi * p.e_i + p.g_e * nYe * p.e_e;
-        const T Vn = ((p.noise_mean + ld0(a.appcur, i)) * dt + E * dt + (T(1) - hdt * G) * V) /
-                     (T(1) + hdt * G);
+        const T Vn = ((wn + ld0(a.appcur, i)) * dt + E * dt + (T(1) - hdt * G) * V) / (T(1) + hdt * G);
         const T out = p.tmax / (T(1) + dexp(-(Vn - p.vt) / p.kp));
         a.V[i] = Vn;
         a.M[i] = nM;
@@ -123,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_hh_leapfrog(int64_t n, HHConst<T> p,
 template <class T>
 static nnx_status hh_step_t(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q,
                             int integrator, int rule, double dt, const void* g_in,
-                            uint64_t* bits) {
+                            uint64_t* bits, uint64_t step, const int64_t* stepp) {
     const int64_t n = st->n;
     const int64_t nwords = ceil_div(n, 64);
     const unsigned blocks = (unsigned)ceil_div(n, kBlock);
@@ -157,8 +173,9 @@ static nnx_status hh_step_t(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_par
         a.in_e = (const T*)st->in_e;
         a.cnt_e = (const T*)st->count_e;
         a.appcur = (const T*)st->appcur;
+        NoiseArg nz{q->noise_seed, step, stepp, q->noise_std != 0.0};
         k_hh_leapfrog<T><<<blocks, kBlock, 0, c->stream>>>(n, p, a, rule == NNX_SPIKE_LEVEL, bits,
-                                                           nwords);
+                                                           nwords, nz);
     }
     NNX_LAUNCHED(c);
     return NNX_OK;
@@ -196,24 +213,43 @@ nnx_status hh_rk4_event_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_
 
 using namespace nnx;
 
-extern "C" nnx_status nnx_hh_step(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q,
-                                  int integrator, int rule, double dt, uint64_t step,
-                                  const void* g_in, uint64_t* bits) {
-    (void)step;
-    if (!c) return NNX_ERR_ARG;
+__global__ void k_noise_normal(uint64_t seed, uint64_t step, int64_t first, int64_t n, double* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = noise_normal(seed, step, (uint64_t)(first + i));
+}
+
+nnx_status nnx::hh_step_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q, int integrator, int rule,
+                               double dt, uint64_t step, const int64_t* stepp, const void* g_in, uint64_t* bits) {
     NNX_CHECK(c, st && q, "null state or params");
     NNX_CHECK(c, st->n >= 0 && st->n < (int64_t)1 << 31, "n out of range: %lld", (long long)st->n);
     NNX_CHECK(c, dt > 0, "dt must be > 0");
     NNX_CHECK(c, integrator == NNX_INT_RK4 || integrator == NNX_INT_LEAPFROG, "bad integrator %d",
               integrator);
     NNX_CHECK(c, rule == NNX_SPIKE_LEVEL || rule == NNX_SPIKE_CROSSING, "bad spike rule %d", rule);
-    if (q->noise_std != 0.0)
-        return fail(c, NNX_ERR_UNSUPPORTED,
-                    "white noise (noise_std != 0) is not implemented; deterministic mode uses its mean");
+    if (q->noise_std != 0.0 && integrator != NNX_INT_LEAPFROG)
+        return fail(c, NNX_ERR_UNSUPPORTED, "white noise (noise_std != 0) is drawn by the leapfrog step only");
     if (st->n == 0) return NNX_OK;
-    if (st->dtype == NNX_F32) return hh_step_t<float>(c, st, q, integrator, rule, dt, g_in, bits);
-    if (st->dtype == NNX_F64) return hh_step_t<double>(c, st, q, integrator, rule, dt, g_in, bits);
+    if (st->dtype == NNX_F32) return hh_step_t<float>(c, st, q, integrator, rule, dt, g_in, bits, step, stepp);
+    if (st->dtype == NNX_F64) return hh_step_t<double>(c, st, q, integrator, rule, dt, g_in, bits, step, stepp);
     return fail(c, NNX_ERR_ARG, "bad dtype %d", st->dtype);
+}
+
+extern "C" nnx_status nnx_hh_step(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q,
+                                  int integrator, int rule, double dt, uint64_t step,
+                                  const void* g_in, uint64_t* bits) {
+    if (!c) return NNX_ERR_ARG;
+    return hh_step_launch(c, st, q, integrator, rule, dt, step, nullptr, g_in, bits);
+}
+
+extern "C" nnx_status nnx_noise_normal(nnx_ctx* c, uint64_t seed, uint64_t step, int64_t first, int64_t n,
+                                       double* d_out) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_CHECK(c, n >= 0 && first >= 0 && (d_out || n == 0), "bad range or null output");
+    if (n == 0) return NNX_OK;
+    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 256), 8192);
+    k_noise_normal<<<blocks, 256, 0, c->stream>>>(seed, step, first, n, d_out);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
 }
 
 extern "C" nnx_status nnx_check_finite(nnx_ctx* c, int dtype, int64_t n, const void* x) {

@@ -120,7 +120,7 @@ template <class T> __device__ __forceinline__ Rates<T> rates(T u) {
 // Model constants in T (cast once on the host from the double parameters).
 template <class T> struct HHConst {
     T C, invC, gna, gk, gl, ena, ek, el, vshift, esyn;
-    T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean;
+    T tmax, vt, kp, ar, ad, g_local, e_local, g_i, e_i, g_e, e_e, noise_mean, noise_std;
     T v_thresh;
     // the level rule compares the output, widened, with this double (the
     // reference's CheckFiringOperator tests a Float against the literal 0.1)

@@ -164,6 +164,32 @@ nnx_status hebbian_launch(nnx_ctx* c, const nnx_graph* g, const nnx_hebbian* h, 
                           int64_t step, const uint64_t* fire);
 nnx_status check_hebbian_plan(nnx_ctx* c, const nnx_step_plan* p);
 
+// Device memory of one CG working set (paracrine.hip): four n-vectors (r,
+// p ping-pong, q), the Jacobi inverse diagonal (n floats, variable
+// coefficients with Jacobi only), 64 scalars, 64 flags and the reduction
+// partials (kCgPartDoubles).
+struct CgMem {
+    float *r, *pa, *pb, *q, *dinv;
+    double* sc;
+    int32_t* fl;
+    double* part;
+};
+constexpr size_t kCgPartDoubles = (size_t)1024 * 16;
+// bytes of a working set for grid g (256-byte aligned pieces) and its
+// layout inside `base` (256-byte aligned, cg_mem_bytes long)
+size_t cg_mem_bytes(const nnx_grid* g, int precond);
+CgMem cg_mem_carve(void* base, const nnx_grid* g, int precond);
+// argument checks and the launches of nnx_cg_solve_async on a working set
+nnx_status cg_async_check(nnx_ctx* c, const nnx_grid* g, const float* b, const float* x, const nnx_cg_opts* o,
+                          const double* d_result);
+nnx_status cg_async_launch(nnx_ctx* c, const nnx_grid* g, const float* b, float* x, const nnx_cg_opts* o,
+                           double* d_result, const CgMem& m);
+
+// nnx_hh_step with the step number read from the device (stepp, when not
+// NULL) - the white noise of a captured step sequence (ephys.hip)
+nnx_status hh_step_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q, int integrator, int rule,
+                          double dt, uint64_t step, const int64_t* stepp, const void* g_in, uint64_t* bits);
+
 // the paracrine solve inside nnx_run_steps (paracrine.hip): plan check,
 // scratch set-up before a capture, and the launches of one solve
 nnx_status para_check(nnx_ctx* c, const nnx_step_plan* p);

@@ -458,18 +458,38 @@ struct CgHost {
     double* out2;
 };
 
-// Device memory of one CG working set: four n-vectors (r, p ping-pong, q),
-// the Jacobi inverse diagonal (n floats, variable coefficients only), 64
-// scalars, 64 flags and the reduction partials (kCgPartDoubles).
-struct CgMem {
-    float *r, *pa, *pb, *q, *dinv;
-    double* sc;
-    int32_t* fl;
-    double* part;
-};
-constexpr size_t kCgPartDoubles = (size_t)kRedBlocks * 16;
+static_assert(kCgPartDoubles == (size_t)kRedBlocks * 16, "partials region");
 
 static bool cg_needs_dinv(const GridK& gk, int precond) { return precond == NNX_PRECOND_JACOBI && gk.kind == 1; }
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t cg_mem_bytes(const nnx_grid* g, int precond) {
+    const GridK gk = make_grid(g);
+    const size_t n = (size_t)gk.nx * gk.ny * gk.nz;
+    const size_t vec = align256(n * 4);
+    return align256(kCgPartDoubles * 8) + 512 + 256 + 256 + 4 * vec + (cg_needs_dinv(gk, precond) ? vec : 0);
+}
+
+CgMem cg_mem_carve(void* base, const nnx_grid* g, int precond) {
+    const GridK gk = make_grid(g);
+    const size_t n = (size_t)gk.nx * gk.ny * gk.nz;
+    const size_t vec = align256(n * 4);
+    char* p = (char*)base;
+    CgMem m;
+    m.part = (double*)p;
+    p += align256(kCgPartDoubles * 8);
+    m.sc = (double*)p;
+    p += 512;
+    m.fl = (int32_t*)p;
+    p += 512;
+    m.r = (float*)p, p += vec;
+    m.pa = (float*)p, p += vec;
+    m.pb = (float*)p, p += vec;
+    m.q = (float*)p, p += vec;
+    m.dinv = cg_needs_dinv(gk, precond) ? (float*)p : nullptr;
+    return m;
+}
 
 // the working set from the context's named scratch buffers `prefix`_*
 static nnx_status cg_mem_scratch(nnx_ctx* c, const char* prefix, const GridK& gk, int precond, CgMem* m) {
@@ -1137,22 +1157,35 @@ nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
 
 }  // namespace nnx
 
-extern "C" nnx_status nnx_cg_solve_async(nnx_ctx* c, const nnx_grid* g, const float* b, float* x,
-                                         const nnx_cg_opts* o, double* d_result) {
-    if (!c) return NNX_ERR_ARG;
+nnx_status nnx::cg_async_check(nnx_ctx* c, const nnx_grid* g, const float* b, const float* x, const nnx_cg_opts* o,
+                               const double* d_result) {
     NNX_TRY(check_grid(c, g));
     NNX_CHECK(c, b && x && o && d_result, "null b, x, opts or result");
     NNX_CHECK(c, o->rtol >= 0 && o->maxit >= 0 && o->maxit <= 100000, "bad rtol/maxit");
     NNX_CHECK(c, o->precond == NNX_PRECOND_NONE || o->precond == NNX_PRECOND_JACOBI, "precond must be NONE or JACOBI");
+    return NNX_OK;
+}
+
+// Enqueues the capturable CG on the working set m (no allocation).
+nnx_status nnx::cg_async_launch(nnx_ctx* c, const nnx_grid* g, const float* b, float* x, const nnx_cg_opts* o,
+                                double* d_result, const CgMem& m) {
     const GridK gk = make_grid(g);
-    CgMem m;
-    NNX_TRY(cg_mem_scratch(c, "cga", gk, o->precond, &m));
     CgHost h;
     NNX_TRY(cg_bind(c, gk, o->precond, m, &h));
     cg_enqueue_fixed(c, gk, h, b, x, o->use_x0, o->rtol, o->maxit);
     k_cg_result<<<1, 256, 0, c->stream>>>(h.d, d_result);
     NNX_LAUNCHED(c);
     return NNX_OK;
+}
+
+extern "C" nnx_status nnx_cg_solve_async(nnx_ctx* c, const nnx_grid* g, const float* b, float* x,
+                                         const nnx_cg_opts* o, double* d_result) {
+    if (!c) return NNX_ERR_ARG;
+    NNX_TRY(cg_async_check(c, g, b, x, o, d_result));
+    const GridK gk = make_grid(g);
+    CgMem m;
+    NNX_TRY(cg_mem_scratch(c, "cga", gk, o->precond, &m));
+    return cg_async_launch(c, g, b, x, o, d_result, m);
 }
 
 namespace {

@@ -59,8 +59,8 @@ static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     }
     if (p->record) NNX_CHECK(c, p->raster && p->raster_len && p->raster_cap >= 0, "record needs raster buffers");
     if (p->voxel_of) NNX_CHECK(c, p->voxel_counts, "voxel_of needs voxel_counts");
-    if (p->params.noise_std != 0.0)
-        return fail(c, NNX_ERR_UNSUPPORTED, "white noise is not implemented (noise_std must be 0)");
+    if (p->params.noise_std != 0.0 && p->integrator != NNX_INT_LEAPFROG)
+        return fail(c, NNX_ERR_UNSUPPORTED, "white noise (noise_std != 0) is drawn by the leapfrog step only");
     NNX_TRY(check_hebbian_plan(c, p));
     NNX_TRY(para_check(c, p));
     return check_readout(c, p);
@@ -89,8 +89,10 @@ static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, i
         void* target = rk4 ? p->g_in : (void*)p->state.in_local;
         NNX_TRY(couple_launch(c, &p->graph, p->couple_mode, p->state.dtype, prev, p->list,
                               p->count, p->state.out_local, target));
-        NNX_TRY(nnx_hh_step(c, &p->state, &p->params, p->integrator, p->spike_rule, p->dt, s,
-                            rk4 ? p->g_in : nullptr, cur));
+        // the step number of the noise from the device counter (a captured
+        // sequence is replayed at other steps)
+        NNX_TRY(hh_step_launch(c, &p->state, &p->params, p->integrator, p->spike_rule, p->dt, s, ctr + 2 + (s & 1),
+                               rk4 ? p->g_in : nullptr, cur));
     }
     CompactExtras ex{};
     if (p->record) {

@@ -201,13 +201,16 @@ class ReferenceSimulation:
 
     def __init__(self, net: Network, sv0: dict, dtype=np.float32, couple: str = "count",
                  record_cap: int = 0, params: HHParams | None = None, dt: float = 0.4,
-                 ctx: _lib.Context | None = None):
+                 ctx: _lib.Context | None = None, noise: bool = False):
+        """noise: the reference's white noise (std 1.5, mean 3) drawn per
+        (seed, step, neuron); otherwise its mean (the deterministic mode)."""
         self.ctx = ctx or _lib.context()
         ctx = self.ctx
         self.T = np.dtype(dtype)
         n = net.n
         self.params = params or _lib.params_reference()
-        self.params.noise_std = 0.0
+        if not noise:
+            self.params.noise_std = 0.0
         self.sv = {k: DeviceArray.from_numpy(ctx, np.asarray(sv0.get(k, np.zeros(n)), self.T))
                    for k in SOA}
         self.graph, self.g = upload_graph(ctx, net)
@@ -253,9 +256,13 @@ class ReferenceSimulation:
         self.plan = p
         self.step = 0
 
-    def run(self, n_steps: int, use_graph: bool = False):
-        self.ctx.call("nnx_run_steps", C.byref(self.plan), self.step, n_steps, int(use_graph))
-        self.step += n_steps
+    def run(self, n_steps: int, use_graph: bool = False, chunk: int | None = None):
+        done = 0
+        while done < n_steps:
+            k = n_steps - done if not chunk else min(chunk, n_steps - done)
+            self.ctx.call("nnx_run_steps", C.byref(self.plan), self.step, k, int(use_graph))
+            self.step += k
+            done += k
 
     def set_readout(self, offsets, idx, every: int, cap: int, firing: bool = True, mean: bool = True):
         """Electrode readout every `every` steps into device rows (readout.attach)."""

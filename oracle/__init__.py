@@ -105,14 +105,47 @@ def hh_rk4(p, dt, V, m, h, ng, s, iext, gin=None):
     return spk
 
 
-def hh_leapfrog(p, dt, sv):
-    """One reference leapfrog step over the 17 SoA arrays (list, in place)."""
+def hh_leapfrog(p, dt, sv, step=0):
+    """One reference leapfrog step over the 17 SoA arrays (list, in place);
+    `step` keys the white noise when p.noise_std != 0."""
     n = sv[0].size
     spk = np.zeros(n, np.uint8)
     arr = (C.c_void_p * 17)(*[a.ctypes.data for a in sv])
     getattr(lib(), "orc_hh_leapfrog_" + _sfx(sv[0].dtype))(C.c_int64(n), C.byref(p),
-                                                            C.c_double(dt), arr, _p(spk))
+                                                            C.c_double(dt), arr, _p(spk),
+                                                            C.c_int64(step))
     return spk
+
+
+def philox4x64(counter, key):
+    """The four 64-bit words of Philox4x64-10 at counter (4 words), key (2)."""
+    c = np.ascontiguousarray(counter, np.uint64)
+    k = np.ascontiguousarray(key, np.uint64)
+    out = np.zeros(4, np.uint64)
+    lib().orc_philox4x64_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def noise_log(u):
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.empty_like(u)
+    lib().orc_noise_log(_p(u), _p(out), C.c_int64(u.size))
+    return out
+
+
+def noise_cos2pi(u):
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.empty_like(u)
+    lib().orc_noise_cos2pi(_p(u), _p(out), C.c_int64(u.size))
+    return out
+
+
+def noise_normal(seed, step, n, first=0):
+    """Standard normals of neurons first .. first+n-1 at `step` (the white
+    noise of the leapfrog step)."""
+    out = np.empty(n, np.float64)
+    lib().orc_noise_normal(C.c_uint64(seed), C.c_uint64(step), C.c_int64(first), C.c_int64(n), _p(out))
+    return out
 
 
 def couple(off, tgt, w, fire, mode, out=None, dtype=np.float32):
@@ -176,12 +209,12 @@ def run_rk4(p, dt, steps, off, tgt, w, V, m, h, ng, s, iext, cap=1 << 20, voxel_
     return raster.reshape(-1, 2)[:min(ln, cap)], ln, vc, sec.value
 
 
-def run_leapfrog(p, dt, steps, off, tgt, mode, sv, cap=1 << 20):
+def run_leapfrog(p, dt, steps, off, tgt, mode, sv, cap=1 << 20, first_step=0):
     raster = np.zeros(2 * cap, np.int32)
     arr = (C.c_void_p * 17)(*[a.ctypes.data for a in sv])
     ln = getattr(lib(), "orc_run_leapfrog_" + _sfx(sv[0].dtype))(
         C.c_int64(sv[0].size), C.byref(p), C.c_double(dt), C.c_int64(steps), _p(off), _p(tgt),
-        mode, arr, _p(raster), C.c_int64(cap))
+        mode, arr, _p(raster), C.c_int64(cap), C.c_int64(first_step))
     return raster.reshape(-1, 2)[:min(ln, cap)], ln
 
 

@@ -87,9 +87,16 @@ static void rk4_arrays(int64_t n, const orc_params* q, double dt, T* V, T* m, T*
     }
 }
 
+// white noise of neuron i at `step`: noise_std z + noise_mean in T (the
+// mean alone when noise_std = 0), z the standard normal of (seed, step, i)
+template <class T> static inline T white_noise(const orc_params* q, const P<T>& p, int64_t step, int64_t i) {
+    if (q->noise_std == 0.0) return p.noise_mean;
+    return T(q->noise_std) * T(noise::normal(q->noise_seed, (uint64_t)step, (uint64_t)i)) + p.noise_mean;
+}
+
 template <class T>
 static void leapfrog_arrays(int64_t n, const orc_params* q, double dt, T* const* sv,
-                            uint8_t* spiked) {
+                            uint8_t* spiked, int64_t step) {
     // sv: the reference's 17 SoA arrays in SIM_PARAMS order (src/defines.cuh:57-78)
     P<T> p = cast<T>(q);
     const T tdt = T(dt), thdt = T(dt) / T(2);
@@ -110,7 +117,7 @@ static void leapfrog_arrays(int64_t n, const orc_params* q, double dt, T* const*
         s.in_i = sv[14][i];
         s.cnt_i = sv[15][i];
         s.appcur = sv[16][i];
-        leapfrog_step(p, tdt, thdt, p.noise_mean, s);
+        leapfrog_step(p, tdt, thdt, white_noise(q, p, step, i), s);
         sv[0][i] = s.V;
         sv[1][i] = s.M;
         sv[2][i] = s.N;
@@ -161,12 +168,28 @@ void orc_hh_rk4_f64(int64_t n, const orc_params* q, double dt, double* V, double
 
 // One reference leapfrog step over the 17 SoA arrays (in place).
 void orc_hh_leapfrog_f32(int64_t n, const orc_params* q, double dt, float* const* sv,
-                         uint8_t* spiked) {
-    leapfrog_arrays<float>(n, q, dt, sv, spiked);
+                         uint8_t* spiked, int64_t step) {
+    leapfrog_arrays<float>(n, q, dt, sv, spiked, step);
 }
 void orc_hh_leapfrog_f64(int64_t n, const orc_params* q, double dt, double* const* sv,
-                         uint8_t* spiked) {
-    leapfrog_arrays<double>(n, q, dt, sv, spiked);
+                         uint8_t* spiked, int64_t step) {
+    leapfrog_arrays<double>(n, q, dt, sv, spiked, step);
+}
+
+// the white-noise generator's parts (for its pins)
+void orc_philox4x64_10(const uint64_t* c, const uint64_t* k, uint64_t* out) {
+    noise::philox(c[0], c[1], c[2], c[3], k[0], k[1], out);
+}
+void orc_noise_log(const double* u, double* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) out[i] = noise::log_u(u[i]);
+}
+void orc_noise_cos2pi(const double* u, double* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) out[i] = noise::cos2pi(u[i]);
+}
+// standard normals of neurons first .. first+n-1 at `step`
+void orc_noise_normal(uint64_t seed, uint64_t step, int64_t first, int64_t n, double* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) out[i] = noise::normal(seed, step, (uint64_t)(first + i));
 }
 
 // Coupling from a 0/1 firing array; g overwritten (see couple_push).
@@ -268,7 +291,7 @@ int64_t orc_run_rk4_f64(int64_t n, const orc_params* q, double dt, int64_t steps
 template <class T>
 static int64_t run_leapfrog(int64_t n, const orc_params* q, double dt, int64_t steps,
                             const int32_t* off, const int32_t* tgt, int mode, T* const* sv,
-                            int32_t* raster, int64_t cap) {
+                            int32_t* raster, int64_t cap, int64_t first_step) {
     std::vector<uint8_t> fire(n, 0);
     std::vector<int32_t> list;
     int64_t len = 0;
@@ -277,7 +300,7 @@ static int64_t run_leapfrog(int64_t n, const orc_params* q, double dt, int64_t s
     compact(n, fire.data(), list);
     for (int64_t k = 0; k < steps; ++k) {
         couple_push<T>(n, off, tgt, nullptr, list, mode, sv[7], sv[8]);
-        leapfrog_arrays<T>(n, q, dt, sv, fire.data());
+        leapfrog_arrays<T>(n, q, dt, sv, fire.data(), first_step + k);
         compact(n, fire.data(), list);
         for (int32_t i : list) {
             if (len < cap) {
@@ -292,13 +315,13 @@ static int64_t run_leapfrog(int64_t n, const orc_params* q, double dt, int64_t s
 extern "C" {
 int64_t orc_run_leapfrog_f32(int64_t n, const orc_params* q, double dt, int64_t steps,
                              const int32_t* off, const int32_t* tgt, int mode, float* const* sv,
-                             int32_t* raster, int64_t cap) {
-    return run_leapfrog<float>(n, q, dt, steps, off, tgt, mode, sv, raster, cap);
+                             int32_t* raster, int64_t cap, int64_t first_step) {
+    return run_leapfrog<float>(n, q, dt, steps, off, tgt, mode, sv, raster, cap, first_step);
 }
 int64_t orc_run_leapfrog_f64(int64_t n, const orc_params* q, double dt, int64_t steps,
                              const int32_t* off, const int32_t* tgt, int mode, double* const* sv,
-                             int32_t* raster, int64_t cap) {
-    return run_leapfrog<double>(n, q, dt, steps, off, tgt, mode, sv, raster, cap);
+                             int32_t* raster, int64_t cap, int64_t first_step) {
+    return run_leapfrog<double>(n, q, dt, steps, off, tgt, mode, sv, raster, cap, first_step);
 }
 
 // Electrode readout, network mean and synchrony (see oracle.hpp).

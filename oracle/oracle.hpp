@@ -131,6 +131,102 @@ template <class T> inline Rates<T> rates(T u) {
     return R;
 }
 
+// --------------------------------------------------------------------------
+// white noise: a standard normal per (seed, step, neuron) from Philox4x64-10
+// (Salmon et al., SC'11; counter {step, neuron, purpose, 0}, key {seed, 0})
+// and the Box-Muller transform (Box & Muller 1958) with ln u from the atanh
+// series after splitting off the exponent and cos 2 pi u from Taylor series
+// after an octant reduction (J.-M. Muller, "Elementary Functions"); every
+// operation an IEEE double one, fma where written, sqrt correctly rounded.
+// Replaces curand_normal of Leapfrog_HH (src/ephys.cu:82), whose XORWOW
+// stream is not reproduced.
+// --------------------------------------------------------------------------
+namespace noise {
+constexpr uint64_t kPurpose = 0x4e4f495345ull;
+
+inline void philox(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0, uint64_t k1, uint64_t o[4]) {
+    const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B97F4A7C15ull;
+            k1 += 0xBB67AE8584CAA73Bull;
+        }
+        const __uint128_t p0 = (__uint128_t)M0 * c0, p1 = (__uint128_t)M1 * c2;
+        const uint64_t n0 = (uint64_t)(p1 >> 64) ^ c1 ^ k0, n2 = (uint64_t)(p0 >> 64) ^ c3 ^ k1;
+        c1 = (uint64_t)p1;
+        c3 = (uint64_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    o[0] = c0, o[1] = c1, o[2] = c2, o[3] = c3;
+}
+
+inline double bits_to_double(uint64_t b) {
+    double d;
+    std::memcpy(&d, &b, 8);
+    return d;
+}
+inline uint64_t double_to_bits(double d) {
+    uint64_t b;
+    std::memcpy(&b, &d, 8);
+    return b;
+}
+
+inline double log_u(double u) {
+    const uint64_t b = double_to_bits(u);
+    int e = (int)((b >> 52) & 0x7ff) - 1022;
+    double m = bits_to_double((b & 0x000fffffffffffffull) | 0x3fe0000000000000ull);
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m + m;
+        e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), z = s * s;
+    static const double c[11] = {0x1.642c8590b2164p-4, 0x1.8618618618618p-4, 0x1.af286bca1af28p-4,
+                                 0x1.e1e1e1e1e1e1ep-4, 0x1.1111111111111p-3, 0x1.3b13b13b13b14p-3,
+                                 0x1.745d1745d1746p-3, 0x1.c71c71c71c71cp-3, 0x1.2492492492492p-2,
+                                 0x1.999999999999ap-2, 0x1.5555555555555p-1};
+    double p = c[0];
+    for (int k = 1; k < 11; ++k) p = std::fma(p, z, c[k]);
+    const double lm = std::fma(s * z, p, s + s);
+    const double de = (double)e;
+    return std::fma(de, 0x1.62e42fefa3800p-1, std::fma(de, 0x1.ef35793c76730p-45, lm));
+}
+
+inline double cos_r(double r) {
+    static const double c[9] = {0x1.20c62c2f2d7f5p-2, -0x1.b6e24f44b128fp+0, 0x1.f9d38a3763cc3p+2,
+                                -0x1.a6d1f2a204a8cp+4, 0x1.e1f506891babbp+5, -0x1.55d3c7e3cbffap+6,
+                                0x1.03c1f081b5ac4p+6, -0x1.3bd3cc9be45dep+4, 1.0};
+    const double y = r * r;
+    double p = -0x1.2a0c591af8314p-5;
+    for (int k = 0; k < 9; ++k) p = std::fma(p, y, c[k]);
+    return p;
+}
+inline double sin_r(double r) {
+    static const double c[9] = {0x1.aaec32af93359p-4, -0x1.6fadb9f155744p-1, 0x1.e8f434d018d63p+1,
+                                -0x1.e3074fde8871fp+3, 0x1.50783487ee782p+5, -0x1.32d2cce62bd86p+6,
+                                0x1.466bc6775aae2p+6, -0x1.4abbce625be53p+5, 0x1.921fb54442d18p+2};
+    const double y = r * r;
+    double p = -0x1.8a404211f9547p-7;
+    for (int k = 0; k < 9; ++k) p = std::fma(p, y, c[k]);
+    return p * r;
+}
+inline double cos2pi(double u) {
+    const double t = u - std::rint(u);
+    const double a = t < 0.0 ? -t : t;
+    if (a <= 0.125) return cos_r(a);
+    if (a <= 0.375) return sin_r(0.25 - a);
+    return -cos_r(0.5 - a);
+}
+
+inline double normal(uint64_t seed, uint64_t step, uint64_t neuron) {
+    uint64_t w[4];
+    philox(step, neuron, kPurpose, 0, seed, 0, w);
+    const double u1 = (double)((w[0] >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(w[1] >> 11) * 0x1.0p-53;
+    return std::sqrt(-2.0 * log_u(u1)) * cos2pi(u2);
+}
+}  // namespace noise
+
 // Parameters cast to T once (the host passes doubles; both sides cast the
 // same doubles, so both see the same T values).
 template <class T> struct P {

@@ -1,11 +1,13 @@
 """The reference-interface layer (neuronephys_clean_amd/compat/nnx_compat.hpp).
 
-compat_demo is the reference's loop (src/main.cu:166-181: AtomicCoupling,
+compat_demo is the reference's loop (src/main.cu:166-181: the reference's
+CouplingStruct over a ChunkedAdjacencyList from Network::ToChunkedAdjacencyList,
 ParacrineModule, HHEphys::SimulateEphys per increment) compiled by the host
-compiler against that header.  The GPU test runs it as a child process and
-checks the raster and final state bit for bit against the oracle's leapfrog
-run, and the paracrine field against the oracle's chain of trilinear scatter,
-Crank-Nicolson update and interpolation.
+compiler against that header.  The GPU test runs it as a child process, with
+the reference's white noise (std 1.5, mean 3) and in the deterministic mode,
+and checks the raster and final state bit for bit against the oracle's
+leapfrog run with the same noise, and the paracrine field against the
+oracle's chain of trilinear scatter, Crank-Nicolson update and interpolation.
 """
 import os
 import subprocess
@@ -42,7 +44,8 @@ def _spmv(off, cols, vals, x):
 
 
 @pytest.mark.gpu
-def test_compat_loop_matches_oracle(tmp_path):
+@pytest.mark.parametrize("noise", [True, False])
+def test_compat_loop_matches_oracle(tmp_path, noise):
     net = configs.c1(seed=7, n=1500, k=40)
     in_deg = net.ensure_csc()[4]
     rng = np.random.default_rng(21)
@@ -64,8 +67,11 @@ def test_compat_loop_matches_oracle(tmp_path):
     print_incs, render_N, kreg = 25, 12, 30
     args = [DEMO, str(tmp_path), str(nincs), str(nx), str(ny), str(nz), repr(h), repr(D),
             repr(decay), repr(par_dt), str(ts), repr(coup), str(print_incs), str(render_N), str(kreg)]
-    print("compat_demo", " ".join(args[1:]), flush=True)
-    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    env = dict(os.environ)
+    if not noise:
+        env["COMPAT_DEMO_NOISE_STD"] = "0"
+    print("compat_demo", " ".join(args[1:]), "noise" if noise else "deterministic", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, env=env)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
 
@@ -73,7 +79,9 @@ def test_compat_loop_matches_oracle(tmp_path):
     # each print row the oracle's region means and network mean of the state
     # entering that increment
     q = _lib.params_reference()
-    q.noise_std = 0.0
+    if not noise:
+        q.noise_std = 0.0
+    assert q.noise_seed == 0
     xyz = np.ascontiguousarray(pos.T)
     centers = (net.n // render_N) * np.arange(render_N)
     regions = np.fromfile(tmp_path / "regions.i32", np.int32)
@@ -86,7 +94,8 @@ def test_compat_loop_matches_oracle(tmp_path):
     roff = np.arange(0, (render_N + 1) * kreg, kreg, dtype=np.int32)
     for r in range(len(rows)):
         if r:
-            oracle.run_leapfrog(oracle.params(q.as_dict()), 0.4, print_incs, net.off, net.tgt, 2, sv_e, cap=1)
+            oracle.run_leapfrog(oracle.params(q.as_dict()), 0.4, print_incs, net.off, net.tgt, 2, sv_e, cap=1,
+                                first_step=(r - 1) * print_incs)
         avg, _ = oracle.electrode_measure(roff, regions, sv_e[0])
         assert np.array_equal(rows[r, :render_N], avg), r
         assert rows[r, render_N] == oracle.network_mean(sv_e[0]), r

@@ -59,8 +59,9 @@ def test_hh_rk4_step_bitexact(ctx, T):
 
 @pytest.mark.parametrize("T", [np.float32, np.float64])
 @pytest.mark.parametrize("rule", [_lib.SPIKE_LEVEL, _lib.SPIKE_CROSSING])
-def test_hh_leapfrog_step_bitexact(ctx, T, rule):
-    say("hh_leapfrog_step", T, rule)
+@pytest.mark.parametrize("noise", [False, True])
+def test_hh_leapfrog_step_bitexact(ctx, T, rule, noise):
+    say("hh_leapfrog_step", T, rule, "noise" if noise else "deterministic")
     rng = np.random.default_rng(1)
     n = 5000 + 7
     sv = [np.zeros(n, T) for _ in range(17)]
@@ -72,8 +73,11 @@ def test_hh_leapfrog_step_bitexact(ctx, T, rule):
     for k in (9, 12, 15):
         sv[k][:] = rng.integers(0, 30, n)
     sv[16][:] = rng.normal(0, 3, n)
-    q = _lib.params_reference()
-    q.noise_std = 0.0
+    q = _lib.params_reference()  # white noise std 1.5, mean 3 (src/defines.cuh:99-101)
+    if not noise:
+        q.noise_std = 0.0
+    q.noise_seed = 12345
+    step = 7
     dev = [DA.from_numpy(ctx, a) for a in sv]
     st = _lib.HHState(dtype=DT[T], n=n, v=dev[0].ptr, m=dev[1].ptr, ng=dev[2].ptr, h=dev[3].ptr,
                       y_local=dev[4].ptr, y_i=dev[5].ptr, y_e=dev[6].ptr, out_local=dev[7].ptr,
@@ -82,9 +86,9 @@ def test_hh_leapfrog_step_bitexact(ctx, T, rule):
                       count_i=dev[15].ptr, appcur=dev[16].ptr)
     bits = DA.zeros(ctx, (n + 63) // 64, np.uint64)
     V0 = sv[0].copy()
-    ctx.call("nnx_hh_step", C.byref(st), C.byref(q), _lib.INT_LEAPFROG, rule, 0.4, 0, None,
+    ctx.call("nnx_hh_step", C.byref(st), C.byref(q), _lib.INT_LEAPFROG, rule, 0.4, step, None,
              bits.ptr)
-    flags = oracle.hh_leapfrog(oracle.params(q.as_dict()), 0.4, sv)
+    flags = oracle.hh_leapfrog(oracle.params(q.as_dict()), 0.4, sv, step=step)
     for k in range(17):
         assert np.array_equal(dev[k].numpy(), sv[k]), SOA[k]
     if rule == _lib.SPIKE_CROSSING:
@@ -92,14 +96,16 @@ def test_hh_leapfrog_step_bitexact(ctx, T, rule):
     assert np.array_equal(bits_to_flags(bits.numpy(), n), flags)
 
 
-def test_hh_step_rejects_noise_and_bad_args(ctx):
+def test_hh_step_rejects_rk4_noise_and_bad_args(ctx):
     n = 10
     a = DA.zeros(ctx, n, np.float32)
     st = _lib.HHState(dtype=_lib.F32, n=n, v=a.ptr, m=a.ptr, h=a.ptr, ng=a.ptr, s=a.ptr,
                       i_ext=a.ptr)
-    q = _lib.params_reference()  # noise_std 1.5
+    q = _lib.params_squid()
+    q.noise_std = 1.0  # the RK4 model has a static drive; noise is the leapfrog step's
     with pytest.raises(_lib.NNXError) as e:
-        ctx.call("nnx_hh_step", C.byref(st), C.byref(q), _lib.INT_LEAPFROG, 0, 0.4, 0, None, None)
+        ctx.call("nnx_hh_step", C.byref(st), C.byref(q), _lib.INT_RK4, _lib.SPIKE_CROSSING, 0.01, 0, a.ptr,
+                 None)
     assert e.value.status == 5
     p = _lib.params_squid()
     with pytest.raises(_lib.NNXError) as e:
@@ -493,9 +499,13 @@ def test_graph_replay_equals_eager(ctx, couple):
 
 @pytest.mark.parametrize("mode", ["count", "signal"])
 @pytest.mark.parametrize("T", [np.float32, np.float64])
-def test_reference_leapfrog_sim_bitexact(ctx, mode, T):
+@pytest.mark.parametrize("noise", [False, True])
+def test_reference_leapfrog_sim_bitexact(ctx, mode, T, noise):
     """The reference's own step (src/main.cu:166-181 with Leapfrog_HH and
-    AtomicCoupling) on a 1,024-neuron UniformRandomNetwork-shaped graph."""
+    AtomicCoupling) on a 1,024-neuron UniformRandomNetwork-shaped graph, in
+    the deterministic mode and with the reference's white noise (std 1.5,
+    mean 3): one 100-step graph replayed at steps 0, 100, 200 and 300, the
+    noise keyed by the device step counter."""
     net = configs.c1(seed=5)
     in_deg = net.ensure_csc()[4]
     rng = np.random.default_rng(14)
@@ -504,13 +514,14 @@ def test_reference_leapfrog_sim_bitexact(ctx, mode, T):
     sv0["LOCAL_COUNT"] = in_deg.astype(np.float64)
     sv0["APPCUR"] = rng.normal(0, 2, net.n)
     steps = 400
-    sim = ReferenceSimulation(net, sv0, T, couple=mode, record_cap=1_000_000)
-    sim.run(steps, use_graph=True)
+    sim = ReferenceSimulation(net, sv0, T, couple=mode, record_cap=1_000_000, noise=noise)
+    sim.run(steps, use_graph=True, chunk=100)
     sim.ctx.synchronize()
     ras, ln = sim.raster()
     sv = [np.asarray(sv0[k], T).copy() for k in SOA]
     q = _lib.params_reference()
-    q.noise_std = 0.0
+    if not noise:
+        q.noise_std = 0.0
     ref, rln = oracle.run_leapfrog(oracle.params(q.as_dict()), 0.4, steps, net.off, net.tgt,
                                    2 if mode == "count" else 3, sv, cap=1_000_000)
     say("reference sim", mode, T, ln, rln)
