@@ -721,7 +721,8 @@ typedef struct nnx_paracrine {
     double release;           /* source added per spike at its voxel (0.01) */
     double rtol;              /* FP64 relative residual of each solve (1e-8) */
     double inner_rtol;        /* tightest inner fp32 CG tolerance (0: 1e-6) */
-    int32_t max_inner;        /* CG iterations enqueued per refinement step (0: 64) */
+    int32_t max_inner;        /* CG iterations enqueued for the first refinement step, half as
+                               * many for each later one (their tolerance is looser) (0: 64) */
     int32_t max_outer;        /* refinement steps enqueued (0: 3) */
     double* conc;             /* device, nvox FP64: the field (in: previous, out: new) */
     float* at_neurons;        /* optional device n floats: conc at each neuron's voxel

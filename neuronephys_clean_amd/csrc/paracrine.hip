@@ -95,13 +95,13 @@ template <int NV> __device__ __forceinline__ void put_partial(double (&v)[NV], d
 // CG state on the device
 // ---------------------------------------------------------------------------
 enum { SC_BB = 0, SC_THR, SC_RR, SC_RZ, SC_N };
-enum { FL_DONE = 0, FL_ITERS, FL_INITDONE, FL_ABS, FL_N };
+enum { FL_DONE = 0, FL_ITERS, FL_INITDONE, FL_ABS, FL_XPEND, FL_N };
 
 struct CgDev {
     float *r, *q, *dinv;  // dinv NULL: constant diagonal (dinv_c)
     float dinv_c;
     double *pu0, *pu1;    // (rz, rr) partials of the updates, ping-pong; pu0 = init's
-    double *ppq;          // pq partials of the operator application
+    double *ppq, *ppq1;   // pq partials of the operator application (ppq1: odd iterations of the fast path)
     int npq;              // their count (blocks of the operator kernel)
     double *pb;           // bb partials of the init
     double* sc;
@@ -177,6 +177,7 @@ __global__ __launch_bounds__(256) void k_cg_init_fin(CgDev d, double tol, int ab
         d.fl[FL_INITDONE] = conv;
         d.fl[FL_DONE] = conv || maxit <= 0;
         d.fl[FL_ITERS] = 0;
+        d.fl[FL_XPEND] = 0;
     }
 }
 
@@ -201,24 +202,45 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 // are issued before the plane's barrier (a barrier drains outstanding loads,
 // so 16-byte loads per thread are what keeps enough bytes in flight).
 // Requires nx % 4 == 0 (16-byte aligned rows).
-//   FUSED = false: y = A x (summation order of the CPU restatement).
-//   FUSED = true : CG iteration k - p_new = dinv r + beta p_old at every
-//                  point, q = A p_new, p_new stored, pq partials.
+//   MODE 0: y = A src (summation order of the CPU restatement); dst = y.
+//   MODE 2: CG iteration k, first half - p_new = dinv r + beta p_old at
+//           every point of the tile and halo, q = A p_new formed in LDS (not
+//           stored), p_new stored, partials of p_new . q; at the own points
+//           the deferred update of the previous iteration, x += alpha_{k-1}
+//           p_old (dst = x; skipped at k = 0).  20 B per voxel.
+//   MODE 3: CG iteration k, second half - q = A p_new again from src =
+//           p_new (the same operations, so the same bits), r -= alpha_k q,
+//           partials of r . dinv r and r . r.  12 B per voxel.
+// The last iteration's x += alpha p is left to k_cg_finish.  Together the
+// two halves move 32 B per voxel where storing q would take 40.
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
-template <bool FUSED>
-__global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int nparts,
-                                                    const float* __restrict__ src,
-                                                    const float* __restrict__ pold,
-                                                    float* __restrict__ pnew,
+template <int MODE>
+__global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, const float* __restrict__ src,
+                                                    const float* __restrict__ pold, float* __restrict__ pnew,
                                                     float* __restrict__ dst) {
+    static_assert(MODE == 0 || MODE == 2 || MODE == 3, "mode");
     __shared__ float t[2][kTY4 + 2][kRow4];
-    float beta = 0.0f, di = 1.0f;
-    if (FUSED) {
+    float beta = 0.0f, di = 1.0f, alpha = 0.0f;
+    if (MODE == 2) {
         double rz, rzp, rr;
         if (!cg_scalars(d, k, true, rz, rzp, rr)) return;
         beta = k ? (float)(rz / rzp) : 0.0f;
+        if (k) {
+            double pq[1];
+            sum_partials<1>(((k - 1) & 1) ? d.ppq1 : d.ppq, d.npq, pq);
+            alpha = (float)(rzp / pq[0]);
+        }
         di = d.dinv_c;
     }
+    if (MODE == 3) {
+        double rz, rzp, rr;
+        if (!cg_scalars(d, k, false, rz, rzp, rr)) return;
+        double pq[1];
+        sum_partials<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, pq);
+        alpha = (float)(rz / pq[0]);
+        di = d.dinv_c;
+    }
+    const bool upd_x = MODE == 2 && k > 0;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int tx = (nx + kTX4 - 1) / kTX4, ty = (ny + kTY4 - 1) / kTY4, tz = (nz + kZ4 - 1) / kZ4;
     const int ntiles = tx * ty * tz;
@@ -232,7 +254,7 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
     const int hslot = tt < 128 + kTY4 ? 3 : 4 + kTX4;
     const float cdiag = g.diag, cc = g.c;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    double v[1] = {0};
+    double v[2] = {0, 0};
     // one tile per workgroup when the grid covers them (the usual case);
     // a grid-stride loop otherwise
     const int nb = gridDim.x;
@@ -249,9 +271,10 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
         const int64_t ocol = xo + (int64_t)nx * j;
         const int64_t rcol = x0 + 4 * hl + (int64_t)nx * gj;
         const int64_t ccol = hx + (int64_t)nx * gj;
+        // tile values: p_new from r and p_old (MODE 2), src otherwise
         auto val4 = [&](bool ok, int64_t e) -> float4 {
             if (!ok) return z4;
-            if (!FUSED) return *reinterpret_cast<const float4*>(src + e);
+            if (MODE != 2) return *reinterpret_cast<const float4*>(src + e);
             const float4 a = *reinterpret_cast<const float4*>(d.r + e);
             const float4 b = *reinterpret_cast<const float4*>(pold + e);
             return make_float4(a.x * di + beta * b.x, a.y * di + beta * b.y, a.z * di + beta * b.z,
@@ -259,17 +282,35 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
         };
         auto val1 = [&](bool ok, int64_t e) -> float {
             if (!ok) return 0.0f;
-            return FUSED ? d.r[e] * di + beta * pold[e] : src[e];
+            return MODE == 2 ? d.r[e] * di + beta * pold[e] : src[e];
         };
+        // the own column: the tile value, and p_old and x (MODE 2) or r
+        // (MODE 3) of the same point
+        auto own4 = [&](bool ok, int64_t e, float4& po, float4& xr) -> float4 {
+            po = z4;
+            xr = z4;
+            if (!ok) return z4;
+            if (MODE == 2) {
+                const float4 a = *reinterpret_cast<const float4*>(d.r + e);
+                po = *reinterpret_cast<const float4*>(pold + e);
+                if (upd_x) xr = *reinterpret_cast<const float4*>(dst + e);
+                return make_float4(a.x * di + beta * po.x, a.y * di + beta * po.y, a.z * di + beta * po.z,
+                                   a.w * di + beta * po.w);
+            }
+            if (MODE == 3) xr = *reinterpret_cast<const float4*>(d.r + e);
+            return *reinterpret_cast<const float4*>(src + e);
+        };
+        float4 poc, xrc;
         float4 pm = val4(own && k0 > 0, ocol + sxy * (k0 - 1));
-        float4 pc = val4(own, ocol + sxy * k0);
+        float4 pc = own4(own, ocol + sxy * k0, poc, xrc);
         *reinterpret_cast<float4*>(&t[0][ly + 1][4 + 4 * lx]) = pc;
         if (hrow) *reinterpret_cast<float4*>(&t[0][hy + 1][4 + 4 * hl]) = val4(hr_ok, rcol + sxy * k0);
         if (hcol) t[0][hy + 1][hslot] = val1(hc_ok, ccol + sxy * k0);
         auto plane = [&](int kz) {
             const int b = (kz - k0) & 1;
             const bool nxt = kz + 1 < nz;
-            const float4 pp = val4(own && nxt, ocol + sxy * (kz + 1));
+            float4 pop, xrp;
+            const float4 pp = own4(own && nxt, ocol + sxy * (kz + 1), pop, xrp);
             const float4 hr = val4(hr_ok && nxt, rcol + sxy * (kz + 1));
             const float hc = val1(hc_ok && nxt, ccol + sxy * (kz + 1));
             __syncthreads();
@@ -283,13 +324,37 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
                 o.z = cdiag * pc.z - cc * (pc.y + pc.w + ym.z + yp.z + pm.z + pp.z);
                 o.w = cdiag * pc.w - cc * (pc.z + right + ym.w + yp.w + pm.w + pp.w);
                 const int64_t id = ocol + sxy * kz;
-                *reinterpret_cast<float4*>(dst + id) = o;
-                if (FUSED) {
+                if (MODE == 0) *reinterpret_cast<float4*>(dst + id) = o;
+                if (MODE == 2) {
                     *reinterpret_cast<float4*>(pnew + id) = pc;
                     v[0] += (double)pc.x * o.x;
                     v[0] += (double)pc.y * o.y;
                     v[0] += (double)pc.z * o.z;
                     v[0] += (double)pc.w * o.w;
+                    if (upd_x) {
+                        float4 xn;
+                        xn.x = xrc.x + alpha * poc.x;
+                        xn.y = xrc.y + alpha * poc.y;
+                        xn.z = xrc.z + alpha * poc.z;
+                        xn.w = xrc.w + alpha * poc.w;
+                        *reinterpret_cast<float4*>(dst + id) = xn;
+                    }
+                }
+                if (MODE == 3) {
+                    float4 ro;
+                    ro.x = xrc.x - alpha * o.x;
+                    ro.y = xrc.y - alpha * o.y;
+                    ro.z = xrc.z - alpha * o.z;
+                    ro.w = xrc.w - alpha * o.w;
+                    *reinterpret_cast<float4*>(d.r + id) = ro;
+                    v[0] += (double)ro.x * (ro.x * di);
+                    v[0] += (double)ro.y * (ro.y * di);
+                    v[0] += (double)ro.z * (ro.z * di);
+                    v[0] += (double)ro.w * (ro.w * di);
+                    v[1] += (double)ro.x * ro.x;
+                    v[1] += (double)ro.y * ro.y;
+                    v[1] += (double)ro.z * ro.z;
+                    v[1] += (double)ro.w * ro.w;
                 }
             }
             *reinterpret_cast<float4*>(&t[b ^ 1][ly + 1][4 + 4 * lx]) = pp;
@@ -297,6 +362,8 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
             if (hcol) t[b ^ 1][hy + 1][hslot] = hc;
             pm = pc;
             pc = pp;
+            poc = pop;
+            xrc = xrp;
         };
         // a full tile's march has a compile-time trip count, so it unrolls
         // and the next plane's loads overlap the current one (a fifth of the
@@ -309,9 +376,47 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, int
         }
         __syncthreads();  // the next tile reuses both buffers
     }
-    if (FUSED) {
-        put_partial<1>(v, d.ppq);
-        (void)nparts;
+    if (MODE == 2) {
+        double pv[1] = {v[0]};
+        put_partial<1>(pv, (k & 1) ? d.ppq1 : d.ppq);
+    }
+    if (MODE == 3) {
+        // kRedBlocks partials are read back: the slots beyond this launch's
+        // workgroups hold zeros (launched with at most kRedBlocks)
+        double* pu = (k & 1) ? d.pu0 : d.pu1;
+        put_partial<2>(v, pu);
+        if (blockIdx.x == 0) {
+            for (int i = (int)gridDim.x * 2 + (int)threadIdx.x; i < kRedBlocks * 2; i += (int)blockDim.x) pu[i] = 0.0;
+            if (threadIdx.x == 0) {
+                d.fl[FL_ITERS] = k + 1;
+                d.fl[FL_XPEND] = 1;
+            }
+        }
+    }
+}
+
+// The deferred x += alpha_k p_k of the last executed iteration k of the
+// 7-point fast path (k_stencil7x4 modes 2 and 3); nothing when none is
+// pending.  p_k is in pa for odd k, pb for even k.
+__global__ __launch_bounds__(256) void k_cg_finish(int64_t n, CgDev d, const float* __restrict__ pa,
+                                                   const float* __restrict__ pb, float* __restrict__ x) {
+    if (!d.fl[FL_XPEND]) return;
+    const int k = d.fl[FL_ITERS] - 1;
+    double rz[2], pq[1];
+    sum_partials<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, rz);
+    sum_partials<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, pq);
+    const float alpha = (float)(rz[0] / pq[0]);
+    const float* __restrict__ p = (k & 1) ? pa : pb;
+    const int64_t n4 = n / 4, stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = blockIdx.x * 256ll + threadIdx.x; c < n4; c += stride) {
+        const float4 xv = reinterpret_cast<const float4*>(x)[c];
+        const float4 pv = reinterpret_cast<const float4*>(p)[c];
+        float4 xo;
+        xo.x = xv.x + alpha * pv.x;
+        xo.y = xv.y + alpha * pv.y;
+        xo.z = xv.z + alpha * pv.z;
+        xo.w = xv.w + alpha * pv.w;
+        reinterpret_cast<float4*>(x)[c] = xo;
     }
 }
 
@@ -530,7 +635,8 @@ static nnx_status cg_bind(nnx_ctx* c, const GridK& gk, int precond, const CgMem&
     h->d.pu1 = P + 2 * kRedBlocks;    // 2 x kRedBlocks
     h->d.ppq = P + 4 * kRedBlocks;    // up to 2048 operator blocks
     h->d.pb = P + 6 * kRedBlocks;     // kRedBlocks
-    h->tr_part = P + 7 * kRedBlocks;  // 2 x kRedBlocks (up to 9 x kRedBlocks used)
+    h->tr_part = P + 7 * kRedBlocks;  // 2 x kRedBlocks
+    h->d.ppq1 = P + 11 * kRedBlocks;  // up to 2048 (9 and 10 x kRedBlocks: the paracrine solve's)
     h->out2 = m.sc + 32;
     h->d.dinv = nullptr;
     h->d.dinv_c = 1.0f;
@@ -561,21 +667,34 @@ static nnx_status cg_readback(nnx_ctx* c, CgHost& h, int32_t* fl, double* sc) {
     return NNX_OK;
 }
 
-// Iteration k of CG on the prepared buffers: the new direction and q = A p
-// (one fused kernel for the constant 7-point stencil), then the update of x
-// and r.  Both kernels return at once when the device flag says the solve
-// is over, so iterations can be enqueued past convergence.
+// Iteration k of CG on the prepared buffers.  Constant 7-point stencil with
+// 16-byte rows and x (fast): the two halves of k_stencil7x4 (modes 2, 3),
+// the x update deferred by one iteration (k_cg_finish applies the last);
+// otherwise the new direction, q = A p, then the update of x and r.  Every
+// kernel returns at once when the device flag says the solve is over, so
+// iterations can be enqueued past convergence.
 static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float* x, int vec, bool fast, int nb4) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
     float* pold = (k & 1) ? h.pb : h.pa;
     float* pnew = (k & 1) ? h.pa : h.pb;
     if (fast) {
-        k_stencil7x4<true><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nb4, nullptr, pold, pnew, h.d.q);
-    } else {
-        k_cg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pold, pnew);
-        k_cg_spmv_gen<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, k, pnew);
+        k_stencil7x4<2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x);
+        k_stencil7x4<3><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr);
+        return;
     }
+    k_cg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pold, pnew);
+    k_cg_spmv_gen<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, k, pnew);
     k_cg_update<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pnew, x, vec);
+}
+
+static bool cg_fast(const GridK& gk, const float* x) {
+    return gk.kind == 0 && gk.nx % 4 == 0 && ((uintptr_t)x % 16) == 0;
+}
+
+// the pending x update of the fast path (nothing otherwise)
+static void cg_finish_launch(nnx_ctx* c, const GridK& gk, CgHost& h, float* x) {
+    const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
+    k_cg_finish<<<flat_blocks(n / 4), 256, 0, c->stream>>>(n, h.d, h.pa, h.pb, x);
 }
 
 // Runs CG on the prepared buffers; x holds the guess when use_x0.  The host
@@ -588,8 +707,8 @@ static nnx_status cg_run(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b,
     NNX_LAUNCHED(c);
     if (check_every < 1) check_every = 1;
     const int vec = ((uintptr_t)x % 16) == 0;
-    // constant 7-point stencil with 16-byte rows: fused LDS float4 kernel
-    const bool fast = gk.kind == 0 && gk.nx % 4 == 0;
+    // constant 7-point stencil with 16-byte rows: the LDS float4 kernels
+    const bool fast = cg_fast(gk, x);
     const int nb4 = stencil4_blocks(gk);
     h.d.npq = fast ? nb4 : kRedBlocks;
     int32_t fl[FL_N];
@@ -603,6 +722,8 @@ static nnx_status cg_run(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b,
         NNX_LAUNCHED(c);
         NNX_TRY(cg_readback(c, h, fl, sc));
     }
+    cg_finish_launch(c, gk, h, x);
+    NNX_LAUNCHED(c);
     if (res) {
         res->iters = fl[FL_ITERS];
         const double bb = sc[SC_BB] > 0 ? sc[SC_BB] : 1.0;
@@ -653,10 +774,11 @@ static void cg_enqueue_fixed(nnx_ctx* c, const GridK& gk, CgHost& h, const float
     k_cg_init<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, b, x, use_x0, h.pa);
     k_cg_init_fin<<<1, 256, 0, c->stream>>>(h.d, tol, 0, maxit);
     const int vec = ((uintptr_t)x % 16) == 0;
-    const bool fast = gk.kind == 0 && gk.nx % 4 == 0;
+    const bool fast = cg_fast(gk, x);
     const int nb4 = stencil4_blocks(gk);
     h.d.npq = fast ? nb4 : kRedBlocks;
     for (int k = 0; k < maxit; ++k) cg_iter_launch(c, gk, h, k, x, vec, fast, nb4);
+    cg_finish_launch(c, gk, h, x);
 }
 
 // The paracrine solve of a timestep (nnx_paracrine): rhs, then max_outer
@@ -726,6 +848,7 @@ __global__ __launch_bounds__(256) void k_para_ctl(CgDev d, const double* prr, co
     }
     if (threadIdx.x != 0) return;
     const double rel = rr[0] <= 0 ? 0.0 : (bb[0] > 0 ? sqrt(rr[0] / bb[0]) : INFINITY);
+    d.fl[FL_XPEND] = 0;
     if (rel <= rtol || final_check) {
         pfl[0] = 1;
         d.fl[FL_DONE] = 1;
@@ -977,8 +1100,7 @@ nnx_status nnx_stencil_apply(nnx_ctx* c, const nnx_grid* g, const float* x, floa
     const GridK gk = make_grid(g);
     if (gk.kind == 0 && gk.nx % 4 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0) {
         CgDev none{};
-        k_stencil7x4<false><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, 0, x, nullptr,
-                                                                       nullptr, y);
+        k_stencil7x4<0><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y);
     } else {
         k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
     }
@@ -1137,14 +1259,17 @@ nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
     const int max_inner = q.max_inner > 0 ? q.max_inner : 64;
     const int max_outer = q.max_outer > 0 ? q.max_outer : 3;
     const double inner_rtol = q.inner_rtol > 0 ? q.inner_rtol : 1e-6;
-    const bool fast = gk.kind == 0 && gk.nx % 4 == 0;
+    const bool fast = cg_fast(gk, ph.xin);
     const int nb4 = stencil4_blocks(gk);
     h.d.npq = fast ? nb4 : kRedBlocks;
     k_para_rhs<<<kRedBlocks, 256, 0, c->stream>>>(nv, q.conc, p->voxel_counts, q.release, ph.b, ph.pbb, ph.pfl);
     for (int o = 0; o < max_outer; ++o) {
+        // the later refinement steps solve to a looser tolerance: half the budget
+        const int budget = o == 0 ? max_inner : std::max(1, max_inner / 2);
         k_para_res<false><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
-        k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, max_inner, ph.pfl, q.stats, 0);
-        for (int k = 0; k < max_inner; ++k) cg_iter_launch(c, gk, h, k, ph.xin, 1, fast, nb4);
+        k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, budget, ph.pfl, q.stats, 0);
+        for (int k = 0; k < budget; ++k) cg_iter_launch(c, gk, h, k, ph.xin, 1, fast, nb4);
+        cg_finish_launch(c, gk, h, ph.xin);
         k_para_axpy<<<flat_blocks(nv), 256, 0, c->stream>>>(nv, ph.xin, q.conc, h.d.fl, ph.pfl);
     }
     k_para_res<true><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
