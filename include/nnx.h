@@ -401,6 +401,18 @@ nnx_status nnx_cg_solve(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float
 nnx_status nnx_cg_solve_async(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float* d_x,
                               const nnx_cg_opts* o, double* d_result);
 
+/* nnx_cg_solve_async without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, hosts that forbid hipMalloc).
+ * The CG working set lives in a fixed 48 MiB array of the library, one per
+ * GPU, held until the process ends: grids up to 2.5M voxels with variable
+ * coefficients and Jacobi (e.g. 128^3), 3.1M otherwise; larger ones return
+ * NNX_ERR_ARG before anything is enqueued.  Same
+ * kernels and results as nnx_cg_solve_async, bit for bit.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams
+ * or threads) race.                                                         */
+nnx_status nnx_cg_solve_async_noalloc(nnx_ctx* ctx, const nnx_grid* g, const float* d_b, float* d_x,
+                                      const nnx_cg_opts* o, double* d_result);
 
 /* Mixed-precision iterative refinement: FP64 b and x, the outer residual
  * r = b - A x recomputed in FP64, inner fp32 CG on A d = r, x += d, until

@@ -199,6 +199,7 @@ _SIGS = {
     "nnx_cg_solve_refine": ([vp, _P(Grid), vp, vp, _P(CgOpts), C.c_double, C.c_int,
                              _P(CgResult)], C.c_int),
     "nnx_cg_solve_async": ([vp, _P(Grid), vp, vp, _P(CgOpts), vp], C.c_int),
+    "nnx_cg_solve_async_noalloc": ([vp, _P(Grid), vp, vp, _P(CgOpts), vp], C.c_int),
     "nnx_graph_cache_clear": ([vp], C.c_int),
     "nnx_graph_cache_stats": ([vp, _P(C.c_int), _P(C.c_int64)], C.c_int),
     "nnx_update_density_reference": ([vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,

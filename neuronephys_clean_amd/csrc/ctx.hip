@@ -155,6 +155,10 @@ nnx_status nnx_create(nnx_ctx** out, int device, void* hip_stream) {
         delete c;
         return NNX_ERR_HIP;
     }
+    // the fixed working memory of the allocation-free entry point on this
+    // device (a failed lookup is reported by that entry point)
+    (void)noalloc_init(c, device);
+    c->err.clear();
     // without a caller stream the context owns a blocking stream (ordered
     // with the legacy null stream, so work other libraries put there stays
     // ordered with ours)

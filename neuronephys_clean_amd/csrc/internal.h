@@ -185,6 +185,10 @@ nnx_status cg_async_check(nnx_ctx* c, const nnx_grid* g, const float* b, const f
 nnx_status cg_async_launch(nnx_ctx* c, const nnx_grid* g, const float* b, float* x, const nnx_cg_opts* o,
                            double* d_result, const CgMem& m);
 
+// the working memory of nnx_cg_solve_async_noalloc (noalloc.hip): looks up
+// the device address of its array on `device` (the current device) once
+nnx_status noalloc_init(nnx_ctx* c, int device);
+
 // nnx_hh_step with the step number read from the device (stepp, when not
 // NULL) - the white noise of a captured step sequence (ephys.hip)
 nnx_status hh_step_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_params* q, int integrator, int rule,
