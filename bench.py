@@ -336,11 +336,11 @@ def profile_pass(ctx, sim, net):
     from neuronephys_clean_amd.devmem import DeviceArray as DA
     from neuronephys_clean_amd.precond import UNet
     from neuronephys_clean_amd.sim import EnsembleSim
-    # 1,000 steps in graphs, then one step at step 1,000 with the in-step
-    # paracrine solve on the releases of those steps (eager dispatches)
+    # 1,000 steps, then one step at step 1,000 with the in-step paracrine
+    # solve on the releases of those steps (eager dispatches throughout)
     every = sim.plan.paracrine.every
     sim.plan.paracrine.every = 0
-    sim.run(1000, use_graph=True, chunk=100)
+    sim.run(1000)
     sim.plan.paracrine.every = every
     sim.run(1)
     sim.synchronize()
@@ -422,7 +422,7 @@ def profile_pass(ctx, sim, net):
     src, dst = DA(ctx, 1 << 28, np.uint8), DA(ctx, 1 << 28, np.uint8)
     ctx.call("nnx_memcpy_d2d", dst.ptr, src.ptr, C.c_size_t(1 << 28))
     ctx.synchronize()
-    log("profile pass done: 1,000 graph steps, 1 step with the in-step solve, 20 fused + 20 unfused steps, 20 HH launches, 20 dense pulls, 20 electrode readouts, "
+    log("profile pass done: 1,000 steps, 1 step with the in-step solve, 20 fused + 20 unfused steps, 20 HH launches, 20 dense pulls, 20 electrode readouts, "
         "21 Diekman and SAN "
         "steps, 20 Hebbian updates, 20 ensemble steps, "
         "5 network applications, "
@@ -761,6 +761,9 @@ def main():
     ap.add_argument("--mode", default="bench", choices=["bench", "profile", "loop"],
                     help="profile: a short eager sequence for rocprofv3 --pmc passes; loop: only the timed C2 "
                          "windows (for a kernel trace); neither prints the JSON line")
+    ap.add_argument("--eager", action="store_true",
+                    help="loop mode: dispatch every kernel eagerly instead of replaying step graphs (for kernel "
+                         "traces; the kernels are the same)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
@@ -816,7 +819,7 @@ def main():
     # a fixed release field: the per-voxel spike counts of the first 1,000
     # steps (the right-hand side of the standalone CG and C5 solves)
     para(False)
-    sim.run(FIXED_RELEASE_STEPS, use_graph=True, chunk=chunk)
+    sim.run(FIXED_RELEASE_STEPS, use_graph=not args.eager, chunk=chunk)
     counts_fixed = sim.a["voxel_counts"].numpy().astype(np.float64)
     b_fixed = net.amount * counts_fixed
     log(f"bench: fixed release field: {int(counts_fixed.sum())} releases in {FIXED_RELEASE_STEPS} steps")
@@ -847,8 +850,17 @@ def main():
 
     if args.mode == "loop":
         for on in (True, False):
-            wall_ms, ev_ms = window(on)
-            log(f"bench loop: paracrine={'on' if on else 'off'} {steps} steps {wall_ms:.1f} ms")
+            if args.eager:
+                para(on)
+                restore()
+                t0 = time.perf_counter()
+                sim.run(steps)
+                sim.synchronize()
+                wall_ms = (time.perf_counter() - t0) * 1e3
+            else:
+                wall_ms, ev_ms = window(on)
+            log(f"bench loop: paracrine={'on' if on else 'off'} {steps} steps {wall_ms:.1f} ms "
+                f"({'eager' if args.eager else 'graphs'}), solves {sim.paracrine_stats()}")
         return
 
     # the headline: HH + coupling + release + paracrine solve every `every` steps

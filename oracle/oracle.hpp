@@ -96,18 +96,20 @@ template <class T> struct Rates { T am, bm, ah, bh, an, bn; };
 //   b = a + 1.5 and e^-b = ea e^-1.5, bh = 1/(e^(-3-0.1u) + 1) = 1/(ea e^0.5 + 1);
 //   w = u + 60:  bn = 0.125 e^(-w/80), ah = 0.07 e^(-u/20-3) = 0.07 (e^(-w/80))^4,
 //   bm = 4 e^(-w/18).
+// a = fma(0.1, u, 3.5), d2 = fma(-ea, e^-1.5, 1), d3 = fma(ea, e^0.5, 1); the
+// arguments -w/80 and -w/18 are fma(u, -1/80, -60/80) and fma(u, -1/18, -60/18).
 // The three quotients share one division: with d1 = 1 - e^-a, d2 = 1 - e^-b,
 // d3 = ea e^0.5 + 1 (a denominator replaced by 1 where its series is used) and
 // inv = 1 / (d1 d2 d3): am = a (d2 d3) inv, an = 0.1 b (d1 d3) inv,
 // bh = (d1 d2) inv; below a = -20 three divisions (the product could overflow).
 template <class T> inline Rates<T> rates(T u) {
     Rates<T> R;
-    const T a = T(3.5) + T(0.1) * u;
+    const T a = std::fma(T(0.1), u, T(3.5));
     const T ea = oexp(-a);
     const T b = a + T(1.5);
     const T d1 = T(1) - ea;
-    const T d2 = T(1) - ea * T(0x1.c8f87724b5c1dp-3);
-    const T d3 = ea * T(0x1.a61298e1e069cp+0) + T(1);
+    const T d2 = std::fma(-ea, T(0x1.c8f87724b5c1dp-3), T(1));
+    const T d3 = std::fma(ea, T(0x1.a61298e1e069cp+0), T(1));
     const bool sa = std::fabs(a) < T(0.5), sb = std::fabs(b) < T(0.5);
     if (a > T(-20)) {
         const T e1 = sa ? T(1) : d1, e2 = sb ? T(1) : d2;
@@ -122,12 +124,11 @@ template <class T> inline Rates<T> rates(T u) {
         R.an = T(0.1) * (b / d2);
         R.bh = T(1) / d3;
     }
-    const T w = u + T(60);
-    const T e80 = oexp(w * T(-0.0125));
+    const T e80 = oexp(std::fma(u, T(-0.0125), T(-0.75)));
     const T e40 = e80 * e80;
     R.bn = T(0.125) * e80;
     R.ah = T(0.07) * (e40 * e40);
-    R.bm = T(4) * oexp(w * T(-1.0 / 18.0));
+    R.bm = T(4) * oexp(std::fma(u, T(-1.0 / 18.0), T(-60.0 / 18.0)));
     return R;
 }
 
@@ -240,17 +241,25 @@ template <class T> struct P {
 template <class T>
 inline void hh_deriv(const P<T>& p, T V, T m, T h, T n, T s, T iext, T& dV, T& dm, T& dh, T& dn) {
     Rates<T> R = rates(V + p.vshift);
-    dm = R.am * (T(1) - m) - R.bm * m;
-    dh = R.ah * (T(1) - h) - R.bh * h;
-    dn = R.an * (T(1) - n) - R.bn * n;
+    // gates as a - (a + b) x, currents as an fma chain i = i_ext - g (V - E)
+    dm = std::fma(-(R.am + R.bm), m, R.am);
+    dh = std::fma(-(R.ah + R.bh), h, R.ah);
+    dn = std::fma(-(R.an + R.bn), n, R.an);
     T m3h = m * m * m * h;
     T n2 = n * n;
     T n4 = n2 * n2;
-    T ina = p.gna * m3h * (V - p.ena);
-    T ik = p.gk * n4 * (V - p.ek);
-    T il = p.gl * (V - p.el);
-    T isyn = s * (V - p.esyn);
-    dV = (iext - ina - ik - il - isyn) * p.invC;
+    T i = std::fma(-(p.gna * m3h), V - p.ena, iext);
+    i = std::fma(-(p.gk * n4), V - p.ek, i);
+    i = std::fma(-p.gl, V - p.el, i);
+    i = std::fma(-s, V - p.esyn, i);
+    dV = i * p.invC;
+}
+
+// x + c (k1 + 2 k2 + 2 k3 + k4), the sum left to right, one fma at the end
+template <class T> inline T rk4_comb(T x, T c, T k1, T k2, T k3, T k4) {
+    T t = std::fma(T(2), k2, k1);
+    t = std::fma(T(2), k3, t);
+    return std::fma(c, t + k4, x);
 }
 
 // Classical RK4 over (V, m, h, n); s is integrated exactly: s0 = s + g_in at
@@ -264,16 +273,16 @@ inline void rk4_step(const P<T>& p, T dt, T hdt, T dt6, T eh, T ef, T& V, T& m, 
     T sf = s0 * ef;
     T k1V, k1m, k1h, k1n, k2V, k2m, k2h, k2n, k3V, k3m, k3h, k3n, k4V, k4m, k4h, k4n;
     hh_deriv(p, V, m, h, n, s0, iext, k1V, k1m, k1h, k1n);
-    hh_deriv(p, V + hdt * k1V, m + hdt * k1m, h + hdt * k1h, n + hdt * k1n, sh, iext, k2V, k2m,
-             k2h, k2n);
-    hh_deriv(p, V + hdt * k2V, m + hdt * k2m, h + hdt * k2h, n + hdt * k2n, sh, iext, k3V, k3m,
-             k3h, k3n);
-    hh_deriv(p, V + dt * k3V, m + dt * k3m, h + dt * k3h, n + dt * k3n, sf, iext, k4V, k4m, k4h,
-             k4n);
-    V = V + dt6 * (k1V + T(2) * k2V + T(2) * k3V + k4V);
-    m = m + dt6 * (k1m + T(2) * k2m + T(2) * k3m + k4m);
-    h = h + dt6 * (k1h + T(2) * k2h + T(2) * k3h + k4h);
-    n = n + dt6 * (k1n + T(2) * k2n + T(2) * k3n + k4n);
+    hh_deriv(p, std::fma(hdt, k1V, V), std::fma(hdt, k1m, m), std::fma(hdt, k1h, h), std::fma(hdt, k1n, n), sh,
+             iext, k2V, k2m, k2h, k2n);
+    hh_deriv(p, std::fma(hdt, k2V, V), std::fma(hdt, k2m, m), std::fma(hdt, k2h, h), std::fma(hdt, k2n, n), sh,
+             iext, k3V, k3m, k3h, k3n);
+    hh_deriv(p, std::fma(dt, k3V, V), std::fma(dt, k3m, m), std::fma(dt, k3h, h), std::fma(dt, k3n, n), sf, iext,
+             k4V, k4m, k4h, k4n);
+    V = rk4_comb(V, dt6, k1V, k2V, k3V, k4V);
+    m = rk4_comb(m, dt6, k1m, k2m, k3m, k4m);
+    h = rk4_comb(h, dt6, k1h, k2h, k3h, k4h);
+    n = rk4_comb(n, dt6, k1n, k2n, k3n, k4n);
     s = sf;
 }
 
