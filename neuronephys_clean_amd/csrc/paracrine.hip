@@ -204,10 +204,12 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 // Requires nx % 4 == 0 (16-byte aligned rows).
 //   MODE 0: y = A src (summation order of the CPU restatement); dst = y.
 //   MODE 2: CG iteration k, first half - p_new = dinv r + beta p_old at
-//           every point of the tile and halo, q = A p_new formed in LDS (not
-//           stored), p_new stored, partials of p_new . q; at the own points
-//           the deferred update of the previous iteration, x += alpha_{k-1}
-//           p_old (dst = x; skipped at k = 0).  20 B per voxel.
+//           every point of the tile and halo (p_old not read at k = 0), q =
+//           A p_new formed in LDS (not stored), p_new stored, partials of
+//           p_new . q; at the own points the deferred update of the previous
+//           iteration, x += alpha_{k-1} p_old (dst = x; skipped at k = 0; with
+//           xzero the solve started from x = 0 without storing it, and k = 1
+//           writes x = alpha_0 p_0 without reading x).  20 B per voxel.
 //   MODE 3: CG iteration k, second half - q = A p_new again from src =
 //           p_new (the same operations, so the same bits), r -= alpha_k q,
 //           partials of r . dinv r and r . r.  12 B per voxel.
@@ -217,7 +219,7 @@ constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8
 template <int MODE>
 __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, const float* __restrict__ src,
                                                     const float* __restrict__ pold, float* __restrict__ pnew,
-                                                    float* __restrict__ dst) {
+                                                    float* __restrict__ dst, int xzero) {
     static_assert(MODE == 0 || MODE == 2 || MODE == 3, "mode");
     __shared__ float t[2][kTY4 + 2][kRow4];
     float beta = 0.0f, di = 1.0f, alpha = 0.0f;
@@ -241,6 +243,8 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
         di = d.dinv_c;
     }
     const bool upd_x = MODE == 2 && k > 0;
+    const bool read_x = upd_x && !(xzero && k == 1);
+    const bool use_pold = MODE == 2 && k > 0;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int tx = (nx + kTX4 - 1) / kTX4, ty = (ny + kTY4 - 1) / kTY4, tz = (nz + kZ4 - 1) / kZ4;
     const int ntiles = tx * ty * tz;
@@ -276,13 +280,15 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
             if (!ok) return z4;
             if (MODE != 2) return *reinterpret_cast<const float4*>(src + e);
             const float4 a = *reinterpret_cast<const float4*>(d.r + e);
+            if (!use_pold) return make_float4(a.x * di, a.y * di, a.z * di, a.w * di);
             const float4 b = *reinterpret_cast<const float4*>(pold + e);
             return make_float4(a.x * di + beta * b.x, a.y * di + beta * b.y, a.z * di + beta * b.z,
                                a.w * di + beta * b.w);
         };
         auto val1 = [&](bool ok, int64_t e) -> float {
             if (!ok) return 0.0f;
-            return MODE == 2 ? d.r[e] * di + beta * pold[e] : src[e];
+            if (MODE != 2) return src[e];
+            return use_pold ? d.r[e] * di + beta * pold[e] : d.r[e] * di;
         };
         // the own column: the tile value, and p_old and x (MODE 2) or r
         // (MODE 3) of the same point
@@ -292,8 +298,9 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
             if (!ok) return z4;
             if (MODE == 2) {
                 const float4 a = *reinterpret_cast<const float4*>(d.r + e);
+                if (!use_pold) return make_float4(a.x * di, a.y * di, a.z * di, a.w * di);
                 po = *reinterpret_cast<const float4*>(pold + e);
-                if (upd_x) xr = *reinterpret_cast<const float4*>(dst + e);
+                if (read_x) xr = *reinterpret_cast<const float4*>(dst + e);
                 return make_float4(a.x * di + beta * po.x, a.y * di + beta * po.y, a.z * di + beta * po.z,
                                    a.w * di + beta * po.w);
             }
@@ -397,11 +404,16 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
 
 // The deferred x += alpha_k p_k of the last executed iteration k of the
 // 7-point fast path (k_stencil7x4 modes 2 and 3); nothing when none is
-// pending.  p_k is in pa for odd k, pb for even k.
+// pending.  p_k is in pa for odd k, pb for even k.  xzero: the solve started
+// from x = 0 without storing it, so after one iteration x = alpha_0 p_0.
+// acc: instead of storing x, acc += (double) x (the refinement's correction
+// added to its FP64 iterate).
 __global__ __launch_bounds__(256) void k_cg_finish(int64_t n, CgDev d, const float* __restrict__ pa,
-                                                   const float* __restrict__ pb, float* __restrict__ x) {
+                                                   const float* __restrict__ pb, float* __restrict__ x, int xzero,
+                                                   double* __restrict__ acc) {
     if (!d.fl[FL_XPEND]) return;
     const int k = d.fl[FL_ITERS] - 1;
+    const bool read_x = !(xzero && k == 0);
     double rz[2], pq[1];
     sum_partials<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, rz);
     sum_partials<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, pq);
@@ -409,15 +421,150 @@ __global__ __launch_bounds__(256) void k_cg_finish(int64_t n, CgDev d, const flo
     const float* __restrict__ p = (k & 1) ? pa : pb;
     const int64_t n4 = n / 4, stride = (int64_t)gridDim.x * 256;
     for (int64_t c = blockIdx.x * 256ll + threadIdx.x; c < n4; c += stride) {
-        const float4 xv = reinterpret_cast<const float4*>(x)[c];
+        const float4 xv = read_x ? reinterpret_cast<const float4*>(x)[c] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 pv = reinterpret_cast<const float4*>(p)[c];
         float4 xo;
         xo.x = xv.x + alpha * pv.x;
         xo.y = xv.y + alpha * pv.y;
         xo.z = xv.z + alpha * pv.z;
         xo.w = xv.w + alpha * pv.w;
-        reinterpret_cast<float4*>(x)[c] = xo;
+        if (acc) {
+            double2* a2 = reinterpret_cast<double2*>(acc) + 2 * c;
+            double2 lo = a2[0], hi = a2[1];
+            lo.x = lo.x + (double)xo.x;
+            lo.y = lo.y + (double)xo.y;
+            hi.x = hi.x + (double)xo.z;
+            hi.y = hi.y + (double)xo.w;
+            a2[0] = lo;
+            a2[1] = hi;
+        } else {
+            reinterpret_cast<float4*>(x)[c] = xo;
+        }
     }
+}
+
+// The FP64 residual of the refinement on the constant 7-point grid, tiled
+// as k_stencil7x4: r = b - A x, A x in the operation order of
+// apply_at<double, double>, and the partials of ||r||^2; unless FINAL also
+// the inner CG's start on r32 = (float) r - residual r32 and the partials
+// k_cg_init would leave (the inner CG's direction and iterate start from
+// zero without being stored: k_stencil7x4 and k_cg_finish with xzero).
+// 20 B per voxel (16 when FINAL).  Launched with at most kRedBlocks
+// workgroups; block 0 zeroes the partial slots beyond them.
+template <bool FINAL>
+__global__ __launch_bounds__(kT4) void k_res64x4(GridK g, const double* __restrict__ x,
+                                                 const double* __restrict__ b, CgDev d, double* prr,
+                                                 const int32_t* __restrict__ pfl) {
+    if (pfl[0]) return;
+    __shared__ double t[2][kTY4 + 2][kRow4];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int tx = (nx + kTX4 - 1) / kTX4, ty = (ny + kTY4 - 1) / kTY4, tz = (nz + kZ4 - 1) / kZ4;
+    const int ntiles = tx * ty * tz;
+    const int64_t sxy = (int64_t)nx * ny;
+    const int tt = threadIdx.x, lx = tt & 63, ly = tt >> 6;
+    const bool hrow = tt < 128, hcol = tt >= 128 && tt < 128 + 2 * kTY4;
+    const int hy = tt < 64 ? -1 : tt < 128 ? kTY4 : (tt < 128 + kTY4 ? tt - 128 : tt - 128 - kTY4);
+    const int hl = tt < 64 ? tt : tt - 64;
+    const int hslot = tt < 128 + kTY4 ? 3 : 4 + kTX4;
+    const double cdiag = g.diagd, cc = g.cd;
+    const float di = d.dinv_c;
+    // (||r32||^2 is both the inner CG's bb and its first rr)
+    double vr[1] = {0}, vu[2] = {0, 0};
+    struct D4 {
+        double x, y, z, w;
+    };
+    auto ld4 = [&](bool ok, const double* p) -> D4 {
+        if (!ok) return D4{0, 0, 0, 0};
+        const double2 a = reinterpret_cast<const double2*>(p)[0], c2 = reinterpret_cast<const double2*>(p)[1];
+        return D4{a.x, a.y, c2.x, c2.y};
+    };
+    auto st4 = [&](double* p, const D4& v) {
+        reinterpret_cast<double2*>(p)[0] = make_double2(v.x, v.y);
+        reinterpret_cast<double2*>(p)[1] = make_double2(v.z, v.w);
+    };
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
+        const int x0 = bx * kTX4, k0 = bz * kZ4, kend = min(k0 + kZ4, nz);
+        const int j = by * kTY4 + ly;
+        const int xo = x0 + 4 * lx;
+        const bool own = xo < nx && j < ny;
+        const int gj = by * kTY4 + hy;
+        const bool hr_ok = hrow && gj >= 0 && gj < ny && x0 + 4 * hl < nx;
+        const int hx = tt < 128 + kTY4 ? x0 - 1 : x0 + kTX4;
+        const bool hc_ok = hcol && gj < ny && hx >= 0 && hx < nx;
+        const int64_t ocol = xo + (int64_t)nx * j;
+        const int64_t rcol = x0 + 4 * hl + (int64_t)nx * gj;
+        const int64_t ccol = hx + (int64_t)nx * gj;
+        D4 pm = ld4(own && k0 > 0, x + ocol + sxy * (k0 - 1));
+        D4 pc = ld4(own, x + ocol + sxy * k0);
+        st4(&t[0][ly + 1][4 + 4 * lx], pc);
+        if (hrow) st4(&t[0][hy + 1][4 + 4 * hl], ld4(hr_ok, x + rcol + sxy * k0));
+        if (hcol) t[0][hy + 1][hslot] = hc_ok ? x[ccol + sxy * k0] : 0.0;
+        auto plane = [&](int kz) {
+            const int bf = (kz - k0) & 1;
+            const bool nxt = kz + 1 < nz;
+            const D4 pp = ld4(own && nxt, x + ocol + sxy * (kz + 1));
+            const D4 bc = ld4(own, b + ocol + sxy * kz);
+            const D4 hr = ld4(hr_ok && nxt, x + rcol + sxy * (kz + 1));
+            const double hc = hc_ok && nxt ? x[ccol + sxy * (kz + 1)] : 0.0;
+            __syncthreads();
+            if (own) {
+                const double left = t[bf][ly + 1][3 + 4 * lx], right = t[bf][ly + 1][8 + 4 * lx];
+                const double* ym = &t[bf][ly][4 + 4 * lx];
+                const double* yp = &t[bf][ly + 2][4 + 4 * lx];
+                const double xs[6] = {left, pc.x, pc.y, pc.z, pc.w, right};
+                const double bs[4] = {bc.x, bc.y, bc.z, bc.w};
+                const double ms[4] = {pm.x, pm.y, pm.z, pm.w};
+                const double ps[4] = {pp.x, pp.y, pp.z, pp.w};
+                float r32[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    double s = xs[q] + xs[q + 2];
+                    s = s + ym[q];
+                    s = s + yp[q];
+                    s = s + ms[q];
+                    s = s + ps[q];
+                    const double r = bs[q] - (cdiag * xs[q + 1] - cc * s);
+                    vr[0] += r * r;
+                    r32[q] = (float)r;
+                }
+                if (!FINAL) {
+                    *reinterpret_cast<float4*>(d.r + ocol + sxy * kz) = make_float4(r32[0], r32[1], r32[2], r32[3]);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        vu[0] += (double)r32[q] * (r32[q] * di);
+                        vu[1] += (double)r32[q] * r32[q];
+                    }
+                }
+            }
+            st4(&t[bf ^ 1][ly + 1][4 + 4 * lx], pp);
+            if (hrow) st4(&t[bf ^ 1][hy + 1][4 + 4 * hl], hr);
+            if (hcol) t[bf ^ 1][hy + 1][hslot] = hc;
+            pm = pc;
+            pc = pp;
+        };
+        if (kend - k0 == kZ4) {
+#pragma unroll
+            for (int q = 0; q < kZ4; ++q) plane(k0 + q);
+        } else {
+            for (int kz = k0; kz < kend; ++kz) plane(kz);
+        }
+        __syncthreads();
+    }
+    put_partial<1>(vr, prr);
+    if (!FINAL) {
+        put_partial<2>(vu, d.pu0);
+        if (threadIdx.x == 0) d.pb[blockIdx.x] = vu[1];
+    }
+    if (blockIdx.x == 0)
+        for (int i = (int)gridDim.x + (int)threadIdx.x; i < kRedBlocks; i += (int)blockDim.x) {
+            prr[i] = 0.0;
+            if (!FINAL) {
+                d.pb[i] = 0.0;
+                d.pu0[2 * i] = 0.0;
+                d.pu0[2 * i + 1] = 0.0;
+            }
+        }
 }
 
 static int stencil4_blocks(const GridK& g) {
@@ -673,13 +820,15 @@ static nnx_status cg_readback(nnx_ctx* c, CgHost& h, int32_t* fl, double* sc) {
 // otherwise the new direction, q = A p, then the update of x and r.  Every
 // kernel returns at once when the device flag says the solve is over, so
 // iterations can be enqueued past convergence.
-static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float* x, int vec, bool fast, int nb4) {
+static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float* x, int vec, bool fast, int nb4,
+                           int xzero = 0) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
     float* pold = (k & 1) ? h.pb : h.pa;
     float* pnew = (k & 1) ? h.pa : h.pb;
     if (fast) {
-        k_stencil7x4<2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x);
-        k_stencil7x4<3><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr);
+        k_stencil7x4<2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
+        k_stencil7x4<3><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
+                                                                         0);
         return;
     }
     k_cg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pold, pnew);
@@ -692,9 +841,10 @@ static bool cg_fast(const GridK& gk, const float* x) {
 }
 
 // the pending x update of the fast path (nothing otherwise)
-static void cg_finish_launch(nnx_ctx* c, const GridK& gk, CgHost& h, float* x) {
+static void cg_finish_launch(nnx_ctx* c, const GridK& gk, CgHost& h, float* x, int xzero = 0,
+                             double* acc = nullptr) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
-    k_cg_finish<<<flat_blocks(n / 4), 256, 0, c->stream>>>(n, h.d, h.pa, h.pb, x);
+    k_cg_finish<<<flat_blocks(n / 4), 256, 0, c->stream>>>(n, h.d, h.pa, h.pb, x, xzero, acc);
 }
 
 // Runs CG on the prepared buffers; x holds the guess when use_x0.  The host
@@ -831,13 +981,14 @@ __global__ __launch_bounds__(256) void k_para_res(GridK g, const double* __restr
     }
 }
 
-// Refinement control after k_para_res: relres = ||r|| / ||b||.  Done when
-// relres <= rtol or at the final check (the outcome goes to stats);
+// Refinement control after the residual pass: relres = ||r|| / ||b||.  Done
+// when relres <= rtol or at the final check (the outcome goes to stats);
 // otherwise the inner CG's state for the tolerance max(inner_rtol,
-// rtol / (2 relres)).
+// rtol / (2 relres)).  after_inner: counts the refinement step just ended
+// and its inner iterations.
 __global__ __launch_bounds__(256) void k_para_ctl(CgDev d, const double* prr, const double* pbb, double rtol,
                                                   double inner_rtol, int maxit, int32_t* pfl, double* stats,
-                                                  int final_check) {
+                                                  int final_check, int after_inner) {
     if (pfl[0]) return;
     double rr[1], bb[1], bin[1] = {0}, u[2] = {0, 0};
     sum_partials<1>(prr, kRedBlocks, rr);
@@ -847,6 +998,10 @@ __global__ __launch_bounds__(256) void k_para_ctl(CgDev d, const double* prr, co
         sum_partials<2>(d.pu0, kRedBlocks, u);
     }
     if (threadIdx.x != 0) return;
+    if (after_inner) {
+        pfl[1] += 1;
+        pfl[2] += d.fl[FL_ITERS];
+    }
     const double rel = rr[0] <= 0 ? 0.0 : (bb[0] > 0 ? sqrt(rr[0] / bb[0]) : INFINITY);
     d.fl[FL_XPEND] = 0;
     if (rel <= rtol || final_check) {
@@ -876,16 +1031,12 @@ __global__ __launch_bounds__(256) void k_para_ctl(CgDev d, const double* prr, co
     d.fl[FL_ITERS] = 0;
 }
 
-// x += correction; counts the refinement step and its inner iterations
+// x += correction (none when the inner CG ran no iteration)
 __global__ __launch_bounds__(256) void k_para_axpy(int64_t n, const float* __restrict__ xin, double* __restrict__ x,
-                                                   const int32_t* __restrict__ cgfl, int32_t* pfl) {
-    if (pfl[0]) return;
+                                                   const int32_t* __restrict__ cgfl, const int32_t* __restrict__ pfl) {
+    if (pfl[0] || cgfl[FL_ITERS] == 0) return;
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         x[i] = x[i] + (double)xin[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pfl[1] += 1;
-        pfl[2] += cgfl[FL_ITERS];
-    }
 }
 
 // the field at each neuron's voxel (the write-back of the solve step)
@@ -1100,7 +1251,7 @@ nnx_status nnx_stencil_apply(nnx_ctx* c, const nnx_grid* g, const float* x, floa
     const GridK gk = make_grid(g);
     if (gk.kind == 0 && gk.nx % 4 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0) {
         CgDev none{};
-        k_stencil7x4<0><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y);
+        k_stencil7x4<0><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
     } else {
         k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
     }
@@ -1259,21 +1410,36 @@ nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
     const int max_inner = q.max_inner > 0 ? q.max_inner : 64;
     const int max_outer = q.max_outer > 0 ? q.max_outer : 3;
     const double inner_rtol = q.inner_rtol > 0 ? q.inner_rtol : 1e-6;
-    const bool fast = cg_fast(gk, ph.xin);
-    const int nb4 = stencil4_blocks(gk);
+    // the tiled path: constant 7-point grid, rows of a multiple of 4 voxels,
+    // 16-byte aligned vectors
+    const bool fast = cg_fast(gk, ph.xin) && (uintptr_t)q.conc % 16 == 0;
+    const int nb4 = stencil4_blocks(gk), nbr = std::min(nb4, kRedBlocks);
     h.d.npq = fast ? nb4 : kRedBlocks;
     k_para_rhs<<<kRedBlocks, 256, 0, c->stream>>>(nv, q.conc, p->voxel_counts, q.release, ph.b, ph.pbb, ph.pfl);
     for (int o = 0; o < max_outer; ++o) {
         // the later refinement steps solve to a looser tolerance: half the budget
         const int budget = o == 0 ? max_inner : std::max(1, max_inner / 2);
-        k_para_res<false><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
-        k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, budget, ph.pfl, q.stats, 0);
-        for (int k = 0; k < budget; ++k) cg_iter_launch(c, gk, h, k, ph.xin, 1, fast, nb4);
-        cg_finish_launch(c, gk, h, ph.xin);
-        k_para_axpy<<<flat_blocks(nv), 256, 0, c->stream>>>(nv, ph.xin, q.conc, h.d.fl, ph.pfl);
+        if (fast)
+            k_res64x4<false><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
+        else
+            k_para_res<false><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr,
+                                                                 ph.pfl);
+        k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, budget, ph.pfl, q.stats, 0,
+                                             o > 0);
+        for (int k = 0; k < budget; ++k) cg_iter_launch(c, gk, h, k, ph.xin, 1, fast, nb4, fast);
+        if (fast) {
+            cg_finish_launch(c, gk, h, ph.xin, 1, q.conc);
+        } else {
+            cg_finish_launch(c, gk, h, ph.xin);
+            k_para_axpy<<<flat_blocks(nv), 256, 0, c->stream>>>(nv, ph.xin, q.conc, h.d.fl, ph.pfl);
+        }
     }
-    k_para_res<true><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
-    k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, max_inner, ph.pfl, q.stats, 1);
+    if (fast)
+        k_res64x4<true><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
+    else
+        k_para_res<true><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
+    k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, max_inner, ph.pfl, q.stats, 1,
+                                         max_outer > 0);
     if (q.at_neurons)
         k_para_neurons<<<flat_blocks(p->state.n), 256, 0, c->stream>>>(p->state.n, p->voxel_of, q.conc, q.at_neurons);
     NNX_LAUNCHED(c);
