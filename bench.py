@@ -193,12 +193,15 @@ def precision_split(ctx, net, steps=(1, 100, 10000)):
     return res
 
 
-def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=0, tune=False):
+def c4_bench(ctx, steps=5000, chunk=100, tune_steps=1000, warmup=100, tune=True):
     """Batched ensemble: 256 networks (sizes 2^U(10,16), Poisson(50)
     out-degree), one fused persistent launch per step, 100 steps per
-    hipGraph, the library's default launch shape (with tune, the
-    autotuner's, whose steps are ordinary steps run after `warmup` steps and
-    before the timed region)."""
+    hipGraph, in the launch shape the autotuner picks: every candidate is
+    timed on the same window of `tune_steps` steps after `warmup` steps (the
+    initial volley and the quiet steps after it), which the search restores.
+    The library's default shape is timed on the same steps from the same
+    state in a second simulation (the trajectory does not depend on the
+    shape), so the two times compare like for like."""
     from neuronephys_clean_amd import _lib, configs
     from neuronephys_clean_amd.sim import EnsembleSim
     t0 = time.time()
@@ -206,38 +209,60 @@ def c4_bench(ctx, steps=5000, chunk=100, tune_steps=30, warmup=0, tune=False):
     live = int(ens.size.sum())
     log(f"bench: C4 ensemble {len(ens.size)} networks, {live} neurons, {ens.nnz} edges, "
         f"built in {time.time() - t0:.1f}s")
-    sim = EnsembleSim(ens, ctx=ctx, deposit=True)
-    if warmup:
-        sim.run(warmup, use_graph=True, chunk=chunk)
-    if tune:
-        best, tune_us, ran = sim.autotune(tune_steps)
-        log(f"bench: C4 autotune -> {best.as_dict()} ({tune_us:.1f} us/step, {ran} steps)")
-    else:
-        best, tune_us, ran = _lib.EnsConfig(block=256, tile=256, grid=8192), None, 0
-        sim.cfg = best
-    sim.run(chunk - sim.step % chunk, use_graph=False)  # align the graph chunks
-    sim.run(chunk, use_graph=True, chunk=chunk)         # capture
-    sim.synchronize()
+    default = _lib.EnsConfig(block=256, tile=256, grid=8192)
     e0, e1 = C.c_void_p(), C.c_void_p()
     ctx.call("nnx_event_create", C.byref(e0))
     ctx.call("nnx_event_create", C.byref(e1))
-    spk0 = int(sim.a["net_spikes"].numpy().sum())
-    ctx.call("nnx_event_record", e0)
-    sim.run(steps, use_graph=True, chunk=chunk)
-    ctx.call("nnx_event_record", e1)
-    sim.synchronize()
-    ms = C.c_float()
-    ctx.call("nnx_event_elapsed", e0, e1, C.byref(ms))
-    spikes = int(sim.a["net_spikes"].numpy().sum()) - spk0
-    sec = ms.value * 1e-3
-    bytes_step = 48.125 * live
+
+    def timed(cfg, autotune):
+        sim = EnsembleSim(ens, ctx=ctx, deposit=True)
+        sim.cfg = cfg
+        sim.run(warmup, use_graph=True, chunk=chunk)
+        info = None
+        if autotune:
+            t1 = time.time()
+            best, tune_us, ran = sim.autotune(tune_steps)
+            info = {"config": best.as_dict(), "us_per_step": tune_us, "window": [sim.step, sim.step + tune_steps],
+                    "search_s": time.time() - t1}
+            log(f"bench: C4 autotune {info}")
+            cfg = sim.cfg = best
+        sim.run(chunk - sim.step % chunk, use_graph=False)  # align the graph chunks
+        sim.run(chunk, use_graph=True, chunk=chunk)         # capture
+        sim.synchronize()
+        first = sim.step
+        spk0 = int(sim.a["net_spikes"].numpy().sum())
+        ctx.call("nnx_event_record", e0)
+        sim.run(steps, use_graph=True, chunk=chunk)
+        ctx.call("nnx_event_record", e1)
+        sim.synchronize()
+        ms = C.c_float()
+        ctx.call("nnx_event_elapsed", e0, e1, C.byref(ms))
+        spikes = int(sim.a["net_spikes"].numpy().sum()) - spk0
+        v = sim.a["v"].numpy()
+        del sim
+        return cfg, ms.value, spikes, first, info, v
+
+    best, ms, spikes, first, info, v_best = timed(default, tune)
     out = {"networks": len(ens.size), "neurons": live, "edges": ens.nnz, "steps": steps,
-           "graph_chunk": chunk, "ms": ms.value, "us_per_step": 1e6 * sec / steps,
-           "neuron_steps_per_s": live * steps / sec, "spikes_per_step": spikes / steps,
-           "config": best.as_dict(), "autotune_us_per_step": tune_us, "autotune_steps": ran, "warmup": warmup,
-           "hbm_GBps_state": bytes_step * steps / sec / 1e9,
-           "frac_hbm_state": bytes_step * steps / sec / HBM_PEAK}
-    del sim
+           "timed_steps": [first, first + steps], "graph_chunk": chunk}
+    sec = ms * 1e-3
+    bytes_step = 48.125 * live
+    out.update({"ms": ms, "us_per_step": 1e6 * sec / steps, "neuron_steps_per_s": live * steps / sec,
+                "spikes_per_step": spikes / steps, "config": best.as_dict(), "autotuned": tune,
+                "autotune": info, "warmup": warmup,
+                "hbm_GBps_state": bytes_step * steps / sec / 1e9,
+                "frac_hbm_state": bytes_step * steps / sec / HBM_PEAK})
+    if tune:
+        if best.as_dict() == default.as_dict():
+            out["default"] = {"config": default.as_dict(), "us_per_step": out["us_per_step"], "same_shape": True}
+        else:
+            _, ms_d, spikes_d, first_d, _, v_d = timed(default, False)
+            out["default"] = {"config": default.as_dict(), "us_per_step": 1e3 * ms_d / steps,
+                              "same_steps": first_d == first and spikes_d == spikes,
+                              "same_final_v": bool(np.array_equal(v_d, v_best))}
+        out["autotuned_over_default"] = out["default"]["us_per_step"] / out["us_per_step"]
+    ctx.call("nnx_event_destroy", e0)
+    ctx.call("nnx_event_destroy", e1)
     return out
 
 
@@ -464,12 +489,13 @@ def restore_snapshot(sim, snap):
 
 
 def cg_iteration_roofline(ctx, n, b32, c, k1=10, k2=40):
-    """One Jacobi-CG iteration on the n^3 constant 7-point grid (the fused
-    direction + stencil kernel and the update kernel): the difference of two
-    capturable solves of k2 and k1 iterations with rtol 0 (every iteration
-    works), HIP events on the library stream.  Algorithmic bytes per
-    iteration: 40 per voxel (read r, p_old; write p, q; read x, p, r, q;
-    write x, r)."""
+    """One Jacobi-CG iteration on the n^3 constant 7-point grid (its two
+    tiled launches): the difference of two capturable solves of k2 and k1
+    iterations with rtol 0 (every iteration works), HIP events on the
+    library stream.  Algorithmic bytes per iteration: 32 per voxel - first
+    half: read r, p_old and x, write p and x (the previous iteration's
+    deferred update), q = A p formed in LDS and not stored; second half:
+    read p and r, write r (q recomputed in LDS)."""
     from neuronephys_clean_amd import _lib
     from neuronephys_clean_amd.devmem import DeviceArray as DA
     g = _lib.Grid(n, n, n, 7, c, 0.0, None, None, None)
@@ -482,12 +508,12 @@ def cg_iteration_roofline(ctx, n, b32, c, k1=10, k2=40):
     t1 = timed_events(ctx, lambda: solve(k1), 5)
     t2 = timed_events(ctx, lambda: solve(k2), 5)
     us = (t2 - t1) * 1e3 / (k2 - k1)
-    b = 40.0 * n ** 3
-    return {"kernel": "k_stencil7x4<true> + k_cg_update (one Jacobi-CG iteration, 256^3 constant 7-point)",
+    b = 32.0 * n ** 3
+    return {"kernel": f"k_stencil7x4<2> + k_stencil7x4<3> (one Jacobi-CG iteration, {n}^3 constant 7-point)",
             "bound": "hbm", "achieved": b / (us * 1e-6) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": b / (us * 1e-6) / HBM_PEAK, "traffic": profiled_traffic("cg-iteration"),
             "avg_iteration_us": us, "alg_bytes_per_launch": b,
-            "alg_bytes_rule": "40 B per voxel per iteration (two launches)",
+            "alg_bytes_rule": "32 B per voxel per iteration (20 + 12, two launches)",
             "time_source": f"HIP events: ({k2}-iteration solve - {k1}-iteration solve) / {k2 - k1}"}
 
 
@@ -881,8 +907,9 @@ def main():
     n_solves = int(pst["solves"])
     paracrine = {"every": every, "solves": n_solves, "grid": f"{net.grid[0]}^3", "c": net.c,
                  "method": "Jacobi-CG (fp32 inner, FP64 dots) with FP64 iterative refinement, warm start from the "
-                           "previous field, captured in the step graphs (40 inner iterations x 2 refinement "
-                           "steps enqueued per solve, kernels after convergence return at once)",
+                           "previous field, captured in the step graphs (2 refinement steps of at most 40 and 20 "
+                           "inner iterations enqueued per solve, kernels after convergence return at once; "
+                           "constant 7-point grid: tiled FP64 residual, two tiled launches per inner iteration)",
                  "rtol_true_fp64": 1e-8, "max_relres": pst["max_relres"], "unconverged": int(pst["unconverged"]),
                  "inner_iters_per_solve": pst["inner_iters"] / max(n_solves, 1),
                  "outer_per_solve": pst["outer_steps"] / max(n_solves, 1),
@@ -992,7 +1019,7 @@ def main():
 
     c4 = None
     if not args.no_c4:
-        c4 = c4_bench(ctx, tune=args.tune)
+        c4 = c4_bench(ctx)
         log(f"bench: C4 {c4}")
 
     c3 = None

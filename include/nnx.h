@@ -497,9 +497,17 @@ typedef struct nnx_ens_config {
  * result does not depend on cfg.                                            */
 nnx_status nnx_ensemble_run(nnx_ctx* ctx, const nnx_ens_plan* plan, const nnx_ens_config* cfg,
                             uint64_t first_step, int n_steps, int use_graph);
-/* Coordinate search over block, tile and grid, timing steps_per_candidate
- * real steps per candidate (the simulation advances by *steps_run steps;
- * the trajectory is the same as with any configuration).                   */
+/* Coordinate search over block, tile and grid (from the default
+ * {256, 256, 8192}; a candidate replaces the current best only when faster).
+ * Every candidate times the same window of steps_per_candidate steps from
+ * first_step, from the same state: the state the steps change (neuron
+ * state, pending events, spike bitmaps, step counters) is saved in a scratch
+ * buffer and restored before each timed run and after the search, and the
+ * candidates run without the plan's accumulating outputs (voxel deposit,
+ * spike counts, raster).  So the simulation does not advance (*steps_run =
+ * 0) and no output changes; choose a window that holds the bursts the run
+ * will see.  Each candidate: one untimed run, then the faster of two timed
+ * ones.  Synchronizes the stream; not under capture.                       */
 nnx_status nnx_ensemble_autotune(nnx_ctx* ctx, const nnx_ens_plan* plan, uint64_t first_step,
                                  int steps_per_candidate, nnx_ens_config* best, double* best_us_per_step,
                                  int* steps_run);

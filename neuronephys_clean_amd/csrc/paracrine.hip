@@ -215,12 +215,17 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 //           partials of r . dinv r and r . r.  12 B per voxel.
 // The last iteration's x += alpha p is left to k_cg_finish.  Together the
 // two halves move 32 B per voxel where storing q would take 40.
+// PF: planes loaded ahead of the one being computed.  With 1, plane k+1's
+// loads are issued at the start of plane k and its z+1 neighbours wait for
+// them; with 2 they were issued one plane earlier, which hides a memory
+// latency per plane (a workgroup marches its tile alone on its CU).
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
-template <int MODE>
+template <int MODE, int PF>
 __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, const float* __restrict__ src,
                                                     const float* __restrict__ pold, float* __restrict__ pnew,
                                                     float* __restrict__ dst, int xzero) {
     static_assert(MODE == 0 || MODE == 2 || MODE == 3, "mode");
+    static_assert(PF == 1 || PF == 2, "prefetch depth");
     __shared__ float t[2][kTY4 + 2][kRow4];
     float beta = 0.0f, di = 1.0f, alpha = 0.0f;
     if (MODE == 2) {
@@ -291,7 +296,9 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
             return use_pold ? d.r[e] * di + beta * pold[e] : d.r[e] * di;
         };
         // the own column: the tile value, and p_old and x (MODE 2) or r
-        // (MODE 3) of the same point
+        // (MODE 3) of the same point.  MODE 2 with PF == 2 reads x only in
+        // its own plane's step (it is not a neighbour value: registers).
+        constexpr bool late_x = MODE == 2 && PF == 2;
         auto own4 = [&](bool ok, int64_t e, float4& po, float4& xr) -> float4 {
             po = z4;
             xr = z4;
@@ -300,7 +307,7 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
                 const float4 a = *reinterpret_cast<const float4*>(d.r + e);
                 if (!use_pold) return make_float4(a.x * di, a.y * di, a.z * di, a.w * di);
                 po = *reinterpret_cast<const float4*>(pold + e);
-                if (read_x) xr = *reinterpret_cast<const float4*>(dst + e);
+                if (read_x && !late_x) xr = *reinterpret_cast<const float4*>(dst + e);
                 return make_float4(a.x * di + beta * po.x, a.y * di + beta * po.y, a.z * di + beta * po.z,
                                    a.w * di + beta * po.w);
             }
@@ -313,13 +320,32 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
         *reinterpret_cast<float4*>(&t[0][ly + 1][4 + 4 * lx]) = pc;
         if (hrow) *reinterpret_cast<float4*>(&t[0][hy + 1][4 + 4 * hl]) = val4(hr_ok, rcol + sxy * k0);
         if (hcol) t[0][hy + 1][hslot] = val1(hc_ok, ccol + sxy * k0);
+        // PF == 2: plane k+1 of the march in registers (z + 1 <= kend)
+        float4 pp1 = z4, pop1 = z4, xrp1 = z4, hr1 = z4;
+        float hc1 = 0.0f;
+        if (PF == 2) {
+            const bool ok1 = k0 + 1 < nz;
+            pp1 = own4(own && ok1, ocol + sxy * (k0 + 1), pop1, xrp1);
+            hr1 = val4(hr_ok && ok1, rcol + sxy * (k0 + 1));
+            hc1 = val1(hc_ok && ok1, ccol + sxy * (k0 + 1));
+        }
         auto plane = [&](int kz) {
             const int b = (kz - k0) & 1;
-            const bool nxt = kz + 1 < nz;
-            float4 pop, xrp;
-            const float4 pp = own4(own && nxt, ocol + sxy * (kz + 1), pop, xrp);
-            const float4 hr = val4(hr_ok && nxt, rcol + sxy * (kz + 1));
-            const float hc = val1(hc_ok && nxt, ccol + sxy * (kz + 1));
+            float4 pp, pop, xrp, hr;
+            float hc;
+            if (PF == 1) {
+                const bool nxt = kz + 1 < nz;
+                pp = own4(own && nxt, ocol + sxy * (kz + 1), pop, xrp);
+                hr = val4(hr_ok && nxt, rcol + sxy * (kz + 1));
+                hc = val1(hc_ok && nxt, ccol + sxy * (kz + 1));
+            } else {
+                pp = pp1, pop = pop1, xrp = xrp1, hr = hr1, hc = hc1;
+                const bool n2 = kz + 2 < nz && kz + 2 <= kend;
+                pp1 = own4(own && n2, ocol + sxy * (kz + 2), pop1, xrp1);
+                hr1 = val4(hr_ok && n2, rcol + sxy * (kz + 2));
+                hc1 = val1(hc_ok && n2, ccol + sxy * (kz + 2));
+            }
+            if (late_x && own && read_x) xrc = *reinterpret_cast<const float4*>(dst + ocol + sxy * kz);
             __syncthreads();
             if (own) {
                 const float left = t[b][ly + 1][3 + 4 * lx], right = t[b][ly + 1][8 + 4 * lx];
@@ -826,8 +852,8 @@ static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float*
     float* pold = (k & 1) ? h.pb : h.pa;
     float* pnew = (k & 1) ? h.pa : h.pb;
     if (fast) {
-        k_stencil7x4<2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
-        k_stencil7x4<3><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
+        k_stencil7x4<2, 2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
+        k_stencil7x4<3, 2><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
                                                                          0);
         return;
     }
@@ -1251,7 +1277,7 @@ nnx_status nnx_stencil_apply(nnx_ctx* c, const nnx_grid* g, const float* x, floa
     const GridK gk = make_grid(g);
     if (gk.kind == 0 && gk.nx % 4 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0) {
         CgDev none{};
-        k_stencil7x4<0><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
+        k_stencil7x4<0, 2><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
     } else {
         k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
     }

@@ -1,7 +1,7 @@
 """The batched ensemble (nnx_ensemble_run / nnx_ensemble_autotune) on the
 GPU: every network's trajectory equals the oracle's run of that network
 alone, bit for bit; results do not depend on the launch configuration or on
-graph replay; the autotuner's steps are ordinary steps."""
+graph replay; the autotuner leaves the run untouched."""
 import ctypes as C
 
 import numpy as np
@@ -82,19 +82,28 @@ def test_ensemble_result_independent_of_config_and_graph(ctx, ens, cfg):
     assert ra[1] == rb[1] and np.array_equal(ra[0], rb[0])
 
 
-def test_ensemble_autotune_steps_are_ordinary_steps(ctx, ens):
-    t = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
-    best, us, ran = t.autotune(steps_per_candidate=3)
+def test_ensemble_autotune_leaves_the_run_untouched(ctx, ens):
+    """The search times every candidate on the same window (the first 150
+    steps: the initial volley) and restores the state after it: the run
+    that follows equals one that never tuned, outputs included."""
+    t = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20, deposit=True)
+    t.run(37)
+    best, us, ran = t.autotune(steps_per_candidate=150)
     say("autotune", best.as_dict(), us, "us/step over", ran, "steps")
+    assert ran == 0 and us > 0
     assert best.block in (64, 128, 256, 512) and best.tile % 64 == 0 and best.grid > 0
-    t.run(100 - ran % 100 + 50, use_graph=True, chunk=50)
-    ref = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
-    ref.run(t.step)
+    t.run(163, use_graph=True, chunk=50)
+    ref = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20, deposit=True)
+    ref.run(200)
     t.synchronize()
     ref.synchronize()
+    assert t.step == ref.step == 200
     for k, v in t.state().items():
         assert np.array_equal(v, ref.state()[k]), k
+    assert t.raster()[1] == ref.raster()[1] > 0
     assert np.array_equal(t.raster()[0], ref.raster()[0])
+    for k in ("net_spikes", "voxel_counts"):
+        assert np.array_equal(t.a[k].numpy(), ref.a[k].numpy()), k
 
 
 def test_full_ensemble_matches_oracle_per_network(ctx):
