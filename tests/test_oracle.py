@@ -128,6 +128,33 @@ def test_leapfrog_voltage_fixed_point_and_output():
     assert np.array_equal(sv[7], sv[10]) and np.array_equal(sv[7], sv[13])
 
 
+def test_leapfrog_white_noise_enters_as_the_drawn_drive():
+    """With noise, neuron i's drive at step s is noise_std z + noise_mean, z
+    the pinned standard normal of (seed, s, i): a drive chosen to make V0 the
+    fixed point including that draw keeps V at V0, and the same state at
+    another step (other draws) moves it."""
+    q = reference()
+    q["noise_std"], q["noise_seed"] = 1.5, 2024
+    p = oracle.params(q)
+    dt, step = 0.4, 77
+    V0 = np.array([-65.0, -58.0, -40.0, -61.0])
+    am, bm, ah, bh, an, bn = oracle.rates(V0).T
+    m, ng, h = am / (am + bm), an / (an + bn), ah / (ah + bh)
+    G = q["gna"] * m ** 3 * h + q["gk"] * ng ** 4 + q["gl"]
+    E = q["gna"] * m ** 3 * h * q["ena"] + q["gk"] * ng ** 4 * q["ek"] + q["gl"] * q["el"]
+    z = oracle.noise_normal(q["noise_seed"], step, V0.size)
+    assert np.all(np.abs(z) > 1e-3)
+    drive = G * V0 - E - (q["noise_std"] * z + q["noise_mean"])
+    sv = _soa(4, V0, (m, ng, h))
+    sv[16][:] = drive
+    oracle.hh_leapfrog(p, dt, sv, step=step)
+    np.testing.assert_allclose(sv[0], V0, rtol=1e-12)
+    sv = _soa(4, V0, (m, ng, h))
+    sv[16][:] = drive
+    oracle.hh_leapfrog(p, dt, sv, step=step + 1)
+    assert np.all(np.abs(sv[0] - V0) > 1e-6)
+
+
 def test_leapfrog_kinetic_synapse_fixed_point():
     q = reference()
     p = oracle.params(q)
