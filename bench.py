@@ -434,6 +434,11 @@ def profile_pass(ctx, sim, net):
     for _ in range(3):
         ctx.call("nnx_mg_apply", C.byref(fg), 0, 0, 0, rr.ptr, zz.ptr)
     ctx.synchronize()
+    # 60 Jacobi-CG iterations on the same operator (the tiled variable-coefficient iteration)
+    zz.upload(np.zeros(256 ** 3, np.float32))
+    ov = _lib.CgOpts(rtol=0.0, maxit=60, precond=_lib.PRECOND_JACOBI, check_every=20)
+    ctx._l.nnx_cg_solve(ctx.h, C.byref(fg), rr.ptr, zz.ptr, C.byref(ov), C.byref(_lib.CgResult()))
+    ctx.synchronize()
     del keep
     vc = sim.a["voxel_counts"].numpy().astype(np.float32)
     b = 0.01 * vc

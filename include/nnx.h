@@ -377,6 +377,8 @@ typedef struct nnx_cg_result {
     double relres;        /* final recursive residual ||r||/||b|| */
     double true_relres;   /* ||b - A x||/||b|| recomputed in FP64 */
     double ms;            /* wall time of the solve (host clock, includes syncs) */
+    double learned_weight; /* learned preconditioners: mean weight theta given to the
+                              network's correction (nnx_fcg_solve); 0 otherwise */
 } nnx_cg_result;
 
 /* y = A x for the grid's operator (fp32).                                   */
@@ -596,12 +598,15 @@ nnx_status nnx_mg_apply(nnx_ctx* ctx, const nnx_grid* g, int nu, int smoother, i
 /* Flexible CG (Notay, one stored direction): fp32 vectors, FP64 reductions,
  * precond LEARNED: a two-level preconditioner with the network N as the
  * coarse correction between damped-Jacobi smoothings (w = 2/3, D the
- * operator's diagonal): z0 = w D^-1 r, z1 = z0 + N(r - A z0), then
- * o->smooth sweeps z <- z + w D^-1 (r - A z) (at least one); d_feat is N's
- * feature input.
+ * operator's diagonal): z0 = w D^-1 r, u = N(r - A z0),
+ * z1 = z0 + theta u with theta = u.(r - A z0) / u.(A u) (the weight that
+ * minimises the energy norm of the error left, so the network's correction
+ * never makes z1 worse than z0 in that norm; res->learned_weight reports
+ * its mean), then o->smooth sweeps z <- z + w D^-1 (r - A z) (at least
+ * one); d_feat is N's feature input.
  * precond MG: one multigrid cycle (nnx_mg_apply with o->mg_nu, mg_smoother, mg_cycle).
- * precond MG_LEARNED: the V-cycle in place of z0 above: z0 = V(r),
- * z1 = z0 + N(r - A z0), then o->smooth damped-Jacobi sweeps.
+ * precond MG_LEARNED: the cycle in place of z0 above: z0 = V(r), then u,
+ * theta and z1 as above and o->smooth damped-Jacobi sweeps.
  * precond JACOBI: M = D^-1;
  * after o->stall_iters (default 10) iterations without a new best
  * residual a learned preconditioner is replaced by its base (Jacobi or the

@@ -75,7 +75,7 @@ class CgOpts(C.Structure):
 class CgResult(C.Structure):
     _fields_ = [("iters", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32),
                 ("precond_used", C.c_int32), ("relres", C.c_double), ("true_relres", C.c_double),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("learned_weight", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -220,13 +220,14 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 // them; with 2 they were issued one plane earlier, which hides a memory
 // latency per plane (a workgroup marches its tile alone on its CU).
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
-template <int MODE, int PF>
-__global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, const float* __restrict__ src,
+template <int MODE, int PF, bool VAR = false, int TY = kTY4>
+__global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k, const float* __restrict__ src,
                                                     const float* __restrict__ pold, float* __restrict__ pnew,
                                                     float* __restrict__ dst, int xzero) {
     static_assert(MODE == 0 || MODE == 2 || MODE == 3, "mode");
     static_assert(PF == 1 || PF == 2, "prefetch depth");
-    __shared__ float t[2][kTY4 + 2][kRow4];
+    static_assert(!VAR || MODE == 2, "variable coefficients: the first half only (k_cg_rupd is the second)");
+    __shared__ float t[2][TY + 2][kRow4];
     float beta = 0.0f, di = 1.0f, alpha = 0.0f;
     if (MODE == 2) {
         double rz, rzp, rr;
@@ -247,20 +248,22 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
         alpha = (float)(rz / pq[0]);
         di = d.dinv_c;
     }
+    // VAR: the Jacobi scaling per voxel (a constant when the solve has none)
+    const float* __restrict__ dvv = VAR ? d.dinv : nullptr;
     const bool upd_x = MODE == 2 && k > 0;
     const bool read_x = upd_x && !(xzero && k == 1);
     const bool use_pold = MODE == 2 && k > 0;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int tx = (nx + kTX4 - 1) / kTX4, ty = (ny + kTY4 - 1) / kTY4, tz = (nz + kZ4 - 1) / kZ4;
+    const int tx = (nx + kTX4 - 1) / kTX4, ty = (ny + TY - 1) / TY, tz = (nz + kZ4 - 1) / kZ4;
     const int ntiles = tx * ty * tz;
     const int64_t sxy = (int64_t)nx * ny;
     const int tt = threadIdx.x, lx = tt & 63, ly = tt >> 6;
-    // halo roles: 0..63 row y=-1, 64..127 row y=kTY4 (float4 each), the next
-    // kTY4 threads column x=-1, the kTY4 after them column x=kTX4 (one voxel each)
-    const bool hrow = tt < 128, hcol = tt >= 128 && tt < 128 + 2 * kTY4;
-    const int hy = tt < 64 ? -1 : tt < 128 ? kTY4 : (tt < 128 + kTY4 ? tt - 128 : tt - 128 - kTY4);
+    // halo roles: 0..63 row y=-1, 64..127 row y=TY (float4 each), the next
+    // TY threads column x=-1, the TY after them column x=kTX4 (one voxel each)
+    const bool hrow = tt < 128, hcol = tt >= 128 && tt < 128 + 2 * TY;
+    const int hy = tt < 64 ? -1 : tt < 128 ? TY : (tt < 128 + TY ? tt - 128 : tt - 128 - TY);
     const int hl = tt < 64 ? tt : tt - 64;
-    const int hslot = tt < 128 + kTY4 ? 3 : 4 + kTX4;
+    const int hslot = tt < 128 + TY ? 3 : 4 + kTX4;
     const float cdiag = g.diag, cc = g.c;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     double v[2] = {0, 0};
@@ -270,30 +273,40 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
     for (int tile = blockIdx.x; tile < ntiles; tile += nb) {
         const int bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
         const int x0 = bx * kTX4, k0 = bz * kZ4, kend = min(k0 + kZ4, nz);
-        const int j = by * kTY4 + ly;
+        const int j = by * TY + ly;
         const int xo = x0 + 4 * lx;
         const bool own = xo < nx && j < ny;
-        const int gj = by * kTY4 + hy;
+        const int gj = by * TY + hy;
         const bool hr_ok = hrow && gj >= 0 && gj < ny && x0 + 4 * hl < nx;
-        const int hx = tt < 128 + kTY4 ? x0 - 1 : x0 + kTX4;
+        const int hx = tt < 128 + TY ? x0 - 1 : x0 + kTX4;
         const bool hc_ok = hcol && gj < ny && hx >= 0 && hx < nx;
         const int64_t ocol = xo + (int64_t)nx * j;
         const int64_t rcol = x0 + 4 * hl + (int64_t)nx * gj;
         const int64_t ccol = hx + (int64_t)nx * gj;
         // tile values: p_new from r and p_old (MODE 2), src otherwise
+        auto dv4 = [&](int64_t e) -> float4 {
+            if (VAR && dvv) return *reinterpret_cast<const float4*>(dvv + e);
+            return make_float4(di, di, di, di);
+        };
+        auto dir4 = [&](int64_t e, float4& po) -> float4 {
+            const float4 a = *reinterpret_cast<const float4*>(d.r + e);
+            const float4 dd = dv4(e);
+            if (!use_pold) return make_float4(a.x * dd.x, a.y * dd.y, a.z * dd.z, a.w * dd.w);
+            po = *reinterpret_cast<const float4*>(pold + e);
+            return make_float4(a.x * dd.x + beta * po.x, a.y * dd.y + beta * po.y, a.z * dd.z + beta * po.z,
+                               a.w * dd.w + beta * po.w);
+        };
         auto val4 = [&](bool ok, int64_t e) -> float4 {
             if (!ok) return z4;
             if (MODE != 2) return *reinterpret_cast<const float4*>(src + e);
-            const float4 a = *reinterpret_cast<const float4*>(d.r + e);
-            if (!use_pold) return make_float4(a.x * di, a.y * di, a.z * di, a.w * di);
-            const float4 b = *reinterpret_cast<const float4*>(pold + e);
-            return make_float4(a.x * di + beta * b.x, a.y * di + beta * b.y, a.z * di + beta * b.z,
-                               a.w * di + beta * b.w);
+            float4 po;
+            return dir4(e, po);
         };
         auto val1 = [&](bool ok, int64_t e) -> float {
             if (!ok) return 0.0f;
             if (MODE != 2) return src[e];
-            return use_pold ? d.r[e] * di + beta * pold[e] : d.r[e] * di;
+            const float de = VAR && dvv ? dvv[e] : di;
+            return use_pold ? d.r[e] * de + beta * pold[e] : d.r[e] * de;
         };
         // the own column: the tile value, and p_old and x (MODE 2) or r
         // (MODE 3) of the same point.  MODE 2 with PF == 2 reads x only in
@@ -304,12 +317,8 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
             xr = z4;
             if (!ok) return z4;
             if (MODE == 2) {
-                const float4 a = *reinterpret_cast<const float4*>(d.r + e);
-                if (!use_pold) return make_float4(a.x * di, a.y * di, a.z * di, a.w * di);
-                po = *reinterpret_cast<const float4*>(pold + e);
-                if (read_x && !late_x) xr = *reinterpret_cast<const float4*>(dst + e);
-                return make_float4(a.x * di + beta * po.x, a.y * di + beta * po.y, a.z * di + beta * po.z,
-                                   a.w * di + beta * po.w);
+                if (use_pold && read_x && !late_x) xr = *reinterpret_cast<const float4*>(dst + e);
+                return dir4(e, po);
             }
             if (MODE == 3) xr = *reinterpret_cast<const float4*>(d.r + e);
             return *reinterpret_cast<const float4*>(src + e);
@@ -346,19 +355,55 @@ __global__ __launch_bounds__(kT4) void k_stencil7x4(GridK g, CgDev d, int k, con
                 hc1 = val1(hc_ok && n2, ccol + sxy * (kz + 2));
             }
             if (late_x && own && read_x) xrc = *reinterpret_cast<const float4*>(dst + ocol + sxy * kz);
+            // VAR: the own voxels' face coefficients (kx: x-faces i .. i+4 of
+            // the row; ky: rows j and j+1; kz: planes kz and kz+1)
+            float kxf[5] = {0, 0, 0, 0, 0};
+            float4 kym = z4, kyp = z4, kzm = z4, kzp = z4;
+            if (VAR && own) {
+                const float* kxr = g.kx + xo + (int64_t)(nx + 1) * (j + (int64_t)ny * kz);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) kxf[q] = kxr[q];
+                const int64_t fy = xo + (int64_t)nx * (j + (int64_t)(ny + 1) * kz);
+                kym = *reinterpret_cast<const float4*>(g.ky + fy);
+                kyp = *reinterpret_cast<const float4*>(g.ky + fy + nx);
+                kzm = *reinterpret_cast<const float4*>(g.kz + ocol + sxy * kz);
+                kzp = *reinterpret_cast<const float4*>(g.kz + ocol + sxy * (kz + 1));
+            }
             __syncthreads();
             if (own) {
                 const float left = t[b][ly + 1][3 + 4 * lx], right = t[b][ly + 1][8 + 4 * lx];
                 const float4 ym = *reinterpret_cast<const float4*>(&t[b][ly][4 + 4 * lx]);
                 const float4 yp = *reinterpret_cast<const float4*>(&t[b][ly + 2][4 + 4 * lx]);
                 float4 o;
-                o.x = cdiag * pc.x - cc * (left + pc.y + ym.x + yp.x + pm.x + pp.x);
-                o.y = cdiag * pc.y - cc * (pc.x + pc.z + ym.y + yp.y + pm.y + pp.y);
-                o.z = cdiag * pc.z - cc * (pc.y + pc.w + ym.z + yp.z + pm.z + pp.z);
-                o.w = cdiag * pc.w - cc * (pc.z + right + ym.w + yp.w + pm.w + pp.w);
+                if (!VAR) {
+                    o.x = cdiag * pc.x - cc * (left + pc.y + ym.x + yp.x + pm.x + pp.x);
+                    o.y = cdiag * pc.y - cc * (pc.x + pc.z + ym.y + yp.y + pm.y + pp.y);
+                    o.z = cdiag * pc.z - cc * (pc.y + pc.w + ym.z + yp.z + pm.z + pp.z);
+                    o.w = cdiag * pc.w - cc * (pc.z + right + ym.w + yp.w + pm.w + pp.w);
+                } else {
+                    // apply_at's variable-coefficient operation order
+                    const float xs[6] = {left, pc.x, pc.y, pc.z, pc.w, right};
+                    const float ymv[4] = {ym.x, ym.y, ym.z, ym.w}, ypv[4] = {yp.x, yp.y, yp.z, yp.w};
+                    const float zmv[4] = {pm.x, pm.y, pm.z, pm.w}, zpv[4] = {pp.x, pp.y, pp.z, pp.w};
+                    const float kymv[4] = {kym.x, kym.y, kym.z, kym.w}, kypv[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
+                    const float kzmv[4] = {kzm.x, kzm.y, kzm.z, kzm.w}, kzpv[4] = {kzp.x, kzp.y, kzp.z, kzp.w};
+                    float ov[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float dd = 1.0f + kxf[q] + kxf[q + 1] + kymv[q] + kypv[q] + kzmv[q] + kzpv[q];
+                        float sm = kxf[q] * xs[q] + kxf[q + 1] * xs[q + 2];
+                        sm = sm + kymv[q] * ymv[q];
+                        sm = sm + kypv[q] * ypv[q];
+                        sm = sm + kzmv[q] * zmv[q];
+                        sm = sm + kzpv[q] * zpv[q];
+                        ov[q] = dd * xs[q + 1] - sm;
+                    }
+                    o = make_float4(ov[0], ov[1], ov[2], ov[3]);
+                }
                 const int64_t id = ocol + sxy * kz;
                 if (MODE == 0) *reinterpret_cast<float4*>(dst + id) = o;
                 if (MODE == 2) {
+                    if (VAR) *reinterpret_cast<float4*>(d.q + id) = o;
                     *reinterpret_cast<float4*>(pnew + id) = pc;
                     v[0] += (double)pc.x * o.x;
                     v[0] += (double)pc.y * o.y;
@@ -466,6 +511,45 @@ __global__ __launch_bounds__(256) void k_cg_finish(int64_t n, CgDev d, const flo
         } else {
             reinterpret_cast<float4*>(x)[c] = xo;
         }
+    }
+}
+
+// Second half of a variable-coefficient iteration after
+// k_stencil7x4<2, PF, true> (which stored q): r -= alpha q with alpha =
+// rz / pq, partials of r . dinv r and r . r; the x update is left to the
+// next iteration's first half (or k_cg_finish), as on the constant grid.
+__global__ __launch_bounds__(256) void k_cg_rupd(int64_t n, CgDev d, int k) {
+    double rz, rzp, rr;
+    if (!cg_scalars(d, k, false, rz, rzp, rr)) return;
+    double pq[1];
+    sum_partials<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, pq);
+    const float alpha = (float)(rz / pq[0]);
+    const float dc = d.dinv_c;
+    double v[2] = {0, 0};
+    const int64_t n4 = n / 4, stride = (int64_t)gridDim.x * 256;
+    for (int64_t c = blockIdx.x * 256ll + threadIdx.x; c < n4; c += stride) {
+        const float4 rv = reinterpret_cast<const float4*>(d.r)[c];
+        const float4 qv = reinterpret_cast<const float4*>(d.q)[c];
+        const float4 dd = d.dinv ? reinterpret_cast<const float4*>(d.dinv)[c] : make_float4(dc, dc, dc, dc);
+        float4 ro;
+        ro.x = rv.x - alpha * qv.x;
+        ro.y = rv.y - alpha * qv.y;
+        ro.z = rv.z - alpha * qv.z;
+        ro.w = rv.w - alpha * qv.w;
+        reinterpret_cast<float4*>(d.r)[c] = ro;
+        v[0] += (double)ro.x * (ro.x * dd.x);
+        v[0] += (double)ro.y * (ro.y * dd.y);
+        v[0] += (double)ro.z * (ro.z * dd.z);
+        v[0] += (double)ro.w * (ro.w * dd.w);
+        v[1] += (double)ro.x * ro.x;
+        v[1] += (double)ro.y * ro.y;
+        v[1] += (double)ro.z * ro.z;
+        v[1] += (double)ro.w * ro.w;
+    }
+    put_partial<2>(v, (k & 1) ? d.pu0 : d.pu1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.fl[FL_ITERS] = k + 1;
+        d.fl[FL_XPEND] = 1;
     }
 }
 
@@ -593,8 +677,13 @@ __global__ __launch_bounds__(kT4) void k_res64x4(GridK g, const double* __restri
         }
 }
 
-static int stencil4_blocks(const GridK& g) {
-    const int64_t nt = ceil_div(g.nx, kTX4) * ceil_div(g.ny, kTY4) * ceil_div(g.nz, kZ4);
+// rows per tile of the variable-coefficient iteration (512 threads: room for
+// the face coefficients' registers)
+constexpr int kTYv = 8;
+enum { kFastConst = 1, kFastVar = 2 };
+
+static int stencil4_blocks(const GridK& g, int ty = kTY4) {
+    const int64_t nt = ceil_div(g.nx, kTX4) * ceil_div(g.ny, ty) * ceil_div(g.nz, kZ4);
     // at most 2048: the fused kernel's partials region holds 2048 entries
     return (int)std::max<int64_t>(1, std::min<int64_t>(nt, 2048));
 }
@@ -846,11 +935,16 @@ static nnx_status cg_readback(nnx_ctx* c, CgHost& h, int32_t* fl, double* sc) {
 // otherwise the new direction, q = A p, then the update of x and r.  Every
 // kernel returns at once when the device flag says the solve is over, so
 // iterations can be enqueued past convergence.
-static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float* x, int vec, bool fast, int nb4,
+static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float* x, int vec, int fast, int nb4,
                            int xzero = 0) {
     const int64_t n = (int64_t)gk.nx * gk.ny * gk.nz;
     float* pold = (k & 1) ? h.pb : h.pa;
     float* pnew = (k & 1) ? h.pa : h.pb;
+    if (fast == kFastVar) {
+        k_stencil7x4<2, 2, true, kTYv><<<nb4, 64 * kTYv, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
+        k_cg_rupd<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k);
+        return;
+    }
     if (fast) {
         k_stencil7x4<2, 2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
         k_stencil7x4<3, 2><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
@@ -862,9 +956,17 @@ static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float*
     k_cg_update<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pnew, x, vec);
 }
 
-static bool cg_fast(const GridK& gk, const float* x) {
-    return gk.kind == 0 && gk.nx % 4 == 0 && ((uintptr_t)x % 16) == 0;
+// The tiled iteration: kFastConst on the constant 7-point grid, kFastVar on
+// the variable-coefficient one (rows of a multiple of 4 voxels, 16-byte
+// aligned vectors and ky, kz); 0: the generic kernels.
+static int cg_fast(const GridK& gk, const float* x) {
+    if (gk.nx % 4 != 0 || (uintptr_t)x % 16 != 0) return 0;
+    if (gk.kind == 0) return kFastConst;
+    if (gk.kind == 1 && (uintptr_t)gk.ky % 16 == 0 && (uintptr_t)gk.kz % 16 == 0) return kFastVar;
+    return 0;
 }
+// blocks of the first-half launch of a tiled iteration
+static int fast_blocks(const GridK& gk, int fast) { return stencil4_blocks(gk, fast == kFastVar ? kTYv : kTY4); }
 
 // the pending x update of the fast path (nothing otherwise)
 static void cg_finish_launch(nnx_ctx* c, const GridK& gk, CgHost& h, float* x, int xzero = 0,
@@ -883,9 +985,9 @@ static nnx_status cg_run(nnx_ctx* c, const GridK& gk, CgHost& h, const float* b,
     NNX_LAUNCHED(c);
     if (check_every < 1) check_every = 1;
     const int vec = ((uintptr_t)x % 16) == 0;
-    // constant 7-point stencil with 16-byte rows: the LDS float4 kernels
-    const bool fast = cg_fast(gk, x);
-    const int nb4 = stencil4_blocks(gk);
+    // 7-point grids with 16-byte rows: the LDS float4 kernels
+    const int fast = cg_fast(gk, x);
+    const int nb4 = fast_blocks(gk, fast);
     h.d.npq = fast ? nb4 : kRedBlocks;
     int32_t fl[FL_N];
     double sc[SC_N];
@@ -950,8 +1052,8 @@ static void cg_enqueue_fixed(nnx_ctx* c, const GridK& gk, CgHost& h, const float
     k_cg_init<<<kRedBlocks, 256, 0, c->stream>>>(gk, h.d, b, x, use_x0, h.pa);
     k_cg_init_fin<<<1, 256, 0, c->stream>>>(h.d, tol, 0, maxit);
     const int vec = ((uintptr_t)x % 16) == 0;
-    const bool fast = cg_fast(gk, x);
-    const int nb4 = stencil4_blocks(gk);
+    const int fast = cg_fast(gk, x);
+    const int nb4 = fast_blocks(gk, fast);
     h.d.npq = fast ? nb4 : kRedBlocks;
     for (int k = 0; k < maxit; ++k) cg_iter_launch(c, gk, h, k, x, vec, fast, nb4);
     cg_finish_launch(c, gk, h, x);
@@ -1114,9 +1216,10 @@ __global__ __launch_bounds__(256) void k_ref_init(GridK g, const float* __restri
 // With a preconditioner M that changes from one application to the next:
 //   z = M(r); p = z - (z.q_old / p_old.q_old) p_old; q = A p;
 //   alpha = p.r / p.q; x += alpha p; r -= alpha q.
-// FCG partials: zq (kRedBlocks), (pq, pr) x 2 (ping-pong), rr, bb.
+// FCG partials: zq (kRedBlocks), (pq, pr) x 2 (ping-pong), rr, bb, and the
+// learned preconditioner's (u.w, u.Au).
 struct FcgPart {
-    double *zq, *pq0, *pq1, *rr, *bb;
+    double *zq, *pq0, *pq1, *rr, *bb, *th;
 };
 
 __global__ __launch_bounds__(256) void k_fcg_init(GridK g, const float* __restrict__ b, float* __restrict__ x,
@@ -1145,10 +1248,15 @@ __global__ __launch_bounds__(256) void k_fcg_jacobi(int64_t n, const float* __re
     for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) z[i] = dinv[i] * r[i];
 }
 
-// Learned preconditioner, two-level: damped-Jacobi pre-smoothing, the
-// network as the coarse correction of what the smoother leaves, damped-
-// Jacobi post-smoothing:
-//   z0 = w D^-1 r;  z1 = z0 + N(r - A z0);  z = z1 + w D^-1 (r - A z1).
+// Learned preconditioner, two-level: damped-Jacobi pre-smoothing (or a
+// multigrid cycle), the network as the coarse correction of what the
+// smoother leaves, damped-Jacobi post-smoothing:
+//   z0 = w D^-1 r;  u = N(r - A z0);  z1 = z0 + theta u;
+//   z = z1 + w D^-1 (r - A z1),
+// theta = u.(r - A z0) / u.(A u): the weight that minimises the energy norm
+// of the error z1 leaves (e - z0 - theta u with A e = r), so the network's
+// correction never makes z1 worse than z0 in that norm, and a correction
+// with rough components the operator amplifies is scaled down.
 constexpr float kSmoothW = 2.0f / 3.0f;
 __global__ __launch_bounds__(256) void k_pc_pre(int64_t n, const float* __restrict__ dinv,
                                                 const float* __restrict__ r, float* __restrict__ z0) {
@@ -1162,8 +1270,30 @@ __global__ __launch_bounds__(256) void k_pc_res(GridK g, const float* __restrict
         out[id] = r[id] - apply_at<float, float>(z, g, i, j, k);
     });
 }
-__global__ __launch_bounds__(256) void k_pc_add(int64_t n, float* __restrict__ z, const float* __restrict__ u) {
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) z[i] = z[i] + u[i];
+// partials of u.w and u.(A u)
+__global__ __launch_bounds__(256) void k_pc_theta(GridK g, const float* __restrict__ u, const float* __restrict__ w,
+                                                  double* part) {
+    double v[2] = {0, 0};
+    for_rows(g.nx, g.ny, g.nz, [&](int i, int j, int k, int64_t id) {
+        const float ui = u[id];
+        v[0] += (double)ui * w[id];
+        v[1] += (double)ui * apply_at<float, float>(u, g, i, j, k);
+    });
+    put_partial<2>(v, part);
+}
+// z += theta u, theta from k_pc_theta's partials (0 when u.Au <= 0); block 0
+// adds theta to stat[0] and one to stat[1]
+__global__ __launch_bounds__(256) void k_pc_addw(int64_t n, float* __restrict__ z, const float* __restrict__ u,
+                                                 const double* part, double* stat) {
+    double t[2];
+    sum_partials<2>(part, kRedBlocks, t);
+    const double th = t[1] > 0 ? t[0] / t[1] : 0.0;
+    const float thf = (float)th;
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) z[i] = z[i] + thf * u[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        stat[0] += th;
+        stat[1] += 1.0;
+    }
 }
 // zout = z1 + w D^-1 (r - A z1)
 __global__ __launch_bounds__(256) void k_pc_post(GridK g, const float* __restrict__ dinv,
@@ -1436,31 +1566,34 @@ nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
     const int max_inner = q.max_inner > 0 ? q.max_inner : 64;
     const int max_outer = q.max_outer > 0 ? q.max_outer : 3;
     const double inner_rtol = q.inner_rtol > 0 ? q.inner_rtol : 1e-6;
-    // the tiled path: constant 7-point grid, rows of a multiple of 4 voxels,
-    // 16-byte aligned vectors
-    const bool fast = cg_fast(gk, ph.xin) && (uintptr_t)q.conc % 16 == 0;
-    const int nb4 = stencil4_blocks(gk), nbr = std::min(nb4, kRedBlocks);
+    // the tiled iteration (7-point grids, rows of a multiple of 4 voxels,
+    // 16-byte aligned vectors); on the constant grid also the tiled FP64
+    // residual, after which the inner CG starts from a zero correction
+    // without storing it and its last update goes straight into the field
+    const int fast = cg_fast(gk, ph.xin);
+    const bool tiled = fast == kFastConst && (uintptr_t)q.conc % 16 == 0;
+    const int nb4 = fast_blocks(gk, fast), nbr = std::min(stencil4_blocks(gk), kRedBlocks);
     h.d.npq = fast ? nb4 : kRedBlocks;
     k_para_rhs<<<kRedBlocks, 256, 0, c->stream>>>(nv, q.conc, p->voxel_counts, q.release, ph.b, ph.pbb, ph.pfl);
     for (int o = 0; o < max_outer; ++o) {
         // the later refinement steps solve to a looser tolerance: half the budget
         const int budget = o == 0 ? max_inner : std::max(1, max_inner / 2);
-        if (fast)
+        if (tiled)
             k_res64x4<false><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
         else
             k_para_res<false><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr,
                                                                  ph.pfl);
         k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, budget, ph.pfl, q.stats, 0,
                                              o > 0);
-        for (int k = 0; k < budget; ++k) cg_iter_launch(c, gk, h, k, ph.xin, 1, fast, nb4, fast);
-        if (fast) {
+        for (int k = 0; k < budget; ++k) cg_iter_launch(c, gk, h, k, ph.xin, 1, fast, nb4, tiled);
+        if (tiled) {
             cg_finish_launch(c, gk, h, ph.xin, 1, q.conc);
         } else {
             cg_finish_launch(c, gk, h, ph.xin);
             k_para_axpy<<<flat_blocks(nv), 256, 0, c->stream>>>(nv, ph.xin, q.conc, h.d.fl, ph.pfl);
         }
     }
-    if (fast)
+    if (tiled)
         k_res64x4<true><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
     else
         k_para_res<true><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
@@ -1542,15 +1675,17 @@ nnx_status fcg_run(nnx_ctx* c, const GridK& gk, int pc, int mg_nu, int mg_smooth
     NNX_TRY(scratch(c, "fcg_qa", n * 4, false, &qa));
     NNX_TRY(scratch(c, "fcg_qb", n * 4, false, &qb));
     NNX_TRY(scratch(c, "fcg_dinv", n * 4, false, &dinv));
-    NNX_TRY(scratch(c, "fcg_part", (size_t)kRedBlocks * 8 * 8, false, &part));
+    NNX_TRY(scratch(c, "fcg_part", (size_t)kRedBlocks * 10 * 8, false, &part));
     NNX_TRY(scratch(c, "fcg_sc", 64, false, &sc));
     double* Pd = (double*)part;
-    FcgPart P{Pd, Pd + kRedBlocks, Pd + 3 * kRedBlocks, Pd + 5 * kRedBlocks, Pd + 6 * kRedBlocks};
+    FcgPart P{Pd, Pd + kRedBlocks, Pd + 3 * kRedBlocks, Pd + 5 * kRedBlocks, Pd + 6 * kRedBlocks,
+              Pd + 7 * kRedBlocks};
     float *R = (float*)r, *Z = (float*)z;
     float* pbuf[2] = {(float*)pa, (float*)pb};
     float* qbuf[2] = {(float*)qa, (float*)qb};
     double* pq[2] = {P.pq0, P.pq1};
     double* S = (double*)sc;
+    NNX_HIP(c, hipMemsetAsync(S + 2, 0, 16, c->stream));  // theta sum and count
     k_dinv<<<flat_blocks(n), 256, 0, c->stream>>>(gk, (float*)dinv);
     k_fcg_init<<<kRedBlocks, 256, 0, c->stream>>>(gk, b, x, use_x0, R, P);
     k_fcg_sum<<<1, 256, 0, c->stream>>>(P.bb, S);
@@ -1576,7 +1711,8 @@ nnx_status fcg_run(nnx_ctx* c, const GridK& gk, int pc, int mg_nu, int mg_smooth
                 k_pc_pre<<<flat_blocks(n), 256, 0, c->stream>>>(n, (const float*)dinv, R, W1);
             k_pc_res<<<kRedBlocks, 256, 0, c->stream>>>(gk, R, W1, W2);
             NNX_TRY(nnx_unet_forward_internal(c, net, gk.nx, gk.ny, gk.nz, 1, W2, feat, Z));
-            k_pc_add<<<flat_blocks(n), 256, 0, c->stream>>>(n, W1, Z);
+            k_pc_theta<<<kRedBlocks, 256, 0, c->stream>>>(gk, Z, W2, P.th);
+            k_pc_addw<<<flat_blocks(n), 256, 0, c->stream>>>(n, W1, Z, P.th, S + 2);
             // post-smoothing sweeps, ping-ponging W1 / Z, ending in Z
             const int sweeps = smooth > 0 ? smooth : 1;
             float* src = W1;
@@ -1619,6 +1755,14 @@ nnx_status fcg_run(nnx_ctx* c, const GridK& gk, int pc, int mg_nu, int mg_smooth
         res->converged = rel <= rtol;
         res->precond_used = use_mg ? (learned ? NNX_PRECOND_MG_LEARNED : NNX_PRECOND_MG)
                                    : (learned ? NNX_PRECOND_LEARNED : NNX_PRECOND_JACOBI);
+        res->learned_weight = 0;
+        if (net) {
+            double th[2];
+            NNX_HIP(c, hipMemcpyAsync(c->pinned, S + 2, 16, hipMemcpyDeviceToHost, c->stream));
+            NNX_HIP(c, hipStreamSynchronize(c->stream));
+            std::memcpy(th, c->pinned, 16);
+            res->learned_weight = th[1] > 0 ? th[0] / th[1] : 0.0;
+        }
     }
     return NNX_OK;
 }
@@ -1684,7 +1828,7 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
     if (!o->use_x0) NNX_HIP(c, hipMemsetAsync(x, 0, n * 8, c->stream));
     nnx_cg_result out{};
     int outer = 0, inner_total = 0, used = o->precond;
-    double rel = 0;
+    double rel = 0, wsum = 0;
     for (;;) {
         double rb[2];
         NNX_TRY((true_residual<double, double>(c, gk, h, x, b, nullptr, rb, (double*)r64)));
@@ -1701,6 +1845,7 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
                         o->maxit,
                         o->stall_iters, o->smooth, &ir));
         inner_total += ir.iters;
+        wsum += ir.learned_weight * ir.iters;
         if (ir.precond_used != o->precond) used = ir.precond_used;
         k_axpy_f64<<<flat_blocks(n), 256, 0, c->stream>>>(n, 1.0 / s, (const float*)d, x);
         NNX_LAUNCHED(c);
@@ -1713,6 +1858,7 @@ nnx_status nnx_fcg_solve_refine(nnx_ctx* c, const nnx_grid* g, nnx_unet* net, co
     out.true_relres = rel;
     out.converged = rel <= o->rtol;
     out.precond_used = used;
+    out.learned_weight = inner_total > 0 ? wsum / inner_total : 0.0;
     out.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (res) *res = out;
     return out.converged ? NNX_OK : fail(c, NNX_ERR_NOCONV, "FCG refinement: %d outer steps, relres %.3e", outer, rel);
