@@ -214,9 +214,11 @@ def c4_bench(ctx, steps=5000, chunk=100, tune_steps=1000, warmup=100, tune=True)
     ctx.call("nnx_event_create", C.byref(e0))
     ctx.call("nnx_event_create", C.byref(e1))
 
-    def timed(cfg, autotune):
+    def timed(cfg, autotune, para=0):
         sim = EnsembleSim(ens, ctx=ctx, deposit=True)
         sim.cfg = cfg
+        if para:
+            sim.set_paracrine(para, rtol=1e-8, inner_rtol=1e-6, max_inner=40, max_outer=2)
         sim.run(warmup, use_graph=True, chunk=chunk)
         info = None
         if autotune:
@@ -239,6 +241,8 @@ def c4_bench(ctx, steps=5000, chunk=100, tune_steps=1000, warmup=100, tune=True)
         ctx.call("nnx_event_elapsed", e0, e1, C.byref(ms))
         spikes = int(sim.a["net_spikes"].numpy().sum()) - spk0
         v = sim.a["v"].numpy()
+        if para:
+            info = sim.paracrine_stats()
         del sim
         return cfg, ms.value, spikes, first, info, v
 
@@ -261,6 +265,16 @@ def c4_bench(ctx, steps=5000, chunk=100, tune_steps=1000, warmup=100, tune=True)
                               "same_steps": first_d == first and spikes_d == spikes,
                               "same_final_v": bool(np.array_equal(v_d, v_best))}
         out["autotuned_over_default"] = out["default"]["us_per_step"] / out["us_per_step"]
+    # every network's own 64^3 paracrine grid solved every 100 steps (one
+    # solve of the box of grids with Dirichlet faces between them)
+    _, ms_p, spikes_p, first_p, pst, v_p = timed(best, False, para=100)
+    n_timed = sum(1 for s_ in range(first_p, first_p + steps) if s_ % 100 == 0)
+    nx, ny, nz = ens.grid
+    out["with_paracrine"] = {"every": 100, "grids": f"{len(ens.size)} x 64^3 in a {nx}x{ny}x{nz} box", "c": ens.c,
+                             "rtol_true_fp64": 1e-8, "us_per_step": 1e3 * ms_p / steps,
+                             "neuron_steps_per_s": live * steps / (ms_p * 1e-3), "solves": pst,
+                             "solves_timed": n_timed, "ms_per_solve": (ms_p - ms) / max(n_timed, 1),
+                             "same_trajectory": bool(np.array_equal(v_p, v_best))}
     ctx.call("nnx_event_destroy", e0)
     ctx.call("nnx_event_destroy", e1)
     return out
@@ -316,8 +330,16 @@ def c5_bench(ctx, counts, smooth=2):
         log(f"bench: C5 {name} {best}")
     for name, _, _ in solvers[1:]:
         out["speedup_" + name] = out["jacobi_cg"]["ms"] / out[name]["ms"]
-    out["speedup_learned_vs_plain"] = max(out["jacobi_cg"]["ms"] / out[k]["ms"]
-                                          for k in ("learned_fcg", "mg_learned_fcg") if k in out)
+    learned = [k for k in ("learned_fcg", "mg_learned_fcg") if k in out]
+    best_l = max(learned, key=lambda k: out["jacobi_cg"]["ms"] / out[k]["ms"])
+    out["speedup_learned_vs_plain"] = out["jacobi_cg"]["ms"] / out[best_l]["ms"]
+    # what the network itself adds: the learned solver against its own base
+    # (the same cycle or smoother without the network) and the mean weight
+    # the energy criterion gave the network's correction
+    out["learned_best"] = {"solver": best_l, "fell_back": out[best_l]["fell_back"],
+                           "vs_its_base": (out["mg_fcg"]["ms"] if best_l == "mg_learned_fcg"
+                                           else out["jacobi_cg"]["ms"]) / out[best_l]["ms"],
+                           "network_weight_mean": out[best_l]["learned_weight"]}
     # one network application (the preconditioner's dense part) on the MFMA roofline
     r32, z32 = DA.from_numpy(ctx, b.astype(np.float32)), DA.zeros(ctx, n ** 3, np.float32)
 

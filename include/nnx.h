@@ -344,7 +344,12 @@ nnx_status nnx_couple(nnx_ctx* ctx, const nnx_graph* g, int mode, int dtype,
  *               diagonal is 1 + the sum of the voxel's six faces.
  *   STENCIL_27: A = (1 + decay_dt/2) I - (c/2) L27, L27 the reference's
  *               high-order compact Laplacian {1,3,14,-128}/30 (src/paracrine.cu:40-43,80-81),
- *               c = dt*D/h^2: the Crank-Nicolson operator.                  */
+ *               c = dt*D/h^2: the Crank-Nicolson operator.
+ * block > 0 (constant STENCIL_7 only; nx, ny, nz multiples of it): the box
+ * is a lattice of independent block^3 grids, each with Dirichlet 0 on its
+ * own faces - the operator has no coupling across a block face, so one
+ * solve on the box solves every block (the per-network grids of an
+ * ensemble laid side by side).                                              */
 typedef struct nnx_grid {
     int32_t nx, ny, nz;
     int32_t stencil;
@@ -353,6 +358,8 @@ typedef struct nnx_grid {
     const float* kx;   /* variable face coefficients or NULL (STENCIL_7 only) */
     const float* ky;
     const float* kz;
+    int32_t block;     /* 0: one grid; > 0: a lattice of block^3 grids (above) */
+    int32_t pad_;
 } nnx_grid;
 
 typedef struct nnx_cg_opts {
@@ -460,6 +467,41 @@ nnx_status nnx_rhs_from_counts(nnx_ctx* ctx, int64_t nvox, const float* d_c_prev
 nnx_status nnx_csr_spmv(nnx_ctx* ctx, int64_t rows, const int32_t* d_offsets, const int32_t* d_cols,
                         const float* d_vals, const float* d_x, float* d_y, int accumulate);
 
+/* The paracrine field solved on its own cadence inside the timestep, as
+ * ParacrineModule::update_on_own_timescale does every `timescale` steps
+ * (src/main.cu:171-177, src/paracrine.cuh:255-265): before each step s with
+ * s % every == 0 the releases counted since the last solve become the
+ * source,
+ *     b = conc + release * voxel_counts,   voxel_counts = 0,
+ * and conc is replaced by the solution of A conc = b (A the grid's
+ * operator), warm-started from the previous field.  The solve is Jacobi-CG
+ * with FP64 iterative refinement: the residual b - A conc in FP64, an inner
+ * fp32 CG (FP64 dots) on it, conc += correction, until the FP64 relative
+ * residual ||b - A conc|| / ||b|| <= rtol.  Nothing is read back: max_outer
+ * refinement steps of max_inner CG iterations each are enqueued (captured
+ * with the steps), and every kernel after convergence returns at once.
+ * Releases: every spike of the steps before s (the reference's
+ * update_on_fine_timescale skips the spikes its coupling reads on a solve
+ * step; here none is skipped).                                             */
+typedef struct nnx_paracrine {
+    int64_t every;            /* the reference's timescale in steps (<= 0: off) */
+    nnx_grid grid;            /* nx*ny*nz = the number of voxel_counts entries */
+    double release;           /* source added per spike at its voxel (0.01) */
+    double rtol;              /* FP64 relative residual of each solve (1e-8) */
+    double inner_rtol;        /* tightest inner fp32 CG tolerance (0: 1e-6) */
+    int32_t max_inner;        /* CG iterations enqueued for the first refinement step, half as
+                               * many for each later one (their tolerance is looser) (0: 64) */
+    int32_t max_outer;        /* refinement steps enqueued (0: 3) */
+    double* conc;             /* device, nvox FP64: the field (in: previous, out: new) */
+    float* at_neurons;        /* optional device n floats: conc at each neuron's voxel
+                               * after each solve (the write-back into d_g_paracrine,
+                               * src/paracrine.cuh:261-264, at the neuron's voxel) */
+    double* stats;            /* optional device 8 doubles, accumulated (the caller
+                               * zeroes them): solves, inner iterations, refinement
+                               * steps, max relres, solves above rtol, and the last
+                               * solve's relres, inner iterations, refinement steps */
+} nnx_paracrine;
+
 /* ------------------------------------------------------------------------ */
 /* ensembles of independent networks (no counterpart in the reference,      */
 /* which simulates one network, src/main.cu:112-182)                         */
@@ -485,6 +527,9 @@ typedef struct nnx_ens_plan {
     int32_t* raster;          /* optional (step, neuron) pairs; within a step unordered */
     int64_t* raster_len;
     int64_t raster_cap;
+    nnx_paracrine paracrine;  /* optional solve on its cadence (every = 0: off), as in
+                               * nnx_step_plan; with a block lattice grid (nnx_grid.block)
+                               * every network's own grid is solved at once */
 } nnx_ens_plan;
 
 /* launch shape of the fused step: threads per workgroup (64, 128, 256, 512),
@@ -496,7 +541,8 @@ typedef struct nnx_ens_config {
 /* n_steps steps from first_step (first_step == 0 clears the pending
  * coupling and the spike bitmaps) with one launch per step, optionally
  * replayed from a captured hipGraph; cfg NULL = {256, 256, 8192}.  The
- * result does not depend on cfg.                                            */
+ * result does not depend on cfg.  With plan->paracrine.every > 0 the solve
+ * runs before every step s with s % every == 0, as in nnx_run_steps.       */
 nnx_status nnx_ensemble_run(nnx_ctx* ctx, const nnx_ens_plan* plan, const nnx_ens_config* cfg,
                             uint64_t first_step, int n_steps, int use_graph);
 /* Coordinate search over block, tile and grid (from the default
@@ -506,7 +552,7 @@ nnx_status nnx_ensemble_run(nnx_ctx* ctx, const nnx_ens_plan* plan, const nnx_en
  * state, pending events, spike bitmaps, step counters) is saved in a scratch
  * buffer and restored before each timed run and after the search, and the
  * candidates run without the plan's accumulating outputs (voxel deposit,
- * spike counts, raster).  So the simulation does not advance (*steps_run =
+ * spike counts, raster) and without its paracrine solve.  So the simulation does not advance (*steps_run =
  * 0) and no output changes; choose a window that holds the bursts the run
  * will see.  Each candidate: one untimed run, then the faster of two timed
  * ones.  Synchronizes the stream; not under capture.                       */
@@ -724,40 +770,6 @@ nnx_status nnx_hebbian_update(nnx_ctx* ctx, const nnx_graph* g, const nnx_hebbia
 /* composite timestep                                                        */
 /* ------------------------------------------------------------------------ */
 
-/* The paracrine field solved on its own cadence inside the timestep, as
- * ParacrineModule::update_on_own_timescale does every `timescale` steps
- * (src/main.cu:171-177, src/paracrine.cuh:255-265): before each step s with
- * s % every == 0 the releases counted since the last solve become the
- * source,
- *     b = conc + release * voxel_counts,   voxel_counts = 0,
- * and conc is replaced by the solution of A conc = b (A the grid's
- * operator), warm-started from the previous field.  The solve is Jacobi-CG
- * with FP64 iterative refinement: the residual b - A conc in FP64, an inner
- * fp32 CG (FP64 dots) on it, conc += correction, until the FP64 relative
- * residual ||b - A conc|| / ||b|| <= rtol.  Nothing is read back: max_outer
- * refinement steps of max_inner CG iterations each are enqueued (captured
- * with the steps), and every kernel after convergence returns at once.
- * Releases: every spike of the steps before s (the reference's
- * update_on_fine_timescale skips the spikes its coupling reads on a solve
- * step; here none is skipped).                                             */
-typedef struct nnx_paracrine {
-    int64_t every;            /* the reference's timescale in steps (<= 0: off) */
-    nnx_grid grid;            /* nx*ny*nz = the number of voxel_counts entries */
-    double release;           /* source added per spike at its voxel (0.01) */
-    double rtol;              /* FP64 relative residual of each solve (1e-8) */
-    double inner_rtol;        /* tightest inner fp32 CG tolerance (0: 1e-6) */
-    int32_t max_inner;        /* CG iterations enqueued for the first refinement step, half as
-                               * many for each later one (their tolerance is looser) (0: 64) */
-    int32_t max_outer;        /* refinement steps enqueued (0: 3) */
-    double* conc;             /* device, nvox FP64: the field (in: previous, out: new) */
-    float* at_neurons;        /* optional device n floats: conc at each neuron's voxel
-                               * after each solve (the write-back into d_g_paracrine,
-                               * src/paracrine.cuh:261-264, at the neuron's voxel) */
-    double* stats;            /* optional device 8 doubles, accumulated (the caller
-                               * zeroes them): solves, inner iterations, refinement
-                               * steps, max relres, solves above rtol, and the last
-                               * solve's relres, inner iterations, refinement steps */
-} nnx_paracrine;
 
 /* Everything one timestep touches, in the reference's order
  * coupling -> paracrine -> ephys (src/main.cu:166-181,

@@ -62,7 +62,8 @@ class Graph(C.Structure):
 
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("stencil", C.c_int32),
-                ("c", C.c_double), ("decay_dt", C.c_double), ("kx", vp), ("ky", vp), ("kz", vp)]
+                ("c", C.c_double), ("decay_dt", C.c_double), ("kx", vp), ("ky", vp), ("kz", vp),
+                ("block", C.c_int32), ("pad_", C.c_int32)]
 
 
 class CgOpts(C.Structure):
@@ -153,7 +154,7 @@ class EnsPlan(C.Structure):
     _fields_ = [("state", HHState), ("params", HHParams), ("graph", Graph), ("nnet", C.c_int32),
                 ("pad_", C.c_int32), ("net_begin", vp), ("net_size", vp), ("dt", C.c_double),
                 ("bits", vp * 2), ("voxel_of", vp), ("voxel_counts", vp), ("net_spikes", vp),
-                ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64)]
+                ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64), ("paracrine", Paracrine)]
 
 
 class EnsConfig(C.Structure):

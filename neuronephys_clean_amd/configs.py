@@ -187,17 +187,40 @@ class Ensemble:
     voxel_of: np.ndarray
     dt: float = 0.01
     members: list = field(default=None, repr=False)
+    lattice: tuple = (1, 1, 1)   # networks' 64^3 grids per x, y, z of the release box
+    c: float = 1.0               # dt D / h^2 of every network's grid
 
     @property
     def nnz(self) -> int:
         return int(self.off[-1])
+
+    @property
+    def grid(self) -> tuple:
+        """The release box: the networks' 64^3 grids side by side."""
+        return tuple(64 * k for k in self.lattice)
+
+    def block_of(self, i: int) -> tuple:
+        """Voxel offsets (x, y, z) of network i's grid in the box."""
+        a, b, _ = self.lattice
+        return 64 * (i % a), 64 * ((i // a) % b), 64 * (i // (a * b))
+
+
+def ensemble_lattice(count: int) -> tuple:
+    """Networks' grids per x, y, z: up to 4 along x and y, the rest along z
+    (256 networks: 4 x 4 x 16, a 256 x 256 x 1024 box)."""
+    a = max(1, min(4, count))
+    b = max(1, min(4, -(-count // a)))
+    return a, b, max(1, -(-count // (a * b)))
 
 
 def c4_ensemble(seed: int = 0, count: int = 256, sizes=None, keep_members: bool = False) -> Ensemble:
     """The batched-ensemble configuration: `count` networks with
     log-uniform sizes 2^U(10,16), Poisson(50) out-degree, uniform targets,
     w ~ U[0, 0.1], I_ext ~ N(mu_i, 2) with mu_i ~ U(6, 14), each with its own
-    64^3 release grid (voxel ids of network i start at i * 64^3)."""
+    64^3 release grid; the grids lie side by side in one box
+    (ensemble_lattice), network i's at block_of(i), so that one solve of the
+    box with Dirichlet faces every 64 voxels (nnx_grid.block) solves every
+    network's grid."""
     sizes = c4_sizes(seed, count) if sizes is None else np.asarray(sizes, np.int64)
     begin = np.zeros(len(sizes), np.int64)
     pos = 0
@@ -210,6 +233,8 @@ def c4_ensemble(seed: int = 0, count: int = 256, sizes=None, keep_members: bool 
     V0 = np.full(n, -65.0)
     m0, h0, n0, iext = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
     vox = np.zeros(n, np.int32)
+    lat = ensemble_lattice(len(sizes))
+    nx, ny = 64 * lat[0], 64 * lat[1]
     for i, s in enumerate(sizes):
         net = c4_member(seed, i, int(s))
         b = int(begin[i])
@@ -219,11 +244,12 @@ def c4_ensemble(seed: int = 0, count: int = 256, sizes=None, keep_members: bool 
         V0[b:b + s], m0[b:b + s], h0[b:b + s], n0[b:b + s], iext[b:b + s] = (
             net.V0, net.m0, net.h0, net.n0, net.iext)
         local = np.minimum(np.floor(net.pos).astype(np.int64), 63)
-        vox[b:b + s] = (i * 64 ** 3 + local[:, 0] + 64 * (local[:, 1] + 64 * local[:, 2])).astype(np.int32)
+        ox, oy, oz = 64 * (i % lat[0]), 64 * ((i // lat[0]) % lat[1]), 64 * (i // (lat[0] * lat[1]))
+        vox[b:b + s] = (ox + local[:, 0] + nx * (oy + local[:, 1] + ny * (oz + local[:, 2]))).astype(np.int32)
         if keep_members:
             members.append(net)
     off = np.zeros(n + 1, np.int32)
     np.cumsum(deg, out=off[1:])
     return Ensemble(n, begin, sizes.copy(), off, np.concatenate(parts_t) if parts_t else np.zeros(0, np.int32),
                     np.concatenate(parts_w) if parts_w else np.zeros(0, np.float32), V0, m0, h0, n0, iext, vox,
-                    members=members if keep_members else None)
+                    members=members if keep_members else None, lattice=lat)

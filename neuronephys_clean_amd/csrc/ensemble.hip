@@ -282,6 +282,7 @@ nnx_status check_ens(nnx_ctx* c, const nnx_ens_plan* p) {
     NNX_CHECK(c, end <= s.n, "networks exceed the state");
     if (p->raster) NNX_CHECK(c, p->raster_len && p->raster_cap >= 0, "raster needs raster_len");
     if (p->voxel_of) NNX_CHECK(c, p->voxel_counts, "voxel_of needs voxel_counts");
+    NNX_TRY(para_check(c, para_io(p)));
     if (p->params.noise_std != 0.0)
         return fail(c, NNX_ERR_UNSUPPORTED, "white noise (noise_std != 0) is drawn by the leapfrog step only");
     return NNX_OK;
@@ -428,7 +429,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, 1);
     NNX_LAUNCHED(c);
     NNX_TRY(readout_prepare(c, p));
-    NNX_TRY(para_prepare(c, p));
+    NNX_TRY(para_prepare(c, para_io(p)));
     auto body = [&]() -> nnx_status {
         k_push_bits<<<2048, 256, 0, c->stream>>>(p->list, p->count, p->graph.out_offsets, p->graph.out_targets,
                                                  p->graph.out_to_in, eb_q0, flag_q0);
@@ -443,7 +444,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
                 NNX_TRY(hebbian_launch(c, &p->graph, &p->hebbian, h.k.stepctr + ((st + 1) & 1), 1, 0,
                                        p->bits[(st + 1) & 1]));
             // paracrine solve on its cadence (releases of the steps before st)
-            if (p->paracrine.every > 0 && st % (uint64_t)p->paracrine.every == 0) NNX_TRY(para_launch(c, p));
+            if (p->paracrine.every > 0 && st % (uint64_t)p->paracrine.every == 0) NNX_TRY(para_launch(c, para_io(p)));
             NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)(st & 1), i + 1 < n_steps));
         }
         CompactExtras ex{};
@@ -487,13 +488,20 @@ nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_con
     const int q0 = (int)((first_step + 1) & 1);
     k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, fresh ? 1 : 0);
     NNX_LAUNCHED(c);
+    const int64_t every = p->paracrine.every;
+    NNX_TRY(para_prepare(c, para_io(p)));  // buffers exist before any capture
     auto body = [&]() -> nnx_status {
-        for (int i = 0; i < n_steps; ++i) NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1), 1));
+        for (int i = 0; i < n_steps; ++i) {
+            // the paracrine solve on its cadence (releases of the steps before this one)
+            if (every > 0 && (first_step + i) % (uint64_t)every == 0) NNX_TRY(para_launch(c, para_io(p)));
+            NNX_TRY(ens_launch(c, h.k, h.hc, f, (int)((first_step + i) & 1), 1));
+        }
         return NNX_OK;
     };
     if (!use_graph) return body();
-    return capture_or_replay(c, {(const void*)(uintptr_t)ens_key(p, f), n_steps * 2 + (int)(first_step & 1)},
-                             body);
+    uint64_t key = ens_key(p, f);
+    if (every > 0) key ^= (first_step % (uint64_t)every + 1) * 0xc2b2ae3d27d4eb4full;
+    return capture_or_replay(c, {(const void*)(uintptr_t)key, n_steps * 2 + (int)(first_step & 1)}, body);
 }
 
 nnx_status nnx_ensemble_autotune(nnx_ctx* c, const nnx_ens_plan* p, uint64_t first_step, int steps_per_candidate,
@@ -507,7 +515,8 @@ nnx_status nnx_ensemble_autotune(nnx_ctx* c, const nnx_ens_plan* p, uint64_t fir
     // events, spike bitmaps, step counters) is saved once and restored before
     // each run and after the search.  The candidates run on a copy of the
     // plan without the accumulating outputs (voxel deposit, spike counts,
-    // raster), which the search therefore leaves untouched.
+    // raster) and without the paracrine solve, which the search therefore
+    // leaves untouched.
     const int64_t n = p->state.n;
     EnsHost h;
     uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
@@ -538,6 +547,7 @@ nnx_status nnx_ensemble_autotune(nnx_ctx* c, const nnx_ens_plan* p, uint64_t fir
     nnx_ens_plan tp = *p;
     tp.voxel_of = nullptr, tp.voxel_counts = nullptr, tp.net_spikes = nullptr;
     tp.raster = nullptr, tp.raster_len = nullptr, tp.raster_cap = 0;
+    tp.paracrine.every = 0;
     hipEvent_t e0, e1;
     NNX_HIP(c, hipEventCreate(&e0));
     NNX_HIP(c, hipEventCreate(&e1));

@@ -12,6 +12,7 @@ namespace nnx {
 
 struct GridK {
     int32_t nx, ny, nz, kind;  // kind: 0 const 7-pt, 1 variable 7-pt, 2 27-pt
+    int32_t block;             // kind 0: Dirichlet faces every `block` voxels (0: none)
     float c, diag, a0, w;      // fp32 constants (cast once on the host)
     double cd, diagd, a0d, wd; // fp64 constants
     const float *kx, *ky, *kz;
@@ -23,6 +24,7 @@ inline GridK make_grid(const nnx_grid* g) {
     k.ny = g->ny;
     k.nz = g->nz;
     k.kind = g->stencil == NNX_STENCIL_27 ? 2 : (g->kx ? 1 : 0);
+    k.block = g->block;
     k.cd = g->c;
     k.diagd = 1.0 + 6.0 * g->c;
     k.a0d = 1.0 + 0.5 * g->decay_dt;
@@ -53,6 +55,18 @@ __device__ __forceinline__ T apply_at(const U* __restrict__ x, const GridK& g, i
     const int64_t id = i + (int64_t)g.nx * (j + (int64_t)g.ny * k);
     const T xc = T(x[id]);
     if (g.kind == 0) {
+        if (g.block > 0) {
+            // a neighbour across a block face is outside that block's grid
+            const int B = g.block;
+            const bool xm = i % B == 0, xp = (i + 1) % B == 0, ym = j % B == 0, yp = (j + 1) % B == 0,
+                       zm = k % B == 0, zp = (k + 1) % B == 0;
+            T s = (xm ? T(0) : T(at(x, g, i - 1, j, k))) + (xp ? T(0) : T(at(x, g, i + 1, j, k)));
+            s = s + (ym ? T(0) : T(at(x, g, i, j - 1, k)));
+            s = s + (yp ? T(0) : T(at(x, g, i, j + 1, k)));
+            s = s + (zm ? T(0) : T(at(x, g, i, j, k - 1)));
+            s = s + (zp ? T(0) : T(at(x, g, i, j, k + 1)));
+            return cst<T>(g, g.diag, g.diagd) * xc - cst<T>(g, g.c, g.cd) * s;
+        }
         T s = T(at(x, g, i - 1, j, k)) + T(at(x, g, i + 1, j, k));
         s = s + T(at(x, g, i, j - 1, k));
         s = s + T(at(x, g, i, j + 1, k));

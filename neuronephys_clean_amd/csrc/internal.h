@@ -196,9 +196,18 @@ nnx_status hh_step_launch(nnx_ctx* c, const nnx_hh_state* st, const nnx_hh_param
 
 // the paracrine solve inside nnx_run_steps (paracrine.hip): plan check,
 // scratch set-up before a capture, and the launches of one solve
-nnx_status para_check(nnx_ctx* c, const nnx_step_plan* p);
-nnx_status para_prepare(nnx_ctx* c, const nnx_step_plan* p);
-nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p);
+// the paracrine solve's inputs: the block, the release deposit, the neurons
+struct ParaIo {
+    const nnx_paracrine* q;
+    const int32_t* voxel_of;
+    int32_t* voxel_counts;
+    int64_t n;
+};
+inline ParaIo para_io(const nnx_step_plan* p) { return {&p->paracrine, p->voxel_of, p->voxel_counts, p->state.n}; }
+inline ParaIo para_io(const nnx_ens_plan* p) { return {&p->paracrine, p->voxel_of, p->voxel_counts, p->state.n}; }
+nnx_status para_check(nnx_ctx* c, const ParaIo& io);
+nnx_status para_prepare(nnx_ctx* c, const ParaIo& io);
+nnx_status para_launch(nnx_ctx* c, const ParaIo& io);
 // graph key component for the print and paracrine phases of first_step
 uint64_t phase_key(const nnx_step_plan* p, uint64_t first_step);
 

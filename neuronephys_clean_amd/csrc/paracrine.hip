@@ -220,7 +220,17 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 // them; with 2 they were issued one plane earlier, which hides a memory
 // latency per plane (a workgroup marches its tile alone on its CU).
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
-template <int MODE, int PF, bool VAR = false, int TY = kTY4>
+
+// Tile of workgroup b in round `round` of a tiled launch of nb workgroups.
+// Workgroups are dealt to the 8 XCDs round-robin (b % 8), each XCD with its
+// own L2; giving the workgroups of one XCD consecutive tiles (neighbours in
+// y, then z) lets a tile's halo rows and planes hit the L2 that its
+// neighbour's reads filled.
+__device__ __forceinline__ int xcd_tile(int b, int round, int nb) {
+    const int t = nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+    return t + round * nb;
+}
+template <int MODE, int PF, bool VAR = false, int TY = kTY4, bool BLK = false>
 __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k, const float* __restrict__ src,
                                                     const float* __restrict__ pold, float* __restrict__ pnew,
                                                     float* __restrict__ dst, int xzero) {
@@ -265,12 +275,15 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
     const int hl = tt < 64 ? tt : tt - 64;
     const int hslot = tt < 128 + TY ? 3 : 4 + kTX4;
     const float cdiag = g.diag, cc = g.c;
+    const int blk = BLK ? g.block : 0;  // BLK: a block lattice (no coupling across block faces)
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     double v[2] = {0, 0};
     // one tile per workgroup when the grid covers them (the usual case);
     // a grid-stride loop otherwise
     const int nb = gridDim.x;
-    for (int tile = blockIdx.x; tile < ntiles; tile += nb) {
+    for (int round = 0; round * nb < ntiles; ++round) {
+        const int tile = xcd_tile(blockIdx.x, round, nb);
+        if (tile >= ntiles) break;
         const int bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
         const int x0 = bx * kTX4, k0 = bz * kZ4, kend = min(k0 + kZ4, nz);
         const int j = by * TY + ly;
@@ -283,6 +296,8 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
         const int64_t ocol = xo + (int64_t)nx * j;
         const int64_t rcol = x0 + 4 * hl + (int64_t)nx * gj;
         const int64_t ccol = hx + (int64_t)nx * gj;
+        const bool cut_xm = blk && xo % blk == 0, cut_xp = blk && (xo + 4) % blk == 0;
+        const bool cut_ym = blk && j % blk == 0, cut_yp = blk && (j + 1) % blk == 0;
         // tile values: p_new from r and p_old (MODE 2), src otherwise
         auto dv4 = [&](int64_t e) -> float4 {
             if (VAR && dvv) return *reinterpret_cast<const float4*>(dvv + e);
@@ -376,10 +391,21 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
                 const float4 yp = *reinterpret_cast<const float4*>(&t[b][ly + 2][4 + 4 * lx]);
                 float4 o;
                 if (!VAR) {
-                    o.x = cdiag * pc.x - cc * (left + pc.y + ym.x + yp.x + pm.x + pp.x);
-                    o.y = cdiag * pc.y - cc * (pc.x + pc.z + ym.y + yp.y + pm.y + pp.y);
-                    o.z = cdiag * pc.z - cc * (pc.y + pc.w + ym.z + yp.z + pm.z + pp.z);
-                    o.w = cdiag * pc.w - cc * (pc.z + right + ym.w + yp.w + pm.w + pp.w);
+                    // block lattice: no neighbour across a block face
+                    float lf = left, rt = right;
+                    float4 ya = ym, yb = yp, za = pm, zb = pp;
+                    if (blk) {
+                        if (cut_xm) lf = 0.0f;
+                        if (cut_xp) rt = 0.0f;
+                        if (cut_ym) ya = z4;
+                        if (cut_yp) yb = z4;
+                        if (kz % blk == 0) za = z4;
+                        if ((kz + 1) % blk == 0) zb = z4;
+                    }
+                    o.x = cdiag * pc.x - cc * (lf + pc.y + ya.x + yb.x + za.x + zb.x);
+                    o.y = cdiag * pc.y - cc * (pc.x + pc.z + ya.y + yb.y + za.y + zb.y);
+                    o.z = cdiag * pc.z - cc * (pc.y + pc.w + ya.z + yb.z + za.z + zb.z);
+                    o.w = cdiag * pc.w - cc * (pc.z + rt + ya.w + yb.w + za.w + zb.w);
                 } else {
                     // apply_at's variable-coefficient operation order
                     const float xs[6] = {left, pc.x, pc.y, pc.z, pc.w, right};
@@ -561,7 +587,7 @@ __global__ __launch_bounds__(256) void k_cg_rupd(int64_t n, CgDev d, int k) {
 // zero without being stored: k_stencil7x4 and k_cg_finish with xzero).
 // 20 B per voxel (16 when FINAL).  Launched with at most kRedBlocks
 // workgroups; block 0 zeroes the partial slots beyond them.
-template <bool FINAL>
+template <bool FINAL, bool BLK = false>
 __global__ __launch_bounds__(kT4) void k_res64x4(GridK g, const double* __restrict__ x,
                                                  const double* __restrict__ b, CgDev d, double* prr,
                                                  const int32_t* __restrict__ pfl) {
@@ -578,6 +604,7 @@ __global__ __launch_bounds__(kT4) void k_res64x4(GridK g, const double* __restri
     const int hslot = tt < 128 + kTY4 ? 3 : 4 + kTX4;
     const double cdiag = g.diagd, cc = g.cd;
     const float di = d.dinv_c;
+    const int blk = BLK ? g.block : 0;
     // (||r32||^2 is both the inner CG's bb and its first rr)
     double vr[1] = {0}, vu[2] = {0, 0};
     struct D4 {
@@ -592,7 +619,9 @@ __global__ __launch_bounds__(kT4) void k_res64x4(GridK g, const double* __restri
         reinterpret_cast<double2*>(p)[0] = make_double2(v.x, v.y);
         reinterpret_cast<double2*>(p)[1] = make_double2(v.z, v.w);
     };
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int round = 0; round * (int)gridDim.x < ntiles; ++round) {
+        const int tile = xcd_tile(blockIdx.x, round, gridDim.x);
+        if (tile >= ntiles) break;
         const int bx = tile % tx, by = (tile / tx) % ty, bz = tile / (tx * ty);
         const int x0 = bx * kTX4, k0 = bz * kZ4, kend = min(k0 + kZ4, nz);
         const int j = by * kTY4 + ly;
@@ -605,6 +634,8 @@ __global__ __launch_bounds__(kT4) void k_res64x4(GridK g, const double* __restri
         const int64_t ocol = xo + (int64_t)nx * j;
         const int64_t rcol = x0 + 4 * hl + (int64_t)nx * gj;
         const int64_t ccol = hx + (int64_t)nx * gj;
+        const bool cut_xm = blk && xo % blk == 0, cut_xp = blk && (xo + 4) % blk == 0;
+        const bool cut_ym = blk && j % blk == 0, cut_yp = blk && (j + 1) % blk == 0;
         D4 pm = ld4(own && k0 > 0, x + ocol + sxy * (k0 - 1));
         D4 pc = ld4(own, x + ocol + sxy * k0);
         st4(&t[0][ly + 1][4 + 4 * lx], pc);
@@ -627,13 +658,16 @@ __global__ __launch_bounds__(kT4) void k_res64x4(GridK g, const double* __restri
                 const double ms[4] = {pm.x, pm.y, pm.z, pm.w};
                 const double ps[4] = {pp.x, pp.y, pp.z, pp.w};
                 float r32[4];
+                // block lattice: no neighbour across a block face
+                const bool cz_m = blk && kz % blk == 0, cz_p = blk && (kz + 1) % blk == 0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    double s = xs[q] + xs[q + 2];
-                    s = s + ym[q];
-                    s = s + yp[q];
-                    s = s + ms[q];
-                    s = s + ps[q];
+                    const double xl = q == 0 && cut_xm ? 0.0 : xs[q], xr = q == 3 && cut_xp ? 0.0 : xs[q + 2];
+                    double s = xl + xr;
+                    s = s + (cut_ym ? 0.0 : ym[q]);
+                    s = s + (cut_yp ? 0.0 : yp[q]);
+                    s = s + (cz_m ? 0.0 : ms[q]);
+                    s = s + (cz_p ? 0.0 : ps[q]);
                     const double r = bs[q] - (cdiag * xs[q + 1] - cc * s);
                     vr[0] += r * r;
                     r32[q] = (float)r;
@@ -815,6 +849,10 @@ static nnx_status check_grid(nnx_ctx* c, const nnx_grid* g) {
     NNX_CHECK(c, g->stencil == NNX_STENCIL_7 || g->stencil == NNX_STENCIL_27, "bad stencil %d", g->stencil);
     NNX_CHECK(c, g->stencil == NNX_STENCIL_7 || !g->kx, "variable coefficients need STENCIL_7");
     NNX_CHECK(c, (!g->kx && !g->ky && !g->kz) || (g->kx && g->ky && g->kz), "kx, ky, kz all or none");
+    NNX_CHECK(c, g->block >= 0, "block must be >= 0");
+    NNX_CHECK(c, g->block == 0 || (g->stencil == NNX_STENCIL_7 && !g->kx), "block needs the constant STENCIL_7");
+    NNX_CHECK(c, g->block == 0 || (g->nx % g->block == 0 && g->ny % g->block == 0 && g->nz % g->block == 0),
+              "grid dims must be multiples of block %d", g->block);
     return NNX_OK;
 }
 
@@ -945,6 +983,12 @@ static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float*
         k_cg_rupd<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k);
         return;
     }
+    if (fast && gk.block) {
+        k_stencil7x4<2, 2, false, kTY4, true><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
+        k_stencil7x4<3, 2, false, kTY4, true><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(
+            gk, h.d, k, pnew, nullptr, nullptr, nullptr, 0);
+        return;
+    }
     if (fast) {
         k_stencil7x4<2, 2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
         k_stencil7x4<3, 2><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
@@ -960,7 +1004,8 @@ static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float*
 // the variable-coefficient one (rows of a multiple of 4 voxels, 16-byte
 // aligned vectors and ky, kz); 0: the generic kernels.
 static int cg_fast(const GridK& gk, const float* x) {
-    if (gk.nx % 4 != 0 || (uintptr_t)x % 16 != 0) return 0;
+    // (a block lattice's faces must fall between the 4-voxel groups)
+    if (gk.nx % 4 != 0 || gk.block % 4 != 0 || (uintptr_t)x % 16 != 0) return 0;
     if (gk.kind == 0) return kFastConst;
     if (gk.kind == 1 && (uintptr_t)gk.ky % 16 == 0 && (uintptr_t)gk.kz % 16 == 0) return kFastVar;
     return 0;
@@ -1405,9 +1450,14 @@ nnx_status nnx_stencil_apply(nnx_ctx* c, const nnx_grid* g, const float* x, floa
     NNX_TRY(check_grid(c, g));
     NNX_CHECK(c, x && y && x != y, "x and y must be distinct device arrays");
     const GridK gk = make_grid(g);
-    if (gk.kind == 0 && gk.nx % 4 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0) {
+    if (gk.kind == 0 && gk.nx % 4 == 0 && gk.block % 4 == 0 && ((uintptr_t)x % 16) == 0 &&
+        ((uintptr_t)y % 16) == 0) {
         CgDev none{};
-        k_stencil7x4<0, 2><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
+        if (gk.block)
+            k_stencil7x4<0, 2, false, kTY4, true>
+                <<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
+        else
+            k_stencil7x4<0, 2><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
     } else {
         k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
     }
@@ -1501,11 +1551,11 @@ struct ParaHost {
     int32_t* pfl;
 };
 
-nnx_status para_check(nnx_ctx* c, const nnx_step_plan* p) {
-    const nnx_paracrine& q = p->paracrine;
+nnx_status para_check(nnx_ctx* c, const ParaIo& io) {
+    const nnx_paracrine& q = *io.q;
     if (q.every <= 0) return NNX_OK;
     NNX_TRY(check_grid(c, &q.grid));
-    NNX_CHECK(c, p->voxel_of && p->voxel_counts, "paracrine needs the release deposit (voxel_of, voxel_counts)");
+    NNX_CHECK(c, io.voxel_of && io.voxel_counts, "paracrine needs the release deposit (voxel_of, voxel_counts)");
     NNX_CHECK(c, q.conc, "paracrine needs conc");
     NNX_CHECK(c, q.rtol > 0 && q.inner_rtol >= 0 && q.release >= 0, "paracrine: rtol > 0, inner_rtol, release >= 0");
     NNX_CHECK(c, q.max_inner >= 0 && q.max_outer >= 0 && q.max_inner <= 100000 && q.max_outer <= 64,
@@ -1548,16 +1598,16 @@ static nnx_status para_host(nnx_ctx* c, const nnx_paracrine& q, const GridK& gk,
     return cg_bind(c, gk, NNX_PRECOND_JACOBI, m, &ph->h);
 }
 
-nnx_status para_prepare(nnx_ctx* c, const nnx_step_plan* p) {
-    if (p->paracrine.every <= 0) return NNX_OK;
-    const GridK gk = make_grid(&p->paracrine.grid);
+nnx_status para_prepare(nnx_ctx* c, const ParaIo& io) {
+    if (io.q->every <= 0) return NNX_OK;
+    const GridK gk = make_grid(&io.q->grid);
     ParaHost ph;
-    return para_host(c, p->paracrine, gk, &ph);
+    return para_host(c, *io.q, gk, &ph);
 }
 
 // One solve: b = conc + release * counts, counts = 0, then the refinement.
-nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
-    const nnx_paracrine& q = p->paracrine;
+nnx_status para_launch(nnx_ctx* c, const ParaIo& io) {
+    const nnx_paracrine& q = *io.q;
     const GridK gk = make_grid(&q.grid);
     const int64_t nv = (int64_t)gk.nx * gk.ny * gk.nz;
     ParaHost ph;
@@ -1574,11 +1624,13 @@ nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
     const bool tiled = fast == kFastConst && (uintptr_t)q.conc % 16 == 0;
     const int nb4 = fast_blocks(gk, fast), nbr = std::min(stencil4_blocks(gk), kRedBlocks);
     h.d.npq = fast ? nb4 : kRedBlocks;
-    k_para_rhs<<<kRedBlocks, 256, 0, c->stream>>>(nv, q.conc, p->voxel_counts, q.release, ph.b, ph.pbb, ph.pfl);
+    k_para_rhs<<<kRedBlocks, 256, 0, c->stream>>>(nv, q.conc, io.voxel_counts, q.release, ph.b, ph.pbb, ph.pfl);
     for (int o = 0; o < max_outer; ++o) {
         // the later refinement steps solve to a looser tolerance: half the budget
         const int budget = o == 0 ? max_inner : std::max(1, max_inner / 2);
-        if (tiled)
+        if (tiled && gk.block)
+            k_res64x4<false, true><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
+        else if (tiled)
             k_res64x4<false><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
         else
             k_para_res<false><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr,
@@ -1593,14 +1645,16 @@ nnx_status para_launch(nnx_ctx* c, const nnx_step_plan* p) {
             k_para_axpy<<<flat_blocks(nv), 256, 0, c->stream>>>(nv, ph.xin, q.conc, h.d.fl, ph.pfl);
         }
     }
-    if (tiled)
+    if (tiled && gk.block)
+        k_res64x4<true, true><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
+    else if (tiled)
         k_res64x4<true><<<nbr, kT4, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.prr, ph.pfl);
     else
         k_para_res<true><<<kRedBlocks, 256, 0, c->stream>>>(gk, q.conc, ph.b, h.d, ph.xin, h.pa, ph.prr, ph.pfl);
     k_para_ctl<<<1, 256, 0, c->stream>>>(h.d, ph.prr, ph.pbb, q.rtol, inner_rtol, max_inner, ph.pfl, q.stats, 1,
                                          max_outer > 0);
     if (q.at_neurons)
-        k_para_neurons<<<flat_blocks(p->state.n), 256, 0, c->stream>>>(p->state.n, p->voxel_of, q.conc, q.at_neurons);
+        k_para_neurons<<<flat_blocks(io.n), 256, 0, c->stream>>>(io.n, io.voxel_of, q.conc, q.at_neurons);
     NNX_LAUNCHED(c);
     return NNX_OK;
 }

@@ -62,7 +62,7 @@ static nnx_status check_plan(nnx_ctx* c, const nnx_step_plan* p) {
     if (p->params.noise_std != 0.0 && p->integrator != NNX_INT_LEAPFROG)
         return fail(c, NNX_ERR_UNSUPPORTED, "white noise (noise_std != 0) is drawn by the leapfrog step only");
     NNX_TRY(check_hebbian_plan(c, p));
-    NNX_TRY(para_check(c, p));
+    NNX_TRY(para_check(c, para_io(p)));
     return check_readout(c, p);
 }
 
@@ -78,7 +78,7 @@ static nnx_status enqueue_step(nnx_ctx* c, const nnx_step_plan* p, uint64_t s, i
     // Hebbian learning on the spikes this step's coupling reads
     if (p->hebbian.window > 0) NNX_TRY(hebbian_launch(c, &p->graph, &p->hebbian, ctr + 2 + (s & 1), 0, 0, prev));
     // paracrine solve on its cadence, from the releases of the steps before s
-    if (p->paracrine.every > 0 && s % (uint64_t)p->paracrine.every == 0) NNX_TRY(para_launch(c, p));
+    if (p->paracrine.every > 0 && s % (uint64_t)p->paracrine.every == 0) NNX_TRY(para_launch(c, para_io(p)));
     if (rk4 && p->couple_mode == NNX_COUPLE_WEIGHTED_EVENT) {
         // push the previous step's spikes into the targets' buckets; the
         // ephys kernel sums each bucket in source order before its update
@@ -126,7 +126,7 @@ extern "C" nnx_status nnx_run_steps(nnx_ctx* c, nnx_step_plan* p, uint64_t first
         NNX_TRY(event_scratch(c, p->graph.n, p->graph.nnz, &a, &b));
     }
     NNX_TRY(readout_prepare(c, p));
-    NNX_TRY(para_prepare(c, p));
+    NNX_TRY(para_prepare(c, para_io(p)));
     void* ctrp;
     NNX_TRY(scratch(c, "step_ctr", 64, true, &ctrp));
     int64_t* ctr = (int64_t*)ctrp;  // [0..1] raster length, [2..3] step, by parity

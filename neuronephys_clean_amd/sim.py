@@ -46,8 +46,23 @@ def upload_graph(ctx, net: Network, device_csc: bool = False):
     return g, d
 
 
+def _drop_graphs(sim):
+    """Drops the context's captured step graphs when a simulation goes
+    away: they hold its buffers' addresses (the cache would otherwise keep
+    them until it evicts them)."""
+    ctx = getattr(sim, "ctx", None)
+    if ctx is not None and getattr(ctx, "h", None):
+        try:
+            ctx.clear_graphs()
+        except Exception:
+            pass
+
+
 class Simulation:
     """RK4 squid network with exponential synapses and weighted coupling."""
+
+    def __del__(self):
+        _drop_graphs(self)
 
     def __init__(self, net: Network, dtype=np.float32, couple: str = "event",
                  record_cap: int = 0, deposit: bool = False, params: HHParams | None = None,
@@ -199,6 +214,9 @@ class ReferenceSimulation:
     into LOCAL_IN with LOCAL_COUNT = in-degree; the 17 SoA arrays of
     src/defines.cuh:57-78."""
 
+    def __del__(self):
+        _drop_graphs(self)
+
     def __init__(self, net: Network, sv0: dict, dtype=np.float32, couple: str = "count",
                  record_cap: int = 0, params: HHParams | None = None, dt: float = 0.4,
                  ctx: _lib.Context | None = None, noise: bool = False):
@@ -284,6 +302,9 @@ class ReferenceSimulation:
 class EnsembleSim:
     """An ensemble (configs.Ensemble) stepped by nnx_ensemble_run."""
 
+    def __del__(self):
+        _drop_graphs(self)
+
     def __init__(self, ens, ctx: _lib.Context | None = None, record_cap: int = 0, deposit: bool = False,
                  params: HHParams | None = None):
         self.ctx = ctx or _lib.context()
@@ -309,7 +330,7 @@ class EnsembleSim:
         a["raster"] = DeviceArray.zeros(ctx, max(2 * record_cap, 2), np.int32)
         a["raster_len"] = DeviceArray.zeros(ctx, 1, np.int64)
         if deposit:
-            a["voxel_counts"] = DeviceArray.zeros(ctx, len(ens.size) * 64 ** 3, np.int32)
+            a["voxel_counts"] = DeviceArray.zeros(ctx, int(np.prod(ens.grid)), np.int32)
             a["voxel_of"] = DeviceArray.from_numpy(ctx, ens.voxel_of)
         self._begin = np.ascontiguousarray(ens.begin, np.int64)
         self._size = np.ascontiguousarray(ens.size, np.int64)
@@ -352,6 +373,37 @@ class EnsembleSim:
             self.ctx.call("nnx_ensemble_run", C.byref(self.plan), pc, self.step, k, 1)
             self.step += k
             done += k
+
+    def set_paracrine(self, every: int, rtol: float = 1e-8, inner_rtol: float = 1e-6, max_inner: int = 0,
+                      max_outer: int = 0, release: float = 0.01):
+        """Solves every network's 64^3 paracrine grid inside the step every
+        `every` steps: one solve of the release box with Dirichlet faces
+        between the networks' grids (nnx_grid.block = 64); needs
+        deposit=True.  `every` <= 0 turns it off."""
+        ens, a, ctx = self.ens, self.a, self.ctx
+        q = self.plan.paracrine
+        if every <= 0:
+            q.every = 0
+            return
+        if "voxel_counts" not in a:
+            raise ValueError("the paracrine solve needs the release deposit (deposit=True)")
+        nx, ny, nz = ens.grid
+        a["conc"] = DeviceArray.zeros(ctx, nx * ny * nz, np.float64)
+        a["para_stats"] = DeviceArray.zeros(ctx, 8, np.float64)
+        q.every = every
+        q.grid = _lib.Grid(nx, ny, nz, _lib.STENCIL_7, ens.c, 0.0, None, None, None, 64)
+        q.release = release
+        q.rtol = rtol
+        q.inner_rtol = inner_rtol
+        q.max_inner = max_inner
+        q.max_outer = max_outer
+        q.conc = a["conc"].ptr
+        q.at_neurons = None
+        q.stats = a["para_stats"].ptr
+
+    def paracrine_stats(self) -> dict:
+        v = self.a["para_stats"].numpy()
+        return dict(zip(_lib.PARA_STATS, (float(x) for x in v)))
 
     def autotune(self, steps_per_candidate: int = 20):
         best, us, ran = _lib.EnsConfig(), C.c_double(), C.c_int()

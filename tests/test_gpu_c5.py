@@ -4,8 +4,9 @@ residual of 1e-8 recomputed on the host in FP64 with the oracle's operator:
 - C2's grid: 256^3, dt D / h^2 = 1, Jacobi-CG with FP64 refinement;
 - C5: 256^3, log-normal diffusivity (log-sigma 1, correlation 8 voxels,
   mean dt D / h^2 = 50, harmonic-mean faces), through Jacobi-CG, flexible CG
-  with the shipped trained network (recording whether it fell back) and
-  flexible CG with the multigrid cycle.
+  with the shipped trained network (recording whether it fell back),
+  flexible CG with the multigrid cycle, and with the shipped network
+  trained on what the cycle leaves correcting it.
 
 The right-hand side is a deterministic release field: 0.01 x the per-voxel
 spike counts of the first 1,000 steps of the C2 network (fused step,
@@ -18,7 +19,7 @@ import pytest
 import oracle
 from neuronephys_clean_amd import _lib, configs
 from neuronephys_clean_amd.devmem import DeviceArray as DA
-from neuronephys_clean_amd.precond import UNet, device_field
+from neuronephys_clean_amd.precond import MG_WEIGHTS, UNet, device_field
 from neuronephys_clean_amd.sim import Simulation
 
 pytestmark = pytest.mark.gpu
@@ -64,8 +65,11 @@ def test_c5_256_solvers_true_1e8(ctx, release_field):
     b = release_field
     db = DA.from_numpy(ctx, b)
     net = UNet.load(ctx)
+    net_mg = UNet.load(ctx, MG_WEIGHTS)
+    nets = {_lib.PRECOND_LEARNED: net, _lib.PRECOND_MG_LEARNED: net_mg}
     out = {}
-    for name, pc in (("jacobi_cg", None), ("learned_fcg", _lib.PRECOND_LEARNED), ("mg_fcg", _lib.PRECOND_MG)):
+    for name, pc in (("jacobi_cg", None), ("learned_fcg", _lib.PRECOND_LEARNED), ("mg_fcg", _lib.PRECOND_MG),
+                     ("mg_learned_fcg", _lib.PRECOND_MG_LEARNED)):
         dx = DA.zeros(ctx, n ** 3, np.float64)
         o = _lib.CgOpts(rtol=1e-8, maxit=20000, precond=_lib.PRECOND_JACOBI if pc is None else pc,
                         check_every=8, stall_iters=10, smooth=2 if pc == _lib.PRECOND_LEARNED else 1)
@@ -73,12 +77,12 @@ def test_c5_256_solvers_true_1e8(ctx, release_field):
         if pc is None:
             ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-6, 30, C.byref(r))
         else:
-            ctx.call("nnx_fcg_solve_refine", C.byref(g), net.h if pc == _lib.PRECOND_LEARNED else None,
+            ctx.call("nnx_fcg_solve_refine", C.byref(g), nets[pc].h if pc in nets else None,
                      keep["feat"].ptr, db.ptr, dx.ptr, C.byref(o), 1e-6, 30, C.byref(r))
         rel = _true_relres(og, dx.numpy(), b)
         d = r.as_dict()
         d["host_relres"] = rel
-        if pc == _lib.PRECOND_LEARNED:
+        if pc in nets:
             d["fell_back"] = r.precond_used != pc
         out[name] = d
         say(f"C5 {name}: {d}")
@@ -87,4 +91,8 @@ def test_c5_256_solvers_true_1e8(ctx, release_field):
     # the multigrid cycle cuts that by two orders of magnitude
     assert out["jacobi_cg"]["iters"] > 1000
     assert out["mg_fcg"]["iters"] * 20 < out["jacobi_cg"]["iters"]
+    # the network correcting the cycle, weighted by its energy-optimal factor,
+    # never falls back here; its mean weight is reported
+    assert not out["mg_learned_fcg"]["fell_back"]
     net.close()
+    net_mg.close()

@@ -6,6 +6,7 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import scipy.fft
 
 import oracle
 from neuronephys_clean_amd import _lib, configs
@@ -137,3 +138,57 @@ def test_full_ensemble_matches_oracle_per_network(ctx):
         total += rln
     say("spikes", total)
     assert total == ln and total > len(full.members) * 1000
+
+
+def _dst_solve64(b, c):
+    th = 2 * np.cos(np.pi * np.arange(1, 65) / 65)
+    tz, ty, tx = np.meshgrid(th, th, th, indexing="ij")
+    lam = 1 + c * ((2 - tx) + (2 - ty) + (2 - tz))
+    return scipy.fft.dstn(scipy.fft.dstn(b, type=1, norm="ortho") / lam, type=1, norm="ortho")
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_ensemble_per_network_paracrine_solves(ctx, ens, graph):
+    """Every network's own 64^3 grid solved inside the ensemble's steps
+    every 100 steps (one solve of the box with Dirichlet faces between the
+    grids): after each solve every network's block equals the exact DST-I
+    solution of its own system, b = its previous field + 0.01 x the
+    releases of its neurons in the window, and the box's FP64 relative
+    residual is <= 1e-8."""
+    every, windows = 100, 6
+    sim = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20, deposit=True)
+    sim.set_paracrine(every, rtol=1e-8, max_inner=48, max_outer=3)
+    nx, ny, nz = ens.grid
+    say("ensemble paracrine: box", ens.grid, "lattice", ens.lattice, "graph", graph)
+    og = oracle.grid(64, 64, 64, 7, ens.c)
+    solved = 0
+    for w in range(windows):
+        prev = sim.a["conc"].numpy().reshape(nz, ny, nx)
+        counts = sim.a["voxel_counts"].numpy().reshape(nz, ny, nx)
+        sim.run(every, use_graph=graph, chunk=every)
+        conc = sim.a["conc"].numpy().reshape(nz, ny, nx)
+        if w == 0:
+            assert not counts.any() and not conc.any()
+            continue
+        worst_rel, worst_err = 0.0, 0.0
+        for i in range(len(SIZES)):
+            ox, oy, oz = ens.block_of(i)
+            sl = np.s_[oz:oz + 64, oy:oy + 64, ox:ox + 64]
+            b = prev[sl] + 0.01 * counts[sl].astype(np.float64)
+            x = conc[sl]
+            if not b.any():  # no release in this network yet
+                assert not x.any(), i
+                continue
+            solved += 1
+            xs = _dst_solve64(b, ens.c)
+            worst_err = max(worst_err, np.linalg.norm(x - xs) / np.linalg.norm(xs))
+            r = b.ravel() - oracle.apply(og, np.ascontiguousarray(x).ravel())
+            worst_rel = max(worst_rel, np.linalg.norm(r) / np.linalg.norm(b))
+        say(f"  solve {w}: releases {int(counts.sum())}, worst network relres {worst_rel:.2e}, "
+            f"worst error vs DST {worst_err:.2e}")
+        assert worst_err <= 1e-7
+        assert worst_rel <= 1e-7
+    st = sim.paracrine_stats()
+    say("  stats", st, "network solves checked", solved)
+    assert solved >= len(SIZES)
+    assert st["solves"] == windows and st["unconverged"] == 0 and st["max_relres"] <= 1e-8
