@@ -774,14 +774,15 @@ def cpu_baseline_full(net, every, b_fixed, all_steps=300, one_steps=20, reps=3):
     oracle.set_threads(0)
     g = oracle.grid(*net.grid, 7, net.c)
     t0 = time.perf_counter()
-    x, outer = oracle.refine(g, b_fixed, 1e-8, 1e-6)
+    x, outer, inner, rel = oracle.refine(g, b_fixed, 1e-8, 1e-6)
     t_solve = time.perf_counter() - t0
     per_step = t_all + (t_solve / every if every > 0 else 0.0)
     return {"value": net.n / per_step, "unit": "neuron-steps/s", "cores": nthreads, "kind": "port",
             "sample": f"oracle fp32 step (RK4 + push coupling in ascending source order + release counting) over "
                       f"the first {all_steps} steps of the same network and initial state, median of {reps} runs "
                       f"on {nthreads} threads ({t_all * 1e3:.2f} ms/step), plus one FP64-refined Jacobi-CG solve of "
-                      f"the 256^3 grid to 1e-8 on the fixed release field ({t_solve:.2f} s, {outer} outer steps) "
+                      f"the 256^3 grid to 1e-8 on the fixed release field ({t_solve:.2f} s, {outer} outer steps, "
+                      f"{inner} inner iterations, relres {rel:.2e}) "
                       f"every {every} steps",
             "one_core": {"value": net.n / (t_one + (t_solve * nthreads / every if every > 0 else 0.0)),
                          "step_ms": t_one * 1e3, "steps": one_steps,
@@ -961,14 +962,17 @@ def main():
                                      "(the step kernel plus its share of one push and one compaction per "
                                      "100-step graph)",
                       "alg_bytes_per_launch": fb,
-                      "alg_bytes_rule": "48.125 B per neuron + 32 B per delivered event"}
+                      "alg_bytes_rule": "48.125 B per neuron + 32 B per delivered event",
+                      "note": "bound by VALU issue, not HBM, on quiet steps: 578 VALU instructions per neuron "
+                              "at ~0.9 of the non-packed issue rate (profiles/valu_rk4_kernels_c2.md), and by "
+                              "random event traffic on the ~290 steps of synchronous firing"}
     cg_iter = cg_iteration_roofline(ctx, net.grid[0], b_fixed.astype(np.float32), net.c)
     log(f"bench: CG iteration {cg_iter}")
     t_cg_iters = cg_iter["avg_iteration_us"] * 1e-3 * pst["inner_iters"]
     t_steps = ev0_ms
-    roof = fused_roof if (fused_roof and t_steps >= t_cg_iters) else cg_iter
-    roof = dict(roof)
-    roof["share_of_timed_window"] = (t_steps if roof is fused_roof else t_cg_iters) / ev_ms
+    use_fused = bool(fused_roof) and t_steps >= t_cg_iters
+    roof = dict(fused_roof if use_fused else cg_iter)
+    roof["share_of_timed_window"] = (t_steps if use_fused else t_cg_iters) / ev_ms
 
     # the HH RK4 kernel of the unfused path, timed alone on the library
     # stream over the same state
