@@ -289,6 +289,62 @@ def test_stencil_apply_bitexact(ctx, kind, dims):
     assert np.array_equal(dy.numpy(), oracle.apply(og, x))
 
 
+@pytest.mark.parametrize("dims,block", [((128, 64, 32), 32), ((96, 40, 24), 8), ((24, 24, 24), 6)])
+def test_stencil_apply_block_lattice(ctx, dims, block):
+    """A block lattice (nnx_grid.block): the apply equals, block by block,
+    the oracle's apply of each block's own Dirichlet grid, bit for bit (the
+    tiled kernel for the first two, whose blocks are multiples of 4; the
+    generic one for the 6-voxel blocks)."""
+    nx, ny, nz = dims
+    c = 1.3
+    g = _lib.Grid(nx, ny, nz, 7, c, 0.0, None, None, None, block)
+    ob = oracle.grid(block, block, block, 7, c)
+    x = np.random.default_rng(8).standard_normal(nx * ny * nz).astype(np.float32)
+    dx, dy = DA.from_numpy(ctx, x), DA.zeros(ctx, x.size, np.float32)
+    ctx.call("nnx_stencil_apply", C.byref(g), dx.ptr, dy.ptr)
+    y, X = dy.numpy().reshape(nz, ny, nx), x.reshape(nz, ny, nx)
+    for k in range(0, nz, block):
+        for j in range(0, ny, block):
+            for i in range(0, nx, block):
+                sl = np.s_[k:k + block, j:j + block, i:i + block]
+                ref = oracle.apply(ob, np.ascontiguousarray(X[sl]).ravel()).reshape(block, block, block)
+                assert np.array_equal(y[sl], ref), (k, j, i)
+
+
+def test_cg_block_lattice_solves_every_block(ctx):
+    """One refined CG solve of a 4 x 2 x 2 lattice of 16^3 blocks equals the
+    exact DST-I solution of every block's own system."""
+    nb, c = 16, 1.0
+    nx, ny, nz = 4 * nb, 2 * nb, 2 * nb
+    g = _lib.Grid(nx, ny, nz, 7, c, 0.0, None, None, None, nb)
+    b = np.random.default_rng(9).random(nx * ny * nz)
+    db, dx = DA.from_numpy(ctx, b), DA.zeros(ctx, b.size, np.float64)
+    o = _lib.CgOpts(rtol=1e-10, maxit=500, precond=_lib.PRECOND_JACOBI, check_every=4)
+    r = _lib.CgResult()
+    ctx.call("nnx_cg_solve_refine", C.byref(g), db.ptr, dx.ptr, C.byref(o), 1e-6, 20, C.byref(r))
+    assert r.converged
+    lam = _lam7(nb, nb, nb, c)[0]
+    X, B = dx.numpy().reshape(nz, ny, nx), b.reshape(nz, ny, nx)
+    worst = 0.0
+    for k in range(0, nz, nb):
+        for j in range(0, ny, nb):
+            for i in range(0, nx, nb):
+                sl = np.s_[k:k + nb, j:j + nb, i:i + nb]
+                xs = _dst_solve(B[sl], lam)
+                worst = max(worst, np.linalg.norm(X[sl] - xs) / np.linalg.norm(xs))
+    say("block lattice CG", r.as_dict(), "worst block error vs DST", worst)
+    assert worst <= 1e-9
+
+
+def test_block_lattice_argument_errors(ctx):
+    x = DA.zeros(ctx, 64 ** 3, np.float32)
+    y = DA.zeros(ctx, 64 ** 3, np.float32)
+    for g in (_lib.Grid(64, 64, 64, 7, 1.0, 0.0, None, None, None, 24),   # 64 not a multiple of 24
+              _lib.Grid(64, 64, 64, 27, 1.0, 0.0, None, None, None, 16),  # 27-point
+              _lib.Grid(64, 64, 64, 7, 1.0, 0.0, None, None, None, -1)):
+        assert ctx._l.nnx_stencil_apply(ctx.h, C.byref(g), x.ptr, y.ptr) == 1  # NNX_ERR_ARG
+
+
 def _lam7(nx, ny, nz, c):
     th = [2 * np.cos(np.pi * np.arange(1, m + 1) / (m + 1)) for m in (nz, ny, nx)]
     tz, ty, tx = np.meshgrid(*th, indexing="ij")
