@@ -58,6 +58,24 @@ __device__ __forceinline__ int tap_dz(int t) { return t / 9; }
 __device__ __forceinline__ int tap_dy(int t) { return (t / 3) % 3; }
 __device__ __forceinline__ int tap_dx(int t) { return t % 3; }
 
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+// two floats to bf16 (round to nearest even) in one instruction
+__device__ __forceinline__ uint32_t pack2bf(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{lo, hi}, bf16x2_t));
+}
+// 8 bf16 of a + s * c, each one fma rounded once to bf16
+__device__ __forceinline__ u32x4_t fma_scaled8(u32x4_t a, u32x4_t c, float s) {
+    u32x4_t o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float lo = __builtin_fmaf(s, __uint_as_float(c[q] << 16), __uint_as_float(a[q] << 16));
+        const float hi = __builtin_fmaf(s, __uint_as_float(c[q] & 0xffff0000u), __uint_as_float(a[q] & 0xffff0000u));
+        o[q] = pack2bf(lo, hi);
+    }
+    return o;
+}
+
 // 8 bf16 of a + s * c, rounded once
 __device__ __forceinline__ u32x4_t add_scaled8(u32x4_t a, u32x4_t c, float s) {
     u32x4_t o;
@@ -284,6 +302,297 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
                 }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward convolution, z-marching form
+// ---------------------------------------------------------------------------
+// A workgroup owns a 16 x 4RY column of output rows over a chunk of zc
+// planes and walks up z one input plane at a time.  The plane, with its
+// one-voxel x/y halo and the upsampled coarse term added, is staged in LDS
+// in two buffers: the next plane's loads are in flight while this one is
+// read.  K is ordered plane by plane (a k-block holds taps (dy, dx) of one
+// input plane x channels), so the B fragment read for an output row serves
+// the three output planes that read this input plane (dz = 0, 1, 2), and
+// three accumulator sets rotate as output planes complete.  The weights
+// stay in registers as A fragments, loaded once per workgroup.  An input
+// plane is read from memory once per chunk (zc + 2 planes for zc outputs);
+// a 16-channel plane's 9 taps take 5 k-blocks (one half-block of zero
+// weights), a 32-channel plane's 9, the 2-channel input's 1 (7 taps of
+// zero weights).
+template <int CIN> struct ZK {
+    static constexpr int TPK = 32 / CIN;             // taps per k-block
+    static constexpr int KBP = (9 + TPK - 1) / TPK;  // k-blocks per input plane
+    static constexpr bool WIDE = CIN >= 8;           // staged in 16-byte chunks of 8 channels
+    static constexpr int NC8 = WIDE ? CIN / 8 : 1;
+};
+
+template <int CIN, int COUT, int RY, int MT, bool CO>
+__global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
+    static_assert(!CO || ZK<CIN>::WIDE, "the 2-channel input takes no upsampled term");
+    static_assert(RY % 2 == 0, "row pairs for the pooled output");
+    // MT m-tiles (16 output channels each) per workgroup: channel group
+    // blockIdx.y % NG of NG
+    constexpr int KBP = ZK<CIN>::KBP, NC8 = ZK<CIN>::NC8, NG = COUT / (16 * MT), TY = 4 * RY, HR = TY + 2;
+    constexpr int KP = kpad(CIN);
+    constexpr bool WIDE = ZK<CIN>::WIDE;
+    constexpr int RS = WIDE ? NC8 * 18 * 8 : 18 * 2;     // bf16 elements per staged row
+    constexpr int PE = HR * RS;                          // bf16 elements per staged plane
+    constexpr int NCH = WIDE ? HR * 18 * NC8 : HR * 18;  // load chunks (16 B or 4 B) per plane
+    constexpr int IT = (NCH + 255) / 256;
+    using SV = typename std::conditional<WIDE, u32x4_t, uint32_t>::type;
+    __shared__ __attribute__((aligned(16))) uint16_t s_pl[2][PE];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, xl = lane & 15;
+    const int nx = a.nx, ny = a.ny, nz = a.nz, ntx = nx / 16;
+    const int x0 = (blockIdx.x % ntx) * 16, y0 = (blockIdx.x / ntx) * TY;
+    const int b = blockIdx.z, z0 = (blockIdx.y / NG) * zc, cb = 16 * MT * (blockIdx.y % NG);
+    const int zn = min(zc, nz - z0), nsteps = zn + 2;
+    const u32x4_t zero4 = u32x4_t{0u, 0u, 0u, 0u};
+    // A fragments: lane (g, xl) holds W[co = 16 mt + xl][k = 8g .. 8g + 7] of
+    // k-block j of input plane dz; taps past the ninth have zero weights
+    u32x4_t wa[3][KBP][MT];
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+        for (int j = 0; j < KBP; ++j)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const uint16_t* wr = a.w + (size_t)(cb + 16 * mt + xl) * KP + dz * 9 * CIN;
+                if constexpr (CIN == 2) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int s = 4 * g + u;
+                        wa[dz][j][mt][u] = s < 9 ? *(const uint32_t*)&wr[2 * s] : 0u;
+                    }
+                } else if constexpr (CIN == 16) {
+                    const int s = 2 * j + (g >> 1);
+                    wa[dz][j][mt] = s < 9 ? *(const u32x4_t*)&wr[16 * s + 8 * (g & 1)] : zero4;
+                } else {
+                    wa[dz][j][mt] = *(const u32x4_t*)&wr[32 * j + 8 * g];
+                }
+            }
+    // B fragment offsets in a staged plane (row 0 = y0 - 1, column 0 = x0 - 1):
+    // lane (g, xl) reads tap (dy, dx) of its k-slice at voxel x0 + xl
+    constexpr int NF = WIDE ? KBP : 4;
+    int foff[NF];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) {
+        if constexpr (WIDE) {
+            const int s = CIN == 32 ? j : min(2 * j + (g >> 1), 8), c8 = CIN == 32 ? g : (g & 1);
+            foff[j] = (s / 3) * RS + c8 * 18 * 8 + (s % 3 + xl) * 8;
+        } else {
+            const int s = min(4 * g + j, 8);
+            foff[j] = (s / 3) * RS + (s % 3 + xl) * 2;
+        }
+    }
+    // staging: chunk i of a plane is (row, x, 8-channel group) in memory
+    // order; a voxel outside the grid reads the nearest inside one and is
+    // stored as zero (unconditional loads)
+    const int64_t pstride = (int64_t)ny * nx * CIN, cpstride = (int64_t)(ny >> 1) * (nx >> 1) * CIN;
+    const uint16_t* inb = a.in + (size_t)b * nz * pstride;
+    const uint16_t* cinb = CO ? a.coarse + (size_t)b * (nz >> 1) * cpstride : nullptr;
+    int gofs[IT], cofs[IT], lofs[IT];
+    uint32_t inside = 0;
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+        int i = tid + 256 * u;
+        i = i < NCH ? i : NCH - 1;
+        const int c8 = i % NC8, vx = (i / NC8) % 18, row = i / (NC8 * 18);
+        const int gx = x0 - 1 + vx, gy = y0 - 1 + row;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) inside |= 1u << u;
+        const int cx = min(max(gx, 0), nx - 1), cy = min(max(gy, 0), ny - 1);
+        gofs[u] = (cy * nx + cx) * CIN + 8 * c8;
+        cofs[u] = ((cy >> 1) * (nx >> 1) + (cx >> 1)) * CIN + 8 * c8;
+        lofs[u] = WIDE ? ((row * NC8 + c8) * 18 + vx) * 8 : (row * 18 + vx) * 2;
+    }
+    // two register sets: plane s + 2 is loaded during step s while plane
+    // s + 1, loaded during step s - 1, is stored to LDS at its end
+    SV v[2][IT];
+    u32x4_t cv[2][CO ? IT : 1];
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int u = 0; u < IT; ++u) v[e][u] = SV{};
+    auto load = [&](auto ec, int iz) __attribute__((always_inline)) {
+        constexpr int E = decltype(ec)::value;
+        const uint16_t* p = inb + (size_t)iz * pstride;
+#pragma unroll
+        for (int u = 0; u < IT; ++u) v[E][u] = *(const SV*)&p[gofs[u]];
+        if constexpr (CO) {
+            const uint16_t* q = cinb + (size_t)(iz >> 1) * cpstride;
+#pragma unroll
+            for (int u = 0; u < IT; ++u) cv[E][u] = *(const u32x4_t*)&q[cofs[u]];
+        }
+    };
+    auto store = [&](auto ec, uint16_t* pl, bool zero) __attribute__((always_inline)) {
+        constexpr int E = decltype(ec)::value;
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            if (tid + 256 * u < NCH) {
+                const bool in = !zero && ((inside >> u) & 1u);
+                if constexpr (WIDE) {
+                    u32x4_t o = v[E][u];
+                    if constexpr (CO) o = fma_scaled8(v[E][u], cv[E][u], a.cscale);
+                    *(u32x4_t*)&pl[lofs[u]] = in ? o : zero4;
+                } else {
+                    *(uint32_t*)&pl[lofs[u]] = in ? v[E][u] : 0u;
+                }
+            }
+        }
+    };
+    // epilogue constants: lane holds voxel x0 + xl, channels cb + 16 mt + 4 g + q
+    // (the 1x1 head needs every channel in one workgroup: NG = 1)
+    float bias_r[MT][4], head_r[MT][4];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int co = cb + 16 * mt + 4 * g + q;
+            bias_r[mt][q] = a.bias ? a.bias[co] : 0.f;
+            head_r[mt][q] = a.head ? a.head[co] : 0.f;
+        }
+    const float hb = a.head ? a.head[COUT] : 0.f, hs = a.hscale ? a.hscale[b] : 1.f;
+    const int yw = y0 + wave * RY;
+    f32x4_t pacc[RY / 2][MT];
+    f32x4_t acc[3][RY][MT];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < RY; ++r)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[p][r][mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    // (lambdas are forced inline: an accumulator set passed out of line
+    // would live in scratch memory)
+    // per-lane offsets within an output plane (voxels); the activation's
+    // slope below zero (1: identity); lane-exchange addresses
+    const int lvox = yw * nx + x0 + xl;
+    const float leak = a.relu ? kLeak : 1.f;
+    const int sw16 = (lane ^ 16) << 2, sw32 = (lane ^ 32) << 2;
+    auto xch = [](int addr, float v) __attribute__((always_inline)) {
+        return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+    };
+    auto epilogue = [&](auto sc, int z) __attribute__((always_inline)) {
+        auto& A = acc[decltype(sc)::value];
+        const int64_t zb = ((int64_t)b * nz + z) * ny * nx;  // first voxel of output plane z
+        uint16_t* outz = a.out ? a.out + zb * COUT : nullptr;
+        float* houtz = a.head ? a.hout + zb : nullptr;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int vo = lvox + r * nx;
+            float hsum = 0.f;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                f32x4_t v4 = A[r][mt];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float s = v4[q] + bias_r[mt][q];
+                    s = fmaxf(s, leak * s);  // leaky ReLU (slope leak <= 1)
+                    v4[q] = s;
+                    hsum = __builtin_fmaf(head_r[mt][q], s, hsum);
+                }
+                A[r][mt] = v4;
+                if (outz) {
+                    u32x2_t o;
+                    o[0] = pack2bf(v4[0], v4[1]);
+                    o[1] = pack2bf(v4[2], v4[3]);
+                    *(u32x2_t*)&outz[vo * COUT + cb + 16 * mt + 4 * g] = o;
+                }
+            }
+            if (houtz) {
+                hsum += xch(sw16, hsum);
+                hsum += xch(sw32, hsum);
+                if (g == 0) houtz[vo] = hs * (hsum + hb);
+            }
+        }
+        if (a.pool) {
+            // 2x2x2 average: (y, y+1) pairs here, (z, z+1) across two
+            // planes, (x, x+1) across lanes
+            const int px = nx >> 1, py = ny >> 1, pz = nz >> 1;
+#pragma unroll
+            for (int j = 0; j < RY / 2; ++j)
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const f32x4_t sm = A[2 * j][mt] + A[2 * j + 1][mt];
+                    if ((z & 1) == 0) {
+                        pacc[j][mt] = sm;
+                    } else {
+                        f32x4_t s = pacc[j][mt] + sm;
+                        // x pairs: lanes (2i, 2i+1) swapped by a quad permutation [1,0,3,2]
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            s[q] = (s[q] + __int_as_float(__builtin_amdgcn_update_dpp(
+                                               0, __float_as_int(s[q]), 0xB1, 0xF, 0xF, false))) *
+                                   0.125f;
+                        if ((xl & 1) == 0) {
+                            u32x2_t o;
+                            o[0] = pack2bf(s[0], s[1]);
+                            o[1] = pack2bf(s[2], s[3]);
+                            const int64_t pv =
+                                (((int64_t)b * pz + (z >> 1)) * py + ((yw >> 1) + j)) * px + ((x0 + xl) >> 1);
+                            *(u32x2_t*)&a.pool[pv * COUT + cb + 16 * mt + 4 * g] = o;
+                        }
+                    }
+                }
+        }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    // input plane z0 - 1 (zero below the grid) to LDS; plane z0 in flight
+    if (z0 > 0) load(I0{}, z0 - 1);
+    store(I0{}, s_pl[0], z0 == 0);
+    load(I1{}, z0);
+    bool zpl[2] = {false, false};  // register set holds a plane outside the grid
+    // every load so far (weights, epilogue constants) complete before the
+    // loop: otherwise the first use of a weight inside the loop carries a
+    // memory-count wait that, on every later step, also waits for the
+    // plane just prefetched
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // Step s reads input plane z0 - 1 + s from buffer s & 1: output plane
+    // z0 + s - dz takes tap plane dz, so accumulator set (s - dz) mod 3 is
+    // updated (set s mod 3 starts afresh) and output plane z0 + s - 2 is
+    // complete at the end of the step.  Updates of planes outside the chunk
+    // land in sets that are restarted before use or never stored.
+    auto step = [&](auto pc, auto ec, int s) __attribute__((always_inline)) {
+        constexpr int P = decltype(pc)::value, E = decltype(ec)::value;
+        const uint16_t* pl = s_pl[E] + wave * RY * RS;
+        const int iz = z0 + s + 1;  // input plane of step s + 2
+        if (s + 2 < nsteps) {
+            zpl[E] = iz >= nz;
+            if (iz < nz) load(ec, iz);
+        }
+#pragma unroll
+        for (int j = 0; j < KBP; ++j)
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                u32x4_t bf;
+                if constexpr (WIDE) {
+                    bf = *(const u32x4_t*)&pl[r * RS + foff[j]];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) bf[u] = *(const uint32_t*)&pl[r * RS + foff[u]];
+                }
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const f32x4_t c0 = j == 0 ? f32x4_t{0.f, 0.f, 0.f, 0.f} : acc[P][r][mt];
+                    acc[P][r][mt] = mfma16(wa[0][j][mt], bf, c0);
+                    acc[(P + 2) % 3][r][mt] = mfma16(wa[1][j][mt], bf, acc[(P + 2) % 3][r][mt]);
+                    acc[(P + 1) % 3][r][mt] = mfma16(wa[2][j][mt], bf, acc[(P + 1) % 3][r][mt]);
+                }
+            }
+        if (s >= 2) epilogue(std::integral_constant<int, (P + 1) % 3>{}, z0 + s - 2);
+        if (s + 1 < nsteps) store(std::integral_constant<int, E ^ 1>{}, s_pl[E ^ 1], zpl[E ^ 1]);
+        __syncthreads();
+    };
+    for (int s0 = 0; s0 < nsteps; s0 += 6) {
+        step(I0{}, I0{}, s0);
+        if (s0 + 1 < nsteps) step(I1{}, I1{}, s0 + 1);
+        if (s0 + 2 < nsteps) step(I2{}, I0{}, s0 + 2);
+        if (s0 + 3 < nsteps) step(I0{}, I1{}, s0 + 3);
+        if (s0 + 4 < nsteps) step(I1{}, I0{}, s0 + 4);
+        if (s0 + 5 < nsteps) step(I2{}, I1{}, s0 + 5);
     }
 }
 
@@ -872,13 +1181,67 @@ struct nnx_unet {
 
 namespace {
 
+// workgroups of 256 threads of kernel k resident at once on the device
+template <class K> int64_t resident_wgs(nnx_ctx* c, K k) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+        cus = 1;
+    return (int64_t)per_cu * cus;
+}
+
+// Chunk length zc (even: chunks start on even planes, as the pooled pairs
+// need) minimising rounds x (zc + 2): a launch of W workgroups per chunk row
+// runs ceil(W nch / resident) rounds of workgroups that each walk zc + 2
+// input planes.
+inline int conv_chunk(int nz, int64_t per_chunk, int64_t resident) {
+    int best = nz + (nz & 1);
+    double best_t = 1e300;
+    for (int n = 1; n <= std::max(1, nz / 2); ++n) {
+        int zc = (int)ceil_div(nz, n);
+        zc += zc & 1;
+        if (ceil_div(nz, zc) != n) continue;
+        const double t = (double)ceil_div(per_chunk * n, resident) * (zc + 2);
+        if (t < best_t) best_t = t, best = zc;
+    }
+    return best;
+}
+
+template <int CIN, int COUT, int RY, int MT>
+nnx_status launch_conv_z(nnx_ctx* c, const ConvArgs& a) {
+    constexpr int TY = 4 * RY, NG = COUT / (16 * MT);
+    NNX_CHECK(c, NG == 1 || !a.head, "the 1x1 head needs every channel in one workgroup");
+    const bool co = ZK<CIN>::WIDE && a.coarse;
+    // residency of each form, measured once
+    static const int64_t res0 = resident_wgs(c, k_conv3z<CIN, COUT, RY, MT, false>);
+    static const int64_t res1 = ZK<CIN>::WIDE ? resident_wgs(c, k_conv3z<CIN, COUT, RY, MT, ZK<CIN>::WIDE>) : res0;
+    const int64_t cols = (int64_t)(a.nx / 16) * (a.ny / TY);
+    const int zc = conv_chunk(a.nz, cols * a.batch * NG, co ? res1 : res0);
+    const int nch = (int)ceil_div(a.nz, zc);
+    const dim3 grid((unsigned)cols, (unsigned)(nch * NG), (unsigned)a.batch);
+    if (co)
+        k_conv3z<CIN, COUT, RY, MT, ZK<CIN>::WIDE><<<grid, 256, 0, c->stream>>>(a, zc);
+    else
+        k_conv3z<CIN, COUT, RY, MT, false><<<grid, 256, 0, c->stream>>>(a, zc);
+    NNX_LAUNCHED(c);
+    return NNX_OK;
+}
+
 template <int CIN, int COUT>
 nnx_status launch_conv(nnx_ctx* c, const ConvArgs& a) {
     constexpr int TY = ConvTile<CIN>::TY, TZ = ConvTile<CIN>::TZ;
-    NNX_CHECK(c, a.nx % 16 == 0 && a.ny % TY == 0 && a.nz % TZ == 0 && a.batch >= 1,
-              "conv %d->%d: dims (%d,%d,%d) must be multiples of (16,%d,%d)", CIN, COUT, a.nx, a.ny, a.nz, TY, TZ);
+    // the z-marching form where its 16 x 4RY columns tile the grid (RY and
+    // the m-tiles per workgroup keep weights, accumulators and the staged
+    // plane within 256 registers), else one 16 x TY x TZ tile per workgroup
+    constexpr int RY = CIN == 32 ? 2 : 4, MTZ = 1;
+    const bool zform = a.ny % (4 * RY) == 0;
+    NNX_CHECK(c, a.nx > 0 && a.ny > 0 && a.nz > 0 && a.nx % 16 == 0 && a.batch >= 1 &&
+                     (zform || (a.ny % TY == 0 && a.nz % TZ == 0)),
+              "conv %d->%d: dims (%d,%d,%d) must be multiples of (16,%d,1) or (16,%d,%d)", CIN, COUT, a.nx, a.ny,
+              a.nz, 4 * RY, TY, TZ);
     if (a.pool || a.coarse)
         NNX_CHECK(c, a.nx % 32 == 0 && a.ny % 2 == 0 && a.nz % 2 == 0, "pooled / upsampled conv needs even dims");
+    if (zform) return launch_conv_z<CIN, COUT, RY, MTZ>(c, a);
     dim3 grid(a.nx / 16, a.ny / TY, (a.nz / TZ) * a.batch);
     k_conv3<CIN, COUT><<<grid, 256, 0, c->stream>>>(a);
     NNX_LAUNCHED(c);

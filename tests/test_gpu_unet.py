@@ -72,8 +72,24 @@ CASES = [(2, 16, False, True), (16, 16, True, False), (16, 32, False, True), (32
 
 @pytest.mark.parametrize("cin,cout,coarse,pool", CASES)
 def test_conv3_forward_matches_torch(ctx, cin, cout, coarse, pool):
+    _conv_case(ctx, cin, cout, coarse, pool, (2, 16, 16, 32))
+
+
+# (2, 22, 112, 128): 112 columns of 16 x 16 rows, enough that the launch
+# splits z into even chunks with a shorter last one (6, 6, 6, 4 at two
+# workgroups per CU); (1, 7, 64, 128) at 16 -> 32: two 16-channel groups per
+# column and an odd last chunk; (1, 8, 24, 32): 24 rows are no multiple of
+# the 16-row column, so the tile-per-workgroup kernel runs
+@pytest.mark.parametrize("cin,cout,coarse,pool,shape", [(16, 16, True, True, (2, 22, 112, 128)),
+                                                        (16, 32, False, False, (1, 7, 64, 128)),
+                                                        (16, 16, True, True, (1, 8, 24, 32))])
+def test_conv3_forward_chunked_and_fallback_shapes(ctx, cin, cout, coarse, pool, shape):
+    _conv_case(ctx, cin, cout, coarse, pool, shape)
+
+
+def _conv_case(ctx, cin, cout, coarse, pool, shape):
     rng = np.random.default_rng(cin * 100 + cout)
-    B, nz, ny, nx = 2, 16, 16, 32
+    B, nz, ny, nx = shape
     x = bf16_bits(rng.normal(0, 1, (B, nz, ny, nx, cin)).astype(np.float32))
     cb = bf16_bits(rng.normal(0, 1, (B, nz // 2, ny // 2, nx // 2, cin)).astype(np.float32)) if coarse else None
     w = (rng.normal(0, 1, (cout, 27, cin)) / np.sqrt(27 * cin)).astype(np.float32)
@@ -82,7 +98,7 @@ def test_conv3_forward_matches_torch(ctx, cin, cout, coarse, pool):
     dc = DA.from_numpy(ctx, cb) if coarse else None
     out = DA.zeros(ctx, (B, nz, ny, nx, cout), np.uint16)
     pl = DA.zeros(ctx, (B, nz // 2, ny // 2, nx // 2, cout), np.uint16) if pool else None
-    say("conv3", cin, cout, "coarse", coarse, "pool", pool)
+    say("conv3", cin, cout, "coarse", coarse, "pool", pool, "shape", shape)
     ctx.call("nnx_conv3_bf16", cin, cout, nx, ny, nz, B, dx.ptr, dc.ptr if dc else None, 0.5,
              dw.ptr, db.ptr, 1, out.ptr, pl.ptr if pl else None)
     xin = to_ncdhw(bits_f64(x), (B, nz, ny, nx, cin))
