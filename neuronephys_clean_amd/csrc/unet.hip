@@ -558,11 +558,14 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
     auto step = [&](auto pc, auto ec, int s) __attribute__((always_inline)) {
         constexpr int P = decltype(pc)::value, E = decltype(ec)::value;
         const uint16_t* pl = s_pl[E] + wave * RY * RS;
-        const int iz = z0 + s + 1;  // input plane of step s + 2
-        if (s + 2 < nsteps) {
-            zpl[E] = iz >= nz;
-            if (iz < nz) load(ec, iz);
-        }
+        // input plane of step s + 2, loaded unconditionally (a plane past the
+        // grid reads the last one and is stored as zeros; one past the chunk
+        // is never stored): a load under a condition makes the compiler's
+        // memory-count wait at the end of the step cover the loads just
+        // issued, not only the set being stored
+        const int iz = z0 + s + 1;
+        zpl[E] = iz >= nz;
+        load(ec, min(iz, nz - 1));
 #pragma unroll
         for (int j = 0; j < KBP; ++j)
 #pragma unroll
@@ -582,17 +585,21 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
                     acc[(P + 1) % 3][r][mt] = mfma16(wa[2][j][mt], bf, acc[(P + 1) % 3][r][mt]);
                 }
             }
-        if (s >= 2) epilogue(std::integral_constant<int, (P + 1) % 3>{}, z0 + s - 2);
-        if (s + 1 < nsteps) store(std::integral_constant<int, E ^ 1>{}, s_pl[E ^ 1], zpl[E ^ 1]);
+        if (s >= 2 && s - 2 < zn) epilogue(std::integral_constant<int, (P + 1) % 3>{}, z0 + s - 2);
+        store(std::integral_constant<int, E ^ 1>{}, s_pl[E ^ 1], zpl[E ^ 1]);
         __syncthreads();
     };
+    // whole groups of six steps (the rotation periods 3 and 2): steps past
+    // the chunk's last only load and stage clamped planes and store nothing,
+    // so no step sits under a condition (a skipped step would merge register
+    // and memory-count states at the loop head)
     for (int s0 = 0; s0 < nsteps; s0 += 6) {
         step(I0{}, I0{}, s0);
-        if (s0 + 1 < nsteps) step(I1{}, I1{}, s0 + 1);
-        if (s0 + 2 < nsteps) step(I2{}, I0{}, s0 + 2);
-        if (s0 + 3 < nsteps) step(I0{}, I1{}, s0 + 3);
-        if (s0 + 4 < nsteps) step(I1{}, I0{}, s0 + 4);
-        if (s0 + 5 < nsteps) step(I2{}, I1{}, s0 + 5);
+        step(I1{}, I1{}, s0 + 1);
+        step(I2{}, I0{}, s0 + 2);
+        step(I0{}, I1{}, s0 + 3);
+        step(I1{}, I0{}, s0 + 4);
+        step(I2{}, I1{}, s0 + 5);
     }
 }
 
@@ -1191,9 +1198,9 @@ template <class K> int64_t resident_wgs(nnx_ctx* c, K k) {
 }
 
 // Chunk length zc (even: chunks start on even planes, as the pooled pairs
-// need) minimising rounds x (zc + 2): a launch of W workgroups per chunk row
+// need) minimising rounds x steps: a launch of W workgroups per chunk row
 // runs ceil(W nch / resident) rounds of workgroups that each walk zc + 2
-// input planes.
+// input planes in whole groups of six steps.
 inline int conv_chunk(int nz, int64_t per_chunk, int64_t resident) {
     int best = nz + (nz & 1);
     double best_t = 1e300;
@@ -1201,7 +1208,7 @@ inline int conv_chunk(int nz, int64_t per_chunk, int64_t resident) {
         int zc = (int)ceil_div(nz, n);
         zc += zc & 1;
         if (ceil_div(nz, zc) != n) continue;
-        const double t = (double)ceil_div(per_chunk * n, resident) * (zc + 2);
+        const double t = (double)ceil_div(per_chunk * n, resident) * (6 * ceil_div(zc + 2, 6));
         if (t < best_t) best_t = t, best = zc;
     }
     return best;
