@@ -763,12 +763,17 @@ __global__ void k_prep_w(const float* __restrict__ w, int cin, int cout, int fli
 }
 
 // per-batch sum of squares of a fp32 field: partials [batch][256]
+// per-block FP64 partial sums of x^2 over batch member blockIdx.y (nper a
+// multiple of 4: the network's grids are)
 __global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ x, int64_t nper, double* __restrict__ part) {
     __shared__ double sh[4];
     const int b = blockIdx.y;
-    const float* xb = x + (size_t)b * nper;
+    const float4* xb = (const float4*)(x + (size_t)b * nper);
     double s = 0;
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nper; i += (int64_t)gridDim.x * 256) s += (double)xb[i] * xb[i];
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nper / 4; i += (int64_t)gridDim.x * 256) {
+        const float4 v = xb[i];
+        s += ((double)v.x * v.x + (double)v.y * v.y) + ((double)v.z * v.z + (double)v.w * v.w);
+    }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -797,13 +802,22 @@ __global__ __launch_bounds__(256) void k_rms_fin(const double* __restrict__ part
 
 // x[v] = (bf16(q), bf16(q feat[v])) with q = r[v] inv[b]: both channels are
 // linear in r, so the bias-free network satisfies N(0) = 0
+// x = [q, q * feat] in bf16, q = r / rms(r), batch member blockIdx.y (nper a
+// multiple of 4)
 __global__ void k_pack(const float* __restrict__ r, const float* __restrict__ feat, const float* __restrict__ inv,
-                       int64_t nper, int batch, uint32_t* __restrict__ x) {
-    const int64_t n = nper * batch;
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int b = (int)(i / nper);
-        const float q = r[i] * inv[b];
-        x[i] = f2bf(q) | (f2bf(feat ? q * feat[i] : 0.f) << 16);
+                       int64_t nper, uint32_t* __restrict__ x) {
+    const int b = blockIdx.y;
+    const float ib = inv[b];
+    const size_t o = (size_t)b * nper;
+    const float4* r4 = (const float4*)(r + o);
+    const float4* f4 = feat ? (const float4*)(feat + o) : nullptr;
+    u32x4_t* x4 = (u32x4_t*)(x + o);
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nper / 4; i += (int64_t)gridDim.x * 256) {
+        const float4 rv = r4[i];
+        const float4 fv = f4 ? f4[i] : float4{0.f, 0.f, 0.f, 0.f};
+        const float q0 = rv.x * ib, q1 = rv.y * ib, q2 = rv.z * ib, q3 = rv.w * ib;
+        x4[i] = u32x4_t{pack2bf(q0, q0 * fv.x), pack2bf(q1, q1 * fv.y), pack2bf(q2, q2 * fv.z),
+                        pack2bf(q3, q3 * fv.w)};
     }
 }
 
@@ -1381,7 +1395,8 @@ nnx_status forward(nnx_ctx* c, nnx_unet* u, int nx, int ny, int nz, int batch, c
     if (u->coarse)
         k_restrict_pack<<<eblocks(n0), 256, 0, c->stream>>>(r, feat, A.inv, nx, ny, nz, batch, (uint32_t*)A.x);
     else
-        k_pack<<<eblocks(n0), 256, 0, c->stream>>>(r, feat, A.inv, nper, batch, (uint32_t*)A.x);
+        k_pack<<<dim3((unsigned)std::min<int64_t>(ceil_div(nper / 4, 256), 2048), batch), 256, 0, c->stream>>>(
+            r, feat, A.inv, nper, (uint32_t*)A.x);
     NNX_LAUNCHED(c);
     ConvArgs a{};
     a.batch = batch;
