@@ -195,18 +195,49 @@ def update_density_reference(data, h, D, k, dt, eb=1e-6, max_iter=20):
 
 
 def run_rk4(p, dt, steps, off, tgt, w, V, m, h, ng, s, iext, cap=1 << 20, voxel_of=None,
-            nvox=0):
+            nvox=0, fire=None):
     """Full RK4 run (state updated in place); returns raster pairs, total
-    count, voxel counts and the wall seconds of the step loop."""
+    count, voxel counts and the wall seconds of the step loop.  fire
+    (optional uint8 array of n): in, the spikes of the step before the first;
+    out, those of the last step (a run continued in the next call)."""
     T = V.dtype
     raster = np.zeros(2 * cap, np.int32)
     vc = np.zeros(max(nvox, 1), np.int32)
     sec = C.c_double()
-    ln = getattr(lib(), "orc_run_rk4_" + _sfx(T))(
-        C.c_int64(V.size), C.byref(p), C.c_double(dt), C.c_int64(steps), _p(off), _p(tgt), _p(w),
-        _p(V), _p(m), _p(h), _p(ng), _p(s), _p(iext), _p(raster), C.c_int64(cap), _p(voxel_of),
-        _p(vc), C.byref(sec))
+    args = [C.c_int64(V.size), C.byref(p), C.c_double(dt), C.c_int64(steps), _p(off), _p(tgt), _p(w),
+            _p(V), _p(m), _p(h), _p(ng), _p(s), _p(iext), _p(raster), C.c_int64(cap), _p(voxel_of),
+            _p(vc), C.byref(sec)]
+    if fire is None:
+        ln = getattr(lib(), "orc_run_rk4_" + _sfx(T))(*args)
+    else:
+        assert fire.dtype == np.uint8 and fire.size == V.size and fire.flags.c_contiguous
+        ln = getattr(lib(), "orc_run_rk4_cont_" + _sfx(T))(*args, _p(fire))
     return raster.reshape(-1, 2)[:min(ln, cap)], ln, vc, sec.value
+
+
+def run_rk4_chunked(p, dt, steps, off, tgt, w, V, m, h, ng, s, iext, cap=1 << 20, voxel_of=None,
+                    nvox=0, chunk=1000, done=None):
+    """run_rk4 over `steps` in calls of `chunk` steps (the spikes of each
+    call's last step carried into the next): one call's raster, count, voxel
+    counts and seconds; done(steps_so_far, spikes_so_far, seconds) after
+    each call."""
+    fire = np.zeros(V.size, np.uint8)
+    parts, total, sec = [], 0, 0.0
+    vc = np.zeros(max(nvox, 1), np.int32)
+    for k0 in range(0, steps, chunk):
+        k = min(chunk, steps - k0)
+        r, ln, c, t = run_rk4(p, dt, k, off, tgt, w, V, m, h, ng, s, iext, cap=max(cap - total, 1),
+                              voxel_of=voxel_of, nvox=nvox, fire=fire)
+        r = r.copy()
+        r[:, 0] += k0
+        parts.append(r)
+        vc += c
+        total += ln
+        sec += t
+        if done:
+            done(k0 + k, total, sec)
+    ras = np.concatenate(parts)[:min(total, cap)] if parts else np.zeros((0, 2), np.int32)
+    return ras, total, vc, sec
 
 
 def run_leapfrog(p, dt, steps, off, tgt, mode, sv, cap=1 << 20, first_step=0):

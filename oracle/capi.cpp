@@ -4,6 +4,7 @@
 // CPU-baseline leg, never by the product.
 #include <omp.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <vector>
@@ -237,9 +238,13 @@ template <class T>
 static int64_t run_rk4(int64_t n, const orc_params* q, double dt, int64_t steps, const int32_t* off,
                        const int32_t* tgt, const float* w, T* V, T* m, T* h, T* ng, T* s,
                        const T* iext, int32_t* raster, int64_t cap, const int32_t* voxel_of,
-                       int32_t* voxel_counts, double* seconds) {
+                       int32_t* voxel_counts, double* seconds, uint8_t* fire_state = nullptr) {
     std::vector<T> g(n, T(0));
+    // fire_state (optional, n bytes): the spikes of the step before the first
+    // (their coupling is the first step's input), and on return those of the
+    // last step, so that a run can continue in the next call
     std::vector<uint8_t> fire(n, 0);
+    if (fire_state) std::copy(fire_state, fire_state + n, fire.begin());
     std::vector<int32_t> list;
     int64_t len = 0;
     auto t0 = std::chrono::steady_clock::now();
@@ -259,10 +264,27 @@ static int64_t run_rk4(int64_t n, const orc_params* q, double dt, int64_t steps,
     }
     if (seconds)
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (fire_state) std::copy(fire.begin(), fire.end(), fire_state);
     return len;
 }
 
 extern "C" {
+int64_t orc_run_rk4_cont_f32(int64_t n, const orc_params* q, double dt, int64_t steps,
+                             const int32_t* off, const int32_t* tgt, const float* w, float* V, float* m,
+                             float* h, float* ng, float* s, const float* iext, int32_t* raster,
+                             int64_t cap, const int32_t* voxel_of, int32_t* voxel_counts,
+                             double* seconds, uint8_t* fire_state) {
+    return run_rk4<float>(n, q, dt, steps, off, tgt, w, V, m, h, ng, s, iext, raster, cap,
+                          voxel_of, voxel_counts, seconds, fire_state);
+}
+int64_t orc_run_rk4_cont_f64(int64_t n, const orc_params* q, double dt, int64_t steps,
+                             const int32_t* off, const int32_t* tgt, const float* w, double* V,
+                             double* m, double* h, double* ng, double* s, const double* iext,
+                             int32_t* raster, int64_t cap, const int32_t* voxel_of,
+                             int32_t* voxel_counts, double* seconds, uint8_t* fire_state) {
+    return run_rk4<double>(n, q, dt, steps, off, tgt, w, V, m, h, ng, s, iext, raster, cap,
+                           voxel_of, voxel_counts, seconds, fire_state);
+}
 int64_t orc_run_rk4_f32(int64_t n, const orc_params* q, double dt, int64_t steps,
                         const int32_t* off, const int32_t* tgt, const float* w, float* V, float* m,
                         float* h, float* ng, float* s, const float* iext, int32_t* raster,

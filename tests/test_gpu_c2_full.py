@@ -4,6 +4,8 @@ edges, release deposit on 256^3): the first 100 ms, 10,000 steps at dt
 from hipGraphs) against the fp32 oracle on the same seeded inputs.  Every
 spike (step, neuron) of the raster, the final state and the per-voxel
 release counts are identical.  About 75 s of oracle time on 16 host cores.
+The 10,000-step oracle runs go in calls of 1,000 steps (the last step's
+spikes carried over), each reported as it ends.
 """
 import numpy as np
 import pytest
@@ -19,7 +21,16 @@ def say(*a):
     print(*a, flush=True)
 
 
-def test_c2_full_scale_100ms_bitexact(ctx):
+def oracle_rk4(capfd, *args, **kw):
+    """The oracle's RK4 run in calls of 1,000 steps, each reported past
+    pytest's output capture as it ends."""
+    def done(k, n, t):
+        with capfd.disabled():
+            print(f"  oracle: {k} steps, {n} spikes, {t:.0f} s", flush=True)
+    return oracle.run_rk4_chunked(*args, chunk=1000, done=done, **kw)
+
+
+def test_c2_full_scale_100ms_bitexact(ctx, capfd):
     net = configs.c2()
     steps, cap = 10000, 16_000_000
     say("c2 full scale", net.n, net.nnz, steps)
@@ -35,9 +46,9 @@ def test_c2_full_scale_100ms_bitexact(ctx):
     V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
     s = np.zeros(net.n, T)
     p = oracle.params(_lib.params_squid().as_dict())
-    ref, rln, rvc, sec = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
-                                        np.asarray(net.iext, T), cap=cap, voxel_of=net.ensure_voxel(),
-                                        nvox=int(np.prod(net.grid)))
+    ref, rln, rvc, sec = oracle_rk4(capfd, p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
+                                    np.asarray(net.iext, T), cap=cap, voxel_of=net.ensure_voxel(),
+                                    nvox=int(np.prod(net.grid)))
     say("oracle spikes", rln, f"in {sec:.1f} s")
     assert ln == rln and ln > 100_000
     assert np.array_equal(ras, ref)
@@ -110,7 +121,7 @@ def test_c2_full_scale_fp64_equals_oracle_and_fp32_tolerance(ctx):
     assert max(d["m"], d["h"], d["ng"]) <= 1e-5
 
 
-def test_c2_full_scale_fp64_100ms_equals_fp64_oracle_and_fp32_agreement(ctx):
+def test_c2_full_scale_fp64_100ms_equals_fp64_oracle_and_fp32_agreement(ctx, capfd):
     """The configuration's rule at full scale: the fp64 device run of C2's
     first 100 ms (10,000 steps, unfused event coupling, graphs of 100)
     produces exactly the FP64 oracle's raster (every step and neuron) and
@@ -138,8 +149,8 @@ def test_c2_full_scale_fp64_100ms_equals_fp64_oracle_and_fp32_agreement(ctx):
     V, m, h, ng = (np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0))
     s = np.zeros(net.n, T)
     p = oracle.params(_lib.params_squid().as_dict())
-    ref, rln, _, sec = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
-                                      np.asarray(net.iext, T), cap=cap)
+    ref, rln, _, sec = oracle_rk4(capfd, p, net.dt, steps, net.off, net.tgt, net.w, V, m, h, ng, s,
+                                  np.asarray(net.iext, T), cap=cap)
     say(f"FP64 oracle: {rln} spikes in {sec:.1f} s")
     ras64, ln64 = res[np.float64][2]
     assert ln64 == rln and rln > 1_000_000

@@ -430,3 +430,33 @@ def test_update_density_reference_early_return_keeps_data():
     data = np.random.default_rng(8).random((6, 6, 6)).astype(np.float32)
     out, it, rr = oracle.update_density_reference(data, 1.0, 0.0, 0.0, 0.4)
     assert it == 0 and rr < 1e-6 and np.array_equal(out, data)
+
+
+@pytest.mark.parametrize("T", [np.float32, np.float64])
+def test_rk4_run_in_chunks_equals_one_run(T):
+    # a spiking network run in calls whose first ends on a step with spikes
+    # (the last step's spikes carried into the next call) gives one call's
+    # raster and state; dropping the carry changes the result
+    from neuronephys_clean_amd import configs
+    net = configs.c1()
+    p = oracle.params(_lib.params_squid().as_dict())
+    steps, iext = 400, np.asarray(net.iext, T)
+
+    def start():
+        st = [np.asarray(a, T).copy() for a in (net.V0, net.m0, net.h0, net.n0)]
+        return st + [np.zeros(net.n, T)]
+    a, b = start(), start()
+    ras1, ln1, _, _ = oracle.run_rk4(p, net.dt, steps, net.off, net.tgt, net.w, *a, iext)
+    assert ln1 > 100
+    chunk = int(ras1[0, 0]) + 1  # the first call ends on the first step with spikes
+    assert chunk < steps // 2
+    seen = []
+    ras2, ln2, _, _ = oracle.run_rk4_chunked(p, net.dt, steps, net.off, net.tgt, net.w, *b, iext, chunk=chunk,
+                                          done=lambda k, n, t: seen.append((k, n)))
+    assert ln2 == ln1 and np.array_equal(ras1, ras2)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert [k for k, _ in seen] == list(range(chunk, steps, chunk)) + [steps] and seen[-1][1] == ln1
+    c = start()
+    oracle.run_rk4(p, net.dt, chunk, net.off, net.tgt, net.w, *c, iext)
+    r3, ln3, _, _ = oracle.run_rk4(p, net.dt, steps - chunk, net.off, net.tgt, net.w, *c, iext)
+    assert not np.array_equal(c[0], a[0])
