@@ -220,6 +220,11 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 // them; with 2 they were issued one plane earlier, which hides a memory
 // latency per plane (a workgroup marches its tile alone on its CU).
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
+// prefetch forms and tile heights the solver launches (measured at 256^3):
+// the apply and the variable-coefficient first half take the deferred form;
+// the constant-coefficient iteration stays at PF 2 with 16-row tiles (its
+// first half gained nothing from the deferred form, 8-row tiles were slower)
+constexpr int kPFa = 3, kPFv = 3, kPFc = 2, kTYc = kTY4;
 
 // Tile of workgroup b in round `round` of a tiled launch of nb workgroups.
 // Workgroups are dealt to the 8 XCDs round-robin (b % 8), each XCD with its
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
                                                     const float* __restrict__ pold, float* __restrict__ pnew,
                                                     float* __restrict__ dst, int xzero) {
     static_assert(MODE == 0 || MODE == 2 || MODE == 3, "mode");
-    static_assert(PF == 1 || PF == 2, "prefetch depth");
+    static_assert(PF == 1 || PF == 2 || PF == 3, "prefetch form");
     static_assert(!VAR || MODE == 2, "variable coefficients: the first half only (k_cg_rupd is the second)");
     __shared__ float t[2][TY + 2][kRow4];
     float beta = 0.0f, di = 1.0f, alpha = 0.0f;
@@ -324,9 +329,9 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
             return use_pold ? d.r[e] * de + beta * pold[e] : d.r[e] * de;
         };
         // the own column: the tile value, and p_old and x (MODE 2) or r
-        // (MODE 3) of the same point.  MODE 2 with PF == 2 reads x only in
+        // (MODE 3) of the same point.  MODE 2 with PF >= 2 reads x only in
         // its own plane's step (it is not a neighbour value: registers).
-        constexpr bool late_x = MODE == 2 && PF == 2;
+        constexpr bool late_x = MODE == 2 && PF >= 2;
         auto own4 = [&](bool ok, int64_t e, float4& po, float4& xr) -> float4 {
             po = z4;
             xr = z4;
@@ -337,6 +342,83 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
             }
             if (MODE == 3) xr = *reinterpret_cast<const float4*>(d.r + e);
             return *reinterpret_cast<const float4*>(src + e);
+        };
+        // PF == 3: a plane's raw values loaded two planes ahead at clamped
+        // addresses (a point outside the grid reads element 0), with no
+        // arithmetic and no condition at the load; the direction, the
+        // Jacobi scaling and the zero outside the grid are applied when the
+        // plane is used, one step later - the same operations as dir4 / val4
+        // / val1, so the same bits - so that no load is waited for in the
+        // step that issues it.
+        struct RawO { float4 a, po, dd, xr; };
+        struct RawR { float4 a, po, dd; };
+        struct RawC { float a, po, dd; };
+        const float* __restrict__ av = MODE == 2 ? d.r : src;
+        auto ld_own = [&](bool ok, int64_t e, RawO& R) {
+            const int64_t ec = ok ? e : 0;
+            R.a = *reinterpret_cast<const float4*>(av + ec);
+            if (MODE == 2 && VAR && dvv) R.dd = *reinterpret_cast<const float4*>(dvv + ec);
+            if (MODE == 2 && use_pold) R.po = *reinterpret_cast<const float4*>(pold + ec);
+            if (MODE == 3) R.xr = *reinterpret_cast<const float4*>(d.r + ec);
+        };
+        auto ld_row = [&](bool ok, int64_t e, RawR& R) {
+            const int64_t ec = ok ? e : 0;
+            R.a = *reinterpret_cast<const float4*>(av + ec);
+            if (MODE == 2 && VAR && dvv) R.dd = *reinterpret_cast<const float4*>(dvv + ec);
+            if (MODE == 2 && use_pold) R.po = *reinterpret_cast<const float4*>(pold + ec);
+        };
+        auto ld_col = [&](bool ok, int64_t e, RawC& R) {
+            const int64_t ec = ok ? e : 0;
+            R.a = av[ec];
+            if (MODE == 2 && VAR && dvv) R.dd = dvv[ec];
+            if (MODE == 2 && use_pold) R.po = pold[ec];
+        };
+        auto dir_raw = [&](const float4& a, const float4& ddr, const float4& po) -> float4 {
+            const float4 dd = VAR && dvv ? ddr : make_float4(di, di, di, di);
+            if (!use_pold) return make_float4(a.x * dd.x, a.y * dd.y, a.z * dd.z, a.w * dd.w);
+            return make_float4(a.x * dd.x + beta * po.x, a.y * dd.y + beta * po.y, a.z * dd.z + beta * po.z,
+                               a.w * dd.w + beta * po.w);
+        };
+        auto fin_own = [&](bool ok, const RawO& R, float4& po, float4& xr) -> float4 {
+            po = z4;
+            xr = z4;
+            if (!ok) return z4;
+            if (MODE == 2) {
+                if (use_pold) po = R.po;
+                return dir_raw(R.a, R.dd, R.po);
+            }
+            if (MODE == 3) xr = R.xr;
+            return R.a;
+        };
+        auto fin_row = [&](bool ok, const RawR& R) -> float4 {
+            if (!ok) return z4;
+            if (MODE != 2) return R.a;
+            return dir_raw(R.a, R.dd, R.po);
+        };
+        auto fin_col = [&](bool ok, const RawC& R) -> float {
+            if (!ok) return 0.0f;
+            if (MODE != 2) return R.a;
+            const float de = VAR && dvv ? R.dd : di;
+            return use_pold ? R.a * de + beta * R.po : R.a * de;
+        };
+        RawO rwo;
+        RawR rwr;
+        RawC rwc;
+        bool rok_o = false, rok_r = false, rok_c = false;
+        // VAR with PF == 3: the next plane's face coefficients of the own
+        // voxels, loaded a plane ahead (element 0 for a plane past the grid);
+        // the z face below a plane is the one above the plane before it
+        float nkx[5] = {0, 0, 0, 0, 0};
+        float4 nkym = z4, nkyp = z4, nkzp = z4, ckzm = z4;
+        auto ld_faces = [&](int kz) {
+            const bool ok = own && kz < nz;
+            const int64_t ex = ok ? xo + (int64_t)(nx + 1) * (j + (int64_t)ny * kz) : 0;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) nkx[q] = g.kx[ex + q];
+            const int64_t fy = ok ? xo + (int64_t)nx * (j + (int64_t)(ny + 1) * kz) : 0;
+            nkym = *reinterpret_cast<const float4*>(g.ky + fy);
+            nkyp = *reinterpret_cast<const float4*>(g.ky + fy + nx);
+            nkzp = *reinterpret_cast<const float4*>(g.kz + (ok ? ocol + sxy * (kz + 1) : 0));
         };
         float4 poc, xrc;
         float4 pm = val4(own && k0 > 0, ocol + sxy * (k0 - 1));
@@ -353,6 +435,17 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
             hr1 = val4(hr_ok && ok1, rcol + sxy * (k0 + 1));
             hc1 = val1(hc_ok && ok1, ccol + sxy * (k0 + 1));
         }
+        if (PF == 3) {
+            const bool ok1 = k0 + 1 < nz;
+            rok_o = own && ok1, rok_r = hr_ok && ok1, rok_c = hc_ok && ok1;
+            ld_own(rok_o, ocol + sxy * (k0 + 1), rwo);
+            ld_row(rok_r, rcol + sxy * (k0 + 1), rwr);
+            ld_col(rok_c, ccol + sxy * (k0 + 1), rwc);
+            if (VAR) {
+                ckzm = *reinterpret_cast<const float4*>(g.kz + (own ? ocol + sxy * k0 : 0));
+                ld_faces(k0);
+            }
+        }
         auto plane = [&](int kz) {
             const int b = (kz - k0) & 1;
             float4 pp, pop, xrp, hr;
@@ -362,19 +455,39 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
                 pp = own4(own && nxt, ocol + sxy * (kz + 1), pop, xrp);
                 hr = val4(hr_ok && nxt, rcol + sxy * (kz + 1));
                 hc = val1(hc_ok && nxt, ccol + sxy * (kz + 1));
-            } else {
+            } else if (PF == 2) {
                 pp = pp1, pop = pop1, xrp = xrp1, hr = hr1, hc = hc1;
                 const bool n2 = kz + 2 < nz && kz + 2 <= kend;
                 pp1 = own4(own && n2, ocol + sxy * (kz + 2), pop1, xrp1);
                 hr1 = val4(hr_ok && n2, rcol + sxy * (kz + 2));
                 hc1 = val1(hc_ok && n2, ccol + sxy * (kz + 2));
+            } else {
+                // plane kz + 1, loaded a step ago; then plane kz + 2's loads,
+                // into the same registers, after this plane's x (read first,
+                // so that its wait after the barrier leaves them in flight)
+                pp = fin_own(rok_o, rwo, pop, xrp);
+                hr = fin_row(rok_r, rwr);
+                hc = fin_col(rok_c, rwc);
+                if (late_x && read_x) xrc = *reinterpret_cast<const float4*>(dst + (own ? ocol + sxy * kz : 0));
+                const bool n2 = kz + 2 < nz && kz + 2 <= kend;
+                rok_o = own && n2, rok_r = hr_ok && n2, rok_c = hc_ok && n2;
+                ld_own(rok_o, ocol + sxy * (kz + 2), rwo);
+                ld_row(rok_r, rcol + sxy * (kz + 2), rwr);
+                ld_col(rok_c, ccol + sxy * (kz + 2), rwc);
             }
-            if (late_x && own && read_x) xrc = *reinterpret_cast<const float4*>(dst + ocol + sxy * kz);
             // VAR: the own voxels' face coefficients (kx: x-faces i .. i+4 of
             // the row; ky: rows j and j+1; kz: planes kz and kz+1)
             float kxf[5] = {0, 0, 0, 0, 0};
             float4 kym = z4, kyp = z4, kzm = z4, kzp = z4;
-            if (VAR && own) {
+            if (VAR && PF == 3) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) kxf[q] = nkx[q];
+                kym = nkym, kyp = nkyp, kzm = ckzm, kzp = nkzp;
+                ckzm = kzp;
+                ld_faces(kz + 1);
+            }
+            if (PF == 2 && late_x && own && read_x) xrc = *reinterpret_cast<const float4*>(dst + ocol + sxy * kz);
+            if (VAR && PF != 3 && own) {
                 const float* kxr = g.kx + xo + (int64_t)(nx + 1) * (j + (int64_t)ny * kz);
 #pragma unroll
                 for (int q = 0; q < 5; ++q) kxf[q] = kxr[q];
@@ -473,8 +586,15 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
         // and the next plane's loads overlap the current one (a fifth of the
         // bandwidth otherwise)
         if (kend - k0 == kZ4) {
+            if (PF == 3) {
+                // the deferred form carries its loads across planes itself;
+                // unrolling further only adds registers
+#pragma unroll 2
+                for (int q = 0; q < kZ4; ++q) plane(k0 + q);
+            } else {
 #pragma unroll
-            for (int q = 0; q < kZ4; ++q) plane(k0 + q);
+                for (int q = 0; q < kZ4; ++q) plane(k0 + q);
+            }
         } else {
             for (int kz = k0; kz < kend; ++kz) plane(kz);
         }
@@ -979,19 +1099,20 @@ static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float*
     float* pold = (k & 1) ? h.pb : h.pa;
     float* pnew = (k & 1) ? h.pa : h.pb;
     if (fast == kFastVar) {
-        k_stencil7x4<2, 2, true, kTYv><<<nb4, 64 * kTYv, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
+        k_stencil7x4<2, kPFv, true, kTYv><<<nb4, 64 * kTYv, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
         k_cg_rupd<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k);
         return;
     }
     if (fast && gk.block) {
-        k_stencil7x4<2, 2, false, kTY4, true><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
-        k_stencil7x4<3, 2, false, kTY4, true><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(
+        k_stencil7x4<2, kPFc, false, kTYc, true><<<nb4, 64 * kTYc, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x,
+                                                                                xzero);
+        k_stencil7x4<3, kPFc, false, kTYc, true><<<std::min(nb4, kRedBlocks), 64 * kTYc, 0, c->stream>>>(
             gk, h.d, k, pnew, nullptr, nullptr, nullptr, 0);
         return;
     }
     if (fast) {
-        k_stencil7x4<2, 2><<<nb4, kT4, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
-        k_stencil7x4<3, 2><<<std::min(nb4, kRedBlocks), kT4, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
+        k_stencil7x4<2, kPFc, false, kTYc><<<nb4, 64 * kTYc, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
+        k_stencil7x4<3, kPFc, false, kTYc><<<std::min(nb4, kRedBlocks), 64 * kTYc, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
                                                                          0);
         return;
     }
@@ -1011,7 +1132,7 @@ static int cg_fast(const GridK& gk, const float* x) {
     return 0;
 }
 // blocks of the first-half launch of a tiled iteration
-static int fast_blocks(const GridK& gk, int fast) { return stencil4_blocks(gk, fast == kFastVar ? kTYv : kTY4); }
+static int fast_blocks(const GridK& gk, int fast) { return stencil4_blocks(gk, fast == kFastVar ? kTYv : kTYc); }
 
 // the pending x update of the fast path (nothing otherwise)
 static void cg_finish_launch(nnx_ctx* c, const GridK& gk, CgHost& h, float* x, int xzero = 0,
@@ -1454,10 +1575,10 @@ nnx_status nnx_stencil_apply(nnx_ctx* c, const nnx_grid* g, const float* x, floa
         ((uintptr_t)y % 16) == 0) {
         CgDev none{};
         if (gk.block)
-            k_stencil7x4<0, 2, false, kTY4, true>
+            k_stencil7x4<0, kPFa, false, kTY4, true>
                 <<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
         else
-            k_stencil7x4<0, 2><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
+            k_stencil7x4<0, kPFa><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
     } else {
         k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
     }
