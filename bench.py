@@ -730,6 +730,11 @@ def tune_fused(ctx, sim, chunk):
 
 C2_STEPS = 10000      # the configuration's 100 ms at dt 0.01 ms
 FIXED_RELEASE_STEPS = 1000
+# CG iterations enqueued for the first refinement step of each in-step solve
+# (half as many for the second): C2's solves take at most 25 in the first
+# and 34 in all (measured over the configuration's 100 solves); iterations
+# past convergence are launches that return at once
+MAX_INNER = 32
 
 
 def cpu_model():
@@ -846,7 +851,7 @@ def main():
     sim = Simulation(net, np.float32, couple=args.couple, record_cap=1 << 24, deposit=True,
                      ctx=ctx, device_csc=True)
     every = args.para_every
-    sim.set_paracrine(max(every, 1), rtol=1e-8, inner_rtol=1e-6, max_inner=40, max_outer=2)
+    sim.set_paracrine(max(every, 1), rtol=1e-8, inner_rtol=1e-6, max_inner=MAX_INNER, max_outer=2)
     sim.synchronize()
     log(f"bench: uploaded + device CSC in {time.time() - t0:.1f}s")
 
@@ -935,8 +940,9 @@ def main():
     n_solves = int(pst["solves"])
     paracrine = {"every": every, "solves": n_solves, "grid": f"{net.grid[0]}^3", "c": net.c,
                  "method": "Jacobi-CG (fp32 inner, FP64 dots) with FP64 iterative refinement, warm start from the "
-                           "previous field, captured in the step graphs (2 refinement steps of at most 40 and 20 "
-                           "inner iterations enqueued per solve, kernels after convergence return at once; "
+                           f"previous field, captured in the step graphs (2 refinement steps of at most {MAX_INNER} "
+                           f"and {MAX_INNER // 2} inner iterations enqueued per solve, kernels after convergence "
+                           "return at once; "
                            "constant 7-point grid: tiled FP64 residual, two tiled launches per inner iteration)",
                  "rtol_true_fp64": 1e-8, "max_relres": pst["max_relres"], "unconverged": int(pst["unconverged"]),
                  "inner_iters_per_solve": pst["inner_iters"] / max(n_solves, 1),
