@@ -328,24 +328,30 @@ template <int CIN> struct ZK {
     static constexpr int NC8 = WIDE ? CIN / 8 : 1;
 };
 
-template <int CIN, int COUT, int RY, int MT, bool CO>
+template <int CIN, int COUT, int RY, int MT, bool CO, int WX = 1>
 __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
     static_assert(!CO || ZK<CIN>::WIDE, "the 2-channel input takes no upsampled term");
     static_assert(RY % 2 == 0, "row pairs for the pooled output");
     // MT m-tiles (16 output channels each) per workgroup: channel group
     // blockIdx.y % NG of NG
-    constexpr int KBP = ZK<CIN>::KBP, NC8 = ZK<CIN>::NC8, NG = COUT / (16 * MT), TY = 4 * RY, HR = TY + 2;
+    // WX of the four waves side by side in x: a column is 16 WX wide and
+    // RY (4 / WX) rows high (the 2-channel input takes WX = 4, so that its
+    // staged rows of 66 voxels x 4 bytes fill whole cache lines)
+    static_assert(WX == 1 || WX == 2 || WX == 4, "waves along x");
+    constexpr int TX = 16 * WX, HX = TX + 2;
+    constexpr int KBP = ZK<CIN>::KBP, NC8 = ZK<CIN>::NC8, NG = COUT / (16 * MT), TY = (4 / WX) * RY, HR = TY + 2;
     constexpr int KP = kpad(CIN);
     constexpr bool WIDE = ZK<CIN>::WIDE;
-    constexpr int RS = WIDE ? NC8 * 18 * 8 : 18 * 2;     // bf16 elements per staged row
+    constexpr int RS = WIDE ? NC8 * HX * 8 : HX * 2;     // bf16 elements per staged row
     constexpr int PE = HR * RS;                          // bf16 elements per staged plane
-    constexpr int NCH = WIDE ? HR * 18 * NC8 : HR * 18;  // load chunks (16 B or 4 B) per plane
+    constexpr int NCH = WIDE ? HR * HX * NC8 : HR * HX;  // load chunks (16 B or 4 B) per plane
     constexpr int IT = (NCH + 255) / 256;
     using SV = typename std::conditional<WIDE, u32x4_t, uint32_t>::type;
     __shared__ __attribute__((aligned(16))) uint16_t s_pl[2][PE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane >> 4, xl = lane & 15;
-    const int nx = a.nx, ny = a.ny, nz = a.nz, ntx = nx / 16;
-    const int x0 = (blockIdx.x % ntx) * 16, y0 = (blockIdx.x / ntx) * TY;
+    const int nx = a.nx, ny = a.ny, nz = a.nz, ntx = nx / TX;
+    const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
+    const int xw = 16 * (wave % WX), wy = RY * (wave / WX);  // this wave's 16 x RY rows in the column
     const int b = blockIdx.z, z0 = (blockIdx.y / NG) * zc, cb = 16 * MT * (blockIdx.y % NG);
     const int zn = min(zc, nz - z0), nsteps = zn + 2;
     const u32x4_t zero4 = u32x4_t{0u, 0u, 0u, 0u};
@@ -380,10 +386,10 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
     for (int j = 0; j < NF; ++j) {
         if constexpr (WIDE) {
             const int s = CIN == 32 ? j : min(2 * j + (g >> 1), 8), c8 = CIN == 32 ? g : (g & 1);
-            foff[j] = (s / 3) * RS + c8 * 18 * 8 + (s % 3 + xl) * 8;
+            foff[j] = (s / 3) * RS + c8 * HX * 8 + (s % 3 + xw + xl) * 8;
         } else {
             const int s = min(4 * g + j, 8);
-            foff[j] = (s / 3) * RS + (s % 3 + xl) * 2;
+            foff[j] = (s / 3) * RS + (s % 3 + xw + xl) * 2;
         }
     }
     // staging: chunk i of a plane is (row, x, 8-channel group) in memory
@@ -398,13 +404,13 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
     for (int u = 0; u < IT; ++u) {
         int i = tid + 256 * u;
         i = i < NCH ? i : NCH - 1;
-        const int c8 = i % NC8, vx = (i / NC8) % 18, row = i / (NC8 * 18);
+        const int c8 = i % NC8, vx = (i / NC8) % HX, row = i / (NC8 * HX);
         const int gx = x0 - 1 + vx, gy = y0 - 1 + row;
         if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) inside |= 1u << u;
         const int cx = min(max(gx, 0), nx - 1), cy = min(max(gy, 0), ny - 1);
         gofs[u] = (cy * nx + cx) * CIN + 8 * c8;
         cofs[u] = ((cy >> 1) * (nx >> 1) + (cx >> 1)) * CIN + 8 * c8;
-        lofs[u] = WIDE ? ((row * NC8 + c8) * 18 + vx) * 8 : (row * 18 + vx) * 2;
+        lofs[u] = WIDE ? ((row * NC8 + c8) * HX + vx) * 8 : (row * HX + vx) * 2;
     }
     // two register sets: plane s + 2 is loaded during step s while plane
     // s + 1, loaded during step s - 1, is stored to LDS at its end
@@ -453,7 +459,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
             head_r[mt][q] = a.head ? a.head[co] : 0.f;
         }
     const float hb = a.head ? a.head[COUT] : 0.f, hs = a.hscale ? a.hscale[b] : 1.f;
-    const int yw = y0 + wave * RY;
+    const int yw = y0 + wy;
     f32x4_t pacc[RY / 2][MT];
     f32x4_t acc[3][RY][MT];
 #pragma unroll
@@ -466,7 +472,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
     // would live in scratch memory)
     // per-lane offsets within an output plane (voxels); the activation's
     // slope below zero (1: identity); lane-exchange addresses
-    const int lvox = yw * nx + x0 + xl;
+    const int lvox = yw * nx + x0 + xw + xl;
     const float leak = a.relu ? kLeak : 1.f;
     const int sw16 = (lane ^ 16) << 2, sw32 = (lane ^ 32) << 2;
     auto xch = [](int addr, float v) __attribute__((always_inline)) {
@@ -529,7 +535,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
                             o[0] = pack2bf(s[0], s[1]);
                             o[1] = pack2bf(s[2], s[3]);
                             const int64_t pv =
-                                (((int64_t)b * pz + (z >> 1)) * py + ((yw >> 1) + j)) * px + ((x0 + xl) >> 1);
+                                (((int64_t)b * pz + (z >> 1)) * py + ((yw >> 1) + j)) * px + ((x0 + xw + xl) >> 1);
                             *(u32x2_t*)&a.pool[pv * COUT + cb + 16 * mt + 4 * g] = o;
                         }
                     }
@@ -557,7 +563,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
     // land in sets that are restarted before use or never stored.
     auto step = [&](auto pc, auto ec, int s) __attribute__((always_inline)) {
         constexpr int P = decltype(pc)::value, E = decltype(ec)::value;
-        const uint16_t* pl = s_pl[E] + wave * RY * RS;
+        const uint16_t* pl = s_pl[E] + wy * RS;
         // input plane of step s + 2, loaded unconditionally (a plane past the
         // grid reads the last one and is stored as zeros; one past the chunk
         // is never stored): a load under a condition makes the compiler's
@@ -585,8 +591,11 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
                     acc[(P + 1) % 3][r][mt] = mfma16(wa[2][j][mt], bf, acc[(P + 1) % 3][r][mt]);
                 }
             }
-        if (s >= 2 && s - 2 < zn) epilogue(std::integral_constant<int, (P + 1) % 3>{}, z0 + s - 2);
+        // the staging store before the epilogue: its memory-count wait then
+        // follows this step's loads and the previous step's output stores,
+        // issued a whole MFMA phase earlier, not the stores just issued
         store(std::integral_constant<int, E ^ 1>{}, s_pl[E ^ 1], zpl[E ^ 1]);
+        if (s >= 2 && s - 2 < zn) epilogue(std::integral_constant<int, (P + 1) % 3>{}, z0 + s - 2);
         __syncthreads();
     };
     // whole groups of six steps (the rotation periods 3 and 2): steps past
@@ -1228,22 +1237,23 @@ inline int conv_chunk(int nz, int64_t per_chunk, int64_t resident) {
     return best;
 }
 
-template <int CIN, int COUT, int RY, int MT>
+template <int CIN, int COUT, int RY, int MT, int WX>
 nnx_status launch_conv_z(nnx_ctx* c, const ConvArgs& a) {
-    constexpr int TY = 4 * RY, NG = COUT / (16 * MT);
+    constexpr int TX = 16 * WX, TY = (4 / WX) * RY, NG = COUT / (16 * MT);
     NNX_CHECK(c, NG == 1 || !a.head, "the 1x1 head needs every channel in one workgroup");
     const bool co = ZK<CIN>::WIDE && a.coarse;
     // residency of each form, measured once
-    static const int64_t res0 = resident_wgs(c, k_conv3z<CIN, COUT, RY, MT, false>);
-    static const int64_t res1 = ZK<CIN>::WIDE ? resident_wgs(c, k_conv3z<CIN, COUT, RY, MT, ZK<CIN>::WIDE>) : res0;
-    const int64_t cols = (int64_t)(a.nx / 16) * (a.ny / TY);
+    static const int64_t res0 = resident_wgs(c, k_conv3z<CIN, COUT, RY, MT, false, WX>);
+    static const int64_t res1 =
+        ZK<CIN>::WIDE ? resident_wgs(c, k_conv3z<CIN, COUT, RY, MT, ZK<CIN>::WIDE, WX>) : res0;
+    const int64_t cols = (int64_t)(a.nx / TX) * (a.ny / TY);
     const int zc = conv_chunk(a.nz, cols * a.batch * NG, co ? res1 : res0);
     const int nch = (int)ceil_div(a.nz, zc);
     const dim3 grid((unsigned)cols, (unsigned)(nch * NG), (unsigned)a.batch);
     if (co)
-        k_conv3z<CIN, COUT, RY, MT, ZK<CIN>::WIDE><<<grid, 256, 0, c->stream>>>(a, zc);
+        k_conv3z<CIN, COUT, RY, MT, ZK<CIN>::WIDE, WX><<<grid, 256, 0, c->stream>>>(a, zc);
     else
-        k_conv3z<CIN, COUT, RY, MT, false><<<grid, 256, 0, c->stream>>>(a, zc);
+        k_conv3z<CIN, COUT, RY, MT, false, WX><<<grid, 256, 0, c->stream>>>(a, zc);
     NNX_LAUNCHED(c);
     return NNX_OK;
 }
@@ -1255,14 +1265,20 @@ nnx_status launch_conv(nnx_ctx* c, const ConvArgs& a) {
     // the m-tiles per workgroup keep weights, accumulators and the staged
     // plane within 256 registers), else one 16 x TY x TZ tile per workgroup
     constexpr int RY = CIN == 32 ? 2 : 4, MTZ = 1;
-    const bool zform = a.ny % (4 * RY) == 0;
+    // the 2-channel input in 64-voxel-wide, 4-row columns where they tile
+    // the grid (its rows are 4 bytes a voxel: 16-voxel columns would read
+    // each input cache line three to four times)
+    constexpr int WXW = CIN == 2 ? 4 : 1;
+    const bool wide = WXW > 1 && a.nx % (16 * WXW) == 0 && a.ny % ((4 / WXW) * RY) == 0;
+    const bool zform = wide || a.ny % (4 * RY) == 0;
     NNX_CHECK(c, a.nx > 0 && a.ny > 0 && a.nz > 0 && a.nx % 16 == 0 && a.batch >= 1 &&
                      (zform || (a.ny % TY == 0 && a.nz % TZ == 0)),
               "conv %d->%d: dims (%d,%d,%d) must be multiples of (16,%d,1) or (16,%d,%d)", CIN, COUT, a.nx, a.ny,
               a.nz, 4 * RY, TY, TZ);
     if (a.pool || a.coarse)
         NNX_CHECK(c, a.nx % 32 == 0 && a.ny % 2 == 0 && a.nz % 2 == 0, "pooled / upsampled conv needs even dims");
-    if (zform) return launch_conv_z<CIN, COUT, RY, MTZ>(c, a);
+    if (wide) return launch_conv_z<CIN, COUT, RY, MTZ, WXW>(c, a);
+    if (zform) return launch_conv_z<CIN, COUT, RY, MTZ, 1>(c, a);
     dim3 grid(a.nx / 16, a.ny / TY, (a.nz / TZ) * a.batch);
     k_conv3<CIN, COUT><<<grid, 256, 0, c->stream>>>(a);
     NNX_LAUNCHED(c);

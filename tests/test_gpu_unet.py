@@ -79,10 +79,14 @@ def test_conv3_forward_matches_torch(ctx, cin, cout, coarse, pool):
 # splits z into even chunks with a shorter last one (6, 6, 6, 4 at two
 # workgroups per CU); (1, 7, 64, 128) at 16 -> 32: two 16-channel groups per
 # column and an odd last chunk; (1, 8, 24, 32): 24 rows are no multiple of
-# the 16-row column, so the tile-per-workgroup kernel runs
+# the 16-row column, so the tile-per-workgroup kernel runs; the 2-channel input
+# at (2, 10, 8, 128) and (1, 6, 4, 64): 64-voxel-wide columns of 4 rows (two
+# and one of them across x)
 @pytest.mark.parametrize("cin,cout,coarse,pool,shape", [(16, 16, True, True, (2, 22, 112, 128)),
                                                         (16, 32, False, False, (1, 7, 64, 128)),
-                                                        (16, 16, True, True, (1, 8, 24, 32))])
+                                                        (16, 16, True, True, (1, 8, 24, 32)),
+                                                        (2, 16, False, True, (2, 10, 8, 128)),
+                                                        (2, 16, False, True, (1, 6, 4, 64))])
 def test_conv3_forward_chunked_and_fallback_shapes(ctx, cin, cout, coarse, pool, shape):
     _conv_case(ctx, cin, cout, coarse, pool, shape)
 
