@@ -308,11 +308,11 @@ __global__ __launch_bounds__(256) void k_conv3(ConvArgs a) {
 // ---------------------------------------------------------------------------
 // forward convolution, z-marching form
 // ---------------------------------------------------------------------------
-// A workgroup owns a 16 x 4RY column of output rows over a chunk of zc
-// planes and walks up z one input plane at a time.  The plane, with its
-// one-voxel x/y halo and the upsampled coarse term added, is staged in LDS
-// in two buffers: the next plane's loads are in flight while this one is
-// read.  K is ordered plane by plane (a k-block holds taps (dy, dx) of one
+// A workgroup owns a (16 WX) x (RY 4 / WX) column of output rows over a
+// chunk of zc planes and walks up z one input plane at a time.  The plane,
+// with its one-voxel x/y halo and the upsampled coarse term added, is staged
+// in LDS in two buffers, its loads issued two planes ahead into one of two
+// register sets.  K is ordered plane by plane (a k-block holds taps (dy, dx) of one
 // input plane x channels), so the B fragment read for an output row serves
 // the three output planes that read this input plane (dz = 0, 1, 2), and
 // three accumulator sets rotate as output planes complete.  The weights
@@ -333,8 +333,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
     static_assert(!CO || ZK<CIN>::WIDE, "the 2-channel input takes no upsampled term");
     static_assert(RY % 2 == 0, "row pairs for the pooled output");
     // MT m-tiles (16 output channels each) per workgroup: channel group
-    // blockIdx.y % NG of NG
-    // WX of the four waves side by side in x: a column is 16 WX wide and
+    // blockIdx.y % NG of NG.  WX of the four waves side by side in x: a column is 16 WX wide and
     // RY (4 / WX) rows high (the 2-channel input takes WX = 4, so that its
     // staged rows of 66 voxels x 4 bytes fill whole cache lines)
     static_assert(WX == 1 || WX == 2 || WX == 4, "waves along x");
@@ -379,7 +378,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
                 }
             }
     // B fragment offsets in a staged plane (row 0 = y0 - 1, column 0 = x0 - 1):
-    // lane (g, xl) reads tap (dy, dx) of its k-slice at voxel x0 + xl
+    // lane (g, xl) reads tap (dy, dx) of its k-slice at voxel x0 + xw + xl
     constexpr int NF = WIDE ? KBP : 4;
     int foff[NF];
 #pragma unroll
@@ -447,7 +446,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
             }
         }
     };
-    // epilogue constants: lane holds voxel x0 + xl, channels cb + 16 mt + 4 g + q
+    // epilogue constants: lane holds voxel x0 + xw + xl, channels cb + 16 mt + 4 g + q
     // (the 1x1 head needs every channel in one workgroup: NG = 1)
     float bias_r[MT][4], head_r[MT][4];
 #pragma unroll
