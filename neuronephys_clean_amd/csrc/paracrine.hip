@@ -218,7 +218,11 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 // PF: planes loaded ahead of the one being computed.  With 1, plane k+1's
 // loads are issued at the start of plane k and its z+1 neighbours wait for
 // them; with 2 they were issued one plane earlier, which hides a memory
-// latency per plane (a workgroup marches its tile alone on its CU).
+// latency per plane (a workgroup marches its tile alone on its CU) - but the
+// direction is formed from the values as they are loaded, so every load is
+// still waited for in the step that issues it.  3, the deferred form: the
+// raw values are loaded two planes ahead at clamped addresses, with no
+// arithmetic and no condition at the load, and finished one step later.
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
 // prefetch forms and tile heights the solver launches (measured at 256^3):
 // the apply and the variable-coefficient first half take the deferred form;
