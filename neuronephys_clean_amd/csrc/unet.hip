@@ -491,7 +491,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
                 f32x4_t v4 = A[r][mt];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    float s = v4[q] + bias_r[mt][q];
+                    float s = v4[q];  // the bias is in the accumulator's start value
                     s = fmaxf(s, leak * s);  // leaky ReLU (slope leak <= 1)
                     v4[q] = s;
                     hsum = __builtin_fmaf(head_r[mt][q], s, hsum);
@@ -584,7 +584,9 @@ __global__ __launch_bounds__(256, 2) void k_conv3z(ConvArgs a, int zc) {
                 }
 #pragma unroll
                 for (int mt = 0; mt < MT; ++mt) {
-                    const f32x4_t c0 = j == 0 ? f32x4_t{0.f, 0.f, 0.f, 0.f} : acc[P][r][mt];
+                    // a plane's first contribution starts from the bias
+                    const f32x4_t c0 = j == 0 ? f32x4_t{bias_r[mt][0], bias_r[mt][1], bias_r[mt][2], bias_r[mt][3]}
+                                              : acc[P][r][mt];
                     acc[P][r][mt] = mfma16(wa[0][j][mt], bf, c0);
                     acc[(P + 2) % 3][r][mt] = mfma16(wa[1][j][mt], bf, acc[(P + 2) % 3][r][mt]);
                     acc[(P + 1) % 3][r][mt] = mfma16(wa[2][j][mt], bf, acc[(P + 1) % 3][r][mt]);
