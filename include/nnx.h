@@ -617,7 +617,11 @@ nnx_status nnx_unet_train_step(nnx_ctx* ctx, nnx_unet* net, int nx, int ny, int 
  * (32,32)}.  out = act(conv3(in + coarse_scale * up(coarse)) + bias) with
  * act the leaky ReLU when relu != 0 (identity otherwise), pool =
  * its 2x2x2 mean; wgrad: dw[co][tap cin + ci] = sum_v g[v][co] in'[v + tap][ci]
- * and db[co] = sum_v g[v][co], in' = in + coarse_scale * up(coarse).        */
+ * and db[co] = sum_v g[v][co], in' = in + coarse_scale * up(coarse).
+ * Dimensions: nx a multiple of 16, and ny a multiple of 16 (8 for cin 32)
+ * for the z-marching kernel, else of 8 (4 for cin 32) with nz a multiple
+ * of 4 for the tile kernel; nx a multiple of 32 and ny, nz even with coarse
+ * or pool.  wgrad: nx a multiple of 16, ny and nz of 4 (8 with coarse).    */
 nnx_status nnx_conv3_bf16(nnx_ctx* ctx, int cin, int cout, int nx, int ny, int nz, int batch,
                           const uint16_t* d_in, const uint16_t* d_coarse, double coarse_scale,
                           const float* d_w, const float* d_bias, int relu, uint16_t* d_out,
