@@ -533,9 +533,13 @@ typedef struct nnx_ens_plan {
 } nnx_ens_plan;
 
 /* launch shape of the fused step: threads per workgroup (64, 128, 256, 512),
- * neurons per tile (a multiple of 64), persistent workgroups.              */
+ * neurons per tile (a multiple of 64), persistent workgroups; and the
+ * flagless-event threshold `bulk`: when a step fires more than `bulk`
+ * neurons, the next step pushes its events without marking their targets,
+ * and every target then scans its event bits (0: the library's rule, more
+ * events than one per 64 targets; < 0: never).  None changes any result. */
 typedef struct nnx_ens_config {
-    int32_t block, tile, grid, pad_;
+    int32_t block, tile, grid, bulk;
 } nnx_ens_config;
 
 /* n_steps steps from first_step (first_step == 0 clears the pending
@@ -801,9 +805,11 @@ typedef struct nnx_step_plan {
      * (workgroups); and the burst threshold: when a step fires more than
      * `fused_burst` neurons (<= 0: never) the next step pulls every target's
      * in-edges instead of reading its pending events (pushing events is the
-     * cheaper of the two until a large part of the network fires at once).
+     * cheaper of the two until a large part of the network fires at once);
+     * and `fused_bulk`, the flagless-event threshold of nnx_ens_config.bulk.
      * None of these changes any result.                                     */
     int32_t fused_block, fused_tile, fused_grid, fused_burst;
+    int32_t fused_bulk, fused_pad_;
     /* electrode readout (optional; readout_every <= 0 = off), as the
      * reference's print step (src/main.cu:133-141) and ElectrodesExample
      * (src_wholebrain/main.cu:36-47): before each step s with

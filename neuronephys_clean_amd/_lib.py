@@ -145,7 +145,8 @@ class StepPlan(C.Structure):
                 ("list", vp), ("count", vp), ("voxel_of", vp), ("voxel_counts", vp),
                 ("raster", vp), ("raster_len", vp), ("raster_cap", C.c_int64),
                 ("fused_block", C.c_int32), ("fused_tile", C.c_int32), ("fused_grid", C.c_int32),
-                ("fused_burst", C.c_int32), ("regions", Regions), ("readout_every", C.c_int64),
+                ("fused_burst", C.c_int32), ("fused_bulk", C.c_int32), ("fused_pad_", C.c_int32),
+                ("regions", Regions), ("readout_every", C.c_int64),
                 ("readout_cap", C.c_int64), ("readout_avg", vp), ("readout_firing", vp),
                 ("readout_mean", vp), ("hebbian", Hebbian), ("paracrine", Paracrine)]
 
@@ -158,10 +159,10 @@ class EnsPlan(C.Structure):
 
 
 class EnsConfig(C.Structure):
-    _fields_ = [("block", C.c_int32), ("tile", C.c_int32), ("grid", C.c_int32), ("pad_", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("tile", C.c_int32), ("grid", C.c_int32), ("bulk", C.c_int32)]
 
     def as_dict(self):
-        return {"block": self.block, "tile": self.tile, "grid": self.grid}
+        return {"block": self.block, "tile": self.tile, "grid": self.grid, "bulk": self.bulk}
 
 
 _P = C.POINTER

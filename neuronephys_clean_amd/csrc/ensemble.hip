@@ -17,9 +17,14 @@
 // flag (plain store), so it never waits on memory.  A flagged target reads
 // the bit words of its CSC range and adds the weights of the set bits in
 // ascending position - ascending source - order, clearing the bits as it
-// goes.  Because every sum has a fixed order, the trajectory is
-// bit-identical for every (block, tile, grid) configuration; the autotuner
-// uses this to time candidate configurations on real steps.
+// goes.  After a step that fired more than `bulk` neurons the next step's
+// pushes leave the flags alone and every target reads its bit words (a word
+// per parity records which form the pending events took): once a sizeable
+// part of the targets receives events, the random flag stores cost more
+// than scanning every target's words.  Because every sum has a fixed
+// order, the trajectory is bit-identical for every (block, tile, grid,
+// bulk) configuration; the autotuner uses this to time candidate
+// configurations on real steps.
 //
 // No counterpart in the reference, which simulates one network
 // (src/main.cu:112-182).
@@ -51,7 +56,9 @@ struct EnsK {
     int32_t ntiles;
     int64_t* stepctr;   // step number by parity
     int32_t* spk3;      // spikes of each step, by step mod 3
+    int32_t* evmode;    // by step parity: 1 when that step pushed its events without target flags
     int32_t burst;      // above this many spikes the next step pulls densely (< 0: never)
+    int32_t bulk;       // above this many spikes the next step pushes without flags (< 0: never)
 };
 
 // Sum of the pending events of the target whose CSC range is [e0, e1)
@@ -124,10 +131,17 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
     uint64_t* __restrict__ bits_w = par ? d.bits1 : d.bits0;
     const int64_t step = d.stepctr[q] + 1;
     const int cur3 = (int)(step % 3);
-    const bool dense = d.burst >= 0 && d.spk3[(cur3 + 2) % 3] > d.burst;
+    const int prev_spk = d.spk3[(cur3 + 2) % 3];  // final: written by the previous launch
+    const bool dense = d.burst >= 0 && prev_spk > d.burst;
+    // flagless events: the previous step pushed without setting target
+    // flags (bulk_r), so every target scans its event bits; this step does
+    // the same when the previous one fired more than `bulk` (bulk_w)
+    const bool bulk_r = d.evmode[q] != 0;
+    const bool bulk_w = d.bulk >= 0 && prev_spk > d.bulk;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         d.stepctr[par] = step;
         d.spk3[(cur3 + 1) % 3] = 0;  // the next step's counter (last read by the previous step)
+        d.evmode[par] = bulk_w;
     }
     const int lane = threadIdx.x & 63;
     for (int t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
@@ -143,14 +157,16 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
             float g = 0.f;
             const int32_t wave0 = i - lane;
             if (!dense) {
-                if (live && flag_r[i]) {
-                    g = gather_bits(d.in_off[i], d.in_off[i + 1], eb_r, d.in_w);
-                    flag_r[i] = 0;
+                if (live && (bulk_r || flag_r[i])) {
+                    const int32_t e0 = d.in_off[i], e1 = d.in_off[i + 1];
+                    if (e1 > e0) g = gather_bits(e0, e1, eb_r, d.in_w);
+                    if (!bulk_r) flag_r[i] = 0;
                 }
             } else {
-                if (live && flag_r[i]) {
-                    clear_bits(d.in_off[i], d.in_off[i + 1], eb_r);
-                    flag_r[i] = 0;
+                if (live && (bulk_r || flag_r[i])) {
+                    const int32_t e0 = d.in_off[i], e1 = d.in_off[i + 1];
+                    if (e1 > e0) clear_bits(e0, e1, eb_r);
+                    if (!bulk_r) flag_r[i] = 0;
                 }
                 uint64_t hm = __ballot(live);
                 while (hm) {
@@ -180,11 +196,11 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
             if (!word) continue;
             const int k = __popcll(word);
             bool do_push = push != 0;
-            if (d.burst >= 0) {
+            if (d.burst >= 0 || d.bulk >= 0) {
                 int old = 0;
                 if (lane == 0) old = atomicAdd(&d.spk3[cur3], k);
                 old = __shfl(old, 0, 64);
-                do_push = do_push && old + k <= d.burst;
+                if (d.burst >= 0) do_push = do_push && old + k <= d.burst;
             }
             // push the new spikes into the next step's buckets, one source at
             // a time across the wavefront
@@ -197,7 +213,7 @@ __global__ __launch_bounds__(B) void k_ens_step(EnsK d, HHConst<float> p, int pa
                 for (int32_t e = d.out_off[j] + lane; e < e1; e += 64) {
                     const int32_t pos = d.out_to_in[e];
                     atomicOr(&eb_w[pos >> 5], 1u << (pos & 31));
-                    flag_w[d.out_tgt[e]] = 1;
+                    if (!bulk_w) flag_w[d.out_tgt[e]] = 1;
                 }
             }
             if (spike && d.voxel_of) atomicAdd(&d.voxel_counts[d.voxel_of[i]], 1);
@@ -244,12 +260,13 @@ __global__ __launch_bounds__(256) void k_push_bits(const int32_t* __restrict__ l
 // run set-up: the step counter of the step before the first; fresh = 1
 // (buckets empty or about to be filled by a full push of the previous
 // step's list) also marks the previous step as no burst
-__global__ void k_ens_begin(int64_t* stepctr, int q, int64_t prev, int32_t* spk3, int fresh) {
+__global__ void k_ens_begin(int64_t* stepctr, int q, int64_t prev, int32_t* spk3, int32_t* evmode, int fresh) {
     stepctr[q] = prev;
     if (fresh) {
         const int p3 = (int)((prev % 3 + 3) % 3);
         spk3[p3] = 0;
         spk3[(p3 + 1) % 3] = 0;
+        evmode[q] = 0;  // pending events (if any) carry target flags
     }
 }
 
@@ -260,6 +277,19 @@ using namespace nnx;
 namespace {
 
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
+
+// Spikes in a step above which the next step pushes its events without
+// target flags, so that every target scans its event bits (`req` > 0: that
+// many; < 0: never; 0: when the events would exceed one per 64 targets,
+// spikes * nnz / n > n / 64, measured on C2 (1M neurons, 2.1e8 edges:
+// 81 spikes) as 6% off the 10,000-step window, and neutral on the 4.6M-neuron
+// ensemble).  Either way the sums are the same.
+int32_t bulk_threshold(int32_t req, int64_t n, int64_t nnz) {
+    if (req != 0) return req < 0 ? -1 : req;
+    if (nnz <= 0) return -1;
+    const double t = (double)n * (double)n / (64.0 * (double)nnz);
+    return (int32_t)std::min(t, 1e9);
+}
 
 nnx_status check_ens(nnx_ctx* c, const nnx_ens_plan* p) {
     NNX_CHECK(c, p, "null plan");
@@ -356,7 +386,9 @@ nnx_status prepare(nnx_ctx* c, const char* kind, const nnx_hh_state& s, const nn
     k.bits0 = bits[0], k.bits1 = bits[1];
     k.stepctr = (int64_t*)ctr;                       // bytes 0..15
     k.spk3 = (int32_t*)((char*)ctr + 16);            // bytes 16..27
+    k.evmode = (int32_t*)((char*)ctr + 32);          // bytes 32..39
     k.burst = -1;  // no dense fallback unless asked for (event pushes are cheaper up to near-synchronous firing)
+    k.bulk = -1;
     h->hc = make_const<float>(&prm, dt);
     return NNX_OK;
 }
@@ -413,6 +445,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
     NNX_TRY(prepare(c, "fstep", s, p->params, p->graph, p->dt, bits, &h));
     h.k.burst = p->fused_burst > 0 ? p->fused_burst : -1;
+    h.k.bulk = bulk_threshold(p->fused_bulk, s.n, p->graph.nnz);
     if (p->voxel_of) h.k.voxel_of = p->voxel_of, h.k.voxel_counts = p->voxel_counts;
     if (p->record) {
         h.k.raster = p->raster;
@@ -426,7 +459,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     uint32_t* eb_q0 = q0 ? h.k.eb1 : h.k.eb0;
     // the step number is set outside the captured sequence (a captured
     // kernel argument would replay the first capture's number)
-    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, 1);
+    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, h.k.evmode, 1);
     NNX_LAUNCHED(c);
     NNX_TRY(readout_prepare(c, p));
     NNX_TRY(para_prepare(c, para_io(p)));
@@ -472,6 +505,7 @@ nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_con
     uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
     NNX_TRY(prepare(c, "ens", p->state, p->params, p->graph, p->dt, bits, &h));
     h.k.voxel_of = p->voxel_of, h.k.voxel_counts = p->voxel_counts, h.k.net_spikes = p->net_spikes;
+    h.k.bulk = bulk_threshold(f.bulk, p->state.n, p->graph.nnz);
     h.k.raster = p->raster, h.k.raster_len = (unsigned long long*)p->raster_len, h.k.raster_cap = p->raster_cap;
     NNX_TRY(build_tiles(c, "ens", p->state.n, p->nnet, p->net_begin, p->net_size, f.tile, &h.k.tiles,
                         &h.k.ntiles));
@@ -486,7 +520,8 @@ nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_con
     // a continuing run reads the buckets and burst counter the previous
     // call's last step left
     const int q0 = (int)((first_step + 1) & 1);
-    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, fresh ? 1 : 0);
+    k_ens_begin<<<1, 1, 0, c->stream>>>(h.k.stepctr, q0, (int64_t)first_step - 1, h.k.spk3, h.k.evmode,
+                                         fresh ? 1 : 0);
     NNX_LAUNCHED(c);
     const int64_t every = p->paracrine.every;
     NNX_TRY(para_prepare(c, para_io(p)));  // buffers exist before any capture

@@ -117,8 +117,9 @@ class Simulation:
         p.raster = a["raster"].ptr
         p.raster_len = a["raster_len"].ptr
         p.raster_cap = record_cap
-        if fused:
-            p.fused_block, p.fused_tile, p.fused_grid, p.fused_burst = fused
+        if fused:  # (block, tile, grid, burst[, bulk])
+            f = tuple(fused) + (0,) * (5 - len(fused))
+            p.fused_block, p.fused_tile, p.fused_grid, p.fused_burst, p.fused_bulk = f
         self.plan = p
         self.step = 0
 
@@ -182,12 +183,15 @@ class Simulation:
         v = self.a["para_stats"].numpy()
         return dict(zip(_lib.PARA_STATS, (float(x) for x in v)))
 
-    def set_fused(self, block=0, tile=0, grid=0, burst=None):
-        """Launch shape of the fused step (results do not depend on it)."""
+    def set_fused(self, block=0, tile=0, grid=0, burst=None, bulk=None):
+        """Launch shape of the fused step and its burst and flagless-event
+        thresholds (results do not depend on them)."""
         p = self.plan
         p.fused_block, p.fused_tile, p.fused_grid = block, tile, grid
         if burst is not None:
             p.fused_burst = burst
+        if bulk is not None:
+            p.fused_bulk = bulk
 
     def raster(self) -> np.ndarray:
         """(step, neuron) rows, ordered by step then neuron."""

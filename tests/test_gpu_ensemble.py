@@ -65,13 +65,16 @@ def test_ensemble_matches_oracle_per_network(ctx, ens):
     assert np.array_equal(vc, ref_vc)
 
 
-@pytest.mark.parametrize("cfg", [(64, 64, 7), (512, 8192, 3), (128, 512, 2048)])
+@pytest.mark.parametrize("cfg", [(64, 64, 7, 1), (512, 8192, 3, -1), (128, 512, 2048, 0), (256, 256, 512, 50)])
 def test_ensemble_result_independent_of_config_and_graph(ctx, ens, cfg):
+    """Any launch shape, with events pushed with target flags (bulk -1),
+    without them (bulk 1) or switching (bulk 50, and 0: the library's
+    rule), gives the default run's bits."""
     steps = 200
     a = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
     a.run(steps)
     b = EnsembleSim(ens, ctx=ctx, record_cap=1 << 20)
-    c = _lib.EnsConfig(*cfg, 0)
+    c = _lib.EnsConfig(*cfg)
     say("config", cfg)
     b.run(steps, cfg=c, use_graph=True, chunk=100)
     a.synchronize()

@@ -603,12 +603,15 @@ def test_sim_c2_scaled_bitexact(ctx, couple):
     assert np.array_equal(sim.a["voxel_counts"].numpy(), vc)
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 7, 3), (512, 4096, 3, 0), (128, 512, 2048, -1), (256, 1024, 64, 40)])
+@pytest.mark.parametrize("shape", [(64, 64, 7, 3, 1), (512, 4096, 3, 0, -1), (128, 512, 2048, -1, 0),
+                                   (256, 1024, 64, 40, -1), (256, 256, 64, 0, 1), (256, 256, 64, 0, 20)])
 def test_fused_step_independent_of_shape_and_bursts(ctx, shape):
     """The fused step on a ragged network (777 neurons): identical to the
-    event path for any launch shape, and when a small burst threshold makes
+    event path for any launch shape, when a small burst threshold makes
     steps fall back to the dense pull (burst 3 and 40 fire on most steps of
-    the initial transient)."""
+    the initial transient), and with events pushed with target flags (bulk
+    -1), without them (bulk 1) or switching between the two (bulk 20, and 0:
+    the library's rule)."""
     net = configs.c1(seed=3, n=777, k=40)
     a = Simulation(net, np.float32, couple="event", record_cap=100000, deposit=True)
     b = Simulation(net, np.float32, couple="fused", record_cap=100000, deposit=True, fused=shape)
