@@ -84,6 +84,22 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, in
     block_sum_all<NV>(out);
 }
 
+// sum_partials with one more value, `flag` of thread 0, summed alongside
+// (so out[NV] is uniform over the block): the NV totals are the same bits as
+// sum_partials<NV>'s (every value has its own reduction).
+template <int NV>
+__device__ __forceinline__ void sum_partials_flag(const double* __restrict__ part, int nb, int32_t flag,
+                                                  double (&out)[NV + 1]) {
+#pragma unroll
+    for (int q = 0; q <= NV; ++q) out[q] = 0;
+    if (threadIdx.x < 256)
+        for (int t = threadIdx.x; t < nb; t += 256)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] += part[t * NV + q];
+    if (threadIdx.x == 0) out[NV] = (double)flag;
+    block_sum_all<NV + 1>(out);
+}
+
 template <int NV> __device__ __forceinline__ void put_partial(double (&v)[NV], double* part) {
     block_sum_all<NV>(v);
     if (threadIdx.x == 0)
@@ -94,7 +110,11 @@ template <int NV> __device__ __forceinline__ void put_partial(double (&v)[NV], d
 // ---------------------------------------------------------------------------
 // CG state on the device
 // ---------------------------------------------------------------------------
-enum { SC_BB = 0, SC_THR, SC_RR, SC_RZ, SC_N };
+// SC_RZK + (k & 1): r.z entering iteration k, stored by block 0 of its
+// first half; SC_ALK + (k & 1): alpha_k (a float, widened), stored by block
+// 0 of its second half.  The next launches read these instead of summing
+// the partials again (the same doubles: every block computes them alike).
+enum { SC_BB = 0, SC_THR, SC_RR, SC_RZ, SC_RZK, SC_RZK1, SC_ALK, SC_ALK1, SC_N };
 enum { FL_DONE = 0, FL_ITERS, FL_INITDONE, FL_ABS, FL_XPEND, FL_N };
 
 struct CgDev {
@@ -248,23 +268,36 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
     static_assert(!VAR || MODE == 2, "variable coefficients: the first half only (k_cg_rupd is the second)");
     __shared__ float t[2][TY + 2][kRow4];
     float beta = 0.0f, di = 1.0f, alpha = 0.0f;
+    // the scalars in one round of loads: the done flag, r.z and alpha of
+    // the previous iteration (stored by block 0 of its launches) and the
+    // partials of this iteration's sums
     if (MODE == 2) {
-        double rz, rzp, rr;
-        if (!cg_scalars(d, k, true, rz, rzp, rr)) return;
-        beta = k ? (float)(rz / rzp) : 0.0f;
-        if (k) {
-            double pq[1];
-            sum_partials<1>(((k - 1) & 1) ? d.ppq1 : d.ppq, d.npq, pq);
-            alpha = (float)(rzp / pq[0]);
+        const int32_t done = d.fl[FL_DONE], absm = d.fl[FL_ABS];
+        const double thr = d.sc[SC_THR];
+        const double rzp = k ? d.sc[SC_RZK + ((k - 1) & 1)] : 1.0;
+        const double alp = k ? d.sc[SC_ALK + ((k - 1) & 1)] : 0.0;
+        double u[3];
+        sum_partials_flag<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, done, u);
+        if (u[2] != 0.0) return;  // the solve is over
+        const double rz = u[0], rr = u[1];
+        if (absm ? (rr < thr) : (rr <= thr)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) d.fl[FL_DONE] = 1;
+            return;
         }
+        if (blockIdx.x == 0 && threadIdx.x == 0) d.sc[SC_RZK + (k & 1)] = rz;
+        beta = k ? (float)(rz / rzp) : 0.0f;
+        alpha = (float)alp;
         di = d.dinv_c;
     }
     if (MODE == 3) {
-        double rz, rzp, rr;
-        if (!cg_scalars(d, k, false, rz, rzp, rr)) return;
-        double pq[1];
-        sum_partials<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, pq);
-        alpha = (float)(rz / pq[0]);
+        // (the first half of this iteration made the stop test)
+        const int32_t done = d.fl[FL_DONE];
+        const double rz = d.sc[SC_RZK + (k & 1)];
+        double u[2];
+        sum_partials_flag<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, done, u);
+        if (u[1] != 0.0) return;
+        alpha = (float)(rz / u[0]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) d.sc[SC_ALK + (k & 1)] = (double)alpha;
         di = d.dinv_c;
     }
     // VAR: the Jacobi scaling per voxel (a constant when the solve has none)
@@ -635,10 +668,7 @@ __global__ __launch_bounds__(256) void k_cg_finish(int64_t n, CgDev d, const flo
     if (!d.fl[FL_XPEND]) return;
     const int k = d.fl[FL_ITERS] - 1;
     const bool read_x = !(xzero && k == 0);
-    double rz[2], pq[1];
-    sum_partials<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, rz);
-    sum_partials<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, pq);
-    const float alpha = (float)(rz[0] / pq[0]);
+    const float alpha = (float)d.sc[SC_ALK + (k & 1)];  // stored by iteration k's second half
     const float* __restrict__ p = (k & 1) ? pa : pb;
     const int64_t n4 = n / 4, stride = (int64_t)gridDim.x * 256;
     for (int64_t c = blockIdx.x * 256ll + threadIdx.x; c < n4; c += stride) {
@@ -669,11 +699,15 @@ __global__ __launch_bounds__(256) void k_cg_finish(int64_t n, CgDev d, const flo
 // rz / pq, partials of r . dinv r and r . r; the x update is left to the
 // next iteration's first half (or k_cg_finish), as on the constant grid.
 __global__ __launch_bounds__(256) void k_cg_rupd(int64_t n, CgDev d, int k) {
-    double rz, rzp, rr;
-    if (!cg_scalars(d, k, false, rz, rzp, rr)) return;
-    double pq[1];
-    sum_partials<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, pq);
-    const float alpha = (float)(rz / pq[0]);
+    // as the second half on the constant grid: r.z from the first half,
+    // which also made the stop test
+    const int32_t done = d.fl[FL_DONE];
+    const double rz = d.sc[SC_RZK + (k & 1)];
+    double u[2];
+    sum_partials_flag<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, done, u);
+    if (u[1] != 0.0) return;
+    const float alpha = (float)(rz / u[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.sc[SC_ALK + (k & 1)] = (double)alpha;
     const float dc = d.dinv_c;
     double v[2] = {0, 0};
     const int64_t n4 = n / 4, stride = (int64_t)gridDim.x * 256;
