@@ -969,8 +969,8 @@ def main():
                                      "100-step graph)",
                       "alg_bytes_per_launch": fb,
                       "alg_bytes_rule": "48.125 B per neuron + 32 B per delivered event",
-                      "note": "bound by VALU issue, not HBM, on quiet steps: 578 VALU instructions per neuron "
-                              "at ~0.9 of the non-packed issue rate (profiles/valu_rk4_kernels_c2.md), and by "
+                      "note": "bound by VALU issue, not HBM, on quiet steps: 599 VALU instructions per neuron "
+                              "at ~0.8-0.9 of the non-packed issue rate (profiles/valu_rk4_kernels_c2.md), and by "
                               "random event traffic on the ~290 steps of synchronous firing"}
     cg_iter = cg_iteration_roofline(ctx, net.grid[0], b_fixed.astype(np.float32), net.c)
     log(f"bench: CG iteration {cg_iter}")

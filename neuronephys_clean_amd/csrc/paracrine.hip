@@ -72,32 +72,44 @@ template <int NV> __device__ __forceinline__ void block_sum_all(double (&v)[NV])
 // Total of nb partials of NV doubles with a fixed tree whatever the block
 // size: threads t < 256 add entries t, t+256, ... in order, the other threads
 // contribute zeros (appending +0 to a sum leaves it unchanged), then the
-// block sum.  Identical in every block of every kernel.
+// block sum.  Identical in every block of every kernel (every total of
+// partials goes through partial_terms).  partial_terms is
+// the per-thread part alone, so that a launch can test its done flag
+// between the loads and the block sum (a launch past convergence then
+// returns without waiting on them or synchronising).
+// A thread's entries are loaded 8 at a time (all of them for nb <= 2048)
+// at clamped addresses before any is added, an entry past nb counting as
+// +0, so that the loads cost one memory latency rather than one each.
 template <int NV>
-__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb, double (&out)[NV]) {
+__device__ __forceinline__ void partial_terms(const double* __restrict__ part, int nb, double (&out)[NV]) {
+    constexpr int kU = 8;
 #pragma unroll
     for (int q = 0; q < NV; ++q) out[q] = 0;
-    if (threadIdx.x < 256)
-        for (int t = threadIdx.x; t < nb; t += 256)
+    const int lt = (int)(threadIdx.x & 255);
+    const bool lane_ok = threadIdx.x < 256;  // from thread 256 on: nothing
+    for (int base = 0; base < nb; base += 256 * kU) {
+        uint64_t v[kU][NV];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) out[q] += part[t * NV + q];
-    block_sum_all<NV>(out);
+        for (int i = 0; i < kU; ++i) {
+            const int t = base + 256 * i + lt;
+            const int tc = t < nb ? t : nb - 1;  // every thread loads, at a valid address
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[i][q] = reinterpret_cast<const uint64_t*>(part)[tc * NV + q];
+        }
+#pragma unroll
+        for (int i = 0; i < kU; ++i) {
+            // +0 (all bits clear) for an entry past nb or a thread past 255
+            const uint64_t keep = (lane_ok && base + 256 * i + lt < nb) ? ~0ull : 0ull;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] += __longlong_as_double((long long)(v[i][q] & keep));
+        }
+    }
 }
 
-// sum_partials with one more value, `flag` of thread 0, summed alongside
-// (so out[NV] is uniform over the block): the NV totals are the same bits as
-// sum_partials<NV>'s (every value has its own reduction).
 template <int NV>
-__device__ __forceinline__ void sum_partials_flag(const double* __restrict__ part, int nb, int32_t flag,
-                                                  double (&out)[NV + 1]) {
-#pragma unroll
-    for (int q = 0; q <= NV; ++q) out[q] = 0;
-    if (threadIdx.x < 256)
-        for (int t = threadIdx.x; t < nb; t += 256)
-#pragma unroll
-            for (int q = 0; q < NV; ++q) out[q] += part[t * NV + q];
-    if (threadIdx.x == 0) out[NV] = (double)flag;
-    block_sum_all<NV + 1>(out);
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb, double (&out)[NV]) {
+    partial_terms<NV>(part, nb, out);
+    block_sum_all<NV>(out);
 }
 
 template <int NV> __device__ __forceinline__ void put_partial(double (&v)[NV], double* part) {
@@ -276,9 +288,14 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
         const double thr = d.sc[SC_THR];
         const double rzp = k ? d.sc[SC_RZK + ((k - 1) & 1)] : 1.0;
         const double alp = k ? d.sc[SC_ALK + ((k - 1) & 1)] : 0.0;
-        double u[3];
-        sum_partials_flag<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, done, u);
-        if (u[2] != 0.0) return;  // the solve is over
+        double u[2];
+        partial_terms<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, u);
+        // the solve is over: return before the block sum.  Block 0 of this
+        // launch sets the flag once it finds the stop test met; threads of
+        // another block that read the flag before that store find the test
+        // met as well and return after the block sum.
+        if (done) return;
+        block_sum_all<2>(u);
         const double rz = u[0], rr = u[1];
         if (absm ? (rr < thr) : (rr <= thr)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) d.fl[FL_DONE] = 1;
@@ -293,9 +310,10 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
         // (the first half of this iteration made the stop test)
         const int32_t done = d.fl[FL_DONE];
         const double rz = d.sc[SC_RZK + (k & 1)];
-        double u[2];
-        sum_partials_flag<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, done, u);
-        if (u[1] != 0.0) return;
+        double u[1];
+        partial_terms<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, u);
+        if (done) return;  // (the flag does not change during this launch)
+        block_sum_all<1>(u);
         alpha = (float)(rz / u[0]);
         if (blockIdx.x == 0 && threadIdx.x == 0) d.sc[SC_ALK + (k & 1)] = (double)alpha;
         di = d.dinv_c;
@@ -703,9 +721,10 @@ __global__ __launch_bounds__(256) void k_cg_rupd(int64_t n, CgDev d, int k) {
     // which also made the stop test
     const int32_t done = d.fl[FL_DONE];
     const double rz = d.sc[SC_RZK + (k & 1)];
-    double u[2];
-    sum_partials_flag<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, done, u);
-    if (u[1] != 0.0) return;
+    double u[1];
+    partial_terms<1>((k & 1) ? d.ppq1 : d.ppq, d.npq, u);
+    if (done) return;
+    block_sum_all<1>(u);
     const float alpha = (float)(rz / u[0]);
     if (blockIdx.x == 0 && threadIdx.x == 0) d.sc[SC_ALK + (k & 1)] = (double)alpha;
     const float dc = d.dinv_c;
