@@ -536,8 +536,9 @@ typedef struct nnx_ens_plan {
  * neurons per tile (a multiple of 64), persistent workgroups; and the
  * flagless-event threshold `bulk`: when a step fires more than `bulk`
  * neurons, the next step pushes its events without marking their targets,
- * and every target then scans its event bits (0: the library's rule, more
- * events than one per 64 targets; < 0: never).  None changes any result. */
+ * and every target then scans its event bits (< 0: never; 0: the library's
+ * rule - for one network, more events than one per 64 targets; for an
+ * ensemble, never).  None changes any result.                           */
 typedef struct nnx_ens_config {
     int32_t block, tile, grid, bulk;
 } nnx_ens_config;

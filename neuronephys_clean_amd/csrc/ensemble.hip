@@ -280,13 +280,16 @@ bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
 
 // Spikes in a step above which the next step pushes its events without
 // target flags, so that every target scans its event bits (`req` > 0: that
-// many; < 0: never; 0: when the events would exceed one per 64 targets,
-// spikes * nnz / n > n / 64, measured on C2 (1M neurons, 2.1e8 edges:
-// 81 spikes) as 6% off the 10,000-step window, and neutral on the 4.6M-neuron
-// ensemble).  Either way the sums are the same.
-int32_t bulk_threshold(int32_t req, int64_t n, int64_t nnz) {
+// many; < 0: never; 0: the library's rule).  The rule for one network: when
+// the events would exceed one per 64 targets, spikes * nnz / n > n / 64
+// (C2, 1M neurons and 2.1e8 edges: 81 spikes; 6% off its 10,000-step
+// window).  For an ensemble: never - its busy steps are mostly a few
+// networks firing while the others are quiet, and scanning every target of
+// the launch then costs more than the flags (C4: 77.1 against 76.0 us per
+// step).  Either way the sums are the same.
+int32_t bulk_threshold(int32_t req, int64_t n, int64_t nnz, bool ensemble) {
     if (req != 0) return req < 0 ? -1 : req;
-    if (nnz <= 0) return -1;
+    if (ensemble || nnz <= 0) return -1;
     const double t = (double)n * (double)n / (64.0 * (double)nnz);
     return (int32_t)std::min(t, 1e9);
 }
@@ -445,7 +448,7 @@ nnx_status nnx::fused_run_steps(nnx_ctx* c, const nnx_step_plan* p, uint64_t fir
     uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
     NNX_TRY(prepare(c, "fstep", s, p->params, p->graph, p->dt, bits, &h));
     h.k.burst = p->fused_burst > 0 ? p->fused_burst : -1;
-    h.k.bulk = bulk_threshold(p->fused_bulk, s.n, p->graph.nnz);
+    h.k.bulk = bulk_threshold(p->fused_bulk, s.n, p->graph.nnz, false);
     if (p->voxel_of) h.k.voxel_of = p->voxel_of, h.k.voxel_counts = p->voxel_counts;
     if (p->record) {
         h.k.raster = p->raster;
@@ -505,7 +508,7 @@ nnx_status nnx_ensemble_run(nnx_ctx* c, const nnx_ens_plan* p, const nnx_ens_con
     uint64_t* const bits[2] = {p->bits[0], p->bits[1]};
     NNX_TRY(prepare(c, "ens", p->state, p->params, p->graph, p->dt, bits, &h));
     h.k.voxel_of = p->voxel_of, h.k.voxel_counts = p->voxel_counts, h.k.net_spikes = p->net_spikes;
-    h.k.bulk = bulk_threshold(f.bulk, p->state.n, p->graph.nnz);
+    h.k.bulk = bulk_threshold(f.bulk, p->state.n, p->graph.nnz, true);
     h.k.raster = p->raster, h.k.raster_len = (unsigned long long*)p->raster_len, h.k.raster_cap = p->raster_cap;
     NNX_TRY(build_tiles(c, "ens", p->state.n, p->nnet, p->net_begin, p->net_size, f.tile, &h.k.tiles,
                         &h.k.ntiles));
