@@ -72,14 +72,13 @@ template <int NV> __device__ __forceinline__ void block_sum_all(double (&v)[NV])
 // Total of nb partials of NV doubles with a fixed tree whatever the block
 // size: threads t < 256 add entries t, t+256, ... in order, the other threads
 // contribute zeros (appending +0 to a sum leaves it unchanged), then the
-// block sum.  Identical in every block of every kernel (every total of
-// partials goes through partial_terms).  partial_terms is
-// the per-thread part alone, so that a launch can test its done flag
-// between the loads and the block sum (a launch past convergence then
-// returns without waiting on them or synchronising).
-// A thread's entries are loaded 8 at a time (all of them for nb <= 2048)
-// at clamped addresses before any is added, an entry past nb counting as
-// +0, so that the loads cost one memory latency rather than one each.
+// block sum.  Identical in every block of every kernel: every total of
+// partials goes through partial_terms, the per-thread part, which a launch
+// may call on its own to test its done flag between the loads and the
+// block sum (a launch past convergence then returns without synchronising).
+// A thread's entries are loaded 8 at a time (all of them for nb <= 2048) at
+// clamped addresses before any is added, an entry past nb counting as +0,
+// so that the loads cost one memory latency rather than one each.
 template <int NV>
 __device__ __forceinline__ void partial_terms(const double* __restrict__ part, int nb, double (&out)[NV]) {
     constexpr int kU = 8;
