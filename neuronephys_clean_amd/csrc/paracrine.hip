@@ -290,9 +290,10 @@ __global__ __launch_bounds__(64 * TY) void k_stencil7x4(GridK g, CgDev d, int k,
         double u[2];
         partial_terms<2>((k & 1) ? d.pu1 : d.pu0, kRedBlocks, u);
         // the solve is over: return before the block sum.  Block 0 of this
-        // launch sets the flag once it finds the stop test met; threads of
+        // launch sets the flag once it finds the stop test met; waves of
         // another block that read the flag before that store find the test
-        // met as well and return after the block sum.
+        // met as well and return after the block sum (s_barrier waits only
+        // for the waves of a workgroup that have not ended).
         if (done) return;
         block_sum_all<2>(u);
         const double rz = u[0], rr = u[1];
