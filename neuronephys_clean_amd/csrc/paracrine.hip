@@ -257,8 +257,9 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
 // prefetch forms and tile heights the solver launches (measured at 256^3):
 // the apply and the variable-coefficient first half take the deferred form;
-// the constant-coefficient iteration stays at PF 2 with 16-row tiles (its
-// first half gained nothing from the deferred form, 8-row tiles were slower)
+// the constant-coefficient iteration stays at PF 2 with 16-row tiles (the
+// deferred form for its second half or for both halves: 98.1-98.5 us per
+// iteration against 96.3-96.6; 8-row tiles were slower too)
 constexpr int kPFa = 3, kPFv = 3, kPFc = 2, kTYc = kTY4;
 
 // Tile of workgroup b in round `round` of a tiled launch of nb workgroups.
