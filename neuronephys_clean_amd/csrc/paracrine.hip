@@ -226,8 +226,9 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 }
 
 // Constant 7-point stencil, 2.5D LDS tiling with four voxels per thread.
-// A block of 1024 threads owns a 256 (x) x 16 (y) tile and marches kZ4 planes
-// in z; thread (lx, ly) holds x = 4lx .. 4lx+3 of row ly as a float4.  Each
+// A block of 64 TY threads owns a 256 (x) x TY (y) tile (TY 16, or 8 and 4
+// where that measured faster) and marches kZ4 planes in z; thread (lx, ly)
+// holds x = 4lx .. 4lx+3 of row ly as a float4.  Each
 // plane (plus its one-voxel halo in x and y) goes into a double-buffered LDS
 // tile; the column's z neighbours stay in registers; the next plane's loads
 // are issued before the plane's barrier (a barrier drains outstanding loads,
@@ -257,10 +258,13 @@ __global__ __launch_bounds__(256) void k_cg_status(CgDev d) {
 constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8;
 // prefetch forms and tile heights the solver launches (measured at 256^3):
 // the apply and the variable-coefficient first half take the deferred form;
-// the constant-coefficient iteration stays at PF 2 with 16-row tiles (the
-// deferred form for its second half or for both halves: 98.1-98.5 us per
-// iteration against 96.3-96.6; 8-row tiles were slower too)
-constexpr int kPFa = 3, kPFv = 3, kPFc = 2, kTYc = kTY4;
+// the constant-coefficient iteration stays at PF 2 (the deferred form for
+// its second half or for both halves: 98.1-98.5 us per iteration against
+// 96.3-96.6), its first half on 16-row tiles and its second half on 4-row
+// tiles, four 256-thread workgroups per CU (94.4-94.5 us per iteration
+// against 97.7-97.9 with 16 rows and 96.2 with 8; the first half on 8 or
+// 4 rows: 95.2 and 95.6)
+constexpr int kPFa = 3, kPFv = 3, kPFc = 2, kTYc = kTY4, kTYc3 = 4;
 
 // Tile of workgroup b in round `round` of a tiled launch of nb workgroups.
 // Workgroups are dealt to the 8 XCDs round-robin (b % 8), each XCD with its
@@ -1164,14 +1168,16 @@ static void cg_iter_launch(nnx_ctx* c, const GridK& gk, CgHost& h, int k, float*
     if (fast && gk.block) {
         k_stencil7x4<2, kPFc, false, kTYc, true><<<nb4, 64 * kTYc, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x,
                                                                                 xzero);
-        k_stencil7x4<3, kPFc, false, kTYc, true><<<std::min(nb4, kRedBlocks), 64 * kTYc, 0, c->stream>>>(
-            gk, h.d, k, pnew, nullptr, nullptr, nullptr, 0);
+        k_stencil7x4<3, kPFc, false, kTYc3, true>
+            <<<std::min(stencil4_blocks(gk, kTYc3), kRedBlocks), 64 * kTYc3, 0, c->stream>>>(gk, h.d, k, pnew, nullptr,
+                                                                                          nullptr, nullptr, 0);
         return;
     }
     if (fast) {
         k_stencil7x4<2, kPFc, false, kTYc><<<nb4, 64 * kTYc, 0, c->stream>>>(gk, h.d, k, nullptr, pold, pnew, x, xzero);
-        k_stencil7x4<3, kPFc, false, kTYc><<<std::min(nb4, kRedBlocks), 64 * kTYc, 0, c->stream>>>(gk, h.d, k, pnew, nullptr, nullptr, nullptr,
-                                                                         0);
+        k_stencil7x4<3, kPFc, false, kTYc3>
+            <<<std::min(stencil4_blocks(gk, kTYc3), kRedBlocks), 64 * kTYc3, 0, c->stream>>>(gk, h.d, k, pnew, nullptr,
+                                                                                          nullptr, nullptr, 0);
         return;
     }
     k_cg_dir<<<kRedBlocks, 256, 0, c->stream>>>(n, h.d, k, pold, pnew);
