@@ -263,8 +263,9 @@ constexpr int kTX4 = 256, kTY4 = 16, kZ4 = 16, kT4 = 64 * kTY4, kRow4 = kTX4 + 8
 // 96.3-96.6), its first half on 16-row tiles and its second half on 4-row
 // tiles, four 256-thread workgroups per CU (94.4-94.5 us per iteration
 // against 97.7-97.9 with 16 rows and 96.2 with 8; the first half on 8 or
-// 4 rows: 95.2 and 95.6)
-constexpr int kPFa = 3, kPFv = 3, kPFc = 2, kTYc = kTY4, kTYc3 = 4;
+// 4 rows: 95.2 and 95.6); the apply on 8-row tiles (21.8 us at 256^3
+// against 22.2 with 16 rows and 22.3 with 4)
+constexpr int kPFa = 3, kPFv = 3, kPFc = 2, kTYc = kTY4, kTYc3 = 4, kTYa = 8;
 
 // Tile of workgroup b in round `round` of a tiled launch of nb workgroups.
 // Workgroups are dealt to the 8 XCDs round-robin (b % 8), each XCD with its
@@ -1639,10 +1640,11 @@ nnx_status nnx_stencil_apply(nnx_ctx* c, const nnx_grid* g, const float* x, floa
         ((uintptr_t)y % 16) == 0) {
         CgDev none{};
         if (gk.block)
-            k_stencil7x4<0, kPFa, false, kTY4, true>
-                <<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
+            k_stencil7x4<0, kPFa, false, kTYa, true>
+                <<<stencil4_blocks(gk, kTYa), 64 * kTYa, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
         else
-            k_stencil7x4<0, kPFa><<<stencil4_blocks(gk), kT4, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
+            k_stencil7x4<0, kPFa, false, kTYa>
+                <<<stencil4_blocks(gk, kTYa), 64 * kTYa, 0, c->stream>>>(gk, none, 0, x, nullptr, nullptr, y, 0);
     } else {
         k_apply<float, float><<<grid3(gk), 256, 0, c->stream>>>(gk, x, y);
     }
