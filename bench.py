@@ -942,7 +942,7 @@ def main():
                  "method": "Jacobi-CG (fp32 inner, FP64 dots) with FP64 iterative refinement, warm start from the "
                            f"previous field, captured in the step graphs (2 refinement steps of at most {MAX_INNER} "
                            f"and {MAX_INNER // 2} inner iterations enqueued per solve, kernels after convergence "
-                           "return at once; "
+                           "return on the device's done flag; "
                            "constant 7-point grid: tiled FP64 residual, two tiled launches per inner iteration)",
                  "rtol_true_fp64": 1e-8, "max_relres": pst["max_relres"], "unconverged": int(pst["unconverged"]),
                  "inner_iters_per_solve": pst["inner_iters"] / max(n_solves, 1),
